@@ -1,0 +1,345 @@
+"""fomo_amd — Python binding of the MI355X FoMo render engine (libfomo_hip.so, include/fomo_hip.h).
+
+The engine is native: HIP kernels for gfx950 behind a C ABI. This module is plumbing for tests,
+the benchmark and scripting; it loads the in-tree shared library with ctypes and raises if it is
+missing. There is no CPU fallback.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+__all__ = [
+    "lib", "FomoError", "init", "shutdown", "device_info", "Dataset", "Table", "Emission", "Batch",
+    "render_nn", "render_thomson", "nn_indices", "gen_loop_cube", "gen_goft_table", "gen_corona",
+    "read_table", "profile_enable", "profile_reset", "profile_stats", "LIB_PATH",
+]
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libfomo_hip.so")
+_LIB = None
+
+_c_int_p = ctypes.POINTER(ctypes.c_int)
+_c_i64_p = ctypes.POINTER(ctypes.c_int64)
+_c_dbl_p = ctypes.POINTER(ctypes.c_double)
+_c_flt_p = ctypes.POINTER(ctypes.c_float)
+_c_i32_p = ctypes.POINTER(ctypes.c_int32)
+
+# name -> (restype, argtypes); mirrors include/fomo_hip.h
+_SIGS = {
+    "fomo_last_error": (ctypes.c_char_p, []),
+    "fomo_version": (ctypes.c_char_p, []),
+    "fomo_init": (ctypes.c_int, [ctypes.c_int]),
+    "fomo_shutdown": (ctypes.c_int, []),
+    "fomo_device_info": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int, _c_dbl_p, _c_int_p]),
+    "fomo_dataset_create": (ctypes.c_int, [ctypes.c_int64, ctypes.c_int, _c_flt_p, _c_flt_p, _c_flt_p, ctypes.c_int,
+                                           ctypes.POINTER(_c_flt_p), ctypes.c_double, _c_int_p]),
+    "fomo_dataset_destroy": (ctypes.c_int, [ctypes.c_int]),
+    "fomo_dataset_info": (ctypes.c_int, [ctypes.c_int, _c_i64_p, _c_int_p, _c_int_p]),
+    "fomo_dataset_reindex": (ctypes.c_int, [ctypes.c_int, ctypes.c_double]),
+    "fomo_table_read": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int, _c_dbl_p, _c_dbl_p,
+                                       ctypes.c_char_p, ctypes.c_int, _c_int_p, _c_int_p, _c_flt_p, _c_flt_p,
+                                       _c_flt_p]),
+    "fomo_table_create": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, _c_flt_p, _c_flt_p, _c_flt_p, _c_int_p]),
+    "fomo_table_destroy": (ctypes.c_int, [ctypes.c_int]),
+    "fomo_emission_chianti": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double,
+                                             ctypes.c_int, ctypes.c_double, _c_int_p]),
+    "fomo_emission_thomson": (ctypes.c_int, [ctypes.c_int, _c_int_p]),
+    "fomo_goft_destroy": (ctypes.c_int, [ctypes.c_int]),
+    "fomo_goft_read": (ctypes.c_int, [ctypes.c_int, _c_flt_p, _c_flt_p]),
+    "fomo_render_nn": (ctypes.c_int, [ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_int,
+                                      ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_int, _c_flt_p, _c_dbl_p]),
+    "fomo_render_thomson": (ctypes.c_int, [ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_int,
+                                           ctypes.c_int, ctypes.c_int, _c_flt_p, _c_dbl_p]),
+    "fomo_nn_indices": (ctypes.c_int, [ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_int,
+                                       ctypes.c_int, ctypes.c_int, _c_i32_p, _c_dbl_p]),
+    "fomo_batch_create": (ctypes.c_int, [ctypes.c_int, _c_int_p, ctypes.c_int, _c_dbl_p, _c_dbl_p, ctypes.c_int,
+                                         ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_int,
+                                         _c_int_p]),
+    "fomo_batch_run": (ctypes.c_int, [ctypes.c_int]),
+    "fomo_batch_launch": (ctypes.c_int, [ctypes.c_int]),
+    "fomo_batch_sync": (ctypes.c_int, [ctypes.c_int]),
+    "fomo_batch_read": (ctypes.c_int, [ctypes.c_int, _c_flt_p, _c_dbl_p]),
+    "fomo_batch_destroy": (ctypes.c_int, [ctypes.c_int]),
+    "fomo_profile_enable": (ctypes.c_int, [ctypes.c_int]),
+    "fomo_profile_reset": (ctypes.c_int, []),
+    "fomo_profile_stats": (ctypes.c_int, [ctypes.c_char_p, _c_i64_p, _c_dbl_p]),
+    "fomo_gen_loop_cube": (ctypes.c_int, [ctypes.c_int, ctypes.c_uint64, ctypes.c_double, ctypes.c_double,
+                                          ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                          _c_flt_p, _c_flt_p, _c_flt_p, _c_flt_p, _c_flt_p, _c_flt_p, _c_flt_p,
+                                          _c_flt_p]),
+    "fomo_gen_goft_table": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_double, ctypes.c_double,
+                                           ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double]),
+    "fomo_gen_corona": (ctypes.c_int, [ctypes.c_int, ctypes.c_uint64, ctypes.c_double, _c_i64_p, _c_flt_p,
+                                       _c_flt_p, _c_flt_p, _c_flt_p]),
+}
+
+
+class FomoError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"fomo error {code}: {msg}")
+        self.code = code
+
+
+def lib():
+    """Load libfomo_hip.so (in-tree). Raises if it has not been built."""
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: build it with `python -m fomo_amd.build`")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in _SIGS.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _LIB = L
+    return _LIB
+
+
+def _check(rc):
+    if rc != 0:
+        raise FomoError(rc, lib().fomo_last_error().decode())
+
+
+def _f32(a):
+    a = np.ascontiguousarray(a, dtype=np.float32)
+    return a, a.ctypes.data_as(_c_flt_p)
+
+
+def _ptr(a, ct):
+    return a.ctypes.data_as(ctypes.POINTER(ct))
+
+
+def init(device=0):
+    _check(lib().fomo_init(device))
+
+
+def shutdown():
+    _check(lib().fomo_shutdown())
+
+
+def device_info():
+    buf = ctypes.create_string_buffer(256)
+    hbm = ctypes.c_double()
+    cus = ctypes.c_int()
+    _check(lib().fomo_device_info(buf, 256, ctypes.byref(hbm), ctypes.byref(cus)))
+    return buf.value.decode(), hbm.value, cus.value
+
+
+def profile_enable(on=True):
+    _check(lib().fomo_profile_enable(1 if on else 0))
+
+
+def profile_reset():
+    _check(lib().fomo_profile_reset())
+
+
+def profile_stats(kernel):
+    c = ctypes.c_int64()
+    ms = ctypes.c_double()
+    _check(lib().fomo_profile_stats(kernel.encode(), ctypes.byref(c), ctypes.byref(ms)))
+    return c.value, ms.value
+
+
+class Dataset:
+    """Device-resident point cloud + cell-list index (DataCube on the device)."""
+
+    def __init__(self, x, y, z, variables, dim=3, cell_scale=0.0):
+        self._keep = []
+        xs, xp = _f32(x)
+        ys, yp = _f32(y)
+        if dim == 3:
+            zs, zp = _f32(z)
+        else:
+            zs, zp = None, None
+        self._keep += [xs, ys, zs]
+        n = xs.size
+        vars_c = []
+        for v in variables:
+            vs, vp = _f32(v)
+            if vs.size != n:
+                raise ValueError("variable length differs from the coordinate length")
+            self._keep.append(vs)
+            vars_c.append(vp)
+        arr = (_c_flt_p * max(1, len(vars_c)))(*vars_c)
+        out = ctypes.c_int()
+        _check(lib().fomo_dataset_create(n, dim, xp, yp, zp, len(vars_c), arr, cell_scale, ctypes.byref(out)))
+        self.id = out.value
+        self.n = n
+        self._keep = None
+
+    def info(self):
+        n = ctypes.c_int64()
+        dims = (ctypes.c_int * 3)()
+        mx = ctypes.c_int()
+        _check(lib().fomo_dataset_info(self.id, ctypes.byref(n), dims, ctypes.byref(mx)))
+        return n.value, tuple(dims), mx.value
+
+    def reindex(self, cell_scale=0.0):
+        _check(lib().fomo_dataset_reindex(self.id, cell_scale))
+
+    def close(self):
+        if self.id is not None and _LIB is not None:
+            lib().fomo_dataset_destroy(self.id)
+            self.id = None
+
+
+def read_table(path):
+    """Parse a CHIANTI G(T,n) table (returns a dict with ion, lambda0, atweight, unit, logt, logn, g)."""
+    ion = ctypes.create_string_buffer(256)
+    unit = ctypes.create_string_buffer(1024)
+    l0 = ctypes.c_double()
+    aw = ctypes.c_double()
+    nt = ctypes.c_int()
+    nr = ctypes.c_int()
+    _check(lib().fomo_table_read(path.encode(), ion, 256, ctypes.byref(l0), ctypes.byref(aw), unit, 1024,
+                                 ctypes.byref(nt), ctypes.byref(nr), None, None, None))
+    logt = np.zeros(nt.value, np.float32)
+    logn = np.zeros(nr.value, np.float32)
+    g = np.zeros(nt.value * nr.value, np.float32)
+    _check(lib().fomo_table_read(path.encode(), ion, 256, ctypes.byref(l0), ctypes.byref(aw), unit, 1024,
+                                 ctypes.byref(nt), ctypes.byref(nr), _ptr(logt, ctypes.c_float),
+                                 _ptr(logn, ctypes.c_float), _ptr(g, ctypes.c_float)))
+    return dict(ion=ion.value.decode(), lambda0=l0.value, atweight=aw.value, unit=unit.value.decode(),
+                nt=nt.value, nrho=nr.value, logt=logt, logn=logn, g=g)
+
+
+class Table:
+    """Device-resident G(T,n) table."""
+
+    def __init__(self, logt, logn, g):
+        lt, ltp = _f32(logt)
+        ln, lnp = _f32(logn)
+        gg, gp = _f32(g)
+        out = ctypes.c_int()
+        _check(lib().fomo_table_create(lt.size, ln.size, ltp, lnp, gp, ctypes.byref(out)))
+        self.id = out.value
+
+    @classmethod
+    def from_file(cls, path):
+        t = read_table(path)
+        obj = cls(t["logt"], t["logn"], t["g"])
+        obj.meta = t
+        return obj
+
+    def close(self):
+        if self.id is not None and _LIB is not None:
+            lib().fomo_table_destroy(self.id)
+            self.id = None
+
+
+class Emission:
+    """Device-resident emission cube (GoftCube): chianti (peak, fwhm) or thomson (n tot, n pol)."""
+
+    def __init__(self, gid, n):
+        self.id = gid
+        self.n = n
+
+    @classmethod
+    def chianti(cls, ds, table, lambda0, atweight, imaging=False, abundratio=1.0):
+        out = ctypes.c_int()
+        _check(lib().fomo_emission_chianti(ds.id, table.id, lambda0, atweight, 1 if imaging else 0, abundratio,
+                                           ctypes.byref(out)))
+        return cls(out.value, ds.n)
+
+    @classmethod
+    def thomson(cls, ds):
+        out = ctypes.c_int()
+        _check(lib().fomo_emission_thomson(ds.id, ctypes.byref(out)))
+        return cls(out.value, ds.n)
+
+    def read(self):
+        a = np.zeros(self.n, np.float32)
+        b = np.zeros(self.n, np.float32)
+        _check(lib().fomo_goft_read(self.id, _ptr(a, ctypes.c_float), _ptr(b, ctypes.c_float)))
+        return a, b
+
+    def close(self):
+        if self.id is not None and _LIB is not None:
+            lib().fomo_goft_destroy(self.id)
+            self.id = None
+
+
+def render_nn(em, l, b, nx, ny, nz, nl, lambda_width, dn=False, out=True):
+    """One view; returns (image[ny*nx*max(nl,1)] float32 or None, meta[10])."""
+    meta = np.zeros(10, np.float64)
+    img = np.zeros(nx * ny * max(nl, 1), np.float32) if out else None
+    _check(lib().fomo_render_nn(em.id, l, b, nx, ny, nz, nl, lambda_width, 1 if dn else 0,
+                                _ptr(img, ctypes.c_float) if out else None, _ptr(meta, ctypes.c_double)))
+    return img, meta
+
+
+def render_thomson(em, l, b, nx, ny, nz):
+    meta = np.zeros(10, np.float64)
+    img = np.zeros(nx * ny, np.float32)
+    _check(lib().fomo_render_thomson(em.id, l, b, nx, ny, nz, _ptr(img, ctypes.c_float), _ptr(meta, ctypes.c_double)))
+    return img, meta
+
+
+def nn_indices(ds, l, b, nx, ny, nz, thomson=False):
+    idx = np.zeros(nx * ny * nz, np.int32)
+    meta = np.zeros(10, np.float64)
+    _check(lib().fomo_nn_indices(ds.id, l, b, nx, ny, nz, 1 if thomson else 0, _ptr(idx, ctypes.c_int32),
+                                 _ptr(meta, ctypes.c_double)))
+    return idx, meta
+
+
+class Batch:
+    """All views x lines of one dataset in one captured HIP graph; results stay on the device."""
+
+    def __init__(self, emissions, ls, bs, nx, ny, nz, nl, lambda_width, dn=False):
+        ids = (ctypes.c_int * len(emissions))(*[e.id for e in emissions])
+        la = np.ascontiguousarray(ls, np.float64)
+        ba = np.ascontiguousarray(bs, np.float64)
+        if la.size != ba.size:
+            raise ValueError("l and b lists must have equal length (one entry per view)")
+        out = ctypes.c_int()
+        _check(lib().fomo_batch_create(len(emissions), ids, la.size, _ptr(la, ctypes.c_double),
+                                       _ptr(ba, ctypes.c_double), nx, ny, nz, nl, lambda_width, 1 if dn else 0,
+                                       ctypes.byref(out)))
+        self.id = out.value
+        self.shape = (la.size, len(emissions), ny, nx, max(nl, 1))
+
+    def run(self):
+        _check(lib().fomo_batch_run(self.id))
+
+    def launch(self):
+        _check(lib().fomo_batch_launch(self.id))
+
+    def sync(self):
+        _check(lib().fomo_batch_sync(self.id))
+
+    def read(self):
+        out = np.zeros(self.shape, np.float32)
+        meta = np.zeros((self.shape[0], 10), np.float64)
+        _check(lib().fomo_batch_read(self.id, _ptr(out, ctypes.c_float), _ptr(meta, ctypes.c_double)))
+        return out, meta
+
+    def close(self):
+        if self.id is not None and _LIB is not None:
+            lib().fomo_batch_destroy(self.id)
+            self.id = None
+
+
+def gen_loop_cube(m, seed=1, t_in=1.0e6, t_out=2.0e6, noise_sigma=0.05, vamp=2.0e4, jitter=0.0, sausage=0.0):
+    """Straight cylindrical loop in a 10 Mm box on an m^3 grid; returns dict of float32 arrays."""
+    n = m ** 3
+    arrs = {k: np.zeros(n, np.float32) for k in ("x", "y", "z", "rho", "T", "vx", "vy", "vz")}
+    _check(lib().fomo_gen_loop_cube(m, seed, t_in, t_out, noise_sigma, vamp, jitter, sausage,
+                                    *[_ptr(arrs[k], ctypes.c_float) for k in ("x", "y", "z", "rho", "T", "vx", "vy",
+                                                                              "vz")]))
+    return arrs
+
+
+def gen_goft_table(path, ion="fe_9", lambda0=171.0, atweight=56.0, nt=101, nrho=41, peak_logt=5.9, sigma=0.15):
+    _check(lib().fomo_gen_goft_table(path.encode(), ion.encode(), lambda0, atweight, nt, nrho, peak_logt, sigma))
+    return path
+
+
+def gen_corona(m, seed=3, sigma=0.1):
+    cnt = ctypes.c_int64()
+    _check(lib().fomo_gen_corona(m, seed, sigma, ctypes.byref(cnt), None, None, None, None))
+    arrs = {k: np.zeros(cnt.value, np.float32) for k in ("x", "y", "z", "rho")}
+    _check(lib().fomo_gen_corona(m, seed, sigma, ctypes.byref(cnt), *[_ptr(arrs[k], ctypes.c_float)
+                                                                      for k in ("x", "y", "z", "rho")]))
+    return arrs

@@ -1,0 +1,155 @@
+/*
+ * fomo_hip.h — C ABI of the MI355X render engine behind the FoMo-C API.
+ *
+ * Plain C: pointers, sizes and int status codes; no C++ types or exceptions cross it.
+ * Every function returns FOMO_OK (0) or a negative FOMO_ERR_* code; the message of the
+ * last failure on the calling thread is returned by fomo_last_error().
+ *
+ * Device state (datasets, emission cubes, G(T,n) tables, per-view scratch) is resident in
+ * HBM and addressed by integer ids. Host buffers passed in are read or written
+ * synchronously before the call returns; the caller owns them.
+ *
+ * Citations name the reference function each entry point replaces (fomo-c/src/...).
+ */
+#ifndef FOMO_HIP_H
+#define FOMO_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FOMO_OK 0
+#define FOMO_ERR_ARG (-1)      /* invalid argument or size */
+#define FOMO_ERR_HIP (-2)      /* HIP runtime failure */
+#define FOMO_ERR_NOTFOUND (-3) /* unknown id */
+#define FOMO_ERR_NOMEM (-4)    /* device allocation failed or over budget */
+#define FOMO_ERR_STATE (-5)    /* not initialised / wrong object kind */
+#define FOMO_ERR_IO (-6)       /* file missing or malformed */
+
+/* Message for the last failing call on this thread ("" if none). */
+const char* fomo_last_error(void);
+
+/* Library version string. */
+const char* fomo_version(void);
+
+/* Select and initialise a device (0 = first visible GPU). Idempotent. */
+int fomo_init(int device);
+/* Release every resident object and the device context. */
+int fomo_shutdown(void);
+/* Device name, HBM bytes and compute-unit count of the initialised device. */
+int fomo_device_info(char* name, int name_len, double* hbm_bytes, int* compute_units);
+
+/* ---------------------------------------------------------------------------------------
+ * Datasets. Replaces DataCube::setdata / push_back (fomo-datacube.cpp:205-279) on the device
+ * side: SoA float32 coordinates (dim 2 or 3; z is taken as 0 when dim==2) and nvars float32
+ * variables per point. The upload builds the view-invariant spatial index (uniform-grid cell
+ * list from a radix sort) that replaces the Boost R-tree (fomo-nearestneighbour.cpp:93-100).
+ * cell_scale multiplies the default cell edge (V/N)^(1/3); 0 selects the default 1.0.
+ * ------------------------------------------------------------------------------------- */
+int fomo_dataset_create(int64_t n, int dim, const float* x, const float* y, const float* z,
+                        int nvars, const float* const* vars, double cell_scale, int* out_id);
+int fomo_dataset_destroy(int id);
+/* n points, cells per axis, maximum points in one cell. */
+int fomo_dataset_info(int id, int64_t* n, int* grid_dims3, int* max_cell_count);
+/* Rebuild the spatial index with another cell_scale (used by the autotuner). */
+int fomo_dataset_reindex(int id, double cell_scale);
+
+/* ---------------------------------------------------------------------------------------
+ * G(T,n) tables. The host-side parse of readgoftfromchianti (fomo-CHIANTI.cpp:143-218) is
+ * fomo_table_read; fomo_table_create uploads a parsed table (nt log T values, nrho log n
+ * values, nrho*nt G values with T fastest).
+ * ------------------------------------------------------------------------------------- */
+/* Parse a table file. With logt==NULL only the header and sizes are filled. */
+int fomo_table_read(const char* path, char* ion, int ion_len, double* lambda0, double* atweight,
+                    char* unit, int unit_len, int* nt, int* nrho, float* logt, float* logn, float* g);
+int fomo_table_create(int nt, int nrho, const float* logt, const float* logn, const float* g, int* out_id);
+int fomo_table_destroy(int id);
+
+/* ---------------------------------------------------------------------------------------
+ * Emission cubes (the GoftCube on the device).
+ * fomo_emission_chianti replaces emissionfromdatacube (fomo-CHIANTI.cpp:228-322): variables 0,1
+ * of the dataset are n and T, 2..4 the velocity. Output per point: peak (n^2 G / fwhm times the
+ * normalisation) and fwhm, computed in FP64 and stored float32. imaging!=0 selects the Imaging
+ * branch (fwhm=1, normalisation 1); abundratio scales the spectroscopic normalisation.
+ * fomo_emission_thomson replaces thomsonfromdatacube (fomo-thomson.cpp:60-92): n*tot and n*pol.
+ * ------------------------------------------------------------------------------------- */
+int fomo_emission_chianti(int dataset, int table, double lambda0, double atweight, int imaging,
+                          double abundratio, int* out_goft);
+int fomo_emission_thomson(int dataset, int* out_goft);
+int fomo_goft_destroy(int id);
+/* Copy the two per-point emission variables back, in the dataset's original point order. */
+int fomo_goft_read(int goft, float* var0, float* var1);
+
+/* ---------------------------------------------------------------------------------------
+ * Rendering. meta (optional) receives 10 doubles per view:
+ *   minx,maxx,miny,maxy,minz,maxz (rotated extents), maxdistance, pathlength, scale, apix.
+ * Output intensity layout is RenderCube's: index (iy*nx + jx)*nl + il, lambda fastest.
+ * ------------------------------------------------------------------------------------- */
+
+/* One view of nearestneighbourinterpolation (fomo-nearestneighbour.cpp:31-281).
+ * lambda_width in m/s; dn!=0 applies the DN/arcsec scaling (:252-265). out: ny*nx*max(nl,1). */
+int fomo_render_nn(int goft, double l, double b, int nx, int ny, int nz, int nl, double lambda_width,
+                   int dn, float* out, double* meta);
+
+/* One view of thomsoninterpolation (fomo-thomson.cpp:94-367): pB image, out: ny*nx. */
+int fomo_render_thomson(int goft, double l, double b, int nx, int ny, int nz, float* out, double* meta);
+
+/* Nearest-neighbour choice of every LOS sample of one view (original point indices, -1 = none),
+ * layout (iy*nx + jx)*nz + k. thomson!=0 uses the Thomson geometry (fomo-thomson.cpp:136-317).
+ * For parity checks of the spatial index against the reference's R-tree query. */
+int fomo_nn_indices(int dataset, double l, double b, int nx, int ny, int nz, int thomson,
+                    int32_t* out_idx, double* meta);
+
+/* Batched render: every view x every emission cube (spectral lines) of one dataset, in one
+ * captured HIP graph. The cell list is shared by all views, the NN result of a view by all lines.
+ * No counterpart in the reference: it replaces the view loop of RenderWithNearestNeighbour
+ * (fomo-nearestneighbour.cpp:285-306) run once per line. Results stay resident on the device;
+ * copy them with fomo_batch_read. flags: bit0 = re-capture the graph even if shapes match. */
+int fomo_batch_create(int ngoft, const int* gofts, int nviews, const double* l, const double* b,
+                      int nx, int ny, int nz, int nl, double lambda_width, int dn, int* out_batch);
+int fomo_batch_run(int batch);               /* launch the graph and wait */
+int fomo_batch_launch(int batch);            /* launch the graph, do not wait */
+int fomo_batch_sync(int batch);
+/* out: [view][goft][ny][nx][nl] floats; meta: nviews*10 doubles (either may be NULL). */
+int fomo_batch_read(int batch, float* out, double* meta);
+int fomo_batch_destroy(int batch);
+
+/* ---------------------------------------------------------------------------------------
+ * Kernel timing with HIP events on the library's launch stream (for roofline reporting).
+ * When enabled, each launch of a named kernel is bracketed by events; stats return the
+ * number of timed launches and their total milliseconds.
+ * ------------------------------------------------------------------------------------- */
+int fomo_profile_enable(int on);
+int fomo_profile_reset(void);
+int fomo_profile_stats(const char* kernel, int64_t* count, double* total_ms);
+
+/* ---------------------------------------------------------------------------------------
+ * Synthetic inputs (host-only, deterministic from the seed; no GPU needed).
+ * fomo_gen_loop_cube: the straight cylindrical loop in a 10 Mm box on an m^3 grid
+ *   (x,y,z in [-5,5] Mm; loop axis z, radius 1 Mm, tanh edge 0.1 Mm; n 1e9.5 inside / 1e9 out,
+ *   T t_in / t_out; lognormal noise sigma on every field; Gaussian kink v of amplitude vamp m/s).
+ *   jitter (cells) displaces points uniformly; sausage (relative amplitude, wavelength 5 Mm)
+ *   modulates the density inside the loop. Outputs have m^3 entries each.
+ * fomo_gen_goft_table: G = 1e-24 exp(-(logT-peak)^2/(2 sigma^2)) (1+0.05 (logn-10)) on
+ *   log T in [4,8] (nt points) x log n in [8,12] (nrho points), written in the CHIANTI table
+ *   text format read by fomo_table_read.
+ * ------------------------------------------------------------------------------------- */
+int fomo_gen_loop_cube(int m, uint64_t seed, double t_in, double t_out, double noise_sigma,
+                       double vamp, double jitter, double sausage, float* x, float* y, float* z,
+                       float* rho, float* T, float* vx, float* vy, float* vz);
+int fomo_gen_goft_table(const char* path, const char* ion, double lambda0, double atweight,
+                        int nt, int nrho, double peak_logt, double sigma);
+/* Corona for Thomson scattering: the points of a Cartesian m^3 grid over [-3,3] Rsun (in Mm) with
+ * 1 < r/Rsun <= 3 (r <= Rsun has no finite scattering factor). n = 1e8 exp(-(r/Rsun-1)/0.07)(1+delta),
+ * delta a seeded random-phase field with power ~ k^(-11/3) and rms sigma. With x==NULL only the
+ * point count is stored in *count; otherwise the arrays must hold *count entries. */
+int fomo_gen_corona(int m, uint64_t seed, double sigma, int64_t* count, float* x, float* y, float* z,
+                    float* rho);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FOMO_HIP_H */

@@ -1,0 +1,181 @@
+"""ctypes wrapper of the CPU oracle (oracle/liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and bench.py's
+cpu_baseline leg, never by the product (fomo_amd/). Mode 0 = the reference's float pipeline,
+mode 1 = FP64 throughout (the accuracy target).
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+_LIB = None
+
+_f = ctypes.POINTER(ctypes.c_float)
+_d = ctypes.POINTER(ctypes.c_double)
+_i = ctypes.POINTER(ctypes.c_int32)
+_ip = ctypes.POINTER(ctypes.c_int)
+
+
+def build():
+    subprocess.check_call(["make", "-s", "-C", HERE])
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = ctypes.CDLL(LIB_PATH)
+        L.orc_table_read.restype = ctypes.c_int
+        L.orc_table_read.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int, _d, _d, ctypes.c_char_p,
+                                     ctypes.c_int, _ip, _ip, _f, _f, _f]
+        L.orc_emission.restype = ctypes.c_int64
+        L.orc_emission.argtypes = [ctypes.c_int, ctypes.c_int64, _f, _f, ctypes.c_int, ctypes.c_int, _f, _f, _f,
+                                   ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_double, _d, _d]
+        L.orc_thomson_factor.restype = None
+        L.orc_thomson_factor.argtypes = [ctypes.c_int, ctypes.c_double, _d, _d]
+        L.orc_thomson_emission.restype = None
+        L.orc_thomson_emission.argtypes = [ctypes.c_int, ctypes.c_int64, _f, _f, _f, _f, _d, _d]
+        L.orc_extents.restype = None
+        L.orc_extents.argtypes = [ctypes.c_int64, _f, _f, _f, ctypes.c_double, ctypes.c_double, _d]
+        for nm in ("orc_nn_bruteforce", "orc_nn_kdtree"):
+            fn = getattr(L, nm)
+            fn.restype = None
+            fn.argtypes = [ctypes.c_int64, _f, _f, _f, ctypes.c_int64, _d, _d, _d, ctypes.c_double, _i]
+        L.orc_render_nn.restype = ctypes.c_int
+        L.orc_render_nn.argtypes = [ctypes.c_int, ctypes.c_int64, _f, _f, _f, _d, _d, _f, _f, _f, ctypes.c_double,
+                                    ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                    ctypes.c_double, ctypes.c_double, ctypes.c_int, _d, _i, _d, _d, ctypes.c_int,
+                                    ctypes.c_int]
+        L.orc_render_thomson.restype = ctypes.c_int
+        L.orc_render_thomson.argtypes = [ctypes.c_int, ctypes.c_int64, _f, _f, _f, _d, _d, ctypes.c_double,
+                                         ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.c_int, _d, _i, _d, _d]
+        L.orc_num_threads.restype = ctypes.c_int
+        _LIB = L
+    return _LIB
+
+
+def _pf(a):
+    return np.ascontiguousarray(a, np.float32)
+
+
+def _pd(a):
+    return np.ascontiguousarray(a, np.float64)
+
+
+def P(a, t):
+    return a.ctypes.data_as(ctypes.POINTER(t))
+
+
+def read_table(path):
+    ion = ctypes.create_string_buffer(256)
+    unit = ctypes.create_string_buffer(1024)
+    l0, aw = ctypes.c_double(), ctypes.c_double()
+    nt, nr = ctypes.c_int(), ctypes.c_int()
+    rc = lib().orc_table_read(path.encode(), ion, 256, ctypes.byref(l0), ctypes.byref(aw), unit, 1024,
+                              ctypes.byref(nt), ctypes.byref(nr), None, None, None)
+    if rc != 0:
+        raise IOError(f"oracle table read failed ({rc}) for {path}")
+    logt = np.zeros(nt.value, np.float32)
+    logn = np.zeros(nr.value, np.float32)
+    g = np.zeros(nt.value * nr.value, np.float32)
+    lib().orc_table_read(path.encode(), ion, 256, ctypes.byref(l0), ctypes.byref(aw), unit, 1024, ctypes.byref(nt),
+                         ctypes.byref(nr), P(logt, ctypes.c_float), P(logn, ctypes.c_float), P(g, ctypes.c_float))
+    return dict(ion=ion.value.decode(), lambda0=l0.value, atweight=aw.value, unit=unit.value.decode(), nt=nt.value,
+                nrho=nr.value, logt=logt, logn=logn, g=g)
+
+
+def emission(mode, rho, T, table, imaging=False, abundratio=1.0):
+    rho, T = _pf(rho), _pf(T)
+    n = rho.size
+    peak = np.zeros(n)
+    fwhm = np.zeros(n)
+    lib().orc_emission(mode, n, P(rho, ctypes.c_float), P(T, ctypes.c_float), table["nt"], table["nrho"],
+                       P(_pf(table["logt"]), ctypes.c_float), P(_pf(table["logn"]), ctypes.c_float),
+                       P(_pf(table["g"]), ctypes.c_float), table["lambda0"], table["atweight"], 1 if imaging else 0,
+                       abundratio, P(peak, ctypes.c_double), P(fwhm, ctypes.c_double))
+    return peak, fwhm
+
+
+def thomson_factor(mode, r):
+    t, p = ctypes.c_double(), ctypes.c_double()
+    lib().orc_thomson_factor(mode, r, ctypes.byref(t), ctypes.byref(p))
+    return t.value, p.value
+
+
+def thomson_emission(mode, x, y, z, dens):
+    x, y, z, dens = _pf(x), _pf(y), _pf(z), _pf(dens)
+    n = x.size
+    tot, pol = np.zeros(n), np.zeros(n)
+    lib().orc_thomson_emission(mode, n, P(x, ctypes.c_float), P(y, ctypes.c_float), P(z, ctypes.c_float),
+                               P(dens, ctypes.c_float), P(tot, ctypes.c_double), P(pol, ctypes.c_double))
+    return tot, pol
+
+
+def extents(x, y, z, l, b):
+    x, y, z = _pf(x), _pf(y), _pf(z)
+    e = np.zeros(6)
+    lib().orc_extents(x.size, P(x, ctypes.c_float), P(y, ctypes.c_float), P(z, ctypes.c_float), l, b,
+                      P(e, ctypes.c_double))
+    return e
+
+
+def nn_query(x, y, z, px, py, pz, md, method="kdtree"):
+    x, y, z = _pf(x), _pf(y), _pf(z)
+    px, py, pz = _pd(px), _pd(py), _pd(pz)
+    out = np.zeros(px.size, np.int32)
+    fn = lib().orc_nn_kdtree if method == "kdtree" else lib().orc_nn_bruteforce
+    fn(x.size, P(x, ctypes.c_float), P(y, ctypes.c_float), P(z, ctypes.c_float), px.size, P(px, ctypes.c_double),
+       P(py, ctypes.c_double), P(pz, ctypes.c_double), md, P(out, ctypes.c_int32))
+    return out
+
+
+def render_nn(mode, x, y, z, peak, fwhm, vx, vy, vz, l, b, nx, ny, nz, nl, lambda_width, lambda0, dn=False,
+              want_idx=False, rows=None):
+    """Returns (image float64 [ny*nx*max(nl,1)], idx int32 [ny*nx*nz] or None, meta[10], timing[3])."""
+    x, y, z, vx, vy, vz = map(_pf, (x, y, z, vx, vy, vz))
+    peak, fwhm = _pd(peak), _pd(fwhm)
+    out = np.zeros(nx * ny * max(nl, 1))
+    idx = np.zeros(nx * ny * nz, np.int32) if want_idx else None
+    meta = np.zeros(10)
+    tim = np.zeros(3)
+    rc = lib().orc_render_nn(mode, x.size, P(x, ctypes.c_float), P(y, ctypes.c_float), P(z, ctypes.c_float),
+                             P(peak, ctypes.c_double), P(fwhm, ctypes.c_double), P(vx, ctypes.c_float),
+                             P(vy, ctypes.c_float), P(vz, ctypes.c_float), l, b, nx, ny, nz, nl, lambda_width,
+                             lambda0, 1 if dn else 0, P(out, ctypes.c_double),
+                             P(idx, ctypes.c_int32) if want_idx else None, P(meta, ctypes.c_double),
+                             P(tim, ctypes.c_double), rows[0] if rows else 0, rows[1] if rows else 0)
+    if rc != 0:
+        raise ValueError(f"oracle render_nn rejected the arguments ({rc})")
+    return out, idx, meta, tim
+
+
+def render_thomson(mode, x, y, z, totn, poln, l, b, nx, ny, nz, want_idx=False):
+    x, y, z = map(_pf, (x, y, z))
+    totn, poln = _pd(totn), _pd(poln)
+    out = np.zeros(nx * ny)
+    idx = np.zeros(nx * ny * nz, np.int32) if want_idx else None
+    meta = np.zeros(10)
+    tim = np.zeros(3)
+    rc = lib().orc_render_thomson(mode, x.size, P(x, ctypes.c_float), P(y, ctypes.c_float), P(z, ctypes.c_float),
+                                  P(totn, ctypes.c_double), P(poln, ctypes.c_double), l, b, nx, ny, nz,
+                                  P(out, ctypes.c_double), P(idx, ctypes.c_int32) if want_idx else None,
+                                  P(meta, ctypes.c_double), P(tim, ctypes.c_double))
+    if rc != 0:
+        raise ValueError(f"oracle render_thomson rejected the arguments ({rc})")
+    return out, idx, meta, tim
+
+
+def num_threads():
+    return lib().orc_num_threads()
+
+
+def rel_l2(a, ref):
+    a = np.asarray(a, np.float64)
+    ref = np.asarray(ref, np.float64)
+    den = np.linalg.norm(ref)
+    return float(np.linalg.norm(a - ref) / den) if den > 0 else float(np.linalg.norm(a - ref))

@@ -1,0 +1,199 @@
+"""GPU parity: the HIP engine (through the C ABI) against the CPU oracle.
+
+Bars: nearest-neighbour choices and view extents bit-exact; images within relative L2 1e-5 of
+the FP64 oracle; emissivity within float32 rounding of FP64.
+"""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+C = 2.99792458e8
+TOL_IMG = 1e-5
+
+
+def _dataset(F, d):
+    return F.Dataset(d["x"], d["y"], d["z"], [d["rho"], d["T"], d["vx"], d["vy"], d["vz"]])
+
+
+def _oracle_render(d, tab, l, b, nx, ny, nz, nl, width, mode=1, imaging=False, dn=False):
+    pk, fw = O.emission(mode, d["rho"], d["T"], tab, imaging=imaging)
+    return O.render_nn(mode, d["x"], d["y"], d["z"], pk, fw, d["vx"], d["vy"], d["vz"], l, b, nx, ny, nz, nl, width,
+                       tab["lambda0"], dn=dn, want_idx=True)
+
+
+def test_c0_emission_matches_fp64(engine, c0_data, fe9_table):
+    F = engine
+    path, tab = fe9_table
+    ds = _dataset(F, c0_data)
+    T = F.Table.from_file(path)
+    em = F.Emission.chianti(ds, T, tab["lambda0"], tab["atweight"])
+    peak, fwhm = em.read()
+    pk, fw = O.emission(1, c0_data["rho"], c0_data["T"], tab)
+    # stored float32 of an FP64 computation: within 1 float ulp (plus libm last-bit slack)
+    np.testing.assert_allclose(fwhm, fw, rtol=2e-7, atol=0)
+    np.testing.assert_allclose(peak, pk, rtol=4e-7, atol=0)
+    assert np.array_equal(peak == 0, pk == 0)
+    em.close(), T.close(), ds.close()
+
+
+def test_c0_full_render(engine, c0_data, fe9_table):
+    """Config 0 end to end: 64^3 loop, view (0, pi/2), 64x64 image, nz 64, nl 30, 0.2 A."""
+    F = engine
+    path, tab = fe9_table
+    d = c0_data
+    width = 0.2 * C / tab["lambda0"]
+    ds = _dataset(F, d)
+    T = F.Table.from_file(path)
+    em = F.Emission.chianti(ds, T, tab["lambda0"], tab["atweight"])
+    l, b, nx, ny, nz, nl = 0.0, np.pi / 2, 64, 64, 64, 30
+    img, meta = F.render_nn(em, l, b, nx, ny, nz, nl, width)
+    idx, imeta = F.nn_indices(ds, l, b, nx, ny, nz)
+    ref, ridx, rmeta, _ = _oracle_render(d, tab, l, b, nx, ny, nz, nl, width)
+    assert np.array_equal(meta[:7], rmeta[:7])
+    assert np.array_equal(idx, ridx), f"{(idx != ridx).sum()} NN choices differ"
+    err = O.rel_l2(img, ref)
+    print(f"C0 image rel L2 vs FP64 oracle: {err:.3e}")
+    assert err <= TOL_IMG
+    ref0, _, _, _ = _oracle_render(d, tab, l, b, nx, ny, nz, nl, width, mode=0)
+    print(f"C0 image rel L2 vs reference-float oracle: {O.rel_l2(img, ref0):.3e}")
+    em.close(), T.close(), ds.close()
+
+
+@pytest.mark.parametrize("l,b", [(0.3, 1.1), (1.0, np.pi / 3), (2.5, 0.2), (0.0, 0.0), (np.pi / 4, np.pi / 4)])
+def test_oblique_views(engine, fe9_table, l, b):
+    F = engine
+    path, tab = fe9_table
+    d = F.gen_loop_cube(28, seed=11)
+    width = 0.2 * C / tab["lambda0"]
+    ds = _dataset(F, d)
+    T = F.Table.from_file(path)
+    em = F.Emission.chianti(ds, T, tab["lambda0"], tab["atweight"])
+    nx, ny, nz, nl = 40, 36, 45, 30
+    img, meta = F.render_nn(em, l, b, nx, ny, nz, nl, width)
+    idx, _ = F.nn_indices(ds, l, b, nx, ny, nz)
+    ref, ridx, rmeta, _ = _oracle_render(d, tab, l, b, nx, ny, nz, nl, width)
+    assert np.array_equal(meta[:7], rmeta[:7])
+    assert np.array_equal(idx, ridx), f"{(idx != ridx).sum()} NN choices differ"
+    assert O.rel_l2(img, ref) <= TOL_IMG
+    em.close(), T.close(), ds.close()
+
+
+def test_jittered_points(engine, fe9_table):
+    """Unstructured (jittered) points, as in the large-scale configs."""
+    F = engine
+    path, tab = fe9_table
+    d = F.gen_loop_cube(30, seed=5, t_out=2.5e6, jitter=0.3, sausage=0.1)
+    width = 0.2 * C / tab["lambda0"]
+    ds = _dataset(F, d)
+    T = F.Table.from_file(path)
+    em = F.Emission.chianti(ds, T, tab["lambda0"], tab["atweight"])
+    for (l, b) in [(0.0, np.pi / 2), (0.7, 0.9)]:
+        nx, ny, nz, nl = 50, 50, 40, 20
+        img, _ = F.render_nn(em, l, b, nx, ny, nz, nl, width)
+        idx, _ = F.nn_indices(ds, l, b, nx, ny, nz)
+        ref, ridx, _, _ = _oracle_render(d, tab, l, b, nx, ny, nz, nl, width)
+        assert np.array_equal(idx, ridx)
+        assert O.rel_l2(img, ref) <= TOL_IMG
+    em.close(), T.close(), ds.close()
+
+
+def test_imaging_nl1(engine, fe9_table):
+    """lambda_pixel == 1 (Imaging): fwhm=1, intensity = sum of peaks (fomo-nearestneighbour.cpp:217-225)."""
+    F = engine
+    path, tab = fe9_table
+    d = F.gen_loop_cube(24, seed=3)
+    ds = _dataset(F, d)
+    T = F.Table.from_file(path)
+    em = F.Emission.chianti(ds, T, tab["lambda0"], tab["atweight"], imaging=True)
+    l, b, nx, ny, nz = 0.5, 1.2, 30, 30, 30
+    img, meta = F.render_nn(em, l, b, nx, ny, nz, 1, 1e5)
+    ref, ridx, rmeta, _ = _oracle_render(d, tab, l, b, nx, ny, nz, 1, 1e5, imaging=True)
+    assert O.rel_l2(img, ref) <= TOL_IMG
+    em.close(), T.close(), ds.close()
+
+
+def test_dn_scaling(engine, fe9_table):
+    F = engine
+    path, tab = fe9_table
+    d = F.gen_loop_cube(20, seed=4)
+    ds = _dataset(F, d)
+    T = F.Table.from_file(path)
+    em = F.Emission.chianti(ds, T, tab["lambda0"], tab["atweight"], imaging=True)
+    img, meta = F.render_nn(em, 0.2, 0.4, 24, 20, 16, 1, 1e5, dn=True)
+    ref, _, rmeta, _ = _oracle_render(d, tab, 0.2, 0.4, 24, 20, 16, 1, 1e5, imaging=True, dn=True)
+    assert meta[9] == rmeta[9] and meta[9] < 1e-5
+    assert O.rel_l2(img, ref) <= TOL_IMG
+    em.close(), T.close(), ds.close()
+
+
+def test_narrow_lines_direct_path(engine, fe9_table):
+    """Lines much narrower than a lambda pixel take the direct-exp pass; results still match."""
+    F = engine
+    path, tab = fe9_table
+    d = F.gen_loop_cube(16, seed=9)
+    ds = _dataset(F, d)
+    T = F.Table.from_file(path)
+    heavy = dict(tab)
+    heavy["atweight"] = 5.6e5  # thermal width / 100
+    em = F.Emission.chianti(ds, T, tab["lambda0"], heavy["atweight"])
+    width = 0.2 * C / tab["lambda0"]
+    img, _ = F.render_nn(em, 0.3, 0.8, 20, 20, 20, 40, width)
+    pk, fw = O.emission(1, d["rho"], d["T"], heavy)
+    ref, _, _, _ = O.render_nn(1, d["x"], d["y"], d["z"], pk, fw, d["vx"], d["vy"], d["vz"], 0.3, 0.8, 20, 20, 20, 40,
+                               width, tab["lambda0"])
+    assert O.rel_l2(img, ref) <= TOL_IMG
+    em.close(), T.close(), ds.close()
+
+
+def test_batch_equals_plain(engine, fe9_table, tmp_path):
+    """A batched graph (views x lines) gives bit-identical images to one render per view and line."""
+    F = engine
+    path, tab = fe9_table
+    p2 = F.gen_goft_table(str(tmp_path / "l2.dat"), peak_logt=6.2)
+    d = F.gen_loop_cube(24, seed=2)
+    ds = _dataset(F, d)
+    T1 = F.Table.from_file(path)
+    T2 = F.Table.from_file(p2)
+    e1 = F.Emission.chianti(ds, T1, tab["lambda0"], tab["atweight"])
+    e2 = F.Emission.chianti(ds, T2, tab["lambda0"], tab["atweight"])
+    ls = [0.0, 0.4, 1.3]
+    bs = [np.pi / 2, np.pi / 3, 0.5]
+    nx, ny, nz, nl = 32, 28, 30, 60
+    width = 0.2 * C / tab["lambda0"]
+    bt = F.Batch([e1, e2], ls, bs, nx, ny, nz, nl, width)
+    bt.run()
+    out, meta = bt.read()
+    for v in range(3):
+        for g, e in enumerate([e1, e2]):
+            img, m = F.render_nn(e, ls[v], bs[v], nx, ny, nz, nl, width)
+            assert np.array_equal(out[v, g].ravel(), img)
+            assert np.array_equal(meta[v], m)
+    bt.run()  # replay gives the same bits
+    out2, _ = bt.read()
+    assert np.array_equal(out, out2)
+    bt.close()
+    for o in (e1, e2, T1, T2, ds):
+        o.close()
+
+
+def test_thomson_small(engine):
+    F = engine
+    d = F.gen_corona(40, seed=3)
+    ds = F.Dataset(d["x"], d["y"], d["z"], [d["rho"]])
+    em = F.Emission.thomson(ds)
+    tot, pol = em.read()
+    rt, rp = O.thomson_emission(1, d["x"], d["y"], d["z"], d["rho"])
+    np.testing.assert_allclose(tot, rt, rtol=4e-7)
+    np.testing.assert_allclose(pol, rp, rtol=4e-7)
+    for (l, b) in [(0.0, np.pi / 2), (0.6, 0.05), (1.0, 0.5)]:
+        nx, ny, nz = 48, 40, 36
+        img, meta = F.render_thomson(em, l, b, nx, ny, nz)
+        idx, _ = F.nn_indices(ds, l, b, nx, ny, nz, thomson=True)
+        ref, ridx, rmeta, _ = O.render_thomson(1, d["x"], d["y"], d["z"], rt, rp, l, b, nx, ny, nz, want_idx=True)
+        assert np.array_equal(meta[:7], rmeta[:7])
+        assert np.array_equal(idx, ridx), f"{(idx != ridx).sum()} NN choices differ"
+        assert O.rel_l2(img, ref) <= TOL_IMG
+    em.close(), ds.close()
