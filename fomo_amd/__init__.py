@@ -59,6 +59,7 @@ _SIGS = {
     "fomo_batch_run": (ctypes.c_int, [ctypes.c_int]),
     "fomo_batch_launch": (ctypes.c_int, [ctypes.c_int]),
     "fomo_batch_sync": (ctypes.c_int, [ctypes.c_int]),
+    "fomo_batch_profile": (ctypes.c_int, [ctypes.c_int]),
     "fomo_batch_read": (ctypes.c_int, [ctypes.c_int, _c_flt_p, _c_dbl_p]),
     "fomo_batch_destroy": (ctypes.c_int, [ctypes.c_int]),
     "fomo_profile_enable": (ctypes.c_int, [ctypes.c_int]),
@@ -308,6 +309,10 @@ class Batch:
 
     def sync(self):
         _check(lib().fomo_batch_sync(self.id))
+
+    def profile(self):
+        """One eager pass of the batch's kernels with HIP events around each launch."""
+        _check(lib().fomo_batch_profile(self.id))
 
     def read(self):
         out = np.zeros(self.shape, np.float32)

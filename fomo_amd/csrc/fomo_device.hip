@@ -663,12 +663,9 @@ struct Batch {
   DevBuf partial, geom, idx, aux, out, flag;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
-  std::vector<hipEvent_t> ev;  // pairs around timed kernels (when profiling)
-  std::vector<std::string> evname;
   ~Batch() {
     if (exec) (void)hipGraphExecDestroy(exec);
     if (graph) (void)hipGraphDestroy(graph);
-    for (auto e : ev) (void)hipEventDestroy(e);
   }
 };
 
@@ -1286,7 +1283,8 @@ int fomo_nn_indices(int dataset, double l, double b, int nx, int ny, int nz, int
 // ---------------------------------------------------------------------------------------
 // batches: all views x lines in one captured graph
 // ---------------------------------------------------------------------------------------
-static int capture_batch(Batch& bt) {
+// Enqueue every view x line of a batch on stream s (captured into the graph, or eager when profiling).
+static int enqueue_batch(Batch& bt, hipStream_t s) {
   Dataset* ds = find_obj(g_ctx.datasets, bt.dataset);
   if (!ds) return fail(FOMO_ERR_NOTFOUND, "dataset of this batch was destroyed");
   std::vector<Goft*> gs;
@@ -1295,48 +1293,34 @@ static int capture_batch(Batch& bt) {
     if (!gc) return fail(FOMO_ERR_NOTFOUND, "emission cube of this batch was destroyed");
     gs.push_back(gc);
   }
-  if (bt.exec) { (void)hipGraphExecDestroy(bt.exec); bt.exec = nullptr; }
-  if (bt.graph) { (void)hipGraphDestroy(bt.graph); bt.graph = nullptr; }
-  for (auto e : bt.ev) (void)hipEventDestroy(e);
-  bt.ev.clear();
-  bt.evname.clear();
   const int nv = (int)bt.l.size();
   const int64_t R = (int64_t)bt.nx * bt.ny;
-  const int nlo = std::max(bt.nl, 1);
-  const size_t outv = (size_t)R * nlo;
-  hipStream_t s = g_ctx.stream;
-  const bool prof = g_ctx.profile;
-  HIPOK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-  // profiling of eager ProfScopes must not run inside capture
-  g_ctx.profile = false;
-  auto rec = [&](const char* name) {
-    if (!prof) return;
-    hipEvent_t e;
-    if (hipEventCreate(&e) == hipSuccess) {
-      bt.ev.push_back(e);
-      bt.evname.push_back(name);
-      (void)hipEventRecord(e, s);
-    }
-  };
+  const size_t outv = (size_t)R * std::max(bt.nl, 1);
   for (int v = 0; v < nv; ++v) {
     double* part = bt.partial.as<double>() + (size_t)v * 6 * kMaxPartials;
     ViewGeom* geom = bt.geom.as<ViewGeom>() + v;
-    rec("k_view_rotate_nn");
     enqueue_view_geom(s, *ds, MODE_NN, bt.l[v], bt.b[v], bt.nx, bt.ny, bt.nz, bt.dn, bt.aux.as<float>(), part, geom);
-    rec("k_view_rotate_nn");
-    rec("k_nn");
     enqueue_nn(s, *ds, geom, MODE_NN, bt.nx, bt.ny, bt.nz, bt.idx.as<int32_t>());
-    rec("k_nn");
     for (size_t gi = 0; gi < gs.size(); ++gi) {
       float* out = bt.out.as<float>() + ((size_t)v * gs.size() + gi) * outv;
-      rec(bt.nl > 1 ? "k_los_spectral" : "k_los_imaging");
       enqueue_los(s, bt.idx.as<int32_t>(), (int)R, bt.nz, gs[gi]->em.as<float2>(), bt.aux.as<float>(), geom, bt.nl,
                   bt.width, gs[gi]->lambda0, bt.flag.as<int32_t>(), out);
-      rec(bt.nl > 1 ? "k_los_spectral" : "k_los_imaging");
     }
   }
-  g_ctx.profile = prof;
+  return FOMO_OK;
+}
+
+static int capture_batch(Batch& bt) {
+  if (bt.exec) { (void)hipGraphExecDestroy(bt.exec); bt.exec = nullptr; }
+  if (bt.graph) { (void)hipGraphDestroy(bt.graph); bt.graph = nullptr; }
+  hipStream_t s = g_ctx.stream;
+  const bool prof = g_ctx.profile;
+  g_ctx.profile = false;  // no event records inside a capture
+  HIPOK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+  int rc = enqueue_batch(bt, s);
   hipError_t ce = hipStreamEndCapture(s, &bt.graph);
+  g_ctx.profile = prof;
+  if (rc) return rc;
   if (ce != hipSuccess) return fail(FOMO_ERR_HIP, std::string("graph capture: ") + hipGetErrorString(ce));
   HIPOK(hipGraphInstantiate(&bt.exec, bt.graph, nullptr, nullptr, 0));
   return FOMO_OK;
@@ -1376,18 +1360,6 @@ int fomo_batch_create(int ngoft, const int* gofts, int nviews, const double* l, 
   return FOMO_OK;
 }
 
-static int batch_collect(Batch& bt) {
-  for (size_t i = 0; i + 1 < bt.ev.size(); i += 2) {
-    float ms = 0;
-    if (hipEventElapsedTime(&ms, bt.ev[i], bt.ev[i + 1]) == hipSuccess) {
-      auto& st = g_ctx.prof[bt.evname[i]];
-      st.count++;
-      st.ms += ms;
-    }
-  }
-  return FOMO_OK;
-}
-
 int fomo_batch_launch(int id) {
   std::lock_guard<std::recursive_mutex> lk(g_mu);
   if (int rc = require_init()) return rc;
@@ -1403,7 +1375,21 @@ int fomo_batch_sync(int id) {
   Batch* bt = find_obj(g_ctx.batches, id);
   if (!bt) return fail(FOMO_ERR_NOTFOUND, "unknown batch id");
   HIPOK(hipStreamSynchronize(g_ctx.stream));
-  return batch_collect(*bt);
+  return FOMO_OK;
+}
+
+int fomo_batch_profile(int id) {
+  std::lock_guard<std::recursive_mutex> lk(g_mu);
+  if (int rc = require_init()) return rc;
+  Batch* bt = find_obj(g_ctx.batches, id);
+  if (!bt) return fail(FOMO_ERR_NOTFOUND, "unknown batch id");
+  const bool prof = g_ctx.profile;
+  g_ctx.profile = true;
+  int rc = enqueue_batch(*bt, g_ctx.stream);
+  HIPOK(hipGetLastError());
+  int rc2 = sync_stream();
+  g_ctx.profile = prof;
+  return rc ? rc : rc2;
 }
 
 int fomo_batch_run(int id) {

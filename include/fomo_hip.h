@@ -106,12 +106,15 @@ int fomo_nn_indices(int dataset, double l, double b, int nx, int ny, int nz, int
  * captured HIP graph. The cell list is shared by all views, the NN result of a view by all lines.
  * No counterpart in the reference: it replaces the view loop of RenderWithNearestNeighbour
  * (fomo-nearestneighbour.cpp:285-306) run once per line. Results stay resident on the device;
- * copy them with fomo_batch_read. flags: bit0 = re-capture the graph even if shapes match. */
+ * copy them with fomo_batch_read. */
 int fomo_batch_create(int ngoft, const int* gofts, int nviews, const double* l, const double* b,
                       int nx, int ny, int nz, int nl, double lambda_width, int dn, int* out_batch);
 int fomo_batch_run(int batch);               /* launch the graph and wait */
 int fomo_batch_launch(int batch);            /* launch the graph, do not wait */
 int fomo_batch_sync(int batch);
+/* Run the batch's kernels once eagerly (same kernels, same order, same stream) with HIP events
+ * around each launch, accumulating per-kernel durations into the profile stats below. */
+int fomo_batch_profile(int batch);
 /* out: [view][goft][ny][nx][nl] floats; meta: nviews*10 doubles (either may be NULL). */
 int fomo_batch_read(int batch, float* out, double* meta);
 int fomo_batch_destroy(int batch);

@@ -1,0 +1,48 @@
+// Drives the FoMo C++ API the way fomo-c/example/example.cpp does: push_back_datapoint,
+// setchiantifile, setrendermethod, setresolution, render, readrendering, writegoftcube.
+// usage: api_render <table> <outprefix> <method> <m> <nx> <ny> <nz> <nl> <l> <b>
+#include <cstdlib>
+#include <iostream>
+#include <vector>
+
+#include "FoMo.h"
+#include "fomo_hip.h"
+
+int main(int argc, char** argv) {
+  if (argc < 11) {
+    std::cerr << "usage: api_render table outprefix method m nx ny nz nl l b" << std::endl;
+    return 2;
+  }
+  const std::string table = argv[1], out = argv[2], method = argv[3];
+  const int m = std::atoi(argv[4]), nx = std::atoi(argv[5]), ny = std::atoi(argv[6]);
+  const int nz = std::atoi(argv[7]), nl = std::atoi(argv[8]);
+  const double l = std::atof(argv[9]), b = std::atof(argv[10]);
+  const size_t n = (size_t)m * m * m;
+  std::vector<float> x(n), y(n), z(n), rho(n), T(n), vx(n), vy(n), vz(n);
+  if (fomo_gen_loop_cube(m, 1, 1e6, 2e6, 0.05, 2e4, 0.0, 0.0, x.data(), y.data(), z.data(), rho.data(), T.data(),
+                         vx.data(), vy.data(), vz.data()) != 0)
+    return 3;
+  FoMo::FoMoObject Object;
+  std::vector<std::string> unitvec{"Mm", "Mm", "Mm", "cm^{-3}", "K", "m s^{-1}", "m s^{-1}", "m s^{-1}"};
+  for (size_t i = 0; i < n; ++i) {
+    std::vector<double> c{x[i], y[i], z[i]};
+    std::vector<double> v{rho[i], T[i], vx[i], vy[i], vz[i]};
+    Object.push_back_datapoint(c, v, &unitvec);
+  }
+  Object.setchiantifile(table);
+  Object.setabundfile("/empty");
+  Object.setrendermethod(method);
+  Object.setobservationtype(FoMo::Spectroscopic);
+  Object.setresolution(nx, ny, nz, nl, 0.2 * 2.99792458e8 / 171.0);
+  Object.setoutfile(out);
+  Object.setwriteoutbinary(true);
+  Object.setwriteouttext(false);
+  Object.render(l, b);
+  FoMo::RenderCube rc = Object.readrendering();
+  int rx, ry, rz, rl;
+  double w;
+  rc.readresolution(rx, ry, rz, rl, w);
+  std::cout << "rendered " << rx << "x" << ry << "x" << rz << " nl " << rl << " method " << rc.readrendermethod()
+            << " ng " << rc.readngrid() << " nvars " << rc.readnvars() << std::endl;
+  return 0;
+}

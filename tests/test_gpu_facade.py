@@ -1,0 +1,59 @@
+"""GPU: the FoMo.h C++ API end to end (push_back_datapoint ... render ... writegoftcube), checked
+against the C ABI render and the FP64 oracle, through the binary file it writes."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import fomo_amd as F
+from oracle import oracle as O
+from tests.fomo_reader import read_dat
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+PKG = os.path.join(ROOT, "fomo_amd")
+C = 2.99792458e8
+
+
+@pytest.fixture(scope="module")
+def api_bin(tmp_path_factory):
+    out = str(tmp_path_factory.mktemp("bin") / "api_render")
+    subprocess.check_call(["g++", "-std=c++17", "-O1", "-I" + os.path.join(ROOT, "include"),
+                           os.path.join(ROOT, "tests", "cpp", "api_render.cpp"), "-o", out, "-L" + PKG, "-lFoMo",
+                           "-lfomo_hip", "-Wl,-rpath," + PKG])
+    return out
+
+
+def test_facade_nn_render_file(api_bin, fe9_table, tmp_path):
+    path, tab = fe9_table
+    m, nx, ny, nz, nl, l, b = 20, 32, 24, 30, 30, 0.3, 1.0
+    prefix = str(tmp_path / "run.")
+    r = subprocess.run([api_bin, path, prefix, "NearestNeighbour", str(m), str(nx), str(ny), str(nz), str(nl), str(l),
+                        str(b)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    f = prefix + "l017b057.dat"
+    assert os.path.exists(f), os.listdir(tmp_path)
+    got = read_dat(f)
+    assert got["dim"] == 3 and got["nvars"] == 1 and got["ng"] == nx * ny * nl
+    assert got["units"][:3] == ["Mm", "Mm", "\\AA{}"] and got["chiantifile"] == path
+    d = F.gen_loop_cube(m, seed=1)
+    pk, fw = O.emission(1, d["rho"], d["T"], tab)
+    width = 0.2 * C / 171.0
+    ref, _, meta, _ = O.render_nn(1, d["x"], d["y"], d["z"], pk, fw, d["vx"], d["vy"], d["vz"], l, b, nx, ny, nz, nl,
+                                  width, 171.0)
+    assert O.rel_l2(got["cols"][3], ref) <= 1e-5
+    # grid columns: x, y, lambda per element (fomo-nearestneighbour.cpp:209-211)
+    j = np.arange(nx)
+    x = (j / (nx - 1) * (meta[1] - meta[0]) + meta[0]).astype(np.float32)
+    assert np.array_equal(got["cols"][0].reshape(ny, nx, nl)[5, :, 3], x)
+    lam = (np.arange(nl) / (nl - 1) * (width * 171.0 / C) - width * 171.0 / C / 2 + 171.0).astype(np.float32)
+    assert np.array_equal(got["cols"][2].reshape(ny, nx, nl)[2, 7, :], lam)
+
+
+def test_facade_unknown_method_exits(api_bin, fe9_table, tmp_path):
+    path, _ = fe9_table
+    r = subprocess.run([api_bin, path, str(tmp_path / "x."), "NoSuchMethod", "8", "8", "8", "8", "4", "0", "0"],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 1 and "unknown rendering method" in r.stderr
