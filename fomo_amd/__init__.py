@@ -37,6 +37,8 @@ _SIGS = {
     "fomo_dataset_destroy": (ctypes.c_int, [ctypes.c_int]),
     "fomo_dataset_info": (ctypes.c_int, [ctypes.c_int, _c_i64_p, _c_int_p, _c_int_p]),
     "fomo_dataset_reindex": (ctypes.c_int, [ctypes.c_int, ctypes.c_double]),
+    "fomo_dataset_set_lookup": (ctypes.c_int, [ctypes.c_int, ctypes.c_int]),
+    "fomo_dataset_lookup_info": (ctypes.c_int, [ctypes.c_int, _c_int_p, _c_i64_p, _c_i64_p, _c_i64_p]),
     "fomo_table_read": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int, _c_dbl_p, _c_dbl_p,
                                        ctypes.c_char_p, ctypes.c_int, _c_int_p, _c_int_p, _c_flt_p, _c_flt_p,
                                        _c_flt_p]),
@@ -175,6 +177,17 @@ class Dataset:
         mx = ctypes.c_int()
         _check(lib().fomo_dataset_info(self.id, ctypes.byref(n), dims, ctypes.byref(mx)))
         return n.value, tuple(dims), mx.value
+
+    def set_lookup(self, subdiv):
+        """Candidate lookup grid subdivision (0 = ring search only). Results do not change."""
+        _check(lib().fomo_dataset_set_lookup(self.id, subdiv))
+
+    def lookup_info(self):
+        sd = ctypes.c_int()
+        cells, cand, fb = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64()
+        _check(lib().fomo_dataset_lookup_info(self.id, ctypes.byref(sd), ctypes.byref(cells), ctypes.byref(cand),
+                                              ctypes.byref(fb)))
+        return dict(subdiv=sd.value, cells=cells.value, candidates=cand.value, fallback_cells=fb.value)
 
     def reindex(self, cell_scale=0.0):
         _check(lib().fomo_dataset_reindex(self.id, cell_scale))

@@ -17,6 +17,7 @@
 #include <hip/hip_runtime.h>
 #include <cstring>
 #include <rocprim/device/device_radix_sort.hpp>
+#include <rocprim/device/device_scan.hpp>
 
 #include <algorithm>
 #include <cmath>
@@ -105,6 +106,16 @@ struct GridInfo {
   int pad;
 };
 
+// Candidate lookup grid: a finer grid over the data bbox whose cells list the only points that
+// can be the (unconstrained) nearest point of any target inside the cell.
+struct LookupInfo {
+  double o[3];      // origin (shifted off the cell-list lattice)
+  double h, invh;   // lookup cell edge
+  double eps;       // box expansion covering the rounding of a target's cell assignment
+  int dims[3];
+  int enabled;
+};
+
 // Per-view geometry, filled on the device by k_view_finalize.
 struct ViewGeom {
   double cb, sb, cl, sl;          // trig of b and l for the LOS samples (host glibc)
@@ -115,6 +126,16 @@ struct ViewGeom {
 };
 
 enum { MODE_NN = 0, MODE_THOMSON = 1 };
+
+// Spectral lines rendered together from one NN result (blockIdx.y selects the line).
+constexpr int kMaxLines = 8;
+struct LineSet {
+  const float2* em[kMaxLines];  // (peak, fwhm) per sorted point
+  float* out[kMaxLines];        // R*nl output per line
+  double wla[kMaxLines];        // lambda window width in Angstrom
+  double l0c[kMaxLines];        // lambda0 / c
+  int n;
+};
 
 constexpr int kBlock = 256;
 constexpr int kMaxPartials = 2048;  // rotate kernel grid cap (grid-stride beyond)
@@ -202,7 +223,14 @@ __global__ void k_cell_maxcount(const int32_t* __restrict__ cstart, int64_t ncel
   int v = 0;
   if (c < ncells) v = cstart[c + 1] - cstart[c];
   v = wave_max(v);
-  if ((threadIdx.x & 63) == 0 && v > 0) atomicMax(mx, v);
+  __shared__ int red[kBlock / 64];
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int m = red[0];
+    for (int w = 1; w < (int)(blockDim.x >> 6); ++w) m = max(m, red[w]);
+    if (m > 0) atomicMax(mx, m);
+  }
 }
 
 __global__ void k_permute_points(const float* __restrict__ x, const float* __restrict__ y, const float* __restrict__ z,
@@ -389,6 +417,201 @@ __global__ void k_view_finalize(const double* __restrict__ partial, int np, int 
 }
 
 // ------------------------------------------------------------------------------------
+// K3b: candidate lookup grid (once per dataset). For lookup cell B (expanded by eps):
+//  U   = min over nearby points of maxdist(B, q)           (any target in B has a point within U)
+//  C(B) = { q : mindist(B,q) <= U' and q not dominated over B by one of the 4 points with the
+//           smallest maxdist },  q dominated by p over B  <=>  min_{t in B}(|t-q|^2-|t-p|^2) > margin.
+// Margins are relative 1e-9 / 1e-10, far above FP64 rounding of the query distances, so the
+// query's argmin over C(B) (by double distance, then lowest index) is the argmin over all points.
+// Cells with no point within 3 coarse rings, or more than cap candidates, are marked -1 and their
+// queries use the exact ring search.
+// ------------------------------------------------------------------------------------
+constexpr int kLookupCap = 24;
+constexpr int kLookupRings = 3;
+
+__device__ __forceinline__ double box_maxdist2(const double lo[3], const double hi[3], const float4 q) {
+  const double a0 = fmax(fabs((double)q.x - lo[0]), fabs((double)q.x - hi[0]));
+  const double a1 = fmax(fabs((double)q.y - lo[1]), fabs((double)q.y - hi[1]));
+  const double a2 = fmax(fabs((double)q.z - lo[2]), fabs((double)q.z - hi[2]));
+  return a0 * a0 + a1 * a1 + a2 * a2;
+}
+__device__ __forceinline__ double box_mindist2(const double lo[3], const double hi[3], const float4 q) {
+  const double a0 = fmax(0.0, fmax(lo[0] - (double)q.x, (double)q.x - hi[0]));
+  const double a1 = fmax(0.0, fmax(lo[1] - (double)q.y, (double)q.y - hi[1]));
+  const double a2 = fmax(0.0, fmax(lo[2] - (double)q.z, (double)q.z - hi[2]));
+  return a0 * a0 + a1 * a1 + a2 * a2;
+}
+// q dominated by p over the box (center c, half widths hw): every t in the box is closer to p
+__device__ __forceinline__ bool dominated(const double c[3], const double hw[3], const float4 q, const float4 p) {
+  const double u0 = (double)q.x - c[0], u1 = (double)q.y - c[1], u2 = (double)q.z - c[2];
+  const double w0 = (double)p.x - c[0], w1 = (double)p.y - c[1], w2 = (double)p.z - c[2];
+  const double uu = u0 * u0 + u1 * u1 + u2 * u2, ww = w0 * w0 + w1 * w1 + w2 * w2;
+  const double gmin = (uu - ww) - 2.0 * (hw[0] * fabs(u0 - w0) + hw[1] * fabs(u1 - w1) + hw[2] * fabs(u2 - w2));
+  const double margin = 1e-10 * (uu + ww + 4.0 * (hw[0] * hw[0] + hw[1] * hw[1] + hw[2] * hw[2]));
+  return gmin > margin;
+}
+
+template <bool FILL>
+__global__ void k_lookup_build(const float4* __restrict__ pts, const int32_t* __restrict__ cstart, GridInfo g,
+                               LookupInfo lk, int64_t ncl, int32_t* __restrict__ cnt, const int64_t* __restrict__ off,
+                               int32_t* __restrict__ cand) {
+  const int64_t L = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (L >= ncl) return;
+  if (FILL && cnt[L] <= 0) return;
+  const int lx = (int)(L % lk.dims[0]);
+  const int ly = (int)((L / lk.dims[0]) % lk.dims[1]);
+  const int lz = (int)(L / ((int64_t)lk.dims[0] * lk.dims[1]));
+  const int li[3] = {lx, ly, lz};
+  double lo[3], hi[3], c[3], hw[3];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    lo[a] = lk.o[a] + (double)li[a] * lk.h - lk.eps;
+    hi[a] = lk.o[a] + (double)(li[a] + 1) * lk.h + lk.eps;
+    c[a] = 0.5 * (lo[a] + hi[a]);
+    hw[a] = 0.5 * (hi[a] - lo[a]);
+  }
+  const int cc0 = cell_coord(g, 0, c[0]), cc1 = cell_coord(g, 1, c[1]), cc2 = cell_coord(g, 2, c[2]);
+  const int dx = g.dims[0], dxy = g.dims[0] * g.dims[1];
+  // step 1: upper bound U and dominators
+  double U2 = INFINITY;
+  float4 dom[4];
+  int dpos[4];
+  double dkey[4];
+  int nd = 0;
+  int found = -1;
+  for (int r = 0; r <= kLookupRings; ++r) {
+    for (int z = max(cc2 - r, 0); z <= min(cc2 + r, g.dims[2] - 1); ++z)
+      for (int y = max(cc1 - r, 0); y <= min(cc1 + r, g.dims[1] - 1); ++y)
+        for (int x = max(cc0 - r, 0); x <= min(cc0 + r, g.dims[0] - 1); ++x) {
+          if (max(max(abs(x - cc0), abs(y - cc1)), abs(z - cc2)) != r) continue;
+          const int cell = z * dxy + y * dx + x;
+          for (int p = cstart[cell]; p < cstart[cell + 1]; ++p) {
+            const float4 q = pts[p];
+            const double m2 = box_maxdist2(lo, hi, q);
+            U2 = fmin(U2, m2);
+            // keep the 4 smallest m2
+            int k = nd < 4 ? nd++ : 4;
+            if (k == 4) {
+              if (m2 >= dkey[3]) continue;
+              k = 3;
+            }
+            while (k > 0 && dkey[k - 1] > m2) {
+              dkey[k] = dkey[k - 1];
+              dom[k] = dom[k - 1];
+              dpos[k] = dpos[k - 1];
+              --k;
+            }
+            dkey[k] = m2;
+            dom[k] = q;
+            dpos[k] = p;
+          }
+        }
+    if (found < 0 && nd > 0) found = r;
+    if (found >= 0 && r >= found + 1) break;
+  }
+  if (nd == 0) {
+    if (!FILL) cnt[L] = -1;
+    return;
+  }
+  const double Up = sqrt(U2) * (1.0 + 1e-9) + 1e-9 * lk.h;
+  const double Up2 = Up * Up;
+  int r0[3], r1[3];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    r0[a] = cell_coord(g, a, lo[a] - Up);
+    r1[a] = cell_coord(g, a, hi[a] + Up);
+  }
+  if ((int64_t)(r1[0] - r0[0] + 1) * (r1[1] - r0[1] + 1) * (r1[2] - r0[2] + 1) > 729) {
+    if (!FILL) cnt[L] = -1;
+    return;
+  }
+  int n = 0;
+  const int64_t base = FILL ? off[L] : 0;
+  for (int z = r0[2]; z <= r1[2]; ++z)
+    for (int y = r0[1]; y <= r1[1]; ++y) {
+      const int row = z * dxy + y * dx;
+      for (int p = cstart[row + r0[0]]; p < cstart[row + r1[0] + 1]; ++p) {
+        const float4 q = pts[p];
+        if (box_mindist2(lo, hi, q) > Up2) continue;
+        bool dm = false;
+        for (int k = 0; k < nd && !dm; ++k)
+          if (dpos[k] != p && dominated(c, hw, q, dom[k])) dm = true;
+        if (dm) continue;
+        if (FILL) cand[base + n] = p;
+        ++n;
+        if (!FILL && n > kLookupCap) {
+          cnt[L] = -1;
+          return;
+        }
+      }
+    }
+  if (!FILL) cnt[L] = n;
+}
+
+__global__ void k_lookup_pack(const int32_t* __restrict__ cnt, const int64_t* __restrict__ off, int64_t ncl,
+                              int2* __restrict__ lcell) {
+  const int64_t L = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (L >= ncl) return;
+  lcell[L] = make_int2((int)off[L], cnt[L]);
+}
+
+__global__ void k_count_negative(const int32_t* __restrict__ cnt, int64_t ncl, int* __restrict__ out) {
+  const int64_t L = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  int v = (L < ncl && cnt[L] < 0) ? 1 : 0;
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  __shared__ int red[kBlock / 64];
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int t = 0;
+    for (int w = 0; w < (int)(blockDim.x >> 6); ++w) t += red[w];
+    if (t) atomicAdd(out, t);
+  }
+}
+
+__global__ void k_clamp_counts(const int32_t* __restrict__ cnt, int64_t ncl, int32_t* __restrict__ pos) {
+  const int64_t L = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (L >= ncl) return;
+  pos[L] = max(cnt[L], 0);
+}
+
+// Fast path: the unconstrained nearest point from the target's lookup cell. Returns its sorted
+// position if it lies strictly inside the search box, else -2 (caller runs the ring search).
+__device__ __forceinline__ int nn_lookup(const float4* __restrict__ pts, const int2* __restrict__ lcell,
+                                         const int32_t* __restrict__ lcand, const LookupInfo& lk, float t0, float t1,
+                                         float t2, float lo0, float lo1, float lo2, float hi0, float hi1, float hi2) {
+  const double v0 = ((double)t0 - lk.o[0]) * lk.invh;
+  const double v1 = ((double)t1 - lk.o[1]) * lk.invh;
+  const double v2 = ((double)t2 - lk.o[2]) * lk.invh;
+  if (!(v0 >= 0.0 && v0 < (double)lk.dims[0] && v1 >= 0.0 && v1 < (double)lk.dims[1] && v2 >= 0.0 &&
+        v2 < (double)lk.dims[2]))
+    return -2;
+  const int64_t L = ((int64_t)(int)v2 * lk.dims[1] + (int)v1) * lk.dims[0] + (int)v0;
+  const int2 e = lcell[L];
+  if (e.y <= 0) return -2;
+  double best = INFINITY;
+  int bidx = 0x7fffffff, bpos = -1;
+  float4 bq = make_float4(0.f, 0.f, 0.f, 0.f);
+  for (int i = 0; i < e.y; ++i) {
+    const int p = lcand[e.x + i];
+    const float4 q = pts[p];
+    const double d0 = (double)q.x - (double)t0;
+    const double d1 = (double)q.y - (double)t1;
+    const double d2 = (double)q.z - (double)t2;
+    const double d = (d0 * d0 + d1 * d1) + d2 * d2;
+    const int oi = __float_as_int(q.w);
+    if (d < best || (d == best && oi < bidx)) {
+      best = d;
+      bidx = oi;
+      bpos = p;
+      bq = q;
+    }
+  }
+  if (lo0 < bq.x && bq.x < hi0 && lo1 < bq.y && bq.y < hi1 && lo2 < bq.z && bq.z < hi2) return bpos;
+  return -2;
+}
+
+// ------------------------------------------------------------------------------------
 // K4: nearest-neighbour query per LOS sample (fomo-nearestneighbour.cpp:148-193).
 // Semantics of rtree.query(nearest(target,1) && within(box)): nearest point strictly inside
 // the float box [float(p-md), float(p+md)], by double squared distance of float coordinates
@@ -477,8 +700,10 @@ __device__ int nn_query(const float4* __restrict__ pts, const int32_t* __restric
 
 // One thread per LOS sample. Block = 16x16 rays at one k (blockIdx.y = k).
 __global__ void __launch_bounds__(256) k_nn(const float4* __restrict__ pts, const int32_t* __restrict__ cstart,
-                                            GridInfo g, const ViewGeom* __restrict__ vgp, int mode, int nx, int ny,
-                                            int nz, int32_t* __restrict__ idx) {
+                                            GridInfo g, const int2* __restrict__ lcell,
+                                            const int32_t* __restrict__ lcand, LookupInfo lk,
+                                            const ViewGeom* __restrict__ vgp, int mode, int nx, int ny, int nz,
+                                            int32_t* __restrict__ idx) {
   const int tilesx = (nx + 15) >> 4;
   const int tx = blockIdx.x % tilesx, ty = blockIdx.x / tilesx;
   const int j = tx * 16 + (threadIdx.x & 15);
@@ -495,7 +720,13 @@ __global__ void __launch_bounds__(256) k_nn(const float4* __restrict__ pts, cons
   const double p1 = -xx * cb * sl + yy * cl - zz * sb * sl;
   const double p2 = -xx * sb + zz * cb;
   (void)mode;
-  const int r = nn_query(pts, cstart, g, p0, p1, p2, vg.md);
+  int r = -2;
+  if (lk.enabled) {
+    const double md = vg.md;
+    r = nn_lookup(pts, lcell, lcand, lk, (float)p0, (float)p1, (float)p2, (float)(p0 - md), (float)(p1 - md),
+                  (float)(p2 - md), (float)(p0 + md), (float)(p1 + md), (float)(p2 + md));
+  }
+  if (r == -2) r = nn_query(pts, cstart, g, p0, p1, p2, vg.md);
   idx[(size_t)k * ((size_t)nx * ny) + (size_t)i * nx + j] = r;
 }
 
@@ -521,12 +752,15 @@ __global__ void k_idx_to_orig(const int32_t* __restrict__ idx, const int32_t* __
 // ------------------------------------------------------------------------------------
 template <int CH, bool DIRECT>
 __global__ void __launch_bounds__(256) k_los_spectral(const int32_t* __restrict__ idx, int R, int nz,
-                                                      const float2* __restrict__ em, const float* __restrict__ vlos,
-                                                      const ViewGeom* __restrict__ vgp, double wla, double lam0_over_c,
-                                                      int nl, int l0, int32_t* __restrict__ flag,
-                                                      float* __restrict__ out) {
+                                                      const LineSet ls, const float* __restrict__ vlos,
+                                                      const ViewGeom* __restrict__ vgp, int nl, int l0,
+                                                      int32_t* __restrict__ flags) {
   const int ray = blockIdx.x * blockDim.x + threadIdx.x;
   if (ray >= R) return;
+  const int line = blockIdx.y;
+  const float2* __restrict__ em = ls.em[line];
+  const double wla = ls.wla[line], lam0_over_c = ls.l0c[line];
+  int32_t* __restrict__ flag = flags + (size_t)line * R;
   if (DIRECT && flag[ray] == 0) return;
   const double dl = wla / (nl - 1);
   const double lamc = double(l0) / (nl - 1) * wla - wla / 2.;  // lambdaval of the first pixel of the pass (:200)
@@ -575,7 +809,7 @@ __global__ void __launch_bounds__(256) k_los_spectral(const int32_t* __restrict_
     if (bad) return;
   }
   const double sc = vgp->scale;
-  float* o = out + (size_t)ray * nl + l0;
+  float* o = ls.out[line] + (size_t)ray * nl + l0;
 #pragma unroll
   for (int c = 0; c < CH; ++c)
     if (l0 + c < nl) o[c] = (float)(sc * acc[c]);
@@ -583,10 +817,11 @@ __global__ void __launch_bounds__(256) k_los_spectral(const int32_t* __restrict_
 
 // nl == 1 (Imaging): intens += peak (fomo-nearestneighbour.cpp:217-225)
 __global__ void __launch_bounds__(256) k_los_imaging(const int32_t* __restrict__ idx, int R, int nz,
-                                                     const float2* __restrict__ em, const ViewGeom* __restrict__ vgp,
-                                                     float* __restrict__ out) {
+                                                     const LineSet ls, const ViewGeom* __restrict__ vgp) {
   const int ray = blockIdx.x * blockDim.x + threadIdx.x;
   if (ray >= R) return;
+  const float2* __restrict__ em = ls.em[blockIdx.y];
+  float* __restrict__ out = ls.out[blockIdx.y];
   double acc = 0.0;
   for (int k = 0; k < nz; ++k) {
     const int p = idx[(size_t)k * R + ray];
@@ -631,6 +866,11 @@ struct Dataset {
   DevBuf cstart;       // int32 [ncells+1]
   std::vector<std::unique_ptr<DevBuf>> svars;  // sorted variables
   DevBuf vel;          // float4 sorted (vx,vy,vz,0) when nvars>=5
+  LookupInfo lk{};     // candidate lookup grid (lk.enabled==0: ring search only)
+  int lookup_subdiv = 2;
+  DevBuf lcell;        // int2 (offset, count) per lookup cell; count -1 = use the ring search
+  DevBuf lcand;        // int32 candidate sorted positions
+  int64_t ncand = 0, nlcells = 0, nfallback = 0;
 };
 
 struct Table {
@@ -752,6 +992,78 @@ int sync_stream() {
 
 inline unsigned nblk(int64_t n, int bs = kBlock) { return (unsigned)((n + bs - 1) / bs); }
 
+// Build the candidate lookup grid of a dataset (after its cell list). subdiv lookup cells per
+// cell-list edge; 0 disables the fast path. The grid is shifted by 0.3 cell off the cell lattice.
+int build_lookup(Dataset& ds) {
+  ds.lk = LookupInfo{};
+  ds.lcell.release();
+  ds.lcand.release();
+  ds.ncand = ds.nlcells = ds.nfallback = 0;
+  if (ds.lookup_subdiv <= 0) return FOMO_OK;
+  hipStream_t s = g_ctx.stream;
+  const GridInfo& g = ds.grid;
+  LookupInfo lk{};
+  lk.h = g.h / ds.lookup_subdiv;
+  lk.invh = 1.0 / lk.h;
+  double ext = 0, mag = 0;
+  int64_t ncl = 1;
+  for (int a = 0; a < 3; ++a) {
+    lk.o[a] = g.o[a] - 0.3 * lk.h;
+    const double e = (double)g.bmax[a] - g.o[a];
+    lk.dims[a] = (int)std::floor((e + 0.3 * lk.h) / lk.h) + 1;
+    ncl *= lk.dims[a];
+    ext = std::max(ext, e);
+    mag = std::max(mag, std::max(std::fabs((double)g.bmin[a]), std::fabs((double)g.bmax[a])));
+  }
+  // keep the lookup grid within ~8 cells per point and 2^30 cells
+  if (ncl > 8 * ds.n + 4096 || ncl > (1LL << 30)) return FOMO_OK;
+  lk.eps = 1e-7 * lk.h + 1e-12 * (mag + ext);
+  lk.enabled = 1;
+  DevBuf cnt, pos, off;
+  if (cnt.alloc(sizeof(int32_t) * ncl) || pos.alloc(sizeof(int32_t) * ncl) || off.alloc(sizeof(int64_t) * ncl))
+    return FOMO_OK;  // not enough memory: ring search only
+  {
+    ProfScope ps("k_lookup_build");
+    k_lookup_build<false><<<nblk(ncl), kBlock, 0, s>>>(ds.pts.as<float4>(), ds.cstart.as<int32_t>(), g, lk, ncl,
+                                                        cnt.as<int32_t>(), nullptr, nullptr);
+  }
+  k_clamp_counts<<<nblk(ncl), kBlock, 0, s>>>(cnt.as<int32_t>(), ncl, pos.as<int32_t>());
+  HIPOK(hipGetLastError());
+  size_t tmpb = 0;
+  HIPOK(rocprim::exclusive_scan(nullptr, tmpb, pos.as<int32_t>(), off.as<int64_t>(), (int64_t)0, (size_t)ncl,
+                                rocprim::plus<int64_t>(), s));
+  HIPOK(g_ctx.sorttmp.ensure(tmpb));
+  HIPOK(rocprim::exclusive_scan(g_ctx.sorttmp.p, tmpb, pos.as<int32_t>(), off.as<int64_t>(), (int64_t)0,
+                                (size_t)ncl, rocprim::plus<int64_t>(), s));
+  int64_t last_off = 0;
+  int32_t last_cnt = 0, nfb = 0;
+  HIPOK(hipMemcpyAsync(&last_off, off.as<int64_t>() + ncl - 1, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+  HIPOK(hipMemcpyAsync(&last_cnt, pos.as<int32_t>() + ncl - 1, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+  if (int rc = sync_stream()) return rc;
+  const int64_t total = last_off + last_cnt;
+  if (total >= (int64_t)INT32_MAX) return FOMO_OK;
+  if (ds.lcand.alloc(sizeof(int32_t) * std::max<int64_t>(total, 1)) || ds.lcell.alloc(sizeof(int2) * ncl)) {
+    ds.lcell.release();
+    ds.lcand.release();
+    return FOMO_OK;
+  }
+  k_lookup_build<true><<<nblk(ncl), kBlock, 0, s>>>(ds.pts.as<float4>(), ds.cstart.as<int32_t>(), g, lk, ncl,
+                                                     cnt.as<int32_t>(), off.as<int64_t>(), ds.lcand.as<int32_t>());
+  k_lookup_pack<<<nblk(ncl), kBlock, 0, s>>>(cnt.as<int32_t>(), off.as<int64_t>(), ncl, ds.lcell.as<int2>());
+  HIPOK(hipGetLastError());
+  // number of cells left to the ring search
+  int* dfb = reinterpret_cast<int*>(pos.p);
+  HIPOK(hipMemsetAsync(dfb, 0, sizeof(int), s));
+  k_count_negative<<<nblk(ncl), kBlock, 0, s>>>(cnt.as<int32_t>(), ncl, dfb);
+  HIPOK(hipMemcpyAsync(&nfb, dfb, sizeof(int), hipMemcpyDeviceToHost, s));
+  if (int rc = sync_stream()) return rc;
+  ds.nfallback = nfb;
+  ds.lk = lk;
+  ds.ncand = total;
+  ds.nlcells = ncl;
+  return FOMO_OK;
+}
+
 // Build (or rebuild) the cell-list index of a dataset on the device.
 int build_index(Dataset& ds, double cell_scale) {
   hipStream_t s = g_ctx.stream;
@@ -858,7 +1170,7 @@ int build_index(Dataset& ds, double cell_scale) {
   rc = sync_stream();
   if (rc) return rc;
   ds.maxcount = hmx;
-  return FOMO_OK;
+  return build_lookup(ds);
 }
 
 template <class T> T* find_obj(std::map<int, std::unique_ptr<T>>& m, int id) {
@@ -892,7 +1204,8 @@ void enqueue_nn(hipStream_t s, const Dataset& ds, const ViewGeom* geom, int mode
   const int tiles = ((nx + 15) / 16) * ((ny + 15) / 16);
   dim3 grid(tiles, nz);
   ProfScope ps("k_nn");
-  k_nn<<<grid, 256, 0, s>>>(ds.pts.as<float4>(), ds.cstart.as<int32_t>(), ds.grid, geom, mode, nx, ny, nz, idx);
+  k_nn<<<grid, 256, 0, s>>>(ds.pts.as<float4>(), ds.cstart.as<int32_t>(), ds.grid, ds.lcell.as<int2>(),
+                            ds.lcand.as<int32_t>(), ds.lk, geom, mode, nx, ny, nz, idx);
 }
 
 // Pick the lambda chunk: <=64 pixels per pass, chunk a multiple of 4.
@@ -905,21 +1218,21 @@ int pick_chunk(int nl, int* npass) {
 }
 
 template <int CH>
-void launch_spectral_t(hipStream_t s, const int32_t* idx, int R, int nz, const float2* em, const float* vlos,
-                       const ViewGeom* geom, double wla, double l0c, int nl, int l0, int32_t* flag, float* out) {
-  k_los_spectral<CH, false><<<nblk(R), 256, 0, s>>>(idx, R, nz, em, vlos, geom, wla, l0c, nl, l0, flag, out);
-  k_los_spectral<CH, true><<<nblk(R), 256, 0, s>>>(idx, R, nz, em, vlos, geom, wla, l0c, nl, l0, flag, out);
+void launch_spectral_t(hipStream_t s, const int32_t* idx, int R, int nz, const LineSet& ls, const float* vlos,
+                       const ViewGeom* geom, int nl, int l0, int32_t* flag) {
+  dim3 grid(nblk(R), ls.n);
+  k_los_spectral<CH, false><<<grid, 256, 0, s>>>(idx, R, nz, ls, vlos, geom, nl, l0, flag);
+  k_los_spectral<CH, true><<<grid, 256, 0, s>>>(idx, R, nz, ls, vlos, geom, nl, l0, flag);
 }
 
-void enqueue_los(hipStream_t s, const int32_t* idx, int R, int nz, const float2* em, const float* vlos,
-                 const ViewGeom* geom, int nl, double lambda_width, double lambda0, int32_t* flag, float* out) {
+// All lines of ls from one NN result. flag: R*ls.n ints of scratch.
+void enqueue_los(hipStream_t s, const int32_t* idx, int R, int nz, const LineSet& ls, const float* vlos,
+                 const ViewGeom* geom, int nl, int32_t* flag) {
   if (nl <= 1) {
     ProfScope ps("k_los_imaging");
-    k_los_imaging<<<nblk(R), 256, 0, s>>>(idx, R, nz, em, geom, out);
+    k_los_imaging<<<dim3(nblk(R), ls.n), 256, 0, s>>>(idx, R, nz, ls, geom);
     return;
   }
-  const double wla = lambda_width * lambda0 / FOMO_C_LIGHT;  // fomo-nearestneighbour.cpp:116
-  const double l0c = lambda0 / FOMO_C_LIGHT;
   int npass = 1;
   const int ch = pick_chunk(nl, &npass);
   ProfScope ps("k_los_spectral");
@@ -927,13 +1240,26 @@ void enqueue_los(hipStream_t s, const int32_t* idx, int R, int nz, const float2*
     const int l0 = pass * ch;
     switch (ch) {
 #define FOMO_CH(C) \
-  case C: launch_spectral_t<C>(s, idx, R, nz, em, vlos, geom, wla, l0c, nl, l0, flag, out); break;
+  case C: launch_spectral_t<C>(s, idx, R, nz, ls, vlos, geom, nl, l0, flag); break;
       FOMO_CH(4) FOMO_CH(8) FOMO_CH(12) FOMO_CH(16) FOMO_CH(20) FOMO_CH(24) FOMO_CH(28) FOMO_CH(32)
       FOMO_CH(36) FOMO_CH(40) FOMO_CH(44) FOMO_CH(48) FOMO_CH(52) FOMO_CH(56) FOMO_CH(60) FOMO_CH(64)
 #undef FOMO_CH
       default: break;
     }
   }
+}
+
+LineSet make_lineset(const std::vector<const float2*>& em, const std::vector<float*>& out,
+                     const std::vector<double>& lambda0, double lambda_width) {
+  LineSet ls{};
+  ls.n = (int)em.size();
+  for (int i = 0; i < ls.n; ++i) {
+    ls.em[i] = em[i];
+    ls.out[i] = out[i];
+    ls.wla[i] = lambda_width * lambda0[i] / FOMO_C_LIGHT;  // fomo-nearestneighbour.cpp:116
+    ls.l0c[i] = lambda0[i] / FOMO_C_LIGHT;
+  }
+  return ls;
 }
 
 int check_res(int nx, int ny, int nz, int nl) {
@@ -1060,6 +1386,28 @@ int fomo_dataset_info(int id, int64_t* n, int* dims3, int* maxcount) {
   if (n) *n = ds->n;
   if (dims3) for (int a = 0; a < 3; ++a) dims3[a] = ds->grid.dims[a];
   if (maxcount) *maxcount = ds->maxcount;
+  return FOMO_OK;
+}
+
+int fomo_dataset_set_lookup(int id, int subdiv) {
+  std::lock_guard<std::recursive_mutex> lk(g_mu);
+  if (int rc = require_init()) return rc;
+  Dataset* ds = find_obj(g_ctx.datasets, id);
+  if (!ds) return fail(FOMO_ERR_NOTFOUND, "unknown dataset id");
+  if (subdiv < 0 || subdiv > 4) return fail(FOMO_ERR_ARG, "lookup subdivision must be in [0,4]");
+  ds->lookup_subdiv = subdiv;
+  return build_lookup(*ds);
+}
+
+int fomo_dataset_lookup_info(int id, int* subdiv, int64_t* cells, int64_t* candidates, int64_t* fallback_cells) {
+  std::lock_guard<std::recursive_mutex> lk(g_mu);
+  if (int rc = require_init()) return rc;
+  Dataset* ds = find_obj(g_ctx.datasets, id);
+  if (!ds) return fail(FOMO_ERR_NOTFOUND, "unknown dataset id");
+  if (subdiv) *subdiv = ds->lk.enabled ? ds->lookup_subdiv : 0;
+  if (cells) *cells = ds->nlcells;
+  if (candidates) *candidates = ds->ncand;
+  if (fallback_cells) *fallback_cells = ds->nfallback;
   return FOMO_OK;
 }
 
@@ -1211,8 +1559,8 @@ static int render_one(int goft, int mode, double l, double b, int nx, int ny, in
   enqueue_view_geom(s, *ds, mode, l, b, nx, ny, nz, dn, vs.aux.as<float>(), vs.partial.as<double>(), geom);
   enqueue_nn(s, *ds, geom, mode, nx, ny, nz, vs.idx.as<int32_t>());
   if (mode == MODE_NN) {
-    enqueue_los(s, vs.idx.as<int32_t>(), (int)R, nz, gc->em.as<float2>(), vs.aux.as<float>(), geom, nl, width,
-                gc->lambda0, vs.flag.as<int32_t>(), vs.out.as<float>());
+    LineSet ls = make_lineset({gc->em.as<float2>()}, {vs.out.as<float>()}, {gc->lambda0}, width);
+    enqueue_los(s, vs.idx.as<int32_t>(), (int)R, nz, ls, vs.aux.as<float>(), geom, nl, vs.flag.as<int32_t>());
   } else {
     ProfScope ps("k_los_thomson");
     k_los_thomson<<<nblk(R), 256, 0, s>>>(vs.idx.as<int32_t>(), (int)R, nz, gc->em.as<float2>(), vs.aux.as<float>(),
@@ -1301,10 +1649,17 @@ static int enqueue_batch(Batch& bt, hipStream_t s) {
     ViewGeom* geom = bt.geom.as<ViewGeom>() + v;
     enqueue_view_geom(s, *ds, MODE_NN, bt.l[v], bt.b[v], bt.nx, bt.ny, bt.nz, bt.dn, bt.aux.as<float>(), part, geom);
     enqueue_nn(s, *ds, geom, MODE_NN, bt.nx, bt.ny, bt.nz, bt.idx.as<int32_t>());
-    for (size_t gi = 0; gi < gs.size(); ++gi) {
-      float* out = bt.out.as<float>() + ((size_t)v * gs.size() + gi) * outv;
-      enqueue_los(s, bt.idx.as<int32_t>(), (int)R, bt.nz, gs[gi]->em.as<float2>(), bt.aux.as<float>(), geom, bt.nl,
-                  bt.width, gs[gi]->lambda0, bt.flag.as<int32_t>(), out);
+    for (size_t g0 = 0; g0 < gs.size(); g0 += kMaxLines) {
+      std::vector<const float2*> ems;
+      std::vector<float*> outs;
+      std::vector<double> l0s;
+      for (size_t gi = g0; gi < std::min(gs.size(), g0 + kMaxLines); ++gi) {
+        ems.push_back(gs[gi]->em.as<float2>());
+        outs.push_back(bt.out.as<float>() + ((size_t)v * gs.size() + gi) * outv);
+        l0s.push_back(gs[gi]->lambda0);
+      }
+      LineSet ls = make_lineset(ems, outs, l0s, bt.width);
+      enqueue_los(s, bt.idx.as<int32_t>(), (int)R, bt.nz, ls, bt.aux.as<float>(), geom, bt.nl, bt.flag.as<int32_t>());
     }
   }
   return FOMO_OK;
@@ -1350,7 +1705,7 @@ int fomo_batch_create(int ngoft, const int* gofts, int nviews, const double* l, 
   const int64_t R = (int64_t)nx * ny, S = R * nz;
   const int nlo = std::max(nl, 1);
   if (bt->partial.alloc(sizeof(double) * 6 * kMaxPartials * nviews) || bt->geom.alloc(sizeof(ViewGeom) * nviews) ||
-      bt->idx.alloc(sizeof(int32_t) * S) || bt->aux.alloc(sizeof(float) * ds->n) || bt->flag.alloc(sizeof(int32_t) * R) ||
+      bt->idx.alloc(sizeof(int32_t) * S) || bt->aux.alloc(sizeof(float) * ds->n) || bt->flag.alloc(sizeof(int32_t) * R * kMaxLines) ||
       bt->out.alloc(sizeof(float) * (size_t)R * nlo * ngoft * nviews))
     return fail(FOMO_ERR_NOMEM, "device allocation failed for batch");
   if (int rc = capture_batch(*bt)) return rc;

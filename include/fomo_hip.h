@@ -55,6 +55,12 @@ int fomo_dataset_destroy(int id);
 int fomo_dataset_info(int id, int64_t* n, int* grid_dims3, int* max_cell_count);
 /* Rebuild the spatial index with another cell_scale (used by the autotuner). */
 int fomo_dataset_reindex(int id, double cell_scale);
+/* Candidate lookup grid (the NN fast path): subdiv lookup cells per cell-list edge (default 2);
+ * 0 disables it, leaving the exact ring search over the cell list. Results are identical either
+ * way; only speed and memory change. */
+int fomo_dataset_set_lookup(int id, int subdiv);
+/* Active subdivision (0 = disabled), lookup cells, stored candidates, cells left to the ring search. */
+int fomo_dataset_lookup_info(int id, int* subdiv, int64_t* cells, int64_t* candidates, int64_t* fallback_cells);
 
 /* ---------------------------------------------------------------------------------------
  * G(T,n) tables. The host-side parse of readgoftfromchianti (fomo-CHIANTI.cpp:143-218) is

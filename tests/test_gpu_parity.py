@@ -197,3 +197,28 @@ def test_thomson_small(engine):
         assert np.array_equal(idx, ridx), f"{(idx != ridx).sum()} NN choices differ"
         assert O.rel_l2(img, ref) <= TOL_IMG
     em.close(), ds.close()
+
+
+@pytest.mark.parametrize("jitter", [0.0, 0.3])
+def test_lookup_fast_path_matches_ring_search(engine, jitter):
+    """The candidate lookup grid changes speed only: NN choices equal the ring search and the oracle."""
+    F = engine
+    d = F.gen_loop_cube(36, seed=21, jitter=jitter)
+    ds = _dataset(F, d)
+    info = ds.lookup_info()
+    assert info["subdiv"] == 2 and info["cells"] > 0
+    assert info["fallback_cells"] < 0.05 * info["cells"], info
+    assert info["candidates"] < 8 * info["cells"], info
+    views = [(0.0, np.pi / 2), (0.37, 1.2), (1.9, 0.4), (np.pi / 4, np.pi / 3)]
+    fast = [F.nn_indices(ds, l, b, 48, 44, 50)[0] for (l, b) in views]
+    ds.set_lookup(0)
+    assert ds.lookup_info()["subdiv"] == 0
+    ring = [F.nn_indices(ds, l, b, 48, 44, 50)[0] for (l, b) in views]
+    for (l, b), a, r in zip(views, fast, ring):
+        assert np.array_equal(a, r)
+        md = O.render_nn(1, d["x"], d["y"], d["z"], np.ones(d["x"].size), np.ones(d["x"].size), d["vx"], d["vy"],
+                         d["vz"], l, b, 48, 44, 50, 1, 1e5, 171.0, want_idx=True)[1]
+        assert np.array_equal(a, md)
+    ds.set_lookup(1)
+    assert np.array_equal(F.nn_indices(ds, *views[1], 48, 44, 50)[0], fast[1])
+    ds.close()
