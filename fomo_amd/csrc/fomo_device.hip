@@ -427,7 +427,8 @@ __global__ void k_view_finalize(const double* __restrict__ partial, int np, int 
 // queries use the exact ring search.
 // ------------------------------------------------------------------------------------
 constexpr int kLookupCap = 24;
-constexpr int kLookupRings = 3;
+constexpr int kLookupMargin = 6;  // cell-list cells the lookup grid extends beyond the data bbox
+constexpr int kLookupRings = kLookupMargin + 2;
 
 __device__ __forceinline__ double box_maxdist2(const double lo[3], const double hi[3], const float4 q) {
   const double a0 = fmax(fabs((double)q.x - lo[0]), fabs((double)q.x - hi[0]));
@@ -521,7 +522,7 @@ __global__ void k_lookup_build(const float4* __restrict__ pts, const int32_t* __
     r0[a] = cell_coord(g, a, lo[a] - Up);
     r1[a] = cell_coord(g, a, hi[a] + Up);
   }
-  if ((int64_t)(r1[0] - r0[0] + 1) * (r1[1] - r0[1] + 1) * (r1[2] - r0[2] + 1) > 729) {
+  if ((int64_t)(r1[0] - r0[0] + 1) * (r1[1] - r0[1] + 1) * (r1[2] - r0[2] + 1) > 2197) {
     if (!FILL) cnt[L] = -1;
     return;
   }
@@ -755,13 +756,20 @@ __global__ void __launch_bounds__(256) k_los_spectral(const int32_t* __restrict_
                                                       const LineSet ls, const float* __restrict__ vlos,
                                                       const ViewGeom* __restrict__ vgp, int nl, int l0,
                                                       int32_t* __restrict__ flags) {
-  const int ray = blockIdx.x * blockDim.x + threadIdx.x;
-  if (ray >= R) return;
+  // wave = 32 rays x 2 halves of the LOS; the halves are summed with a cross-lane exchange
+  const int lane = threadIdx.x & 63;
+  const int half = lane >> 5;
+  const int ray = (blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) * 32 + (lane & 31);
+  const bool live = ray < R;
   const int line = blockIdx.y;
   const float2* __restrict__ em = ls.em[line];
   const double wla = ls.wla[line], lam0_over_c = ls.l0c[line];
   int32_t* __restrict__ flag = flags + (size_t)line * R;
-  if (DIRECT && flag[ray] == 0) return;
+  if (DIRECT) {
+    // whole waves leave together (no lane of this wave needs the direct pass)
+    const bool need = live && flag[ray] != 0;
+    if (__ballot(need) == 0) return;
+  }
   const double dl = wla / (nl - 1);
   const double lamc = double(l0) / (nl - 1) * wla - wla / 2.;  // lambdaval of the first pixel of the pass (:200)
   const double k4ln2 = 4. * log(2.);
@@ -769,15 +777,25 @@ __global__ void __launch_bounds__(256) k_los_spectral(const int32_t* __restrict_
 #pragma unroll
   for (int c = 0; c < CH; ++c) acc[c] = 0.0;
   bool bad = false;
-  int pn = idx[ray];
-  for (int k = 0; k < nz; ++k) {
+  const int kh = (nz + 1) >> 1;
+  const int k0 = half ? kh : 0, k1 = half ? nz : kh;
+  const int rr_ = live ? ray : 0;
+  // one-sample lookahead on the gathers
+  int pn = (k0 < k1) ? idx[(size_t)k0 * R + rr_] : -1;
+  float2 en = make_float2(0.f, 0.f);
+  float vn = 0.f;
+  if (pn >= 0) { en = em[pn]; vn = vlos[pn]; }
+  for (int k = k0; k < k1; ++k) {
     const int p = pn;
-    if (k + 1 < nz) pn = idx[(size_t)(k + 1) * R + ray];
-    if (p < 0) continue;
-    const float2 e = em[p];
-    if (e.x == 0.f) continue;
+    const float2 e = en;
+    const float v = vn;
+    if (k + 1 < k1) {
+      pn = idx[(size_t)(k + 1) * R + rr_];
+      if (pn >= 0) { en = em[pn]; vn = vlos[pn]; }
+    }
+    if (!live || p < 0 || e.x == 0.f) continue;
     const double peak = e.x, w = e.y;
-    const double mu = (double)vlos[p] * lam0_over_c;
+    const double mu = (double)v * lam0_over_c;
     const double kap = k4ln2 / (w * w);
     const double d0 = lamc - mu;
     if (DIRECT) {
@@ -790,24 +808,63 @@ __global__ void __launch_bounds__(256) k_los_spectral(const int32_t* __restrict_
       const double a0 = -kap * d0 * d0;
       const double ar = -kap * (2.0 * d0 * dl + dl * dl);
       const double aq = -2.0 * kap * dl * dl;
-      if (a0 > -600.0 && ar < 600.0 && ar > -600.0 && aq > -600.0) {
-        double gg = peak * exp(a0), rr = exp(ar);
-        const double qq = exp(aq);
+      if (a0 > -600.0 && fabs(ar) < 100.0 && aq > -30.0) {
+        // stride-4 recurrence: four independent chains c = j, j+4, ...
+        //   g_{c+4} = g_c R_c,  R_{c+4} = R_c q^16,  R_0 = r_0 r_1 r_2 r_3
+        const double q = exp(aq);
+        double r = exp(ar);
+        double G0 = peak * exp(a0);
+        const double G1 = G0 * r;
+        r *= q;
+        const double G2 = G1 * r;
+        r *= q;
+        const double G3 = G2 * r;
+        const double q2 = q * q, q4 = q2 * q2, q16 = (q4 * q4) * (q4 * q4);
+        double R0 = exp(4.0 * ar + 6.0 * aq);
+        double R1 = R0 * q4, R2 = R1 * q4, R3 = R2 * q4;
+        double G[4] = {G0, G1, G2, G3};
+        double RR[4] = {R0, R1, R2, R3};
 #pragma unroll
-        for (int c = 0; c < CH; ++c) {
-          acc[c] += gg;
-          gg *= rr;
-          rr *= qq;
+        for (int c0 = 0; c0 < CH; c0 += 4) {
+#pragma unroll
+          for (int j = 0; j < 4; ++j) {
+            acc[c0 + j] += G[j];
+            G[j] *= RR[j];
+            RR[j] *= q16;
+          }
         }
       } else {
         bad = true;
       }
     }
   }
-  if (!DIRECT) {
-    flag[ray] = bad ? 1 : 0;
-    if (bad) return;
+  // sum the two halves: the upper half hands its partial sums to the lower half through LDS,
+  // 8 lambda pixels at a time (wave-local exchange; LDS accesses of one wave complete in order)
+  __shared__ double xch[4][8][32];
+  const int wv = threadIdx.x >> 6, l32 = lane & 31;
+#pragma unroll
+  for (int c0 = 0; c0 < CH; c0 += 8) {
+    if (half) {
+#pragma unroll
+      for (int j = 0; j < 8 && c0 + j < CH; ++j) xch[wv][j][l32] = acc[c0 + j];
+    }
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    if (!half) {
+#pragma unroll
+      for (int j = 0; j < 8 && c0 + j < CH; ++j) acc[c0 + j] += xch[wv][j][l32];
+    }
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
   }
+  if (!DIRECT) {
+    const bool anybad = bad || (__shfl_xor(bad ? 1 : 0, 32, 64) != 0);
+    if (live && half == 0) flag[ray] = anybad ? 1 : 0;
+    if (anybad) return;
+  } else {
+    if (!live || flag[ray] == 0) return;
+  }
+  if (!live || half) return;
   const double sc = vgp->scale;
   float* o = ls.out[line] + (size_t)ray * nl + l0;
 #pragma unroll
@@ -993,7 +1050,9 @@ int sync_stream() {
 inline unsigned nblk(int64_t n, int bs = kBlock) { return (unsigned)((n + bs - 1) / bs); }
 
 // Build the candidate lookup grid of a dataset (after its cell list). subdiv lookup cells per
-// cell-list edge; 0 disables the fast path. The grid is shifted by 0.3 cell off the cell lattice.
+// cell-list edge; 0 disables the fast path. The grid extends kLookupMargin cell-list cells beyond
+// the data bbox (so targets just outside the data, within a search box of them, take the fast
+// path too) and is shifted by 0.3 lookup cell off the cell lattice.
 int build_lookup(Dataset& ds) {
   ds.lk = LookupInfo{};
   ds.lcell.release();
@@ -1008,15 +1067,16 @@ int build_lookup(Dataset& ds) {
   double ext = 0, mag = 0;
   int64_t ncl = 1;
   for (int a = 0; a < 3; ++a) {
-    lk.o[a] = g.o[a] - 0.3 * lk.h;
+    const double mrg = (g.dims[a] > 1 ? kLookupMargin : 0) * g.h;
+    lk.o[a] = g.o[a] - mrg - 0.3 * lk.h;
     const double e = (double)g.bmax[a] - g.o[a];
-    lk.dims[a] = (int)std::floor((e + 0.3 * lk.h) / lk.h) + 1;
+    lk.dims[a] = (int)std::floor((e + 2 * mrg + 0.3 * lk.h) / lk.h) + 1;
     ncl *= lk.dims[a];
     ext = std::max(ext, e);
     mag = std::max(mag, std::max(std::fabs((double)g.bmin[a]), std::fabs((double)g.bmax[a])));
   }
-  // keep the lookup grid within ~8 cells per point and 2^30 cells
-  if (ncl > 8 * ds.n + 4096 || ncl > (1LL << 30)) return FOMO_OK;
+  // keep the lookup grid within ~24 cells per point and 2^30 cells
+  if (ncl > 24 * ds.n + 65536 || ncl > (1LL << 30)) return FOMO_OK;
   lk.eps = 1e-7 * lk.h + 1e-12 * (mag + ext);
   lk.enabled = 1;
   DevBuf cnt, pos, off;
@@ -1220,7 +1280,8 @@ int pick_chunk(int nl, int* npass) {
 template <int CH>
 void launch_spectral_t(hipStream_t s, const int32_t* idx, int R, int nz, const LineSet& ls, const float* vlos,
                        const ViewGeom* geom, int nl, int l0, int32_t* flag) {
-  dim3 grid(nblk(R), ls.n);
+  // 128 rays per 256-thread block (each wave: 32 rays x 2 halves of the LOS)
+  dim3 grid((unsigned)((R + 127) / 128), ls.n);
   k_los_spectral<CH, false><<<grid, 256, 0, s>>>(idx, R, nz, ls, vlos, geom, nl, l0, flag);
   k_los_spectral<CH, true><<<grid, 256, 0, s>>>(idx, R, nz, ls, vlos, geom, nl, l0, flag);
 }
