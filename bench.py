@@ -103,6 +103,11 @@ def main():
     ms_step = dt / args.steps * 1e3
     log(f"batched: {ms_step:.2f} ms/step, {value:.3e} W/s")
 
+    # ---- device copy probe (plain float4 copy of 1 GiB: 1 GiB read + 1 GiB written) --------------
+    copy_ms = F.copy_probe(1 << 30, 10)
+    copy_gbps = 2 * (1 << 30) / (copy_ms * 1e-3) / 1e9
+    log(f"copy probe: {copy_ms:.3f} ms per 1 GiB copy, {copy_gbps:.0f} GB/s read+write")
+
     # ---- per-kernel timing: HIP events around each launch of an eager pass of the same work ----
     F.profile_reset()
     bt.profile()
@@ -185,6 +190,7 @@ def main():
         "plain": {"value": plain_value, "ms_per_render": dtp / nplain * 1e3, "renders_timed": nplain,
                   "what": "one render per view and line: emissivity + render + copy to host, index resident"},
         "index_build_ms": t_index * 1e3,
+        "copy_probe_GBps": copy_gbps,
         "lookup": linfo,
         "batched_vs_plain": value / plain_value,
         "roofline": roof,

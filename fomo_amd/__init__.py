@@ -67,6 +67,7 @@ _SIGS = {
     "fomo_profile_enable": (ctypes.c_int, [ctypes.c_int]),
     "fomo_profile_reset": (ctypes.c_int, []),
     "fomo_profile_stats": (ctypes.c_int, [ctypes.c_char_p, _c_i64_p, _c_dbl_p]),
+    "fomo_copy_probe": (ctypes.c_int, [ctypes.c_int64, ctypes.c_int, _c_dbl_p]),
     "fomo_gen_loop_cube": (ctypes.c_int, [ctypes.c_int, ctypes.c_uint64, ctypes.c_double, ctypes.c_double,
                                           ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double,
                                           _c_flt_p, _c_flt_p, _c_flt_p, _c_flt_p, _c_flt_p, _c_flt_p, _c_flt_p,
@@ -135,6 +136,13 @@ def profile_enable(on=True):
 
 def profile_reset():
     _check(lib().fomo_profile_reset())
+
+
+def copy_probe(nbytes, reps=10):
+    """Mean ms of a plain device copy of nbytes (reads nbytes, writes nbytes)."""
+    ms = ctypes.c_double()
+    _check(lib().fomo_copy_probe(int(nbytes), reps, ctypes.byref(ms)))
+    return ms.value
 
 
 def profile_stats(kernel):

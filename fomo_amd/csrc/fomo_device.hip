@@ -704,7 +704,8 @@ __global__ void __launch_bounds__(256) k_nn(const float4* __restrict__ pts, cons
                                             GridInfo g, const int2* __restrict__ lcell,
                                             const int32_t* __restrict__ lcand, LookupInfo lk,
                                             const ViewGeom* __restrict__ vgp, int mode, int nx, int ny, int nz,
-                                            int32_t* __restrict__ idx) {
+                                            int32_t* __restrict__ idx, const float4* __restrict__ vel,
+                                            float* __restrict__ svlos) {
   const int tilesx = (nx + 15) >> 4;
   const int tx = blockIdx.x % tilesx, ty = blockIdx.x / tilesx;
   const int j = tx * 16 + (threadIdx.x & 15);
@@ -728,7 +729,18 @@ __global__ void __launch_bounds__(256) k_nn(const float4* __restrict__ pts, cons
                   (float)(p2 - md), (float)(p0 + md), (float)(p1 + md), (float)(p2 + md));
   }
   if (r == -2) r = nn_query(pts, cstart, g, p0, p1, p2, vg.md);
-  idx[(size_t)k * ((size_t)nx * ny) + (size_t)i * nx + j] = r;
+  const size_t sidx = (size_t)k * ((size_t)nx * ny) + (size_t)i * nx + j;
+  idx[sidx] = r;
+  if (svlos) {
+    // LOS velocity of the chosen point, unit vector (sin b cos l, -sin b sin l, cos b)
+    // (fomo-nearestneighbour.cpp:87-89), stored per sample for the spectral kernel
+    float vl = 0.f;
+    if (r >= 0) {
+      const float4 w = vel[r];
+      vl = (float)(((0.0 + (sb * cl) * (double)w.x) + (-sb * sl) * (double)w.y) + cb * (double)w.z);
+    }
+    svlos[sidx] = vl;
+  }
 }
 
 // sorted positions -> original indices, transposed to [ray][k] (parity output)
@@ -784,14 +796,14 @@ __global__ void __launch_bounds__(256) k_los_spectral(const int32_t* __restrict_
   int pn = (k0 < k1) ? idx[(size_t)k0 * R + rr_] : -1;
   float2 en = make_float2(0.f, 0.f);
   float vn = 0.f;
-  if (pn >= 0) { en = em[pn]; vn = vlos[pn]; }
+  if (pn >= 0) { en = em[pn]; vn = vlos[(size_t)k0 * R + rr_]; }
   for (int k = k0; k < k1; ++k) {
     const int p = pn;
     const float2 e = en;
     const float v = vn;
     if (k + 1 < k1) {
       pn = idx[(size_t)(k + 1) * R + rr_];
-      if (pn >= 0) { en = em[pn]; vn = vlos[pn]; }
+      if (pn >= 0) { en = em[pn]; vn = vlos[(size_t)(k + 1) * R + rr_]; }
     }
     if (!live || p < 0 || e.x == 0.f) continue;
     const double peak = e.x, w = e.y;
@@ -1245,8 +1257,8 @@ void enqueue_view_geom(hipStream_t s, const Dataset& ds, int mode, double l, dou
   const double cb = cos(b), sb = sin(b), cl = cos(l), sl = sin(l);
   if (mode == MODE_NN) {
     ProfScope ps("k_view_rotate_nn");
-    k_view_rotate_nn<<<nb, kBlock, 0, s>>>(ds.pts.as<float4>(), ds.nvars >= 5 ? ds.vel.as<float4>() : nullptr, ds.n, cb,
-                                          sb, cl, sl, ds.nvars >= 5 ? aux : nullptr, partial);
+    (void)aux;
+    k_view_rotate_nn<<<nb, kBlock, 0, s>>>(ds.pts.as<float4>(), nullptr, ds.n, cb, sb, cl, sl, nullptr, partial);
   } else {
     const double eps = 1e-6;  // fomo-thomson.cpp:139-152
     double cbc = cb, sbc = sb;
@@ -1260,12 +1272,15 @@ void enqueue_view_geom(hipStream_t s, const Dataset& ds, int mode, double l, dou
   k_view_finalize<<<1, kBlock, 0, s>>>(partial, nb, mode, nx, ny, nz, dn, cb, sb, cl, sl, bf, geom);
 }
 
-void enqueue_nn(hipStream_t s, const Dataset& ds, const ViewGeom* geom, int mode, int nx, int ny, int nz, int32_t* idx) {
+void enqueue_nn(hipStream_t s, const Dataset& ds, const ViewGeom* geom, int mode, int nx, int ny, int nz, int32_t* idx,
+                float* svlos = nullptr) {
   const int tiles = ((nx + 15) / 16) * ((ny + 15) / 16);
   dim3 grid(tiles, nz);
   ProfScope ps("k_nn");
   k_nn<<<grid, 256, 0, s>>>(ds.pts.as<float4>(), ds.cstart.as<int32_t>(), ds.grid, ds.lcell.as<int2>(),
-                            ds.lcand.as<int32_t>(), ds.lk, geom, mode, nx, ny, nz, idx);
+                            ds.lcand.as<int32_t>(), ds.lk, geom, mode, nx, ny, nz, idx,
+                            (svlos && ds.nvars >= 5) ? ds.vel.as<float4>() : nullptr,
+                            (svlos && ds.nvars >= 5) ? svlos : nullptr);
 }
 
 // Pick the lambda chunk: <=64 pixels per pass, chunk a multiple of 4.
@@ -1613,12 +1628,13 @@ static int render_one(int goft, int mode, double l, double b, int nx, int ny, in
   HIPOK(vs.partial.ensure(sizeof(double) * 6 * (kMaxPartials + 1)));
   HIPOK(vs.geom.ensure(sizeof(ViewGeom)));
   HIPOK(vs.idx.ensure(sizeof(int32_t) * S));
-  HIPOK(vs.aux.ensure(sizeof(float) * ds->n));
+  HIPOK(vs.aux.ensure(sizeof(float) * std::max<int64_t>(ds->n, S)));
   HIPOK(vs.out.ensure(sizeof(float) * R * nlo));
   HIPOK(vs.flag.ensure(sizeof(int32_t) * R));
   ViewGeom* geom = vs.geom.as<ViewGeom>();
   enqueue_view_geom(s, *ds, mode, l, b, nx, ny, nz, dn, vs.aux.as<float>(), vs.partial.as<double>(), geom);
-  enqueue_nn(s, *ds, geom, mode, nx, ny, nz, vs.idx.as<int32_t>());
+  // NN mode: aux receives the per-sample LOS velocity; Thomson: per-point sin^2(chi)
+  enqueue_nn(s, *ds, geom, mode, nx, ny, nz, vs.idx.as<int32_t>(), mode == MODE_NN ? vs.aux.as<float>() : nullptr);
   if (mode == MODE_NN) {
     LineSet ls = make_lineset({gc->em.as<float2>()}, {vs.out.as<float>()}, {gc->lambda0}, width);
     enqueue_los(s, vs.idx.as<int32_t>(), (int)R, nz, ls, vs.aux.as<float>(), geom, nl, vs.flag.as<int32_t>());
@@ -1709,7 +1725,7 @@ static int enqueue_batch(Batch& bt, hipStream_t s) {
     double* part = bt.partial.as<double>() + (size_t)v * 6 * kMaxPartials;
     ViewGeom* geom = bt.geom.as<ViewGeom>() + v;
     enqueue_view_geom(s, *ds, MODE_NN, bt.l[v], bt.b[v], bt.nx, bt.ny, bt.nz, bt.dn, bt.aux.as<float>(), part, geom);
-    enqueue_nn(s, *ds, geom, MODE_NN, bt.nx, bt.ny, bt.nz, bt.idx.as<int32_t>());
+    enqueue_nn(s, *ds, geom, MODE_NN, bt.nx, bt.ny, bt.nz, bt.idx.as<int32_t>(), bt.aux.as<float>());
     for (size_t g0 = 0; g0 < gs.size(); g0 += kMaxLines) {
       std::vector<const float2*> ems;
       std::vector<float*> outs;
@@ -1766,7 +1782,7 @@ int fomo_batch_create(int ngoft, const int* gofts, int nviews, const double* l, 
   const int64_t R = (int64_t)nx * ny, S = R * nz;
   const int nlo = std::max(nl, 1);
   if (bt->partial.alloc(sizeof(double) * 6 * kMaxPartials * nviews) || bt->geom.alloc(sizeof(ViewGeom) * nviews) ||
-      bt->idx.alloc(sizeof(int32_t) * S) || bt->aux.alloc(sizeof(float) * ds->n) || bt->flag.alloc(sizeof(int32_t) * R * kMaxLines) ||
+      bt->idx.alloc(sizeof(int32_t) * S) || bt->aux.alloc(sizeof(float) * S) || bt->flag.alloc(sizeof(int32_t) * R * kMaxLines) ||
       bt->out.alloc(sizeof(float) * (size_t)R * nlo * ngoft * nviews))
     return fail(FOMO_ERR_NOMEM, "device allocation failed for batch");
   if (int rc = capture_batch(*bt)) return rc;
@@ -1841,6 +1857,37 @@ int fomo_batch_destroy(int id) {
   if (int rc = require_init()) return rc;
   (void)hipStreamSynchronize(g_ctx.stream);
   if (!g_ctx.batches.erase(id)) return fail(FOMO_ERR_NOTFOUND, "unknown batch id");
+  return FOMO_OK;
+}
+
+// Plain streaming copy (float4), for calibrating hardware byte counters and as a bandwidth probe.
+__global__ void k_copy_f4(const float4* __restrict__ a, float4* __restrict__ b, int64_t n) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    b[i] = a[i];
+}
+
+int fomo_copy_probe(int64_t bytes, int reps, double* ms_per_copy) {
+  std::lock_guard<std::recursive_mutex> lk(g_mu);
+  if (int rc = require_init()) return rc;
+  if (bytes < 16 || reps < 1) return fail(FOMO_ERR_ARG, "bytes >= 16 and reps >= 1 required");
+  const int64_t n = bytes / 16;
+  DevBuf a, b;
+  if (a.alloc(n * 16) || b.alloc(n * 16)) return fail(FOMO_ERR_NOMEM, "device allocation failed");
+  HIPOK(hipMemsetAsync(a.p, 0, n * 16, g_ctx.stream));
+  hipEvent_t e0, e1;
+  HIPOK(hipEventCreate(&e0));
+  HIPOK(hipEventCreate(&e1));
+  const unsigned nb = (unsigned)std::min<int64_t>((n + 255) / 256, 8192);
+  k_copy_f4<<<nb, 256, 0, g_ctx.stream>>>(a.as<float4>(), b.as<float4>(), n);  // warm
+  HIPOK(hipEventRecord(e0, g_ctx.stream));
+  for (int r = 0; r < reps; ++r) k_copy_f4<<<nb, 256, 0, g_ctx.stream>>>(a.as<float4>(), b.as<float4>(), n);
+  HIPOK(hipEventRecord(e1, g_ctx.stream));
+  HIPOK(hipEventSynchronize(e1));
+  float ms = 0;
+  HIPOK(hipEventElapsedTime(&ms, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (ms_per_copy) *ms_per_copy = ms / reps;
   return FOMO_OK;
 }
 

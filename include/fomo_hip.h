@@ -133,6 +133,10 @@ int fomo_batch_destroy(int batch);
 int fomo_profile_enable(int on);
 int fomo_profile_reset(void);
 int fomo_profile_stats(const char* kernel, int64_t* count, double* total_ms);
+/* Device bandwidth probe: copies `bytes` between two device buffers `reps` times with a plain
+ * float4 streaming kernel (k_copy_f4); returns the mean milliseconds per copy. Used to calibrate
+ * hardware byte counters (a copy of B bytes moves B read + B written). */
+int fomo_copy_probe(int64_t bytes, int reps, double* ms_per_copy);
 
 /* ---------------------------------------------------------------------------------------
  * Synthetic inputs (host-only, deterministic from the seed; no GPU needed).
