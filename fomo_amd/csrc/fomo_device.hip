@@ -130,7 +130,7 @@ enum { MODE_NN = 0, MODE_THOMSON = 1 };
 // Spectral lines rendered together from one NN result (blockIdx.y selects the line).
 constexpr int kMaxLines = 8;
 struct LineSet {
-  const float2* em[kMaxLines];  // (peak, fwhm) per sorted point
+  const float2* em[kMaxLines];  // (peak, fwhm): per sorted point (record build) or per sample (K5)
   float* out[kMaxLines];        // R*nl output per line
   double wla[kMaxLines];        // lambda window width in Angstrom
   double l0c[kMaxLines];        // lambda0 / c
@@ -700,12 +700,17 @@ __device__ int nn_query(const float4* __restrict__ pts, const int32_t* __restric
 }
 
 // One thread per LOS sample. Block = 16x16 rays at one k (blockIdx.y = k).
+// RS4 > 0: per-point records of RS4 float4 {vx,vy,vz, peak_0,fwhm_0, ..., peak_{L-1},fwhm_{L-1}}
+// are gathered once for the chosen point, and the kernel writes coalesced per-sample arrays:
+// svlos[s] (LOS velocity, fomo-nearestneighbour.cpp:87-89) and sem[l*S + s] = (peak_l, fwhm_l),
+// zero when no point is inside the search box (:190-193). idx (optional) gets the sorted position.
+template <int RS4>
 __global__ void __launch_bounds__(256) k_nn(const float4* __restrict__ pts, const int32_t* __restrict__ cstart,
                                             GridInfo g, const int2* __restrict__ lcell,
                                             const int32_t* __restrict__ lcand, LookupInfo lk,
-                                            const ViewGeom* __restrict__ vgp, int mode, int nx, int ny, int nz,
-                                            int32_t* __restrict__ idx, const float4* __restrict__ vel,
-                                            float* __restrict__ svlos) {
+                                            const ViewGeom* __restrict__ vgp, int nx, int ny, int nz,
+                                            int32_t* __restrict__ idx, const float4* __restrict__ rec, int nlines,
+                                            float* __restrict__ svlos, float2* __restrict__ sem) {
   const int tilesx = (nx + 15) >> 4;
   const int tx = blockIdx.x % tilesx, ty = blockIdx.x / tilesx;
   const int j = tx * 16 + (threadIdx.x & 15);
@@ -721,7 +726,6 @@ __global__ void __launch_bounds__(256) k_nn(const float4* __restrict__ pts, cons
   const double p0 = xx * cb * cl + yy * sl + zz * sb * cl;        // :165
   const double p1 = -xx * cb * sl + yy * cl - zz * sb * sl;
   const double p2 = -xx * sb + zz * cb;
-  (void)mode;
   int r = -2;
   if (lk.enabled) {
     const double md = vg.md;
@@ -729,18 +733,58 @@ __global__ void __launch_bounds__(256) k_nn(const float4* __restrict__ pts, cons
                   (float)(p2 - md), (float)(p0 + md), (float)(p1 + md), (float)(p2 + md));
   }
   if (r == -2) r = nn_query(pts, cstart, g, p0, p1, p2, vg.md);
+  const size_t S = (size_t)nx * ny * nz;
   const size_t sidx = (size_t)k * ((size_t)nx * ny) + (size_t)i * nx + j;
-  idx[sidx] = r;
-  if (svlos) {
-    // LOS velocity of the chosen point, unit vector (sin b cos l, -sin b sin l, cos b)
-    // (fomo-nearestneighbour.cpp:87-89), stored per sample for the spectral kernel
-    float vl = 0.f;
+  if (idx) idx[sidx] = r;
+  if (RS4 > 0) {
+    float f[4 * (RS4 > 0 ? RS4 : 1)];
+#pragma unroll
+    for (int q = 0; q < 4 * (RS4 > 0 ? RS4 : 1); ++q) f[q] = 0.f;
     if (r >= 0) {
-      const float4 w = vel[r];
-      vl = (float)(((0.0 + (sb * cl) * (double)w.x) + (-sb * sl) * (double)w.y) + cb * (double)w.z);
+      const float4* rp = rec + (size_t)r * RS4;
+#pragma unroll
+      for (int q = 0; q < RS4; ++q) {
+        const float4 a = rp[q];
+        f[4 * q] = a.x;
+        f[4 * q + 1] = a.y;
+        f[4 * q + 2] = a.z;
+        f[4 * q + 3] = a.w;
+      }
     }
-    svlos[sidx] = vl;
+    // unit vector (sin b cos l, -sin b sin l, cos b)
+    svlos[sidx] = r >= 0 ? (float)(((0.0 + (sb * cl) * (double)f[0]) + (-sb * sl) * (double)f[1]) + cb * (double)f[2])
+                         : 0.f;
+    // a record of RS4 float4 holds (4*RS4-3)/2 lines after the velocity
+#pragma unroll
+    for (int l = 0; l < (4 * RS4 - 3) / 2; ++l)
+      if (l < nlines) sem[(size_t)l * S + sidx] = make_float2(f[3 + 2 * l], f[4 + 2 * l]);
   }
+}
+
+// Per-point records for k_nn (sorted order): {vx,vy,vz, peak_0,fwhm_0, ...}, rs4 float4 each.
+__global__ void k_build_rec(const float4* __restrict__ vel, const LineSet ls, int64_t n, int rs4,
+                            float4* __restrict__ rec) {
+  const int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  float f[32];
+#pragma unroll
+  for (int q = 0; q < 32; ++q) f[q] = 0.f;
+  if (vel) {
+    const float4 v = vel[p];
+    f[0] = v.x;
+    f[1] = v.y;
+    f[2] = v.z;
+  }
+#pragma unroll
+  for (int l = 0; l < kMaxLines; ++l)
+    if (l < ls.n) {
+      const float2 e = ls.em[l][p];
+      f[3 + 2 * l] = e.x;
+      f[4 + 2 * l] = e.y;
+    }
+#pragma unroll
+  for (int q = 0; q < 8; ++q)
+    if (q < rs4) rec[(size_t)p * rs4 + q] = make_float4(f[4 * q], f[4 * q + 1], f[4 * q + 2], f[4 * q + 3]);
 }
 
 // sorted positions -> original indices, transposed to [ray][k] (parity output)
@@ -764,8 +808,7 @@ __global__ void k_idx_to_orig(const int32_t* __restrict__ idx, const int32_t* __
 // left to the DIRECT instantiation (one exp per sample x lambda), launched right after.
 // ------------------------------------------------------------------------------------
 template <int CH, bool DIRECT>
-__global__ void __launch_bounds__(256) k_los_spectral(const int32_t* __restrict__ idx, int R, int nz,
-                                                      const LineSet ls, const float* __restrict__ vlos,
+__global__ void __launch_bounds__(256) k_los_spectral(int R, int nz, const LineSet ls, const float* __restrict__ vlos,
                                                       const ViewGeom* __restrict__ vgp, int nl, int l0,
                                                       int32_t* __restrict__ flags) {
   // wave = 32 rays x 2 halves of the LOS; the halves are summed with a cross-lane exchange
@@ -792,20 +835,18 @@ __global__ void __launch_bounds__(256) k_los_spectral(const int32_t* __restrict_
   const int kh = (nz + 1) >> 1;
   const int k0 = half ? kh : 0, k1 = half ? nz : kh;
   const int rr_ = live ? ray : 0;
-  // one-sample lookahead on the gathers
-  int pn = (k0 < k1) ? idx[(size_t)k0 * R + rr_] : -1;
+  // per-sample (peak, fwhm) and LOS velocity written by k_nn; one-sample lookahead
   float2 en = make_float2(0.f, 0.f);
   float vn = 0.f;
-  if (pn >= 0) { en = em[pn]; vn = vlos[(size_t)k0 * R + rr_]; }
+  if (k0 < k1) { en = em[(size_t)k0 * R + rr_]; vn = vlos[(size_t)k0 * R + rr_]; }
   for (int k = k0; k < k1; ++k) {
-    const int p = pn;
     const float2 e = en;
     const float v = vn;
     if (k + 1 < k1) {
-      pn = idx[(size_t)(k + 1) * R + rr_];
-      if (pn >= 0) { en = em[pn]; vn = vlos[(size_t)(k + 1) * R + rr_]; }
+      en = em[(size_t)(k + 1) * R + rr_];
+      vn = vlos[(size_t)(k + 1) * R + rr_];
     }
-    if (!live || p < 0 || e.x == 0.f) continue;
+    if (!live || e.x == 0.f) continue;
     const double peak = e.x, w = e.y;
     const double mu = (double)v * lam0_over_c;
     const double kap = k4ln2 / (w * w);
@@ -885,17 +926,14 @@ __global__ void __launch_bounds__(256) k_los_spectral(const int32_t* __restrict_
 }
 
 // nl == 1 (Imaging): intens += peak (fomo-nearestneighbour.cpp:217-225)
-__global__ void __launch_bounds__(256) k_los_imaging(const int32_t* __restrict__ idx, int R, int nz,
-                                                     const LineSet ls, const ViewGeom* __restrict__ vgp) {
+__global__ void __launch_bounds__(256) k_los_imaging(int R, int nz, const LineSet ls,
+                                                     const ViewGeom* __restrict__ vgp) {
   const int ray = blockIdx.x * blockDim.x + threadIdx.x;
   if (ray >= R) return;
-  const float2* __restrict__ em = ls.em[blockIdx.y];
+  const float2* __restrict__ em = ls.em[blockIdx.y];  // per-sample (peak, fwhm), zero when no point
   float* __restrict__ out = ls.out[blockIdx.y];
   double acc = 0.0;
-  for (int k = 0; k < nz; ++k) {
-    const int p = idx[(size_t)k * R + ray];
-    if (p >= 0) acc += (double)em[p].x;
-  }
+  for (int k = 0; k < nz; ++k) acc += (double)em[(size_t)k * R + ray].x;
   out[ray] = (float)(vgp->scale * acc);
 }
 
@@ -952,6 +990,8 @@ struct Goft {
   int kind = 0;  // 0 chianti, 1 thomson
   double lambda0 = 0;
   DevBuf em;     // float2 per sorted point
+  DevBuf rec;    // chianti: per-point records {vx,vy,vz,peak,fwhm} for k_nn
+  int rs4 = 0;
 };
 
 struct ViewScratch {
@@ -961,6 +1001,7 @@ struct ViewScratch {
   DevBuf aux;      // float N (vlos or sin2chi)
   DevBuf out;      // float R*nl
   DevBuf flag;     // int32 R (rays needing the direct-exp pass)
+  DevBuf sem;      // float2 S: per-sample (peak, fwhm)
 };
 
 struct Batch {
@@ -969,7 +1010,10 @@ struct Batch {
   std::vector<double> l, b;
   int nx = 0, ny = 0, nz = 0, nl = 0, dn = 0;
   double width = 0;
-  DevBuf partial, geom, idx, aux, out, flag;
+  DevBuf partial, geom, aux, out, flag;
+  DevBuf rec;  // per-point records {vel, all lines}
+  DevBuf sem;  // per-sample (peak, fwhm), nlines x S
+  int rs4 = 0;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   ~Batch() {
@@ -1272,15 +1316,40 @@ void enqueue_view_geom(hipStream_t s, const Dataset& ds, int mode, double l, dou
   k_view_finalize<<<1, kBlock, 0, s>>>(partial, nb, mode, nx, ny, nz, dn, cb, sb, cl, sl, bf, geom);
 }
 
-void enqueue_nn(hipStream_t s, const Dataset& ds, const ViewGeom* geom, int mode, int nx, int ny, int nz, int32_t* idx,
-                float* svlos = nullptr) {
+// Per-point records shared by the lines of a render: RS4 float4 per point.
+int rec_width(int nlines) {
+  const int f4 = (3 + 2 * nlines + 3) / 4;
+  return f4 <= 2 ? 2 : (f4 <= 4 ? 4 : 8);
+}
+
+// NN query for every sample of a view. With rec: per-sample vlos and emissivities (nlines) too.
+void enqueue_nn(hipStream_t s, const Dataset& ds, const ViewGeom* geom, int nx, int ny, int nz, int32_t* idx,
+                const float4* rec = nullptr, int rs4 = 0, int nlines = 0, float* svlos = nullptr,
+                float2* sem = nullptr) {
   const int tiles = ((nx + 15) / 16) * ((ny + 15) / 16);
   dim3 grid(tiles, nz);
   ProfScope ps("k_nn");
-  k_nn<<<grid, 256, 0, s>>>(ds.pts.as<float4>(), ds.cstart.as<int32_t>(), ds.grid, ds.lcell.as<int2>(),
-                            ds.lcand.as<int32_t>(), ds.lk, geom, mode, nx, ny, nz, idx,
-                            (svlos && ds.nvars >= 5) ? ds.vel.as<float4>() : nullptr,
-                            (svlos && ds.nvars >= 5) ? svlos : nullptr);
+#define FOMO_NN_ARGS                                                                                           \
+  ds.pts.as<float4>(), ds.cstart.as<int32_t>(), ds.grid, ds.lcell.as<int2>(), ds.lcand.as<int32_t>(), ds.lk, \
+      geom, nx, ny, nz, idx, rec, nlines, svlos, sem
+  if (!rec) k_nn<0><<<grid, 256, 0, s>>>(FOMO_NN_ARGS);
+  else if (rs4 == 2) k_nn<2><<<grid, 256, 0, s>>>(FOMO_NN_ARGS);
+  else if (rs4 == 4) k_nn<4><<<grid, 256, 0, s>>>(FOMO_NN_ARGS);
+  else k_nn<8><<<grid, 256, 0, s>>>(FOMO_NN_ARGS);
+#undef FOMO_NN_ARGS
+}
+
+// Build records {vel, lines} for a set of emission cubes of one dataset.
+int build_records(hipStream_t s, const Dataset& ds, const std::vector<const float2*>& ems, DevBuf& rec, int* rs4) {
+  *rs4 = rec_width((int)ems.size());
+  if (rec.ensure(sizeof(float4) * (*rs4) * ds.n)) return fail(FOMO_ERR_NOMEM, "device allocation failed (records)");
+  LineSet ls{};
+  ls.n = (int)ems.size();
+  for (int i = 0; i < ls.n; ++i) ls.em[i] = ems[i];
+  k_build_rec<<<nblk(ds.n), kBlock, 0, s>>>(ds.nvars >= 5 ? ds.vel.as<float4>() : nullptr, ls, ds.n, *rs4,
+                                            rec.as<float4>());
+  HIPOK(hipGetLastError());
+  return FOMO_OK;
 }
 
 // Pick the lambda chunk: <=64 pixels per pass, chunk a multiple of 4.
@@ -1293,20 +1362,20 @@ int pick_chunk(int nl, int* npass) {
 }
 
 template <int CH>
-void launch_spectral_t(hipStream_t s, const int32_t* idx, int R, int nz, const LineSet& ls, const float* vlos,
-                       const ViewGeom* geom, int nl, int l0, int32_t* flag) {
+void launch_spectral_t(hipStream_t s, int R, int nz, const LineSet& ls, const float* vlos, const ViewGeom* geom,
+                       int nl, int l0, int32_t* flag) {
   // 128 rays per 256-thread block (each wave: 32 rays x 2 halves of the LOS)
   dim3 grid((unsigned)((R + 127) / 128), ls.n);
-  k_los_spectral<CH, false><<<grid, 256, 0, s>>>(idx, R, nz, ls, vlos, geom, nl, l0, flag);
-  k_los_spectral<CH, true><<<grid, 256, 0, s>>>(idx, R, nz, ls, vlos, geom, nl, l0, flag);
+  k_los_spectral<CH, false><<<grid, 256, 0, s>>>(R, nz, ls, vlos, geom, nl, l0, flag);
+  k_los_spectral<CH, true><<<grid, 256, 0, s>>>(R, nz, ls, vlos, geom, nl, l0, flag);
 }
 
-// All lines of ls from one NN result. flag: R*ls.n ints of scratch.
-void enqueue_los(hipStream_t s, const int32_t* idx, int R, int nz, const LineSet& ls, const float* vlos,
-                 const ViewGeom* geom, int nl, int32_t* flag) {
+// All lines of ls (per-sample emissivities written by k_nn). flag: R*ls.n ints of scratch.
+void enqueue_los(hipStream_t s, int R, int nz, const LineSet& ls, const float* vlos, const ViewGeom* geom, int nl,
+                 int32_t* flag) {
   if (nl <= 1) {
     ProfScope ps("k_los_imaging");
-    k_los_imaging<<<dim3(nblk(R), ls.n), 256, 0, s>>>(idx, R, nz, ls, geom);
+    k_los_imaging<<<dim3(nblk(R), ls.n), 256, 0, s>>>(R, nz, ls, geom);
     return;
   }
   int npass = 1;
@@ -1316,7 +1385,7 @@ void enqueue_los(hipStream_t s, const int32_t* idx, int R, int nz, const LineSet
     const int l0 = pass * ch;
     switch (ch) {
 #define FOMO_CH(C) \
-  case C: launch_spectral_t<C>(s, idx, R, nz, ls, vlos, geom, nl, l0, flag); break;
+  case C: launch_spectral_t<C>(s, R, nz, ls, vlos, geom, nl, l0, flag); break;
       FOMO_CH(4) FOMO_CH(8) FOMO_CH(12) FOMO_CH(16) FOMO_CH(20) FOMO_CH(24) FOMO_CH(28) FOMO_CH(32)
       FOMO_CH(36) FOMO_CH(40) FOMO_CH(44) FOMO_CH(48) FOMO_CH(52) FOMO_CH(56) FOMO_CH(60) FOMO_CH(64)
 #undef FOMO_CH
@@ -1388,6 +1457,7 @@ int fomo_shutdown(void) {
   g_ctx.vs.aux.release();
   g_ctx.vs.out.release();
   g_ctx.vs.flag.release();
+  g_ctx.vs.sem.release();
   for (auto& b : g_ctx.sortkeys) b.release();
   g_ctx.sortvals.release();
   g_ctx.sorttmp.release();
@@ -1551,6 +1621,7 @@ int fomo_emission_chianti(int dataset, int table, double lambda0, double atweigh
                                                          tb->nt, tb->nrho, K, normc, imaging, gc->em.as<float2>());
   }
   HIPOK(hipGetLastError());
+  if (int rc = build_records(g_ctx.stream, *ds, {gc->em.as<float2>()}, gc->rec, &gc->rs4)) return rc;
   if (int rc = sync_stream()) return rc;
   int id = g_ctx.next_id++;
   g_ctx.gofts[id] = std::move(gc);
@@ -1631,14 +1702,17 @@ static int render_one(int goft, int mode, double l, double b, int nx, int ny, in
   HIPOK(vs.aux.ensure(sizeof(float) * std::max<int64_t>(ds->n, S)));
   HIPOK(vs.out.ensure(sizeof(float) * R * nlo));
   HIPOK(vs.flag.ensure(sizeof(int32_t) * R));
+  if (mode == MODE_NN) HIPOK(vs.sem.ensure(sizeof(float2) * S));
   ViewGeom* geom = vs.geom.as<ViewGeom>();
   enqueue_view_geom(s, *ds, mode, l, b, nx, ny, nz, dn, vs.aux.as<float>(), vs.partial.as<double>(), geom);
-  // NN mode: aux receives the per-sample LOS velocity; Thomson: per-point sin^2(chi)
-  enqueue_nn(s, *ds, geom, mode, nx, ny, nz, vs.idx.as<int32_t>(), mode == MODE_NN ? vs.aux.as<float>() : nullptr);
   if (mode == MODE_NN) {
-    LineSet ls = make_lineset({gc->em.as<float2>()}, {vs.out.as<float>()}, {gc->lambda0}, width);
-    enqueue_los(s, vs.idx.as<int32_t>(), (int)R, nz, ls, vs.aux.as<float>(), geom, nl, vs.flag.as<int32_t>());
+    // aux: per-sample LOS velocity; sem: per-sample (peak, fwhm)
+    enqueue_nn(s, *ds, geom, nx, ny, nz, nullptr, gc->rec.as<float4>(), gc->rs4, 1, vs.aux.as<float>(),
+               vs.sem.as<float2>());
+    LineSet ls = make_lineset({vs.sem.as<float2>()}, {vs.out.as<float>()}, {gc->lambda0}, width);
+    enqueue_los(s, (int)R, nz, ls, vs.aux.as<float>(), geom, nl, vs.flag.as<int32_t>());
   } else {
+    enqueue_nn(s, *ds, geom, nx, ny, nz, vs.idx.as<int32_t>());
     ProfScope ps("k_los_thomson");
     k_los_thomson<<<nblk(R), 256, 0, s>>>(vs.idx.as<int32_t>(), (int)R, nz, gc->em.as<float2>(), vs.aux.as<float>(),
                                           geom, vs.out.as<float>());
@@ -1687,7 +1761,7 @@ int fomo_nn_indices(int dataset, double l, double b, int nx, int ny, int nz, int
   HIPOK(vs.aux.ensure(sizeof(float) * std::max<int64_t>(ds->n, S)));
   ViewGeom* geom = vs.geom.as<ViewGeom>();
   enqueue_view_geom(s, *ds, mode, l, b, nx, ny, nz, 0, vs.aux.as<float>(), vs.partial.as<double>(), geom);
-  enqueue_nn(s, *ds, geom, mode, nx, ny, nz, vs.idx.as<int32_t>());
+  enqueue_nn(s, *ds, geom, nx, ny, nz, vs.idx.as<int32_t>());
   int32_t* tmp = reinterpret_cast<int32_t*>(vs.aux.p);
   k_idx_to_orig<<<nblk(S), kBlock, 0, s>>>(vs.idx.as<int32_t>(), ds->perm.as<int32_t>(), R, nz, tmp);
   HIPOK(hipGetLastError());
@@ -1725,19 +1799,19 @@ static int enqueue_batch(Batch& bt, hipStream_t s) {
     double* part = bt.partial.as<double>() + (size_t)v * 6 * kMaxPartials;
     ViewGeom* geom = bt.geom.as<ViewGeom>() + v;
     enqueue_view_geom(s, *ds, MODE_NN, bt.l[v], bt.b[v], bt.nx, bt.ny, bt.nz, bt.dn, bt.aux.as<float>(), part, geom);
-    enqueue_nn(s, *ds, geom, MODE_NN, bt.nx, bt.ny, bt.nz, bt.idx.as<int32_t>(), bt.aux.as<float>());
-    for (size_t g0 = 0; g0 < gs.size(); g0 += kMaxLines) {
-      std::vector<const float2*> ems;
-      std::vector<float*> outs;
-      std::vector<double> l0s;
-      for (size_t gi = g0; gi < std::min(gs.size(), g0 + kMaxLines); ++gi) {
-        ems.push_back(gs[gi]->em.as<float2>());
-        outs.push_back(bt.out.as<float>() + ((size_t)v * gs.size() + gi) * outv);
-        l0s.push_back(gs[gi]->lambda0);
-      }
-      LineSet ls = make_lineset(ems, outs, l0s, bt.width);
-      enqueue_los(s, bt.idx.as<int32_t>(), (int)R, bt.nz, ls, bt.aux.as<float>(), geom, bt.nl, bt.flag.as<int32_t>());
+    const size_t S = (size_t)R * bt.nz;
+    enqueue_nn(s, *ds, geom, bt.nx, bt.ny, bt.nz, nullptr, bt.rec.as<float4>(), bt.rs4, (int)gs.size(),
+               bt.aux.as<float>(), bt.sem.as<float2>());
+    std::vector<const float2*> ems;
+    std::vector<float*> outs;
+    std::vector<double> l0s;
+    for (size_t gi = 0; gi < gs.size(); ++gi) {
+      ems.push_back(bt.sem.as<float2>() + gi * S);
+      outs.push_back(bt.out.as<float>() + ((size_t)v * gs.size() + gi) * outv);
+      l0s.push_back(gs[gi]->lambda0);
     }
+    LineSet ls = make_lineset(ems, outs, l0s, bt.width);
+    enqueue_los(s, (int)R, bt.nz, ls, bt.aux.as<float>(), geom, bt.nl, bt.flag.as<int32_t>());
   }
   return FOMO_OK;
 }
@@ -1763,6 +1837,7 @@ int fomo_batch_create(int ngoft, const int* gofts, int nviews, const double* l, 
   std::lock_guard<std::recursive_mutex> lk(g_mu);
   if (int rc = require_init()) return rc;
   if (ngoft < 1 || !gofts || nviews < 1 || !l || !b || !out_batch) return fail(FOMO_ERR_ARG, "bad batch arguments");
+  if (ngoft > kMaxLines) return fail(FOMO_ERR_ARG, "a batch takes at most 8 spectral lines");
   if (int rc = check_res(nx, ny, nz, nl)) return rc;
   std::unique_ptr<Batch> bt(new Batch());
   for (int i = 0; i < ngoft; ++i) {
@@ -1782,9 +1857,15 @@ int fomo_batch_create(int ngoft, const int* gofts, int nviews, const double* l, 
   const int64_t R = (int64_t)nx * ny, S = R * nz;
   const int nlo = std::max(nl, 1);
   if (bt->partial.alloc(sizeof(double) * 6 * kMaxPartials * nviews) || bt->geom.alloc(sizeof(ViewGeom) * nviews) ||
-      bt->idx.alloc(sizeof(int32_t) * S) || bt->aux.alloc(sizeof(float) * S) || bt->flag.alloc(sizeof(int32_t) * R * kMaxLines) ||
+      bt->sem.alloc(sizeof(float2) * S * ngoft) || bt->aux.alloc(sizeof(float) * S) || bt->flag.alloc(sizeof(int32_t) * R * kMaxLines) ||
       bt->out.alloc(sizeof(float) * (size_t)R * nlo * ngoft * nviews))
     return fail(FOMO_ERR_NOMEM, "device allocation failed for batch");
+  {
+    std::vector<const float2*> ems;
+    for (int gid : bt->gofts) ems.push_back(find_obj(g_ctx.gofts, gid)->em.as<float2>());
+    if (int rc = build_records(g_ctx.stream, *ds, ems, bt->rec, &bt->rs4)) return rc;
+    if (int rc = sync_stream()) return rc;
+  }
   if (int rc = capture_batch(*bt)) return rc;
   int id = g_ctx.next_id++;
   g_ctx.batches[id] = std::move(bt);
