@@ -455,7 +455,7 @@ __device__ __forceinline__ bool dominated(const double c[3], const double hw[3],
 template <bool FILL>
 __global__ void k_lookup_build(const float4* __restrict__ pts, const int32_t* __restrict__ cstart, GridInfo g,
                                LookupInfo lk, int64_t ncl, int32_t* __restrict__ cnt, const int64_t* __restrict__ off,
-                               int32_t* __restrict__ cand) {
+                               float4* __restrict__ cand) {
   const int64_t L = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (L >= ncl) return;
   if (FILL && cnt[L] <= 0) return;
@@ -538,7 +538,7 @@ __global__ void k_lookup_build(const float4* __restrict__ pts, const int32_t* __
         for (int k = 0; k < nd && !dm; ++k)
           if (dpos[k] != p && dominated(c, hw, q, dom[k])) dm = true;
         if (dm) continue;
-        if (FILL) cand[base + n] = p;
+        if (FILL) cand[base + n] = make_float4(q.x, q.y, q.z, __int_as_float(p));
         ++n;
         if (!FILL && n > kLookupCap) {
           cnt[L] = -1;
@@ -579,7 +579,7 @@ __global__ void k_clamp_counts(const int32_t* __restrict__ cnt, int64_t ncl, int
 // Fast path: the unconstrained nearest point from the target's lookup cell. Returns its sorted
 // position if it lies strictly inside the search box, else -2 (caller runs the ring search).
 __device__ __forceinline__ int nn_lookup(const float4* __restrict__ pts, const int2* __restrict__ lcell,
-                                         const int32_t* __restrict__ lcand, const LookupInfo& lk, float t0, float t1,
+                                         const float4* __restrict__ lcand, const LookupInfo& lk, float t0, float t1,
                                          float t2, float lo0, float lo1, float lo2, float hi0, float hi1, float hi2) {
   const double v0 = ((double)t0 - lk.o[0]) * lk.invh;
   const double v1 = ((double)t1 - lk.o[1]) * lk.invh;
@@ -590,20 +590,22 @@ __device__ __forceinline__ int nn_lookup(const float4* __restrict__ pts, const i
   const int64_t L = ((int64_t)(int)v2 * lk.dims[1] + (int)v1) * lk.dims[0] + (int)v0;
   const int2 e = lcell[L];
   if (e.y <= 0) return -2;
+  // candidates hold their coordinates inline (x, y, z, sorted position); the original index
+  // (tie-break) is read from pts only on an exact distance tie
   double best = INFINITY;
-  int bidx = 0x7fffffff, bpos = -1;
+  int bpos = -1;
   float4 bq = make_float4(0.f, 0.f, 0.f, 0.f);
   for (int i = 0; i < e.y; ++i) {
-    const int p = lcand[e.x + i];
-    const float4 q = pts[p];
+    const float4 q = lcand[e.x + i];
     const double d0 = (double)q.x - (double)t0;
     const double d1 = (double)q.y - (double)t1;
     const double d2 = (double)q.z - (double)t2;
     const double d = (d0 * d0 + d1 * d1) + d2 * d2;
-    const int oi = __float_as_int(q.w);
-    if (d < best || (d == best && oi < bidx)) {
+    const int p = __float_as_int(q.w);
+    bool take = d < best;
+    if (d == best && bpos >= 0) take = __float_as_int(pts[p].w) < __float_as_int(pts[bpos].w);
+    if (take) {
       best = d;
-      bidx = oi;
       bpos = p;
       bq = q;
     }
@@ -707,7 +709,7 @@ __device__ int nn_query(const float4* __restrict__ pts, const int32_t* __restric
 template <int RS4>
 __global__ void __launch_bounds__(256) k_nn(const float4* __restrict__ pts, const int32_t* __restrict__ cstart,
                                             GridInfo g, const int2* __restrict__ lcell,
-                                            const int32_t* __restrict__ lcand, LookupInfo lk,
+                                            const float4* __restrict__ lcand, LookupInfo lk,
                                             const ViewGeom* __restrict__ vgp, int nx, int ny, int nz,
                                             int32_t* __restrict__ idx, const float4* __restrict__ rec, int nlines,
                                             float* __restrict__ svlos, float2* __restrict__ sem) {
@@ -976,7 +978,7 @@ struct Dataset {
   LookupInfo lk{};     // candidate lookup grid (lk.enabled==0: ring search only)
   int lookup_subdiv = 2;
   DevBuf lcell;        // int2 (offset, count) per lookup cell; count -1 = use the ring search
-  DevBuf lcand;        // int32 candidate sorted positions
+  DevBuf lcand;        // float4 candidates (x, y, z, sorted position as int bits)
   int64_t ncand = 0, nlcells = 0, nfallback = 0;
 };
 
@@ -1158,13 +1160,13 @@ int build_lookup(Dataset& ds) {
   if (int rc = sync_stream()) return rc;
   const int64_t total = last_off + last_cnt;
   if (total >= (int64_t)INT32_MAX) return FOMO_OK;
-  if (ds.lcand.alloc(sizeof(int32_t) * std::max<int64_t>(total, 1)) || ds.lcell.alloc(sizeof(int2) * ncl)) {
+  if (ds.lcand.alloc(sizeof(float4) * std::max<int64_t>(total, 1)) || ds.lcell.alloc(sizeof(int2) * ncl)) {
     ds.lcell.release();
     ds.lcand.release();
     return FOMO_OK;
   }
   k_lookup_build<true><<<nblk(ncl), kBlock, 0, s>>>(ds.pts.as<float4>(), ds.cstart.as<int32_t>(), g, lk, ncl,
-                                                     cnt.as<int32_t>(), off.as<int64_t>(), ds.lcand.as<int32_t>());
+                                                     cnt.as<int32_t>(), off.as<int64_t>(), ds.lcand.as<float4>());
   k_lookup_pack<<<nblk(ncl), kBlock, 0, s>>>(cnt.as<int32_t>(), off.as<int64_t>(), ncl, ds.lcell.as<int2>());
   HIPOK(hipGetLastError());
   // number of cells left to the ring search
@@ -1330,7 +1332,7 @@ void enqueue_nn(hipStream_t s, const Dataset& ds, const ViewGeom* geom, int nx, 
   dim3 grid(tiles, nz);
   ProfScope ps("k_nn");
 #define FOMO_NN_ARGS                                                                                           \
-  ds.pts.as<float4>(), ds.cstart.as<int32_t>(), ds.grid, ds.lcell.as<int2>(), ds.lcand.as<int32_t>(), ds.lk, \
+  ds.pts.as<float4>(), ds.cstart.as<int32_t>(), ds.grid, ds.lcell.as<int2>(), ds.lcand.as<float4>(), ds.lk, \
       geom, nx, ny, nz, idx, rec, nlines, svlos, sem
   if (!rec) k_nn<0><<<grid, 256, 0, s>>>(FOMO_NN_ARGS);
   else if (rs4 == 2) k_nn<2><<<grid, 256, 0, s>>>(FOMO_NN_ARGS);
