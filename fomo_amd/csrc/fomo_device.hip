@@ -1016,11 +1016,13 @@ struct Batch {
   DevBuf rec;  // per-point records {vel, all lines}
   DevBuf sem;  // per-sample (peak, fwhm), nlines x S
   int rs4 = 0;
+  std::vector<hipEvent_t> ev;  // 2 per view: NN done, integration done (graph dependencies)
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   ~Batch() {
     if (exec) (void)hipGraphExecDestroy(exec);
     if (graph) (void)hipGraphDestroy(graph);
+    for (auto e : ev) (void)hipEventDestroy(e);
   }
 };
 
@@ -1033,6 +1035,7 @@ struct Ctx {
   bool init = false;
   int device = -1;
   hipStream_t stream = nullptr;
+  hipStream_t stream2 = nullptr;  // second stream for overlapped batch work (default priority)
   hipDeviceProp_t prop{};
   int next_id = 1;
   std::map<int, std::unique_ptr<Dataset>> datasets;
@@ -1439,6 +1442,7 @@ int fomo_init(int device) {
   HIPOK(hipSetDevice(device));
   HIPOK(hipGetDeviceProperties(&g_ctx.prop, device));
   HIPOK(hipStreamCreateWithFlags(&g_ctx.stream, hipStreamNonBlocking));
+  HIPOK(hipStreamCreateWithFlags(&g_ctx.stream2, hipStreamNonBlocking));
   g_ctx.device = device;
   g_ctx.init = true;
   return FOMO_OK;
@@ -1467,6 +1471,8 @@ int fomo_shutdown(void) {
   for (auto e : g_ctx.evpool) (void)hipEventDestroy(e);
   g_ctx.evpool.clear();
   (void)hipStreamDestroy(g_ctx.stream);
+  (void)hipStreamDestroy(g_ctx.stream2);
+  g_ctx.stream2 = nullptr;
   g_ctx.stream = nullptr;
   g_ctx.init = false;
   g_ctx.device = -1;
@@ -1785,7 +1791,11 @@ int fomo_nn_indices(int dataset, double l, double b, int nx, int ny, int nz, int
 // batches: all views x lines in one captured graph
 // ---------------------------------------------------------------------------------------
 // Enqueue every view x line of a batch on stream s (captured into the graph, or eager when profiling).
-static int enqueue_batch(Batch& bt, hipStream_t s) {
+// Enqueue every view x line of a batch. With s2 (graph capture): the NN pass of view v runs on s
+// and the line integration of view v on s2, so the (memory-bound) NN search of view v+1 overlaps
+// the (VALU-bound) integration of view v; per-sample buffers are double-buffered and the NN pass
+// of view v+2 waits for the integration of view v. Without s2 everything runs in order on s.
+static int enqueue_batch(Batch& bt, hipStream_t s, hipStream_t s2) {
   Dataset* ds = find_obj(g_ctx.datasets, bt.dataset);
   if (!ds) return fail(FOMO_ERR_NOTFOUND, "dataset of this batch was destroyed");
   std::vector<Goft*> gs;
@@ -1796,25 +1806,39 @@ static int enqueue_batch(Batch& bt, hipStream_t s) {
   }
   const int nv = (int)bt.l.size();
   const int64_t R = (int64_t)bt.nx * bt.ny;
+  const size_t S = (size_t)R * bt.nz;
   const size_t outv = (size_t)R * std::max(bt.nl, 1);
+  const int L = (int)gs.size();
+  if (s2 && (int)bt.ev.size() < 2 * nv) return fail(FOMO_ERR_STATE, "batch events missing");
   for (int v = 0; v < nv; ++v) {
+    const int buf = s2 ? (v & 1) : 0;
+    float* svl = bt.aux.as<float>() + buf * S;
+    float2* sem = bt.sem.as<float2>() + (size_t)buf * S * L;
+    int32_t* flag = bt.flag.as<int32_t>() + (size_t)buf * R * kMaxLines;
     double* part = bt.partial.as<double>() + (size_t)v * 6 * kMaxPartials;
     ViewGeom* geom = bt.geom.as<ViewGeom>() + v;
-    enqueue_view_geom(s, *ds, MODE_NN, bt.l[v], bt.b[v], bt.nx, bt.ny, bt.nz, bt.dn, bt.aux.as<float>(), part, geom);
-    const size_t S = (size_t)R * bt.nz;
-    enqueue_nn(s, *ds, geom, bt.nx, bt.ny, bt.nz, nullptr, bt.rec.as<float4>(), bt.rs4, (int)gs.size(),
-               bt.aux.as<float>(), bt.sem.as<float2>());
+    if (s2 && v >= 2) HIPOK(hipStreamWaitEvent(s, bt.ev[nv + v - 2], 0));
+    enqueue_view_geom(s, *ds, MODE_NN, bt.l[v], bt.b[v], bt.nx, bt.ny, bt.nz, bt.dn, nullptr, part, geom);
+    enqueue_nn(s, *ds, geom, bt.nx, bt.ny, bt.nz, nullptr, bt.rec.as<float4>(), bt.rs4, L, svl, sem);
     std::vector<const float2*> ems;
     std::vector<float*> outs;
     std::vector<double> l0s;
-    for (size_t gi = 0; gi < gs.size(); ++gi) {
-      ems.push_back(bt.sem.as<float2>() + gi * S);
-      outs.push_back(bt.out.as<float>() + ((size_t)v * gs.size() + gi) * outv);
+    for (int gi = 0; gi < L; ++gi) {
+      ems.push_back(sem + (size_t)gi * S);
+      outs.push_back(bt.out.as<float>() + ((size_t)v * L + gi) * outv);
       l0s.push_back(gs[gi]->lambda0);
     }
     LineSet ls = make_lineset(ems, outs, l0s, bt.width);
-    enqueue_los(s, (int)R, bt.nz, ls, bt.aux.as<float>(), geom, bt.nl, bt.flag.as<int32_t>());
+    hipStream_t si = s;
+    if (s2) {
+      HIPOK(hipEventRecord(bt.ev[v], s));
+      HIPOK(hipStreamWaitEvent(s2, bt.ev[v], 0));
+      si = s2;
+    }
+    enqueue_los(si, (int)R, bt.nz, ls, svl, geom, bt.nl, flag);
+    if (s2) HIPOK(hipEventRecord(bt.ev[nv + v], s2));
   }
+  if (s2) HIPOK(hipStreamWaitEvent(s, bt.ev[2 * nv - 1], 0));  // join
   return FOMO_OK;
 }
 
@@ -1825,7 +1849,7 @@ static int capture_batch(Batch& bt) {
   const bool prof = g_ctx.profile;
   g_ctx.profile = false;  // no event records inside a capture
   HIPOK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-  int rc = enqueue_batch(bt, s);
+  int rc = enqueue_batch(bt, s, g_ctx.stream2);
   hipError_t ce = hipStreamEndCapture(s, &bt.graph);
   g_ctx.profile = prof;
   if (rc) return rc;
@@ -1859,7 +1883,8 @@ int fomo_batch_create(int ngoft, const int* gofts, int nviews, const double* l, 
   const int64_t R = (int64_t)nx * ny, S = R * nz;
   const int nlo = std::max(nl, 1);
   if (bt->partial.alloc(sizeof(double) * 6 * kMaxPartials * nviews) || bt->geom.alloc(sizeof(ViewGeom) * nviews) ||
-      bt->sem.alloc(sizeof(float2) * S * ngoft) || bt->aux.alloc(sizeof(float) * S) || bt->flag.alloc(sizeof(int32_t) * R * kMaxLines) ||
+      bt->sem.alloc(sizeof(float2) * S * ngoft * 2) || bt->aux.alloc(sizeof(float) * S * 2) ||
+      bt->flag.alloc(sizeof(int32_t) * R * kMaxLines * 2) ||
       bt->out.alloc(sizeof(float) * (size_t)R * nlo * ngoft * nviews))
     return fail(FOMO_ERR_NOMEM, "device allocation failed for batch");
   {
@@ -1867,6 +1892,11 @@ int fomo_batch_create(int ngoft, const int* gofts, int nviews, const double* l, 
     for (int gid : bt->gofts) ems.push_back(find_obj(g_ctx.gofts, gid)->em.as<float2>());
     if (int rc = build_records(g_ctx.stream, *ds, ems, bt->rec, &bt->rs4)) return rc;
     if (int rc = sync_stream()) return rc;
+  }
+  for (int i = 0; i < 2 * nviews; ++i) {
+    hipEvent_t e;
+    HIPOK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    bt->ev.push_back(e);
   }
   if (int rc = capture_batch(*bt)) return rc;
   int id = g_ctx.next_id++;
@@ -1900,7 +1930,7 @@ int fomo_batch_profile(int id) {
   if (!bt) return fail(FOMO_ERR_NOTFOUND, "unknown batch id");
   const bool prof = g_ctx.profile;
   g_ctx.profile = true;
-  int rc = enqueue_batch(*bt, g_ctx.stream);
+  int rc = enqueue_batch(*bt, g_ctx.stream, nullptr);
   HIPOK(hipGetLastError());
   int rc2 = sync_stream();
   g_ctx.profile = prof;
