@@ -50,6 +50,8 @@ def main():
     ap.add_argument("--cpu-rows", type=int, default=16, help="image rows in the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cell-scale", type=float, default=0.0)
+    ap.add_argument("--workload", choices=["batched", "thomson"], default="batched")
+    ap.add_argument("--corona-m", type=int, default=645, help="corona grid points per axis (thomson workload)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -61,6 +63,8 @@ def main():
     F.init(0)
     dev, hbm, cus = F.device_info()
     log(f"device {dev}, {hbm / 2**30:.0f} GiB HBM, {cus} CUs")
+    if args.workload == "thomson":
+        return thomson(F, args)
 
     m, nx, ny, nz, nl = args.m, args.nx, args.nx, args.nz, args.nl
     t0 = time.time()
@@ -200,6 +204,46 @@ def main():
     bt.close()
     for e in ems:
         e.close()
+    ds.close()
+    F.shutdown()
+
+
+def thomson(F, args):
+    """Thomson-scattering white-light render (pB): corona on 1 < r <= 3 Rsun, 16 views at b = pi/2
+    (l = 2 pi k / 16), 1024^2 images with 512 LOS samples. W = LOS samples (nl = 1)."""
+    t0 = time.time()
+    d = F.gen_corona(args.corona_m, seed=3, sigma=0.1)
+    n = d["x"].size
+    log(f"corona: {n} points (grid {args.corona_m}^3 inside 1 < r <= 3 Rsun) in {time.time() - t0:.1f}s")
+    t0 = time.time()
+    ds = F.Dataset(d["x"], d["y"], d["z"], [d["rho"]], cell_scale=args.cell_scale)
+    em = F.Emission.thomson(ds)
+    log(f"upload + index + emission: {time.time() - t0:.2f}s, lookup {ds.lookup_info()}")
+    nx = ny = args.nx if args.nx != 256 else 1024
+    nz = args.nz if args.nz != 128 else 512
+    nv = 16
+    ls = [2 * np.pi * k / nv for k in range(nv)]
+    b = np.pi / 2
+    for _ in range(max(1, args.warmup)):
+        F.render_thomson(em, ls[0], b, nx, ny, nz)
+    t0 = time.perf_counter()
+    for k in range(nv):
+        img, meta = F.render_thomson(em, ls[k], b, nx, ny, nz)
+    dt = time.perf_counter() - t0
+    W = float(nv) * nx * ny * nz
+    F.profile_reset()
+    F.profile_enable(True)
+    F.render_thomson(em, ls[1], b, nx, ny, nz)
+    F.profile_enable(False)
+    st = {k: F.profile_stats(k) for k in ("k_view_rotate_thomson", "k_nn", "k_los_thomson")}
+    log("per-kernel (count, ms): " + json.dumps(st))
+    out = {"metric": "LOS samples per second, Thomson pB render (16 views)", "value": W / dt,
+           "unit": "LOS samples/s", "n_gpus": 1, "steps": nv, "warmup": args.warmup, "ms_per_step": dt / nv * 1e3,
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+           "config": {"workload": f"Thomson pB, corona {n} points, 16 views b=pi/2, {nx}x{ny} image, nz {nz}"},
+           "kernels_ms": {k: (v[1] / v[0] if v[0] else None) for k, v in st.items()}}
+    print(json.dumps(out), flush=True)
+    em.close()
     ds.close()
     F.shutdown()
 

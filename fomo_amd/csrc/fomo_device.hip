@@ -701,6 +701,150 @@ __device__ int nn_query(const float4* __restrict__ pts, const int32_t* __restric
   return bpos;
 }
 
+// ------------------------------------------------------------------------------------
+// Implicit BVH over the points in Morton order (exact fallback of the NN query).
+// Level 0 nodes bound 8 consecutive Morton-ordered points, level k nodes bound 8 level k-1
+// nodes. The query is the oracle's kd-tree query: best-first descent, a node is skipped when no
+// point of it can be strictly inside the search box or when its mindist exceeds the best distance.
+// ------------------------------------------------------------------------------------
+constexpr int kBvhMaxLevels = 12;
+struct BvhInfo {
+  int levels;                 // number of node levels (level levels-1 is the root level)
+  int64_t n;                  // points
+  int64_t off[kBvhMaxLevels]; // node offset of each level in the node arrays
+  int64_t cnt[kBvhMaxLevels]; // nodes per level
+};
+
+__global__ void k_morton_keys(const float4* __restrict__ pts, int64_t n, GridInfo g, uint32_t* __restrict__ key,
+                              int32_t* __restrict__ val) {
+  const int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  const float4 q = pts[p];
+  const float c[3] = {q.x, q.y, q.z};
+  uint32_t code = 0;
+  uint32_t u[3];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    const double ext = (double)g.bmax[a] - (double)g.bmin[a];
+    double t = ext > 0 ? ((double)c[a] - (double)g.bmin[a]) / ext : 0.0;
+    t = fmin(fmax(t, 0.0), 1.0);
+    u[a] = (uint32_t)fmin(t * 1024.0, 1023.0);
+  }
+#pragma unroll
+  for (int bit = 9; bit >= 0; --bit)
+    code = (code << 3) | (((u[0] >> bit) & 1u) << 2) | (((u[1] >> bit) & 1u) << 1) | ((u[2] >> bit) & 1u);
+  key[p] = code;
+  val[p] = (int32_t)p;
+}
+
+// Morton-ordered points (x, y, z, sorted position) and level-0 boxes
+__global__ void k_bvh_leaves(const float4* __restrict__ pts, const int32_t* __restrict__ order, int64_t n,
+                             float4* __restrict__ mpts, float4* __restrict__ lo, float4* __restrict__ hi) {
+  const int64_t b = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const int64_t nb = (n + 7) / 8;
+  if (b >= nb) return;
+  float4 l = make_float4(INFINITY, INFINITY, INFINITY, 0.f), h = make_float4(-INFINITY, -INFINITY, -INFINITY, 0.f);
+  for (int64_t i = b * 8; i < min(n, b * 8 + 8); ++i) {
+    const int32_t p = order[i];
+    const float4 q = pts[p];
+    mpts[i] = make_float4(q.x, q.y, q.z, __int_as_float(p));
+    l.x = fminf(l.x, q.x); l.y = fminf(l.y, q.y); l.z = fminf(l.z, q.z);
+    h.x = fmaxf(h.x, q.x); h.y = fmaxf(h.y, q.y); h.z = fmaxf(h.z, q.z);
+  }
+  lo[b] = l;
+  hi[b] = h;
+}
+
+__global__ void k_bvh_level(const float4* __restrict__ clo, const float4* __restrict__ chi, int64_t nchild,
+                            float4* __restrict__ lo, float4* __restrict__ hi) {
+  const int64_t b = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const int64_t nb = (nchild + 7) / 8;
+  if (b >= nb) return;
+  float4 l = make_float4(INFINITY, INFINITY, INFINITY, 0.f), h = make_float4(-INFINITY, -INFINITY, -INFINITY, 0.f);
+  for (int64_t i = b * 8; i < min(nchild, b * 8 + 8); ++i) {
+    const float4 a = clo[i], c = chi[i];
+    l.x = fminf(l.x, a.x); l.y = fminf(l.y, a.y); l.z = fminf(l.z, a.z);
+    h.x = fmaxf(h.x, c.x); h.y = fmaxf(h.y, c.y); h.z = fmaxf(h.z, c.z);
+  }
+  lo[b] = l;
+  hi[b] = h;
+}
+
+__device__ __forceinline__ double node_mindist2(const float4 l, const float4 h, float t0, float t1, float t2) {
+  double s = 0.0, d;
+  d = t0 < l.x ? (double)l.x - (double)t0 : (t0 > h.x ? (double)t0 - (double)h.x : 0.0);
+  s = s + d * d;
+  d = t1 < l.y ? (double)l.y - (double)t1 : (t1 > h.y ? (double)t1 - (double)h.y : 0.0);
+  s = s + d * d;
+  d = t2 < l.z ? (double)l.z - (double)t2 : (t2 > h.z ? (double)t2 - (double)h.z : 0.0);
+  s = s + d * d;
+  return s;
+}
+
+// Nearest point strictly inside the box, ties -> lowest original index; returns the sorted
+// position or -1. Best-first with an explicit stack (level in the top 4 bits).
+__device__ int bvh_query(const float4* __restrict__ pts, const float4* __restrict__ mpts,
+                         const float4* __restrict__ nlo, const float4* __restrict__ nhi, const BvhInfo& bv,
+                         float t0, float t1, float t2, float lo0, float lo1, float lo2, float hi0, float hi1,
+                         float hi2) {
+  if (bv.levels <= 0) return -1;
+  double best = INFINITY;
+  int bpos = -1, borig = 0x7fffffff;
+  uint32_t stack[8 * kBvhMaxLevels + 8];
+  int sp = 0;
+  const int top = bv.levels - 1;
+  for (int64_t i = bv.cnt[top] - 1; i >= 0; --i) stack[sp++] = ((uint32_t)top << 28) | (uint32_t)i;
+  while (sp > 0) {
+    const uint32_t e = stack[--sp];
+    const int lev = (int)(e >> 28);
+    const int64_t idx = (int64_t)(e & 0x0fffffffu);
+    const float4 l = nlo[bv.off[lev] + idx], h = nhi[bv.off[lev] + idx];
+    if (!(h.x > lo0 && l.x < hi0 && h.y > lo1 && l.y < hi1 && h.z > lo2 && l.z < hi2)) continue;
+    if (node_mindist2(l, h, t0, t1, t2) > best) continue;
+    if (lev == 0) {
+      for (int64_t i = idx * 8; i < min(bv.n, idx * 8 + 8); ++i) {
+        const float4 q = mpts[i];
+        if (!(lo0 < q.x && q.x < hi0 && lo1 < q.y && q.y < hi1 && lo2 < q.z && q.z < hi2)) continue;
+        const double d0 = (double)q.x - (double)t0;
+        const double d1 = (double)q.y - (double)t1;
+        const double d2 = (double)q.z - (double)t2;
+        const double d = (d0 * d0 + d1 * d1) + d2 * d2;
+        const int p = __float_as_int(q.w);
+        if (d < best || d == best) {
+          const int oi = __float_as_int(pts[p].w);
+          if (d < best || oi < borig) {
+            best = d;
+            bpos = p;
+            borig = oi;
+          }
+        }
+      }
+    } else {
+      // children sorted by mindist; pushed farthest first so the nearest is popped first
+      const int64_t c0 = idx * 8, c1 = min(bv.cnt[lev - 1], idx * 8 + 8);
+      double key[8];
+      uint32_t ent[8];
+      int nc = 0;
+      for (int64_t c = c0; c < c1; ++c) {
+        const float4 cl = nlo[bv.off[lev - 1] + c], ch = nhi[bv.off[lev - 1] + c];
+        if (!(ch.x > lo0 && cl.x < hi0 && ch.y > lo1 && cl.y < hi1 && ch.z > lo2 && cl.z < hi2)) continue;
+        const double md = node_mindist2(cl, ch, t0, t1, t2);
+        if (md > best) continue;
+        int j = nc++;
+        while (j > 0 && key[j - 1] < md) {
+          key[j] = key[j - 1];
+          ent[j] = ent[j - 1];
+          --j;
+        }
+        key[j] = md;
+        ent[j] = ((uint32_t)(lev - 1) << 28) | (uint32_t)c;
+      }
+      for (int j = 0; j < nc; ++j) stack[sp++] = ent[j];
+    }
+  }
+  return bpos;
+}
+
 // One thread per LOS sample. Block = 16x16 rays at one k (blockIdx.y = k).
 // RS4 > 0: per-point records of RS4 float4 {vx,vy,vz, peak_0,fwhm_0, ..., peak_{L-1},fwhm_{L-1}}
 // are gathered once for the chosen point, and the kernel writes coalesced per-sample arrays:
@@ -710,6 +854,8 @@ template <int RS4>
 __global__ void __launch_bounds__(256) k_nn(const float4* __restrict__ pts, const int32_t* __restrict__ cstart,
                                             GridInfo g, const int2* __restrict__ lcell,
                                             const float4* __restrict__ lcand, LookupInfo lk,
+                                            const float4* __restrict__ mpts, const float4* __restrict__ nlo,
+                                            const float4* __restrict__ nhi, BvhInfo bv,
                                             const ViewGeom* __restrict__ vgp, int nx, int ny, int nz,
                                             int32_t* __restrict__ idx, const float4* __restrict__ rec, int nlines,
                                             float* __restrict__ svlos, float2* __restrict__ sem) {
@@ -734,7 +880,18 @@ __global__ void __launch_bounds__(256) k_nn(const float4* __restrict__ pts, cons
     r = nn_lookup(pts, lcell, lcand, lk, (float)p0, (float)p1, (float)p2, (float)(p0 - md), (float)(p1 - md),
                   (float)(p2 - md), (float)(p0 + md), (float)(p1 + md), (float)(p2 + md));
   }
-  if (r == -2) r = nn_query(pts, cstart, g, p0, p1, p2, vg.md);
+  if (r == -2) {
+    const double md = vg.md;
+    const float lo0 = (float)(p0 - md), lo1 = (float)(p1 - md), lo2 = (float)(p2 - md);
+    const float hi0 = (float)(p0 + md), hi1 = (float)(p1 + md), hi2 = (float)(p2 + md);
+    if (!(hi0 > g.bmin[0] && lo0 < g.bmax[0] && hi1 > g.bmin[1] && lo1 < g.bmax[1] && hi2 > g.bmin[2] &&
+          lo2 < g.bmax[2]))
+      r = -1;  // no data point can be strictly inside the box
+    else if (bv.levels > 0)
+      r = bvh_query(pts, mpts, nlo, nhi, bv, (float)p0, (float)p1, (float)p2, lo0, lo1, lo2, hi0, hi1, hi2);
+    else
+      r = nn_query(pts, cstart, g, p0, p1, p2, md);
+  }
   const size_t S = (size_t)nx * ny * nz;
   const size_t sidx = (size_t)k * ((size_t)nx * ny) + (size_t)i * nx + j;
   if (idx) idx[sidx] = r;
@@ -979,6 +1136,9 @@ struct Dataset {
   int lookup_subdiv = 2;
   DevBuf lcell;        // int2 (offset, count) per lookup cell; count -1 = use the ring search
   DevBuf lcand;        // float4 candidates (x, y, z, sorted position as int bits)
+  BvhInfo bvh{};       // implicit BVH over Morton-ordered points (exact NN fallback)
+  DevBuf mpts;         // float4 (x, y, z, sorted position) in Morton order
+  DevBuf nlo, nhi;     // node boxes, all levels
   int64_t ncand = 0, nlcells = 0, nfallback = 0;
 };
 
@@ -1109,6 +1269,55 @@ int sync_stream() {
 }
 
 inline unsigned nblk(int64_t n, int bs = kBlock) { return (unsigned)((n + bs - 1) / bs); }
+
+// Implicit BVH over the dataset's points (Morton order of 10-bit quantised coordinates).
+int build_bvh(Dataset& ds) {
+  hipStream_t s = g_ctx.stream;
+  const int64_t n = ds.n;
+  ds.bvh = BvhInfo{};
+  HIPOK(g_ctx.sortkeys[0].ensure(sizeof(uint32_t) * n));
+  HIPOK(g_ctx.sortkeys[1].ensure(sizeof(uint32_t) * n));
+  HIPOK(g_ctx.sortvals.ensure(sizeof(int32_t) * n));
+  DevBuf order;
+  if (order.alloc(sizeof(int32_t) * n)) return fail(FOMO_ERR_NOMEM, "device allocation failed (bvh)");
+  k_morton_keys<<<nblk(n), kBlock, 0, s>>>(ds.pts.as<float4>(), n, ds.grid, g_ctx.sortkeys[0].as<uint32_t>(),
+                                           g_ctx.sortvals.as<int32_t>());
+  HIPOK(hipGetLastError());
+  size_t tmpb = 0;
+  HIPOK(rocprim::radix_sort_pairs(nullptr, tmpb, g_ctx.sortkeys[0].as<uint32_t>(), g_ctx.sortkeys[1].as<uint32_t>(),
+                                  g_ctx.sortvals.as<int32_t>(), order.as<int32_t>(), (size_t)n, 0, 30, s));
+  HIPOK(g_ctx.sorttmp.ensure(tmpb));
+  HIPOK(rocprim::radix_sort_pairs(g_ctx.sorttmp.p, tmpb, g_ctx.sortkeys[0].as<uint32_t>(),
+                                  g_ctx.sortkeys[1].as<uint32_t>(), g_ctx.sortvals.as<int32_t>(), order.as<int32_t>(),
+                                  (size_t)n, 0, 30, s));
+  BvhInfo bv{};
+  bv.n = n;
+  int64_t c = (n + 7) / 8, total = 0;
+  int lev = 0;
+  for (;;) {
+    if (lev >= kBvhMaxLevels) return fail(FOMO_ERR_ARG, "too many points for the BVH");
+    bv.off[lev] = total;
+    bv.cnt[lev] = c;
+    total += c;
+    ++lev;
+    if (c <= 8) break;
+    c = (c + 7) / 8;
+  }
+  bv.levels = lev;
+  if (ds.mpts.ensure(sizeof(float4) * n) || ds.nlo.ensure(sizeof(float4) * total) ||
+      ds.nhi.ensure(sizeof(float4) * total))
+    return fail(FOMO_ERR_NOMEM, "device allocation failed (bvh)");
+  k_bvh_leaves<<<nblk(bv.cnt[0]), kBlock, 0, s>>>(ds.pts.as<float4>(), order.as<int32_t>(), n, ds.mpts.as<float4>(),
+                                                 ds.nlo.as<float4>(), ds.nhi.as<float4>());
+  for (int l = 1; l < bv.levels; ++l)
+    k_bvh_level<<<nblk(bv.cnt[l]), kBlock, 0, s>>>(ds.nlo.as<float4>() + bv.off[l - 1], ds.nhi.as<float4>() + bv.off[l - 1],
+                                                  bv.cnt[l - 1], ds.nlo.as<float4>() + bv.off[l],
+                                                  ds.nhi.as<float4>() + bv.off[l]);
+  HIPOK(hipGetLastError());
+  if (int rc = sync_stream()) return rc;
+  ds.bvh = bv;
+  return FOMO_OK;
+}
 
 // Build the candidate lookup grid of a dataset (after its cell list). subdiv lookup cells per
 // cell-list edge; 0 disables the fast path. The grid extends kLookupMargin cell-list cells beyond
@@ -1291,6 +1500,7 @@ int build_index(Dataset& ds, double cell_scale) {
   rc = sync_stream();
   if (rc) return rc;
   ds.maxcount = hmx;
+  if (int rc2 = build_bvh(ds)) return rc2;
   return build_lookup(ds);
 }
 
@@ -1336,6 +1546,7 @@ void enqueue_nn(hipStream_t s, const Dataset& ds, const ViewGeom* geom, int nx, 
   ProfScope ps("k_nn");
 #define FOMO_NN_ARGS                                                                                           \
   ds.pts.as<float4>(), ds.cstart.as<int32_t>(), ds.grid, ds.lcell.as<int2>(), ds.lcand.as<float4>(), ds.lk, \
+      ds.mpts.as<float4>(), ds.nlo.as<float4>(), ds.nhi.as<float4>(), ds.bvh,                                 \
       geom, nx, ny, nz, idx, rec, nlines, svlos, sem
   if (!rec) k_nn<0><<<grid, 256, 0, s>>>(FOMO_NN_ARGS);
   else if (rs4 == 2) k_nn<2><<<grid, 256, 0, s>>>(FOMO_NN_ARGS);

@@ -2,6 +2,7 @@
 // the deterministic synthetic-input generators used by the tests and the benchmark.
 
 #include <cmath>
+#include <complex>
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
@@ -224,14 +225,26 @@ int fomo_gen_corona(int m, uint64_t seed, double sigma, int64_t* count, float* x
     asum += amp[q] * amp[q] / 2;
   }
   const double norm = sigma / std::sqrt(asum);
+  // plane waves are separable: exp(i k.x) = exp(i kx x) exp(i ky y) exp(i kz z); tabulate the
+  // factors per mode along each axis so a point costs two complex products per mode
+  typedef std::complex<double> cd;
+  std::vector<cd> ex((size_t)nmodes * m), ey((size_t)nmodes * m), ez((size_t)nmodes * m);
+  for (int q = 0; q < nmodes; ++q)
+    for (int i = 0; i < m; ++i) {
+      const double c = -3 * rs + h * i;
+      ex[(size_t)q * m + i] = std::polar(1.0, kx[q] * c);
+      ey[(size_t)q * m + i] = std::polar(1.0, ky[q] * c);
+      ez[(size_t)q * m + i] = std::polar(amp[q] * norm, kz[q] * c + ph[q]);
+    }
   parallel_for(m, [&](int64_t k) {
     int64_t w = slab[k];
-    for (int64_t q = 0; q < (int64_t)m * m; ++q) {
+    for (int64_t q2 = 0; q2 < (int64_t)m * m; ++q2) {
       double xc, yc, zc, r;
-      if (!inside(k * m * m + q, &xc, &yc, &zc, &r)) continue;
+      if (!inside(k * m * m + q2, &xc, &yc, &zc, &r)) continue;
+      const int64_t i = q2 % m, j = q2 / m;
       double d = 0;
-      for (int t = 0; t < nmodes; ++t) d += amp[t] * std::cos(kx[t] * xc + ky[t] * yc + kz[t] * zc + ph[t]);
-      d *= norm;
+      for (int t = 0; t < nmodes; ++t)
+        d += (ex[(size_t)t * m + i] * ey[(size_t)t * m + j] * ez[(size_t)t * m + k]).real();
       if (d < -0.9) d = -0.9;
       x[w] = (float)xc;
       y[w] = (float)yc;

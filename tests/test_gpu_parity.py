@@ -222,3 +222,23 @@ def test_lookup_fast_path_matches_ring_search(engine, jitter):
     ds.set_lookup(1)
     assert np.array_equal(F.nn_indices(ds, *views[1], 48, 44, 50)[0], fast[1])
     ds.close()
+
+
+def test_thomson_solar_disk_samples(engine):
+    """Samples inside the solar disk hole and outside the shell resolve through the exact BVH path."""
+    F = engine
+    d = F.gen_corona(72, seed=8)
+    ds = F.Dataset(d["x"], d["y"], d["z"], [d["rho"]])
+    em = F.Emission.thomson(ds)
+    rt, rp = O.thomson_emission(1, d["x"], d["y"], d["z"], d["rho"])
+    for (l, b) in [(0.0, np.pi / 2), (0.9, 0.3)]:
+        nx, ny, nz = 64, 60, 48
+        img, meta = F.render_thomson(em, l, b, nx, ny, nz)
+        idx, _ = F.nn_indices(ds, l, b, nx, ny, nz, thomson=True)
+        ref, ridx, rmeta, _ = O.render_thomson(1, d["x"], d["y"], d["z"], rt, rp, l, b, nx, ny, nz, want_idx=True)
+        assert np.array_equal(idx, ridx), f"{(idx != ridx).sum()} NN choices differ"
+        assert O.rel_l2(img, ref) <= TOL_IMG
+    ds.set_lookup(0)  # BVH only
+    idx0, _ = F.nn_indices(ds, 0.9, 0.3, 64, 60, 48, thomson=True)
+    assert np.array_equal(idx0, ridx)
+    em.close(), ds.close()
