@@ -225,10 +225,14 @@ def thomson(F, args):
     ls = [2 * np.pi * k / nv for k in range(nv)]
     b = np.pi / 2
     for _ in range(max(1, args.warmup)):
+        t1 = time.perf_counter()
         F.render_thomson(em, ls[0], b, nx, ny, nz)
+        log(f"warmup view: {(time.perf_counter() - t1) * 1e3:.1f} ms")
     t0 = time.perf_counter()
     for k in range(nv):
+        t1 = time.perf_counter()
         img, meta = F.render_thomson(em, ls[k], b, nx, ny, nz)
+        log(f"view {k}: {(time.perf_counter() - t1) * 1e3:.1f} ms")
     dt = time.perf_counter() - t0
     W = float(nv) * nx * ny * nz
     F.profile_reset()

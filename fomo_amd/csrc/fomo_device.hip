@@ -454,8 +454,8 @@ __device__ __forceinline__ bool dominated(const double c[3], const double hw[3],
 
 template <bool FILL>
 __global__ void k_lookup_build(const float4* __restrict__ pts, const int32_t* __restrict__ cstart, GridInfo g,
-                               LookupInfo lk, int64_t ncl, int32_t* __restrict__ cnt, const int64_t* __restrict__ off,
-                               float4* __restrict__ cand) {
+                               LookupInfo lk, int64_t ncl, const uint8_t* __restrict__ near,
+                               int32_t* __restrict__ cnt, const int64_t* __restrict__ off, float4* __restrict__ cand) {
   const int64_t L = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (L >= ncl) return;
   if (FILL && cnt[L] <= 0) return;
@@ -473,6 +473,11 @@ __global__ void k_lookup_build(const float4* __restrict__ pts, const int32_t* __
   }
   const int cc0 = cell_coord(g, 0, c[0]), cc1 = cell_coord(g, 1, c[1]), cc2 = cell_coord(g, 2, c[2]);
   const int dx = g.dims[0], dxy = g.dims[0] * g.dims[1];
+  // no point within kLookupRings cells: leave the cell to the exact BVH query
+  if (!near[(int64_t)cc2 * dxy + (int64_t)cc1 * dx + cc0]) {
+    if (!FILL) cnt[L] = -1;
+    return;
+  }
   // step 1: upper bound U and dominators
   double U2 = INFINITY;
   float4 dom[4];
@@ -482,9 +487,14 @@ __global__ void k_lookup_build(const float4* __restrict__ pts, const int32_t* __
   int found = -1;
   for (int r = 0; r <= kLookupRings; ++r) {
     for (int z = max(cc2 - r, 0); z <= min(cc2 + r, g.dims[2] - 1); ++z)
-      for (int y = max(cc1 - r, 0); y <= min(cc1 + r, g.dims[1] - 1); ++y)
-        for (int x = max(cc0 - r, 0); x <= min(cc0 + r, g.dims[0] - 1); ++x) {
-          if (max(max(abs(x - cc0), abs(y - cc1)), abs(z - cc2)) != r) continue;
+      for (int y = max(cc1 - r, 0); y <= min(cc1 + r, g.dims[1] - 1); ++y) {
+        // cells at Chebyshev distance exactly r: the whole x-range on an edge row, else x = cc0 -/+ r
+        const bool edge = (abs(z - cc2) == r) || (abs(y - cc1) == r);
+        const int xs = max(cc0 - r, 0), xe = min(cc0 + r, g.dims[0] - 1);
+        const int nvis = edge ? (xe - xs + 1) : 2;
+        for (int t = 0; t < nvis; ++t) {
+          const int x = edge ? xs + t : (t == 0 ? cc0 - r : cc0 + r);
+          if (x < 0 || x >= g.dims[0]) continue;
           const int cell = z * dxy + y * dx + x;
           for (int p = cstart[cell]; p < cstart[cell + 1]; ++p) {
             const float4 q = pts[p];
@@ -507,6 +517,7 @@ __global__ void k_lookup_build(const float4* __restrict__ pts, const int32_t* __
             dpos[k] = p;
           }
         }
+      }
     if (found < 0 && nd > 0) found = r;
     if (found >= 0 && r >= found + 1) break;
   }
@@ -547,6 +558,24 @@ __global__ void k_lookup_build(const float4* __restrict__ pts, const int32_t* __
       }
     }
   if (!FILL) cnt[L] = n;
+}
+
+// near[c] = 1 when a non-empty cell lies within Chebyshev distance R of cell c (separable max filter)
+__global__ void k_occupancy(const int32_t* __restrict__ cstart, int64_t ncells, uint8_t* __restrict__ occ) {
+  const int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (c < ncells) occ[c] = cstart[c + 1] > cstart[c] ? 1 : 0;
+}
+__global__ void k_dilate_axis(const uint8_t* __restrict__ in, uint8_t* __restrict__ out, GridInfo g, int axis, int R) {
+  const int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const int64_t n = (int64_t)g.dims[0] * g.dims[1] * g.dims[2];
+  if (c >= n) return;
+  const int64_t stride = axis == 0 ? 1 : (axis == 1 ? (int64_t)g.dims[0] : (int64_t)g.dims[0] * g.dims[1]);
+  const int coord = axis == 0 ? (int)(c % g.dims[0])
+                              : (axis == 1 ? (int)((c / g.dims[0]) % g.dims[1]) : (int)(c / ((int64_t)g.dims[0] * g.dims[1])));
+  const int lo = max(coord - R, 0) - coord, hi = min(coord + R, g.dims[axis] - 1) - coord;
+  uint8_t v = 0;
+  for (int d = lo; d <= hi && !v; ++d) v = in[c + d * stride];
+  out[c] = v;
 }
 
 __global__ void k_lookup_pack(const int32_t* __restrict__ cnt, const int64_t* __restrict__ off, int64_t ncl,
@@ -1133,7 +1162,8 @@ struct Dataset {
   std::vector<std::unique_ptr<DevBuf>> svars;  // sorted variables
   DevBuf vel;          // float4 sorted (vx,vy,vz,0) when nvars>=5
   LookupInfo lk{};     // candidate lookup grid (lk.enabled==0: ring search only)
-  int lookup_subdiv = 2;
+  int lookup_subdiv = 2;         // requested subdivision
+  int lookup_active_subdiv = 0;  // subdivision actually built (0: none)
   DevBuf lcell;        // int2 (offset, count) per lookup cell; count -1 = use the ring search
   DevBuf lcand;        // float4 candidates (x, y, z, sorted position as int bits)
   BvhInfo bvh{};       // implicit BVH over Morton-ordered points (exact NN fallback)
@@ -1331,8 +1361,19 @@ int build_lookup(Dataset& ds) {
   if (ds.lookup_subdiv <= 0) return FOMO_OK;
   hipStream_t s = g_ctx.stream;
   const GridInfo& g = ds.grid;
+  // subdivision: the requested one, reduced until the grid fits ~24 cells per point and 2^30
+  int subdiv = ds.lookup_subdiv;
+  for (; subdiv >= 1; --subdiv) {
+    double ncl_est = 1;
+    for (int a = 0; a < 3; ++a) {
+      const double mrg = (g.dims[a] > 1 ? kLookupMargin : 0) * g.h;
+      ncl_est *= std::floor(((double)g.bmax[a] - g.o[a] + 2 * mrg + 0.3 * g.h / subdiv) / (g.h / subdiv)) + 1;
+    }
+    if (ncl_est <= 24.0 * ds.n + 65536 && ncl_est <= (double)(1LL << 30)) break;
+  }
+  if (subdiv < 1) return FOMO_OK;
   LookupInfo lk{};
-  lk.h = g.h / ds.lookup_subdiv;
+  lk.h = g.h / subdiv;
   lk.invh = 1.0 / lk.h;
   double ext = 0, mag = 0;
   int64_t ncl = 1;
@@ -1349,13 +1390,21 @@ int build_lookup(Dataset& ds) {
   if (ncl > 24 * ds.n + 65536 || ncl > (1LL << 30)) return FOMO_OK;
   lk.eps = 1e-7 * lk.h + 1e-12 * (mag + ext);
   lk.enabled = 1;
+  // cells of the cell list with a point within kLookupRings cells
+  DevBuf near0, near1;
+  if (near0.alloc(ds.ncells) || near1.alloc(ds.ncells)) return FOMO_OK;
+  k_occupancy<<<nblk(ds.ncells), kBlock, 0, s>>>(ds.cstart.as<int32_t>(), ds.ncells, near0.as<uint8_t>());
+  k_dilate_axis<<<nblk(ds.ncells), kBlock, 0, s>>>(near0.as<uint8_t>(), near1.as<uint8_t>(), g, 0, kLookupRings);
+  k_dilate_axis<<<nblk(ds.ncells), kBlock, 0, s>>>(near1.as<uint8_t>(), near0.as<uint8_t>(), g, 1, kLookupRings);
+  k_dilate_axis<<<nblk(ds.ncells), kBlock, 0, s>>>(near0.as<uint8_t>(), near1.as<uint8_t>(), g, 2, kLookupRings);
+  HIPOK(hipGetLastError());
   DevBuf cnt, pos, off;
   if (cnt.alloc(sizeof(int32_t) * ncl) || pos.alloc(sizeof(int32_t) * ncl) || off.alloc(sizeof(int64_t) * ncl))
     return FOMO_OK;  // not enough memory: ring search only
   {
     ProfScope ps("k_lookup_build");
     k_lookup_build<false><<<nblk(ncl), kBlock, 0, s>>>(ds.pts.as<float4>(), ds.cstart.as<int32_t>(), g, lk, ncl,
-                                                        cnt.as<int32_t>(), nullptr, nullptr);
+                                                        near1.as<uint8_t>(), cnt.as<int32_t>(), nullptr, nullptr);
   }
   k_clamp_counts<<<nblk(ncl), kBlock, 0, s>>>(cnt.as<int32_t>(), ncl, pos.as<int32_t>());
   HIPOK(hipGetLastError());
@@ -1378,7 +1427,8 @@ int build_lookup(Dataset& ds) {
     return FOMO_OK;
   }
   k_lookup_build<true><<<nblk(ncl), kBlock, 0, s>>>(ds.pts.as<float4>(), ds.cstart.as<int32_t>(), g, lk, ncl,
-                                                     cnt.as<int32_t>(), off.as<int64_t>(), ds.lcand.as<float4>());
+                                                     near1.as<uint8_t>(), cnt.as<int32_t>(), off.as<int64_t>(),
+                                                     ds.lcand.as<float4>());
   k_lookup_pack<<<nblk(ncl), kBlock, 0, s>>>(cnt.as<int32_t>(), off.as<int64_t>(), ncl, ds.lcell.as<int2>());
   HIPOK(hipGetLastError());
   // number of cells left to the ring search
@@ -1388,6 +1438,7 @@ int build_lookup(Dataset& ds) {
   HIPOK(hipMemcpyAsync(&nfb, dfb, sizeof(int), hipMemcpyDeviceToHost, s));
   if (int rc = sync_stream()) return rc;
   ds.nfallback = nfb;
+  ds.lookup_active_subdiv = subdiv;
   ds.lk = lk;
   ds.ncand = total;
   ds.nlcells = ncl;
@@ -1769,7 +1820,7 @@ int fomo_dataset_lookup_info(int id, int* subdiv, int64_t* cells, int64_t* candi
   if (int rc = require_init()) return rc;
   Dataset* ds = find_obj(g_ctx.datasets, id);
   if (!ds) return fail(FOMO_ERR_NOTFOUND, "unknown dataset id");
-  if (subdiv) *subdiv = ds->lk.enabled ? ds->lookup_subdiv : 0;
+  if (subdiv) *subdiv = ds->lk.enabled ? ds->lookup_active_subdiv : 0;
   if (cells) *cells = ds->nlcells;
   if (candidates) *candidates = ds->ncand;
   if (fallback_cells) *fallback_cells = ds->nfallback;
