@@ -644,93 +644,6 @@ __device__ __forceinline__ int nn_lookup(const float4* __restrict__ pts, const i
 }
 
 // ------------------------------------------------------------------------------------
-// K4: nearest-neighbour query per LOS sample (fomo-nearestneighbour.cpp:148-193).
-// Semantics of rtree.query(nearest(target,1) && within(box)): nearest point strictly inside
-// the float box [float(p-md), float(p+md)], by double squared distance of float coordinates
-// to target=float(p); ties -> lowest original index. Ring search over the cell list, stopping
-// once every unsearched cell is provably farther than the best candidate.
-// Output: sorted position of the chosen point, or -1; layout idx[k*R + ray].
-// ------------------------------------------------------------------------------------
-__device__ __forceinline__ void nn_visit(const float4* __restrict__ pts, int s, int e, float t0, float t1, float t2,
-                                         float lo0, float lo1, float lo2, float hi0, float hi1, float hi2, double& best,
-                                         int& bidx, int& bpos) {
-  for (int p = s; p < e; ++p) {
-    const float4 q = pts[p];
-    if (lo0 < q.x && q.x < hi0 && lo1 < q.y && q.y < hi1 && lo2 < q.z && q.z < hi2) {
-      const double d0 = (double)q.x - (double)t0;
-      const double d1 = (double)q.y - (double)t1;
-      const double d2 = (double)q.z - (double)t2;
-      const double d = (d0 * d0 + d1 * d1) + d2 * d2;
-      const int oi = __float_as_int(q.w);
-      if (d < best || (d == best && oi < bidx)) {
-        best = d;
-        bidx = oi;
-        bpos = p;
-      }
-    }
-  }
-}
-
-__device__ int nn_query(const float4* __restrict__ pts, const int32_t* __restrict__ cstart, const GridInfo& g,
-                        double p0, double p1, double p2, double md) {
-  const float t0 = (float)p0, t1 = (float)p1, t2 = (float)p2;
-  const float lo0 = (float)(p0 - md), lo1 = (float)(p1 - md), lo2 = (float)(p2 - md);
-  const float hi0 = (float)(p0 + md), hi1 = (float)(p1 + md), hi2 = (float)(p2 + md);
-  // no data point can be strictly inside the box
-  if (!(hi0 > g.bmin[0] && lo0 < g.bmax[0] && hi1 > g.bmin[1] && lo1 < g.bmax[1] && hi2 > g.bmin[2] &&
-        lo2 < g.bmax[2]))
-    return -1;
-  const int rl0 = cell_coord(g, 0, lo0), rh0 = cell_coord(g, 0, hi0);
-  const int rl1 = cell_coord(g, 1, lo1), rh1 = cell_coord(g, 1, hi1);
-  const int rl2 = cell_coord(g, 2, lo2), rh2 = cell_coord(g, 2, hi2);
-  const int c0 = min(max(cell_coord(g, 0, t0), rl0), rh0);
-  const int c1 = min(max(cell_coord(g, 1, t1), rl1), rh1);
-  const int c2 = min(max(cell_coord(g, 2, t2), rl2), rh2);
-  const int dx = g.dims[0], dxy = g.dims[0] * g.dims[1];
-  double best = INFINITY;
-  int bidx = 0x7fffffff, bpos = -1;
-  for (int r = 0;; ++r) {
-    const int z0 = max(c2 - r, rl2), z1 = min(c2 + r, rh2);
-    const int y0 = max(c1 - r, rl1), y1 = min(c1 + r, rh1);
-    const int x0 = max(c0 - r, rl0), x1 = min(c0 + r, rh0);
-    for (int zc = z0; zc <= z1; ++zc) {
-      const bool ze = (zc == c2 - r) || (zc == c2 + r);
-      for (int yc = y0; yc <= y1; ++yc) {
-        const bool ye = ze || (yc == c1 - r) || (yc == c1 + r);
-        const int rowbase = zc * dxy + yc * dx;
-        if (ye) {
-          // whole x-row of the shell face: cells x0..x1 are contiguous in the sorted order
-          const int s = cstart[rowbase + x0], e = cstart[rowbase + x1 + 1];
-          nn_visit(pts, s, e, t0, t1, t2, lo0, lo1, lo2, hi0, hi1, hi2, best, bidx, bpos);
-        } else {
-          if (c0 - r >= rl0) {
-            const int cc = rowbase + c0 - r;
-            nn_visit(pts, cstart[cc], cstart[cc + 1], t0, t1, t2, lo0, lo1, lo2, hi0, hi1, hi2, best, bidx, bpos);
-          }
-          if (r > 0 && c0 + r <= rh0) {
-            const int cc = rowbase + c0 + r;
-            nn_visit(pts, cstart[cc], cstart[cc + 1], t0, t1, t2, lo0, lo1, lo2, hi0, hi1, hi2, best, bidx, bpos);
-          }
-        }
-      }
-    }
-    // lower bound on the distance of any point in a cell not yet searched
-    bool open = false;
-    double lb = INFINITY;
-    if (c0 - r > rl0) { open = true; lb = fmin(lb, (double)t0 - (g.o[0] + (double)(c0 - r) * g.h)); }
-    if (c0 + r < rh0) { open = true; lb = fmin(lb, (g.o[0] + (double)(c0 + r + 1) * g.h) - (double)t0); }
-    if (c1 - r > rl1) { open = true; lb = fmin(lb, (double)t1 - (g.o[1] + (double)(c1 - r) * g.h)); }
-    if (c1 + r < rh1) { open = true; lb = fmin(lb, (g.o[1] + (double)(c1 + r + 1) * g.h) - (double)t1); }
-    if (c2 - r > rl2) { open = true; lb = fmin(lb, (double)t2 - (g.o[2] + (double)(c2 - r) * g.h)); }
-    if (c2 + r < rh2) { open = true; lb = fmin(lb, (g.o[2] + (double)(c2 + r + 1) * g.h) - (double)t2); }
-    if (!open) break;
-    lb -= g.slack;
-    if (lb > 0.0 && best < lb * lb * (1.0 - 1e-12)) break;
-  }
-  return bpos;
-}
-
-// ------------------------------------------------------------------------------------
 // Implicit BVH over the points in Morton order (exact fallback of the NN query).
 // Level 0 nodes bound 8 consecutive Morton-ordered points, level k nodes bound 8 level k-1
 // nodes. The query is the oracle's kd-tree query: best-first descent, a node is skipped when no
@@ -812,13 +725,12 @@ __device__ __forceinline__ double node_mindist2(const float4 l, const float4 h, 
 
 // Nearest point strictly inside the box, ties -> lowest original index; returns the sorted
 // position or -1. Best-first with an explicit stack (level in the top 4 bits).
+// (best, bpos, borig) may carry a known candidate (a point strictly inside the box) as the start.
 __device__ int bvh_query(const float4* __restrict__ pts, const float4* __restrict__ mpts,
                          const float4* __restrict__ nlo, const float4* __restrict__ nhi, const BvhInfo& bv,
                          float t0, float t1, float t2, float lo0, float lo1, float lo2, float hi0, float hi1,
-                         float hi2) {
-  if (bv.levels <= 0) return -1;
-  double best = INFINITY;
-  int bpos = -1, borig = 0x7fffffff;
+                         float hi2, double best = INFINITY, int bpos = -1, int borig = 0x7fffffff) {
+  if (bv.levels <= 0) return bpos;
   uint32_t stack[8 * kBvhMaxLevels + 8];
   int sp = 0;
   const int top = bv.levels - 1;
@@ -916,15 +828,12 @@ __global__ void __launch_bounds__(256) k_nn(const float4* __restrict__ pts, cons
     if (!(hi0 > g.bmin[0] && lo0 < g.bmax[0] && hi1 > g.bmin[1] && lo1 < g.bmax[1] && hi2 > g.bmin[2] &&
           lo2 < g.bmax[2]))
       r = -1;  // no data point can be strictly inside the box
-    else if (bv.levels > 0)
-      r = bvh_query(pts, mpts, nlo, nhi, bv, (float)p0, (float)p1, (float)p2, lo0, lo1, lo2, hi0, hi1, hi2);
-    else
-      r = nn_query(pts, cstart, g, p0, p1, p2, md);
+    // else: left as -2 for k_nn_rays (exact BVH query warm-started along the ray)
   }
   const size_t S = (size_t)nx * ny * nz;
   const size_t sidx = (size_t)k * ((size_t)nx * ny) + (size_t)i * nx + j;
-  if (idx) idx[sidx] = r;
-  if (RS4 > 0) {
+  idx[sidx] = r;
+  if (RS4 > 0 && r != -2) {
     float f[4 * (RS4 > 0 ? RS4 : 1)];
 #pragma unroll
     for (int q = 0; q < 4 * (RS4 > 0 ? RS4 : 1); ++q) f[q] = 0.f;
@@ -946,6 +855,78 @@ __global__ void __launch_bounds__(256) k_nn(const float4* __restrict__ pts, cons
 #pragma unroll
     for (int l = 0; l < (4 * RS4 - 3) / 2; ++l)
       if (l < nlines) sem[(size_t)l * S + sidx] = make_float2(f[3 + 2 * l], f[4 + 2 * l]);
+  }
+}
+
+// Second NN pass, one thread per ray in LOS order: samples k_nn left unresolved (-2) get the
+// exact BVH query, started from the previous sample's point when it lies inside the box (a
+// valid upper bound, so the query only visits nodes near the answer). Same outputs as k_nn.
+template <int RS4>
+__global__ void __launch_bounds__(256) k_nn_rays(const float4* __restrict__ pts, const float4* __restrict__ mpts,
+                                                 const float4* __restrict__ nlo, const float4* __restrict__ nhi,
+                                                 BvhInfo bv, const ViewGeom* __restrict__ vgp, int nx, int ny,
+                                                 int nz, int32_t* __restrict__ idx, const float4* __restrict__ rec,
+                                                 int nlines, float* __restrict__ svlos, float2* __restrict__ sem) {
+  const int64_t R = (int64_t)nx * ny;
+  const int64_t ray = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (ray >= R) return;
+  const int j = (int)(ray % nx), i = (int)(ray / nx);
+  const ViewGeom vg = *vgp;
+  const double minx = vg.ext[0], maxx = vg.ext[1], miny = vg.ext[2], maxy = vg.ext[3], minz = vg.ext[4];
+  const double xx = double(j) / (nx - 1) * (maxx - minx) + minx;
+  const double yy = double(i) / (ny - 1) * (maxy - miny) + miny;
+  const double cb = vg.cb, sb = vg.sb, cl = vg.cl, sl = vg.sl, md = vg.md;
+  const size_t S = (size_t)R * nz;
+  int prev = -1;
+  for (int k = 0; k < nz; ++k) {
+    const size_t sidx = (size_t)k * R + ray;
+    int r = idx[sidx];
+    if (r != -2) {
+      if (r >= 0) prev = r;
+      continue;
+    }
+    const double zz = double(k) * vg.deltaz + minz;
+    const double p0 = xx * cb * cl + yy * sl + zz * sb * cl;
+    const double p1 = -xx * cb * sl + yy * cl - zz * sb * sl;
+    const double p2 = -xx * sb + zz * cb;
+    const float t0 = (float)p0, t1 = (float)p1, t2 = (float)p2;
+    const float lo0 = (float)(p0 - md), lo1 = (float)(p1 - md), lo2 = (float)(p2 - md);
+    const float hi0 = (float)(p0 + md), hi1 = (float)(p1 + md), hi2 = (float)(p2 + md);
+    double best = INFINITY;
+    int bpos = -1, borig = 0x7fffffff;
+    if (prev >= 0) {
+      const float4 q = pts[prev];
+      if (lo0 < q.x && q.x < hi0 && lo1 < q.y && q.y < hi1 && lo2 < q.z && q.z < hi2) {
+        const double d0 = (double)q.x - (double)t0, d1 = (double)q.y - (double)t1, d2 = (double)q.z - (double)t2;
+        best = (d0 * d0 + d1 * d1) + d2 * d2;
+        bpos = prev;
+        borig = __float_as_int(q.w);
+      }
+    }
+    r = bvh_query(pts, mpts, nlo, nhi, bv, t0, t1, t2, lo0, lo1, lo2, hi0, hi1, hi2, best, bpos, borig);
+    idx[sidx] = r;
+    if (r >= 0) prev = r;
+    if (RS4 > 0) {
+      float f[4 * (RS4 > 0 ? RS4 : 1)];
+#pragma unroll
+      for (int q = 0; q < 4 * (RS4 > 0 ? RS4 : 1); ++q) f[q] = 0.f;
+      if (r >= 0) {
+        const float4* rp = rec + (size_t)r * RS4;
+#pragma unroll
+        for (int q = 0; q < RS4; ++q) {
+          const float4 a = rp[q];
+          f[4 * q] = a.x;
+          f[4 * q + 1] = a.y;
+          f[4 * q + 2] = a.z;
+          f[4 * q + 3] = a.w;
+        }
+      }
+      svlos[sidx] =
+          r >= 0 ? (float)(((0.0 + (sb * cl) * (double)f[0]) + (-sb * sl) * (double)f[1]) + cb * (double)f[2]) : 0.f;
+#pragma unroll
+      for (int l = 0; l < (4 * RS4 - 3) / 2; ++l)
+        if (l < nlines) sem[(size_t)l * S + sidx] = make_float2(f[3 + 2 * l], f[4 + 2 * l]);
+    }
   }
 }
 
@@ -1203,6 +1184,7 @@ struct Batch {
   int nx = 0, ny = 0, nz = 0, nl = 0, dn = 0;
   double width = 0;
   DevBuf partial, geom, aux, out, flag;
+  DevBuf idx;  // per-sample sorted positions (double-buffered)
   DevBuf rec;  // per-point records {vel, all lines}
   DevBuf sem;  // per-sample (peak, fwhm), nlines x S
   int rs4 = 0;
@@ -1588,22 +1570,38 @@ int rec_width(int nlines) {
   return f4 <= 2 ? 2 : (f4 <= 4 ? 4 : 8);
 }
 
-// NN query for every sample of a view. With rec: per-sample vlos and emissivities (nlines) too.
+// NN query for every sample of a view (idx: S ints, always written). With rec: per-sample vlos
+// and emissivities (nlines) too. Two passes: k_nn (lookup grid) and k_nn_rays (exact BVH for the rest).
 void enqueue_nn(hipStream_t s, const Dataset& ds, const ViewGeom* geom, int nx, int ny, int nz, int32_t* idx,
                 const float4* rec = nullptr, int rs4 = 0, int nlines = 0, float* svlos = nullptr,
                 float2* sem = nullptr) {
   const int tiles = ((nx + 15) / 16) * ((ny + 15) / 16);
   dim3 grid(tiles, nz);
-  ProfScope ps("k_nn");
+  const int64_t R = (int64_t)nx * ny;
+  {
+    ProfScope ps("k_nn");
 #define FOMO_NN_ARGS                                                                                           \
   ds.pts.as<float4>(), ds.cstart.as<int32_t>(), ds.grid, ds.lcell.as<int2>(), ds.lcand.as<float4>(), ds.lk, \
       ds.mpts.as<float4>(), ds.nlo.as<float4>(), ds.nhi.as<float4>(), ds.bvh,                                 \
       geom, nx, ny, nz, idx, rec, nlines, svlos, sem
-  if (!rec) k_nn<0><<<grid, 256, 0, s>>>(FOMO_NN_ARGS);
-  else if (rs4 == 2) k_nn<2><<<grid, 256, 0, s>>>(FOMO_NN_ARGS);
-  else if (rs4 == 4) k_nn<4><<<grid, 256, 0, s>>>(FOMO_NN_ARGS);
-  else k_nn<8><<<grid, 256, 0, s>>>(FOMO_NN_ARGS);
+    if (!rec) k_nn<0><<<grid, 256, 0, s>>>(FOMO_NN_ARGS);
+    else if (rs4 == 2) k_nn<2><<<grid, 256, 0, s>>>(FOMO_NN_ARGS);
+    else if (rs4 == 4) k_nn<4><<<grid, 256, 0, s>>>(FOMO_NN_ARGS);
+    else k_nn<8><<<grid, 256, 0, s>>>(FOMO_NN_ARGS);
 #undef FOMO_NN_ARGS
+  }
+  {
+    ProfScope ps("k_nn_rays");
+#define FOMO_NNR_ARGS                                                                                          \
+  ds.pts.as<float4>(), ds.mpts.as<float4>(), ds.nlo.as<float4>(), ds.nhi.as<float4>(), ds.bvh, geom, nx, ny, \
+      nz, idx, rec, nlines, svlos, sem
+    const unsigned nb = (unsigned)((R + 255) / 256);
+    if (!rec) k_nn_rays<0><<<nb, 256, 0, s>>>(FOMO_NNR_ARGS);
+    else if (rs4 == 2) k_nn_rays<2><<<nb, 256, 0, s>>>(FOMO_NNR_ARGS);
+    else if (rs4 == 4) k_nn_rays<4><<<nb, 256, 0, s>>>(FOMO_NNR_ARGS);
+    else k_nn_rays<8><<<nb, 256, 0, s>>>(FOMO_NNR_ARGS);
+#undef FOMO_NNR_ARGS
+  }
 }
 
 // Build records {vel, lines} for a set of emission cubes of one dataset.
@@ -1977,7 +1975,7 @@ static int render_one(int goft, int mode, double l, double b, int nx, int ny, in
   enqueue_view_geom(s, *ds, mode, l, b, nx, ny, nz, dn, vs.aux.as<float>(), vs.partial.as<double>(), geom);
   if (mode == MODE_NN) {
     // aux: per-sample LOS velocity; sem: per-sample (peak, fwhm)
-    enqueue_nn(s, *ds, geom, nx, ny, nz, nullptr, gc->rec.as<float4>(), gc->rs4, 1, vs.aux.as<float>(),
+    enqueue_nn(s, *ds, geom, nx, ny, nz, vs.idx.as<int32_t>(), gc->rec.as<float4>(), gc->rs4, 1, vs.aux.as<float>(),
                vs.sem.as<float2>());
     LineSet ls = make_lineset({vs.sem.as<float2>()}, {vs.out.as<float>()}, {gc->lambda0}, width);
     enqueue_los(s, (int)R, nz, ls, vs.aux.as<float>(), geom, nl, vs.flag.as<int32_t>());
@@ -2081,7 +2079,8 @@ static int enqueue_batch(Batch& bt, hipStream_t s, hipStream_t s2) {
     ViewGeom* geom = bt.geom.as<ViewGeom>() + v;
     if (s2 && v >= 2) HIPOK(hipStreamWaitEvent(s, bt.ev[nv + v - 2], 0));
     enqueue_view_geom(s, *ds, MODE_NN, bt.l[v], bt.b[v], bt.nx, bt.ny, bt.nz, bt.dn, nullptr, part, geom);
-    enqueue_nn(s, *ds, geom, bt.nx, bt.ny, bt.nz, nullptr, bt.rec.as<float4>(), bt.rs4, L, svl, sem);
+    enqueue_nn(s, *ds, geom, bt.nx, bt.ny, bt.nz, bt.idx.as<int32_t>() + buf * S, bt.rec.as<float4>(), bt.rs4, L, svl,
+               sem);
     std::vector<const float2*> ems;
     std::vector<float*> outs;
     std::vector<double> l0s;
@@ -2146,6 +2145,7 @@ int fomo_batch_create(int ngoft, const int* gofts, int nviews, const double* l, 
   const int nlo = std::max(nl, 1);
   if (bt->partial.alloc(sizeof(double) * 6 * kMaxPartials * nviews) || bt->geom.alloc(sizeof(ViewGeom) * nviews) ||
       bt->sem.alloc(sizeof(float2) * S * ngoft * 2) || bt->aux.alloc(sizeof(float) * S * 2) ||
+      bt->idx.alloc(sizeof(int32_t) * S * 2) ||
       bt->flag.alloc(sizeof(int32_t) * R * kMaxLines * 2) ||
       bt->out.alloc(sizeof(float) * (size_t)R * nlo * ngoft * nviews))
     return fail(FOMO_ERR_NOMEM, "device allocation failed for batch");
