@@ -116,7 +116,7 @@ def main():
     F.profile_reset()
     bt.profile()
     bt.profile()
-    stats = {k: F.profile_stats(k) for k in ("k_view_rotate_nn", "k_nn", "k_los_spectral")}
+    stats = {k: F.profile_stats(k) for k in ("k_view_rotate_nn", "k_nn", "k_nn_rays", "k_los_spectral")}
     log("per-kernel stats (count, total ms): " + json.dumps(stats))
     bytes_per = {
         # algorithmic bytes per launch
@@ -239,7 +239,7 @@ def thomson(F, args):
     F.profile_enable(True)
     F.render_thomson(em, ls[1], b, nx, ny, nz)
     F.profile_enable(False)
-    st = {k: F.profile_stats(k) for k in ("k_view_rotate_thomson", "k_nn", "k_los_thomson")}
+    st = {k: F.profile_stats(k) for k in ("k_view_rotate_thomson", "k_nn", "k_nn_rays", "k_los_thomson")}
     log("per-kernel (count, ms): " + json.dumps(st))
     out = {"metric": "LOS samples per second, Thomson pB render (16 views)", "value": W / dt,
            "unit": "LOS samples/s", "n_gpus": 1, "steps": nv, "warmup": args.warmup, "ms_per_step": dt / nv * 1e3,

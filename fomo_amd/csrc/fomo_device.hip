@@ -90,6 +90,12 @@ struct DevBuf {
     if (b <= bytes && p) return hipSuccess;
     return alloc(b);
   }
+  // grow-only reuse for buffers that must read zero whenever they are idle
+  hipError_t ensure_zeroed(size_t b, hipStream_t s) {
+    if (b <= bytes && p) return hipSuccess;
+    hipError_t e = alloc(b);
+    return e == hipSuccess ? hipMemsetAsync(p, 0, bytes, s) : e;
+  }
   template <class T> T* as() const { return reinterpret_cast<T*>(p); }
 };
 
@@ -799,7 +805,8 @@ __global__ void __launch_bounds__(256) k_nn(const float4* __restrict__ pts, cons
                                             const float4* __restrict__ nhi, BvhInfo bv,
                                             const ViewGeom* __restrict__ vgp, int nx, int ny, int nz,
                                             int32_t* __restrict__ idx, const float4* __restrict__ rec, int nlines,
-                                            float* __restrict__ svlos, float2* __restrict__ sem) {
+                                            float* __restrict__ svlos, float2* __restrict__ sem,
+                                            uint8_t* __restrict__ rayflag) {
   const int tilesx = (nx + 15) >> 4;
   const int tx = blockIdx.x % tilesx, ty = blockIdx.x / tilesx;
   const int j = tx * 16 + (threadIdx.x & 15);
@@ -828,7 +835,8 @@ __global__ void __launch_bounds__(256) k_nn(const float4* __restrict__ pts, cons
     if (!(hi0 > g.bmin[0] && lo0 < g.bmax[0] && hi1 > g.bmin[1] && lo1 < g.bmax[1] && hi2 > g.bmin[2] &&
           lo2 < g.bmax[2]))
       r = -1;  // no data point can be strictly inside the box
-    // else: left as -2 for k_nn_rays (exact BVH query warm-started along the ray)
+    else
+      rayflag[(size_t)i * nx + j] = 1;  // left as -2 for k_nn_rays (exact BVH, warm-started along the ray)
   }
   const size_t S = (size_t)nx * ny * nz;
   const size_t sidx = (size_t)k * ((size_t)nx * ny) + (size_t)i * nx + j;
@@ -866,10 +874,14 @@ __global__ void __launch_bounds__(256) k_nn_rays(const float4* __restrict__ pts,
                                                  const float4* __restrict__ nlo, const float4* __restrict__ nhi,
                                                  BvhInfo bv, const ViewGeom* __restrict__ vgp, int nx, int ny,
                                                  int nz, int32_t* __restrict__ idx, const float4* __restrict__ rec,
-                                                 int nlines, float* __restrict__ svlos, float2* __restrict__ sem) {
+                                                 int nlines, float* __restrict__ svlos, float2* __restrict__ sem,
+                                                 uint8_t* __restrict__ rayflag) {
   const int64_t R = (int64_t)nx * ny;
   const int64_t ray = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (ray >= R) return;
+  // only rays with a sample left unresolved by k_nn; the flag is cleared for the next view
+  if (!rayflag[ray]) return;
+  rayflag[ray] = 0;
   const int j = (int)(ray % nx), i = (int)(ray / nx);
   const ViewGeom vg = *vgp;
   const double minx = vg.ext[0], maxx = vg.ext[1], miny = vg.ext[2], maxy = vg.ext[3], minz = vg.ext[4];
@@ -1175,6 +1187,7 @@ struct ViewScratch {
   DevBuf out;      // float R*nl
   DevBuf flag;     // int32 R (rays needing the direct-exp pass)
   DevBuf sem;      // float2 S: per-sample (peak, fwhm)
+  DevBuf rayflag;  // uint8 R: rays with samples for the BVH pass (kept all-zero between uses)
 };
 
 struct Batch {
@@ -1184,7 +1197,8 @@ struct Batch {
   int nx = 0, ny = 0, nz = 0, nl = 0, dn = 0;
   double width = 0;
   DevBuf partial, geom, aux, out, flag;
-  DevBuf idx;  // per-sample sorted positions (double-buffered)
+  DevBuf idx;      // per-sample sorted positions (double-buffered)
+  DevBuf rayflag;  // uint8 R per buffer, all-zero between uses
   DevBuf rec;  // per-point records {vel, all lines}
   DevBuf sem;  // per-sample (peak, fwhm), nlines x S
   int rs4 = 0;
@@ -1572,8 +1586,9 @@ int rec_width(int nlines) {
 
 // NN query for every sample of a view (idx: S ints, always written). With rec: per-sample vlos
 // and emissivities (nlines) too. Two passes: k_nn (lookup grid) and k_nn_rays (exact BVH for the rest).
+// rayflag: R bytes, zero on entry and on exit (set by k_nn, cleared by k_nn_rays).
 void enqueue_nn(hipStream_t s, const Dataset& ds, const ViewGeom* geom, int nx, int ny, int nz, int32_t* idx,
-                const float4* rec = nullptr, int rs4 = 0, int nlines = 0, float* svlos = nullptr,
+                uint8_t* rayflag, const float4* rec = nullptr, int rs4 = 0, int nlines = 0, float* svlos = nullptr,
                 float2* sem = nullptr) {
   const int tiles = ((nx + 15) / 16) * ((ny + 15) / 16);
   dim3 grid(tiles, nz);
@@ -1583,7 +1598,7 @@ void enqueue_nn(hipStream_t s, const Dataset& ds, const ViewGeom* geom, int nx, 
 #define FOMO_NN_ARGS                                                                                           \
   ds.pts.as<float4>(), ds.cstart.as<int32_t>(), ds.grid, ds.lcell.as<int2>(), ds.lcand.as<float4>(), ds.lk, \
       ds.mpts.as<float4>(), ds.nlo.as<float4>(), ds.nhi.as<float4>(), ds.bvh,                                 \
-      geom, nx, ny, nz, idx, rec, nlines, svlos, sem
+      geom, nx, ny, nz, idx, rec, nlines, svlos, sem, rayflag
     if (!rec) k_nn<0><<<grid, 256, 0, s>>>(FOMO_NN_ARGS);
     else if (rs4 == 2) k_nn<2><<<grid, 256, 0, s>>>(FOMO_NN_ARGS);
     else if (rs4 == 4) k_nn<4><<<grid, 256, 0, s>>>(FOMO_NN_ARGS);
@@ -1594,7 +1609,7 @@ void enqueue_nn(hipStream_t s, const Dataset& ds, const ViewGeom* geom, int nx, 
     ProfScope ps("k_nn_rays");
 #define FOMO_NNR_ARGS                                                                                          \
   ds.pts.as<float4>(), ds.mpts.as<float4>(), ds.nlo.as<float4>(), ds.nhi.as<float4>(), ds.bvh, geom, nx, ny, \
-      nz, idx, rec, nlines, svlos, sem
+      nz, idx, rec, nlines, svlos, sem, rayflag
     const unsigned nb = (unsigned)((R + 255) / 256);
     if (!rec) k_nn_rays<0><<<nb, 256, 0, s>>>(FOMO_NNR_ARGS);
     else if (rs4 == 2) k_nn_rays<2><<<nb, 256, 0, s>>>(FOMO_NNR_ARGS);
@@ -1724,6 +1739,7 @@ int fomo_shutdown(void) {
   g_ctx.vs.out.release();
   g_ctx.vs.flag.release();
   g_ctx.vs.sem.release();
+  g_ctx.vs.rayflag.release();
   for (auto& b : g_ctx.sortkeys) b.release();
   g_ctx.sortvals.release();
   g_ctx.sorttmp.release();
@@ -1970,17 +1986,18 @@ static int render_one(int goft, int mode, double l, double b, int nx, int ny, in
   HIPOK(vs.aux.ensure(sizeof(float) * std::max<int64_t>(ds->n, S)));
   HIPOK(vs.out.ensure(sizeof(float) * R * nlo));
   HIPOK(vs.flag.ensure(sizeof(int32_t) * R));
+  HIPOK(vs.rayflag.ensure_zeroed(R, s));
   if (mode == MODE_NN) HIPOK(vs.sem.ensure(sizeof(float2) * S));
   ViewGeom* geom = vs.geom.as<ViewGeom>();
   enqueue_view_geom(s, *ds, mode, l, b, nx, ny, nz, dn, vs.aux.as<float>(), vs.partial.as<double>(), geom);
   if (mode == MODE_NN) {
     // aux: per-sample LOS velocity; sem: per-sample (peak, fwhm)
-    enqueue_nn(s, *ds, geom, nx, ny, nz, vs.idx.as<int32_t>(), gc->rec.as<float4>(), gc->rs4, 1, vs.aux.as<float>(),
-               vs.sem.as<float2>());
+    enqueue_nn(s, *ds, geom, nx, ny, nz, vs.idx.as<int32_t>(), vs.rayflag.as<uint8_t>(), gc->rec.as<float4>(), gc->rs4,
+               1, vs.aux.as<float>(), vs.sem.as<float2>());
     LineSet ls = make_lineset({vs.sem.as<float2>()}, {vs.out.as<float>()}, {gc->lambda0}, width);
     enqueue_los(s, (int)R, nz, ls, vs.aux.as<float>(), geom, nl, vs.flag.as<int32_t>());
   } else {
-    enqueue_nn(s, *ds, geom, nx, ny, nz, vs.idx.as<int32_t>());
+    enqueue_nn(s, *ds, geom, nx, ny, nz, vs.idx.as<int32_t>(), vs.rayflag.as<uint8_t>());
     ProfScope ps("k_los_thomson");
     k_los_thomson<<<nblk(R), 256, 0, s>>>(vs.idx.as<int32_t>(), (int)R, nz, gc->em.as<float2>(), vs.aux.as<float>(),
                                           geom, vs.out.as<float>());
@@ -2027,9 +2044,10 @@ int fomo_nn_indices(int dataset, double l, double b, int nx, int ny, int nz, int
   HIPOK(vs.geom.ensure(sizeof(ViewGeom)));
   HIPOK(vs.idx.ensure(sizeof(int32_t) * S));
   HIPOK(vs.aux.ensure(sizeof(float) * std::max<int64_t>(ds->n, S)));
+  HIPOK(vs.rayflag.ensure_zeroed(R, s));
   ViewGeom* geom = vs.geom.as<ViewGeom>();
   enqueue_view_geom(s, *ds, mode, l, b, nx, ny, nz, 0, vs.aux.as<float>(), vs.partial.as<double>(), geom);
-  enqueue_nn(s, *ds, geom, nx, ny, nz, vs.idx.as<int32_t>());
+  enqueue_nn(s, *ds, geom, nx, ny, nz, vs.idx.as<int32_t>(), vs.rayflag.as<uint8_t>());
   int32_t* tmp = reinterpret_cast<int32_t*>(vs.aux.p);
   k_idx_to_orig<<<nblk(S), kBlock, 0, s>>>(vs.idx.as<int32_t>(), ds->perm.as<int32_t>(), R, nz, tmp);
   HIPOK(hipGetLastError());
@@ -2079,8 +2097,8 @@ static int enqueue_batch(Batch& bt, hipStream_t s, hipStream_t s2) {
     ViewGeom* geom = bt.geom.as<ViewGeom>() + v;
     if (s2 && v >= 2) HIPOK(hipStreamWaitEvent(s, bt.ev[nv + v - 2], 0));
     enqueue_view_geom(s, *ds, MODE_NN, bt.l[v], bt.b[v], bt.nx, bt.ny, bt.nz, bt.dn, nullptr, part, geom);
-    enqueue_nn(s, *ds, geom, bt.nx, bt.ny, bt.nz, bt.idx.as<int32_t>() + buf * S, bt.rec.as<float4>(), bt.rs4, L, svl,
-               sem);
+    enqueue_nn(s, *ds, geom, bt.nx, bt.ny, bt.nz, bt.idx.as<int32_t>() + buf * S, bt.rayflag.as<uint8_t>() + buf * R,
+               bt.rec.as<float4>(), bt.rs4, L, svl, sem);
     std::vector<const float2*> ems;
     std::vector<float*> outs;
     std::vector<double> l0s;
@@ -2145,10 +2163,11 @@ int fomo_batch_create(int ngoft, const int* gofts, int nviews, const double* l, 
   const int nlo = std::max(nl, 1);
   if (bt->partial.alloc(sizeof(double) * 6 * kMaxPartials * nviews) || bt->geom.alloc(sizeof(ViewGeom) * nviews) ||
       bt->sem.alloc(sizeof(float2) * S * ngoft * 2) || bt->aux.alloc(sizeof(float) * S * 2) ||
-      bt->idx.alloc(sizeof(int32_t) * S * 2) ||
+      bt->idx.alloc(sizeof(int32_t) * S * 2) || bt->rayflag.alloc(R * 2) ||
       bt->flag.alloc(sizeof(int32_t) * R * kMaxLines * 2) ||
       bt->out.alloc(sizeof(float) * (size_t)R * nlo * ngoft * nviews))
     return fail(FOMO_ERR_NOMEM, "device allocation failed for batch");
+  HIPOK(hipMemsetAsync(bt->rayflag.p, 0, R * 2, g_ctx.stream));
   {
     std::vector<const float2*> ems;
     for (int gid : bt->gofts) ems.push_back(find_obj(g_ctx.gofts, gid)->em.as<float2>());
