@@ -11,7 +11,7 @@ import numpy as np
 
 __all__ = [
     "lib", "FomoError", "init", "shutdown", "device_info", "Dataset", "Table", "Emission", "Batch",
-    "render_nn", "render_thomson", "nn_indices", "gen_loop_cube", "gen_goft_table", "gen_corona",
+    "render_nn", "render_thomson", "nn_indices", "Server", "METHOD_NN", "METHOD_THOMSON", "gen_loop_cube", "gen_goft_table", "gen_corona",
     "read_table", "profile_enable", "profile_reset", "profile_stats", "LIB_PATH",
 ]
 
@@ -55,6 +55,18 @@ _SIGS = {
                                            ctypes.c_int, ctypes.c_int, _c_flt_p, _c_dbl_p]),
     "fomo_nn_indices": (ctypes.c_int, [ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_int,
                                        ctypes.c_int, ctypes.c_int, _c_i32_p, _c_dbl_p]),
+    "fomo_server_create": (ctypes.c_int, [ctypes.c_int64, ctypes.c_int64, ctypes.c_int, ctypes.c_int, _c_int_p]),
+    "fomo_server_destroy": (ctypes.c_int, [ctypes.c_int]),
+    "fomo_server_footprint": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                             ctypes.c_int, _c_i64_p, _c_i64_p]),
+    "fomo_server_submit": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double,
+                                          ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double,
+                                          ctypes.c_int, _c_flt_p, _c_dbl_p, _c_i64_p]),
+    "fomo_server_poll": (ctypes.c_int, [ctypes.c_int, ctypes.c_int64, _c_int_p]),
+    "fomo_server_wait": (ctypes.c_int, [ctypes.c_int, ctypes.c_int64, ctypes.c_double, _c_int_p]),
+    "fomo_server_wait_all": (ctypes.c_int, [ctypes.c_int, ctypes.c_double]),
+    "fomo_server_times": (ctypes.c_int, [ctypes.c_int, ctypes.c_int64, _c_dbl_p]),
+    "fomo_server_stats": (ctypes.c_int, [ctypes.c_int, _c_dbl_p]),
     "fomo_batch_create": (ctypes.c_int, [ctypes.c_int, _c_int_p, ctypes.c_int, _c_dbl_p, _c_dbl_p, ctypes.c_int,
                                          ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_int,
                                          _c_int_p]),
@@ -345,6 +357,74 @@ class Batch:
         if self.id is not None and _LIB is not None:
             lib().fomo_batch_destroy(self.id)
             self.id = None
+
+
+METHOD_NN = 0
+METHOD_THOMSON = 1
+
+
+class Server:
+    """Render server over resident emission cubes (continuous batching on several streams).
+
+    submit() queues one view and returns a ticket; result() waits for it and returns
+    (image, meta). Output arrays are allocated here and kept alive until the ticket finishes."""
+
+    def __init__(self, device_bytes=4 << 30, host_bytes=1 << 30, max_inflight=8, nstreams=4):
+        out = ctypes.c_int()
+        _check(lib().fomo_server_create(int(device_bytes), int(host_bytes), max_inflight, nstreams, ctypes.byref(out)))
+        self.id = out.value
+        self._bufs = {}
+
+    @staticmethod
+    def footprint(em, method, nx, ny, nz, nl=1):
+        d, h = ctypes.c_int64(), ctypes.c_int64()
+        _check(lib().fomo_server_footprint(em.id, method, nx, ny, nz, nl, ctypes.byref(d), ctypes.byref(h)))
+        return d.value, h.value
+
+    def submit(self, em, method, l, b, nx, ny, nz, nl=1, lambda_width=0.0, dn=False):
+        nlo = 1 if method == METHOD_THOMSON else max(nl, 1)
+        img = np.zeros(nx * ny * nlo, np.float32)
+        meta = np.zeros(10, np.float64)
+        t = ctypes.c_int64()
+        _check(lib().fomo_server_submit(self.id, em.id, method, l, b, nx, ny, nz, nl, lambda_width, 1 if dn else 0,
+                                        _ptr(img, ctypes.c_float), _ptr(meta, ctypes.c_double), ctypes.byref(t)))
+        self._bufs[t.value] = (img, meta)
+        return t.value
+
+    def poll(self, ticket):
+        st = ctypes.c_int()
+        rc = lib().fomo_server_poll(self.id, ticket, ctypes.byref(st))
+        if rc != 0 and st.value >= 0:
+            _check(rc)
+        return st.value
+
+    def result(self, ticket, timeout=-1.0):
+        """Wait for the ticket; returns (image, meta). Raises FomoError if it failed."""
+        st = ctypes.c_int()
+        _check(lib().fomo_server_wait(self.id, ticket, float(timeout), ctypes.byref(st)))
+        return self._bufs.pop(ticket)
+
+    def wait_all(self, timeout=-1.0):
+        _check(lib().fomo_server_wait_all(self.id, float(timeout)))
+
+    def times(self, ticket):
+        """(submitted, admitted, finished) in seconds since the server was created (-1: not yet)."""
+        t = np.zeros(3, np.float64)
+        _check(lib().fomo_server_times(self.id, ticket, _ptr(t, ctypes.c_double)))
+        return t
+
+    def stats(self):
+        st = np.zeros(12, np.float64)
+        _check(lib().fomo_server_stats(self.id, _ptr(st, ctypes.c_double)))
+        keys = ("submitted", "completed", "failed", "queued", "running", "peak_running", "device_in_use",
+                "device_peak", "device_budget", "host_in_use", "host_peak", "host_budget")
+        return dict(zip(keys, (int(v) for v in st)))
+
+    def close(self):
+        if self.id is not None and _LIB is not None:
+            lib().fomo_server_destroy(self.id)
+            self.id = None
+            self._bufs.clear()
 
 
 def gen_loop_cube(m, seed=1, t_in=1.0e6, t_out=2.0e6, noise_sigma=0.05, vamp=2.0e4, jitter=0.0, sausage=0.0):

@@ -20,17 +20,23 @@
 #include <rocprim/device/device_scan.hpp>
 
 #include <algorithm>
+#include <atomic>
+#include <chrono>
 #include <cmath>
+#include <condition_variable>
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
+#include <deque>
 #include <map>
 #include <memory>
 #include <mutex>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/fomo_hip.h"
+#include "arena.h"
 
 // ------------------------------------------------------------------------------------
 // constants (GSL CODATA-2006 values used by fomo-CHIANTI.cpp:17-19)
@@ -1163,6 +1169,7 @@ struct Dataset {
   DevBuf mpts;         // float4 (x, y, z, sorted position) in Morton order
   DevBuf nlo, nhi;     // node boxes, all levels
   int64_t ncand = 0, nlcells = 0, nfallback = 0;
+  std::atomic<int> inuse{0};  // batches and queued/running server requests reading this dataset
 };
 
 struct Table {
@@ -1177,6 +1184,7 @@ struct Goft {
   DevBuf em;     // float2 per sorted point
   DevBuf rec;    // chianti: per-point records {vx,vy,vz,peak,fwhm} for k_nn
   int rs4 = 0;
+  std::atomic<int> inuse{0};  // queued/running server requests reading this cube
 };
 
 struct ViewScratch {
@@ -1217,6 +1225,8 @@ struct ProfStat {
   double ms = 0;
 };
 
+struct Server;
+
 struct Ctx {
   bool init = false;
   int device = -1;
@@ -1228,6 +1238,7 @@ struct Ctx {
   std::map<int, std::unique_ptr<Table>> tables;
   std::map<int, std::unique_ptr<Goft>> gofts;
   std::map<int, std::unique_ptr<Batch>> batches;
+  std::map<int, std::shared_ptr<Server>> servers;
   ViewScratch vs;
   DevBuf sortkeys[2], sortvals, sorttmp, tmpf;
   bool profile = false;
@@ -1255,19 +1266,22 @@ hipEvent_t pool_event() {
 }
 
 // Event pair around an eager launch when profiling is on.
+// Events around one launch, recorded on the launch's stream. Only work on the library's main
+// stream is timed (server streams run concurrently, so their durations would overlap).
 struct ProfScope {
   const char* name;
+  hipStream_t s;
   hipEvent_t a = nullptr, b = nullptr;
-  explicit ProfScope(const char* n) : name(n) {
-    if (g_ctx.profile) {
+  ProfScope(const char* n, hipStream_t st) : name(n), s(st) {
+    if (g_ctx.profile && s == g_ctx.stream) {
       a = pool_event();
       b = pool_event();
-      (void)hipEventRecord(a, g_ctx.stream);
+      (void)hipEventRecord(a, s);
     }
   }
   ~ProfScope() {
     if (a) {
-      (void)hipEventRecord(b, g_ctx.stream);
+      (void)hipEventRecord(b, s);
       g_ctx.pending.push_back({name, {a, b}});
     }
   }
@@ -1398,7 +1412,7 @@ int build_lookup(Dataset& ds) {
   if (cnt.alloc(sizeof(int32_t) * ncl) || pos.alloc(sizeof(int32_t) * ncl) || off.alloc(sizeof(int64_t) * ncl))
     return FOMO_OK;  // not enough memory: ring search only
   {
-    ProfScope ps("k_lookup_build");
+    ProfScope ps("k_lookup_build", s);
     k_lookup_build<false><<<nblk(ncl), kBlock, 0, s>>>(ds.pts.as<float4>(), ds.cstart.as<int32_t>(), g, lk, ncl,
                                                         near1.as<uint8_t>(), cnt.as<int32_t>(), nullptr, nullptr);
   }
@@ -1452,7 +1466,7 @@ int build_index(Dataset& ds, double cell_scale) {
   HIPOK(g_ctx.vs.partial.ensure(sizeof(double) * 6 * (kMaxPartials + 1)));
   double* part = g_ctx.vs.partial.as<double>();
   {
-    ProfScope ps("k_bbox");
+    ProfScope ps("k_bbox", s);
     k_bbox<<<nb, kBlock, 0, s>>>(ds.raw[0].as<float>(), ds.raw[1].as<float>(), ds.raw[2].as<float>(), n, part);
   }
   k_reduce_partials<<<1, kBlock, 0, s>>>(part, nb, part + 6 * kMaxPartials);
@@ -1499,7 +1513,7 @@ int build_index(Dataset& ds, double cell_scale) {
   HIPOK(g_ctx.sortvals.ensure(sizeof(int32_t) * n));
   HIPOK(ds.perm.ensure(sizeof(int32_t) * n));
   {
-    ProfScope ps("k_cell_keys");
+    ProfScope ps("k_cell_keys", s);
     k_cell_keys<<<nblk(n), kBlock, 0, s>>>(ds.raw[0].as<float>(), ds.raw[1].as<float>(), ds.raw[2].as<float>(), n, g,
                                            g_ctx.sortkeys[0].as<uint32_t>(), g_ctx.sortvals.as<int32_t>());
   }
@@ -1511,14 +1525,14 @@ int build_index(Dataset& ds, double cell_scale) {
                                   g_ctx.sortvals.as<int32_t>(), ds.perm.as<int32_t>(), (size_t)n, 0, endbit, s));
   HIPOK(g_ctx.sorttmp.ensure(tmpb));
   {
-    ProfScope ps("radix_sort");
+    ProfScope ps("radix_sort", s);
     HIPOK(rocprim::radix_sort_pairs(g_ctx.sorttmp.p, tmpb, g_ctx.sortkeys[0].as<uint32_t>(),
                                     g_ctx.sortkeys[1].as<uint32_t>(), g_ctx.sortvals.as<int32_t>(),
                                     ds.perm.as<int32_t>(), (size_t)n, 0, endbit, s));
   }
   HIPOK(ds.cstart.ensure(sizeof(int32_t) * (ds.ncells + 1)));
   {
-    ProfScope ps("k_cell_start");
+    ProfScope ps("k_cell_start", s);
     k_cell_start<<<nblk(n + 1), kBlock, 0, s>>>(g_ctx.sortkeys[1].as<uint32_t>(), n, ds.ncells, ds.cstart.as<int32_t>());
   }
   HIPOK(ds.pts.ensure(sizeof(float4) * n));
@@ -1562,7 +1576,7 @@ void enqueue_view_geom(hipStream_t s, const Dataset& ds, int mode, double l, dou
   const unsigned nb = std::min<unsigned>(nblk(ds.n), kMaxPartials);
   const double cb = cos(b), sb = sin(b), cl = cos(l), sl = sin(l);
   if (mode == MODE_NN) {
-    ProfScope ps("k_view_rotate_nn");
+    ProfScope ps("k_view_rotate_nn", s);
     (void)aux;
     k_view_rotate_nn<<<nb, kBlock, 0, s>>>(ds.pts.as<float4>(), nullptr, ds.n, cb, sb, cl, sl, nullptr, partial);
   } else {
@@ -1571,7 +1585,7 @@ void enqueue_view_geom(hipStream_t s, const Dataset& ds, int mode, double l, dou
     if (std::abs(cbc) < eps) cbc = eps;
     if (std::abs(sbc) < eps) sbc = eps;
     const double blend = 1.0 / (1.0 + exp(-20.0 * (std::fabs(b) - 1.309)));  // :173-176
-    ProfScope ps("k_view_rotate_thomson");
+    ProfScope ps("k_view_rotate_thomson", s);
     k_view_rotate_thomson<<<nb, kBlock, 0, s>>>(ds.pts.as<float4>(), ds.n, cbc, sbc, cl, sl, blend, aux, partial);
   }
   const double bf = 1.0 / (1.0 + exp(-10 * (std::fabs(b) - 0.1745)));  // fomo-thomson.cpp:249
@@ -1594,7 +1608,7 @@ void enqueue_nn(hipStream_t s, const Dataset& ds, const ViewGeom* geom, int nx, 
   dim3 grid(tiles, nz);
   const int64_t R = (int64_t)nx * ny;
   {
-    ProfScope ps("k_nn");
+    ProfScope ps("k_nn", s);
 #define FOMO_NN_ARGS                                                                                           \
   ds.pts.as<float4>(), ds.cstart.as<int32_t>(), ds.grid, ds.lcell.as<int2>(), ds.lcand.as<float4>(), ds.lk, \
       ds.mpts.as<float4>(), ds.nlo.as<float4>(), ds.nhi.as<float4>(), ds.bvh,                                 \
@@ -1606,7 +1620,7 @@ void enqueue_nn(hipStream_t s, const Dataset& ds, const ViewGeom* geom, int nx, 
 #undef FOMO_NN_ARGS
   }
   {
-    ProfScope ps("k_nn_rays");
+    ProfScope ps("k_nn_rays", s);
 #define FOMO_NNR_ARGS                                                                                          \
   ds.pts.as<float4>(), ds.mpts.as<float4>(), ds.nlo.as<float4>(), ds.nhi.as<float4>(), ds.bvh, geom, nx, ny, \
       nz, idx, rec, nlines, svlos, sem, rayflag
@@ -1654,13 +1668,13 @@ void launch_spectral_t(hipStream_t s, int R, int nz, const LineSet& ls, const fl
 void enqueue_los(hipStream_t s, int R, int nz, const LineSet& ls, const float* vlos, const ViewGeom* geom, int nl,
                  int32_t* flag) {
   if (nl <= 1) {
-    ProfScope ps("k_los_imaging");
+    ProfScope ps("k_los_imaging", s);
     k_los_imaging<<<dim3(nblk(R), ls.n), 256, 0, s>>>(R, nz, ls, geom);
     return;
   }
   int npass = 1;
   const int ch = pick_chunk(nl, &npass);
-  ProfScope ps("k_los_spectral");
+  ProfScope ps("k_los_spectral", s);
   for (int pass = 0; pass < npass; ++pass) {
     const int l0 = pass * ch;
     switch (ch) {
@@ -1695,6 +1709,332 @@ int check_res(int nx, int ny, int nz, int nl) {
   return FOMO_OK;
 }
 
+// Device buffers of one view render (plain render or server request), in render_sizes order.
+struct RenderBufs {
+  double* partial;
+  ViewGeom* geom;
+  int32_t* idx;      // S
+  float* aux;        // NN: per-sample LOS velocity (S); Thomson: per-point sin^2 chi (N)
+  float* out;        // R * max(nl,1)
+  int32_t* flag;     // NN: R rays needing the direct-exp pass
+  float2* sem;       // NN: per-sample (peak, fwhm), S
+  uint8_t* rayflag;  // R, zero on entry (left zero)
+};
+constexpr int kRenderBufs = 8;
+
+void render_sizes(int mode, int64_t n, int64_t R, int64_t S, int nl, int64_t sz[kRenderBufs]) {
+  sz[0] = (int64_t)sizeof(double) * 6 * (kMaxPartials + 1);
+  sz[1] = (int64_t)sizeof(ViewGeom);
+  sz[2] = (int64_t)sizeof(int32_t) * S;
+  sz[3] = (int64_t)sizeof(float) * (mode == MODE_NN ? S : n);
+  sz[4] = (int64_t)sizeof(float) * R * std::max(nl, 1);
+  sz[5] = mode == MODE_NN ? (int64_t)sizeof(int32_t) * R : 0;
+  sz[6] = mode == MODE_NN ? (int64_t)sizeof(float2) * S : 0;
+  sz[7] = R;
+}
+
+// Argument and object checks shared by the synchronous renders and the server.
+int resolve_render(int goft, int mode, int nx, int ny, int nz, int nl, Goft** gcp, Dataset** dsp) {
+  if (int rc = check_res(nx, ny, nz, nl)) return rc;
+  Goft* gc = find_obj(g_ctx.gofts, goft);
+  if (!gc) return fail(FOMO_ERR_NOTFOUND, "unknown emission id");
+  if ((mode == MODE_THOMSON) != (gc->kind == 1))
+    return fail(FOMO_ERR_STATE, "emission cube kind does not match the render method");
+  Dataset* ds = find_obj(g_ctx.datasets, gc->dataset);
+  if (!ds) return fail(FOMO_ERR_NOTFOUND, "dataset of this emission cube was destroyed");
+  if (mode == MODE_NN && nl > 1 && ds->nvars < 5) return fail(FOMO_ERR_ARG, "spectroscopic render needs vx,vy,vz");
+  *gcp = gc;
+  *dsp = ds;
+  return FOMO_OK;
+}
+
+// All kernels of one view on stream s; the image lands in rb.out, the geometry in rb.geom.
+int enqueue_render(hipStream_t s, const Dataset& ds, const Goft& gc, int mode, double l, double b, int nx, int ny,
+                   int nz, int nl, double width, int dn, const RenderBufs& rb) {
+  const int64_t R = (int64_t)nx * ny;
+  enqueue_view_geom(s, ds, mode, l, b, nx, ny, nz, dn, rb.aux, rb.partial, rb.geom);
+  if (mode == MODE_NN) {
+    enqueue_nn(s, ds, rb.geom, nx, ny, nz, rb.idx, rb.rayflag, gc.rec.as<float4>(), gc.rs4, 1, rb.aux, rb.sem);
+    LineSet ls = make_lineset({rb.sem}, {rb.out}, {gc.lambda0}, width);
+    enqueue_los(s, (int)R, nz, ls, rb.aux, rb.geom, nl, rb.flag);
+  } else {
+    enqueue_nn(s, ds, rb.geom, nx, ny, nz, rb.idx, rb.rayflag);
+    ProfScope ps("k_los_thomson", s);
+    k_los_thomson<<<nblk(R), 256, 0, s>>>(rb.idx, (int)R, nz, gc.em.as<float2>(), rb.aux, rb.geom, rb.out);
+  }
+  HIPOK(hipGetLastError());
+  return FOMO_OK;
+}
+
+void geom_to_meta(const ViewGeom& hg, double* meta) {
+  for (int a = 0; a < 6; ++a) meta[a] = hg.ext[a];
+  meta[6] = hg.md;
+  meta[7] = hg.pathlength;
+  meta[8] = hg.scale;
+  meta[9] = hg.apix;
+}
+
+
+// ---------------------------------------------------------------------------------------
+// Render server: continuous batching of independent view requests over resident data.
+//
+// Each request is one view (NN or Thomson) of a resident emission cube. A worker thread admits
+// queued requests in arrival order whenever an in-flight slot and a block of the server's
+// device arena (the request's footprint: per-sample NN buffers, image, scratch) and of its
+// pinned host arena (image + geometry staging) are free; smaller requests may pass one that
+// does not fit, at most kMaxBypass times. An admitted request is enqueued on the least loaded of
+// the server's streams, so requests overlap on the device. Completion is seen through an
+// event; the worker then copies the image to the caller's buffer and frees the blocks.
+// Requests run the same kernels as the synchronous renders on private buffers, so a result does
+// not depend on what else is in flight. The worker takes no global lock: datasets and emission
+// cubes referenced by queued or running requests cannot be destroyed or re-indexed (in-use
+// counts), and every buffer it writes belongs to the request.
+// ---------------------------------------------------------------------------------------
+constexpr int kMaxBypass = 16;
+enum { kTicketQueued = 0, kTicketRunning = 1, kTicketDone = 2 };
+
+struct Ticket {
+  Dataset* ds = nullptr;
+  Goft* gc = nullptr;
+  int mode = MODE_NN;
+  double l = 0, b = 0, width = 0;
+  int nx = 0, ny = 0, nz = 0, nl = 1, dn = 0;
+  float* out = nullptr;
+  double* meta = nullptr;
+  int state = kTicketQueued;  // or a negative FOMO_ERR_* code
+  std::string err;
+  double t_submit = 0, t_admit = 0, t_done = 0;
+  int64_t sz[kRenderBufs] = {};
+  int64_t dev_bytes = 0, host_bytes = 0;
+  int64_t dev_off = -1, host_off = -1;
+  int stream = -1;
+  int bypassed = 0;
+  hipEvent_t done = nullptr;
+};
+
+int64_t footprint_device(const int64_t sz[kRenderBufs]) {
+  int64_t t = 0;
+  for (int i = 0; i < kRenderBufs; ++i) t += fomo::Arena::round_up(sz[i]);
+  return t;
+}
+int64_t footprint_host(const int64_t sz[kRenderBufs]) {
+  return fomo::Arena::round_up(sz[4]) + fomo::Arena::round_up(sizeof(ViewGeom));
+}
+
+struct Server {
+  int device = 0;
+  int max_inflight = 1;
+  std::vector<hipStream_t> streams;
+  std::vector<int> load;  // requests in flight per stream
+  void* dbase = nullptr;
+  void* hbase = nullptr;
+  fomo::Arena dev, host;
+  std::vector<hipEvent_t> evpool;
+  std::mutex mu;
+  std::condition_variable cv_work, cv_done;
+  std::deque<int64_t> pending;
+  std::vector<int64_t> inflight;
+  std::deque<Ticket> tickets;  // ticket id = position (references stay valid on push_back)
+  bool stop = false;
+  int64_t completed = 0, failed = 0, peak_inflight = 0;
+  std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now();
+  std::thread worker;
+
+  double now() const {
+    return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  }
+  ~Server() {
+    for (auto e : evpool) (void)hipEventDestroy(e);
+    for (auto& t : tickets)
+      if (t.done) (void)hipEventDestroy(t.done);
+    for (auto st : streams) (void)hipStreamDestroy(st);
+    if (dbase) (void)hipFree(dbase);
+    if (hbase) (void)hipHostFree(hbase);
+  }
+};
+
+void ticket_release_objects(Ticket& t) {
+  if (t.ds) t.ds->inuse.fetch_sub(1);
+  if (t.gc) t.gc->inuse.fetch_sub(1);
+  t.ds = nullptr;
+  t.gc = nullptr;
+}
+
+// Enqueue one admitted request (server lock held). Returns FOMO_OK or an error for this ticket.
+int server_enqueue(Server& sv, Ticket& t) {
+  hipStream_t s = sv.streams[t.stream];
+  char* d = static_cast<char*>(sv.dbase) + t.dev_off;
+  char* p[kRenderBufs];
+  for (int i = 0; i < kRenderBufs; ++i) {
+    p[i] = d;
+    d += fomo::Arena::round_up(t.sz[i]);
+  }
+  const RenderBufs rb{reinterpret_cast<double*>(p[0]),  reinterpret_cast<ViewGeom*>(p[1]),
+                      reinterpret_cast<int32_t*>(p[2]), reinterpret_cast<float*>(p[3]),
+                      reinterpret_cast<float*>(p[4]),   reinterpret_cast<int32_t*>(p[5]),
+                      reinterpret_cast<float2*>(p[6]),  reinterpret_cast<uint8_t*>(p[7])};
+  HIPOK(hipMemsetAsync(rb.rayflag, 0, t.sz[7], s));
+  if (int rc = enqueue_render(s, *t.ds, *t.gc, t.mode, t.l, t.b, t.nx, t.ny, t.nz, t.nl, t.width, t.dn, rb))
+    return rc;
+  char* h = static_cast<char*>(sv.hbase) + t.host_off;
+  HIPOK(hipMemcpyAsync(h, rb.out, t.sz[4], hipMemcpyDeviceToHost, s));
+  HIPOK(hipMemcpyAsync(h + fomo::Arena::round_up(t.sz[4]), rb.geom, sizeof(ViewGeom), hipMemcpyDeviceToHost, s));
+  if (sv.evpool.empty()) {
+    hipEvent_t e = nullptr;
+    HIPOK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    sv.evpool.push_back(e);
+  }
+  t.done = sv.evpool.back();
+  sv.evpool.pop_back();
+  HIPOK(hipEventRecord(t.done, s));
+  return FOMO_OK;
+}
+
+void server_fail_ticket(Server& sv, Ticket& t, int code, const std::string& msg) {
+  t.state = code;
+  t.err = msg;
+  t.t_done = sv.now();
+  ticket_release_objects(t);
+  ++sv.failed;
+}
+
+// Admit what fits (server lock held). Returns true when anything was admitted or failed.
+bool server_admit(Server& sv) {
+  bool any = false;
+  Ticket* blocked = nullptr;  // oldest request that did not fit
+  for (auto it = sv.pending.begin(); it != sv.pending.end();) {
+    if ((int64_t)sv.inflight.size() >= sv.max_inflight) break;
+    Ticket& t = sv.tickets[*it];
+    const int64_t doff = sv.dev.alloc(t.dev_bytes);
+    const int64_t hoff = doff >= 0 ? sv.host.alloc(t.host_bytes) : -1;
+    if (hoff < 0) {
+      if (doff >= 0) sv.dev.release(doff);
+      if (!blocked) {
+        blocked = &t;
+        if (t.bypassed >= kMaxBypass) break;  // nothing may pass it any more
+      }
+      ++it;
+      continue;
+    }
+    t.dev_off = doff;
+    t.host_off = hoff;
+    int best = 0;
+    for (int i = 1; i < (int)sv.streams.size(); ++i)
+      if (sv.load[i] < sv.load[best]) best = i;
+    t.stream = best;
+    const int64_t id = *it;
+    it = sv.pending.erase(it);
+    any = true;
+    t.t_admit = sv.now();
+    if (int rc = server_enqueue(sv, t)) {
+      // launches already made stay on the stream; keep the blocks until it drains
+      (void)hipStreamSynchronize(sv.streams[t.stream]);
+      sv.dev.release(t.dev_off);
+      sv.host.release(t.host_off);
+      server_fail_ticket(sv, t, rc, g_err);
+      sv.cv_done.notify_all();
+      continue;
+    }
+    t.state = kTicketRunning;
+    sv.load[best]++;
+    sv.inflight.push_back(id);
+    sv.peak_inflight = std::max<int64_t>(sv.peak_inflight, (int64_t)sv.inflight.size());
+    if (blocked) blocked->bypassed++;
+  }
+  return any;
+}
+
+// One pass of the worker: retire finished requests, then admit. Returns whether anything changed.
+bool server_step(Server& sv) {
+  // pointers, not ids: the deque's index map may change under a concurrent submit
+  std::vector<Ticket*> fin;
+  {
+    std::lock_guard<std::mutex> lk(sv.mu);
+    for (size_t i = 0; i < sv.inflight.size();) {
+      Ticket& t = sv.tickets[sv.inflight[i]];
+      const hipError_t q = hipEventQuery(t.done);
+      if (q == hipErrorNotReady) {
+        ++i;
+        continue;
+      }
+      if (q != hipSuccess) {
+        t.err = std::string("request failed on the device: ") + hipGetErrorString(q);
+        t.state = FOMO_ERR_HIP;
+      }
+      fin.push_back(&t);
+      sv.inflight[i] = sv.inflight.back();
+      sv.inflight.pop_back();
+    }
+  }
+  // copy results out without the lock (these tickets and their staging blocks are ours now)
+  for (Ticket* t : fin) {
+    if (t->state != kTicketRunning) continue;
+    const char* h = static_cast<const char*>(sv.hbase) + t->host_off;
+    if (t->out) std::memcpy(t->out, h, (size_t)t->sz[4]);
+    if (t->meta) {
+      ViewGeom hg;
+      std::memcpy(&hg, h + fomo::Arena::round_up(t->sz[4]), sizeof(ViewGeom));
+      geom_to_meta(hg, t->meta);
+    }
+  }
+  std::lock_guard<std::mutex> lk(sv.mu);
+  for (Ticket* t : fin) {
+    sv.dev.release(t->dev_off);
+    sv.host.release(t->host_off);
+    sv.load[t->stream]--;
+    sv.evpool.push_back(t->done);
+    t->done = nullptr;
+    if (t->state == kTicketRunning) {
+      t->state = kTicketDone;
+      t->t_done = sv.now();
+      ticket_release_objects(*t);
+      ++sv.completed;
+    } else {
+      server_fail_ticket(sv, *t, t->state, t->err);
+    }
+  }
+  if (!fin.empty()) sv.cv_done.notify_all();
+  const bool admitted = server_admit(sv);
+  return admitted || !fin.empty();
+}
+
+void server_loop(Server* sv) {
+  (void)hipSetDevice(sv->device);
+  for (;;) {
+    {
+      std::unique_lock<std::mutex> lk(sv->mu);
+      if (sv->pending.empty() && sv->inflight.empty()) {
+        if (sv->stop) return;
+        sv->cv_work.wait(lk, [&] { return sv->stop || !sv->pending.empty(); });
+        continue;
+      }
+    }
+    if (!server_step(*sv)) {
+      std::unique_lock<std::mutex> lk(sv->mu);
+      sv->cv_work.wait_for(lk, std::chrono::microseconds(50));
+    }
+  }
+}
+
+// Stop a server: queued requests fail with FOMO_ERR_STATE, running ones complete.
+void server_shutdown(Server& sv) {
+  {
+    std::lock_guard<std::mutex> lk(sv.mu);
+    sv.stop = true;
+    for (int64_t id : sv.pending) server_fail_ticket(sv, sv.tickets[id], FOMO_ERR_STATE, "server destroyed");
+    sv.pending.clear();
+    sv.cv_work.notify_all();
+    sv.cv_done.notify_all();
+  }
+  if (sv.worker.joinable()) sv.worker.join();
+}
+
+std::shared_ptr<Server> find_server(int id) {
+  std::lock_guard<std::recursive_mutex> lk(g_mu);
+  auto it = g_ctx.servers.find(id);
+  return it == g_ctx.servers.end() ? nullptr : it->second;
+}
+
 }  // namespace
 
 // ======================================================================================
@@ -1724,6 +2064,16 @@ int fomo_init(int device) {
 }
 
 int fomo_shutdown(void) {
+  {
+    // servers first, outside the global lock (their workers finish in-flight requests)
+    std::vector<std::shared_ptr<Server>> svs;
+    {
+      std::lock_guard<std::recursive_mutex> lk(g_mu);
+      for (auto& kv : g_ctx.servers) svs.push_back(kv.second);
+      g_ctx.servers.clear();
+    }
+    for (auto& sv : svs) server_shutdown(*sv);
+  }
   std::lock_guard<std::recursive_mutex> lk(g_mu);
   if (!g_ctx.init) return FOMO_OK;
   (void)hipStreamSynchronize(g_ctx.stream);
@@ -1804,7 +2154,10 @@ int fomo_dataset_destroy(int id) {
   std::lock_guard<std::recursive_mutex> lk(g_mu);
   if (int rc = require_init()) return rc;
   (void)hipStreamSynchronize(g_ctx.stream);
-  if (!g_ctx.datasets.erase(id)) return fail(FOMO_ERR_NOTFOUND, "unknown dataset id");
+  Dataset* ds = find_obj(g_ctx.datasets, id);
+  if (!ds) return fail(FOMO_ERR_NOTFOUND, "unknown dataset id");
+  if (ds->inuse.load() > 0) return fail(FOMO_ERR_STATE, "dataset is in use by a batch or a server request");
+  g_ctx.datasets.erase(id);
   return FOMO_OK;
 }
 
@@ -1825,6 +2178,7 @@ int fomo_dataset_set_lookup(int id, int subdiv) {
   Dataset* ds = find_obj(g_ctx.datasets, id);
   if (!ds) return fail(FOMO_ERR_NOTFOUND, "unknown dataset id");
   if (subdiv < 0 || subdiv > 4) return fail(FOMO_ERR_ARG, "lookup subdivision must be in [0,4]");
+  if (ds->inuse.load() > 0) return fail(FOMO_ERR_STATE, "dataset is in use by a batch or a server request");
   ds->lookup_subdiv = subdiv;
   return build_lookup(*ds);
 }
@@ -1846,6 +2200,7 @@ int fomo_dataset_reindex(int id, double cell_scale) {
   if (int rc = require_init()) return rc;
   Dataset* ds = find_obj(g_ctx.datasets, id);
   if (!ds) return fail(FOMO_ERR_NOTFOUND, "unknown dataset id");
+  if (ds->inuse.load() > 0) return fail(FOMO_ERR_STATE, "dataset is in use by a batch or a server request");
   return build_index(*ds, cell_scale);
 }
 
@@ -1898,7 +2253,7 @@ int fomo_emission_chianti(int dataset, int table, double lambda0, double atweigh
                    FOMO_C_LIGHT;
   const double normc = imaging ? 1.0 : (1. / FOMO_ALPHA) * abundratio;
   {
-    ProfScope ps("k_emission_chianti");
+    ProfScope ps("k_emission_chianti", g_ctx.stream);
     const unsigned nb = std::min<unsigned>(nblk(ds->n), 4096);
     k_emission_chianti<<<nb, kBlock, 0, g_ctx.stream>>>(ds->svars[0]->as<float>(), ds->svars[1]->as<float>(), ds->n,
                                                          tb->logt.as<float>(), tb->logn.as<float>(), tb->g.as<float>(),
@@ -1926,7 +2281,7 @@ int fomo_emission_thomson(int dataset, int* out_goft) {
   gc->lambda0 = 0;
   if (gc->em.alloc(sizeof(float2) * ds->n)) return fail(FOMO_ERR_NOMEM, "device allocation failed");
   {
-    ProfScope ps("k_emission_thomson");
+    ProfScope ps("k_emission_thomson", g_ctx.stream);
     const unsigned nb = std::min<unsigned>(nblk(ds->n), 4096);
     k_emission_thomson<<<nb, kBlock, 0, g_ctx.stream>>>(ds->pts.as<float4>(), ds->svars[0]->as<float>(), ds->n,
                                                          gc->em.as<float2>());
@@ -1942,7 +2297,10 @@ int fomo_emission_thomson(int dataset, int* out_goft) {
 int fomo_goft_destroy(int id) {
   std::lock_guard<std::recursive_mutex> lk(g_mu);
   if (int rc = require_init()) return rc;
-  if (!g_ctx.gofts.erase(id)) return fail(FOMO_ERR_NOTFOUND, "unknown emission id");
+  Goft* gc = find_obj(g_ctx.gofts, id);
+  if (!gc) return fail(FOMO_ERR_NOTFOUND, "unknown emission id");
+  if (gc->inuse.load() > 0) return fail(FOMO_ERR_STATE, "emission cube is in use by a server request");
+  g_ctx.gofts.erase(id);
   return FOMO_OK;
 }
 
@@ -1968,52 +2326,30 @@ int fomo_goft_read(int id, float* var0, float* var1) {
 static int render_one(int goft, int mode, double l, double b, int nx, int ny, int nz, int nl, double width, int dn,
                       float* out, double* meta) {
   if (int rc = require_init()) return rc;
-  if (int rc = check_res(nx, ny, nz, nl)) return rc;
-  Goft* gc = find_obj(g_ctx.gofts, goft);
-  if (!gc) return fail(FOMO_ERR_NOTFOUND, "unknown emission id");
-  if ((mode == MODE_THOMSON) != (gc->kind == 1))
-    return fail(FOMO_ERR_STATE, "emission cube kind does not match the render method");
-  Dataset* ds = find_obj(g_ctx.datasets, gc->dataset);
-  if (!ds) return fail(FOMO_ERR_NOTFOUND, "dataset of this emission cube was destroyed");
-  if (mode == MODE_NN && nl > 1 && ds->nvars < 5) return fail(FOMO_ERR_ARG, "spectroscopic render needs vx,vy,vz");
+  Goft* gc = nullptr;
+  Dataset* ds = nullptr;
+  if (int rc = resolve_render(goft, mode, nx, ny, nz, nl, &gc, &ds)) return rc;
   hipStream_t s = g_ctx.stream;
   const int64_t R = (int64_t)nx * ny, S = R * nz;
-  const int nlo = std::max(nl, 1);
+  int64_t sz[kRenderBufs];
+  render_sizes(mode, ds->n, R, S, nl, sz);
   ViewScratch& vs = g_ctx.vs;
-  HIPOK(vs.partial.ensure(sizeof(double) * 6 * (kMaxPartials + 1)));
-  HIPOK(vs.geom.ensure(sizeof(ViewGeom)));
-  HIPOK(vs.idx.ensure(sizeof(int32_t) * S));
-  HIPOK(vs.aux.ensure(sizeof(float) * std::max<int64_t>(ds->n, S)));
-  HIPOK(vs.out.ensure(sizeof(float) * R * nlo));
-  HIPOK(vs.flag.ensure(sizeof(int32_t) * R));
-  HIPOK(vs.rayflag.ensure_zeroed(R, s));
-  if (mode == MODE_NN) HIPOK(vs.sem.ensure(sizeof(float2) * S));
-  ViewGeom* geom = vs.geom.as<ViewGeom>();
-  enqueue_view_geom(s, *ds, mode, l, b, nx, ny, nz, dn, vs.aux.as<float>(), vs.partial.as<double>(), geom);
-  if (mode == MODE_NN) {
-    // aux: per-sample LOS velocity; sem: per-sample (peak, fwhm)
-    enqueue_nn(s, *ds, geom, nx, ny, nz, vs.idx.as<int32_t>(), vs.rayflag.as<uint8_t>(), gc->rec.as<float4>(), gc->rs4,
-               1, vs.aux.as<float>(), vs.sem.as<float2>());
-    LineSet ls = make_lineset({vs.sem.as<float2>()}, {vs.out.as<float>()}, {gc->lambda0}, width);
-    enqueue_los(s, (int)R, nz, ls, vs.aux.as<float>(), geom, nl, vs.flag.as<int32_t>());
-  } else {
-    enqueue_nn(s, *ds, geom, nx, ny, nz, vs.idx.as<int32_t>(), vs.rayflag.as<uint8_t>());
-    ProfScope ps("k_los_thomson");
-    k_los_thomson<<<nblk(R), 256, 0, s>>>(vs.idx.as<int32_t>(), (int)R, nz, gc->em.as<float2>(), vs.aux.as<float>(),
-                                          geom, vs.out.as<float>());
-  }
-  HIPOK(hipGetLastError());
-  if (out) HIPOK(hipMemcpyAsync(out, vs.out.p, sizeof(float) * R * nlo, hipMemcpyDeviceToHost, s));
+  HIPOK(vs.partial.ensure(sz[0]));
+  HIPOK(vs.geom.ensure(sz[1]));
+  HIPOK(vs.idx.ensure(sz[2]));
+  HIPOK(vs.aux.ensure(sz[3]));
+  HIPOK(vs.out.ensure(sz[4]));
+  HIPOK(vs.flag.ensure(sz[5]));
+  HIPOK(vs.sem.ensure(sz[6]));
+  HIPOK(vs.rayflag.ensure_zeroed(sz[7], s));
+  const RenderBufs rb{vs.partial.as<double>(), vs.geom.as<ViewGeom>(), vs.idx.as<int32_t>(), vs.aux.as<float>(),
+                      vs.out.as<float>(),      vs.flag.as<int32_t>(),  vs.sem.as<float2>(),  vs.rayflag.as<uint8_t>()};
+  if (int rc = enqueue_render(s, *ds, *gc, mode, l, b, nx, ny, nz, nl, width, dn, rb)) return rc;
+  if (out) HIPOK(hipMemcpyAsync(out, rb.out, sz[4], hipMemcpyDeviceToHost, s));
   ViewGeom hg;
-  HIPOK(hipMemcpyAsync(&hg, geom, sizeof(ViewGeom), hipMemcpyDeviceToHost, s));
+  HIPOK(hipMemcpyAsync(&hg, rb.geom, sizeof(ViewGeom), hipMemcpyDeviceToHost, s));
   if (int rc = sync_stream()) return rc;
-  if (meta) {
-    for (int a = 0; a < 6; ++a) meta[a] = hg.ext[a];
-    meta[6] = hg.md;
-    meta[7] = hg.pathlength;
-    meta[8] = hg.scale;
-    meta[9] = hg.apix;
-  }
+  if (meta) geom_to_meta(hg, meta);
   return FOMO_OK;
 }
 
@@ -2181,6 +2517,7 @@ int fomo_batch_create(int ngoft, const int* gofts, int nviews, const double* l, 
   }
   if (int rc = capture_batch(*bt)) return rc;
   int id = g_ctx.next_id++;
+  ds->inuse.fetch_add(1);
   g_ctx.batches[id] = std::move(bt);
   *out_batch = id;
   return FOMO_OK;
@@ -2250,7 +2587,10 @@ int fomo_batch_destroy(int id) {
   std::lock_guard<std::recursive_mutex> lk(g_mu);
   if (int rc = require_init()) return rc;
   (void)hipStreamSynchronize(g_ctx.stream);
-  if (!g_ctx.batches.erase(id)) return fail(FOMO_ERR_NOTFOUND, "unknown batch id");
+  Batch* bt = find_obj(g_ctx.batches, id);
+  if (!bt) return fail(FOMO_ERR_NOTFOUND, "unknown batch id");
+  if (Dataset* ds = find_obj(g_ctx.datasets, bt->dataset)) ds->inuse.fetch_sub(1);
+  g_ctx.batches.erase(id);
   return FOMO_OK;
 }
 
@@ -2303,6 +2643,182 @@ int fomo_profile_stats(const char* kernel, int64_t* count, double* total_ms) {
   auto it = g_ctx.prof.find(kernel);
   if (count) *count = it == g_ctx.prof.end() ? 0 : it->second.count;
   if (total_ms) *total_ms = it == g_ctx.prof.end() ? 0.0 : it->second.ms;
+  return FOMO_OK;
+}
+
+// ---------------------------------------------------------------------------------------
+// render server
+// ---------------------------------------------------------------------------------------
+int fomo_server_footprint(int goft, int method, int nx, int ny, int nz, int nl, int64_t* device_bytes,
+                          int64_t* host_bytes) {
+  std::lock_guard<std::recursive_mutex> lk(g_mu);
+  if (int rc = require_init()) return rc;
+  if (method != FOMO_METHOD_NN && method != FOMO_METHOD_THOMSON) return fail(FOMO_ERR_ARG, "unknown method");
+  const int mode = method == FOMO_METHOD_NN ? MODE_NN : MODE_THOMSON;
+  if (mode == MODE_THOMSON) nl = 1;
+  Goft* gc = nullptr;
+  Dataset* ds = nullptr;
+  if (int rc = resolve_render(goft, mode, nx, ny, nz, nl, &gc, &ds)) return rc;
+  int64_t sz[kRenderBufs];
+  render_sizes(mode, ds->n, (int64_t)nx * ny, (int64_t)nx * ny * nz, nl, sz);
+  if (device_bytes) *device_bytes = footprint_device(sz);
+  if (host_bytes) *host_bytes = footprint_host(sz);
+  return FOMO_OK;
+}
+
+int fomo_server_create(int64_t device_bytes, int64_t host_bytes, int max_inflight, int nstreams, int* out_id) {
+  std::lock_guard<std::recursive_mutex> lk(g_mu);
+  if (int rc = require_init()) return rc;
+  if (!out_id || device_bytes < (1 << 20) || host_bytes < (1 << 16) || max_inflight < 1 || nstreams < 1 ||
+      nstreams > 16)
+    return fail(FOMO_ERR_ARG, "server needs >= 1 MiB device, >= 64 KiB host, max_inflight >= 1, 1..16 streams");
+  auto sv = std::make_shared<Server>();
+  sv->device = g_ctx.device;
+  sv->max_inflight = max_inflight;
+  if (hipMalloc(&sv->dbase, (size_t)device_bytes) != hipSuccess) {
+    sv->dbase = nullptr;
+    (void)hipGetLastError();
+    return fail(FOMO_ERR_NOMEM, "cannot allocate the server's device arena");
+  }
+  if (hipHostMalloc(&sv->hbase, (size_t)host_bytes, hipHostMallocDefault) != hipSuccess) {
+    sv->hbase = nullptr;
+    (void)hipGetLastError();
+    return fail(FOMO_ERR_NOMEM, "cannot allocate the server's pinned host arena");
+  }
+  sv->dev.reset(device_bytes);
+  sv->host.reset(host_bytes);
+  for (int i = 0; i < nstreams; ++i) {
+    hipStream_t st = nullptr;
+    HIPOK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+    sv->streams.push_back(st);
+    sv->load.push_back(0);
+  }
+  sv->worker = std::thread(server_loop, sv.get());
+  const int id = g_ctx.next_id++;
+  g_ctx.servers[id] = sv;
+  *out_id = id;
+  return FOMO_OK;
+}
+
+int fomo_server_submit(int server, int goft, int method, double l, double b, int nx, int ny, int nz, int nl,
+                       double lambda_width, int dn, float* out, double* meta, int64_t* out_ticket) {
+  std::lock_guard<std::recursive_mutex> lk(g_mu);
+  if (int rc = require_init()) return rc;
+  auto it = g_ctx.servers.find(server);
+  if (it == g_ctx.servers.end()) return fail(FOMO_ERR_NOTFOUND, "unknown server id");
+  Server& sv = *it->second;
+  if (!out || !out_ticket) return fail(FOMO_ERR_ARG, "null output");
+  if (method != FOMO_METHOD_NN && method != FOMO_METHOD_THOMSON) return fail(FOMO_ERR_ARG, "unknown method");
+  Ticket t;
+  t.mode = method == FOMO_METHOD_NN ? MODE_NN : MODE_THOMSON;
+  if (t.mode == MODE_THOMSON) nl = 1, lambda_width = 0.0, dn = 0;
+  if (int rc = resolve_render(goft, t.mode, nx, ny, nz, nl, &t.gc, &t.ds)) return rc;
+  t.l = l, t.b = b, t.width = lambda_width;
+  t.nx = nx, t.ny = ny, t.nz = nz, t.nl = nl, t.dn = dn;
+  t.out = out;
+  t.meta = meta;
+  render_sizes(t.mode, t.ds->n, (int64_t)nx * ny, (int64_t)nx * ny * nz, nl, t.sz);
+  t.dev_bytes = footprint_device(t.sz);
+  t.host_bytes = footprint_host(t.sz);
+  if (t.dev_bytes > sv.dev.capacity() || t.host_bytes > sv.host.capacity())
+    return fail(FOMO_ERR_NOMEM, "request footprint exceeds the server's memory budget");
+  t.ds->inuse.fetch_add(1);
+  t.gc->inuse.fetch_add(1);
+  std::lock_guard<std::mutex> slk(sv.mu);
+  if (sv.stop) {
+    ticket_release_objects(t);
+    return fail(FOMO_ERR_STATE, "server is shutting down");
+  }
+  t.t_submit = sv.now();
+  const int64_t id = (int64_t)sv.tickets.size();
+  sv.tickets.push_back(std::move(t));
+  sv.pending.push_back(id);
+  *out_ticket = id;
+  sv.cv_work.notify_one();
+  return FOMO_OK;
+}
+
+int fomo_server_poll(int server, int64_t ticket, int* state) {
+  auto sv = find_server(server);
+  if (!sv) return fail(FOMO_ERR_NOTFOUND, "unknown server id");
+  std::lock_guard<std::mutex> lk(sv->mu);
+  if (ticket < 0 || ticket >= (int64_t)sv->tickets.size()) return fail(FOMO_ERR_NOTFOUND, "unknown ticket");
+  const Ticket& t = sv->tickets[ticket];
+  if (state) *state = t.state;
+  if (t.state < 0) return fail(t.state, t.err);
+  return FOMO_OK;
+}
+
+int fomo_server_wait(int server, int64_t ticket, double timeout_s, int* state) {
+  auto sv = find_server(server);
+  if (!sv) return fail(FOMO_ERR_NOTFOUND, "unknown server id");
+  std::unique_lock<std::mutex> lk(sv->mu);
+  if (ticket < 0 || ticket >= (int64_t)sv->tickets.size()) return fail(FOMO_ERR_NOTFOUND, "unknown ticket");
+  const Ticket& t = sv->tickets[ticket];
+  auto finished = [&] { return t.state == kTicketDone || t.state < 0; };
+  if (timeout_s < 0) sv->cv_done.wait(lk, finished);
+  else sv->cv_done.wait_for(lk, std::chrono::duration<double>(timeout_s), finished);
+  if (state) *state = t.state;
+  if (t.state < 0) return fail(t.state, t.err);
+  if (t.state != kTicketDone) return fail(FOMO_ERR_STATE, "timed out");
+  return FOMO_OK;
+}
+
+int fomo_server_wait_all(int server, double timeout_s) {
+  auto sv = find_server(server);
+  if (!sv) return fail(FOMO_ERR_NOTFOUND, "unknown server id");
+  std::unique_lock<std::mutex> lk(sv->mu);
+  auto idle = [&] { return sv->pending.empty() && sv->inflight.empty(); };
+  if (timeout_s < 0) sv->cv_done.wait(lk, idle);
+  else if (!sv->cv_done.wait_for(lk, std::chrono::duration<double>(timeout_s), idle))
+    return fail(FOMO_ERR_STATE, "timed out");
+  return FOMO_OK;
+}
+
+int fomo_server_times(int server, int64_t ticket, double* times3) {
+  auto sv = find_server(server);
+  if (!sv) return fail(FOMO_ERR_NOTFOUND, "unknown server id");
+  if (!times3) return fail(FOMO_ERR_ARG, "null output");
+  std::lock_guard<std::mutex> lk(sv->mu);
+  if (ticket < 0 || ticket >= (int64_t)sv->tickets.size()) return fail(FOMO_ERR_NOTFOUND, "unknown ticket");
+  const Ticket& t = sv->tickets[ticket];
+  times3[0] = t.t_submit;
+  times3[1] = t.state == kTicketQueued ? -1.0 : t.t_admit;
+  times3[2] = (t.state == kTicketDone || t.state < 0) ? t.t_done : -1.0;
+  return FOMO_OK;
+}
+
+int fomo_server_stats(int server, double* st) {
+  auto sv = find_server(server);
+  if (!sv) return fail(FOMO_ERR_NOTFOUND, "unknown server id");
+  if (!st) return fail(FOMO_ERR_ARG, "null output");
+  std::lock_guard<std::mutex> lk(sv->mu);
+  st[0] = (double)sv->tickets.size();
+  st[1] = (double)sv->completed;
+  st[2] = (double)sv->failed;
+  st[3] = (double)sv->pending.size();
+  st[4] = (double)sv->inflight.size();
+  st[5] = (double)sv->peak_inflight;
+  st[6] = (double)sv->dev.in_use();
+  st[7] = (double)sv->dev.peak();
+  st[8] = (double)sv->dev.capacity();
+  st[9] = (double)sv->host.in_use();
+  st[10] = (double)sv->host.peak();
+  st[11] = (double)sv->host.capacity();
+  return FOMO_OK;
+}
+
+int fomo_server_destroy(int server) {
+  std::shared_ptr<Server> sv;
+  {
+    std::lock_guard<std::recursive_mutex> lk(g_mu);
+    if (int rc = require_init()) return rc;
+    auto it = g_ctx.servers.find(server);
+    if (it == g_ctx.servers.end()) return fail(FOMO_ERR_NOTFOUND, "unknown server id");
+    sv = it->second;
+    g_ctx.servers.erase(it);
+  }
+  server_shutdown(*sv);
   return FOMO_OK;
 }
 

@@ -126,6 +126,47 @@ int fomo_batch_read(int batch, float* out, double* meta);
 int fomo_batch_destroy(int batch);
 
 /* ---------------------------------------------------------------------------------------
+ * Render server: continuous batching of independent view requests over resident datasets and
+ * emission cubes. No counterpart in the reference (its render() is one synchronous call per
+ * object, fomo-object.cpp:374-440); each request computes exactly what fomo_render_nn /
+ * fomo_render_thomson compute for the same arguments.
+ *
+ * A worker thread admits queued requests in arrival order whenever one of max_inflight slots and
+ * a block of the server's device arena (device_bytes: per-sample buffers and image of a request,
+ * see fomo_server_footprint) and pinned host arena (host_bytes: image staging) are free; smaller
+ * requests may pass one that does not fit, a bounded number of times. Admitted requests run on
+ * nstreams HIP streams (default priority) and overlap on the device. The image is copied to
+ * `out` (ny*nx*max(nl,1) floats, RenderCube layout) and `meta` (10 doubles, as above, or NULL)
+ * before the ticket turns done; both must stay valid until then. Ticket states:
+ * 0 queued, 1 running, 2 done, negative = failed with that FOMO_ERR_* code.
+ * Datasets and emission cubes referenced by queued or running requests cannot be destroyed or
+ * re-indexed (FOMO_ERR_STATE). Destroying a server fails its queued requests and waits for the
+ * running ones.
+ * ------------------------------------------------------------------------------------- */
+#define FOMO_METHOD_NN 0      /* "NearestNeighbour": fomo_render_nn */
+#define FOMO_METHOD_THOMSON 1 /* "Thomson": fomo_render_thomson (nl, lambda_width, dn ignored) */
+int fomo_server_create(int64_t device_bytes, int64_t host_bytes, int max_inflight, int nstreams,
+                       int* out_server);
+int fomo_server_destroy(int server);
+/* Memory a request of this shape takes from the device and host arenas while it runs. */
+int fomo_server_footprint(int goft, int method, int nx, int ny, int nz, int nl,
+                          int64_t* device_bytes, int64_t* host_bytes);
+/* Queue one view; FOMO_ERR_NOMEM when its footprint exceeds the server's arenas. */
+int fomo_server_submit(int server, int goft, int method, double l, double b, int nx, int ny,
+                       int nz, int nl, double lambda_width, int dn, float* out, double* meta,
+                       int64_t* out_ticket);
+int fomo_server_poll(int server, int64_t ticket, int* state);
+/* Block until the ticket is done or failed (timeout_s < 0: no limit); FOMO_ERR_STATE on timeout,
+ * the ticket's own error code when it failed. */
+int fomo_server_wait(int server, int64_t ticket, double timeout_s, int* state);
+int fomo_server_wait_all(int server, double timeout_s);
+/* Seconds since the server was created: submitted, admitted (-1 if queued), finished (-1). */
+int fomo_server_times(int server, int64_t ticket, double* times3);
+/* 12 doubles: submitted, completed, failed, queued, running, peak running, device bytes in use,
+ * device peak, device budget, host bytes in use, host peak, host budget. */
+int fomo_server_stats(int server, double* stats12);
+
+/* ---------------------------------------------------------------------------------------
  * Kernel timing with HIP events on the library's launch stream (for roofline reporting).
  * When enabled, each launch of a named kernel is bracketed by events; stats return the
  * number of timed launches and their total milliseconds.
