@@ -50,7 +50,9 @@ def main():
     ap.add_argument("--cpu-rows", type=int, default=16, help="image rows in the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cell-scale", type=float, default=0.0)
-    ap.add_argument("--workload", choices=["batched", "thomson"], default="batched")
+    ap.add_argument("--workload", choices=["batched", "thomson", "serve"], default="batched")
+    ap.add_argument("--requests", type=int, default=1000, help="requests per arrival rate (serve workload)")
+    ap.add_argument("--serve-requests", type=int, default=240, help="requests in the serving phase of the default run")
     ap.add_argument("--corona-m", type=int, default=645, help="corona grid points per axis (thomson workload)")
     args = ap.parse_args()
 
@@ -65,6 +67,8 @@ def main():
     log(f"device {dev}, {hbm / 2**30:.0f} GiB HBM, {cus} CUs")
     if args.workload == "thomson":
         return thomson(F, args)
+    if args.workload == "serve":
+        return serve(F, args)
 
     m, nx, ny, nz, nl = args.m, args.nx, args.nx, args.nz, args.nl
     t0 = time.time()
@@ -116,7 +120,7 @@ def main():
     F.profile_reset()
     bt.profile()
     bt.profile()
-    stats = {k: F.profile_stats(k) for k in ("k_view_rotate_nn", "k_nn", "k_nn_rays", "k_los_spectral")}
+    stats = {k: F.profile_stats(k) for k in ("k_view_rotate_nn", "k_nn", "k_nn_far", "k_los_spectral")}
     log("per-kernel stats (count, total ms): " + json.dumps(stats))
     bytes_per = {
         # algorithmic bytes per launch
@@ -148,12 +152,22 @@ def main():
     W_render = float(nx) * ny * nz * nl
     plain_value = W_render * nplain / dtp
     log(f"plain: {dtp / nplain * 1e3:.2f} ms/render, {plain_value:.3e} W/s")
-    # index (cell list + candidate lookup grid) build time, once per dataset
+    # index (cell list + candidate lookup grid) build time, once per dataset (the batch graph holds
+    # pointers into the index, so it goes first)
+    bt.close()
     t0 = time.perf_counter()
     ds.reindex(args.cell_scale)
     t_index = time.perf_counter() - t0
     linfo = ds.lookup_info()
     log(f"index rebuild: {t_index * 1e3:.1f} ms, lookup {linfo}")
+
+    # ---- serving: requests of mixed sizes on the resident cube, Poisson arrivals at 0.8 x the
+    #      saturated request rate; latency = submit -> image in the caller's buffer -------------
+    emi = F.Emission.chianti(ds, tabs[0], lambda0, atw, imaging=True)
+    rng = np.random.default_rng(7)
+    targets = [{"spec": ems[0], "imag": emi, "weight": 1.0, "name": f"{m}^3 loop"}]
+    serving = serve_measure(F, targets, width, rng, args.serve_requests, [0.8])
+    emi.close()
 
     # ---- CPU baseline: reference float pipeline (oracle), 1 view x 1 line, row sample ---------
     cpu = None
@@ -199,12 +213,147 @@ def main():
         "batched_vs_plain": value / plain_value,
         "roofline": roof,
         "cpu_baseline": cpu,
+        "serving": serving,
     }
     print(json.dumps(out), flush=True)
-    bt.close()
     for e in ems:
         e.close()
     ds.close()
+    F.shutdown()
+
+
+# Request mix of the serving workload: image sizes x LOS samples x lambda pixels drawn uniformly.
+C4_RES = (64, 128, 256, 512)
+C4_NZ = (64, 128, 256)
+C4_NL = (1, 30, 100)
+
+
+def draw_requests(F, targets, width, rng, count):
+    """targets: dicts with "spec" (spectroscopic emission), "imag" (imaging emission, nl = 1) or
+    "thomson" (Thomson emission), a "weight" and a "name". Views l ~ U(0, 2 pi), b ~ U(0.1, pi/2)."""
+    w = np.array([t["weight"] for t in targets], float)
+    w /= w.sum()
+    reqs = []
+    for _ in range(count):
+        t = targets[rng.choice(len(targets), p=w)]
+        nx = int(rng.choice(C4_RES))
+        nz = int(rng.choice(C4_NZ))
+        nl = int(rng.choice(C4_NL))
+        l, b = float(rng.uniform(0, 2 * np.pi)), float(rng.uniform(0.1, np.pi / 2))
+        if "thomson" in t:
+            reqs.append((t["thomson"], F.METHOD_THOMSON, l, b, nx, nx, nz, 1, 0.0))
+        elif nl == 1:
+            reqs.append((t["imag"], F.METHOD_NN, l, b, nx, nx, nz, 1, 0.0))
+        else:
+            reqs.append((t["spec"], F.METHOD_NN, l, b, nx, nx, nz, nl, width))
+    return reqs
+
+
+def req_work(r):
+    return float(r[4]) * r[5] * r[6] * max(r[7], 1)
+
+
+def serve_measure(F, targets, width, rng, count, loads, device_bytes=24 << 30, host_bytes=2 << 30,
+                  max_inflight=8, nstreams=4):
+    """Saturated throughput (all requests queued at once), the same requests one synchronous render
+    at a time, then open-loop Poisson arrivals at each load x the saturated rate: p50/p99 latency."""
+    sv = F.Server(device_bytes=device_bytes, host_bytes=host_bytes, max_inflight=max_inflight, nstreams=nstreams)
+    try:
+        warm = draw_requests(F, targets, width, rng, 8)
+        for r in warm:
+            sv.submit(*r[:8], lambda_width=r[8])
+        sv.wait_all(300)
+        reqs = draw_requests(F, targets, width, rng, count)
+        W = sum(req_work(r) for r in reqs)
+        t0 = time.perf_counter()
+        tks = [sv.submit(*r[:8], lambda_width=r[8]) for r in reqs]
+        sv.wait_all(600)
+        dt_sat = time.perf_counter() - t0
+        for t in tks:
+            sv.result(t, 1)
+        sat_rps = count / dt_sat
+        log(f"serving saturated: {count} requests in {dt_sat:.2f}s, {sat_rps:.1f} req/s, {W / dt_sat:.3e} W/s")
+        t0 = time.perf_counter()
+        for r in reqs:
+            if r[1] == F.METHOD_THOMSON:
+                F.render_thomson(r[0], *r[2:7])
+            else:
+                F.render_nn(r[0], *r[2:8], r[8])
+        dt_seq = time.perf_counter() - t0
+        log(f"one synchronous render at a time: {dt_seq:.2f}s, {W / dt_seq:.3e} W/s")
+        runs = []
+        for load in loads:
+            rate = load * sat_rps
+            reqs = draw_requests(F, targets, width, rng, count)
+            arr = np.cumsum(rng.exponential(1.0 / rate, count))
+            tks = []
+            t0 = time.perf_counter()
+            for a, r in zip(arr, reqs):
+                d = a - (time.perf_counter() - t0)
+                if d > 0:
+                    time.sleep(d)
+                tks.append(sv.submit(*r[:8], lambda_width=r[8]))
+            sv.wait_all(900)
+            span = time.perf_counter() - t0
+            lat = np.array([sv.times(t)[2] - sv.times(t)[0] for t in tks]) * 1e3
+            for t in tks:
+                sv.result(t, 1)
+            Wl = sum(req_work(r) for r in reqs)
+            run = {"load": load, "arrival_rate_req_s": rate, "requests": count, "p50_ms": float(np.percentile(lat, 50)),
+                   "p99_ms": float(np.percentile(lat, 99)), "mean_ms": float(lat.mean()),
+                   "max_ms": float(lat.max()), "throughput_W_s": Wl / span}
+            log(f"serving at {load:.2f} x saturation ({rate:.1f} req/s): p50 {run['p50_ms']:.2f} ms, "
+                f"p99 {run['p99_ms']:.2f} ms, {Wl / span:.3e} W/s")
+            runs.append(run)
+        st = sv.stats()
+    finally:
+        sv.close()
+    return {"mix": " + ".join(t["name"] for t in targets) + "; image {64,128,256,512}^2 x nz {64,128,256} x "
+                   "nl {1,30,100} uniform, l ~ U(0,2pi), b ~ U(0.1,pi/2)",
+            "saturated_req_s": sat_rps, "saturated_W_s": W / dt_sat, "sequential_W_s": W / dt_seq,
+            "served_vs_sequential": dt_seq / dt_sat, "max_inflight": max_inflight, "streams": nstreams,
+            "device_budget_GiB": device_bytes / 2**30, "device_peak_GiB": st["device_peak"] / 2**30,
+            "runs": runs,
+            "p50_ms": runs[0]["p50_ms"] if len(runs) == 1 else None,
+            "p99_ms": runs[0]["p99_ms"] if len(runs) == 1 else None}
+
+
+def serve(F, args):
+    """Serving workload: resident 128^3 and 256^3 loop cubes (a spectroscopic and an imaging
+    emission cube each) and a corona for Thomson requests; requests drawn uniformly from the size
+    mix, Poisson arrivals at 0.5, 0.8 and 0.95 x the saturated request rate."""
+    tmp = tempfile.mkdtemp()
+    path = F.gen_goft_table(os.path.join(tmp, "fe9.dat"), ion="fe_9", lambda0=171.0, atweight=56.0, peak_logt=6.2)
+    tab = F.Table.from_file(path)
+    lambda0, atw = 171.0, 56.0
+    width = 0.2 * C / lambda0
+    objs, targets = [], []
+    for m, seed in ((128, 4), (256, 2)):
+        d = F.gen_loop_cube(m, seed=seed)
+        ds = F.Dataset(d["x"], d["y"], d["z"], [d["rho"], d["T"], d["vx"], d["vy"], d["vz"]])
+        es = F.Emission.chianti(ds, tab, lambda0, atw)
+        ei = F.Emission.chianti(ds, tab, lambda0, atw, imaging=True)
+        objs += [es, ei, ds]
+        targets.append({"spec": es, "imag": ei, "weight": 0.45, "name": f"{m}^3 loop (NN)"})
+    c = F.gen_corona(args.corona_m if args.corona_m != 645 else 128, seed=3, sigma=0.1)
+    dsc = F.Dataset(c["x"], c["y"], c["z"], [c["rho"]])
+    et = F.Emission.thomson(dsc)
+    objs += [et, dsc]
+    targets.append({"thomson": et, "weight": 0.10, "name": f"corona {c['x'].size} pts (Thomson)"})
+    log("resident: " + ", ".join(t["name"] for t in targets))
+    rng = np.random.default_rng(11)
+    res = serve_measure(F, targets, width, rng, args.requests, [0.5, 0.8, 0.95])
+    mid = [r for r in res["runs"] if r["load"] == 0.8][0]
+    out = {"metric": "render latency p50/p99 under Poisson load (continuous batching server)",
+           "value": res["saturated_W_s"], "unit": "LOS samples*lambda/s", "n_gpus": 1, "steps": args.requests,
+           "warmup": 8, "ms_per_step": 1e3 / res["saturated_req_s"], "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+           "config": {"workload": "serving mix: " + res["mix"]},
+           "p50_ms_at_0.8": mid["p50_ms"], "p99_ms_at_0.8": mid["p99_ms"], "serving": res}
+    print(json.dumps(out), flush=True)
+    for o in objs:
+        o.close()
+    tab.close()
     F.shutdown()
 
 
@@ -239,7 +388,7 @@ def thomson(F, args):
     F.profile_enable(True)
     F.render_thomson(em, ls[1], b, nx, ny, nz)
     F.profile_enable(False)
-    st = {k: F.profile_stats(k) for k in ("k_view_rotate_thomson", "k_nn", "k_nn_rays", "k_los_thomson")}
+    st = {k: F.profile_stats(k) for k in ("k_view_rotate_thomson", "k_nn", "k_nn_far", "k_los_thomson")}
     log("per-kernel (count, ms): " + json.dumps(st))
     out = {"metric": "LOS samples per second, Thomson pB render (16 views)", "value": W / dt,
            "unit": "LOS samples/s", "n_gpus": 1, "steps": nv, "warmup": args.warmup, "ms_per_step": dt / nv * 1e3,

@@ -381,9 +381,15 @@ class Server:
         _check(lib().fomo_server_footprint(em.id, method, nx, ny, nz, nl, ctypes.byref(d), ctypes.byref(h)))
         return d.value, h.value
 
-    def submit(self, em, method, l, b, nx, ny, nz, nl=1, lambda_width=0.0, dn=False):
+    def submit(self, em, method, l, b, nx, ny, nz, nl=1, lambda_width=0.0, dn=False, out=None):
+        """out: optional float32 array of ny*nx*max(nl,1) values to receive the image."""
         nlo = 1 if method == METHOD_THOMSON else max(nl, 1)
-        img = np.zeros(nx * ny * nlo, np.float32)
+        if out is None:
+            img = np.empty(nx * ny * nlo, np.float32)
+        else:
+            img = out
+            if img.dtype != np.float32 or not img.flags.c_contiguous or img.size != nx * ny * nlo:
+                raise ValueError("out must be a contiguous float32 array of ny*nx*max(nl,1) values")
         meta = np.zeros(10, np.float64)
         t = ctypes.c_int64()
         _check(lib().fomo_server_submit(self.id, em.id, method, l, b, nx, ny, nz, nl, lambda_width, 1 if dn else 0,

@@ -37,6 +37,7 @@
 
 #include "../../include/fomo_hip.h"
 #include "arena.h"
+#include "octree.h"
 
 // ------------------------------------------------------------------------------------
 // constants (GSL CODATA-2006 values used by fomo-CHIANTI.cpp:17-19)
@@ -656,17 +657,21 @@ __device__ __forceinline__ int nn_lookup(const float4* __restrict__ pts, const i
 }
 
 // ------------------------------------------------------------------------------------
-// Implicit BVH over the points in Morton order (exact fallback of the NN query).
-// Level 0 nodes bound 8 consecutive Morton-ordered points, level k nodes bound 8 level k-1
-// nodes. The query is the oracle's kd-tree query: best-first descent, a node is skipped when no
-// point of it can be strictly inside the search box or when its mindist exceeds the best distance.
+// Octree over the points in Morton order (exact fallback of the NN query), in depth-first
+// preorder. Every node covers a contiguous range of the Morton-sorted points; an internal node
+// splits its range by the highest 3-bit Morton group in which its codes differ (2..8 children),
+// a range of at most 8 points is a leaf. Layout per node i (two float4):
+//   nlo[i] = (box min, w), nhi[i] = (box max, first)
+//   w >= 0: internal node, first child at i+1, w = index just past its subtree (skip pointer);
+//   w <  0: leaf of -w points mpts[first .. first-w).
+// The query walks the array without a stack: a node whose box cannot hold a better point
+// strictly inside the search box is skipped with its subtree. Result: the nearest point strictly
+// inside the box, ties to the lowest original index (the reference's R-tree query,
+// fomo-nearestneighbour.cpp:179-183, fomo-thomson.cpp:310).
 // ------------------------------------------------------------------------------------
-constexpr int kBvhMaxLevels = 12;
 struct BvhInfo {
-  int levels;                 // number of node levels (level levels-1 is the root level)
-  int64_t n;                  // points
-  int64_t off[kBvhMaxLevels]; // node offset of each level in the node arrays
-  int64_t cnt[kBvhMaxLevels]; // nodes per level
+  int64_t nodes;  // 0: no tree
+  int64_t n;      // points
 };
 
 __global__ void k_morton_keys(const float4* __restrict__ pts, int64_t n, GridInfo g, uint32_t* __restrict__ key,
@@ -674,128 +679,87 @@ __global__ void k_morton_keys(const float4* __restrict__ pts, int64_t n, GridInf
   const int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (p >= n) return;
   const float4 q = pts[p];
-  const float c[3] = {q.x, q.y, q.z};
-  uint32_t code = 0;
-  uint32_t u[3];
-#pragma unroll
-  for (int a = 0; a < 3; ++a) {
-    const double ext = (double)g.bmax[a] - (double)g.bmin[a];
-    double t = ext > 0 ? ((double)c[a] - (double)g.bmin[a]) / ext : 0.0;
-    t = fmin(fmax(t, 0.0), 1.0);
-    u[a] = (uint32_t)fmin(t * 1024.0, 1023.0);
-  }
-#pragma unroll
-  for (int bit = 9; bit >= 0; --bit)
-    code = (code << 3) | (((u[0] >> bit) & 1u) << 2) | (((u[1] >> bit) & 1u) << 1) | ((u[2] >> bit) & 1u);
-  key[p] = code;
+  key[p] = fomo::morton30(q.x, q.y, q.z, g.bmin, g.bmax);
   val[p] = (int32_t)p;
 }
 
-// Morton-ordered points (x, y, z, sorted position) and level-0 boxes
-__global__ void k_bvh_leaves(const float4* __restrict__ pts, const int32_t* __restrict__ order, int64_t n,
-                             float4* __restrict__ mpts, float4* __restrict__ lo, float4* __restrict__ hi) {
-  const int64_t b = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  const int64_t nb = (n + 7) / 8;
-  if (b >= nb) return;
-  float4 l = make_float4(INFINITY, INFINITY, INFINITY, 0.f), h = make_float4(-INFINITY, -INFINITY, -INFINITY, 0.f);
-  for (int64_t i = b * 8; i < min(n, b * 8 + 8); ++i) {
-    const int32_t p = order[i];
-    const float4 q = pts[p];
-    mpts[i] = make_float4(q.x, q.y, q.z, __int_as_float(p));
-    l.x = fminf(l.x, q.x); l.y = fminf(l.y, q.y); l.z = fminf(l.z, q.z);
-    h.x = fmaxf(h.x, q.x); h.y = fmaxf(h.y, q.y); h.z = fmaxf(h.z, q.z);
-  }
-  lo[b] = l;
-  hi[b] = h;
+// Morton-ordered points (x, y, z, sorted position)
+__global__ void k_morton_points(const float4* __restrict__ pts, const int32_t* __restrict__ order, int64_t n,
+                                float4* __restrict__ mpts) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int32_t p = order[i];
+  const float4 q = pts[p];
+  mpts[i] = make_float4(q.x, q.y, q.z, __int_as_float(p));
 }
 
-__global__ void k_bvh_level(const float4* __restrict__ clo, const float4* __restrict__ chi, int64_t nchild,
-                            float4* __restrict__ lo, float4* __restrict__ hi) {
-  const int64_t b = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  const int64_t nb = (nchild + 7) / 8;
-  if (b >= nb) return;
-  float4 l = make_float4(INFINITY, INFINITY, INFINITY, 0.f), h = make_float4(-INFINITY, -INFINITY, -INFINITY, 0.f);
-  for (int64_t i = b * 8; i < min(nchild, b * 8 + 8); ++i) {
-    const float4 a = clo[i], c = chi[i];
-    l.x = fminf(l.x, a.x); l.y = fminf(l.y, a.y); l.z = fminf(l.z, a.z);
-    h.x = fmaxf(h.x, c.x); h.y = fmaxf(h.y, c.y); h.z = fmaxf(h.z, c.z);
-  }
-  lo[b] = l;
-  hi[b] = h;
-}
-
-__device__ __forceinline__ double node_mindist2(const float4 l, const float4 h, float t0, float t1, float t2) {
-  double s = 0.0, d;
-  d = t0 < l.x ? (double)l.x - (double)t0 : (t0 > h.x ? (double)t0 - (double)h.x : 0.0);
-  s = s + d * d;
-  d = t1 < l.y ? (double)l.y - (double)t1 : (t1 > h.y ? (double)t1 - (double)h.y : 0.0);
-  s = s + d * d;
-  d = t2 < l.z ? (double)l.z - (double)t2 : (t2 > h.z ? (double)t2 - (double)h.z : 0.0);
-  s = s + d * d;
-  return s;
-}
-
-// Nearest point strictly inside the box, ties -> lowest original index; returns the sorted
-// position or -1. Best-first with an explicit stack (level in the top 4 bits).
-// (best, bpos, borig) may carry a known candidate (a point strictly inside the box) as the start.
-__device__ int bvh_query(const float4* __restrict__ pts, const float4* __restrict__ mpts,
-                         const float4* __restrict__ nlo, const float4* __restrict__ nhi, const BvhInfo& bv,
-                         float t0, float t1, float t2, float lo0, float lo1, float lo2, float hi0, float hi1,
-                         float hi2, double best = INFINITY, int bpos = -1, int borig = 0x7fffffff) {
-  if (bv.levels <= 0) return bpos;
-  uint32_t stack[8 * kBvhMaxLevels + 8];
-  int sp = 0;
-  const int top = bv.levels - 1;
-  for (int64_t i = bv.cnt[top] - 1; i >= 0; --i) stack[sp++] = ((uint32_t)top << 28) | (uint32_t)i;
-  while (sp > 0) {
-    const uint32_t e = stack[--sp];
-    const int lev = (int)(e >> 28);
-    const int64_t idx = (int64_t)(e & 0x0fffffffu);
-    const float4 l = nlo[bv.off[lev] + idx], h = nhi[bv.off[lev] + idx];
-    if (!(h.x > lo0 && l.x < hi0 && h.y > lo1 && l.y < hi1 && h.z > lo2 && l.z < hi2)) continue;
-    if (node_mindist2(l, h, t0, t1, t2) > best) continue;
-    if (lev == 0) {
-      for (int64_t i = idx * 8; i < min(bv.n, idx * 8 + 8); ++i) {
-        const float4 q = mpts[i];
-        if (!(lo0 < q.x && q.x < hi0 && lo1 < q.y && q.y < hi1 && lo2 < q.z && q.z < hi2)) continue;
-        const double d0 = (double)q.x - (double)t0;
-        const double d1 = (double)q.y - (double)t1;
-        const double d2 = (double)q.z - (double)t2;
-        const double d = (d0 * d0 + d1 * d1) + d2 * d2;
-        const int p = __float_as_int(q.w);
-        if (d < best || d == best) {
-          const int oi = __float_as_int(pts[p].w);
-          if (d < best || oi < borig) {
-            best = d;
-            bpos = p;
-            borig = oi;
-          }
-        }
-      }
-    } else {
-      // children sorted by mindist; pushed farthest first so the nearest is popped first
-      const int64_t c0 = idx * 8, c1 = min(bv.cnt[lev - 1], idx * 8 + 8);
-      double key[8];
-      uint32_t ent[8];
-      int nc = 0;
-      for (int64_t c = c0; c < c1; ++c) {
-        const float4 cl = nlo[bv.off[lev - 1] + c], ch = nhi[bv.off[lev - 1] + c];
-        if (!(ch.x > lo0 && cl.x < hi0 && ch.y > lo1 && cl.y < hi1 && ch.z > lo2 && cl.z < hi2)) continue;
-        const double md = node_mindist2(cl, ch, t0, t1, t2);
-        if (md > best) continue;
-        int j = nc++;
-        while (j > 0 && key[j - 1] < md) {
-          key[j] = key[j - 1];
-          ent[j] = ent[j - 1];
-          --j;
-        }
-        key[j] = md;
-        ent[j] = ((uint32_t)(lev - 1) << 28) | (uint32_t)c;
-      }
-      for (int j = 0; j < nc; ++j) stack[sp++] = ent[j];
+// Boxes of the nodes listed in ids (all of one depth, deepest first): leaves from their points,
+// internal nodes from their children (already done).
+__global__ void k_oct_boxes(const int32_t* __restrict__ ids, int64_t cnt, const float4* __restrict__ mpts,
+                            float4* __restrict__ nlo, float4* __restrict__ nhi) {
+  const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (t >= cnt) return;
+  const int32_t i = ids[t];
+  const float4 a0 = nlo[i], b0 = nhi[i];
+  const int w = __float_as_int(a0.w);
+  float3 l = make_float3(INFINITY, INFINITY, INFINITY), h = make_float3(-INFINITY, -INFINITY, -INFINITY);
+  if (w < 0) {
+    const int first = __float_as_int(b0.w);
+    for (int q = first; q < first - w; ++q) {
+      const float4 v = mpts[q];
+      l.x = fminf(l.x, v.x); l.y = fminf(l.y, v.y); l.z = fminf(l.z, v.z);
+      h.x = fmaxf(h.x, v.x); h.y = fmaxf(h.y, v.y); h.z = fmaxf(h.z, v.z);
+    }
+  } else {
+    for (int c = i + 1; c < w;) {
+      const float4 cl = nlo[c], ch = nhi[c];
+      l.x = fminf(l.x, cl.x); l.y = fminf(l.y, cl.y); l.z = fminf(l.z, cl.z);
+      h.x = fmaxf(h.x, ch.x); h.y = fmaxf(h.y, ch.y); h.z = fmaxf(h.z, ch.z);
+      const int cw = __float_as_int(cl.w);
+      c = cw < 0 ? c + 1 : cw;
     }
   }
-  return bpos;
+  nlo[i] = make_float4(l.x, l.y, l.z, a0.w);
+  nhi[i] = make_float4(h.x, h.y, h.z, b0.w);
+}
+
+using fomo::SearchBox;
+
+__device__ __forceinline__ int bvh_query(const float4* __restrict__ pts, const float4* __restrict__ mpts,
+                                         const float4* __restrict__ nlo, const float4* __restrict__ nhi,
+                                         const BvhInfo& bv, const SearchBox& sb, double best = INFINITY,
+                                         int bpos = -1, int borig = 0x7fffffff) {
+  return fomo::oct_query(pts, mpts, nlo, nhi, bv.nodes, sb, best, bpos, borig);
+}
+
+// Per-sample outputs of the NN pass for the point at sorted position r (-1: none):
+// svlos[s] (LOS velocity, fomo-nearestneighbour.cpp:87-89) and sem[l*S + s] = (peak_l, fwhm_l),
+// zero when no point is inside the search box (:190-193).
+template <int RS4>
+__device__ __forceinline__ void write_sample(int r, const float4* __restrict__ rec, int nlines, size_t S, size_t sidx,
+                                             double cb, double sb, double cl, double sl, float* __restrict__ svlos,
+                                             float2* __restrict__ sem) {
+  float f[4 * RS4];
+#pragma unroll
+  for (int q = 0; q < 4 * RS4; ++q) f[q] = 0.f;
+  if (r >= 0) {
+    const float4* rp = rec + (size_t)r * RS4;
+#pragma unroll
+    for (int q = 0; q < RS4; ++q) {
+      const float4 a = rp[q];
+      f[4 * q] = a.x;
+      f[4 * q + 1] = a.y;
+      f[4 * q + 2] = a.z;
+      f[4 * q + 3] = a.w;
+    }
+  }
+  // unit vector (sin b cos l, -sin b sin l, cos b)
+  svlos[sidx] = r >= 0 ? (float)(((0.0 + (sb * cl) * (double)f[0]) + (-sb * sl) * (double)f[1]) + cb * (double)f[2])
+                       : 0.f;
+  // a record of RS4 float4 holds (4*RS4-3)/2 lines after the velocity
+#pragma unroll
+  for (int l = 0; l < (4 * RS4 - 3) / 2; ++l)
+    if (l < nlines) sem[(size_t)l * S + sidx] = make_float2(f[3 + 2 * l], f[4 + 2 * l]);
 }
 
 // One thread per LOS sample. Block = 16x16 rays at one k (blockIdx.y = k).
@@ -807,8 +771,6 @@ template <int RS4>
 __global__ void __launch_bounds__(256) k_nn(const float4* __restrict__ pts, const int32_t* __restrict__ cstart,
                                             GridInfo g, const int2* __restrict__ lcell,
                                             const float4* __restrict__ lcand, LookupInfo lk,
-                                            const float4* __restrict__ mpts, const float4* __restrict__ nlo,
-                                            const float4* __restrict__ nhi, BvhInfo bv,
                                             const ViewGeom* __restrict__ vgp, int nx, int ny, int nz,
                                             int32_t* __restrict__ idx, const float4* __restrict__ rec, int nlines,
                                             float* __restrict__ svlos, float2* __restrict__ sem,
@@ -842,110 +804,74 @@ __global__ void __launch_bounds__(256) k_nn(const float4* __restrict__ pts, cons
           lo2 < g.bmax[2]))
       r = -1;  // no data point can be strictly inside the box
     else
-      rayflag[(size_t)i * nx + j] = 1;  // left as -2 for k_nn_rays (exact BVH, warm-started along the ray)
+      rayflag[(size_t)i * nx + j] = 1;  // left as -2 for k_nn_far (exact octree query)
   }
   const size_t S = (size_t)nx * ny * nz;
   const size_t sidx = (size_t)k * ((size_t)nx * ny) + (size_t)i * nx + j;
   idx[sidx] = r;
-  if (RS4 > 0 && r != -2) {
-    float f[4 * (RS4 > 0 ? RS4 : 1)];
-#pragma unroll
-    for (int q = 0; q < 4 * (RS4 > 0 ? RS4 : 1); ++q) f[q] = 0.f;
-    if (r >= 0) {
-      const float4* rp = rec + (size_t)r * RS4;
-#pragma unroll
-      for (int q = 0; q < RS4; ++q) {
-        const float4 a = rp[q];
-        f[4 * q] = a.x;
-        f[4 * q + 1] = a.y;
-        f[4 * q + 2] = a.z;
-        f[4 * q + 3] = a.w;
-      }
-    }
-    // unit vector (sin b cos l, -sin b sin l, cos b)
-    svlos[sidx] = r >= 0 ? (float)(((0.0 + (sb * cl) * (double)f[0]) + (-sb * sl) * (double)f[1]) + cb * (double)f[2])
-                         : 0.f;
-    // a record of RS4 float4 holds (4*RS4-3)/2 lines after the velocity
-#pragma unroll
-    for (int l = 0; l < (4 * RS4 - 3) / 2; ++l)
-      if (l < nlines) sem[(size_t)l * S + sidx] = make_float2(f[3 + 2 * l], f[4 + 2 * l]);
+  if constexpr (RS4 > 0) {
+    if (r != -2) write_sample<RS4>(r, rec, nlines, S, sidx, cb, sb, cl, sl, svlos, sem);
   }
 }
 
-// Second NN pass, one thread per ray in LOS order: samples k_nn left unresolved (-2) get the
-// exact BVH query, started from the previous sample's point when it lies inside the box (a
-// valid upper bound, so the query only visits nodes near the answer). Same outputs as k_nn.
+// Second NN pass over the samples k_nn left unresolved (-2): exact octree query. Same grid as
+// k_nn; rays without such samples exit at once (rayflag, cleared afterwards by k_clear_u8).
+// The starting bound is the previous sample's point on the ray when k_nn resolved it and it lies
+// strictly inside this sample's box, else a greedy descent.
 template <int RS4>
-__global__ void __launch_bounds__(256) k_nn_rays(const float4* __restrict__ pts, const float4* __restrict__ mpts,
-                                                 const float4* __restrict__ nlo, const float4* __restrict__ nhi,
-                                                 BvhInfo bv, const ViewGeom* __restrict__ vgp, int nx, int ny,
-                                                 int nz, int32_t* __restrict__ idx, const float4* __restrict__ rec,
-                                                 int nlines, float* __restrict__ svlos, float2* __restrict__ sem,
-                                                 uint8_t* __restrict__ rayflag) {
-  const int64_t R = (int64_t)nx * ny;
-  const int64_t ray = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (ray >= R) return;
-  // only rays with a sample left unresolved by k_nn; the flag is cleared for the next view
+__global__ void __launch_bounds__(256) k_nn_far(const float4* __restrict__ pts, const float4* __restrict__ mpts,
+                                                const float4* __restrict__ nlo, const float4* __restrict__ nhi,
+                                                BvhInfo bv, const ViewGeom* __restrict__ vgp, int nx, int ny, int nz,
+                                                int32_t* __restrict__ idx, const float4* __restrict__ rec, int nlines,
+                                                float* __restrict__ svlos, float2* __restrict__ sem,
+                                                const uint8_t* __restrict__ rayflag) {
+  const int tilesx = (nx + 15) >> 4;
+  const int tx = blockIdx.x % tilesx, ty = blockIdx.x / tilesx;
+  const int j = tx * 16 + (threadIdx.x & 15);
+  const int i = ty * 16 + (threadIdx.x >> 4);
+  const int k = blockIdx.y;
+  if (j >= nx || i >= ny) return;
+  const size_t R = (size_t)nx * ny;
+  const size_t ray = (size_t)i * nx + j;
   if (!rayflag[ray]) return;
-  rayflag[ray] = 0;
-  const int j = (int)(ray % nx), i = (int)(ray / nx);
+  const size_t sidx = (size_t)k * R + ray;
+  if (idx[sidx] != -2) return;
   const ViewGeom vg = *vgp;
   const double minx = vg.ext[0], maxx = vg.ext[1], miny = vg.ext[2], maxy = vg.ext[3], minz = vg.ext[4];
-  const double xx = double(j) / (nx - 1) * (maxx - minx) + minx;
-  const double yy = double(i) / (ny - 1) * (maxy - miny) + miny;
+  const double xx = double(j) / (nx - 1) * (maxx - minx) + minx;  // fomo-nearestneighbour.cpp:152
+  const double yy = double(i) / (ny - 1) * (maxy - miny) + miny;  // :153
+  const double zz = double(k) * vg.deltaz + minz;                 // :162
   const double cb = vg.cb, sb = vg.sb, cl = vg.cl, sl = vg.sl, md = vg.md;
-  const size_t S = (size_t)R * nz;
-  int prev = -1;
-  for (int k = 0; k < nz; ++k) {
-    const size_t sidx = (size_t)k * R + ray;
-    int r = idx[sidx];
-    if (r != -2) {
-      if (r >= 0) prev = r;
-      continue;
-    }
-    const double zz = double(k) * vg.deltaz + minz;
-    const double p0 = xx * cb * cl + yy * sl + zz * sb * cl;
-    const double p1 = -xx * cb * sl + yy * cl - zz * sb * sl;
-    const double p2 = -xx * sb + zz * cb;
-    const float t0 = (float)p0, t1 = (float)p1, t2 = (float)p2;
-    const float lo0 = (float)(p0 - md), lo1 = (float)(p1 - md), lo2 = (float)(p2 - md);
-    const float hi0 = (float)(p0 + md), hi1 = (float)(p1 + md), hi2 = (float)(p2 + md);
-    double best = INFINITY;
-    int bpos = -1, borig = 0x7fffffff;
+  const double p0 = xx * cb * cl + yy * sl + zz * sb * cl;        // :165
+  const double p1 = -xx * cb * sl + yy * cl - zz * sb * sl;
+  const double p2 = -xx * sb + zz * cb;
+  SearchBox box;
+  box.t0 = (float)p0, box.t1 = (float)p1, box.t2 = (float)p2;
+  box.lo0 = (float)(p0 - md), box.lo1 = (float)(p1 - md), box.lo2 = (float)(p2 - md);
+  box.hi0 = (float)(p0 + md), box.hi1 = (float)(p1 + md), box.hi2 = (float)(p2 + md);
+  double best = INFINITY;
+  int bpos = -1, borig = 0x7fffffff;
+  if (k > 0) {
+    const int prev = idx[sidx - R];  // k_nn's result, or this pass's (any value >= 0 is a point)
     if (prev >= 0) {
       const float4 q = pts[prev];
-      if (lo0 < q.x && q.x < hi0 && lo1 < q.y && q.y < hi1 && lo2 < q.z && q.z < hi2) {
-        const double d0 = (double)q.x - (double)t0, d1 = (double)q.y - (double)t1, d2 = (double)q.z - (double)t2;
+      if (box.holds(q)) {
+        const double d0 = (double)q.x - (double)box.t0, d1 = (double)q.y - (double)box.t1,
+                     d2 = (double)q.z - (double)box.t2;
         best = (d0 * d0 + d1 * d1) + d2 * d2;
         bpos = prev;
         borig = __float_as_int(q.w);
       }
     }
-    r = bvh_query(pts, mpts, nlo, nhi, bv, t0, t1, t2, lo0, lo1, lo2, hi0, hi1, hi2, best, bpos, borig);
-    idx[sidx] = r;
-    if (r >= 0) prev = r;
-    if (RS4 > 0) {
-      float f[4 * (RS4 > 0 ? RS4 : 1)];
-#pragma unroll
-      for (int q = 0; q < 4 * (RS4 > 0 ? RS4 : 1); ++q) f[q] = 0.f;
-      if (r >= 0) {
-        const float4* rp = rec + (size_t)r * RS4;
-#pragma unroll
-        for (int q = 0; q < RS4; ++q) {
-          const float4 a = rp[q];
-          f[4 * q] = a.x;
-          f[4 * q + 1] = a.y;
-          f[4 * q + 2] = a.z;
-          f[4 * q + 3] = a.w;
-        }
-      }
-      svlos[sidx] =
-          r >= 0 ? (float)(((0.0 + (sb * cl) * (double)f[0]) + (-sb * sl) * (double)f[1]) + cb * (double)f[2]) : 0.f;
-#pragma unroll
-      for (int l = 0; l < (4 * RS4 - 3) / 2; ++l)
-        if (l < nlines) sem[(size_t)l * S + sidx] = make_float2(f[3 + 2 * l], f[4 + 2 * l]);
-    }
   }
+  const int r = bvh_query(pts, mpts, nlo, nhi, bv, box, best, bpos, borig);
+  idx[sidx] = r;
+  if constexpr (RS4 > 0) write_sample<RS4>(r, rec, nlines, R * nz, sidx, cb, sb, cl, sl, svlos, sem);
+}
+
+__global__ void k_clear_u8(uint8_t* __restrict__ a, int64_t n) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i < n) a[i] = 0;
 }
 
 // Per-point records for k_nn (sorted order): {vx,vy,vz, peak_0,fwhm_0, ...}, rs4 float4 each.
@@ -1315,11 +1241,12 @@ int build_bvh(Dataset& ds) {
   hipStream_t s = g_ctx.stream;
   const int64_t n = ds.n;
   ds.bvh = BvhInfo{};
+  if (n > (int64_t)INT32_MAX / 2) return fail(FOMO_ERR_ARG, "too many points for the octree");
   HIPOK(g_ctx.sortkeys[0].ensure(sizeof(uint32_t) * n));
   HIPOK(g_ctx.sortkeys[1].ensure(sizeof(uint32_t) * n));
   HIPOK(g_ctx.sortvals.ensure(sizeof(int32_t) * n));
   DevBuf order;
-  if (order.alloc(sizeof(int32_t) * n)) return fail(FOMO_ERR_NOMEM, "device allocation failed (bvh)");
+  if (order.alloc(sizeof(int32_t) * n)) return fail(FOMO_ERR_NOMEM, "device allocation failed (octree)");
   k_morton_keys<<<nblk(n), kBlock, 0, s>>>(ds.pts.as<float4>(), n, ds.grid, g_ctx.sortkeys[0].as<uint32_t>(),
                                            g_ctx.sortvals.as<int32_t>());
   HIPOK(hipGetLastError());
@@ -1330,32 +1257,46 @@ int build_bvh(Dataset& ds) {
   HIPOK(rocprim::radix_sort_pairs(g_ctx.sorttmp.p, tmpb, g_ctx.sortkeys[0].as<uint32_t>(),
                                   g_ctx.sortkeys[1].as<uint32_t>(), g_ctx.sortvals.as<int32_t>(), order.as<int32_t>(),
                                   (size_t)n, 0, 30, s));
-  BvhInfo bv{};
-  bv.n = n;
-  int64_t c = (n + 7) / 8, total = 0;
-  int lev = 0;
-  for (;;) {
-    if (lev >= kBvhMaxLevels) return fail(FOMO_ERR_ARG, "too many points for the BVH");
-    bv.off[lev] = total;
-    bv.cnt[lev] = c;
-    total += c;
-    ++lev;
-    if (c <= 8) break;
-    c = (c + 7) / 8;
+  if (ds.mpts.ensure(sizeof(float4) * n)) return fail(FOMO_ERR_NOMEM, "device allocation failed (octree)");
+  k_morton_points<<<nblk(n), kBlock, 0, s>>>(ds.pts.as<float4>(), order.as<int32_t>(), n, ds.mpts.as<float4>());
+  HIPOK(hipGetLastError());
+  std::vector<uint32_t> keys(n);
+  HIPOK(hipMemcpyAsync(keys.data(), g_ctx.sortkeys[1].p, sizeof(uint32_t) * n, hipMemcpyDeviceToHost, s));
+  if (int rc = sync_stream()) return rc;
+  fomo::OctBuilder ob;
+  ob.build(keys.data(), n);
+  const int64_t nodes = (int64_t)ob.w.size();
+  // initial node records (boxes filled on the device): nlo.w = skip / -count, nhi.w = first
+  std::vector<float4> lo(nodes), hi(nodes);
+  for (int64_t i = 0; i < nodes; ++i) {
+    lo[i] = make_float4(0.f, 0.f, 0.f, fomo::i2f(ob.w[i]));
+    hi[i] = make_float4(0.f, 0.f, 0.f, fomo::i2f(ob.first[i]));
   }
-  bv.levels = lev;
-  if (ds.mpts.ensure(sizeof(float4) * n) || ds.nlo.ensure(sizeof(float4) * total) ||
-      ds.nhi.ensure(sizeof(float4) * total))
-    return fail(FOMO_ERR_NOMEM, "device allocation failed (bvh)");
-  k_bvh_leaves<<<nblk(bv.cnt[0]), kBlock, 0, s>>>(ds.pts.as<float4>(), order.as<int32_t>(), n, ds.mpts.as<float4>(),
-                                                 ds.nlo.as<float4>(), ds.nhi.as<float4>());
-  for (int l = 1; l < bv.levels; ++l)
-    k_bvh_level<<<nblk(bv.cnt[l]), kBlock, 0, s>>>(ds.nlo.as<float4>() + bv.off[l - 1], ds.nhi.as<float4>() + bv.off[l - 1],
-                                                  bv.cnt[l - 1], ds.nlo.as<float4>() + bv.off[l],
-                                                  ds.nhi.as<float4>() + bv.off[l]);
+  std::vector<int32_t> ids;
+  std::vector<int64_t> doff;
+  for (int d = (int)ob.bydepth.size() - 1; d >= 0; --d) {
+    doff.push_back((int64_t)ids.size());
+    ids.insert(ids.end(), ob.bydepth[d].begin(), ob.bydepth[d].end());
+  }
+  doff.push_back((int64_t)ids.size());
+  DevBuf dids;
+  if (ds.nlo.alloc(sizeof(float4) * std::max<int64_t>(nodes, 1)) ||
+      ds.nhi.alloc(sizeof(float4) * std::max<int64_t>(nodes, 1)) ||
+      dids.alloc(sizeof(int32_t) * std::max<size_t>(ids.size(), 1)))
+    return fail(FOMO_ERR_NOMEM, "device allocation failed (octree)");
+  HIPOK(hipMemcpyAsync(ds.nlo.p, lo.data(), sizeof(float4) * nodes, hipMemcpyHostToDevice, s));
+  HIPOK(hipMemcpyAsync(ds.nhi.p, hi.data(), sizeof(float4) * nodes, hipMemcpyHostToDevice, s));
+  HIPOK(hipMemcpyAsync(dids.p, ids.data(), sizeof(int32_t) * ids.size(), hipMemcpyHostToDevice, s));
+  for (size_t d = 0; d + 1 < doff.size(); ++d) {
+    const int64_t cnt = doff[d + 1] - doff[d];
+    if (cnt > 0)
+      k_oct_boxes<<<nblk(cnt), kBlock, 0, s>>>(dids.as<int32_t>() + doff[d], cnt, ds.mpts.as<float4>(),
+                                              ds.nlo.as<float4>(), ds.nhi.as<float4>());
+  }
   HIPOK(hipGetLastError());
   if (int rc = sync_stream()) return rc;
-  ds.bvh = bv;
+  ds.bvh.nodes = nodes;
+  ds.bvh.n = n;
   return FOMO_OK;
 }
 
@@ -1599,8 +1540,9 @@ int rec_width(int nlines) {
 }
 
 // NN query for every sample of a view (idx: S ints, always written). With rec: per-sample vlos
-// and emissivities (nlines) too. Two passes: k_nn (lookup grid) and k_nn_rays (exact BVH for the rest).
-// rayflag: R bytes, zero on entry and on exit (set by k_nn, cleared by k_nn_rays).
+// and emissivities (nlines) too. Two passes: k_nn (lookup grid) and k_nn_far (exact octree query
+// for the rest).
+// rayflag: R bytes, zero on entry and on exit (set by k_nn, cleared by k_clear_u8).
 void enqueue_nn(hipStream_t s, const Dataset& ds, const ViewGeom* geom, int nx, int ny, int nz, int32_t* idx,
                 uint8_t* rayflag, const float4* rec = nullptr, int rs4 = 0, int nlines = 0, float* svlos = nullptr,
                 float2* sem = nullptr) {
@@ -1609,9 +1551,8 @@ void enqueue_nn(hipStream_t s, const Dataset& ds, const ViewGeom* geom, int nx, 
   const int64_t R = (int64_t)nx * ny;
   {
     ProfScope ps("k_nn", s);
-#define FOMO_NN_ARGS                                                                                           \
+#define FOMO_NN_ARGS                                                                                          \
   ds.pts.as<float4>(), ds.cstart.as<int32_t>(), ds.grid, ds.lcell.as<int2>(), ds.lcand.as<float4>(), ds.lk, \
-      ds.mpts.as<float4>(), ds.nlo.as<float4>(), ds.nhi.as<float4>(), ds.bvh,                                 \
       geom, nx, ny, nz, idx, rec, nlines, svlos, sem, rayflag
     if (!rec) k_nn<0><<<grid, 256, 0, s>>>(FOMO_NN_ARGS);
     else if (rs4 == 2) k_nn<2><<<grid, 256, 0, s>>>(FOMO_NN_ARGS);
@@ -1620,16 +1561,16 @@ void enqueue_nn(hipStream_t s, const Dataset& ds, const ViewGeom* geom, int nx, 
 #undef FOMO_NN_ARGS
   }
   {
-    ProfScope ps("k_nn_rays", s);
-#define FOMO_NNR_ARGS                                                                                          \
+    ProfScope ps("k_nn_far", s);
+#define FOMO_NNF_ARGS                                                                                         \
   ds.pts.as<float4>(), ds.mpts.as<float4>(), ds.nlo.as<float4>(), ds.nhi.as<float4>(), ds.bvh, geom, nx, ny, \
       nz, idx, rec, nlines, svlos, sem, rayflag
-    const unsigned nb = (unsigned)((R + 255) / 256);
-    if (!rec) k_nn_rays<0><<<nb, 256, 0, s>>>(FOMO_NNR_ARGS);
-    else if (rs4 == 2) k_nn_rays<2><<<nb, 256, 0, s>>>(FOMO_NNR_ARGS);
-    else if (rs4 == 4) k_nn_rays<4><<<nb, 256, 0, s>>>(FOMO_NNR_ARGS);
-    else k_nn_rays<8><<<nb, 256, 0, s>>>(FOMO_NNR_ARGS);
-#undef FOMO_NNR_ARGS
+    if (!rec) k_nn_far<0><<<grid, 256, 0, s>>>(FOMO_NNF_ARGS);
+    else if (rs4 == 2) k_nn_far<2><<<grid, 256, 0, s>>>(FOMO_NNF_ARGS);
+    else if (rs4 == 4) k_nn_far<4><<<grid, 256, 0, s>>>(FOMO_NNF_ARGS);
+    else k_nn_far<8><<<grid, 256, 0, s>>>(FOMO_NNF_ARGS);
+#undef FOMO_NNF_ARGS
+    k_clear_u8<<<nblk(R), kBlock, 0, s>>>(rayflag, R);
   }
 }
 
@@ -1791,6 +1732,7 @@ void geom_to_meta(const ViewGeom& hg, double* meta) {
 // counts), and every buffer it writes belongs to the request.
 // ---------------------------------------------------------------------------------------
 constexpr int kMaxBypass = 16;
+constexpr int kCopyThreads = 6;  // host threads copying finished images out of pinned staging
 enum { kTicketQueued = 0, kTicketRunning = 1, kTicketDone = 2 };
 
 struct Ticket {
@@ -1810,7 +1752,24 @@ struct Ticket {
   int stream = -1;
   int bypassed = 0;
   hipEvent_t done = nullptr;
+  std::atomic<int> chunks_left{0};  // copy-out chunks not yet copied
+  Ticket() = default;
+  Ticket(Ticket&& o) noexcept
+      : ds(o.ds), gc(o.gc), mode(o.mode), l(o.l), b(o.b), width(o.width), nx(o.nx), ny(o.ny), nz(o.nz),
+        nl(o.nl), dn(o.dn), out(o.out), meta(o.meta), state(o.state), err(std::move(o.err)),
+        t_submit(o.t_submit), t_admit(o.t_admit), t_done(o.t_done), dev_bytes(o.dev_bytes),
+        host_bytes(o.host_bytes), dev_off(o.dev_off), host_off(o.host_off), stream(o.stream),
+        bypassed(o.bypassed), done(o.done) {
+    std::memcpy(sz, o.sz, sizeof(sz));
+  }
 };
+
+// One piece of a finished request's image copy (pinned staging -> caller's buffer).
+struct CopyJob {
+  Ticket* t;
+  int64_t off, len;
+};
+constexpr int64_t kCopyChunk = 4 << 20;
 
 int64_t footprint_device(const int64_t sz[kRenderBufs]) {
   int64_t t = 0;
@@ -1835,6 +1794,10 @@ struct Server {
   std::deque<int64_t> pending;
   std::vector<int64_t> inflight;
   std::deque<Ticket> tickets;  // ticket id = position (references stay valid on push_back)
+  std::deque<CopyJob> copyq;   // copy-out work for the copier threads
+  std::condition_variable cv_copy;
+  std::vector<std::thread> copiers;
+  int64_t copying = 0;         // requests whose image is being copied out
   bool stop = false;
   int64_t completed = 0, failed = 0, peak_inflight = 0;
   std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now();
@@ -1843,6 +1806,7 @@ struct Server {
   double now() const {
     return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   }
+  bool idle() const { return pending.empty() && inflight.empty() && copying == 0; }
   ~Server() {
     for (auto e : evpool) (void)hipEventDestroy(e);
     for (auto& t : tickets)
@@ -1944,58 +1908,80 @@ bool server_admit(Server& sv) {
   return any;
 }
 
-// One pass of the worker: retire finished requests, then admit. Returns whether anything changed.
+// Last step of a request (server lock held): geometry to meta, free the staging block, done.
+void server_finish(Server& sv, Ticket& t) {
+  if (t.meta) {
+    ViewGeom hg;
+    std::memcpy(&hg, static_cast<const char*>(sv.hbase) + t.host_off + fomo::Arena::round_up(t.sz[4]),
+                sizeof(ViewGeom));
+    geom_to_meta(hg, t.meta);
+  }
+  sv.host.release(t.host_off);
+  t.state = kTicketDone;
+  t.t_done = sv.now();
+  ticket_release_objects(t);
+  ++sv.completed;
+  --sv.copying;
+  sv.cv_done.notify_all();
+  sv.cv_work.notify_one();  // the staging block may let a queued request in
+}
+
+// One pass of the worker: retire requests whose device work is done (their device block is freed
+// at once; the image copy goes to the copier threads), then admit. Returns whether anything changed.
 bool server_step(Server& sv) {
-  // pointers, not ids: the deque's index map may change under a concurrent submit
-  std::vector<Ticket*> fin;
-  {
-    std::lock_guard<std::mutex> lk(sv.mu);
-    for (size_t i = 0; i < sv.inflight.size();) {
-      Ticket& t = sv.tickets[sv.inflight[i]];
-      const hipError_t q = hipEventQuery(t.done);
-      if (q == hipErrorNotReady) {
-        ++i;
-        continue;
-      }
-      if (q != hipSuccess) {
-        t.err = std::string("request failed on the device: ") + hipGetErrorString(q);
-        t.state = FOMO_ERR_HIP;
-      }
-      fin.push_back(&t);
-      sv.inflight[i] = sv.inflight.back();
-      sv.inflight.pop_back();
-    }
-  }
-  // copy results out without the lock (these tickets and their staging blocks are ours now)
-  for (Ticket* t : fin) {
-    if (t->state != kTicketRunning) continue;
-    const char* h = static_cast<const char*>(sv.hbase) + t->host_off;
-    if (t->out) std::memcpy(t->out, h, (size_t)t->sz[4]);
-    if (t->meta) {
-      ViewGeom hg;
-      std::memcpy(&hg, h + fomo::Arena::round_up(t->sz[4]), sizeof(ViewGeom));
-      geom_to_meta(hg, t->meta);
-    }
-  }
   std::lock_guard<std::mutex> lk(sv.mu);
-  for (Ticket* t : fin) {
-    sv.dev.release(t->dev_off);
-    sv.host.release(t->host_off);
-    sv.load[t->stream]--;
-    sv.evpool.push_back(t->done);
-    t->done = nullptr;
-    if (t->state == kTicketRunning) {
-      t->state = kTicketDone;
-      t->t_done = sv.now();
-      ticket_release_objects(*t);
-      ++sv.completed;
-    } else {
-      server_fail_ticket(sv, *t, t->state, t->err);
+  bool changed = false;
+  for (size_t i = 0; i < sv.inflight.size();) {
+    Ticket& t = sv.tickets[sv.inflight[i]];
+    const hipError_t q = hipEventQuery(t.done);
+    if (q == hipErrorNotReady) {
+      ++i;
+      continue;
+    }
+    changed = true;
+    sv.inflight[i] = sv.inflight.back();
+    sv.inflight.pop_back();
+    sv.dev.release(t.dev_off);
+    sv.load[t.stream]--;
+    sv.evpool.push_back(t.done);
+    t.done = nullptr;
+    if (q != hipSuccess) {
+      sv.host.release(t.host_off);
+      server_fail_ticket(sv, t, FOMO_ERR_HIP, std::string("request failed on the device: ") + hipGetErrorString(q));
+      sv.cv_done.notify_all();
+      continue;
+    }
+    ++sv.copying;
+    const int64_t bytes = t.out ? t.sz[4] : 0;
+    const int n = (int)std::max<int64_t>(1, (bytes + kCopyChunk - 1) / kCopyChunk);
+    t.chunks_left.store(n);
+    for (int c = 0; c < n; ++c)
+      sv.copyq.push_back({&t, c * kCopyChunk, std::min<int64_t>(kCopyChunk, bytes - c * kCopyChunk)});
+    sv.cv_copy.notify_all();
+  }
+  if (server_admit(sv)) changed = true;
+  return changed;
+}
+
+void server_copier(Server* sv) {
+  for (;;) {
+    CopyJob j;
+    {
+      std::unique_lock<std::mutex> lk(sv->mu);
+      sv->cv_copy.wait(lk, [&] { return !sv->copyq.empty() || (sv->stop && sv->idle()); });
+      if (sv->copyq.empty()) return;
+      j = sv->copyq.front();
+      sv->copyq.pop_front();
+    }
+    if (j.len > 0)
+      std::memcpy(reinterpret_cast<char*>(j.t->out) + j.off,
+                  static_cast<const char*>(sv->hbase) + j.t->host_off + j.off, (size_t)j.len);
+    if (j.t->chunks_left.fetch_sub(1) == 1) {
+      std::lock_guard<std::mutex> lk(sv->mu);
+      server_finish(*sv, *j.t);
+      if (sv->stop && sv->idle()) sv->cv_copy.notify_all();
     }
   }
-  if (!fin.empty()) sv.cv_done.notify_all();
-  const bool admitted = server_admit(sv);
-  return admitted || !fin.empty();
 }
 
 void server_loop(Server* sv) {
@@ -2005,6 +1991,7 @@ void server_loop(Server* sv) {
       std::unique_lock<std::mutex> lk(sv->mu);
       if (sv->pending.empty() && sv->inflight.empty()) {
         if (sv->stop) return;
+        // copies in progress free staging blocks and wake this thread through cv_work
         sv->cv_work.wait(lk, [&] { return sv->stop || !sv->pending.empty(); });
         continue;
       }
@@ -2027,6 +2014,12 @@ void server_shutdown(Server& sv) {
     sv.cv_done.notify_all();
   }
   if (sv.worker.joinable()) sv.worker.join();
+  {
+    std::lock_guard<std::mutex> lk(sv.mu);
+    sv.cv_copy.notify_all();
+  }
+  for (auto& c : sv.copiers)
+    if (c.joinable()) c.join();
 }
 
 std::shared_ptr<Server> find_server(int id) {
@@ -2694,6 +2687,7 @@ int fomo_server_create(int64_t device_bytes, int64_t host_bytes, int max_infligh
     sv->load.push_back(0);
   }
   sv->worker = std::thread(server_loop, sv.get());
+  for (int i = 0; i < kCopyThreads; ++i) sv->copiers.emplace_back(server_copier, sv.get());
   const int id = g_ctx.next_id++;
   g_ctx.servers[id] = sv;
   *out_id = id;
@@ -2768,7 +2762,7 @@ int fomo_server_wait_all(int server, double timeout_s) {
   auto sv = find_server(server);
   if (!sv) return fail(FOMO_ERR_NOTFOUND, "unknown server id");
   std::unique_lock<std::mutex> lk(sv->mu);
-  auto idle = [&] { return sv->pending.empty() && sv->inflight.empty(); };
+  auto idle = [&] { return sv->idle(); };
   if (timeout_s < 0) sv->cv_done.wait(lk, idle);
   else if (!sv->cv_done.wait_for(lk, std::chrono::duration<double>(timeout_s), idle))
     return fail(FOMO_ERR_STATE, "timed out");
@@ -2797,7 +2791,7 @@ int fomo_server_stats(int server, double* st) {
   st[1] = (double)sv->completed;
   st[2] = (double)sv->failed;
   st[3] = (double)sv->pending.size();
-  st[4] = (double)sv->inflight.size();
+  st[4] = (double)(sv->inflight.size() + sv->copying);
   st[5] = (double)sv->peak_inflight;
   st[6] = (double)sv->dev.in_use();
   st[7] = (double)sv->dev.peak();

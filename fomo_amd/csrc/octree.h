@@ -1,0 +1,206 @@
+// Octree over Morton-ordered points for the exact nearest-neighbour query inside a search box.
+// Shared by the device kernels (fomo_device.hip) and a host unit test: the query is a template
+// over the array accessor type, so the same code runs on raw device pointers and on bounds-checked
+// host arrays.
+//
+// Layout: nodes in depth-first preorder; every node covers a contiguous range of the
+// Morton-sorted points. Node i is two 4-float records:
+//   lo[i] = (box min, w), hi[i] = (box max, first)
+//   w >= 0: internal node, first child at i+1, w = index just past its subtree (skip pointer);
+//   w <  0: leaf of -w points mpts[first .. first-w).
+// mpts[m] = (x, y, z, sorted position as int bits); pts[p].w = original index as int bits.
+//
+// The query returns the nearest point strictly inside the open search box, comparing squared
+// distances of float coordinates in double ((d0^2 + d1^2) + d2^2, no FMA), ties to the lowest
+// original index: the reference's R-tree query (fomo-nearestneighbour.cpp:179-183,
+// fomo-thomson.cpp:310) with its tie order fixed.
+#pragma once
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <vector>
+
+#if defined(__HIPCC__)
+#define FOMO_HD __host__ __device__
+#else
+#define FOMO_HD
+#endif
+
+namespace fomo {
+
+FOMO_HD inline int f2i(float f) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return __float_as_int(f);
+#else
+  int i;
+  std::memcpy(&i, &f, sizeof i);
+  return i;
+#endif
+}
+
+inline float i2f(int32_t v) {
+  float f;
+  std::memcpy(&f, &v, sizeof f);
+  return f;
+}
+
+struct SearchBox {
+  float t0, t1, t2;                    // target
+  float lo0, lo1, lo2, hi0, hi1, hi2;  // open search box
+  template <class V> FOMO_HD bool node_overlaps(const V& l, const V& h) const {
+    return h.x > lo0 && l.x < hi0 && h.y > lo1 && l.y < hi1 && h.z > lo2 && l.z < hi2;
+  }
+  template <class V> FOMO_HD bool holds(const V& q) const {
+    return lo0 < q.x && q.x < hi0 && lo1 < q.y && q.y < hi1 && lo2 < q.z && q.z < hi2;
+  }
+};
+
+template <class V>
+FOMO_HD inline double node_mindist2(const V& l, const V& h, float t0, float t1, float t2) {
+  double s = 0.0, d;
+  d = t0 < l.x ? (double)l.x - (double)t0 : (t0 > h.x ? (double)t0 - (double)h.x : 0.0);
+  s = s + d * d;
+  d = t1 < l.y ? (double)l.y - (double)t1 : (t1 > h.y ? (double)t1 - (double)h.y : 0.0);
+  s = s + d * d;
+  d = t2 < l.z ? (double)l.z - (double)t2 : (t2 > h.z ? (double)t2 - (double)h.z : 0.0);
+  s = s + d * d;
+  return s;
+}
+
+// Points of one leaf against the running best (sorted position bpos, original index borig).
+template <class A>
+FOMO_HD inline void leaf_scan(const A& pts, const A& mpts, int first, int count, const SearchBox& sb, double& best,
+                              int& bpos, int& borig) {
+  for (int i = first; i < first + count; ++i) {
+    const auto q = mpts[i];
+    if (!sb.holds(q)) continue;
+    const double d0 = (double)q.x - (double)sb.t0;
+    const double d1 = (double)q.y - (double)sb.t1;
+    const double d2 = (double)q.z - (double)sb.t2;
+    const double d = (d0 * d0 + d1 * d1) + d2 * d2;
+    if (d <= best) {
+      const int p = f2i(q.w);
+      const int oi = f2i(pts[p].w);
+      if (d < best || oi < borig) {
+        best = d;
+        bpos = p;
+        borig = oi;
+      }
+    }
+  }
+}
+
+// Exact NN strictly inside the search box. (best, bpos, borig) may carry a known point strictly
+// inside the box as the starting bound; without one (bpos < 0) a greedy descent (nearest child
+// box at each level) supplies the first bound. Then a stackless preorder walk: a node whose box
+// cannot hold a point strictly inside the search box at distance <= best is skipped with its
+// subtree. Returns the sorted position of the answer or -1.
+template <class A>
+FOMO_HD inline int oct_query(const A& pts, const A& mpts, const A& nlo, const A& nhi, int64_t nodes,
+                             const SearchBox& sb, double best, int bpos, int borig) {
+  if (nodes <= 0) return bpos;
+  if (bpos < 0) {
+    int e = 0;
+    int w = f2i(nlo[0].w);
+    while (w >= 0) {
+      int bc = -1;
+      double bm = INFINITY;
+      for (int c = e + 1; c < w;) {
+        const auto cl = nlo[c];
+        const auto ch = nhi[c];
+        if (sb.node_overlaps(cl, ch)) {
+          const double md = node_mindist2(cl, ch, sb.t0, sb.t1, sb.t2);
+          if (md < bm) {
+            bm = md;
+            bc = c;
+          }
+        }
+        const int cw = f2i(cl.w);
+        c = cw < 0 ? c + 1 : cw;
+      }
+      if (bc < 0) break;
+      e = bc;
+      w = f2i(nlo[e].w);
+    }
+    if (w < 0) leaf_scan(pts, mpts, f2i(nhi[e].w), -w, sb, best, bpos, borig);
+  }
+  const int n = (int)nodes;
+  for (int e = 0; e < n;) {
+    const auto l = nlo[e];
+    const auto h = nhi[e];
+    const int w = f2i(l.w);
+    const bool in = sb.node_overlaps(l, h) && node_mindist2(l, h, sb.t0, sb.t1, sb.t2) <= best;
+    if (w < 0) {
+      if (in) leaf_scan(pts, mpts, f2i(h.w), -w, sb, best, bpos, borig);
+      ++e;
+    } else {
+      e = in ? e + 1 : w;
+    }
+  }
+  return bpos;
+}
+
+// Host side: nodes in depth-first preorder from the sorted 30-bit Morton keys (a node is a
+// contiguous key range, so only the keys are needed). The children of a range are the runs of its
+// highest differing 3-bit group (2..8 runs); a range of identical keys splits into up to 8 equal
+// parts; a range of at most 8 points is a leaf.
+struct OctBuilder {
+  const uint32_t* key = nullptr;
+  std::vector<int32_t> w, first;              // per node: skip (internal) or -count (leaf); range start
+  std::vector<std::vector<int32_t>> bydepth;  // node ids per depth (boxes are filled deepest first)
+
+  void build(const uint32_t* sorted_keys, int64_t n) {
+    key = sorted_keys;
+    w.clear();
+    first.clear();
+    bydepth.clear();
+    if (n > 0) node(0, n, 27, 0);
+  }
+
+ private:
+  void node(int64_t s, int64_t e, int shift, int depth) {
+    const int32_t me = (int32_t)w.size();
+    w.push_back(0);
+    first.push_back((int32_t)s);
+    if ((int)bydepth.size() <= depth) bydepth.resize(depth + 1);
+    bydepth[depth].push_back(me);
+    if (e - s <= 8) {
+      w[me] = -(int32_t)(e - s);
+      return;
+    }
+    while (shift >= 0 && (key[s] >> shift) == (key[e - 1] >> shift)) shift -= 3;
+    if (shift < 0) {
+      const int64_t part = (e - s + 7) / 8;
+      for (int64_t a = s; a < e; a += part) node(a, std::min(e, a + part), -3, depth + 1);
+    } else {
+      for (int64_t a = s; a < e;) {
+        const uint32_t pre = key[a] >> shift;
+        const int64_t b =
+            std::upper_bound(key + a, key + e, pre, [shift](uint32_t v, uint32_t k) { return v < (k >> shift); }) - key;
+        node(a, b, shift - 3, depth + 1);
+        a = b;
+      }
+    }
+    w[me] = (int32_t)w.size();
+  }
+};
+
+// 30-bit Morton code of a point in the box [bmin, bmax] (10 bits per axis, x highest).
+FOMO_HD inline uint32_t morton30(float x, float y, float z, const float bmin[3], const float bmax[3]) {
+  const float c[3] = {x, y, z};
+  uint32_t u[3];
+  for (int a = 0; a < 3; ++a) {
+    const double ext = (double)bmax[a] - (double)bmin[a];
+    double t = ext > 0 ? ((double)c[a] - (double)bmin[a]) / ext : 0.0;
+    t = fmin(fmax(t, 0.0), 1.0);
+    u[a] = (uint32_t)fmin(t * 1024.0, 1023.0);
+  }
+  uint32_t code = 0;
+  for (int bit = 9; bit >= 0; --bit)
+    code = (code << 3) | (((u[0] >> bit) & 1u) << 2) | (((u[1] >> bit) & 1u) << 1) | ((u[2] >> bit) & 1u);
+  return code;
+}
+
+}  // namespace fomo
