@@ -11,7 +11,7 @@ import numpy as np
 
 __all__ = [
     "lib", "FomoError", "init", "shutdown", "device_info", "Dataset", "Table", "Emission", "Batch",
-    "render_nn", "render_thomson", "nn_indices", "Server", "METHOD_NN", "METHOD_THOMSON", "gen_loop_cube", "gen_goft_table", "gen_corona",
+    "render_nn", "render_thomson", "nn_indices", "Server", "METHOD_NN", "METHOD_THOMSON", "delaunay", "gen_loop_cube", "gen_goft_table", "gen_corona",
     "read_table", "profile_enable", "profile_reset", "profile_stats", "LIB_PATH",
 ]
 
@@ -67,6 +67,8 @@ _SIGS = {
     "fomo_server_wait_all": (ctypes.c_int, [ctypes.c_int, ctypes.c_double]),
     "fomo_server_times": (ctypes.c_int, [ctypes.c_int, ctypes.c_int64, _c_dbl_p]),
     "fomo_server_stats": (ctypes.c_int, [ctypes.c_int, _c_dbl_p]),
+    "fomo_delaunay": (ctypes.c_int, [ctypes.c_int64, _c_flt_p, _c_flt_p, _c_flt_p, ctypes.c_int64, _c_i64_p,
+                                     _c_i32_p]),
     "fomo_batch_create": (ctypes.c_int, [ctypes.c_int, _c_int_p, ctypes.c_int, _c_dbl_p, _c_dbl_p, ctypes.c_int,
                                          ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_int,
                                          _c_int_p]),
@@ -431,6 +433,20 @@ class Server:
             lib().fomo_server_destroy(self.id)
             self.id = None
             self._bufs.clear()
+
+
+def delaunay(x, y, z):
+    """Delaunay tets of the points (host builder): int32 array (ntets, 4) of point indices."""
+    (xa, xp), (ya, yp), (za, zp) = _f32(x), _f32(y), _f32(z)
+    n = xa.size
+    cnt = ctypes.c_int64()
+    cap = 8 * n + 64
+    while True:
+        out = np.zeros((cap, 4), np.int32)
+        _check(lib().fomo_delaunay(n, xp, yp, zp, cap, ctypes.byref(cnt), _ptr(out, ctypes.c_int32)))
+        if cnt.value <= cap:
+            return out[: cnt.value].copy()
+        cap = cnt.value
 
 
 def gen_loop_cube(m, seed=1, t_in=1.0e6, t_out=2.0e6, noise_sigma=0.05, vamp=2.0e4, jitter=0.0, sausage=0.0):

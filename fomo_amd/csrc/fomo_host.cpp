@@ -13,6 +13,7 @@
 #include <vector>
 
 #include "../../include/fomo_hip.h"
+#include "delaunay.h"
 
 namespace {
 
@@ -253,6 +254,23 @@ int fomo_gen_corona(int m, uint64_t seed, double sigma, int64_t* count, float* x
       w++;
     }
   });
+  return FOMO_OK;
+}
+
+// Delaunay triangulation of a point set on the host (the structure the tetra render walks).
+int fomo_delaunay(int64_t n, const float* x, const float* y, const float* z, int64_t cap, int64_t* ntets,
+                  int32_t* tets) {
+  if (n < 4 || !x || !y || !z || !ntets) return FOMO_ERR_ARG;
+  fomo::Delaunay dt;
+  if (!dt.build(x, y, z, n)) return FOMO_ERR_ARG;
+  int64_t m = 0;
+  for (const auto& t : dt.T) {
+    if (t.v[0] == fomo::Delaunay::kDead || !dt.finite(t)) continue;
+    if (tets && m < cap)
+      for (int k = 0; k < 4; ++k) tets[4 * m + k] = t.v[k];
+    ++m;
+  }
+  *ntets = m;
   return FOMO_OK;
 }
 

@@ -202,6 +202,13 @@ int fomo_gen_goft_table(const char* path, const char* ion, double lambda0, doubl
 int fomo_gen_corona(int m, uint64_t seed, double sigma, int64_t* count, float* x, float* y, float* z,
                     float* rho);
 
+/* Delaunay triangulation of n points (host-only; the structure of the tetra render path, which
+ * replaces the CGAL triangulation of fomo-CGAL.cpp:33-73). Writes up to cap finite tets as 4
+ * input point indices each (positively oriented: det[b-a; c-a; d-a] > 0) and stores the total
+ * count in *ntets. Duplicate points appear once, as their last input index. */
+int fomo_delaunay(int64_t n, const float* x, const float* y, const float* z, int64_t cap,
+                  int64_t* ntets, int32_t* tets);
+
 #ifdef __cplusplus
 }
 #endif

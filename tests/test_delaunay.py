@@ -1,0 +1,139 @@
+"""CPU tests of the host Delaunay builder (fomo_amd/csrc/delaunay.h, predicates.h), the structure
+of the tetra render path (replacing CGAL's Delaunay_triangulation_3, fomo-CGAL.cpp:33-73).
+
+* predicates: signs equal exact rational arithmetic on near-degenerate inputs;
+* general position: the tets equal scipy.spatial.Delaunay's (Qhull, an independent
+  implementation) as sets;
+* exact grids (every cube cospherical): positively oriented tets, volumes summing to the hull
+  volume, no point strictly inside any circumsphere;
+* duplicates: one vertex per location, carrying the last input index.
+"""
+import os
+import subprocess
+from fractions import Fraction
+
+import numpy as np
+import pytest
+from scipy.spatial import ConvexHull, Delaunay, cKDTree
+
+import fomo_amd as F
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _exact_orient(p):
+    a, b, c, d = [[Fraction(v) for v in q] for q in p]
+    m = [[b[i] - a[i] for i in range(3)], [c[i] - a[i] for i in range(3)], [d[i] - a[i] for i in range(3)]]
+    det = (m[0][0] * (m[1][1] * m[2][2] - m[1][2] * m[2][1]) - m[0][1] * (m[1][0] * m[2][2] - m[1][2] * m[2][0])
+           + m[0][2] * (m[1][0] * m[2][1] - m[1][1] * m[2][0]))
+    return (det > 0) - (det < 0)
+
+
+def _det(m):
+    n = len(m)
+    if n == 1:
+        return m[0][0]
+    return sum(((-1) ** j) * m[0][j] * _det([r[:j] + r[j + 1:] for r in m[1:]]) for j in range(n))
+
+
+def _exact_insphere5(p):
+    rows = []
+    for q in p:
+        x, y, z = (Fraction(v) for v in q)
+        rows.append([x, y, z, x * x + y * y + z * z, Fraction(1)])
+    d = _det(rows)
+    return (d > 0) - (d < 0)
+
+
+def test_predicates_match_exact_rationals(tmp_path):
+    exe = str(tmp_path / "pred_test")
+    subprocess.run(["g++", "-O2", "-std=c++17", "-ffp-contract=off", "-Wall", "-Werror", "-o", exe,
+                    os.path.join(ROOT, "tests", "cpp", "pred_test.cpp")], check=True)
+    rng = np.random.default_rng(5)
+    cases, want = [], []
+    for t in range(300):
+        if t % 2 == 0:
+            # nearly coplanar: d on the plane of a,b,c, then nudged by a few ulps (or exactly on it)
+            a, b, c = rng.normal(size=(3, 3)) * 10
+            u, v = rng.random(2)
+            d = a + u * (b - a) + v * (c - a)
+            if t % 4 == 0:
+                d = d + np.spacing(d) * rng.integers(-3, 4, 3)
+            pts = [a, b, c, d]
+            cases.append("O " + " ".join(repr(float(x)) for q in pts for x in q))
+            want.append(_exact_orient(pts))
+        else:
+            # nearly cospherical: 5 points on a sphere (rounded to double), some exactly on a grid
+            if t % 6 == 1:
+                pts = [np.array(q, float) for q in [(0, 0, 0), (1, 0, 0), (0, 1, 0), (0, 0, 1), (1, 1, 1)]]
+            else:
+                cen, r = rng.normal(size=3), 1 + rng.random()
+                dirs = rng.normal(size=(5, 3))
+                pts = list(cen + r * dirs / np.linalg.norm(dirs, axis=1)[:, None])
+            cases.append("S " + " ".join(repr(float(x)) for q in pts for x in q))
+            want.append(_exact_insphere5(pts))
+    r = subprocess.run([exe], input="\n".join(cases) + "\n", capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    got = [int(v) for v in r.stdout.split()]
+    assert got == want
+    assert 0 in want and 1 in want and -1 in want  # the degenerate and both signs are exercised
+
+
+@pytest.mark.parametrize("n,seed", [(10, 1), (300, 2), (20000, 3)])
+def test_general_position_equals_qhull(n, seed):
+    p = np.random.default_rng(seed).random((n, 3)).astype(np.float32)
+    tets = F.delaunay(p[:, 0], p[:, 1], p[:, 2])
+    ref = Delaunay(p.astype(np.float64)).simplices
+    assert set(map(tuple, np.sort(tets, 1))) == set(map(tuple, np.sort(ref, 1)))
+
+
+def _valid(p, tets):
+    P = p.astype(np.float64)
+    a, b, c, d = (P[tets[:, i]] for i in range(4))
+    vol = np.einsum("ij,ij->i", b - a, np.cross(c - a, d - a)) / 6
+    A = np.stack([b - a, c - a, d - a], 1)
+    rhs = 0.5 * np.stack([np.sum((b - a) ** 2, 1), np.sum((c - a) ** 2, 1), np.sum((d - a) ** 2, 1)], 1)
+    cen = np.linalg.solve(A, rhs[..., None])[..., 0] + a
+    rad = np.linalg.norm(cen - a, axis=1)
+    tree = cKDTree(P)
+    inside = sum(len(tree.query_ball_point(cen[i], rad[i] * (1 - 1e-9))) for i in range(len(tets)))
+    return vol.min(), vol.sum() / ConvexHull(P).volume, inside
+
+
+def test_jittered_grid_equals_qhull_and_is_valid():
+    g = np.stack(np.meshgrid(*[np.arange(9.0)] * 3, indexing="ij"), -1).reshape(-1, 3)
+    j = (g + np.random.default_rng(4).uniform(-0.3, 0.3, g.shape)).astype(np.float32)
+    tets = F.delaunay(j[:, 0], j[:, 1], j[:, 2])
+    ref = Delaunay(j.astype(np.float64)).simplices
+    assert set(map(tuple, np.sort(tets, 1))) == set(map(tuple, np.sort(ref, 1)))
+    vmin, vfrac, inside = _valid(j, tets)
+    assert vmin > 0 and abs(vfrac - 1) < 1e-12 and inside == 0
+
+
+def test_exact_grid_is_a_valid_delaunay_triangulation():
+    """Every unit cube is cospherical: the symbolic perturbation must still give non-flat,
+    positively oriented tets that tile the hull, with empty circumspheres."""
+    g = np.stack(np.meshgrid(np.arange(7.0), np.arange(6.0), np.arange(5.0), indexing="ij"), -1).reshape(-1, 3)
+    g = g.astype(np.float32)
+    tets = F.delaunay(g[:, 0], g[:, 1], g[:, 2])
+    vmin, vfrac, inside = _valid(g, tets)
+    assert vmin > 0 and abs(vfrac - 1) < 1e-12 and inside == 0
+    # insertion-order independence: a permuted input gives the same tets (as point sets)
+    perm = np.random.default_rng(0).permutation(len(g))
+    t2 = F.delaunay(g[perm, 0], g[perm, 1], g[perm, 2])
+    s1 = set(map(tuple, np.sort(tets, 1)))
+    s2 = set(map(tuple, np.sort(perm[t2], 1)))
+    assert s1 == s2
+
+
+def test_duplicates_keep_the_last_index():
+    p = np.random.default_rng(6).random((400, 3)).astype(np.float32)
+    d = np.concatenate([p, p[:50], p[:50]])  # points 0..49 appear three times
+    tets = F.delaunay(d[:, 0], d[:, 1], d[:, 2])
+    used = np.unique(tets)
+    assert not np.isin(np.arange(50), used).any() and not np.isin(np.arange(400, 450), used).any()
+    assert np.isin(np.arange(450, 500), used).all()
+    ref = F.delaunay(p[:, 0], p[:, 1], p[:, 2])
+    remap = np.arange(len(p))
+    remap[:50] = np.arange(450, 500)
+    assert set(map(tuple, np.sort(remap[ref], 1))) == set(map(tuple, np.sort(tets, 1)))
