@@ -11,7 +11,8 @@ import numpy as np
 
 __all__ = [
     "lib", "FomoError", "init", "shutdown", "device_info", "Dataset", "Table", "Emission", "Batch",
-    "render_nn", "render_thomson", "nn_indices", "Server", "METHOD_NN", "METHOD_THOMSON", "delaunay", "gen_loop_cube", "gen_goft_table", "gen_corona",
+    "render_nn", "render_thomson", "nn_indices", "Server", "METHOD_NN", "METHOD_THOMSON", "METHOD_TET", "METHOD_LINEAR", "delaunay", "render_tet",
+    "tet_vertices", "gen_loop_cube", "gen_goft_table", "gen_corona",
     "read_table", "profile_enable", "profile_reset", "profile_stats", "LIB_PATH",
 ]
 
@@ -69,6 +70,12 @@ _SIGS = {
     "fomo_server_stats": (ctypes.c_int, [ctypes.c_int, _c_dbl_p]),
     "fomo_delaunay": (ctypes.c_int, [ctypes.c_int64, _c_flt_p, _c_flt_p, _c_flt_p, ctypes.c_int64, _c_i64_p,
                                      _c_i32_p]),
+    "fomo_render_tet": (ctypes.c_int, [ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_int,
+                                       ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_int,
+                                       _c_flt_p, _c_dbl_p]),
+    "fomo_dataset_triangulate": (ctypes.c_int, [ctypes.c_int, _c_i64_p, _c_dbl_p]),
+    "fomo_tet_vertices": (ctypes.c_int, [ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_int,
+                                         ctypes.c_int, _c_i32_p, _c_dbl_p]),
     "fomo_batch_create": (ctypes.c_int, [ctypes.c_int, _c_int_p, ctypes.c_int, _c_dbl_p, _c_dbl_p, ctypes.c_int,
                                          ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_int,
                                          _c_int_p]),
@@ -211,6 +218,12 @@ class Dataset:
                                               ctypes.byref(fb)))
         return dict(subdiv=sd.value, cells=cells.value, candidates=cand.value, fallback_cells=fb.value)
 
+    def triangulate(self):
+        """Build the Delaunay triangulation for the tetra path; returns (tets, seconds)."""
+        nt, sec = ctypes.c_int64(), ctypes.c_double()
+        _check(lib().fomo_dataset_triangulate(self.id, ctypes.byref(nt), ctypes.byref(sec)))
+        return nt.value, sec.value
+
     def reindex(self, cell_scale=0.0):
         _check(lib().fomo_dataset_reindex(self.id, cell_scale))
 
@@ -312,6 +325,23 @@ def render_thomson(em, l, b, nx, ny, nz):
     return img, meta
 
 
+def render_tet(em, l, b, nx, ny, nz, nl, lambda_width, dn=False, linear=False):
+    """One view of the tetra path (nearest vertex of the containing Delaunay tet, or barycentric
+    with linear=True); returns (image[ny*nx*max(nl,1)] float32, meta[10])."""
+    meta = np.zeros(10, np.float64)
+    img = np.zeros(nx * ny * max(nl, 1), np.float32)
+    _check(lib().fomo_render_tet(em.id, l, b, nx, ny, nz, nl, lambda_width, 1 if dn else 0, 1 if linear else 0,
+                                 _ptr(img, ctypes.c_float), _ptr(meta, ctypes.c_double)))
+    return img, meta
+
+
+def tet_vertices(ds, l, b, nx, ny, nz):
+    out = np.zeros(nx * ny * nz, np.int32)
+    meta = np.zeros(10, np.float64)
+    _check(lib().fomo_tet_vertices(ds.id, l, b, nx, ny, nz, _ptr(out, ctypes.c_int32), _ptr(meta, ctypes.c_double)))
+    return out, meta
+
+
 def nn_indices(ds, l, b, nx, ny, nz, thomson=False):
     idx = np.zeros(nx * ny * nz, np.int32)
     meta = np.zeros(10, np.float64)
@@ -363,6 +393,8 @@ class Batch:
 
 METHOD_NN = 0
 METHOD_THOMSON = 1
+METHOD_TET = 2
+METHOD_LINEAR = 3
 
 
 class Server:

@@ -37,6 +37,7 @@
 
 #include "../../include/fomo_hip.h"
 #include "arena.h"
+#include "delaunay.h"
 #include "octree.h"
 
 // ------------------------------------------------------------------------------------
@@ -138,7 +139,7 @@ struct ViewGeom {
   double pathlength, apix, scale; // output scaling
 };
 
-enum { MODE_NN = 0, MODE_THOMSON = 1 };
+enum { MODE_NN = 0, MODE_THOMSON = 1, MODE_TET = 2, MODE_LINEAR = 3 };
 
 // Spectral lines rendered together from one NN result (blockIdx.y selects the line).
 constexpr int kMaxLines = 8;
@@ -874,6 +875,170 @@ __global__ void k_clear_u8(uint8_t* __restrict__ a, int64_t n) {
   if (i < n) a[i] = 0;
 }
 
+// ------------------------------------------------------------------------------------
+// Tetra path (fomo-CGAL.cpp:75-307): per ray, walk the Delaunay tets from sample to sample; the
+// sample takes the values of the nearest vertex of its containing tet (nearest_vertex_in_cell,
+// :211-222), or of none outside the convex hull (:225-228). MODE_LINEAR (an extension) takes the
+// barycentric interpolation of the tet's vertex values instead.
+// Orientation tests are FP64 with a conservative filter; an undecided test (a sample on a face
+// plane to within 1e-12 of the determinant's magnitude) counts as "not beyond" the face, so a
+// sample on a shared face stays in the tet the walk reached first.
+// ------------------------------------------------------------------------------------
+__constant__ int c_face[4][3] = {{2, 1, 3}, {0, 2, 3}, {1, 0, 3}, {0, 1, 2}};
+constexpr int kTetMaxSteps = 1 << 16;
+
+__device__ __forceinline__ int orient_d(const float4 a, const float4 b, const float4 c, double px, double py,
+                                        double pz, double* det_out = nullptr) {
+  const double ax = a.x, ay = a.y, az = a.z;
+  const double bx = (double)b.x - ax, by = (double)b.y - ay, bz = (double)b.z - az;
+  const double cx = (double)c.x - ax, cy = (double)c.y - ay, cz = (double)c.z - az;
+  const double dx = px - ax, dy = py - ay, dz = pz - az;
+  const double m1 = cy * dz - cz * dy, m2 = cx * dz - cz * dx, m3 = cx * dy - cy * dx;
+  const double det = bx * m1 - by * m2 + bz * m3;
+  if (det_out) *det_out = det;
+  const double perm = fabs(bx) * (fabs(cy * dz) + fabs(cz * dy)) + fabs(by) * (fabs(cx * dz) + fabs(cz * dx)) +
+                      fabs(bz) * (fabs(cx * dy) + fabs(cy * dx));
+  const double bound = 1e-12 * perm;
+  return det > bound ? 1 : (det < -bound ? -1 : 0);
+}
+
+__device__ __forceinline__ int ivec(const int4 v, int i) { return i == 0 ? v.x : (i == 1 ? v.y : (i == 2 ? v.z : v.w)); }
+
+// One thread per ray. Writes per-sample svlos / sem like k_nn (RS4 > 0) and, when vert != null,
+// the chosen vertex (sorted position, -1 outside) per sample in k-major order.
+template <int RS4, bool LINEAR>
+__global__ void __launch_bounds__(256) k_tet_walk(const float4* __restrict__ pts, const int4* __restrict__ tv,
+                                                  const int4* __restrict__ tn, const int32_t* __restrict__ tstart,
+                                                  LookupInfo tk, const ViewGeom* __restrict__ vgp, int nx, int ny,
+                                                  int nz, const float4* __restrict__ rec, int nlines,
+                                                  float* __restrict__ svlos, float2* __restrict__ sem,
+                                                  int32_t* __restrict__ vert) {
+  const int64_t R = (int64_t)nx * ny;
+  const int64_t ray = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (ray >= R) return;
+  const int j = (int)(ray % nx), i = (int)(ray / nx);
+  const ViewGeom vg = *vgp;
+  const double minx = vg.ext[0], maxx = vg.ext[1], miny = vg.ext[2], maxy = vg.ext[3], minz = vg.ext[4],
+               maxz = vg.ext[5];
+  const double x = double(j) / (nx - 1) * (maxx - minx) + minx;  // fomo-CGAL.cpp:187
+  const double y = double(i) / (ny - 1) * (maxy - miny) + miny;  // :188
+  const double cb = vg.cb, sb = vg.sb, cl = vg.cl, sl = vg.sl;
+  const size_t S = (size_t)R * nz;
+  int t = -1;
+  for (int k = 0; k < nz; ++k) {
+    // :189 (a single z pixel has no spacing: the sample sits at minz)
+    const double z = nz > 1 ? double(k) / (nz - 1) * (maxz - minz) + minz : minz;
+    const double px = x * cb * cl + y * sl + z * sb * cl;  // :192
+    const double py = -x * cb * sl + y * cl - z * sb * sl;
+    const double pz = -x * sb + z * cb;
+    if (t < 0) {
+      int c[3];
+      const double pc[3] = {px, py, pz};
+#pragma unroll
+      for (int a = 0; a < 3; ++a) {
+        const double v = (pc[a] - tk.o[a]) * tk.invh;
+        c[a] = v < 0 ? 0 : (v >= tk.dims[a] ? tk.dims[a] - 1 : (int)v);
+      }
+      t = tstart[((int64_t)c[2] * tk.dims[1] + c[1]) * tk.dims[0] + c[0]];
+    }
+    bool outside = false;
+    int steps = 0;
+    int4 v;
+    float4 q[4];
+    for (;;) {
+      v = tv[t];
+      const int4 nb = tn[t];
+      int inf = -1;
+#pragma unroll
+      for (int m = 0; m < 4; ++m)
+        if (ivec(v, m) < 0) inf = m;
+      if (inf >= 0) {
+        const float4 a = pts[ivec(v, c_face[inf][0])], b = pts[ivec(v, c_face[inf][1])],
+                     c = pts[ivec(v, c_face[inf][2])];
+        if (orient_d(a, b, c, px, py, pz) > 0) {
+          outside = true;
+          break;
+        }
+        t = ivec(nb, inf);
+      } else {
+#pragma unroll
+        for (int m = 0; m < 4; ++m) q[m] = pts[ivec(v, m)];
+        int next = -1;
+#pragma unroll
+        for (int f = 0; f < 4; ++f)
+          if (next < 0 && orient_d(q[c_face[f][0]], q[c_face[f][1]], q[c_face[f][2]], px, py, pz) < 0)
+            next = ivec(nb, f);
+        if (next < 0) break;
+        t = next;
+      }
+      if (++steps > kTetMaxSteps) {  // never expected (the walk on a Delaunay triangulation ends)
+        outside = true;
+        break;
+      }
+    }
+    const size_t sidx = (size_t)k * R + ray;
+    if (outside) {
+      if (vert) vert[sidx] = -1;
+      if constexpr (RS4 > 0) write_sample<RS4>(-1, rec, nlines, S, sidx, cb, sb, cl, sl, svlos, sem);
+      continue;
+    }
+    if constexpr (!LINEAR) {
+      // nearest vertex of the cell (first of equals in the cell's vertex order)
+      int best = 0;
+      double bd = INFINITY;
+#pragma unroll
+      for (int m = 0; m < 4; ++m) {
+        const double d0 = (double)q[m].x - px, d1 = (double)q[m].y - py, d2 = (double)q[m].z - pz;
+        const double d = (d0 * d0 + d1 * d1) + d2 * d2;
+        if (d < bd) {
+          bd = d;
+          best = m;
+        }
+      }
+      const int r = ivec(v, best);
+      if (vert) vert[sidx] = r;
+      if constexpr (RS4 > 0) write_sample<RS4>(r, rec, nlines, S, sidx, cb, sb, cl, sl, svlos, sem);
+    } else {
+      // barycentric weights: volume of the tet with vertex m replaced by the sample, over the volume
+      double w[4], tot = 0.0;
+#pragma unroll
+      for (int f = 0; f < 4; ++f) {
+        double det;
+        (void)orient_d(q[c_face[f][0]], q[c_face[f][1]], q[c_face[f][2]], px, py, pz, &det);
+        w[f] = fmax(det, 0.0);
+        tot += w[f];
+      }
+      if (vert) vert[sidx] = ivec(v, 0);
+      if constexpr (RS4 > 0) {
+        double vl = 0.0, pk[(4 * RS4 - 3) / 2], fw[(4 * RS4 - 3) / 2];
+#pragma unroll
+        for (int l = 0; l < (4 * RS4 - 3) / 2; ++l) pk[l] = fw[l] = 0.0;
+#pragma unroll
+        for (int m = 0; m < 4; ++m) {
+          const double wm = tot > 0 ? w[m] / tot : 0.25;
+          const float4* rp = rec + (size_t)ivec(v, m) * RS4;
+          float f[4 * RS4];
+#pragma unroll
+          for (int qq = 0; qq < RS4; ++qq) {
+            const float4 a = rp[qq];
+            f[4 * qq] = a.x, f[4 * qq + 1] = a.y, f[4 * qq + 2] = a.z, f[4 * qq + 3] = a.w;
+          }
+          vl += wm * (((0.0 + (sb * cl) * (double)f[0]) + (-sb * sl) * (double)f[1]) + cb * (double)f[2]);
+#pragma unroll
+          for (int l = 0; l < (4 * RS4 - 3) / 2; ++l) {
+            pk[l] += wm * (double)f[3 + 2 * l];
+            fw[l] += wm * (double)f[4 + 2 * l];
+          }
+        }
+        svlos[sidx] = (float)vl;
+#pragma unroll
+        for (int l = 0; l < (4 * RS4 - 3) / 2; ++l)
+          if (l < nlines) sem[(size_t)l * S + sidx] = make_float2((float)pk[l], (float)fw[l]);
+      }
+    }
+  }
+}
+
 // Per-point records for k_nn (sorted order): {vx,vy,vz, peak_0,fwhm_0, ...}, rs4 float4 each.
 __global__ void k_build_rec(const float4* __restrict__ vel, const LineSet ls, int64_t n, int rs4,
                             float4* __restrict__ rec) {
@@ -1095,6 +1260,12 @@ struct Dataset {
   DevBuf mpts;         // float4 (x, y, z, sorted position) in Morton order
   DevBuf nlo, nhi;     // node boxes, all levels
   int64_t ncand = 0, nlcells = 0, nfallback = 0;
+  // tetra path: Delaunay triangulation of the points, built on the host on first use
+  int64_t ntets = 0;      // 0: not built
+  DevBuf tv, tn;          // int4 per tet: vertices (sorted positions, -1 = infinite), neighbours
+  DevBuf tstart;          // int32 per start-grid cell: a finite tet near the cell centre
+  LookupInfo tk{};        // start grid geometry
+  double tet_build_s = 0;
   std::atomic<int> inuse{0};  // batches and queued/running server requests reading this dataset
 };
 
@@ -1300,6 +1471,96 @@ int build_bvh(Dataset& ds) {
   return FOMO_OK;
 }
 
+// Delaunay triangulation of the dataset for the tetra path (host builder, fomo/delaunay.h), then
+// upload: vertex ids become sorted positions; a start grid (up to 64^3 cells over the bbox) holds
+// a finite tet near each cell centre, located by a host walk, as the first tet of each ray's walk.
+int build_tets(Dataset& ds) {
+  if (ds.ntets > 0) return FOMO_OK;
+  hipStream_t s = g_ctx.stream;
+  const int64_t n = ds.n;
+  if (n < 4) return fail(FOMO_ERR_ARG, "the tetra path needs at least 4 points");
+  if (n > (int64_t)INT32_MAX / 8) return fail(FOMO_ERR_ARG, "too many points for the tetra path");
+  const auto t0 = std::chrono::steady_clock::now();
+  std::vector<float> hx(n), hy(n), hz(n);
+  std::vector<int32_t> perm(n);
+  HIPOK(hipMemcpyAsync(hx.data(), ds.raw[0].p, sizeof(float) * n, hipMemcpyDeviceToHost, s));
+  HIPOK(hipMemcpyAsync(hy.data(), ds.raw[1].p, sizeof(float) * n, hipMemcpyDeviceToHost, s));
+  HIPOK(hipMemcpyAsync(hz.data(), ds.raw[2].p, sizeof(float) * n, hipMemcpyDeviceToHost, s));
+  HIPOK(hipMemcpyAsync(perm.data(), ds.perm.p, sizeof(int32_t) * n, hipMemcpyDeviceToHost, s));
+  if (int rc = sync_stream()) return rc;
+  fomo::Delaunay dt;
+  if (!dt.build(hx.data(), hy.data(), hz.data(), n))
+    return fail(FOMO_ERR_ARG, "Delaunay triangulation failed: " + dt.error);
+  int64_t nfin = 0;
+  std::vector<fomo::Tet> T = dt.compact(&nfin);
+  std::vector<int32_t> sorted_of(n);
+  for (int64_t p = 0; p < n; ++p) sorted_of[perm[p]] = (int32_t)p;
+  // start grid: cells ~ 4 points each along an axis, at most 64 per axis
+  LookupInfo tk{};
+  double ext = 0;
+  for (int a = 0; a < 3; ++a) ext = std::max(ext, (double)ds.bbox[2 * a + 1] - ds.bbox[2 * a]);
+  const double h = std::max(ext / 64.0, ext > 0 ? ext * 1e-6 : 1.0);
+  for (int a = 0; a < 3; ++a) {
+    tk.o[a] = ds.bbox[2 * a];
+    tk.dims[a] = std::max(1, std::min(64, (int)std::ceil(((double)ds.bbox[2 * a + 1] - ds.bbox[2 * a]) / h)));
+  }
+  tk.h = h;
+  tk.invh = 1.0 / h;
+  const int64_t ng = (int64_t)tk.dims[0] * tk.dims[1] * tk.dims[2];
+  std::vector<int32_t> start(ng, 0);
+  {
+    // walk with exact predicates on the original-index tets
+    int32_t t = 0;
+    for (size_t q = 0; q < T.size(); ++q)
+      if (dt.finite(T[q])) {
+        t = (int32_t)q;
+        break;
+      }
+    for (int64_t c = 0; c < ng; ++c) {
+      const int cx = (int)(c % tk.dims[0]), cy = (int)((c / tk.dims[0]) % tk.dims[1]),
+                cz = (int)(c / ((int64_t)tk.dims[0] * tk.dims[1]));
+      const fomo::pred::P3 q{tk.o[0] + (cx + 0.5) * h, tk.o[1] + (cy + 0.5) * h, tk.o[2] + (cz + 0.5) * h};
+      for (int64_t steps = 0; steps < 10000000; ++steps) {
+        const fomo::Tet& u = T[t];
+        int inf = -1;
+        for (int m = 0; m < 4; ++m)
+          if (u.v[m] < 0) inf = m;
+        if (inf >= 0) {  // outside the hull: take the finite tet across the hull face
+          t = u.n[inf];
+          break;
+        }
+        int next = -1;
+        for (int f = 0; f < 4 && next < 0; ++f)
+          if (fomo::pred::orient(dt.P[u.v[fomo::kFace[f][0]]], dt.P[u.v[fomo::kFace[f][1]]],
+                                 dt.P[u.v[fomo::kFace[f][2]]], q) < 0)
+            next = u.n[f];
+        if (next < 0) break;
+        t = next;
+      }
+      start[c] = t;
+    }
+  }
+  std::vector<int4> hv(T.size()), hn(T.size());
+  for (size_t q = 0; q < T.size(); ++q) {
+    int32_t vv[4];
+    for (int m = 0; m < 4; ++m) vv[m] = T[q].v[m] < 0 ? -1 : sorted_of[T[q].v[m]];
+    hv[q] = make_int4(vv[0], vv[1], vv[2], vv[3]);
+    hn[q] = make_int4(T[q].n[0], T[q].n[1], T[q].n[2], T[q].n[3]);
+  }
+  if (ds.tv.alloc(sizeof(int4) * hv.size()) || ds.tn.alloc(sizeof(int4) * hn.size()) ||
+      ds.tstart.alloc(sizeof(int32_t) * ng))
+    return fail(FOMO_ERR_NOMEM, "device allocation failed (tets)");
+  HIPOK(hipMemcpyAsync(ds.tv.p, hv.data(), sizeof(int4) * hv.size(), hipMemcpyHostToDevice, s));
+  HIPOK(hipMemcpyAsync(ds.tn.p, hn.data(), sizeof(int4) * hn.size(), hipMemcpyHostToDevice, s));
+  HIPOK(hipMemcpyAsync(ds.tstart.p, start.data(), sizeof(int32_t) * ng, hipMemcpyHostToDevice, s));
+  if (int rc = sync_stream()) return rc;
+  tk.enabled = 1;
+  ds.tk = tk;
+  ds.ntets = (int64_t)T.size();
+  ds.tet_build_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  return FOMO_OK;
+}
+
 // Build the candidate lookup grid of a dataset (after its cell list). subdiv lookup cells per
 // cell-list edge; 0 disables the fast path. The grid extends kLookupMargin cell-list cells beyond
 // the data bbox (so targets just outside the data, within a search box of them, take the fast
@@ -1400,6 +1661,11 @@ int build_lookup(Dataset& ds) {
 int build_index(Dataset& ds, double cell_scale) {
   hipStream_t s = g_ctx.stream;
   const int64_t n = ds.n;
+  // the triangulation refers to sorted positions, which the new cell order changes
+  ds.ntets = 0;
+  ds.tv.release();
+  ds.tn.release();
+  ds.tstart.release();
   if (cell_scale <= 0) cell_scale = 1.0;
   ds.cell_scale = cell_scale;
   // bbox
@@ -1574,6 +1840,35 @@ void enqueue_nn(hipStream_t s, const Dataset& ds, const ViewGeom* geom, int nx, 
   }
 }
 
+// Tetra path for every sample of a view: per-sample vlos / emissivities like enqueue_nn, and the
+// chosen vertex (sorted position, -1 outside the hull) in vert when given.
+int enqueue_tet(hipStream_t s, const Dataset& ds, const ViewGeom* geom, int nx, int ny, int nz, bool linear,
+                const float4* rec, int rs4, int nlines, float* svlos, float2* sem, int32_t* vert) {
+  if (ds.ntets <= 0) return fail(FOMO_ERR_STATE, "dataset has no triangulation (fomo_dataset_triangulate)");
+  const int64_t R = (int64_t)nx * ny;
+  const unsigned nb = (unsigned)((R + 255) / 256);
+  ProfScope ps(linear ? "k_tet_walk_linear" : "k_tet_walk", s);
+#define FOMO_TET_ARGS                                                                                           \
+  ds.pts.as<float4>(), ds.tv.as<int4>(), ds.tn.as<int4>(), ds.tstart.as<int32_t>(), ds.tk, geom, nx, ny, nz, rec, \
+      nlines, svlos, sem, vert
+  if (!rec) {
+    if (linear) return fail(FOMO_ERR_ARG, "linear interpolation needs per-point records");
+    k_tet_walk<0, false><<<nb, 256, 0, s>>>(FOMO_TET_ARGS);
+  } else if (rs4 == 2) {
+    if (linear) k_tet_walk<2, true><<<nb, 256, 0, s>>>(FOMO_TET_ARGS);
+    else k_tet_walk<2, false><<<nb, 256, 0, s>>>(FOMO_TET_ARGS);
+  } else if (rs4 == 4) {
+    if (linear) k_tet_walk<4, true><<<nb, 256, 0, s>>>(FOMO_TET_ARGS);
+    else k_tet_walk<4, false><<<nb, 256, 0, s>>>(FOMO_TET_ARGS);
+  } else {
+    if (linear) k_tet_walk<8, true><<<nb, 256, 0, s>>>(FOMO_TET_ARGS);
+    else k_tet_walk<8, false><<<nb, 256, 0, s>>>(FOMO_TET_ARGS);
+  }
+#undef FOMO_TET_ARGS
+  HIPOK(hipGetLastError());
+  return FOMO_OK;
+}
+
 // Build records {vel, lines} for a set of emission cubes of one dataset.
 int build_records(hipStream_t s, const Dataset& ds, const std::vector<const float2*>& ems, DevBuf& rec, int* rs4) {
   *rs4 = rec_width((int)ems.size());
@@ -1667,10 +1962,10 @@ void render_sizes(int mode, int64_t n, int64_t R, int64_t S, int nl, int64_t sz[
   sz[0] = (int64_t)sizeof(double) * 6 * (kMaxPartials + 1);
   sz[1] = (int64_t)sizeof(ViewGeom);
   sz[2] = (int64_t)sizeof(int32_t) * S;
-  sz[3] = (int64_t)sizeof(float) * (mode == MODE_NN ? S : n);
+  sz[3] = (int64_t)sizeof(float) * (mode != MODE_THOMSON ? S : n);
   sz[4] = (int64_t)sizeof(float) * R * std::max(nl, 1);
-  sz[5] = mode == MODE_NN ? (int64_t)sizeof(int32_t) * R : 0;
-  sz[6] = mode == MODE_NN ? (int64_t)sizeof(float2) * S : 0;
+  sz[5] = mode != MODE_THOMSON ? (int64_t)sizeof(int32_t) * R : 0;
+  sz[6] = mode != MODE_THOMSON ? (int64_t)sizeof(float2) * S : 0;
   sz[7] = R;
 }
 
@@ -1683,7 +1978,8 @@ int resolve_render(int goft, int mode, int nx, int ny, int nz, int nl, Goft** gc
     return fail(FOMO_ERR_STATE, "emission cube kind does not match the render method");
   Dataset* ds = find_obj(g_ctx.datasets, gc->dataset);
   if (!ds) return fail(FOMO_ERR_NOTFOUND, "dataset of this emission cube was destroyed");
-  if (mode == MODE_NN && nl > 1 && ds->nvars < 5) return fail(FOMO_ERR_ARG, "spectroscopic render needs vx,vy,vz");
+  if (mode != MODE_THOMSON && nl > 1 && ds->nvars < 5)
+    return fail(FOMO_ERR_ARG, "spectroscopic render needs vx,vy,vz");
   *gcp = gc;
   *dsp = ds;
   return FOMO_OK;
@@ -1693,8 +1989,15 @@ int resolve_render(int goft, int mode, int nx, int ny, int nz, int nl, Goft** gc
 int enqueue_render(hipStream_t s, const Dataset& ds, const Goft& gc, int mode, double l, double b, int nx, int ny,
                    int nz, int nl, double width, int dn, const RenderBufs& rb) {
   const int64_t R = (int64_t)nx * ny;
-  enqueue_view_geom(s, ds, mode, l, b, nx, ny, nz, dn, rb.aux, rb.partial, rb.geom);
-  if (mode == MODE_NN) {
+  enqueue_view_geom(s, ds, mode == MODE_THOMSON ? MODE_THOMSON : MODE_NN, l, b, nx, ny, nz, dn, rb.aux, rb.partial,
+                    rb.geom);
+  if (mode == MODE_TET || mode == MODE_LINEAR) {
+    if (int rc = enqueue_tet(s, ds, rb.geom, nx, ny, nz, mode == MODE_LINEAR, gc.rec.as<float4>(), gc.rs4, 1, rb.aux,
+                             rb.sem, nullptr))
+      return rc;
+    LineSet ls = make_lineset({rb.sem}, {rb.out}, {gc.lambda0}, width);
+    enqueue_los(s, (int)R, nz, ls, rb.aux, rb.geom, nl, rb.flag);
+  } else if (mode == MODE_NN) {
     enqueue_nn(s, ds, rb.geom, nx, ny, nz, rb.idx, rb.rayflag, gc.rec.as<float4>(), gc.rs4, 1, rb.aux, rb.sem);
     LineSet ls = make_lineset({rb.sem}, {rb.out}, {gc.lambda0}, width);
     enqueue_los(s, (int)R, nz, ls, rb.aux, rb.geom, nl, rb.flag);
@@ -2028,6 +2331,16 @@ std::shared_ptr<Server> find_server(int id) {
   return it == g_ctx.servers.end() ? nullptr : it->second;
 }
 
+int method_mode(int method, int* mode) {
+  switch (method) {
+    case FOMO_METHOD_NN: *mode = MODE_NN; return FOMO_OK;
+    case FOMO_METHOD_THOMSON: *mode = MODE_THOMSON; return FOMO_OK;
+    case FOMO_METHOD_TET: *mode = MODE_TET; return FOMO_OK;
+    case FOMO_METHOD_LINEAR: *mode = MODE_LINEAR; return FOMO_OK;
+    default: return fail(FOMO_ERR_ARG, "unknown method");
+  }
+}
+
 }  // namespace
 
 // ======================================================================================
@@ -2322,6 +2635,11 @@ static int render_one(int goft, int mode, double l, double b, int nx, int ny, in
   Goft* gc = nullptr;
   Dataset* ds = nullptr;
   if (int rc = resolve_render(goft, mode, nx, ny, nz, nl, &gc, &ds)) return rc;
+  if (mode == MODE_TET || mode == MODE_LINEAR) {
+    if (ds->inuse.load() > 0 && ds->ntets == 0)
+      return fail(FOMO_ERR_STATE, "dataset is in use: triangulate it before serving tetra requests");
+    if (int rc = build_tets(*ds)) return rc;
+  }
   hipStream_t s = g_ctx.stream;
   const int64_t R = (int64_t)nx * ny, S = R * nz;
   int64_t sz[kRenderBufs];
@@ -2355,6 +2673,57 @@ int fomo_render_nn(int goft, double l, double b, int nx, int ny, int nz, int nl,
 int fomo_render_thomson(int goft, double l, double b, int nx, int ny, int nz, float* out, double* meta) {
   std::lock_guard<std::recursive_mutex> lk(g_mu);
   return render_one(goft, MODE_THOMSON, l, b, nx, ny, nz, 1, 0.0, 0, out, meta);
+}
+
+int fomo_render_tet(int goft, double l, double b, int nx, int ny, int nz, int nl, double width, int dn, int linear,
+                    float* out, double* meta) {
+  std::lock_guard<std::recursive_mutex> lk(g_mu);
+  return render_one(goft, linear ? MODE_LINEAR : MODE_TET, l, b, nx, ny, nz, nl, width, dn, out, meta);
+}
+
+int fomo_dataset_triangulate(int dataset, int64_t* ntets, double* seconds) {
+  std::lock_guard<std::recursive_mutex> lk(g_mu);
+  if (int rc = require_init()) return rc;
+  Dataset* ds = find_obj(g_ctx.datasets, dataset);
+  if (!ds) return fail(FOMO_ERR_NOTFOUND, "unknown dataset id");
+  if (ds->ntets == 0 && ds->inuse.load() > 0)
+    return fail(FOMO_ERR_STATE, "dataset is in use by a batch or a server request");
+  if (int rc = build_tets(*ds)) return rc;
+  if (ntets) *ntets = ds->ntets;
+  if (seconds) *seconds = ds->tet_build_s;
+  return FOMO_OK;
+}
+
+int fomo_tet_vertices(int dataset, double l, double b, int nx, int ny, int nz, int32_t* out_vertex, double* meta) {
+  std::lock_guard<std::recursive_mutex> lk(g_mu);
+  if (int rc = require_init()) return rc;
+  if (int rc = check_res(nx, ny, nz, 1)) return rc;
+  if (!out_vertex) return fail(FOMO_ERR_ARG, "null output");
+  Dataset* ds = find_obj(g_ctx.datasets, dataset);
+  if (!ds) return fail(FOMO_ERR_NOTFOUND, "unknown dataset id");
+  if (ds->ntets == 0 && ds->inuse.load() > 0)
+    return fail(FOMO_ERR_STATE, "dataset is in use by a batch or a server request");
+  if (int rc = build_tets(*ds)) return rc;
+  hipStream_t s = g_ctx.stream;
+  const int64_t R = (int64_t)nx * ny, S = R * nz;
+  ViewScratch& vs = g_ctx.vs;
+  HIPOK(vs.partial.ensure(sizeof(double) * 6 * (kMaxPartials + 1)));
+  HIPOK(vs.geom.ensure(sizeof(ViewGeom)));
+  HIPOK(vs.idx.ensure(sizeof(int32_t) * S));
+  HIPOK(vs.aux.ensure(sizeof(float) * std::max<int64_t>(ds->n, S)));
+  ViewGeom* geom = vs.geom.as<ViewGeom>();
+  enqueue_view_geom(s, *ds, MODE_NN, l, b, nx, ny, nz, 0, vs.aux.as<float>(), vs.partial.as<double>(), geom);
+  if (int rc = enqueue_tet(s, *ds, geom, nx, ny, nz, false, nullptr, 0, 0, nullptr, nullptr, vs.idx.as<int32_t>()))
+    return rc;
+  int32_t* tmp = reinterpret_cast<int32_t*>(vs.aux.p);
+  k_idx_to_orig<<<nblk(S), kBlock, 0, s>>>(vs.idx.as<int32_t>(), ds->perm.as<int32_t>(), R, nz, tmp);
+  HIPOK(hipGetLastError());
+  HIPOK(hipMemcpyAsync(out_vertex, tmp, sizeof(int32_t) * S, hipMemcpyDeviceToHost, s));
+  ViewGeom hg;
+  HIPOK(hipMemcpyAsync(&hg, geom, sizeof(ViewGeom), hipMemcpyDeviceToHost, s));
+  if (int rc = sync_stream()) return rc;
+  if (meta) geom_to_meta(hg, meta);
+  return FOMO_OK;
 }
 
 int fomo_nn_indices(int dataset, double l, double b, int nx, int ny, int nz, int thomson, int32_t* out_idx,
@@ -2646,8 +3015,8 @@ int fomo_server_footprint(int goft, int method, int nx, int ny, int nz, int nl, 
                           int64_t* host_bytes) {
   std::lock_guard<std::recursive_mutex> lk(g_mu);
   if (int rc = require_init()) return rc;
-  if (method != FOMO_METHOD_NN && method != FOMO_METHOD_THOMSON) return fail(FOMO_ERR_ARG, "unknown method");
-  const int mode = method == FOMO_METHOD_NN ? MODE_NN : MODE_THOMSON;
+  int mode = 0;
+  if (int rc = method_mode(method, &mode)) return rc;
   if (mode == MODE_THOMSON) nl = 1;
   Goft* gc = nullptr;
   Dataset* ds = nullptr;
@@ -2702,11 +3071,12 @@ int fomo_server_submit(int server, int goft, int method, double l, double b, int
   if (it == g_ctx.servers.end()) return fail(FOMO_ERR_NOTFOUND, "unknown server id");
   Server& sv = *it->second;
   if (!out || !out_ticket) return fail(FOMO_ERR_ARG, "null output");
-  if (method != FOMO_METHOD_NN && method != FOMO_METHOD_THOMSON) return fail(FOMO_ERR_ARG, "unknown method");
   Ticket t;
-  t.mode = method == FOMO_METHOD_NN ? MODE_NN : MODE_THOMSON;
+  if (int rc = method_mode(method, &t.mode)) return rc;
   if (t.mode == MODE_THOMSON) nl = 1, lambda_width = 0.0, dn = 0;
   if (int rc = resolve_render(goft, t.mode, nx, ny, nz, nl, &t.gc, &t.ds)) return rc;
+  if ((t.mode == MODE_TET || t.mode == MODE_LINEAR) && t.ds->ntets == 0)
+    return fail(FOMO_ERR_STATE, "tetra requests need a triangulated dataset (fomo_dataset_triangulate)");
   t.l = l, t.b = b, t.width = lambda_width;
   t.nx = nx, t.ny = ny, t.nz = nz, t.nl = nl, t.dn = dn;
   t.out = out;

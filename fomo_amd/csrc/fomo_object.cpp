@@ -480,8 +480,10 @@ void FoMo::FoMoObject::render(const std::vector<double> lvec, const std::vector<
   rendering.setobservationtype(nl <= 1 ? Imaging : Spectroscopic);
   const std::string method = rendering.readrendermethod();
   const bool thomson = method == "Thomson";
-  if (method != "NearestNeighbour" && !thomson) {
-    if (method == "Projection" || method == "CGAL" || method == "CGAL2D")
+  // "CGAL": the tetra path (fomo-CGAL.cpp:309-338); "Linear": its barycentric variant (extension)
+  const bool tet = method == "CGAL" || method == "Linear";
+  if (method != "NearestNeighbour" && !thomson && !tet) {
+    if (method == "Projection" || method == "CGAL2D")
       die("Error: render method " + method + " is not available in this build.");
     die("Error: unknown rendering method.");
   }
@@ -555,6 +557,8 @@ void FoMo::FoMoObject::render(const std::vector<double> lvec, const std::vector<
     for (double bv : bvec) {
       double m[10];
       int rc = thomson ? fomo_render_thomson(goft, lv, bv, nx, ny, nz, img.data(), m)
+               : tet   ? fomo_render_tet(goft, lv, bv, nx, ny, nz, nl, width, dn ? 1 : 0, method == "Linear" ? 1 : 0,
+                                         img.data(), m)
                        : fomo_render_nn(goft, lv, bv, nx, ny, nz, nl, width, dn ? 1 : 0, img.data(), m);
       if (rc != FOMO_OK) die(std::string("Error: render failed: ") + fomo_last_error());
       const double minx = m[0], maxx = m[1], miny = m[2], maxy = m[3];
@@ -562,7 +566,8 @@ void FoMo::FoMoObject::render(const std::vector<double> lvec, const std::vector<
       FoMo::tgrid newgrid;
       FoMo::tvars newdata;
       if (!thomson) {
-        // fomo-nearestneighbour.cpp:124-128, 152-153, 200-211, 233-279
+        // fomo-nearestneighbour.cpp:124-128, 152-153, 200-211, 233-279 (the CGAL path builds the same
+        // grids, fomo-CGAL.cpp:187-188, 210-218, 262-285)
         const size_t ng3 = (size_t)nx * ny * nlo;
         FoMo::tcoord xv(ng3), yv(ng3), lv3(nl > 1 ? ng3 : 0);
         const double wla = width * lambda0 / kC;
@@ -593,7 +598,7 @@ void FoMo::FoMoObject::render(const std::vector<double> lvec, const std::vector<
         if (nl > 1) newgrid.push_back(std::move(lv3));
         newdata.push_back(img);
         rc3.setdata(newgrid, newdata, &units);
-        rc3.setrendermethod("NearestNeighbour");
+        rc3.setrendermethod(tet ? method : std::string("NearestNeighbour"));
         rc3.setresolution(nx, ny, nz, nl, width);
         rc3.setobservationtype(nl == 1 ? FoMo::Imaging : FoMo::Spectroscopic);
       } else {

@@ -102,6 +102,23 @@ int fomo_render_nn(int goft, double l, double b, int nx, int ny, int nz, int nl,
 /* One view of thomsoninterpolation (fomo-thomson.cpp:94-367): pB image, out: ny*nx. */
 int fomo_render_thomson(int goft, double l, double b, int nx, int ny, int nz, float* out, double* meta);
 
+/* One view of CGALinterpolation (fomo-CGAL.cpp:75-307): each LOS sample takes the values of the
+ * nearest vertex of its Delaunay tet, 0 outside the convex hull (:194-228); samples at
+ * z = k/(nz-1)*(maxz-minz)+minz (:189). linear != 0 (an extension, method "Linear") interpolates
+ * the tet's vertex values barycentrically instead. The dataset is triangulated on first use (host
+ * Delaunay builder, replacing triangulationfromdatacube, :33-73). Arguments as fomo_render_nn. */
+int fomo_render_tet(int goft, double l, double b, int nx, int ny, int nz, int nl, double lambda_width,
+                    int dn, int linear, float* out, double* meta);
+
+/* Build (once; again after fomo_dataset_reindex) the Delaunay triangulation of a dataset and
+ * upload it. ntets counts finite and infinite tets; seconds is the build time. */
+int fomo_dataset_triangulate(int dataset, int64_t* ntets, double* seconds);
+
+/* The tetra path's vertex choice for every LOS sample of one view (original point indices, -1 =
+ * outside the convex hull), layout (iy*nx + jx)*nz + k. For parity checks. */
+int fomo_tet_vertices(int dataset, double l, double b, int nx, int ny, int nz, int32_t* out_vertex,
+                      double* meta);
+
 /* Nearest-neighbour choice of every LOS sample of one view (original point indices, -1 = none),
  * layout (iy*nx + jx)*nz + k. thomson!=0 uses the Thomson geometry (fomo-thomson.cpp:136-317).
  * For parity checks of the spatial index against the reference's R-tree query. */
@@ -145,6 +162,8 @@ int fomo_batch_destroy(int batch);
  * ------------------------------------------------------------------------------------- */
 #define FOMO_METHOD_NN 0      /* "NearestNeighbour": fomo_render_nn */
 #define FOMO_METHOD_THOMSON 1 /* "Thomson": fomo_render_thomson (nl, lambda_width, dn ignored) */
+#define FOMO_METHOD_TET 2     /* "CGAL": fomo_render_tet (dataset must be triangulated first) */
+#define FOMO_METHOD_LINEAR 3  /* "Linear": fomo_render_tet with linear != 0 (likewise) */
 int fomo_server_create(int64_t device_bytes, int64_t host_bytes, int max_inflight, int nstreams,
                        int* out_server);
 int fomo_server_destroy(int server);

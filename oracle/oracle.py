@@ -179,3 +179,80 @@ def rel_l2(a, ref):
     ref = np.asarray(ref, np.float64)
     den = np.linalg.norm(ref)
     return float(np.linalg.norm(a - ref) / den) if den > 0 else float(np.linalg.norm(a - ref))
+
+
+def render_tet(x, y, z, peak, fwhm, vx, vy, vz, l, b, nx, ny, nz, nl, lambda_width, lambda0, dn=False,
+               linear=False):
+    """The reference's CGAL render (fomo-CGAL.cpp:75-307) restated in numpy, FP64, on an independent
+    triangulation: scipy.spatial.Delaunay (Qhull) and its find_simplex for the point location
+    (CGAL's locate, :194-206). Each sample takes the values of the nearest vertex of its tet
+    (nearest_vertex_in_cell, :211-222), none outside the convex hull (:225-228); linear=True takes
+    the barycentric interpolation instead (the "Linear" extension). Extents come from the NN
+    oracle's rotation (the same formula, :118-121). Returns (image [ny*nx*max(nl,1)],
+    vertex per sample [(iy*nx + jx)*nz + k], original index or -1, meta[10])."""
+    import math
+
+    from scipy.spatial import Delaunay
+
+    x, y, z, vx, vy, vz = map(_pf, (x, y, z, vx, vy, vz))
+    peak, fwhm = _pd(peak), _pd(fwhm)
+    pts = np.stack([x, y, z], 1).astype(np.float64)
+    tri = Delaunay(pts)
+    minx, maxx, miny, maxy, minz, maxz = extents(x, y, z, l, b)
+    cb, sb, cl, sl = math.cos(b), math.sin(b), math.cos(l), math.sin(l)
+    jj = np.arange(nx, dtype=np.float64)[None, :, None]
+    ii = np.arange(ny, dtype=np.float64)[:, None, None]
+    kk = np.arange(nz, dtype=np.float64)[None, None, :]
+    xx = jj / (nx - 1) * (maxx - minx) + minx                                      # :187
+    yy = ii / (ny - 1) * (maxy - miny) + miny                                      # :188
+    zz = kk / (nz - 1) * (maxz - minz) + minz if nz > 1 else np.full_like(kk, minz)  # :189
+    shape = (ny, nx, nz)
+    px = np.broadcast_to(xx * cb * cl + yy * sl + zz * sb * cl, shape)             # :192
+    py = np.broadcast_to(-xx * cb * sl + yy * cl - zz * sb * sl, shape)
+    pz = np.broadcast_to(-xx * sb + zz * cb, shape)
+    q = np.stack([px.ravel(), py.ravel(), pz.ravel()], 1)
+    s = tri.find_simplex(q)
+    inside = s >= 0
+    verts = tri.simplices[np.where(inside, s, 0)]
+    ux, uy, uz = sb * cl, -sb * sl, cb                                             # :110
+    vlos_pt = ((0.0 + ux * vx.astype(np.float64)) + uy * vy.astype(np.float64)) + uz * vz.astype(np.float64)
+    if not linear:
+        d = np.stack([((pts[verts[:, m], 0] - q[:, 0]) ** 2 + (pts[verts[:, m], 1] - q[:, 1]) ** 2)
+                      + (pts[verts[:, m], 2] - q[:, 2]) ** 2 for m in range(4)], 1)
+        nv = verts[np.arange(len(q)), np.argmin(d, 1)]
+        vert = np.where(inside, nv, -1).astype(np.int32)
+        pk = np.where(inside, peak[nv], 0.0)
+        fw = np.where(inside, fwhm[nv], 1.0)
+        vl = np.where(inside, vlos_pt[nv], 0.0)
+    else:
+        T = tri.transform[np.where(inside, s, 0)]
+        bc = np.einsum("nij,nj->ni", T[:, :3], q - T[:, 3])
+        w = np.concatenate([bc, 1 - bc.sum(1, keepdims=True)], 1)
+        vert = np.where(inside, verts[:, 0], -1).astype(np.int32)
+        pk = np.where(inside, np.sum(w * peak[verts], 1), 0.0)
+        fw = np.where(inside, np.sum(w * fwhm[verts], 1), 1.0)
+        vl = np.where(inside, np.sum(w * vlos_pt[verts], 1), 0.0)
+    pk, fw, vl = pk.reshape(shape), fw.reshape(shape), vl.reshape(shape)
+    C = 2.99792458e8
+    nlo = max(nl, 1)
+    img = np.zeros((ny, nx, nlo))
+    if nl > 1:
+        wla = lambda_width * lambda0 / C
+        for il in range(nl):
+            lam = il / (nl - 1) * wla - wla / 2.0                                  # :237
+            with np.errstate(over="ignore", invalid="ignore", divide="ignore"):
+                t = pk * np.exp(-((lam - vl / C * lambda0) ** 2) / fw ** 2 * 4.0 * math.log(2.0))
+            img[:, :, il] = np.where(pk != 0, t, 0.0).sum(2)
+    else:
+        img[:, :, 0] = pk.sum(2)
+    pathlength = (maxz - minz) / (nz - 1) if nz > 1 else 1.0                      # :263-271
+    if maxz == minz:
+        pathlength = 1.0
+    apix = 1.0
+    if dn:
+        dxf = np.float32((maxx - minx) / (nx - 1))
+        dyf = np.float32((maxy - miny) / (ny - 1))
+        apix = (float(dxf) / 0.715) * (float(dyf) / 0.715) * (math.pi / 180.0 / 3600.0) ** 2
+    scale = pathlength * 1e8 * apix
+    meta = np.array([minx, maxx, miny, maxy, minz, maxz, 0.0, pathlength, scale, apix])
+    return (img * scale).reshape(-1), vert.reshape(ny, nx, nz).reshape(-1), meta

@@ -1,0 +1,92 @@
+"""GPU tests of the tetra path (the reference's CGAL render, fomo-CGAL.cpp:75-307).
+
+The oracle is independent of the engine's own Delaunay builder: scipy's Qhull triangulation and
+find_simplex (oracle.render_tet). Bars: the vertex chosen for every LOS sample (nearest vertex of
+the containing tet, -1 outside the hull) equal to the oracle's on jittered data (general position,
+no ties); images within relative L2 1e-5 of the FP64 oracle, nearest-vertex and linear.
+"""
+import numpy as np
+import pytest
+
+import fomo_amd as F
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+C = 2.99792458e8
+TOL_IMG = 1e-5
+
+
+@pytest.fixture(scope="module")
+def loop(engine, fe9_table):
+    path, tab = fe9_table
+    d = F.gen_loop_cube(22, seed=31, jitter=0.3, sausage=0.1)
+    ds = F.Dataset(d["x"], d["y"], d["z"], [d["rho"], d["T"], d["vx"], d["vy"], d["vz"]])
+    nt, sec = ds.triangulate()
+    T = F.Table.from_file(path)
+    em = F.Emission.chianti(ds, T, tab["lambda0"], tab["atweight"])
+    emi = F.Emission.chianti(ds, T, tab["lambda0"], tab["atweight"], imaging=True)
+    pk, fw = O.emission(1, d["rho"], d["T"], tab)
+    pki, fwi = O.emission(1, d["rho"], d["T"], tab, imaging=True)
+    yield {"d": d, "tab": tab, "ds": ds, "em": em, "emi": emi, "pk": pk, "fw": fw, "pki": pki, "fwi": fwi,
+           "ntets": nt}
+    em.close(), emi.close(), T.close(), ds.close()
+
+
+VIEWS = [(0.4, 1.1), (0.0, np.pi / 2), (2.3, 0.35), (1.0, 0.0), (np.pi / 4, np.pi / 4)]
+
+
+@pytest.mark.parametrize("l,b", VIEWS)
+def test_vertex_choice_matches_qhull_oracle(loop, l, b):
+    d = loop["d"]
+    nx, ny, nz = 30, 26, 40
+    vert, meta = F.tet_vertices(loop["ds"], l, b, nx, ny, nz)
+    _, rvert, rmeta = O.render_tet(d["x"], d["y"], d["z"], loop["pk"], loop["fw"], d["vx"], d["vy"], d["vz"], l, b,
+                                   nx, ny, nz, 1, 0.0, loop["tab"]["lambda0"])
+    assert np.array_equal(meta[:6], rmeta[:6])
+    assert (rvert >= 0).mean() > 0.05
+    assert np.array_equal(vert, rvert), f"{(vert != rvert).sum()} of {vert.size} samples differ"
+
+
+@pytest.mark.parametrize("l,b,nl,dn,linear", [(0.4, 1.1, 30, False, False), (2.3, 0.35, 12, False, False),
+                                               (0.7, 0.9, 1, False, False), (1.2, 0.6, 1, True, False),
+                                               (0.4, 1.1, 30, False, True), (2.0, 1.3, 1, False, True)])
+def test_tet_render_matches_oracle(loop, l, b, nl, dn, linear):
+    d, tab = loop["d"], loop["tab"]
+    width = 0.2 * C / tab["lambda0"]
+    nx, ny, nz = 28, 24, 36
+    em = loop["em"] if nl > 1 else loop["emi"]
+    pk, fw = (loop["pk"], loop["fw"]) if nl > 1 else (loop["pki"], loop["fwi"])
+    img, meta = F.render_tet(em, l, b, nx, ny, nz, nl, width, dn=dn, linear=linear)
+    ref, _, rmeta = O.render_tet(d["x"], d["y"], d["z"], pk, fw, d["vx"], d["vy"], d["vz"], l, b, nx, ny, nz, nl,
+                                 width, tab["lambda0"], dn=dn, linear=linear)
+    assert np.array_equal(meta[:6], rmeta[:6])
+    np.testing.assert_allclose(meta[7:10], rmeta[7:10], rtol=1e-12)
+    err = O.rel_l2(img, ref)
+    print(f"tet render (nl={nl}, dn={dn}, linear={linear}) rel L2 vs FP64 oracle: {err:.3e}")
+    assert np.count_nonzero(ref) > 0
+    assert err <= TOL_IMG
+
+
+def test_tet_server_equals_sync(loop):
+    width = 0.2 * C / loop["tab"]["lambda0"]
+    reqs = [(loop["em"], F.METHOD_TET, 0.4, 1.1, 32, 30, 40, 30, width),
+            (loop["emi"], F.METHOD_TET, 1.7, 0.5, 24, 24, 24, 1, 0.0),
+            (loop["em"], F.METHOD_LINEAR, 2.2, 0.8, 20, 28, 33, 12, width),
+            (loop["em"], F.METHOD_NN, 0.4, 1.1, 32, 30, 40, 30, width)]
+    sync = []
+    for r in reqs:
+        if r[1] == F.METHOD_NN:
+            sync.append(F.render_nn(r[0], *r[2:8], r[8]))
+        else:
+            sync.append(F.render_tet(r[0], *r[2:8], r[8], linear=r[1] == F.METHOD_LINEAR))
+    sv = F.Server(device_bytes=256 << 20, host_bytes=16 << 20, max_inflight=4, nstreams=2)
+    try:
+        tks = [sv.submit(*r[:8], lambda_width=r[8]) for r in reqs * 2]
+        res = [sv.result(t, timeout=60) for t in tks]
+    finally:
+        sv.close()
+    for k, r in enumerate(reqs * 2):
+        assert np.array_equal(res[k][0], sync[k % len(reqs)][0])
+    # the tetra image differs from the nearest-neighbour one of the same view (different method)
+    assert not np.array_equal(sync[0][0], sync[3][0])
