@@ -50,7 +50,7 @@ def main():
     ap.add_argument("--cpu-rows", type=int, default=16, help="image rows in the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cell-scale", type=float, default=0.0)
-    ap.add_argument("--workload", choices=["batched", "thomson", "serve"], default="batched")
+    ap.add_argument("--workload", choices=["batched", "thomson", "serve", "tet"], default="batched")
     ap.add_argument("--requests", type=int, default=1000, help="requests per arrival rate (serve workload)")
     ap.add_argument("--serve-requests", type=int, default=240, help="requests in the serving phase of the default run")
     ap.add_argument("--corona-m", type=int, default=645, help="corona grid points per axis (thomson workload)")
@@ -69,6 +69,8 @@ def main():
         return thomson(F, args)
     if args.workload == "serve":
         return serve(F, args)
+    if args.workload == "tet":
+        return tet(F, args)
 
     m, nx, ny, nz, nl = args.m, args.nx, args.nx, args.nz, args.nl
     t0 = time.time()
@@ -353,6 +355,64 @@ def serve(F, args):
     print(json.dumps(out), flush=True)
     for o in objs:
         o.close()
+    tab.close()
+    F.shutdown()
+
+
+def tet(F, args):
+    """Tetra-path render at scale (the reference's CGAL method): a 256^3 loop cube jittered by
+    +-0.3 cell (16.8M points, unstructured Delaunay), 2.5 MK corona, 10 % sausage mode; one view
+    (l=0, b=pi/2) of 512x512 pixels, 256 LOS samples, 100 lambda pixels over 0.2 A. The Delaunay
+    triangulation (host, once per dataset) is timed separately; a step is one render."""
+    m = args.m
+    nx = ny = args.nx if args.nx != 256 else 512
+    nz = args.nz if args.nz != 128 else 256
+    nl = args.nl if args.nl != 60 else 100
+    t0 = time.time()
+    d = F.gen_loop_cube(m, seed=1, t_in=1.0e6, t_out=2.5e6, jitter=0.3, sausage=0.1)
+    log(f"generated {m}^3 jittered loop cube in {time.time() - t0:.1f}s")
+    tmp = tempfile.mkdtemp()
+    path = F.gen_goft_table(os.path.join(tmp, "fe9.dat"), ion="fe_9", lambda0=171.0, atweight=56.0, peak_logt=5.9)
+    tab = F.Table.from_file(path)
+    lambda0, atw = 171.0, 56.0
+    width = 0.2 * C / lambda0
+    ds = F.Dataset(d["x"], d["y"], d["z"], [d["rho"], d["T"], d["vx"], d["vy"], d["vz"]])
+    t0 = time.time()
+    os.environ["FOMO_PROGRESS"] = "1"  # the builder reports every 1M inserted points
+    ntets, tbuild = ds.triangulate()
+    log(f"Delaunay: {ntets} tets in {tbuild:.1f}s (host build + upload), {time.time() - t0:.1f}s wall")
+    em = F.Emission.chianti(ds, tab, lambda0, atw)
+    l, b = 0.0, np.pi / 2
+    for _ in range(max(1, args.warmup)):
+        F.render_tet(em, l, b, nx, ny, nz, nl, width)
+    steps = max(1, args.steps)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        img, meta = F.render_tet(em, l, b, nx, ny, nz, nl, width)
+    dt = (time.perf_counter() - t0) / steps
+    W = float(nx) * ny * nz * nl
+    F.profile_reset()
+    F.profile_enable(True)
+    F.render_tet(em, l, b, nx, ny, nz, nl, width)
+    F.profile_enable(False)
+    st = {k: F.profile_stats(k) for k in ("k_view_rotate_nn", "k_tet_walk", "k_los_spectral")}
+    log(f"tet render: {dt * 1e3:.1f} ms/view, {W / dt:.3e} W/s; kernels " + json.dumps(st))
+    # same view, nearest-neighbour method, for comparison
+    t0 = time.perf_counter()
+    F.render_nn(em, l, b, nx, ny, nz, nl, width)
+    dtn = time.perf_counter() - t0
+    out = {"metric": "LOS samples x lambda per second, tetra (CGAL-method) render", "value": W / dt,
+           "unit": "LOS samples*lambda/s", "n_gpus": 1, "steps": steps, "warmup": args.warmup,
+           "ms_per_step": dt * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+           "data": "synthetic",
+           "config": {"workload": f"tetra render, {m}^3 jittered loop cube ({d['x'].size} points), 1 view, "
+                                  f"{nx}x{ny} image, nz {nz}, nl {nl}"},
+           "delaunay": {"tets": ntets, "build_s": tbuild},
+           "kernels_ms": {k: (v[1] / v[0] if v[0] else None) for k, v in st.items()},
+           "nn_same_view_ms": dtn * 1e3}
+    print(json.dumps(out), flush=True)
+    em.close()
+    ds.close()
     tab.close()
     F.shutdown()
 

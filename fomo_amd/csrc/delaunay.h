@@ -20,6 +20,7 @@
 #include <algorithm>
 #include <cstdint>
 #include <cstdio>
+#include <cstdlib>
 #include <string>
 #include <vector>
 
@@ -104,8 +105,15 @@ class Delaunay {
     std::vector<int32_t> order(seq.size());
     for (size_t i = 0; i < seq.size(); ++i) order[i] = (int32_t)i;
     bool ok = init(order);
-    for (size_t i = 0; ok && i < order.size(); ++i)
+    // optional progress lines on stderr (FOMO_PROGRESS=1), as the reference prints its own
+    // "Doing Delaunay triangulation" progress (fomo-CGAL.cpp:55)
+    const char* pe = std::getenv("FOMO_PROGRESS");
+    const bool progress = pe && pe[0] == '1';
+    for (size_t i = 0; ok && i < order.size(); ++i) {
       if (!inserted_[order[i]]) ok = insert(order[i]);
+      if (progress && (i + 1) % 1000000 == 0)
+        std::fprintf(stderr, "Delaunay: %zu of %zu points inserted, %zu tets\n", i + 1, order.size(), T.size());
+    }
     for (auto& t : T)
       if (t.v[0] != kDead)
         for (int k = 0; k < 4; ++k)
