@@ -29,6 +29,10 @@ sys.path.insert(0, ROOT)
 
 C = 2.99792458e8
 PEAK_HBM = 8.0e12  # B/s, MI355X HBM3E
+# HBM bytes per launch (read + write) of the default workload's kernels, from two rocprofv3 --pmc
+# runs (FETCH_SIZE x 2.000, WRITE_SIZE x 1.000, the factors a 1 GiB copy kernel gives):
+# profiles/batched_render_256cube_counters.md
+MEASURED_TRAFFIC = {"k_nn": 2.027e9, "k_los_spectral": 4.49e8}
 
 
 def log(*a):
@@ -136,8 +140,12 @@ def main():
     if cnt > 0:
         avg_s = tot / cnt / 1e3
         ach = bytes_per[dom] / avg_s
+        # HBM bytes per launch measured with rocprofv3 FETCH_SIZE / WRITE_SIZE (calibrated on a plain
+        # copy kernel) for this exact workload: profiles/batched_render_256cube_counters.md
+        default_cfg = (m, nx, nz, nl, len(ems)) == (256, 256, 128, 60, 4)
+        traffic = MEASURED_TRAFFIC.get(dom) if default_cfg else None
         roof = {"kernel": dom, "bound": "hbm", "achieved": ach / 1e9, "peak": PEAK_HBM / 1e9, "unit": "GB/s",
-                "frac": ach / PEAK_HBM, "traffic": None, "avg_launch_ms": avg_s * 1e3,
+                "frac": ach / PEAK_HBM, "traffic": traffic, "avg_launch_ms": avg_s * 1e3,
                 "algorithmic_bytes_per_launch": bytes_per[dom],
                 "share_of_step": {k: stats[k][1] / max(1e-9, sum(v[1] for v in stats.values())) for k in stats}}
 

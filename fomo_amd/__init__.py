@@ -11,7 +11,7 @@ import numpy as np
 
 __all__ = [
     "lib", "FomoError", "init", "shutdown", "device_info", "Dataset", "Table", "Emission", "Batch",
-    "render_nn", "render_thomson", "nn_indices", "Server", "METHOD_NN", "METHOD_THOMSON", "METHOD_TET", "METHOD_LINEAR", "delaunay", "render_tet",
+    "render_nn", "render_thomson", "nn_indices", "Server", "METHOD_NN", "METHOD_THOMSON", "METHOD_TET", "METHOD_LINEAR", "delaunay", "render_tet", "autotune",
     "tet_vertices", "gen_loop_cube", "gen_goft_table", "gen_corona",
     "read_table", "profile_enable", "profile_reset", "profile_stats", "LIB_PATH",
 ]
@@ -76,6 +76,8 @@ _SIGS = {
     "fomo_dataset_triangulate": (ctypes.c_int, [ctypes.c_int, _c_i64_p, _c_dbl_p]),
     "fomo_tet_vertices": (ctypes.c_int, [ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_int,
                                          ctypes.c_int, _c_i32_p, _c_dbl_p]),
+    "fomo_autotune": (ctypes.c_int, [ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_int,
+                                     ctypes.c_int, ctypes.c_int, _c_dbl_p]),
     "fomo_batch_create": (ctypes.c_int, [ctypes.c_int, _c_int_p, ctypes.c_int, _c_dbl_p, _c_dbl_p, ctypes.c_int,
                                          ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_int,
                                          _c_int_p]),
@@ -323,6 +325,14 @@ def render_thomson(em, l, b, nx, ny, nz):
     img = np.zeros(nx * ny, np.float32)
     _check(lib().fomo_render_thomson(em.id, l, b, nx, ny, nz, _ptr(img, ctypes.c_float), _ptr(meta, ctypes.c_double)))
     return img, meta
+
+
+def autotune(em, l, b, nx, ny, nz, reps=3):
+    """Pick the fastest index configuration (cell scale x lookup subdivision) for views like this one;
+    the dataset keeps it. Returns dict(cell_scale, subdiv, ms, ms_before)."""
+    r = np.zeros(4, np.float64)
+    _check(lib().fomo_autotune(em.id, l, b, nx, ny, nz, reps, _ptr(r, ctypes.c_double)))
+    return {"cell_scale": float(r[0]), "subdiv": int(r[1]), "ms": float(r[2]), "ms_before": float(r[3])}
 
 
 def render_tet(em, l, b, nx, ny, nz, nl, lambda_width, dn=False, linear=False):

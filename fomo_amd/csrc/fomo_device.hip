@@ -2681,6 +2681,87 @@ int fomo_render_tet(int goft, double l, double b, int nx, int ny, int nz, int nl
   return render_one(goft, linear ? MODE_LINEAR : MODE_TET, l, b, nx, ny, nz, nl, width, dn, out, meta);
 }
 
+// Autotuner: times the NN passes of one representative view (k_nn + k_nn_far, with the
+// emission cube's per-point records, exactly as a render runs them) for each candidate index
+// configuration and keeps the fastest. Candidates: cell-list scale x lookup subdivision.
+int fomo_autotune(int goft, double l, double b, int nx, int ny, int nz, int reps, double* result) {
+  std::lock_guard<std::recursive_mutex> lk(g_mu);
+  if (int rc = require_init()) return rc;
+  if (int rc = check_res(nx, ny, nz, 1)) return rc;
+  Goft* gc = find_obj(g_ctx.gofts, goft);
+  if (!gc) return fail(FOMO_ERR_NOTFOUND, "unknown emission id");
+  Dataset* ds = find_obj(g_ctx.datasets, gc->dataset);
+  if (!ds) return fail(FOMO_ERR_NOTFOUND, "dataset of this emission cube was destroyed");
+  if (ds->inuse.load() > 0) return fail(FOMO_ERR_STATE, "dataset is in use by a batch or a server request");
+  if (reps < 1) reps = 1;
+  const int mode = gc->kind == 1 ? MODE_THOMSON : MODE_NN;
+  hipStream_t s = g_ctx.stream;
+  const int64_t R = (int64_t)nx * ny, S = R * nz;
+  ViewScratch& vs = g_ctx.vs;
+  HIPOK(vs.partial.ensure(sizeof(double) * 6 * (kMaxPartials + 1)));
+  HIPOK(vs.geom.ensure(sizeof(ViewGeom)));
+  HIPOK(vs.idx.ensure(sizeof(int32_t) * S));
+  HIPOK(vs.aux.ensure(sizeof(float) * std::max<int64_t>(ds->n, S)));
+  HIPOK(vs.sem.ensure(sizeof(float2) * S));
+  HIPOK(vs.rayflag.ensure_zeroed(R, s));
+  hipEvent_t e0, e1;
+  HIPOK(hipEventCreate(&e0));
+  HIPOK(hipEventCreate(&e1));
+  auto time_nn = [&](double* ms) -> int {
+    ViewGeom* geom = vs.geom.as<ViewGeom>();
+    enqueue_view_geom(s, *ds, mode, l, b, nx, ny, nz, 0, vs.aux.as<float>(), vs.partial.as<double>(), geom);
+    const bool recs = mode == MODE_NN && gc->rec.p;
+    auto once = [&] {
+      enqueue_nn(s, *ds, geom, nx, ny, nz, vs.idx.as<int32_t>(), vs.rayflag.as<uint8_t>(),
+                 recs ? gc->rec.as<float4>() : nullptr, recs ? gc->rs4 : 0, recs ? 1 : 0,
+                 recs ? vs.aux.as<float>() : nullptr, recs ? vs.sem.as<float2>() : nullptr);
+    };
+    once();  // warm-up
+    HIPOK(hipEventRecord(e0, s));
+    for (int r = 0; r < reps; ++r) once();
+    HIPOK(hipEventRecord(e1, s));
+    HIPOK(hipEventSynchronize(e1));
+    float t = 0;
+    HIPOK(hipEventElapsedTime(&t, e0, e1));
+    *ms = t / reps;
+    return FOMO_OK;
+  };
+  const double scales[] = {0.7, 1.0, 1.4};
+  const int subdivs[] = {1, 2, 3};
+  const double cs0 = ds->cell_scale;
+  const int sd0 = ds->lookup_subdiv;
+  double base = 0, best = 1e300, bscale = cs0;
+  int bsub = sd0, rc = FOMO_OK;
+  if ((rc = time_nn(&base))) goto done;
+  best = base;
+  for (double sc : scales)
+    for (int sd : subdivs) {
+      ds->lookup_subdiv = sd;
+      if ((rc = build_index(*ds, sc))) goto done;
+      if (ds->lookup_active_subdiv != sd && sd > 1) continue;  // did not fit: same as a smaller one
+      double ms = 0;
+      if ((rc = time_nn(&ms))) goto done;
+      if (ms < best) best = ms, bscale = sc, bsub = sd;
+    }
+  ds->lookup_subdiv = bsub;
+  rc = build_index(*ds, bscale);
+done:
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (rc) {
+    ds->lookup_subdiv = sd0;
+    (void)build_index(*ds, cs0);
+    return rc;
+  }
+  if (result) {
+    result[0] = bscale;
+    result[1] = bsub;
+    result[2] = best;
+    result[3] = base;
+  }
+  return FOMO_OK;
+}
+
 int fomo_dataset_triangulate(int dataset, int64_t* ntets, double* seconds) {
   std::lock_guard<std::recursive_mutex> lk(g_mu);
   if (int rc = require_init()) return rc;
