@@ -54,6 +54,9 @@ def main():
     ap.add_argument("--cpu-rows", type=int, default=16, help="image rows in the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cell-scale", type=float, default=0.0)
+    ap.add_argument("--autotune", action="store_true",
+                    help="let the autotuner pick the index configuration first (for C2 the default one is as fast "
+                         "inside the batch, where the NN pass shares the GPU with the integration)")
     ap.add_argument("--workload", choices=["batched", "thomson", "serve", "tet"], default="batched")
     ap.add_argument("--requests", type=int, default=1000, help="requests per arrival rate (serve workload)")
     ap.add_argument("--serve-requests", type=int, default=240, help="requests in the serving phase of the default run")
@@ -97,6 +100,14 @@ def main():
     nv = args.nviews
     ls = [np.pi * (k % (nv // 2)) / (nv // 2) for k in range(nv)]
     bs = [np.pi / 2 if k < nv // 2 else np.pi / 3 for k in range(nv)]
+    # index configuration picked by the autotuner on one view of the workload (setup, not timed)
+    tune = None
+    if args.autotune and args.cell_scale <= 0:
+        t0 = time.time()
+        tune = F.autotune(ems, ls[min(1, nv - 1)], bs[min(1, nv - 1)], nx, ny, nz, reps=5)
+        log(f"autotune: cell scale {tune['cell_scale']}, lookup subdivision {tune['subdiv']}: NN pass "
+            f"{tune['ms_before']:.3f} -> {tune['ms']:.3f} ms ({time.time() - t0:.1f}s)")
+        n, dims, maxc = ds.info()
     W_step = float(nv * len(ems)) * nx * ny * nz * nl
     S_view = nx * ny * nz
     R = nx * ny
@@ -166,7 +177,7 @@ def main():
     # pointers into the index, so it goes first)
     bt.close()
     t0 = time.perf_counter()
-    ds.reindex(args.cell_scale)
+    ds.reindex(tune["cell_scale"] if tune else args.cell_scale)
     t_index = time.perf_counter() - t0
     linfo = ds.lookup_info()
     log(f"index rebuild: {t_index * 1e3:.1f} ms, lookup {linfo}")
@@ -220,6 +231,7 @@ def main():
         "index_build_ms": t_index * 1e3,
         "copy_probe_GBps": copy_gbps,
         "lookup": linfo,
+        "autotune": tune,
         "batched_vs_plain": value / plain_value,
         "roofline": roof,
         "cpu_baseline": cpu,

@@ -76,8 +76,8 @@ _SIGS = {
     "fomo_dataset_triangulate": (ctypes.c_int, [ctypes.c_int, _c_i64_p, _c_dbl_p]),
     "fomo_tet_vertices": (ctypes.c_int, [ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_int,
                                          ctypes.c_int, _c_i32_p, _c_dbl_p]),
-    "fomo_autotune": (ctypes.c_int, [ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_int,
-                                     ctypes.c_int, ctypes.c_int, _c_dbl_p]),
+    "fomo_autotune": (ctypes.c_int, [ctypes.c_int, _c_int_p, ctypes.c_double, ctypes.c_double, ctypes.c_int,
+                                     ctypes.c_int, ctypes.c_int, ctypes.c_int, _c_dbl_p]),
     "fomo_batch_create": (ctypes.c_int, [ctypes.c_int, _c_int_p, ctypes.c_int, _c_dbl_p, _c_dbl_p, ctypes.c_int,
                                          ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_int,
                                          _c_int_p]),
@@ -327,11 +327,14 @@ def render_thomson(em, l, b, nx, ny, nz):
     return img, meta
 
 
-def autotune(em, l, b, nx, ny, nz, reps=3):
-    """Pick the fastest index configuration (cell scale x lookup subdivision) for views like this one;
-    the dataset keeps it. Returns dict(cell_scale, subdiv, ms, ms_before)."""
+def autotune(ems, l, b, nx, ny, nz, reps=3):
+    """Pick the fastest index configuration (cell scale x lookup subdivision) for views like this one
+    of the given emission cube(s) (a batch's lines); the dataset keeps it. Returns
+    dict(cell_scale, subdiv, ms, ms_before)."""
+    ems = ems if isinstance(ems, (list, tuple)) else [ems]
+    ids = (ctypes.c_int * len(ems))(*[e.id for e in ems])
     r = np.zeros(4, np.float64)
-    _check(lib().fomo_autotune(em.id, l, b, nx, ny, nz, reps, _ptr(r, ctypes.c_double)))
+    _check(lib().fomo_autotune(len(ems), ids, l, b, nx, ny, nz, reps, _ptr(r, ctypes.c_double)))
     return {"cell_scale": float(r[0]), "subdiv": int(r[1]), "ms": float(r[2]), "ms_before": float(r[3])}
 
 

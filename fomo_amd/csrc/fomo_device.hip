@@ -262,6 +262,28 @@ __global__ void k_permute_var(const float* __restrict__ src, const int32_t* __re
   dst[p] = src[perm[p]];
 }
 
+// inv[perm[p]] = p
+__global__ void k_invert_perm(const int32_t* __restrict__ perm, int64_t n, int32_t* __restrict__ inv) {
+  const int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (p < n) inv[perm[p]] = (int32_t)p;
+}
+
+// Move per-point data of `width` float4 from the previous sorted order to the current one:
+// point at old position p (original index old_perm[p]) goes to new position inv_new[old_perm[p]].
+__global__ void k_reorder_f4(const float4* __restrict__ src, const int32_t* __restrict__ old_perm,
+                             const int32_t* __restrict__ inv_new, int64_t n, int width, float4* __restrict__ dst) {
+  const int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  const int64_t q = inv_new[old_perm[p]];
+  for (int k = 0; k < width; ++k) dst[q * width + k] = src[p * width + k];
+}
+__global__ void k_reorder_f2(const float2* __restrict__ src, const int32_t* __restrict__ old_perm,
+                             const int32_t* __restrict__ inv_new, int64_t n, float2* __restrict__ dst) {
+  const int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  dst[inv_new[old_perm[p]]] = src[p];
+}
+
 __global__ void k_pack_vel(const float* __restrict__ vx, const float* __restrict__ vy, const float* __restrict__ vz,
                            const int32_t* __restrict__ perm, int64_t n, float4* __restrict__ vel) {
   int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
@@ -769,20 +791,12 @@ __device__ __forceinline__ void write_sample(int r, const float4* __restrict__ r
 // svlos[s] (LOS velocity, fomo-nearestneighbour.cpp:87-89) and sem[l*S + s] = (peak_l, fwhm_l),
 // zero when no point is inside the search box (:190-193). idx (optional) gets the sorted position.
 template <int RS4>
-__global__ void __launch_bounds__(256) k_nn(const float4* __restrict__ pts, const int32_t* __restrict__ cstart,
-                                            GridInfo g, const int2* __restrict__ lcell,
-                                            const float4* __restrict__ lcand, LookupInfo lk,
-                                            const ViewGeom* __restrict__ vgp, int nx, int ny, int nz,
-                                            int32_t* __restrict__ idx, const float4* __restrict__ rec, int nlines,
-                                            float* __restrict__ svlos, float2* __restrict__ sem,
-                                            uint8_t* __restrict__ rayflag) {
-  const int tilesx = (nx + 15) >> 4;
-  const int tx = blockIdx.x % tilesx, ty = blockIdx.x / tilesx;
-  const int j = tx * 16 + (threadIdx.x & 15);
-  const int i = ty * 16 + (threadIdx.x >> 4);
-  const int k = blockIdx.y;
-  if (j >= nx || i >= ny) return;
-  const ViewGeom vg = *vgp;
+__device__ __forceinline__ void nn_sample(const float4* __restrict__ pts, const GridInfo& g,
+                                          const int2* __restrict__ lcell, const float4* __restrict__ lcand,
+                                          const LookupInfo& lk, const ViewGeom& vg, int nx, int ny, int nz, int i,
+                                          int j, int k, int32_t* __restrict__ idx, const float4* __restrict__ rec,
+                                          int nlines, float* __restrict__ svlos, float2* __restrict__ sem,
+                                          uint8_t* __restrict__ rayflag) {
   const double minx = vg.ext[0], maxx = vg.ext[1], miny = vg.ext[2], maxy = vg.ext[3], minz = vg.ext[4];
   const double xx = double(j) / (nx - 1) * (maxx - minx) + minx;  // :152
   const double yy = double(i) / (ny - 1) * (maxy - miny) + miny;  // :153
@@ -813,6 +827,23 @@ __global__ void __launch_bounds__(256) k_nn(const float4* __restrict__ pts, cons
   if constexpr (RS4 > 0) {
     if (r != -2) write_sample<RS4>(r, rec, nlines, S, sidx, cb, sb, cl, sl, svlos, sem);
   }
+}
+
+
+// One thread per LOS sample: block = 16x16 rays of one LOS plane (blockIdx.y = k).
+template <int RS4>
+__global__ void __launch_bounds__(256) k_nn(const float4* __restrict__ pts, GridInfo g,
+                                            const int2* __restrict__ lcell, const float4* __restrict__ lcand,
+                                            LookupInfo lk, const ViewGeom* __restrict__ vgp, int nx, int ny, int nz,
+                                            int32_t* __restrict__ idx, const float4* __restrict__ rec, int nlines,
+                                            float* __restrict__ svlos, float2* __restrict__ sem,
+                                            uint8_t* __restrict__ rayflag) {
+  const int tilesx = (nx + 15) >> 4;
+  const int j = (blockIdx.x % tilesx) * 16 + (threadIdx.x & 15);
+  const int i = (blockIdx.x / tilesx) * 16 + (threadIdx.x >> 4);
+  if (j >= nx || i >= ny) return;
+  nn_sample<RS4>(pts, g, lcell, lcand, lk, *vgp, nx, ny, nz, i, j, (int)blockIdx.y, idx, rec, nlines, svlos, sem,
+                 rayflag);
 }
 
 // Second NN pass over the samples k_nn left unresolved (-2): exact octree query. Same grid as
@@ -1817,9 +1848,9 @@ void enqueue_nn(hipStream_t s, const Dataset& ds, const ViewGeom* geom, int nx, 
   const int64_t R = (int64_t)nx * ny;
   {
     ProfScope ps("k_nn", s);
-#define FOMO_NN_ARGS                                                                                          \
-  ds.pts.as<float4>(), ds.cstart.as<int32_t>(), ds.grid, ds.lcell.as<int2>(), ds.lcand.as<float4>(), ds.lk, \
-      geom, nx, ny, nz, idx, rec, nlines, svlos, sem, rayflag
+#define FOMO_NN_ARGS                                                                                         \
+  ds.pts.as<float4>(), ds.grid, ds.lcell.as<int2>(), ds.lcand.as<float4>(), ds.lk, geom, nx, ny, nz, idx, rec, \
+      nlines, svlos, sem, rayflag
     if (!rec) k_nn<0><<<grid, 256, 0, s>>>(FOMO_NN_ARGS);
     else if (rs4 == 2) k_nn<2><<<grid, 256, 0, s>>>(FOMO_NN_ARGS);
     else if (rs4 == 4) k_nn<4><<<grid, 256, 0, s>>>(FOMO_NN_ARGS);
@@ -2331,6 +2362,40 @@ std::shared_ptr<Server> find_server(int id) {
   return it == g_ctx.servers.end() ? nullptr : it->second;
 }
 
+// Re-index a dataset (new cell scale or lookup subdivision) and carry its emission cubes and their
+// per-point records over to the new sorted order, so cubes made before stay valid.
+int reindex_dataset(int id, Dataset& ds, double cell_scale) {
+  hipStream_t s = g_ctx.stream;
+  const int64_t n = ds.n;
+  DevBuf old_perm;
+  if (old_perm.alloc(sizeof(int32_t) * n)) return fail(FOMO_ERR_NOMEM, "device allocation failed (reindex)");
+  HIPOK(hipMemcpyAsync(old_perm.p, ds.perm.p, sizeof(int32_t) * n, hipMemcpyDeviceToDevice, s));
+  if (int rc = build_index(ds, cell_scale)) return rc;
+  DevBuf inv;
+  if (inv.alloc(sizeof(int32_t) * n)) return fail(FOMO_ERR_NOMEM, "device allocation failed (reindex)");
+  k_invert_perm<<<nblk(n), kBlock, 0, s>>>(ds.perm.as<int32_t>(), n, inv.as<int32_t>());
+  for (auto& kv : g_ctx.gofts) {
+    Goft& gc = *kv.second;
+    if (gc.dataset != id) continue;
+    DevBuf em;
+    if (em.alloc(sizeof(float2) * n)) return fail(FOMO_ERR_NOMEM, "device allocation failed (reindex)");
+    k_reorder_f2<<<nblk(n), kBlock, 0, s>>>(gc.em.as<float2>(), old_perm.as<int32_t>(), inv.as<int32_t>(), n,
+                                           em.as<float2>());
+    std::swap(gc.em.p, em.p);
+    std::swap(gc.em.bytes, em.bytes);
+    if (gc.rec.p && gc.rs4 > 0) {
+      DevBuf rec;
+      if (rec.alloc(sizeof(float4) * gc.rs4 * n)) return fail(FOMO_ERR_NOMEM, "device allocation failed (reindex)");
+      k_reorder_f4<<<nblk(n), kBlock, 0, s>>>(gc.rec.as<float4>(), old_perm.as<int32_t>(), inv.as<int32_t>(), n,
+                                             gc.rs4, rec.as<float4>());
+      std::swap(gc.rec.p, rec.p);
+      std::swap(gc.rec.bytes, rec.bytes);
+    }
+    HIPOK(hipGetLastError());
+  }
+  return sync_stream();
+}
+
 int method_mode(int method, int* mode) {
   switch (method) {
     case FOMO_METHOD_NN: *mode = MODE_NN; return FOMO_OK;
@@ -2507,7 +2572,7 @@ int fomo_dataset_reindex(int id, double cell_scale) {
   Dataset* ds = find_obj(g_ctx.datasets, id);
   if (!ds) return fail(FOMO_ERR_NOTFOUND, "unknown dataset id");
   if (ds->inuse.load() > 0) return fail(FOMO_ERR_STATE, "dataset is in use by a batch or a server request");
-  return build_index(*ds, cell_scale);
+  return reindex_dataset(id, *ds, cell_scale);
 }
 
 int fomo_table_create(int nt, int nrho, const float* logt, const float* logn, const float* g, int* out_id) {
@@ -2681,20 +2746,31 @@ int fomo_render_tet(int goft, double l, double b, int nx, int ny, int nz, int nl
   return render_one(goft, linear ? MODE_LINEAR : MODE_TET, l, b, nx, ny, nz, nl, width, dn, out, meta);
 }
 
-// Autotuner: times the NN passes of one representative view (k_nn + k_nn_far, with the
-// emission cube's per-point records, exactly as a render runs them) for each candidate index
-// configuration and keeps the fastest. Candidates: cell-list scale x lookup subdivision.
-int fomo_autotune(int goft, double l, double b, int nx, int ny, int nz, int reps, double* result) {
+// Autotuner: times the NN passes of one representative view (k_nn + k_nn_far, gathering the same
+// per-point records a batch of these emission cubes gathers) for each candidate index
+// configuration, and keeps the fastest if it beats the configuration on entry by more than 3 %.
+// Candidates: cell-list scale x lookup subdivision.
+int fomo_autotune(int ngoft, const int* gofts, double l, double b, int nx, int ny, int nz, int reps,
+                  double* result) {
   std::lock_guard<std::recursive_mutex> lk(g_mu);
   if (int rc = require_init()) return rc;
   if (int rc = check_res(nx, ny, nz, 1)) return rc;
-  Goft* gc = find_obj(g_ctx.gofts, goft);
-  if (!gc) return fail(FOMO_ERR_NOTFOUND, "unknown emission id");
-  Dataset* ds = find_obj(g_ctx.datasets, gc->dataset);
+  if (ngoft < 1 || ngoft > kMaxLines || !gofts) return fail(FOMO_ERR_ARG, "1..8 emission cubes");
+  Goft* gc0 = find_obj(g_ctx.gofts, gofts[0]);
+  if (!gc0) return fail(FOMO_ERR_NOTFOUND, "unknown emission id");
+  const int dsid = gc0->dataset;
+  Dataset* ds = find_obj(g_ctx.datasets, dsid);
   if (!ds) return fail(FOMO_ERR_NOTFOUND, "dataset of this emission cube was destroyed");
+  std::vector<Goft*> gcs;
+  for (int i = 0; i < ngoft; ++i) {
+    Goft* g = find_obj(g_ctx.gofts, gofts[i]);
+    if (!g || g->dataset != dsid || g->kind != gc0->kind)
+      return fail(FOMO_ERR_ARG, "emission cubes must exist, share one dataset and one kind");
+    gcs.push_back(g);
+  }
   if (ds->inuse.load() > 0) return fail(FOMO_ERR_STATE, "dataset is in use by a batch or a server request");
   if (reps < 1) reps = 1;
-  const int mode = gc->kind == 1 ? MODE_THOMSON : MODE_NN;
+  const int mode = gc0->kind == 1 ? MODE_THOMSON : MODE_NN;
   hipStream_t s = g_ctx.stream;
   const int64_t R = (int64_t)nx * ny, S = R * nz;
   ViewScratch& vs = g_ctx.vs;
@@ -2702,19 +2778,26 @@ int fomo_autotune(int goft, double l, double b, int nx, int ny, int nz, int reps
   HIPOK(vs.geom.ensure(sizeof(ViewGeom)));
   HIPOK(vs.idx.ensure(sizeof(int32_t) * S));
   HIPOK(vs.aux.ensure(sizeof(float) * std::max<int64_t>(ds->n, S)));
-  HIPOK(vs.sem.ensure(sizeof(float2) * S));
+  HIPOK(vs.sem.ensure(sizeof(float2) * S * ngoft));
   HIPOK(vs.rayflag.ensure_zeroed(R, s));
   hipEvent_t e0, e1;
   HIPOK(hipEventCreate(&e0));
   HIPOK(hipEventCreate(&e1));
   auto time_nn = [&](double* ms) -> int {
+    // records in the current sorted order, as a batch of these cubes builds them
+    DevBuf rec;
+    int rs4 = 0;
+    if (mode == MODE_NN) {
+      std::vector<const float2*> ems;
+      for (Goft* g : gcs) ems.push_back(g->em.as<float2>());
+      if (int rc = build_records(s, *ds, ems, rec, &rs4)) return rc;
+    }
     ViewGeom* geom = vs.geom.as<ViewGeom>();
     enqueue_view_geom(s, *ds, mode, l, b, nx, ny, nz, 0, vs.aux.as<float>(), vs.partial.as<double>(), geom);
-    const bool recs = mode == MODE_NN && gc->rec.p;
     auto once = [&] {
       enqueue_nn(s, *ds, geom, nx, ny, nz, vs.idx.as<int32_t>(), vs.rayflag.as<uint8_t>(),
-                 recs ? gc->rec.as<float4>() : nullptr, recs ? gc->rs4 : 0, recs ? 1 : 0,
-                 recs ? vs.aux.as<float>() : nullptr, recs ? vs.sem.as<float2>() : nullptr);
+                 rs4 ? rec.as<float4>() : nullptr, rs4, rs4 ? ngoft : 0, rs4 ? vs.aux.as<float>() : nullptr,
+                 rs4 ? vs.sem.as<float2>() : nullptr);
     };
     once();  // warm-up
     HIPOK(hipEventRecord(e0, s));
@@ -2733,24 +2816,24 @@ int fomo_autotune(int goft, double l, double b, int nx, int ny, int nz, int reps
   double base = 0, best = 1e300, bscale = cs0;
   int bsub = sd0, rc = FOMO_OK;
   if ((rc = time_nn(&base))) goto done;
-  best = base;
   for (double sc : scales)
     for (int sd : subdivs) {
       ds->lookup_subdiv = sd;
-      if ((rc = build_index(*ds, sc))) goto done;
+      if ((rc = reindex_dataset(dsid, *ds, sc))) goto done;
       if (ds->lookup_active_subdiv != sd && sd > 1) continue;  // did not fit: same as a smaller one
       double ms = 0;
       if ((rc = time_nn(&ms))) goto done;
       if (ms < best) best = ms, bscale = sc, bsub = sd;
     }
+  if (!(best < 0.97 * base)) best = base, bscale = cs0, bsub = sd0;
   ds->lookup_subdiv = bsub;
-  rc = build_index(*ds, bscale);
+  rc = reindex_dataset(dsid, *ds, bscale);
 done:
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   if (rc) {
     ds->lookup_subdiv = sd0;
-    (void)build_index(*ds, cs0);
+    (void)reindex_dataset(dsid, *ds, cs0);
     return rc;
   }
   if (result) {

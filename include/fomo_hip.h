@@ -110,13 +110,16 @@ int fomo_render_thomson(int goft, double l, double b, int nx, int ny, int nz, fl
 int fomo_render_tet(int goft, double l, double b, int nx, int ny, int nz, int nl, double lambda_width,
                     int dn, int linear, float* out, double* meta);
 
-/* Autotuner (no counterpart in the reference): for the dataset of emission cube goft, times the
- * nearest-neighbour passes of one representative view (l, b, nx x ny x nz; Thomson geometry for a
- * Thomson cube) over cell-list scales {0.7, 1, 1.4} x lookup subdivisions {1, 2, 3}, `reps` timed
- * repetitions each, and leaves the dataset indexed with the fastest. result (optional, 4 doubles):
- * best cell scale, best subdivision, its ms per view, the ms per view of the configuration on entry.
- * Results of renders do not depend on the configuration (NN choices are exact for every one). */
-int fomo_autotune(int goft, double l, double b, int nx, int ny, int nz, int reps, double* result);
+/* Autotuner (no counterpart in the reference): for the dataset of the ngoft emission cubes (one
+ * dataset, one kind; the lines of a batch), times the nearest-neighbour passes of one
+ * representative view (l, b, nx x ny x nz; Thomson geometry for a Thomson cube), gathering the
+ * same per-point records a batch of these cubes gathers, over cell-list scales {0.7, 1, 1.4} x
+ * lookup subdivisions {1, 2, 3}, `reps` timed repetitions each. The dataset keeps the fastest
+ * configuration when it beats the one on entry by more than 3 %, else the one on entry. result
+ * (optional, 4 doubles): chosen cell scale, subdivision, its ms per view, the ms per view on entry.
+ * Render results do not depend on the configuration (NN choices are exact for every one). */
+int fomo_autotune(int ngoft, const int* gofts, double l, double b, int nx, int ny, int nz, int reps,
+                  double* result);
 
 /* Build (once; again after fomo_dataset_reindex) the Delaunay triangulation of a dataset and
  * upload it. ntets counts finite and infinite tets; seconds is the build time. */
