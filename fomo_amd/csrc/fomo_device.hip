@@ -143,6 +143,13 @@ enum { MODE_NN = 0, MODE_THOMSON = 1, MODE_TET = 2, MODE_LINEAR = 3 };
 
 // Spectral lines rendered together from one NN result (blockIdx.y selects the line).
 constexpr int kMaxLines = 8;
+// Per-point records for the NN passes: {vx, vy, vz, T} then the peak of every line. A line's
+// fwhm is not stored: it is recomputed exactly as the emission kernel computed it,
+// (float)(K * sqrt((double)T)), or 1 for imaging cubes (K == 0) (fomo-CHIANTI.cpp:220-225, :284-285).
+struct RecInfo {
+  double K[kMaxLines];
+};
+
 struct LineSet {
   const float2* em[kMaxLines];  // (peak, fwhm): per sorted point (record build) or per sample (K5)
   float* out[kMaxLines];        // R*nl output per line
@@ -759,9 +766,9 @@ __device__ __forceinline__ int bvh_query(const float4* __restrict__ pts, const f
 // svlos[s] (LOS velocity, fomo-nearestneighbour.cpp:87-89) and sem[l*S + s] = (peak_l, fwhm_l),
 // zero when no point is inside the search box (:190-193).
 template <int RS4>
-__device__ __forceinline__ void write_sample(int r, const float4* __restrict__ rec, int nlines, size_t S, size_t sidx,
-                                             double cb, double sb, double cl, double sl, float* __restrict__ svlos,
-                                             float2* __restrict__ sem) {
+__device__ __forceinline__ void write_sample(int r, const float4* __restrict__ rec, const RecInfo& ri, int nlines,
+                                             size_t S, size_t sidx, double cb, double sb, double cl, double sl,
+                                             float* __restrict__ svlos, float2* __restrict__ sem) {
   float f[4 * RS4];
 #pragma unroll
   for (int q = 0; q < 4 * RS4; ++q) f[q] = 0.f;
@@ -779,10 +786,14 @@ __device__ __forceinline__ void write_sample(int r, const float4* __restrict__ r
   // unit vector (sin b cos l, -sin b sin l, cos b)
   svlos[sidx] = r >= 0 ? (float)(((0.0 + (sb * cl) * (double)f[0]) + (-sb * sl) * (double)f[1]) + cb * (double)f[2])
                        : 0.f;
-  // a record of RS4 float4 holds (4*RS4-3)/2 lines after the velocity
+  const double st = r >= 0 ? sqrt((double)f[3]) : 0.0;
+  // a record of RS4 float4 holds 4*RS4-4 lines after {v, T}
 #pragma unroll
-  for (int l = 0; l < (4 * RS4 - 3) / 2; ++l)
-    if (l < nlines) sem[(size_t)l * S + sidx] = make_float2(f[3 + 2 * l], f[4 + 2 * l]);
+  for (int l = 0; l < 4 * RS4 - 4; ++l)
+    if (l < nlines) {
+      const float fw = r < 0 ? 0.f : (ri.K[l] > 0 ? (float)(ri.K[l] * st) : 1.f);
+      sem[(size_t)l * S + sidx] = make_float2(f[4 + l], fw);
+    }
 }
 
 // One thread per LOS sample. Block = 16x16 rays at one k (blockIdx.y = k).
@@ -795,8 +806,8 @@ __device__ __forceinline__ void nn_sample(const float4* __restrict__ pts, const 
                                           const int2* __restrict__ lcell, const float4* __restrict__ lcand,
                                           const LookupInfo& lk, const ViewGeom& vg, int nx, int ny, int nz, int i,
                                           int j, int k, int32_t* __restrict__ idx, const float4* __restrict__ rec,
-                                          int nlines, float* __restrict__ svlos, float2* __restrict__ sem,
-                                          uint8_t* __restrict__ rayflag) {
+                                          const RecInfo& ri, int nlines, float* __restrict__ svlos,
+                                          float2* __restrict__ sem, uint8_t* __restrict__ rayflag) {
   const double minx = vg.ext[0], maxx = vg.ext[1], miny = vg.ext[2], maxy = vg.ext[3], minz = vg.ext[4];
   const double xx = double(j) / (nx - 1) * (maxx - minx) + minx;  // :152
   const double yy = double(i) / (ny - 1) * (maxy - miny) + miny;  // :153
@@ -825,7 +836,7 @@ __device__ __forceinline__ void nn_sample(const float4* __restrict__ pts, const 
   const size_t sidx = (size_t)k * ((size_t)nx * ny) + (size_t)i * nx + j;
   idx[sidx] = r;
   if constexpr (RS4 > 0) {
-    if (r != -2) write_sample<RS4>(r, rec, nlines, S, sidx, cb, sb, cl, sl, svlos, sem);
+    if (r != -2) write_sample<RS4>(r, rec, ri, nlines, S, sidx, cb, sb, cl, sl, svlos, sem);
   }
 }
 
@@ -835,15 +846,15 @@ template <int RS4>
 __global__ void __launch_bounds__(256) k_nn(const float4* __restrict__ pts, GridInfo g,
                                             const int2* __restrict__ lcell, const float4* __restrict__ lcand,
                                             LookupInfo lk, const ViewGeom* __restrict__ vgp, int nx, int ny, int nz,
-                                            int32_t* __restrict__ idx, const float4* __restrict__ rec, int nlines,
-                                            float* __restrict__ svlos, float2* __restrict__ sem,
+                                            int32_t* __restrict__ idx, const float4* __restrict__ rec, RecInfo ri,
+                                            int nlines, float* __restrict__ svlos, float2* __restrict__ sem,
                                             uint8_t* __restrict__ rayflag) {
   const int tilesx = (nx + 15) >> 4;
   const int j = (blockIdx.x % tilesx) * 16 + (threadIdx.x & 15);
   const int i = (blockIdx.x / tilesx) * 16 + (threadIdx.x >> 4);
   if (j >= nx || i >= ny) return;
-  nn_sample<RS4>(pts, g, lcell, lcand, lk, *vgp, nx, ny, nz, i, j, (int)blockIdx.y, idx, rec, nlines, svlos, sem,
-                 rayflag);
+  nn_sample<RS4>(pts, g, lcell, lcand, lk, *vgp, nx, ny, nz, i, j, (int)blockIdx.y, idx, rec, ri, nlines, svlos,
+                 sem, rayflag);
 }
 
 // Second NN pass over the samples k_nn left unresolved (-2): exact octree query. Same grid as
@@ -854,8 +865,8 @@ template <int RS4>
 __global__ void __launch_bounds__(256) k_nn_far(const float4* __restrict__ pts, const float4* __restrict__ mpts,
                                                 const float4* __restrict__ nlo, const float4* __restrict__ nhi,
                                                 BvhInfo bv, const ViewGeom* __restrict__ vgp, int nx, int ny, int nz,
-                                                int32_t* __restrict__ idx, const float4* __restrict__ rec, int nlines,
-                                                float* __restrict__ svlos, float2* __restrict__ sem,
+                                                int32_t* __restrict__ idx, const float4* __restrict__ rec, RecInfo ri,
+                                                int nlines, float* __restrict__ svlos, float2* __restrict__ sem,
                                                 const uint8_t* __restrict__ rayflag) {
   const int tilesx = (nx + 15) >> 4;
   const int tx = blockIdx.x % tilesx, ty = blockIdx.x / tilesx;
@@ -898,7 +909,7 @@ __global__ void __launch_bounds__(256) k_nn_far(const float4* __restrict__ pts, 
   }
   const int r = bvh_query(pts, mpts, nlo, nhi, bv, box, best, bpos, borig);
   idx[sidx] = r;
-  if constexpr (RS4 > 0) write_sample<RS4>(r, rec, nlines, R * nz, sidx, cb, sb, cl, sl, svlos, sem);
+  if constexpr (RS4 > 0) write_sample<RS4>(r, rec, ri, nlines, R * nz, sidx, cb, sb, cl, sl, svlos, sem);
 }
 
 __global__ void k_clear_u8(uint8_t* __restrict__ a, int64_t n) {
@@ -941,7 +952,7 @@ template <int RS4, bool LINEAR>
 __global__ void __launch_bounds__(256) k_tet_walk(const float4* __restrict__ pts, const int4* __restrict__ tv,
                                                   const int4* __restrict__ tn, const int32_t* __restrict__ tstart,
                                                   LookupInfo tk, const ViewGeom* __restrict__ vgp, int nx, int ny,
-                                                  int nz, const float4* __restrict__ rec, int nlines,
+                                                  int nz, const float4* __restrict__ rec, RecInfo ri, int nlines,
                                                   float* __restrict__ svlos, float2* __restrict__ sem,
                                                   int32_t* __restrict__ vert) {
   const int64_t R = (int64_t)nx * ny;
@@ -1010,7 +1021,7 @@ __global__ void __launch_bounds__(256) k_tet_walk(const float4* __restrict__ pts
     const size_t sidx = (size_t)k * R + ray;
     if (outside) {
       if (vert) vert[sidx] = -1;
-      if constexpr (RS4 > 0) write_sample<RS4>(-1, rec, nlines, S, sidx, cb, sb, cl, sl, svlos, sem);
+      if constexpr (RS4 > 0) write_sample<RS4>(-1, rec, ri, nlines, S, sidx, cb, sb, cl, sl, svlos, sem);
       continue;
     }
     if constexpr (!LINEAR) {
@@ -1028,7 +1039,7 @@ __global__ void __launch_bounds__(256) k_tet_walk(const float4* __restrict__ pts
       }
       const int r = ivec(v, best);
       if (vert) vert[sidx] = r;
-      if constexpr (RS4 > 0) write_sample<RS4>(r, rec, nlines, S, sidx, cb, sb, cl, sl, svlos, sem);
+      if constexpr (RS4 > 0) write_sample<RS4>(r, rec, ri, nlines, S, sidx, cb, sb, cl, sl, svlos, sem);
     } else {
       // barycentric weights: volume of the tet with vertex m replaced by the sample, over the volume
       double w[4], tot = 0.0;
@@ -1041,9 +1052,10 @@ __global__ void __launch_bounds__(256) k_tet_walk(const float4* __restrict__ pts
       }
       if (vert) vert[sidx] = ivec(v, 0);
       if constexpr (RS4 > 0) {
-        double vl = 0.0, pk[(4 * RS4 - 3) / 2], fw[(4 * RS4 - 3) / 2];
+        constexpr int NL = 4 * RS4 - 4;  // lines in a record
+        double vl = 0.0, pk[NL], fw[NL];
 #pragma unroll
-        for (int l = 0; l < (4 * RS4 - 3) / 2; ++l) pk[l] = fw[l] = 0.0;
+        for (int l = 0; l < NL; ++l) pk[l] = fw[l] = 0.0;
 #pragma unroll
         for (int m = 0; m < 4; ++m) {
           const double wm = tot > 0 ? w[m] / tot : 0.25;
@@ -1055,24 +1067,25 @@ __global__ void __launch_bounds__(256) k_tet_walk(const float4* __restrict__ pts
             f[4 * qq] = a.x, f[4 * qq + 1] = a.y, f[4 * qq + 2] = a.z, f[4 * qq + 3] = a.w;
           }
           vl += wm * (((0.0 + (sb * cl) * (double)f[0]) + (-sb * sl) * (double)f[1]) + cb * (double)f[2]);
+          const double st = sqrt((double)f[3]);
 #pragma unroll
-          for (int l = 0; l < (4 * RS4 - 3) / 2; ++l) {
-            pk[l] += wm * (double)f[3 + 2 * l];
-            fw[l] += wm * (double)f[4 + 2 * l];
+          for (int l = 0; l < NL; ++l) {
+            pk[l] += wm * (double)f[4 + l];
+            fw[l] += wm * (double)(ri.K[l] > 0 ? (float)(ri.K[l] * st) : 1.f);
           }
         }
         svlos[sidx] = (float)vl;
 #pragma unroll
-        for (int l = 0; l < (4 * RS4 - 3) / 2; ++l)
+        for (int l = 0; l < NL; ++l)
           if (l < nlines) sem[(size_t)l * S + sidx] = make_float2((float)pk[l], (float)fw[l]);
       }
     }
   }
 }
 
-// Per-point records for k_nn (sorted order): {vx,vy,vz, peak_0,fwhm_0, ...}, rs4 float4 each.
-__global__ void k_build_rec(const float4* __restrict__ vel, const LineSet ls, int64_t n, int rs4,
-                            float4* __restrict__ rec) {
+// Per-point records for k_nn (sorted order): {vx, vy, vz, T, peak_0, peak_1, ...}, rs4 float4 each.
+__global__ void k_build_rec(const float4* __restrict__ vel, const float* __restrict__ T, const LineSet ls, int64_t n,
+                            int rs4, float4* __restrict__ rec) {
   const int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (p >= n) return;
   float f[32];
@@ -1084,13 +1097,10 @@ __global__ void k_build_rec(const float4* __restrict__ vel, const LineSet ls, in
     f[1] = v.y;
     f[2] = v.z;
   }
+  f[3] = T[p];
 #pragma unroll
   for (int l = 0; l < kMaxLines; ++l)
-    if (l < ls.n) {
-      const float2 e = ls.em[l][p];
-      f[3 + 2 * l] = e.x;
-      f[4 + 2 * l] = e.y;
-    }
+    if (l < ls.n) f[4 + l] = ls.em[l][p].x;
 #pragma unroll
   for (int q = 0; q < 8; ++q)
     if (q < rs4) rec[(size_t)p * rs4 + q] = make_float4(f[4 * q], f[4 * q + 1], f[4 * q + 2], f[4 * q + 3]);
@@ -1174,7 +1184,8 @@ __global__ void __launch_bounds__(256) k_los_spectral(int R, int nz, const LineS
         // stride-4 recurrence: four independent chains c = j, j+4, ...
         //   g_{c+4} = g_c R_c,  R_{c+4} = R_c q^16,  R_0 = r_0 r_1 r_2 r_3
         const double q = exp(aq);
-        double r = exp(ar);
+        const double ra = exp(ar);
+        double r = ra;
         double G0 = peak * exp(a0);
         const double G1 = G0 * r;
         r *= q;
@@ -1182,7 +1193,9 @@ __global__ void __launch_bounds__(256) k_los_spectral(int R, int nz, const LineS
         r *= q;
         const double G3 = G2 * r;
         const double q2 = q * q, q4 = q2 * q2, q16 = (q4 * q4) * (q4 * q4);
-        double R0 = exp(4.0 * ar + 6.0 * aq);
+        // R_0 = r_0 r_1 r_2 r_3 = r^4 q^6 from the anchors (no fourth exp)
+        const double ra2 = ra * ra;
+        double R0 = (ra2 * ra2) * (q4 * q2);
         double R1 = R0 * q4, R2 = R1 * q4, R3 = R2 * q4;
         double G[4] = {G0, G1, G2, G3};
         double RR[4] = {R0, R1, R2, R3};
@@ -1310,7 +1323,9 @@ struct Goft {
   int kind = 0;  // 0 chianti, 1 thomson
   double lambda0 = 0;
   DevBuf em;     // float2 per sorted point
-  DevBuf rec;    // chianti: per-point records {vx,vy,vz,peak,fwhm} for k_nn
+  double K = 0;  // fwhm = (float)(K * sqrt((double)T)); 0 for imaging (fwhm 1)
+  DevBuf rec;    // chianti: per-point records {vx, vy, vz, T, peak} for the NN passes
+  RecInfo ri{};  // K of the record's line
   int rs4 = 0;
   std::atomic<int> inuse{0};  // queued/running server requests reading this cube
 };
@@ -1335,7 +1350,8 @@ struct Batch {
   DevBuf partial, geom, aux, out, flag;
   DevBuf idx;      // per-sample sorted positions (double-buffered)
   DevBuf rayflag;  // uint8 R per buffer, all-zero between uses
-  DevBuf rec;  // per-point records {vel, all lines}
+  DevBuf rec;  // per-point records {vel, T, all lines' peaks}
+  RecInfo ri{};
   DevBuf sem;  // per-sample (peak, fwhm), nlines x S
   int rs4 = 0;
   std::vector<hipEvent_t> ev;  // 2 per view: NN done, integration done (graph dependencies)
@@ -1832,7 +1848,7 @@ void enqueue_view_geom(hipStream_t s, const Dataset& ds, int mode, double l, dou
 
 // Per-point records shared by the lines of a render: RS4 float4 per point.
 int rec_width(int nlines) {
-  const int f4 = (3 + 2 * nlines + 3) / 4;
+  const int f4 = (4 + nlines + 3) / 4;
   return f4 <= 2 ? 2 : (f4 <= 4 ? 4 : 8);
 }
 
@@ -1842,7 +1858,7 @@ int rec_width(int nlines) {
 // rayflag: R bytes, zero on entry and on exit (set by k_nn, cleared by k_clear_u8).
 void enqueue_nn(hipStream_t s, const Dataset& ds, const ViewGeom* geom, int nx, int ny, int nz, int32_t* idx,
                 uint8_t* rayflag, const float4* rec = nullptr, int rs4 = 0, int nlines = 0, float* svlos = nullptr,
-                float2* sem = nullptr) {
+                float2* sem = nullptr, const RecInfo& ri = RecInfo{}) {
   const int tiles = ((nx + 15) / 16) * ((ny + 15) / 16);
   dim3 grid(tiles, nz);
   const int64_t R = (int64_t)nx * ny;
@@ -1850,7 +1866,7 @@ void enqueue_nn(hipStream_t s, const Dataset& ds, const ViewGeom* geom, int nx, 
     ProfScope ps("k_nn", s);
 #define FOMO_NN_ARGS                                                                                         \
   ds.pts.as<float4>(), ds.grid, ds.lcell.as<int2>(), ds.lcand.as<float4>(), ds.lk, geom, nx, ny, nz, idx, rec, \
-      nlines, svlos, sem, rayflag
+      ri, nlines, svlos, sem, rayflag
     if (!rec) k_nn<0><<<grid, 256, 0, s>>>(FOMO_NN_ARGS);
     else if (rs4 == 2) k_nn<2><<<grid, 256, 0, s>>>(FOMO_NN_ARGS);
     else if (rs4 == 4) k_nn<4><<<grid, 256, 0, s>>>(FOMO_NN_ARGS);
@@ -1861,7 +1877,7 @@ void enqueue_nn(hipStream_t s, const Dataset& ds, const ViewGeom* geom, int nx, 
     ProfScope ps("k_nn_far", s);
 #define FOMO_NNF_ARGS                                                                                         \
   ds.pts.as<float4>(), ds.mpts.as<float4>(), ds.nlo.as<float4>(), ds.nhi.as<float4>(), ds.bvh, geom, nx, ny, \
-      nz, idx, rec, nlines, svlos, sem, rayflag
+      nz, idx, rec, ri, nlines, svlos, sem, rayflag
     if (!rec) k_nn_far<0><<<grid, 256, 0, s>>>(FOMO_NNF_ARGS);
     else if (rs4 == 2) k_nn_far<2><<<grid, 256, 0, s>>>(FOMO_NNF_ARGS);
     else if (rs4 == 4) k_nn_far<4><<<grid, 256, 0, s>>>(FOMO_NNF_ARGS);
@@ -1874,14 +1890,14 @@ void enqueue_nn(hipStream_t s, const Dataset& ds, const ViewGeom* geom, int nx, 
 // Tetra path for every sample of a view: per-sample vlos / emissivities like enqueue_nn, and the
 // chosen vertex (sorted position, -1 outside the hull) in vert when given.
 int enqueue_tet(hipStream_t s, const Dataset& ds, const ViewGeom* geom, int nx, int ny, int nz, bool linear,
-                const float4* rec, int rs4, int nlines, float* svlos, float2* sem, int32_t* vert) {
+                const float4* rec, int rs4, const RecInfo& ri, int nlines, float* svlos, float2* sem, int32_t* vert) {
   if (ds.ntets <= 0) return fail(FOMO_ERR_STATE, "dataset has no triangulation (fomo_dataset_triangulate)");
   const int64_t R = (int64_t)nx * ny;
   const unsigned nb = (unsigned)((R + 255) / 256);
   ProfScope ps(linear ? "k_tet_walk_linear" : "k_tet_walk", s);
 #define FOMO_TET_ARGS                                                                                           \
   ds.pts.as<float4>(), ds.tv.as<int4>(), ds.tn.as<int4>(), ds.tstart.as<int32_t>(), ds.tk, geom, nx, ny, nz, rec, \
-      nlines, svlos, sem, vert
+      ri, nlines, svlos, sem, vert
   if (!rec) {
     if (linear) return fail(FOMO_ERR_ARG, "linear interpolation needs per-point records");
     k_tet_walk<0, false><<<nb, 256, 0, s>>>(FOMO_TET_ARGS);
@@ -1900,15 +1916,21 @@ int enqueue_tet(hipStream_t s, const Dataset& ds, const ViewGeom* geom, int nx, 
   return FOMO_OK;
 }
 
-// Build records {vel, lines} for a set of emission cubes of one dataset.
-int build_records(hipStream_t s, const Dataset& ds, const std::vector<const float2*>& ems, DevBuf& rec, int* rs4) {
-  *rs4 = rec_width((int)ems.size());
+// Build records {vel, T, peaks} for a set of emission cubes of one dataset; ri gets their K.
+int build_records(hipStream_t s, const Dataset& ds, const std::vector<const Goft*>& gcs, DevBuf& rec, int* rs4,
+                  RecInfo* ri) {
+  if (ds.nvars < 2) return fail(FOMO_ERR_STATE, "records need the temperature variable");
+  *rs4 = rec_width((int)gcs.size());
   if (rec.ensure(sizeof(float4) * (*rs4) * ds.n)) return fail(FOMO_ERR_NOMEM, "device allocation failed (records)");
   LineSet ls{};
-  ls.n = (int)ems.size();
-  for (int i = 0; i < ls.n; ++i) ls.em[i] = ems[i];
-  k_build_rec<<<nblk(ds.n), kBlock, 0, s>>>(ds.nvars >= 5 ? ds.vel.as<float4>() : nullptr, ls, ds.n, *rs4,
-                                            rec.as<float4>());
+  ls.n = (int)gcs.size();
+  for (int i = 0; i < ls.n; ++i) ls.em[i] = gcs[i]->em.as<float2>();
+  if (ri) {
+    *ri = RecInfo{};
+    for (int i = 0; i < ls.n; ++i) ri->K[i] = gcs[i]->K;
+  }
+  k_build_rec<<<nblk(ds.n), kBlock, 0, s>>>(ds.nvars >= 5 ? ds.vel.as<float4>() : nullptr, ds.svars[1]->as<float>(), ls,
+                                            ds.n, *rs4, rec.as<float4>());
   HIPOK(hipGetLastError());
   return FOMO_OK;
 }
@@ -2023,13 +2045,13 @@ int enqueue_render(hipStream_t s, const Dataset& ds, const Goft& gc, int mode, d
   enqueue_view_geom(s, ds, mode == MODE_THOMSON ? MODE_THOMSON : MODE_NN, l, b, nx, ny, nz, dn, rb.aux, rb.partial,
                     rb.geom);
   if (mode == MODE_TET || mode == MODE_LINEAR) {
-    if (int rc = enqueue_tet(s, ds, rb.geom, nx, ny, nz, mode == MODE_LINEAR, gc.rec.as<float4>(), gc.rs4, 1, rb.aux,
-                             rb.sem, nullptr))
+    if (int rc = enqueue_tet(s, ds, rb.geom, nx, ny, nz, mode == MODE_LINEAR, gc.rec.as<float4>(), gc.rs4, gc.ri, 1,
+                             rb.aux, rb.sem, nullptr))
       return rc;
     LineSet ls = make_lineset({rb.sem}, {rb.out}, {gc.lambda0}, width);
     enqueue_los(s, (int)R, nz, ls, rb.aux, rb.geom, nl, rb.flag);
   } else if (mode == MODE_NN) {
-    enqueue_nn(s, ds, rb.geom, nx, ny, nz, rb.idx, rb.rayflag, gc.rec.as<float4>(), gc.rs4, 1, rb.aux, rb.sem);
+    enqueue_nn(s, ds, rb.geom, nx, ny, nz, rb.idx, rb.rayflag, gc.rec.as<float4>(), gc.rs4, 1, rb.aux, rb.sem, gc.ri);
     LineSet ls = make_lineset({rb.sem}, {rb.out}, {gc.lambda0}, width);
     enqueue_los(s, (int)R, nz, ls, rb.aux, rb.geom, nl, rb.flag);
   } else {
@@ -2631,7 +2653,8 @@ int fomo_emission_chianti(int dataset, int table, double lambda0, double atweigh
                                                          tb->nt, tb->nrho, K, normc, imaging, gc->em.as<float2>());
   }
   HIPOK(hipGetLastError());
-  if (int rc = build_records(g_ctx.stream, *ds, {gc->em.as<float2>()}, gc->rec, &gc->rs4)) return rc;
+  gc->K = imaging ? 0.0 : K;
+  if (int rc = build_records(g_ctx.stream, *ds, {gc.get()}, gc->rec, &gc->rs4, &gc->ri)) return rc;
   if (int rc = sync_stream()) return rc;
   int id = g_ctx.next_id++;
   g_ctx.gofts[id] = std::move(gc);
@@ -2787,17 +2810,17 @@ int fomo_autotune(int ngoft, const int* gofts, double l, double b, int nx, int n
     // records in the current sorted order, as a batch of these cubes builds them
     DevBuf rec;
     int rs4 = 0;
+    RecInfo ri{};
     if (mode == MODE_NN) {
-      std::vector<const float2*> ems;
-      for (Goft* g : gcs) ems.push_back(g->em.as<float2>());
-      if (int rc = build_records(s, *ds, ems, rec, &rs4)) return rc;
+      std::vector<const Goft*> cg(gcs.begin(), gcs.end());
+      if (int rc = build_records(s, *ds, cg, rec, &rs4, &ri)) return rc;
     }
     ViewGeom* geom = vs.geom.as<ViewGeom>();
     enqueue_view_geom(s, *ds, mode, l, b, nx, ny, nz, 0, vs.aux.as<float>(), vs.partial.as<double>(), geom);
     auto once = [&] {
       enqueue_nn(s, *ds, geom, nx, ny, nz, vs.idx.as<int32_t>(), vs.rayflag.as<uint8_t>(),
                  rs4 ? rec.as<float4>() : nullptr, rs4, rs4 ? ngoft : 0, rs4 ? vs.aux.as<float>() : nullptr,
-                 rs4 ? vs.sem.as<float2>() : nullptr);
+                 rs4 ? vs.sem.as<float2>() : nullptr, ri);
     };
     once();  // warm-up
     HIPOK(hipEventRecord(e0, s));
@@ -2877,7 +2900,8 @@ int fomo_tet_vertices(int dataset, double l, double b, int nx, int ny, int nz, i
   HIPOK(vs.aux.ensure(sizeof(float) * std::max<int64_t>(ds->n, S)));
   ViewGeom* geom = vs.geom.as<ViewGeom>();
   enqueue_view_geom(s, *ds, MODE_NN, l, b, nx, ny, nz, 0, vs.aux.as<float>(), vs.partial.as<double>(), geom);
-  if (int rc = enqueue_tet(s, *ds, geom, nx, ny, nz, false, nullptr, 0, 0, nullptr, nullptr, vs.idx.as<int32_t>()))
+  if (int rc = enqueue_tet(s, *ds, geom, nx, ny, nz, false, nullptr, 0, RecInfo{}, 0, nullptr, nullptr,
+                           vs.idx.as<int32_t>()))
     return rc;
   int32_t* tmp = reinterpret_cast<int32_t*>(vs.aux.p);
   k_idx_to_orig<<<nblk(S), kBlock, 0, s>>>(vs.idx.as<int32_t>(), ds->perm.as<int32_t>(), R, nz, tmp);
@@ -2960,7 +2984,7 @@ static int enqueue_batch(Batch& bt, hipStream_t s, hipStream_t s2) {
     if (s2 && v >= 2) HIPOK(hipStreamWaitEvent(s, bt.ev[nv + v - 2], 0));
     enqueue_view_geom(s, *ds, MODE_NN, bt.l[v], bt.b[v], bt.nx, bt.ny, bt.nz, bt.dn, nullptr, part, geom);
     enqueue_nn(s, *ds, geom, bt.nx, bt.ny, bt.nz, bt.idx.as<int32_t>() + buf * S, bt.rayflag.as<uint8_t>() + buf * R,
-               bt.rec.as<float4>(), bt.rs4, L, svl, sem);
+               bt.rec.as<float4>(), bt.rs4, L, svl, sem, bt.ri);
     std::vector<const float2*> ems;
     std::vector<float*> outs;
     std::vector<double> l0s;
@@ -3031,9 +3055,9 @@ int fomo_batch_create(int ngoft, const int* gofts, int nviews, const double* l, 
     return fail(FOMO_ERR_NOMEM, "device allocation failed for batch");
   HIPOK(hipMemsetAsync(bt->rayflag.p, 0, R * 2, g_ctx.stream));
   {
-    std::vector<const float2*> ems;
-    for (int gid : bt->gofts) ems.push_back(find_obj(g_ctx.gofts, gid)->em.as<float2>());
-    if (int rc = build_records(g_ctx.stream, *ds, ems, bt->rec, &bt->rs4)) return rc;
+    std::vector<const Goft*> cg;
+    for (int gid : bt->gofts) cg.push_back(find_obj(g_ctx.gofts, gid));
+    if (int rc = build_records(g_ctx.stream, *ds, cg, bt->rec, &bt->rs4, &bt->ri)) return rc;
     if (int rc = sync_stream()) return rc;
   }
   for (int i = 0; i < 2 * nviews; ++i) {
