@@ -137,13 +137,13 @@ def main():
     F.profile_reset()
     bt.profile()
     bt.profile()
-    stats = {k: F.profile_stats(k) for k in ("k_view_rotate_nn", "k_nn", "k_nn_far", "k_los_spectral")}
+    stats = {k: F.profile_stats(k) for k in ("k_view_extents_multi", "k_nn", "k_nn_far", "k_los_spectral")}
     log("per-kernel stats (count, total ms): " + json.dumps(stats))
     bytes_per = {
         # algorithmic bytes per launch
         "k_nn": S_view * 4 + n * 16 + (dims[0] * dims[1] * dims[2] + 1) * 4,
         "k_los_spectral": S_view * 16 + R * nl * 4,
-        "k_view_rotate_nn": n * 36,
+        "k_view_extents_multi": n * 16,
     }
     dom = max(stats, key=lambda k: stats[k][1])
     cnt, tot = stats[dom]

@@ -388,6 +388,54 @@ __global__ void k_view_rotate_nn(const float4* __restrict__ pts, const float4* _
   block_minmax6(v, partial + 6 * blockIdx.x);
 }
 
+// Extents of up to kViewsPerPass NN views in one pass over the points (batches): the same
+// rotation and min/max as k_view_rotate_nn per view (fomo-nearestneighbour.cpp:83-85, 95-100).
+// partial[v * vstride + 6 * block + a], the layout k_view_finalize reads.
+constexpr int kViewsPerPass = 8;
+struct ViewTrigs {
+  double cb[kViewsPerPass], sb[kViewsPerPass], cl[kViewsPerPass], sl[kViewsPerPass];
+};
+__global__ void __launch_bounds__(256) k_view_extents_multi(const float4* __restrict__ pts, int64_t n, ViewTrigs t,
+                                                            int nv, double* __restrict__ partial, int64_t vstride) {
+  double v[kViewsPerPass][6];
+#pragma unroll
+  for (int w = 0; w < kViewsPerPass; ++w)
+#pragma unroll
+    for (int a = 0; a < 6; ++a) v[w][a] = (a & 1) ? -INFINITY : INFINITY;
+  for (int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < n; p += (int64_t)gridDim.x * blockDim.x) {
+    const float4 q = pts[p];
+    const double g0 = q.x, g1 = q.y, g2 = q.z;
+#pragma unroll
+    for (int w = 0; w < kViewsPerPass; ++w) {
+      if (w >= nv) break;
+      const double cb = t.cb[w], sb = t.sb[w], cl = t.cl[w], sl = t.sl[w];
+      const double xr = g0 * cb * cl - g1 * cb * sl - g2 * sb;
+      const double yr = g0 * sl + g1 * cl;
+      const double zr = g0 * sb * cl - g1 * sb * sl + g2 * cb;
+      v[w][0] = fmin(v[w][0], xr); v[w][1] = fmax(v[w][1], xr);
+      v[w][2] = fmin(v[w][2], yr); v[w][3] = fmax(v[w][3], yr);
+      v[w][4] = fmin(v[w][4], zr); v[w][5] = fmax(v[w][5], zr);
+    }
+  }
+  __shared__ double red[kBlock / 64][kViewsPerPass * 6];
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+  for (int w = 0; w < kViewsPerPass; ++w)
+#pragma unroll
+    for (int a = 0; a < 6; ++a) {
+      const double r = (a & 1) ? wave_max(v[w][a]) : wave_min(v[w][a]);
+      if (lane == 0) red[wid][w * 6 + a] = r;
+    }
+  __syncthreads();
+  const int e = threadIdx.x;
+  if (e < kViewsPerPass * 6 && e / 6 < nv) {
+    const int w = e / 6, a = e % 6;
+    double r = red[0][e];
+    for (int k = 1; k < (int)(blockDim.x >> 6); ++k) r = (a & 1) ? fmax(r, red[k][e]) : fmin(r, red[k][e]);
+    partial[(int64_t)w * vstride + 6 * blockIdx.x + a] = r;
+  }
+}
+
 __global__ void k_view_rotate_thomson(const float4* __restrict__ pts, int64_t n, double cbc, double sbc, double cl,
                                       double sl, double blend, float* __restrict__ sin2chi,
                                       double* __restrict__ partial) {
@@ -1825,6 +1873,13 @@ template <class T> T* find_obj(std::map<int, std::unique_ptr<T>>& m, int id) {
 }
 
 // Enqueue the per-view rotate + finalize kernels (mode NN or Thomson) on stream s.
+void enqueue_view_finalize(hipStream_t s, int mode, double l, double b, int nx, int ny, int nz, int dn,
+                           const double* partial, unsigned nb, ViewGeom* geom) {
+  const double cb = cos(b), sb = sin(b), cl = cos(l), sl = sin(l);
+  const double bf = 1.0 / (1.0 + exp(-10 * (std::fabs(b) - 0.1745)));  // fomo-thomson.cpp:249
+  k_view_finalize<<<1, kBlock, 0, s>>>(partial, nb, mode, nx, ny, nz, dn, cb, sb, cl, sl, bf, geom);
+}
+
 void enqueue_view_geom(hipStream_t s, const Dataset& ds, int mode, double l, double b, int nx, int ny, int nz, int dn,
                        float* aux, double* partial, ViewGeom* geom) {
   const unsigned nb = std::min<unsigned>(nblk(ds.n), kMaxPartials);
@@ -1842,8 +1897,27 @@ void enqueue_view_geom(hipStream_t s, const Dataset& ds, int mode, double l, dou
     ProfScope ps("k_view_rotate_thomson", s);
     k_view_rotate_thomson<<<nb, kBlock, 0, s>>>(ds.pts.as<float4>(), ds.n, cbc, sbc, cl, sl, blend, aux, partial);
   }
-  const double bf = 1.0 / (1.0 + exp(-10 * (std::fabs(b) - 0.1745)));  // fomo-thomson.cpp:249
-  k_view_finalize<<<1, kBlock, 0, s>>>(partial, nb, mode, nx, ny, nz, dn, cb, sb, cl, sl, bf, geom);
+  enqueue_view_finalize(s, mode, l, b, nx, ny, nz, dn, partial, nb, geom);
+}
+
+// Extents of many NN views (a batch), kViewsPerPass views per pass over the points; partial of
+// view v at partial + v * 6 * kMaxPartials. Returns the number of partials per view.
+unsigned enqueue_views_extents(hipStream_t s, const Dataset& ds, const std::vector<double>& l,
+                               const std::vector<double>& b, double* partial) {
+  const unsigned nb = std::min<unsigned>(nblk(ds.n), kMaxPartials);
+  const int nv = (int)l.size();
+  for (int v0 = 0; v0 < nv; v0 += kViewsPerPass) {
+    ViewTrigs t{};
+    const int m = std::min(kViewsPerPass, nv - v0);
+    for (int w = 0; w < m; ++w) {
+      t.cb[w] = cos(b[v0 + w]), t.sb[w] = sin(b[v0 + w]);
+      t.cl[w] = cos(l[v0 + w]), t.sl[w] = sin(l[v0 + w]);
+    }
+    ProfScope ps("k_view_extents_multi", s);
+    k_view_extents_multi<<<nb, kBlock, 0, s>>>(ds.pts.as<float4>(), ds.n, t, m,
+                                               partial + (int64_t)v0 * 6 * kMaxPartials, 6 * kMaxPartials);
+  }
+  return nb;
 }
 
 // Per-point records shared by the lines of a render: RS4 float4 per point.
@@ -2974,6 +3048,8 @@ static int enqueue_batch(Batch& bt, hipStream_t s, hipStream_t s2) {
   const size_t outv = (size_t)R * std::max(bt.nl, 1);
   const int L = (int)gs.size();
   if (s2 && (int)bt.ev.size() < 2 * nv) return fail(FOMO_ERR_STATE, "batch events missing");
+  // extents of all views: one pass over the points per kViewsPerPass views
+  const unsigned nbp = enqueue_views_extents(s, *ds, bt.l, bt.b, bt.partial.as<double>());
   for (int v = 0; v < nv; ++v) {
     const int buf = s2 ? (v & 1) : 0;
     float* svl = bt.aux.as<float>() + buf * S;
@@ -2982,7 +3058,7 @@ static int enqueue_batch(Batch& bt, hipStream_t s, hipStream_t s2) {
     double* part = bt.partial.as<double>() + (size_t)v * 6 * kMaxPartials;
     ViewGeom* geom = bt.geom.as<ViewGeom>() + v;
     if (s2 && v >= 2) HIPOK(hipStreamWaitEvent(s, bt.ev[nv + v - 2], 0));
-    enqueue_view_geom(s, *ds, MODE_NN, bt.l[v], bt.b[v], bt.nx, bt.ny, bt.nz, bt.dn, nullptr, part, geom);
+    enqueue_view_finalize(s, MODE_NN, bt.l[v], bt.b[v], bt.nx, bt.ny, bt.nz, bt.dn, part, nbp, geom);
     enqueue_nn(s, *ds, geom, bt.nx, bt.ny, bt.nz, bt.idx.as<int32_t>() + buf * S, bt.rayflag.as<uint8_t>() + buf * R,
                bt.rec.as<float4>(), bt.rs4, L, svl, sem, bt.ri);
     std::vector<const float2*> ems;
