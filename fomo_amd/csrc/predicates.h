@@ -6,7 +6,7 @@
 // strictly inside the circumsphere of a,b,c,d. Both are evaluated in double with a conservative error
 // filter; when the filter cannot decide, the determinant is recomputed exactly with floating-point
 // expansion arithmetic (sums and products of non-overlapping double components, the method of
-// Priest and Shewchuk: two-sum, two-product via FMA, expansion sum, expansion scaling).
+// Priest and Shewchuk: two-sum, Dekker's two-product, expansion sum, scaling and compression).
 //
 // Degenerate (cospherical) cases get a symbolic perturbation of the lifted coordinate:
 // w_i -> w_i + eps^rank(i), ranks by descending lexicographic (x, y, z) order, which makes every
@@ -21,7 +21,47 @@
 namespace fomo {
 namespace pred {
 
-using Exp = std::vector<double>;  // non-overlapping components, increasing magnitude
+// Expansion: non-overlapping double components in increasing magnitude. Inline storage for the
+// common short expansions, heap beyond that (no allocation on the usual exact paths).
+class Exp {
+ public:
+  Exp() = default;
+  Exp(const Exp& o) { assign(o); }
+  Exp& operator=(const Exp& o) {
+    if (this != &o) assign(o);
+    return *this;
+  }
+  size_t size() const { return n_; }
+  bool empty() const { return n_ == 0; }
+  double operator[](size_t i) const { return data()[i]; }
+  double& operator[](size_t i) { return data()[i]; }
+  void reserve(size_t m) {
+    if (m <= cap()) return;
+    std::vector<double> h(m);
+    for (size_t i = 0; i < n_; ++i) h[i] = data()[i];
+    heap_.swap(h);
+  }
+  void push_back(double x) {
+    if (n_ == cap()) reserve(2 * cap());
+    data()[n_++] = x;
+  }
+
+ private:
+  static constexpr size_t kInline = 96;
+  double inl_[kInline];
+  std::vector<double> heap_;
+  size_t n_ = 0;
+  size_t cap() const { return heap_.empty() ? kInline : heap_.size(); }
+  double* data() { return heap_.empty() ? inl_ : heap_.data(); }
+  const double* data() const { return heap_.empty() ? inl_ : heap_.data(); }
+  void assign(const Exp& o) {
+    n_ = 0;
+    heap_.clear();
+    reserve(o.n_);
+    for (size_t i = 0; i < o.n_; ++i) data()[i] = o[i];
+    n_ = o.n_;
+  }
+};
 
 inline void two_sum(double a, double b, double& x, double& y) {
   x = a + b;
@@ -30,43 +70,51 @@ inline void two_sum(double a, double b, double& x, double& y) {
   y = (a - av) + (b - bv);
 }
 
-inline void two_prod(double a, double b, double& x, double& y) {
-  x = a * b;
-  y = std::fma(a, b, -x);
+// Veltkamp split of a double into two 26-bit halves (hi + lo == a exactly)
+inline void split(double a, double& hi, double& lo) {
+  const double c = 134217729.0 * a;  // 2^27 + 1
+  const double big = c - a;
+  hi = c - big;
+  lo = a - hi;
 }
 
-// h = e + f (both expansions), zero components dropped
-inline Exp add(const Exp& e, const Exp& f) {
+// x + y == a * b exactly (Dekker's product; no FMA instruction or library call needed)
+inline void two_prod(double a, double b, double& x, double& y) {
+  x = a * b;
+  double ah, al, bh, bl;
+  split(a, ah, al);
+  split(b, bh, bl);
+  const double e1 = x - ah * bh;
+  const double e2 = e1 - al * bh;
+  const double e3 = e2 - ah * bl;
+  y = al * bl - e3;
+}
+
+// h = e + (sign) f (both expansions), zero components dropped: merge by magnitude, then
+// accumulate with two-sum
+inline Exp add_signed(const Exp& e, const Exp& f, double fs) {
   Exp h;
-  h.reserve(e.size() + f.size());
-  // merge by magnitude, then accumulate with two-sum (linear expansion sum)
-  Exp g;
-  g.reserve(e.size() + f.size());
+  const size_t ne = e.size(), nf = f.size();
+  if (ne + nf == 0) return h;
+  h.reserve(ne + nf);
   size_t i = 0, j = 0;
-  while (i < e.size() && j < f.size()) {
-    if (std::fabs(e[i]) < std::fabs(f[j])) g.push_back(e[i++]);
-    else g.push_back(f[j++]);
-  }
-  while (i < e.size()) g.push_back(e[i++]);
-  while (j < f.size()) g.push_back(f[j++]);
-  if (g.empty()) return h;
-  double q = g[0];
-  for (size_t k = 1; k < g.size(); ++k) {
-    double s, t;
-    two_sum(q, g[k], s, t);
+  auto next = [&]() -> double {
+    if (i < ne && (j >= nf || std::fabs(e[i]) < std::fabs(f[j]))) return e[i++];
+    return fs * f[j++];
+  };
+  double q = next();
+  for (size_t k = 1; k < ne + nf; ++k) {
+    double sm, t;
+    two_sum(q, next(), sm, t);
     if (t != 0.0) h.push_back(t);
-    q = s;
+    q = sm;
   }
   if (q != 0.0 || h.empty()) h.push_back(q);
   return h;
 }
 
-inline Exp neg(Exp e) {
-  for (double& v : e) v = -v;
-  return e;
-}
-
-inline Exp sub(const Exp& e, const Exp& f) { return add(e, neg(f)); }
+inline Exp add(const Exp& e, const Exp& f) { return add_signed(e, f, 1.0); }
+inline Exp sub(const Exp& e, const Exp& f) { return add_signed(e, f, -1.0); }
 
 // h = e * b
 inline Exp scale(const Exp& e, double b) {
@@ -77,21 +125,57 @@ inline Exp scale(const Exp& e, double b) {
   two_prod(e[0], b, q, t);
   if (t != 0.0) h.push_back(t);
   for (size_t i = 1; i < e.size(); ++i) {
-    double p1, p0, s, u;
+    double p1, p0, sm, u;
     two_prod(e[i], b, p1, p0);
-    two_sum(q, p0, s, u);
+    two_sum(q, p0, sm, u);
     if (u != 0.0) h.push_back(u);
-    two_sum(p1, s, q, u);
+    two_sum(p1, sm, q, u);
     if (u != 0.0) h.push_back(u);
   }
   if (q != 0.0 || h.empty()) h.push_back(q);
   return h;
 }
 
+// Shortest equivalent expansion (Priest/Shewchuk compression): a top-down pass gathers the
+// components into non-adjacent ones, a bottom-up pass renormalises. The value is unchanged.
+inline Exp compress(const Exp& e) {
+  const size_t m = e.size();
+  if (m < 3) return e;
+  Exp g;
+  g.reserve(m);
+  for (size_t i = 0; i < m; ++i) g.push_back(0.0);
+  size_t bottom = m - 1;
+  double q = e[m - 1];
+  for (size_t i = m - 1; i-- > 0;) {
+    double qn, r;
+    two_sum(q, e[i], qn, r);
+    if (r != 0.0) {
+      g[bottom--] = qn;
+      q = r;
+    } else {
+      q = qn;
+    }
+  }
+  g[bottom] = q;
+  Exp h;
+  h.reserve(m - bottom);
+  for (size_t i = bottom + 1; i < m; ++i) {
+    double qn, r;
+    two_sum(g[i], q, qn, r);
+    if (r != 0.0) h.push_back(r);
+    q = qn;
+  }
+  h.push_back(q);
+  return h;
+}
+
 inline Exp mul(const Exp& e, const Exp& f) {
   Exp h;
-  for (double c : f) h = add(h, scale(e, c));
-  return h;
+  for (size_t i = 0; i < f.size(); ++i) {
+    h = add(h, scale(e, f[i]));
+    if (h.size() > 16) h = compress(h);
+  }
+  return h.size() > 4 ? compress(h) : h;
 }
 
 inline Exp diff(double a, double b) {
