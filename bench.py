@@ -252,7 +252,8 @@ C4_NL = (1, 30, 100)
 
 def draw_requests(F, targets, width, rng, count):
     """targets: dicts with "spec" (spectroscopic emission), "imag" (imaging emission, nl = 1) or
-    "thomson" (Thomson emission), a "weight" and a "name". Views l ~ U(0, 2 pi), b ~ U(0.1, pi/2)."""
+    "thomson" (Thomson emission), a "weight" and a "name"; "tet": True sends the target's requests
+    through the tetra path. Views l ~ U(0, 2 pi), b ~ U(0.1, pi/2)."""
     w = np.array([t["weight"] for t in targets], float)
     w /= w.sum()
     reqs = []
@@ -262,12 +263,15 @@ def draw_requests(F, targets, width, rng, count):
         nz = int(rng.choice(C4_NZ))
         nl = int(rng.choice(C4_NL))
         l, b = float(rng.uniform(0, 2 * np.pi)), float(rng.uniform(0.1, np.pi / 2))
+        method = F.METHOD_NN
+        if "tet" in t:  # tetra path: the CGAL method or its linear variant, half each
+            method = F.METHOD_TET if rng.random() < 0.5 else F.METHOD_LINEAR
         if "thomson" in t:
             reqs.append((t["thomson"], F.METHOD_THOMSON, l, b, nx, nx, nz, 1, 0.0))
         elif nl == 1:
-            reqs.append((t["imag"], F.METHOD_NN, l, b, nx, nx, nz, 1, 0.0))
+            reqs.append((t["imag"], method, l, b, nx, nx, nz, 1, 0.0))
         else:
-            reqs.append((t["spec"], F.METHOD_NN, l, b, nx, nx, nz, nl, width))
+            reqs.append((t["spec"], method, l, b, nx, nx, nz, nl, width))
     return reqs
 
 
@@ -299,6 +303,8 @@ def serve_measure(F, targets, width, rng, count, loads, device_bytes=24 << 30, h
         for r in reqs:
             if r[1] == F.METHOD_THOMSON:
                 F.render_thomson(r[0], *r[2:7])
+            elif r[1] in (F.METHOD_TET, F.METHOD_LINEAR):
+                F.render_tet(r[0], *r[2:8], r[8], linear=r[1] == F.METHOD_LINEAR)
             else:
                 F.render_nn(r[0], *r[2:8], r[8])
         dt_seq = time.perf_counter() - t0
@@ -342,8 +348,9 @@ def serve_measure(F, targets, width, rng, count, loads, device_bytes=24 << 30, h
 
 def serve(F, args):
     """Serving workload: resident 128^3 and 256^3 loop cubes (a spectroscopic and an imaging
-    emission cube each) and a corona for Thomson requests; requests drawn uniformly from the size
-    mix, Poisson arrivals at 0.5, 0.8 and 0.95 x the saturated request rate."""
+    emission cube each) for nearest-neighbour requests, a triangulated jittered 128^3 cube for
+    tetra (CGAL and linear) requests and a corona for Thomson requests; requests drawn uniformly
+    from the size mix, Poisson arrivals at 0.5, 0.8 and 0.95 x the saturated request rate."""
     tmp = tempfile.mkdtemp()
     path = F.gen_goft_table(os.path.join(tmp, "fe9.dat"), ion="fe_9", lambda0=171.0, atweight=56.0, peak_logt=6.2)
     tab = F.Table.from_file(path)
@@ -356,7 +363,17 @@ def serve(F, args):
         es = F.Emission.chianti(ds, tab, lambda0, atw)
         ei = F.Emission.chianti(ds, tab, lambda0, atw, imaging=True)
         objs += [es, ei, ds]
-        targets.append({"spec": es, "imag": ei, "weight": 0.45, "name": f"{m}^3 loop (NN)"})
+        targets.append({"spec": es, "imag": ei, "weight": 0.35, "name": f"{m}^3 loop (NN)"})
+    # linear-interpolation requests: the tetra path on a jittered 128^3 cube (triangulated once)
+    d = F.gen_loop_cube(128, seed=5, jitter=0.3)
+    dst = F.Dataset(d["x"], d["y"], d["z"], [d["rho"], d["T"], d["vx"], d["vy"], d["vz"]])
+    ntets, tsec = dst.triangulate()
+    log(f"triangulated the jittered 128^3 cube: {ntets} tets in {tsec:.1f}s")
+    es = F.Emission.chianti(dst, tab, lambda0, atw)
+    ei = F.Emission.chianti(dst, tab, lambda0, atw, imaging=True)
+    objs += [es, ei, dst]
+    targets.append({"spec": es, "imag": ei, "tet": True, "weight": 0.2,
+                    "name": "jittered 128^3 loop (CGAL / Linear tetra path)"})
     c = F.gen_corona(args.corona_m if args.corona_m != 645 else 128, seed=3, sigma=0.1)
     dsc = F.Dataset(c["x"], c["y"], c["z"], [c["rho"]])
     et = F.Emission.thomson(dsc)

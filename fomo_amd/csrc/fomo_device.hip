@@ -130,6 +130,14 @@ struct LookupInfo {
   int enabled;
 };
 
+// Start grid of the tetra walk (a finite tet near each cell centre) and the points' bounding box,
+// which contains the convex hull: a sample outside the box is outside the hull without a walk.
+struct TetGrid {
+  double o[3], h, invh;
+  double lo[3], hi[3];
+  int dims[3];
+};
+
 // Per-view geometry, filled on the device by k_view_finalize.
 struct ViewGeom {
   double cb, sb, cl, sl;          // trig of b and l for the LOS samples (host glibc)
@@ -974,8 +982,9 @@ __global__ void k_clear_u8(uint8_t* __restrict__ a, int64_t n) {
 // plane to within 1e-12 of the determinant's magnitude) counts as "not beyond" the face, so a
 // sample on a shared face stays in the tet the walk reached first.
 // ------------------------------------------------------------------------------------
-__constant__ int c_face[4][3] = {{2, 1, 3}, {0, 2, 3}, {1, 0, 3}, {0, 1, 2}};
 constexpr int kTetMaxSteps = 1 << 16;
+constexpr int64_t kTetThreads = 1 << 18;  // segments per launch worth splitting rays for
+constexpr int kTetMaxSegment = 16;          // longest run of LOS samples one thread walks
 
 __device__ __forceinline__ int orient_d(const float4 a, const float4 b, const float4 c, double px, double py,
                                         double pz, double* det_out = nullptr) {
@@ -994,18 +1003,55 @@ __device__ __forceinline__ int orient_d(const float4 a, const float4 b, const fl
 
 __device__ __forceinline__ int ivec(const int4 v, int i) { return i == 0 ? v.x : (i == 1 ? v.y : (i == 2 ? v.z : v.w)); }
 
-// One thread per ray. Writes per-sample svlos / sem like k_nn (RS4 > 0) and, when vert != null,
-// the chosen vertex (sorted position, -1 outside) per sample in k-major order.
+__device__ __forceinline__ int tet_start(const TetGrid& tg, const int32_t* __restrict__ tstart, double px, double py,
+                                         double pz) {
+  int c[3];
+  const double pc[3] = {px, py, pz};
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    const double v = (pc[a] - tg.o[a]) * tg.invh;
+    c[a] = v < 0 ? 0 : (v >= tg.dims[a] ? tg.dims[a] - 1 : (int)v);
+  }
+  return tstart[((int64_t)c[2] * tg.dims[1] + c[1]) * tg.dims[0] + c[0]];
+}
+
+// Tet records: the vertex coordinates sit beside the vertex ids, so one step of the walk is one
+// round of loads (record + neighbours) instead of two dependent ones.
+__global__ void k_tet_records(const int4* __restrict__ tv, const float4* __restrict__ pts, int64_t n,
+                              float4* __restrict__ trec) {
+  const int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (q >= n) return;
+  const int4 v = tv[q];
+#pragma unroll
+  for (int m = 0; m < 4; ++m) {
+    const int id = ivec(v, m);
+    float4 r = id >= 0 ? pts[id] : make_float4(0.f, 0.f, 0.f, 0.f);
+    r.w = __int_as_float(id);
+    trec[4 * q + m] = r;
+  }
+}
+
+// One thread per segment of seglen samples of a ray: the walk is a chain of dependent loads, so
+// small views split their rays to keep enough walks in flight. Each segment's first sample starts
+// from the start grid, the others from the previous sample's tet. A warm start on a hull tet whose
+// face does not exclude the sample (a ray entering the hull) restarts from the start grid too, so
+// no walk crosses the hull surface from where the ray left it. Writes per-sample svlos / sem like
+// k_nn (RS4 > 0) and, when vert != null, the chosen vertex (sorted position, -1 outside) per
+// sample in k-major order.
 template <int RS4, bool LINEAR>
-__global__ void __launch_bounds__(256) k_tet_walk(const float4* __restrict__ pts, const int4* __restrict__ tv,
-                                                  const int4* __restrict__ tn, const int32_t* __restrict__ tstart,
-                                                  LookupInfo tk, const ViewGeom* __restrict__ vgp, int nx, int ny,
-                                                  int nz, const float4* __restrict__ rec, RecInfo ri, int nlines,
+__global__ void __launch_bounds__(256) k_tet_walk(const float4* __restrict__ trec, const int4* __restrict__ tn,
+                                                  const int32_t* __restrict__ tstart, TetGrid tg,
+                                                  const ViewGeom* __restrict__ vgp, int nx, int ny, int nz,
+                                                  int seglen, const float4* __restrict__ rec, RecInfo ri, int nlines,
                                                   float* __restrict__ svlos, float2* __restrict__ sem,
                                                   int32_t* __restrict__ vert) {
   const int64_t R = (int64_t)nx * ny;
-  const int64_t ray = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (ray >= R) return;
+  const int64_t tid = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const int nseg = (nz + seglen - 1) / seglen;
+  if (tid >= R * nseg) return;
+  // neighbouring threads take neighbouring rays (coalesced sample writes)
+  const int64_t ray = tid % R;
+  const int seg = (int)(tid / R);
   const int j = (int)(ray % nx), i = (int)(ray / nx);
   const ViewGeom vg = *vgp;
   const double minx = vg.ext[0], maxx = vg.ext[1], miny = vg.ext[2], maxy = vg.ext[3], minz = vg.ext[4],
@@ -1014,59 +1060,62 @@ __global__ void __launch_bounds__(256) k_tet_walk(const float4* __restrict__ pts
   const double y = double(i) / (ny - 1) * (maxy - miny) + miny;  // :188
   const double cb = vg.cb, sb = vg.sb, cl = vg.cl, sl = vg.sl;
   const size_t S = (size_t)R * nz;
-  int t = -1;
-  for (int k = 0; k < nz; ++k) {
+  int t = -1;  // tet of the previous sample (-1: none)
+  const int k0 = seg * seglen, k1 = min(nz, k0 + seglen);
+  for (int k = k0; k < k1; ++k) {
     // :189 (a single z pixel has no spacing: the sample sits at minz)
     const double z = nz > 1 ? double(k) / (nz - 1) * (maxz - minz) + minz : minz;
     const double px = x * cb * cl + y * sl + z * sb * cl;  // :192
     const double py = -x * cb * sl + y * cl - z * sb * sl;
     const double pz = -x * sb + z * cb;
-    if (t < 0) {
-      int c[3];
-      const double pc[3] = {px, py, pz};
-#pragma unroll
-      for (int a = 0; a < 3; ++a) {
-        const double v = (pc[a] - tk.o[a]) * tk.invh;
-        c[a] = v < 0 ? 0 : (v >= tk.dims[a] ? tk.dims[a] - 1 : (int)v);
-      }
-      t = tstart[((int64_t)c[2] * tk.dims[1] + c[1]) * tk.dims[0] + c[0]];
-    }
-    bool outside = false;
-    int steps = 0;
-    int4 v;
+    const size_t sidx = (size_t)k * R + ray;
+    bool outside = px < tg.lo[0] || px > tg.hi[0] || py < tg.lo[1] || py > tg.hi[1] || pz < tg.lo[2] ||
+                   pz > tg.hi[2];
     float4 q[4];
-    for (;;) {
-      v = tv[t];
-      const int4 nb = tn[t];
-      int inf = -1;
+    int4 nb;
+    if (!outside) {
+      const bool warm = t >= 0;
+      if (!warm) t = tet_start(tg, tstart, px, py, pz);
+      int steps = 0;
+      for (;;) {
+        const float4* tr = trec + (size_t)t * 4;
+        q[0] = tr[0], q[1] = tr[1], q[2] = tr[2], q[3] = tr[3];
+        nb = tn[t];
+        int inf = -1;
 #pragma unroll
-      for (int m = 0; m < 4; ++m)
-        if (ivec(v, m) < 0) inf = m;
-      if (inf >= 0) {
-        const float4 a = pts[ivec(v, c_face[inf][0])], b = pts[ivec(v, c_face[inf][1])],
-                     c = pts[ivec(v, c_face[inf][2])];
-        if (orient_d(a, b, c, px, py, pz) > 0) {
+        for (int m = 0; m < 4; ++m)
+          if (__float_as_int(q[m].w) < 0) inf = m;
+        if (inf >= 0) {
+          // hull face opposite the infinite vertex (the face order of fomo/delaunay.h)
+          int o;
+          switch (inf) {
+            case 0: o = orient_d(q[2], q[1], q[3], px, py, pz); break;
+            case 1: o = orient_d(q[0], q[2], q[3], px, py, pz); break;
+            case 2: o = orient_d(q[1], q[0], q[3], px, py, pz); break;
+            default: o = orient_d(q[0], q[1], q[2], px, py, pz); break;
+          }
+          if (o > 0) {
+            outside = true;
+            break;
+          }
+          t = (steps == 0 && warm) ? tet_start(tg, tstart, px, py, pz) : ivec(nb, inf);
+        } else {
+          // faces opposite v0..v3: (2,1,3), (0,2,3), (1,0,3), (0,1,2)
+          int next = -1;
+          if (orient_d(q[2], q[1], q[3], px, py, pz) < 0) next = nb.x;
+          else if (orient_d(q[0], q[2], q[3], px, py, pz) < 0) next = nb.y;
+          else if (orient_d(q[1], q[0], q[3], px, py, pz) < 0) next = nb.z;
+          else if (orient_d(q[0], q[1], q[2], px, py, pz) < 0) next = nb.w;
+          if (next < 0) break;
+          t = next;
+        }
+        if (++steps > kTetMaxSteps) {  // never expected (the walk on a Delaunay triangulation ends)
           outside = true;
+          t = -1;
           break;
         }
-        t = ivec(nb, inf);
-      } else {
-#pragma unroll
-        for (int m = 0; m < 4; ++m) q[m] = pts[ivec(v, m)];
-        int next = -1;
-#pragma unroll
-        for (int f = 0; f < 4; ++f)
-          if (next < 0 && orient_d(q[c_face[f][0]], q[c_face[f][1]], q[c_face[f][2]], px, py, pz) < 0)
-            next = ivec(nb, f);
-        if (next < 0) break;
-        t = next;
-      }
-      if (++steps > kTetMaxSteps) {  // never expected (the walk on a Delaunay triangulation ends)
-        outside = true;
-        break;
       }
     }
-    const size_t sidx = (size_t)k * R + ray;
     if (outside) {
       if (vert) vert[sidx] = -1;
       if constexpr (RS4 > 0) write_sample<RS4>(-1, rec, ri, nlines, S, sidx, cb, sb, cl, sl, svlos, sem);
@@ -1085,20 +1134,22 @@ __global__ void __launch_bounds__(256) k_tet_walk(const float4* __restrict__ pts
           best = m;
         }
       }
-      const int r = ivec(v, best);
+      const int r = __float_as_int(best == 0 ? q[0].w : (best == 1 ? q[1].w : (best == 2 ? q[2].w : q[3].w)));
       if (vert) vert[sidx] = r;
       if constexpr (RS4 > 0) write_sample<RS4>(r, rec, ri, nlines, S, sidx, cb, sb, cl, sl, svlos, sem);
     } else {
       // barycentric weights: volume of the tet with vertex m replaced by the sample, over the volume
       double w[4], tot = 0.0;
+      (void)orient_d(q[2], q[1], q[3], px, py, pz, &w[0]);
+      (void)orient_d(q[0], q[2], q[3], px, py, pz, &w[1]);
+      (void)orient_d(q[1], q[0], q[3], px, py, pz, &w[2]);
+      (void)orient_d(q[0], q[1], q[2], px, py, pz, &w[3]);
 #pragma unroll
       for (int f = 0; f < 4; ++f) {
-        double det;
-        (void)orient_d(q[c_face[f][0]], q[c_face[f][1]], q[c_face[f][2]], px, py, pz, &det);
-        w[f] = fmax(det, 0.0);
+        w[f] = fmax(w[f], 0.0);
         tot += w[f];
       }
-      if (vert) vert[sidx] = ivec(v, 0);
+      if (vert) vert[sidx] = __float_as_int(q[0].w);
       if constexpr (RS4 > 0) {
         constexpr int NL = 4 * RS4 - 4;  // lines in a record
         double vl = 0.0, pk[NL], fw[NL];
@@ -1107,7 +1158,7 @@ __global__ void __launch_bounds__(256) k_tet_walk(const float4* __restrict__ pts
 #pragma unroll
         for (int m = 0; m < 4; ++m) {
           const double wm = tot > 0 ? w[m] / tot : 0.25;
-          const float4* rp = rec + (size_t)ivec(v, m) * RS4;
+          const float4* rp = rec + (size_t)__float_as_int(q[m].w) * RS4;
           float f[4 * RS4];
 #pragma unroll
           for (int qq = 0; qq < RS4; ++qq) {
@@ -1354,9 +1405,10 @@ struct Dataset {
   int64_t ncand = 0, nlcells = 0, nfallback = 0;
   // tetra path: Delaunay triangulation of the points, built on the host on first use
   int64_t ntets = 0;      // 0: not built
-  DevBuf tv, tn;          // int4 per tet: vertices (sorted positions, -1 = infinite), neighbours
+  DevBuf trec;            // 4 float4 per tet: vertex {x, y, z, sorted position as int bits (-1 = infinite)}
+  DevBuf tn;              // int4 per tet: neighbour opposite each vertex
   DevBuf tstart;          // int32 per start-grid cell: a finite tet near the cell centre
-  LookupInfo tk{};        // start grid geometry
+  TetGrid tk{};           // start grid geometry
   double tet_build_s = 0;
   std::atomic<int> inuse{0};  // batches and queued/running server requests reading this dataset
 };
@@ -1591,12 +1643,13 @@ int build_tets(Dataset& ds) {
   std::vector<int32_t> sorted_of(n);
   for (int64_t p = 0; p < n; ++p) sorted_of[perm[p]] = (int32_t)p;
   // start grid: cells ~ 4 points each along an axis, at most 64 per axis
-  LookupInfo tk{};
+  TetGrid tk{};
   double ext = 0;
   for (int a = 0; a < 3; ++a) ext = std::max(ext, (double)ds.bbox[2 * a + 1] - ds.bbox[2 * a]);
   const double h = std::max(ext / 64.0, ext > 0 ? ext * 1e-6 : 1.0);
   for (int a = 0; a < 3; ++a) {
-    tk.o[a] = ds.bbox[2 * a];
+    tk.o[a] = tk.lo[a] = ds.bbox[2 * a];
+    tk.hi[a] = ds.bbox[2 * a + 1];
     tk.dims[a] = std::max(1, std::min(64, (int)std::ceil(((double)ds.bbox[2 * a + 1] - ds.bbox[2 * a]) / h)));
   }
   tk.h = h;
@@ -1642,14 +1695,21 @@ int build_tets(Dataset& ds) {
     hv[q] = make_int4(vv[0], vv[1], vv[2], vv[3]);
     hn[q] = make_int4(T[q].n[0], T[q].n[1], T[q].n[2], T[q].n[3]);
   }
-  if (ds.tv.alloc(sizeof(int4) * hv.size()) || ds.tn.alloc(sizeof(int4) * hn.size()) ||
-      ds.tstart.alloc(sizeof(int32_t) * ng))
+  DevBuf tv;
+  if (tv.alloc(sizeof(int4) * hv.size()) || ds.trec.alloc(4 * sizeof(float4) * hv.size()) ||
+      ds.tn.alloc(sizeof(int4) * hn.size()) || ds.tstart.alloc(sizeof(int32_t) * ng)) {
+    ds.trec.release();
+    ds.tn.release();
+    ds.tstart.release();
     return fail(FOMO_ERR_NOMEM, "device allocation failed (tets)");
-  HIPOK(hipMemcpyAsync(ds.tv.p, hv.data(), sizeof(int4) * hv.size(), hipMemcpyHostToDevice, s));
+  }
+  HIPOK(hipMemcpyAsync(tv.p, hv.data(), sizeof(int4) * hv.size(), hipMemcpyHostToDevice, s));
   HIPOK(hipMemcpyAsync(ds.tn.p, hn.data(), sizeof(int4) * hn.size(), hipMemcpyHostToDevice, s));
   HIPOK(hipMemcpyAsync(ds.tstart.p, start.data(), sizeof(int32_t) * ng, hipMemcpyHostToDevice, s));
+  k_tet_records<<<nblk((int64_t)hv.size()), kBlock, 0, s>>>(tv.as<int4>(), ds.pts.as<float4>(), (int64_t)hv.size(),
+                                                             ds.trec.as<float4>());
+  HIPOK(hipGetLastError());
   if (int rc = sync_stream()) return rc;
-  tk.enabled = 1;
   ds.tk = tk;
   ds.ntets = (int64_t)T.size();
   ds.tet_build_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
@@ -1758,7 +1818,7 @@ int build_index(Dataset& ds, double cell_scale) {
   const int64_t n = ds.n;
   // the triangulation refers to sorted positions, which the new cell order changes
   ds.ntets = 0;
-  ds.tv.release();
+  ds.trec.release();
   ds.tn.release();
   ds.tstart.release();
   if (cell_scale <= 0) cell_scale = 1.0;
@@ -1967,11 +2027,15 @@ int enqueue_tet(hipStream_t s, const Dataset& ds, const ViewGeom* geom, int nx, 
                 const float4* rec, int rs4, const RecInfo& ri, int nlines, float* svlos, float2* sem, int32_t* vert) {
   if (ds.ntets <= 0) return fail(FOMO_ERR_STATE, "dataset has no triangulation (fomo_dataset_triangulate)");
   const int64_t R = (int64_t)nx * ny;
-  const unsigned nb = (unsigned)((R + 255) / 256);
+  // split rays into segments only as far as needed to keep kTetThreads walks in flight
+  const int64_t want = (R * nz + kTetThreads - 1) / kTetThreads;
+  const int seglen = (int)std::max<int64_t>(1, std::min<int64_t>({want, (int64_t)kTetMaxSegment, (int64_t)nz}));
+  const int64_t nthreads = R * ((nz + seglen - 1) / seglen);
+  const unsigned nb = (unsigned)((nthreads + 255) / 256);
   ProfScope ps(linear ? "k_tet_walk_linear" : "k_tet_walk", s);
-#define FOMO_TET_ARGS                                                                                           \
-  ds.pts.as<float4>(), ds.tv.as<int4>(), ds.tn.as<int4>(), ds.tstart.as<int32_t>(), ds.tk, geom, nx, ny, nz, rec, \
-      ri, nlines, svlos, sem, vert
+#define FOMO_TET_ARGS                                                                                        \
+  ds.trec.as<float4>(), ds.tn.as<int4>(), ds.tstart.as<int32_t>(), ds.tk, geom, nx, ny, nz, seglen, rec, ri, \
+      nlines, svlos, sem, vert
   if (!rec) {
     if (linear) return fail(FOMO_ERR_ARG, "linear interpolation needs per-point records");
     k_tet_walk<0, false><<<nb, 256, 0, s>>>(FOMO_TET_ARGS);
