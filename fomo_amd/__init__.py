@@ -11,8 +11,8 @@ import numpy as np
 
 __all__ = [
     "lib", "FomoError", "init", "shutdown", "device_info", "Dataset", "Table", "Emission", "Batch",
-    "render_nn", "render_thomson", "nn_indices", "Server", "METHOD_NN", "METHOD_THOMSON", "METHOD_TET", "METHOD_LINEAR", "delaunay", "render_tet", "autotune",
-    "tet_vertices", "gen_loop_cube", "gen_goft_table", "gen_corona",
+    "render_nn", "render_thomson", "nn_indices", "Server", "METHOD_NN", "METHOD_THOMSON", "METHOD_TET", "METHOD_LINEAR", "METHOD_PROJECTION", "delaunay", "render_tet",
+    "autotune", "tet_vertices", "render_projection", "projection_pixels", "gen_loop_cube", "gen_goft_table", "gen_corona",
     "read_table", "profile_enable", "profile_reset", "profile_stats", "LIB_PATH",
 ]
 
@@ -73,6 +73,11 @@ _SIGS = {
     "fomo_render_tet": (ctypes.c_int, [ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_int,
                                        ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_int,
                                        _c_flt_p, _c_dbl_p]),
+    "fomo_render_projection": (ctypes.c_int, [ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_int,
+                                              ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_int,
+                                              _c_flt_p, _c_dbl_p]),
+    "fomo_projection_pixels": (ctypes.c_int, [ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_int,
+                                              ctypes.c_int, ctypes.c_int, _c_i32_p, _c_dbl_p]),
     "fomo_dataset_triangulate": (ctypes.c_int, [ctypes.c_int, _c_i64_p, _c_dbl_p]),
     "fomo_tet_vertices": (ctypes.c_int, [ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_int,
                                          ctypes.c_int, _c_i32_p, _c_dbl_p]),
@@ -348,6 +353,25 @@ def render_tet(em, l, b, nx, ny, nz, nl, lambda_width, dn=False, linear=False):
     return img, meta
 
 
+def render_projection(em, l, b, nx, ny, nz, nl, lambda_width, dn=False):
+    """One view of the projection method (each point's line profile added to the pixel of its
+    rotated position); returns (image[ny*nx*max(nl,1)] float32, meta[10])."""
+    meta = np.zeros(10, np.float64)
+    img = np.zeros(nx * ny * max(nl, 1), np.float32)
+    _check(lib().fomo_render_projection(em.id, l, b, nx, ny, nz, nl, lambda_width, 1 if dn else 0,
+                                        _ptr(img, ctypes.c_float), _ptr(meta, ctypes.c_double)))
+    return img, meta
+
+
+def projection_pixels(ds, l, b, nx, ny, nz=2):
+    """Pixel iy*nx + jx of every point (original order) for one projection view; returns (pixels, meta)."""
+    out = np.zeros(ds.n, np.int32)
+    meta = np.zeros(10, np.float64)
+    _check(lib().fomo_projection_pixels(ds.id, l, b, nx, ny, nz, _ptr(out, ctypes.c_int32),
+                                        _ptr(meta, ctypes.c_double)))
+    return out, meta
+
+
 def tet_vertices(ds, l, b, nx, ny, nz):
     out = np.zeros(nx * ny * nz, np.int32)
     meta = np.zeros(10, np.float64)
@@ -408,6 +432,7 @@ METHOD_NN = 0
 METHOD_THOMSON = 1
 METHOD_TET = 2
 METHOD_LINEAR = 3
+METHOD_PROJECTION = 4
 
 
 class Server:

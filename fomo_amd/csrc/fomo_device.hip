@@ -147,7 +147,7 @@ struct ViewGeom {
   double pathlength, apix, scale; // output scaling
 };
 
-enum { MODE_NN = 0, MODE_THOMSON = 1, MODE_TET = 2, MODE_LINEAR = 3 };
+enum { MODE_NN = 0, MODE_THOMSON = 1, MODE_TET = 2, MODE_LINEAR = 3, MODE_PROJ = 4 };
 
 // Spectral lines rendered together from one NN result (blockIdx.y selects the line).
 constexpr int kMaxLines = 8;
@@ -1182,6 +1182,89 @@ __global__ void __launch_bounds__(256) k_tet_walk(const float4* __restrict__ tre
   }
 }
 
+// ------------------------------------------------------------------------------------
+// Projection method (fomo-projection.cpp:17-214): every point adds its line profile to the pixel
+// its rotated (x', y') rounds to (:126-158). The reference scatters with OpenMP atomics, so its
+// summation order is not fixed. Here the points are sorted by pixel with a stable radix sort of
+// (pixel, sorted position) pairs laid out in original point order, and each (pixel, lambda) sums
+// its points in that order in float, as intens.at(ind) += tempintens does (:144, :154): the sums
+// of a one-thread run of the reference.
+// ------------------------------------------------------------------------------------
+__device__ __forceinline__ int proj_pixel(double v, double mn, double mx, int n) {
+  // :129, :131 with std::round (half away from zero). A degenerate extent has no pixel formula in
+  // the reference (0/0): every point goes to pixel 0. The clamp only guards memory.
+  if (!(mx > mn)) return 0;
+  const int q = (int)round((v - mn) * (n - 1) / (mx - mn));
+  return q < 0 ? 0 : (q > n - 1 ? n - 1 : q);
+}
+
+// key[o] = pixel of the point with original index o, val[o] = its sorted position.
+__global__ void k_proj_keys(const float4* __restrict__ pts, const int32_t* __restrict__ perm, int64_t n,
+                            const ViewGeom* __restrict__ vgp, int nx, int ny, uint32_t* __restrict__ key,
+                            int32_t* __restrict__ val) {
+  const int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  const double cb = vgp->cb, sb = vgp->sb, cl = vgp->cl, sl = vgp->sl;
+  const float4 q = pts[p];
+  const double g0 = q.x, g1 = q.y, g2 = q.z;
+  const double xr = g0 * cb * cl - g1 * cb * sl - g2 * sb;  // :50
+  const double yr = g0 * sl + g1 * cl;                      // :51
+  const int j = proj_pixel(xr, vgp->ext[0], vgp->ext[1], nx);
+  const int i = proj_pixel(yr, vgp->ext[2], vgp->ext[3], ny);
+  const int32_t o = perm[p];
+  key[o] = (uint32_t)i * (uint32_t)nx + (uint32_t)j;
+  val[o] = (int32_t)p;
+}
+
+// start[q] = first position of pixel q in the sorted keys (q = 0..R; start[R] = n).
+__global__ void k_proj_starts(const uint32_t* __restrict__ key, int64_t n, int64_t R, int32_t* __restrict__ start) {
+  const int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (q > R) return;
+  int64_t lo = 0, hi = n;
+  while (lo < hi) {
+    const int64_t mid = (lo + hi) >> 1;
+    if ((int64_t)key[mid] < q) lo = mid + 1;
+    else hi = mid;
+  }
+  start[q] = (int32_t)lo;
+}
+
+// One thread per (pixel, lambda pixel); nl == 1 sums the peaks (Imaging, :148-155).
+template <int RS4>
+__global__ void __launch_bounds__(256) k_proj_spectral(const int32_t* __restrict__ start,
+                                                       const int32_t* __restrict__ order,
+                                                       const float4* __restrict__ rec, RecInfo ri,
+                                                       const ViewGeom* __restrict__ vgp, int64_t R, int nl,
+                                                       double wla, double lambda0, float* __restrict__ out) {
+  const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const int nlo = nl > 1 ? nl : 1;
+  if (t >= R * nlo) return;
+  const int64_t pix = t / nlo;
+  const int il = (int)(t - pix * nlo);
+  const int q0 = start[pix], q1 = start[pix + 1];
+  float acc = 0.f;
+  if (nl > 1) {
+    const double cb = vgp->cb, sb = vgp->sb, cl = vgp->cl, sl = vgp->sl;
+    const double u0 = sb * cl, u1 = -sb * sl, u2 = cb;                               // :34
+    const double lambdaval = static_cast<double>(il) / (nl - 1) * wla - wla / 2.;  // :138
+    const double ln2 = log(2.);
+    const double K = ri.K[0];
+    for (int q = q0; q < q1; ++q) {
+      const float4* rp = rec + (size_t)order[q] * RS4;
+      const float4 a = rp[0];  // vx, vy, vz, T
+      const float pk = rp[1].x;
+      const double losvel = ((0.0 + u0 * (double)a.x) + u1 * (double)a.y) + u2 * (double)a.z;  // :53-54
+      const double fw = K > 0 ? (double)(float)(K * sqrt((double)a.w)) : 1.0;
+      const double d = lambdaval - losvel / FOMO_C_LIGHT * lambda0;
+      const double tmp = (double)pk * exp((-(d * d)) / (fw * fw) * 4. * ln2);  // :139
+      acc = (float)((double)acc + tmp);                                           // :144
+    }
+  } else {
+    for (int q = q0; q < q1; ++q) acc = acc + rec[(size_t)order[q] * RS4 + 1].x;  // :154
+  }
+  out[t] = (float)(vgp->scale * (double)acc);  // :200
+}
+
 // Per-point records for k_nn (sorted order): {vx, vy, vz, T, peak_0, peak_1, ...}, rs4 float4 each.
 __global__ void k_build_rec(const float4* __restrict__ vel, const float* __restrict__ T, const LineSet ls, int64_t n,
                             int rs4, float4* __restrict__ rec) {
@@ -2140,8 +2223,8 @@ int check_res(int nx, int ny, int nz, int nl) {
 struct RenderBufs {
   double* partial;
   ViewGeom* geom;
-  int32_t* idx;      // S
-  float* aux;        // NN: per-sample LOS velocity (S); Thomson: per-point sin^2 chi (N)
+  int32_t* idx;      // S; projection: the ProjLayout arrays
+  float* aux;        // NN: per-sample LOS velocity (S); Thomson: per-point sin^2 chi (N); projection: sort scratch
   float* out;        // R * max(nl,1)
   int32_t* flag;     // NN: R rays needing the direct-exp pass
   float2* sem;       // NN: per-sample (peak, fwhm), S
@@ -2149,9 +2232,43 @@ struct RenderBufs {
 };
 constexpr int kRenderBufs = 8;
 
+// Projection buffers inside RenderBufs::idx (int32 offsets, 256-byte aligned): pixel keys and
+// sorted positions in original point order, the same sorted by pixel, and the pixel starts.
+struct ProjLayout {
+  int64_t key_in, val_in, key_out, val_out, start, total;
+  int bits;  // key bits to sort
+};
+ProjLayout proj_layout(int64_t n, int64_t R) {
+  const auto up = [](int64_t a) { return (a + 63) & ~(int64_t)63; };
+  ProjLayout L;
+  L.key_in = 0;
+  L.val_in = up(n);
+  L.key_out = 2 * up(n);
+  L.val_out = 3 * up(n);
+  L.start = 4 * up(n);
+  L.total = L.start + up(R + 1);
+  L.bits = 1;
+  while (L.bits < 32 && ((uint64_t)(R - 1) >> L.bits) != 0) ++L.bits;
+  return L;
+}
+size_t proj_sort_bytes(int64_t n, int bits) {
+  size_t b = 0;
+  (void)rocprim::radix_sort_pairs(nullptr, b, (uint32_t*)nullptr, (uint32_t*)nullptr, (int32_t*)nullptr,
+                                  (int32_t*)nullptr, (size_t)n, 0, bits, (hipStream_t)0);
+  return b;
+}
+
 void render_sizes(int mode, int64_t n, int64_t R, int64_t S, int nl, int64_t sz[kRenderBufs]) {
   sz[0] = (int64_t)sizeof(double) * 6 * (kMaxPartials + 1);
   sz[1] = (int64_t)sizeof(ViewGeom);
+  if (mode == MODE_PROJ) {
+    const ProjLayout L = proj_layout(n, R);
+    sz[2] = (int64_t)sizeof(int32_t) * L.total;
+    sz[3] = (int64_t)proj_sort_bytes(n, L.bits);
+    sz[4] = (int64_t)sizeof(float) * R * std::max(nl, 1);
+    sz[5] = sz[6] = sz[7] = 0;
+    return;
+  }
   sz[2] = (int64_t)sizeof(int32_t) * S;
   sz[3] = (int64_t)sizeof(float) * (mode != MODE_THOMSON ? S : n);
   sz[4] = (int64_t)sizeof(float) * R * std::max(nl, 1);
@@ -2176,6 +2293,42 @@ int resolve_render(int goft, int mode, int nx, int ny, int nz, int nl, Goft** gc
   return FOMO_OK;
 }
 
+// Projection of one view after its geometry (rb.geom): pixel keys, stable sort by pixel, pixel
+// starts, per-(pixel, lambda) sums into rb.out.
+int enqueue_projection(hipStream_t s, const Dataset& ds, const Goft& gc, const RenderBufs& rb, int nx, int ny,
+                       int nl, double width) {
+  const int64_t n = ds.n, R = (int64_t)nx * ny;
+  const ProjLayout L = proj_layout(n, R);
+  uint32_t* key_in = reinterpret_cast<uint32_t*>(rb.idx + L.key_in);
+  uint32_t* key_out = reinterpret_cast<uint32_t*>(rb.idx + L.key_out);
+  int32_t* val_in = rb.idx + L.val_in;
+  int32_t* val_out = rb.idx + L.val_out;
+  int32_t* start = rb.idx + L.start;
+  {
+    ProfScope ps("k_proj_keys", s);
+    k_proj_keys<<<nblk(n), kBlock, 0, s>>>(ds.pts.as<float4>(), ds.perm.as<int32_t>(), n, rb.geom, nx, ny, key_in,
+                                           val_in);
+    HIPOK(hipGetLastError());
+  }
+  {
+    ProfScope ps("proj_sort", s);
+    size_t tmpb = proj_sort_bytes(n, L.bits);
+    HIPOK(rocprim::radix_sort_pairs(rb.aux, tmpb, key_in, key_out, val_in, val_out, (size_t)n, 0, L.bits, s));
+  }
+  k_proj_starts<<<nblk(R + 1), kBlock, 0, s>>>(key_out, n, R, start);
+  HIPOK(hipGetLastError());
+  const int64_t nt = R * std::max(nl, 1);
+  const double wla = width * gc.lambda0 / FOMO_C_LIGHT;  // fomo-projection.cpp:69
+  ProfScope ps("k_proj_spectral", s);
+#define FOMO_PROJ_ARGS start, val_out, gc.rec.as<float4>(), gc.ri, rb.geom, R, nl, wla, gc.lambda0, rb.out
+  if (gc.rs4 == 2) k_proj_spectral<2><<<nblk(nt), 256, 0, s>>>(FOMO_PROJ_ARGS);
+  else if (gc.rs4 == 4) k_proj_spectral<4><<<nblk(nt), 256, 0, s>>>(FOMO_PROJ_ARGS);
+  else k_proj_spectral<8><<<nblk(nt), 256, 0, s>>>(FOMO_PROJ_ARGS);
+#undef FOMO_PROJ_ARGS
+  HIPOK(hipGetLastError());
+  return FOMO_OK;
+}
+
 // All kernels of one view on stream s; the image lands in rb.out, the geometry in rb.geom.
 int enqueue_render(hipStream_t s, const Dataset& ds, const Goft& gc, int mode, double l, double b, int nx, int ny,
                    int nz, int nl, double width, int dn, const RenderBufs& rb) {
@@ -2188,6 +2341,8 @@ int enqueue_render(hipStream_t s, const Dataset& ds, const Goft& gc, int mode, d
       return rc;
     LineSet ls = make_lineset({rb.sem}, {rb.out}, {gc.lambda0}, width);
     enqueue_los(s, (int)R, nz, ls, rb.aux, rb.geom, nl, rb.flag);
+  } else if (mode == MODE_PROJ) {
+    if (int rc = enqueue_projection(s, ds, gc, rb, nx, ny, nl, width)) return rc;
   } else if (mode == MODE_NN) {
     enqueue_nn(s, ds, rb.geom, nx, ny, nz, rb.idx, rb.rayflag, gc.rec.as<float4>(), gc.rs4, 1, rb.aux, rb.sem, gc.ri);
     LineSet ls = make_lineset({rb.sem}, {rb.out}, {gc.lambda0}, width);
@@ -2562,6 +2717,7 @@ int method_mode(int method, int* mode) {
     case FOMO_METHOD_THOMSON: *mode = MODE_THOMSON; return FOMO_OK;
     case FOMO_METHOD_TET: *mode = MODE_TET; return FOMO_OK;
     case FOMO_METHOD_LINEAR: *mode = MODE_LINEAR; return FOMO_OK;
+    case FOMO_METHOD_PROJECTION: *mode = MODE_PROJ; return FOMO_OK;
     default: return fail(FOMO_ERR_ARG, "unknown method");
   }
 }
@@ -2905,6 +3061,40 @@ int fomo_render_tet(int goft, double l, double b, int nx, int ny, int nz, int nl
                     float* out, double* meta) {
   std::lock_guard<std::recursive_mutex> lk(g_mu);
   return render_one(goft, linear ? MODE_LINEAR : MODE_TET, l, b, nx, ny, nz, nl, width, dn, out, meta);
+}
+
+int fomo_render_projection(int goft, double l, double b, int nx, int ny, int nz, int nl, double width, int dn,
+                           float* out, double* meta) {
+  std::lock_guard<std::recursive_mutex> lk(g_mu);
+  return render_one(goft, MODE_PROJ, l, b, nx, ny, nz, nl, width, dn, out, meta);
+}
+
+int fomo_projection_pixels(int dataset, double l, double b, int nx, int ny, int nz, int32_t* out_pixel,
+                           double* meta) {
+  std::lock_guard<std::recursive_mutex> lk(g_mu);
+  if (int rc = require_init()) return rc;
+  if (int rc = check_res(nx, ny, nz, 1)) return rc;
+  if (!out_pixel) return fail(FOMO_ERR_ARG, "null output");
+  Dataset* ds = find_obj(g_ctx.datasets, dataset);
+  if (!ds) return fail(FOMO_ERR_NOTFOUND, "unknown dataset id");
+  hipStream_t s = g_ctx.stream;
+  const int64_t n = ds->n;
+  ViewScratch& vs = g_ctx.vs;
+  HIPOK(vs.partial.ensure(sizeof(double) * 6 * (kMaxPartials + 1)));
+  HIPOK(vs.geom.ensure(sizeof(ViewGeom)));
+  HIPOK(vs.idx.ensure(sizeof(int32_t) * 2 * n));
+  ViewGeom* geom = vs.geom.as<ViewGeom>();
+  enqueue_view_geom(s, *ds, MODE_NN, l, b, nx, ny, nz, 0, nullptr, vs.partial.as<double>(), geom);
+  uint32_t* key = reinterpret_cast<uint32_t*>(vs.idx.as<int32_t>());
+  k_proj_keys<<<nblk(n), kBlock, 0, s>>>(ds->pts.as<float4>(), ds->perm.as<int32_t>(), n, geom, nx, ny, key,
+                                         vs.idx.as<int32_t>() + n);
+  HIPOK(hipGetLastError());
+  HIPOK(hipMemcpyAsync(out_pixel, key, sizeof(int32_t) * n, hipMemcpyDeviceToHost, s));
+  ViewGeom hg;
+  HIPOK(hipMemcpyAsync(&hg, geom, sizeof(ViewGeom), hipMemcpyDeviceToHost, s));
+  if (int rc = sync_stream()) return rc;
+  if (meta) geom_to_meta(hg, meta);
+  return FOMO_OK;
 }
 
 // Autotuner: times the NN passes of one representative view (k_nn + k_nn_far, gathering the same

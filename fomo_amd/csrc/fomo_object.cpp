@@ -482,9 +482,9 @@ void FoMo::FoMoObject::render(const std::vector<double> lvec, const std::vector<
   const bool thomson = method == "Thomson";
   // "CGAL": the tetra path (fomo-CGAL.cpp:309-338); "Linear": its barycentric variant (extension)
   const bool tet = method == "CGAL" || method == "Linear";
-  if (method != "NearestNeighbour" && !thomson && !tet) {
-    if (method == "Projection" || method == "CGAL2D")
-      die("Error: render method " + method + " is not available in this build.");
+  const bool proj = method == "Projection";  // fomo-projection.cpp:218-239
+  if (method != "NearestNeighbour" && !thomson && !tet && !proj) {
+    if (method == "CGAL2D") die("Error: render method " + method + " is not available in this build.");
     die("Error: unknown rendering method.");
   }
   const std::bitset<noptions> woptions = goftcube.getwriteoptions();
@@ -559,6 +559,7 @@ void FoMo::FoMoObject::render(const std::vector<double> lvec, const std::vector<
       int rc = thomson ? fomo_render_thomson(goft, lv, bv, nx, ny, nz, img.data(), m)
                : tet   ? fomo_render_tet(goft, lv, bv, nx, ny, nz, nl, width, dn ? 1 : 0, method == "Linear" ? 1 : 0,
                                          img.data(), m)
+               : proj  ? fomo_render_projection(goft, lv, bv, nx, ny, nz, nl, width, dn ? 1 : 0, img.data(), m)
                        : fomo_render_nn(goft, lv, bv, nx, ny, nz, nl, width, dn ? 1 : 0, img.data(), m);
       if (rc != FOMO_OK) die(std::string("Error: render failed: ") + fomo_last_error());
       const double minx = m[0], maxx = m[1], miny = m[2], maxy = m[3];
@@ -567,7 +568,8 @@ void FoMo::FoMoObject::render(const std::vector<double> lvec, const std::vector<
       FoMo::tvars newdata;
       if (!thomson) {
         // fomo-nearestneighbour.cpp:124-128, 152-153, 200-211, 233-279 (the CGAL path builds the same
-        // grids, fomo-CGAL.cpp:187-188, 210-218, 262-285)
+        // grids, fomo-CGAL.cpp:187-188, 210-218, 262-285, and so does the projection,
+        // fomo-projection.cpp:76-118, 161-212, whose view cubes also say "NearestNeighbour", :203)
         const size_t ng3 = (size_t)nx * ny * nlo;
         FoMo::tcoord xv(ng3), yv(ng3), lv3(nl > 1 ? ng3 : 0);
         const double wla = width * lambda0 / kC;

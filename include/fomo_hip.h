@@ -110,6 +110,20 @@ int fomo_render_thomson(int goft, double l, double b, int nx, int ny, int nz, fl
 int fomo_render_tet(int goft, double l, double b, int nx, int ny, int nz, int nl, double lambda_width,
                     int dn, int linear, float* out, double* meta);
 
+/* One view of projectioninterpolation (fomo-projection.cpp:17-214), method "Projection": every
+ * point adds its line profile (nl > 1) or its peak (nl == 1) to the pixel its rotated (x', y')
+ * rounds to (:126-158); no LOS samples. nz only sets the path length (:163-170). Each pixel sums
+ * its points in original point order in float, the sums of a one-thread run of the reference
+ * (which adds with OpenMP atomics, in no fixed order). Arguments and output as fomo_render_nn;
+ * meta[6] (the NN search half-width) has no meaning here. */
+int fomo_render_projection(int goft, double l, double b, int nx, int ny, int nz, int nl,
+                           double lambda_width, int dn, float* out, double* meta);
+
+/* The projection's pixel iy*nx + jx of every point (original order) for one view. For parity
+ * checks of the pixel assignment (:129-131). */
+int fomo_projection_pixels(int dataset, double l, double b, int nx, int ny, int nz, int32_t* out_pixel,
+                           double* meta);
+
 /* Autotuner (no counterpart in the reference): for the dataset of the ngoft emission cubes (one
  * dataset, one kind; the lines of a batch), times the nearest-neighbour passes of one
  * representative view (l, b, nx x ny x nz; Thomson geometry for a Thomson cube), gathering the
@@ -175,6 +189,7 @@ int fomo_batch_destroy(int batch);
 #define FOMO_METHOD_THOMSON 1 /* "Thomson": fomo_render_thomson (nl, lambda_width, dn ignored) */
 #define FOMO_METHOD_TET 2     /* "CGAL": fomo_render_tet (dataset must be triangulated first) */
 #define FOMO_METHOD_LINEAR 3  /* "Linear": fomo_render_tet with linear != 0 (likewise) */
+#define FOMO_METHOD_PROJECTION 4 /* "Projection": fomo_render_projection */
 int fomo_server_create(int64_t device_bytes, int64_t host_bytes, int max_inflight, int nstreams,
                        int* out_server);
 int fomo_server_destroy(int server);

@@ -22,6 +22,8 @@
 //   rotation/samples : identity view (image = grid), exact replay.
 //   NN query         : brute-force argmin on small inputs; scipy cKDTree fixtures.
 //   integration      : uniform slab, line integral and Doppler shift known answers.
+//   projection       : pixels of grid points in an axis view (exact), half-away-from-zero
+//                      rounding cases, point-order float sums, one-point Doppler profile.
 // Unpinned (no reference import possible): Boost R-tree tie order
 // (lowest index is used here), GSL constant versions (CODATA 2006 values used).
 
@@ -520,6 +522,77 @@ int orc_render_nn(int mode, int64_t n, const float* x, const float* y, const flo
     std::memcpy(meta, m, sizeof(m));
   }
   if (timing) { timing[0] = t1 - t0; timing[1] = t2 - t1; timing[2] = t3 - t2; }
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
+// Projection method: projectioninterpolation, fomo-projection.cpp:17-214. Every point adds its
+// line profile (nl > 1, :133-146) or its peak (nl == 1, :148-155) to the pixel its rotated
+// (x', y') rounds to (:126-131, std::round). The reference adds with OpenMP atomics in no fixed
+// order; this restatement adds in point order (a one-thread run of the reference): mode 0 in
+// float as intens.at(ind) += tempintens, mode 1 in double. A degenerate extent (maxx == minx or
+// maxy == miny: 0/0 in the reference's pixel formula) puts every point in column (row) 0.
+// out: ny*nx*max(nl,1) scaled intensities; out_pix (optional): pixel i*nx+j per point;
+// meta as in orc_render_nn with meta[6] = 0.
+// ---------------------------------------------------------------------------
+static int proj_index(double v, double mn, double mx, int n) {
+  if (!(mx > mn)) return 0;
+  return (int)std::round((v - mn) * (n - 1) / (mx - mn));  // :129, :131
+}
+
+int orc_render_projection(int mode, int64_t n, const float* x, const float* y, const float* z,
+                          const double* peak, const double* fwhm, const float* vx, const float* vy,
+                          const float* vz, double l, double b, int nx, int ny, int nz, int nl,
+                          double lambda_width, double lambda0, int dn, double* out, int32_t* out_pix,
+                          double* meta) {
+  if (nx < 2 || ny < 2 || nz < 1 || nl < 1 || n < 1) return -1;
+  double ext[6];
+  orc_extents(n, x, y, z, l, b, ext);  // :45-64 (the same rotation as the NN path)
+  const double minx = ext[0], maxx = ext[1], miny = ext[2], maxy = ext[3], minz = ext[4], maxz = ext[5];
+  const double lambda_width_in_A = lambda_width * lambda0 / kSpeedOfLight;  // :69
+  const int nlo = nl > 1 ? nl : 1;
+  const size_t ntot = (size_t)nx * ny * nlo;
+  std::vector<float> accf(mode == 0 ? ntot : 0, 0.f);
+  std::vector<double> accd(mode == 0 ? 0 : ntot, 0.0);
+  const double unit[3] = {sin(b) * cos(l), -sin(b) * sin(l), cos(b)};  // :34
+  for (int64_t k = 0; k < n; ++k) {
+    double xr, yr, zr;
+    rotate_point(x[k], y[k], z[k], l, b, &xr, &yr, &zr);  // :50-52
+    const int j = proj_index(xr, minx, maxx, nx);
+    const int i = proj_index(yr, miny, maxy, ny);
+    if (out_pix) out_pix[k] = i * nx + j;
+    if (nl > 1) {
+      const double v[3] = {vx[k], vy[k], vz[k]};
+      const double losvel = ((0.0 + unit[0] * v[0]) + unit[1] * v[1]) + unit[2] * v[2];  // :53-55
+      const double pk = mode == 0 ? (double)(float)peak[k] : peak[k];
+      const double fw = mode == 0 ? (double)(float)fwhm[k] : fwhm[k];
+      for (int il = 0; il < nl; ++il) {
+        const double lambdaval = static_cast<double>(il) / (nl - 1) * lambda_width_in_A - lambda_width_in_A / 2.;
+        const double t = pk * exp(-pow(lambdaval - losvel / kSpeedOfLight * lambda0, 2) / pow(fw, 2) * 4. * log(2.));
+        const size_t ind = ((size_t)i * nx + j) * nl + il;  // :140
+        if (mode == 0) accf[ind] = (float)((double)accf[ind] + t);
+        else accd[ind] += t;
+      }
+    } else {
+      const size_t ind = (size_t)i * nx + j;  // :150
+      if (mode == 0) accf[ind] = accf[ind] + (float)peak[k];
+      else accd[ind] += peak[k];
+    }
+  }
+  // :163-176 path length, :179-200 DN branch and scaling
+  double pathlength = (maxz - minz) / (nz - 1);
+  if ((maxz == minz) || (nz == 1)) pathlength = 1.;
+  double apix = 1.;
+  if (dn) {
+    float dx = (maxx - minx) / (nx - 1), dy = (maxy - miny) / (ny - 1);
+    apix = (dx / kMmPerArcsec) * (dy / kMmPerArcsec) * pow(kPi / 180. / 3600., 2);
+  }
+  const double sc = pathlength * 1e8 * apix;
+  for (size_t q = 0; q < ntot; ++q) out[q] = mode == 0 ? (double)(float)(sc * (double)accf[q]) : sc * accd[q];
+  if (meta) {
+    double m[10] = {minx, maxx, miny, maxy, minz, maxz, 0.0, pathlength, sc, apix};
+    std::memcpy(meta, m, sizeof(m));
+  }
   return 0;
 }
 

@@ -54,6 +54,11 @@ def lib():
         L.orc_render_thomson.restype = ctypes.c_int
         L.orc_render_thomson.argtypes = [ctypes.c_int, ctypes.c_int64, _f, _f, _f, _d, _d, ctypes.c_double,
                                          ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.c_int, _d, _i, _d, _d]
+        L.orc_render_projection.restype = ctypes.c_int
+        L.orc_render_projection.argtypes = [ctypes.c_int, ctypes.c_int64, _f, _f, _f, _d, _d, _f, _f, _f,
+                                            ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_int,
+                                            ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double,
+                                            ctypes.c_int, _d, _i, _d]
         L.orc_num_threads.restype = ctypes.c_int
         _LIB = L
     return _LIB
@@ -168,6 +173,25 @@ def render_thomson(mode, x, y, z, totn, poln, l, b, nx, ny, nz, want_idx=False):
     if rc != 0:
         raise ValueError(f"oracle render_thomson rejected the arguments ({rc})")
     return out, idx, meta, tim
+
+
+def render_projection(mode, x, y, z, peak, fwhm, vx, vy, vz, l, b, nx, ny, nz, nl, lambda_width, lambda0,
+                      dn=False, want_pix=False):
+    """The reference's projection method (fomo-projection.cpp:17-214), sums in point order.
+    Returns (image float64 [ny*nx*max(nl,1)], pixel per point int32 or None, meta[10])."""
+    x, y, z, vx, vy, vz = map(_pf, (x, y, z, vx, vy, vz))
+    peak, fwhm = _pd(peak), _pd(fwhm)
+    out = np.zeros(nx * ny * max(nl, 1))
+    pix = np.zeros(x.size, np.int32) if want_pix else None
+    meta = np.zeros(10)
+    rc = lib().orc_render_projection(mode, x.size, P(x, ctypes.c_float), P(y, ctypes.c_float),
+                                     P(z, ctypes.c_float), P(peak, ctypes.c_double), P(fwhm, ctypes.c_double),
+                                     P(vx, ctypes.c_float), P(vy, ctypes.c_float), P(vz, ctypes.c_float), l, b, nx,
+                                     ny, nz, nl, lambda_width, lambda0, 1 if dn else 0, P(out, ctypes.c_double),
+                                     P(pix, ctypes.c_int32) if want_pix else None, P(meta, ctypes.c_double))
+    if rc != 0:
+        raise ValueError(f"oracle render_projection rejected the arguments ({rc})")
+    return out, pix, meta
 
 
 def num_threads():

@@ -57,3 +57,25 @@ def test_facade_unknown_method_exits(api_bin, fe9_table, tmp_path):
     r = subprocess.run([api_bin, path, str(tmp_path / "x."), "NoSuchMethod", "8", "8", "8", "8", "4", "0", "0"],
                        capture_output=True, text=True, timeout=120)
     assert r.returncode == 1 and "unknown rendering method" in r.stderr
+
+
+def test_facade_projection_render_file(api_bin, fe9_table, tmp_path):
+    path, tab = fe9_table
+    m, nx, ny, nz, nl, l, b = 20, 28, 22, 30, 24, 0.7, 1.2
+    prefix = str(tmp_path / "proj.")
+    r = subprocess.run([api_bin, path, prefix, "Projection", str(m), str(nx), str(ny), str(nz), str(nl), str(l),
+                        str(b)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    f = prefix + "l040b069.dat"
+    assert os.path.exists(f), os.listdir(tmp_path)
+    got = read_dat(f)
+    assert got["dim"] == 3 and got["nvars"] == 1 and got["ng"] == nx * ny * nl
+    d = F.gen_loop_cube(m, seed=1)
+    width = 0.2 * C / 171.0
+    pk, fw = O.emission(0, d["rho"], d["T"], tab)
+    ref, _, meta = O.render_projection(0, d["x"], d["y"], d["z"], pk, fw, d["vx"], d["vy"], d["vz"], l, b, nx, ny,
+                                       nz, nl, width, 171.0)
+    assert np.count_nonzero(ref) > 0
+    assert O.rel_l2(got["cols"][3], ref) <= 1e-6
+    lam = (np.arange(nl) / (nl - 1) * (width * 171.0 / C) - width * 171.0 / C / 2 + 171.0).astype(np.float32)
+    assert np.array_equal(got["cols"][2].reshape(ny, nx, nl)[3, 4, :], lam)
