@@ -526,6 +526,12 @@ __global__ void k_view_finalize(const double* __restrict__ partial, int np, int 
 // queries use the exact ring search.
 // ------------------------------------------------------------------------------------
 constexpr int kLookupCap = 24;
+// Lookup cell record: kSlots float4 (64 bytes) holding up to kSlots candidates (x, y, z, sorted
+// position as int bits; unused slots have w = -1). Slot 0 with w = -1: no candidates (the query
+// falls back to the exact search); w = -2: more than kSlots candidates, at lcand[x] .. (y of them,
+// x and y as int bits). One 64-byte record per query instead of a (offset, count) pair plus a
+// separate candidate list.
+constexpr int kSlots = 4;
 constexpr int kLookupMargin = 6;  // cell-list cells the lookup grid extends beyond the data bbox
 constexpr int kLookupRings = kLookupMargin + 2;
 
@@ -554,7 +560,8 @@ __device__ __forceinline__ bool dominated(const double c[3], const double hw[3],
 template <bool FILL>
 __global__ void k_lookup_build(const float4* __restrict__ pts, const int32_t* __restrict__ cstart, GridInfo g,
                                LookupInfo lk, int64_t ncl, const uint8_t* __restrict__ near,
-                               int32_t* __restrict__ cnt, const int64_t* __restrict__ off, float4* __restrict__ cand) {
+                               int32_t* __restrict__ cnt, const int64_t* __restrict__ off, float4* __restrict__ cand,
+                               float4* __restrict__ slot) {
   const int64_t L = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (L >= ncl) return;
   if (FILL && cnt[L] <= 0) return;
@@ -637,7 +644,8 @@ __global__ void k_lookup_build(const float4* __restrict__ pts, const int32_t* __
     return;
   }
   int n = 0;
-  const int64_t base = FILL ? off[L] : 0;
+  // small lists go to the cell's own record, longer ones to the overflow list
+  float4* const dst = FILL ? (cnt[L] <= kSlots ? slot + (int64_t)kSlots * L : cand + off[L]) : nullptr;
   for (int z = r0[2]; z <= r1[2]; ++z)
     for (int y = r0[1]; y <= r1[1]; ++y) {
       const int row = z * dxy + y * dx;
@@ -648,7 +656,7 @@ __global__ void k_lookup_build(const float4* __restrict__ pts, const int32_t* __
         for (int k = 0; k < nd && !dm; ++k)
           if (dpos[k] != p && dominated(c, hw, q, dom[k])) dm = true;
         if (dm) continue;
-        if (FILL) cand[base + n] = make_float4(q.x, q.y, q.z, __int_as_float(p));
+        if (FILL) dst[n] = make_float4(q.x, q.y, q.z, __int_as_float(p));
         ++n;
         if (!FILL && n > kLookupCap) {
           cnt[L] = -1;
@@ -677,11 +685,29 @@ __global__ void k_dilate_axis(const uint8_t* __restrict__ in, uint8_t* __restric
   out[c] = v;
 }
 
-__global__ void k_lookup_pack(const int32_t* __restrict__ cnt, const int64_t* __restrict__ off, int64_t ncl,
-                              int2* __restrict__ lcell) {
+// Headers of the lookup cell records: the no-candidate and overflow markers, and w = -1 in the
+// slots a short list leaves unused (k_lookup_build<true> filled the others).
+__global__ void k_lookup_headers(const int32_t* __restrict__ cnt, const int64_t* __restrict__ off, int64_t ncl,
+                                 float4* __restrict__ slot) {
   const int64_t L = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (L >= ncl) return;
-  lcell[L] = make_int2((int)off[L], cnt[L]);
+  const int c = cnt[L];
+  float4* r = slot + (int64_t)kSlots * L;
+  const float4 unused = make_float4(0.f, 0.f, 0.f, __int_as_float(-1));
+  if (c <= 0) {
+    r[0] = unused;
+  } else if (c > kSlots) {
+    r[0] = make_float4(__int_as_float((int)off[L]), __int_as_float(c), 0.f, __int_as_float(-2));
+  } else {
+    for (int q = c; q < kSlots; ++q) r[q] = unused;
+  }
+}
+
+// pos = the overflow list length of each cell (cells with more than kSlots candidates)
+__global__ void k_overflow_counts(const int32_t* __restrict__ cnt, int64_t ncl, int32_t* __restrict__ pos) {
+  const int64_t L = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (L >= ncl) return;
+  pos[L] = cnt[L] > kSlots ? cnt[L] : 0;
 }
 
 __global__ void k_count_negative(const int32_t* __restrict__ cnt, int64_t ncl, int* __restrict__ out) {
@@ -706,7 +732,7 @@ __global__ void k_clamp_counts(const int32_t* __restrict__ cnt, int64_t ncl, int
 
 // Fast path: the unconstrained nearest point from the target's lookup cell. Returns its sorted
 // position if it lies strictly inside the search box, else -2 (caller runs the ring search).
-__device__ __forceinline__ int nn_lookup(const float4* __restrict__ pts, const int2* __restrict__ lcell,
+__device__ __forceinline__ int nn_lookup(const float4* __restrict__ pts, const float4* __restrict__ lslot,
                                          const float4* __restrict__ lcand, const LookupInfo& lk, float t0, float t1,
                                          float t2, float lo0, float lo1, float lo2, float hi0, float hi1, float hi2) {
   const double v0 = ((double)t0 - lk.o[0]) * lk.invh;
@@ -716,15 +742,19 @@ __device__ __forceinline__ int nn_lookup(const float4* __restrict__ pts, const i
         v2 < (double)lk.dims[2]))
     return -2;
   const int64_t L = ((int64_t)(int)v2 * lk.dims[1] + (int)v1) * lk.dims[0] + (int)v0;
-  const int2 e = lcell[L];
-  if (e.y <= 0) return -2;
+  // the whole 64-byte record at once (one sector); the overflow list only for long lists
+  const float4* rp = lslot + (int64_t)kSlots * L;
+  float4 sl[kSlots];
+#pragma unroll
+  for (int q = 0; q < kSlots; ++q) sl[q] = rp[q];
+  const int w0 = __float_as_int(sl[0].w);
+  if (w0 == -1) return -2;
   // candidates hold their coordinates inline (x, y, z, sorted position); the original index
   // (tie-break) is read from pts only on an exact distance tie
   double best = INFINITY;
   int bpos = -1;
   float4 bq = make_float4(0.f, 0.f, 0.f, 0.f);
-  for (int i = 0; i < e.y; ++i) {
-    const float4 q = lcand[e.x + i];
+  const auto consider = [&](const float4 q) {
     const double d0 = (double)q.x - (double)t0;
     const double d1 = (double)q.y - (double)t1;
     const double d2 = (double)q.z - (double)t2;
@@ -737,6 +767,15 @@ __device__ __forceinline__ int nn_lookup(const float4* __restrict__ pts, const i
       bpos = p;
       bq = q;
     }
+  };
+  if (w0 != -2) {
+#pragma unroll
+    for (int q = 0; q < kSlots; ++q)
+      if (__float_as_int(sl[q].w) >= 0) consider(sl[q]);
+  } else {
+    const float4* ov = lcand + __float_as_int(sl[0].x);
+    const int n = __float_as_int(sl[0].y);
+    for (int i = 0; i < n; ++i) consider(ov[i]);
   }
   if (lo0 < bq.x && bq.x < hi0 && lo1 < bq.y && bq.y < hi1 && lo2 < bq.z && bq.z < hi2) return bpos;
   return -2;
@@ -859,7 +898,7 @@ __device__ __forceinline__ void write_sample(int r, const float4* __restrict__ r
 // zero when no point is inside the search box (:190-193). idx (optional) gets the sorted position.
 template <int RS4>
 __device__ __forceinline__ void nn_sample(const float4* __restrict__ pts, const GridInfo& g,
-                                          const int2* __restrict__ lcell, const float4* __restrict__ lcand,
+                                          const float4* __restrict__ lslot, const float4* __restrict__ lcand,
                                           const LookupInfo& lk, const ViewGeom& vg, int nx, int ny, int nz, int i,
                                           int j, int k, int32_t* __restrict__ idx, const float4* __restrict__ rec,
                                           const RecInfo& ri, int nlines, float* __restrict__ svlos,
@@ -875,7 +914,7 @@ __device__ __forceinline__ void nn_sample(const float4* __restrict__ pts, const 
   int r = -2;
   if (lk.enabled) {
     const double md = vg.md;
-    r = nn_lookup(pts, lcell, lcand, lk, (float)p0, (float)p1, (float)p2, (float)(p0 - md), (float)(p1 - md),
+    r = nn_lookup(pts, lslot, lcand, lk, (float)p0, (float)p1, (float)p2, (float)(p0 - md), (float)(p1 - md),
                   (float)(p2 - md), (float)(p0 + md), (float)(p1 + md), (float)(p2 + md));
   }
   if (r == -2) {
@@ -900,7 +939,7 @@ __device__ __forceinline__ void nn_sample(const float4* __restrict__ pts, const 
 // One thread per LOS sample: block = 16x16 rays of one LOS plane (blockIdx.y = k).
 template <int RS4>
 __global__ void __launch_bounds__(256) k_nn(const float4* __restrict__ pts, GridInfo g,
-                                            const int2* __restrict__ lcell, const float4* __restrict__ lcand,
+                                            const float4* __restrict__ lslot, const float4* __restrict__ lcand,
                                             LookupInfo lk, const ViewGeom* __restrict__ vgp, int nx, int ny, int nz,
                                             int32_t* __restrict__ idx, const float4* __restrict__ rec, RecInfo ri,
                                             int nlines, float* __restrict__ svlos, float2* __restrict__ sem,
@@ -909,7 +948,7 @@ __global__ void __launch_bounds__(256) k_nn(const float4* __restrict__ pts, Grid
   const int j = (blockIdx.x % tilesx) * 16 + (threadIdx.x & 15);
   const int i = (blockIdx.x / tilesx) * 16 + (threadIdx.x >> 4);
   if (j >= nx || i >= ny) return;
-  nn_sample<RS4>(pts, g, lcell, lcand, lk, *vgp, nx, ny, nz, i, j, (int)blockIdx.y, idx, rec, ri, nlines, svlos,
+  nn_sample<RS4>(pts, g, lslot, lcand, lk, *vgp, nx, ny, nz, i, j, (int)blockIdx.y, idx, rec, ri, nlines, svlos,
                  sem, rayflag);
 }
 
@@ -1480,8 +1519,8 @@ struct Dataset {
   LookupInfo lk{};     // candidate lookup grid (lk.enabled==0: ring search only)
   int lookup_subdiv = 2;         // requested subdivision
   int lookup_active_subdiv = 0;  // subdivision actually built (0: none)
-  DevBuf lcell;        // int2 (offset, count) per lookup cell; count -1 = use the ring search
-  DevBuf lcand;        // float4 candidates (x, y, z, sorted position as int bits)
+  DevBuf lslot;        // kSlots float4 per lookup cell: inline candidates or an overflow header
+  DevBuf lcand;        // float4 overflow candidates (x, y, z, sorted position as int bits)
   BvhInfo bvh{};       // implicit BVH over Morton-ordered points (exact NN fallback)
   DevBuf mpts;         // float4 (x, y, z, sorted position) in Morton order
   DevBuf nlo, nhi;     // node boxes, all levels
@@ -1805,7 +1844,7 @@ int build_tets(Dataset& ds) {
 // path too) and is shifted by 0.3 lookup cell off the cell lattice.
 int build_lookup(Dataset& ds) {
   ds.lk = LookupInfo{};
-  ds.lcell.release();
+  ds.lslot.release();
   ds.lcand.release();
   ds.ncand = ds.nlcells = ds.nfallback = 0;
   if (ds.lookup_subdiv <= 0) return FOMO_OK;
@@ -1854,33 +1893,41 @@ int build_lookup(Dataset& ds) {
   {
     ProfScope ps("k_lookup_build", s);
     k_lookup_build<false><<<nblk(ncl), kBlock, 0, s>>>(ds.pts.as<float4>(), ds.cstart.as<int32_t>(), g, lk, ncl,
-                                                        near1.as<uint8_t>(), cnt.as<int32_t>(), nullptr, nullptr);
+                                                        near1.as<uint8_t>(), cnt.as<int32_t>(), nullptr, nullptr,
+                                                        nullptr);
   }
-  k_clamp_counts<<<nblk(ncl), kBlock, 0, s>>>(cnt.as<int32_t>(), ncl, pos.as<int32_t>());
-  HIPOK(hipGetLastError());
+  // candidates in all (for the statistics), then the overflow lists' offsets
   size_t tmpb = 0;
   HIPOK(rocprim::exclusive_scan(nullptr, tmpb, pos.as<int32_t>(), off.as<int64_t>(), (int64_t)0, (size_t)ncl,
                                 rocprim::plus<int64_t>(), s));
   HIPOK(g_ctx.sorttmp.ensure(tmpb));
-  HIPOK(rocprim::exclusive_scan(g_ctx.sorttmp.p, tmpb, pos.as<int32_t>(), off.as<int64_t>(), (int64_t)0,
-                                (size_t)ncl, rocprim::plus<int64_t>(), s));
-  int64_t last_off = 0;
-  int32_t last_cnt = 0, nfb = 0;
-  HIPOK(hipMemcpyAsync(&last_off, off.as<int64_t>() + ncl - 1, sizeof(int64_t), hipMemcpyDeviceToHost, s));
-  HIPOK(hipMemcpyAsync(&last_cnt, pos.as<int32_t>() + ncl - 1, sizeof(int32_t), hipMemcpyDeviceToHost, s));
-  if (int rc = sync_stream()) return rc;
-  const int64_t total = last_off + last_cnt;
+  int64_t total = 0, ntotal = 0;
+  for (int pass = 0; pass < 2; ++pass) {
+    if (pass == 0) k_clamp_counts<<<nblk(ncl), kBlock, 0, s>>>(cnt.as<int32_t>(), ncl, pos.as<int32_t>());
+    else k_overflow_counts<<<nblk(ncl), kBlock, 0, s>>>(cnt.as<int32_t>(), ncl, pos.as<int32_t>());
+    HIPOK(hipGetLastError());
+    HIPOK(rocprim::exclusive_scan(g_ctx.sorttmp.p, tmpb, pos.as<int32_t>(), off.as<int64_t>(), (int64_t)0,
+                                  (size_t)ncl, rocprim::plus<int64_t>(), s));
+    int64_t last_off = 0;
+    int32_t last_cnt = 0;
+    HIPOK(hipMemcpyAsync(&last_off, off.as<int64_t>() + ncl - 1, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+    HIPOK(hipMemcpyAsync(&last_cnt, pos.as<int32_t>() + ncl - 1, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    if (int rc = sync_stream()) return rc;
+    (pass == 0 ? ntotal : total) = last_off + last_cnt;
+  }
   if (total >= (int64_t)INT32_MAX) return FOMO_OK;
-  if (ds.lcand.alloc(sizeof(float4) * std::max<int64_t>(total, 1)) || ds.lcell.alloc(sizeof(int2) * ncl)) {
-    ds.lcell.release();
+  if (ds.lcand.alloc(sizeof(float4) * std::max<int64_t>(total, 1)) ||
+      ds.lslot.alloc(sizeof(float4) * kSlots * ncl)) {
+    ds.lslot.release();
     ds.lcand.release();
     return FOMO_OK;
   }
   k_lookup_build<true><<<nblk(ncl), kBlock, 0, s>>>(ds.pts.as<float4>(), ds.cstart.as<int32_t>(), g, lk, ncl,
                                                      near1.as<uint8_t>(), cnt.as<int32_t>(), off.as<int64_t>(),
-                                                     ds.lcand.as<float4>());
-  k_lookup_pack<<<nblk(ncl), kBlock, 0, s>>>(cnt.as<int32_t>(), off.as<int64_t>(), ncl, ds.lcell.as<int2>());
+                                                     ds.lcand.as<float4>(), ds.lslot.as<float4>());
+  k_lookup_headers<<<nblk(ncl), kBlock, 0, s>>>(cnt.as<int32_t>(), off.as<int64_t>(), ncl, ds.lslot.as<float4>());
   HIPOK(hipGetLastError());
+  int32_t nfb = 0;
   // number of cells left to the ring search
   int* dfb = reinterpret_cast<int*>(pos.p);
   HIPOK(hipMemsetAsync(dfb, 0, sizeof(int), s));
@@ -1890,7 +1937,7 @@ int build_lookup(Dataset& ds) {
   ds.nfallback = nfb;
   ds.lookup_active_subdiv = subdiv;
   ds.lk = lk;
-  ds.ncand = total;
+  ds.ncand = ntotal;
   ds.nlcells = ncl;
   return FOMO_OK;
 }
@@ -2082,7 +2129,7 @@ void enqueue_nn(hipStream_t s, const Dataset& ds, const ViewGeom* geom, int nx, 
   {
     ProfScope ps("k_nn", s);
 #define FOMO_NN_ARGS                                                                                         \
-  ds.pts.as<float4>(), ds.grid, ds.lcell.as<int2>(), ds.lcand.as<float4>(), ds.lk, geom, nx, ny, nz, idx, rec, \
+  ds.pts.as<float4>(), ds.grid, ds.lslot.as<float4>(), ds.lcand.as<float4>(), ds.lk, geom, nx, ny, nz, idx, rec, \
       ri, nlines, svlos, sem, rayflag
     if (!rec) k_nn<0><<<grid, 256, 0, s>>>(FOMO_NN_ARGS);
     else if (rs4 == 2) k_nn<2><<<grid, 256, 0, s>>>(FOMO_NN_ARGS);
