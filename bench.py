@@ -137,11 +137,15 @@ def main():
     F.profile_reset()
     bt.profile()
     bt.profile()
-    stats = {k: F.profile_stats(k) for k in ("k_view_extents_multi", "k_nn", "k_nn_far", "k_los_spectral")}
+    stats = {k: F.profile_stats(k) for k in ("k_view_extents_multi", "k_nn_los", "k_nn", "k_nn_far",
+                                             "k_los_spectral")}
+    stats = {k: v for k, v in stats.items() if v[0] > 0}
     log("per-kernel stats (count, total ms): " + json.dumps(stats))
     bytes_per = {
         # algorithmic bytes per launch
         "k_nn": S_view * 4 + n * 16 + (dims[0] * dims[1] * dims[2] + 1) * 4,
+        # fused NN + integration: the points and the cell list once, the images of all lines
+        "k_nn_los": n * 16 + (dims[0] * dims[1] * dims[2] + 1) * 4 + R * nl * 4 * len(ems),
         "k_los_spectral": S_view * 16 + R * nl * 4,
         "k_view_extents_multi": n * 16,
     }

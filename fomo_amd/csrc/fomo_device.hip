@@ -1351,10 +1351,8 @@ template <int CH, bool DIRECT>
 __global__ void __launch_bounds__(256) k_los_spectral(int R, int nz, const LineSet ls, const float* __restrict__ vlos,
                                                       const ViewGeom* __restrict__ vgp, int nl, int l0,
                                                       int32_t* __restrict__ flags) {
-  // wave = 32 rays x 2 halves of the LOS; the halves are summed with a cross-lane exchange
-  const int lane = threadIdx.x & 63;
-  const int half = lane >> 5;
-  const int ray = (blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) * 32 + (lane & 31);
+  // one lane per ray: the samples are added in LOS order (the order of k_nn_los)
+  const int ray = blockIdx.x * blockDim.x + threadIdx.x;
   const bool live = ray < R;
   const int line = blockIdx.y;
   const float2* __restrict__ em = ls.em[line];
@@ -1372,17 +1370,15 @@ __global__ void __launch_bounds__(256) k_los_spectral(int R, int nz, const LineS
 #pragma unroll
   for (int c = 0; c < CH; ++c) acc[c] = 0.0;
   bool bad = false;
-  const int kh = (nz + 1) >> 1;
-  const int k0 = half ? kh : 0, k1 = half ? nz : kh;
   const int rr_ = live ? ray : 0;
   // per-sample (peak, fwhm) and LOS velocity written by k_nn; one-sample lookahead
   float2 en = make_float2(0.f, 0.f);
   float vn = 0.f;
-  if (k0 < k1) { en = em[(size_t)k0 * R + rr_]; vn = vlos[(size_t)k0 * R + rr_]; }
-  for (int k = k0; k < k1; ++k) {
+  if (nz > 0) { en = em[rr_]; vn = vlos[rr_]; }
+  for (int k = 0; k < nz; ++k) {
     const float2 e = en;
     const float v = vn;
-    if (k + 1 < k1) {
+    if (k + 1 < nz) {
       en = em[(size_t)(k + 1) * R + rr_];
       vn = vlos[(size_t)(k + 1) * R + rr_];
     }
@@ -1434,38 +1430,204 @@ __global__ void __launch_bounds__(256) k_los_spectral(int R, int nz, const LineS
       }
     }
   }
-  // sum the two halves: the upper half hands its partial sums to the lower half through LDS,
-  // 8 lambda pixels at a time (wave-local exchange; LDS accesses of one wave complete in order)
-  __shared__ double xch[4][8][32];
-  const int wv = threadIdx.x >> 6, l32 = lane & 31;
-#pragma unroll
-  for (int c0 = 0; c0 < CH; c0 += 8) {
-    if (half) {
-#pragma unroll
-      for (int j = 0; j < 8 && c0 + j < CH; ++j) xch[wv][j][l32] = acc[c0 + j];
-    }
-    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    if (!half) {
-#pragma unroll
-      for (int j = 0; j < 8 && c0 + j < CH; ++j) acc[c0 + j] += xch[wv][j][l32];
-    }
-    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-  }
   if (!DIRECT) {
-    const bool anybad = bad || (__shfl_xor(bad ? 1 : 0, 32, 64) != 0);
-    if (live && half == 0) flag[ray] = anybad ? 1 : 0;
-    if (anybad) return;
+    if (live) flag[ray] = bad ? 1 : 0;
+    if (bad) return;
   } else {
     if (!live || flag[ray] == 0) return;
   }
-  if (!live || half) return;
+  if (!live) return;
   const double sc = vgp->scale;
   float* o = ls.out[line] + (size_t)ray * nl + l0;
 #pragma unroll
   for (int c = 0; c < CH; ++c)
     if (l0 + c < nl) o[c] = (float)(sc * acc[c]);
+}
+
+// ------------------------------------------------------------------------------------
+// Fused NN query + spectral integration (k_nn, k_nn_far and k_los_spectral in one pass, for
+// 2 <= nl <= 64 and several lines). G lanes per ray (G = lines rounded up to 2, 4, 8); lane q
+// integrates line q. The G lanes of a ray query G consecutive samples at once (lane q: sample
+// kb + q; lookup grid, then the exact octree query), then hand each sample's (LOS velocity,
+// sqrt T, peaks) to all G lanes by cross-lane shuffles, and every lane adds the G samples in LOS
+// order with the integration step of k_los_spectral. The per-sample values never go through
+// HBM, and the memory-bound queries of some waves overlap the FP64 integration of others. Each
+// (ray, line) sees the same per-sample values in the same order as k_nn + k_los_spectral, so a
+// batch of L lines and L single-line renders give identical images. As in k_los_spectral, a
+// (ray, line) with a sample whose recurrence anchors would under- or overflow is flagged
+// (flags[line * R + ray]) and redone by the DIRECT instantiation, launched right after, with one
+// exp per sample and lambda pixel.
+// ------------------------------------------------------------------------------------
+template <int RS4, int CH, int G, bool DIRECT>
+__global__ void __launch_bounds__(256) k_nn_los(const float4* __restrict__ pts, GridInfo g,
+                                                const float4* __restrict__ lslot, const float4* __restrict__ lcand,
+                                                LookupInfo lk, const float4* __restrict__ mpts,
+                                                const float4* __restrict__ nlo, const float4* __restrict__ nhi,
+                                                BvhInfo bv, const ViewGeom* __restrict__ vgp, int nx, int ny, int nz,
+                                                const float4* __restrict__ rec, RecInfo ri, const LineSet ls, int nl,
+                                                int32_t* __restrict__ flags) {
+  constexpr int NL = 4 * RS4 - 4;  // lines a record holds
+  constexpr int RPW = 64 / G;      // rays per wave
+  const int lane = threadIdx.x & 63;
+  const int q = lane & (G - 1);
+  const int64_t R = (int64_t)nx * ny;
+  const int64_t ray = ((int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) * RPW + lane / G;
+  const bool live = ray < R;
+  const int i = live ? (int)(ray / nx) : 0, j = live ? (int)(ray % nx) : 0;
+  const ViewGeom vg = *vgp;
+  const double minx = vg.ext[0], maxx = vg.ext[1], miny = vg.ext[2], maxy = vg.ext[3], minz = vg.ext[4];
+  const double xx = double(j) / (nx - 1) * (maxx - minx) + minx;  // fomo-nearestneighbour.cpp:152
+  const double yy = double(i) / (ny - 1) * (maxy - miny) + miny;  // :153
+  const double cb = vg.cb, sb = vg.sb, cl = vg.cl, sl = vg.sl, md = vg.md;
+  // line q's constants, selected without lane-dependent indexing of the kernel arguments
+  double Kq = 0.0, wla = 0.0, l0c = 0.0;
+  float* out = nullptr;
+#pragma unroll
+  for (int l = 0; l < kMaxLines; ++l)
+    if (l == q) {
+      Kq = ri.K[l];
+      wla = ls.wla[l];
+      l0c = ls.l0c[l];
+      out = ls.out[l];
+    }
+  bool integ = live && q < ls.n;
+  int32_t* const flag = flags + (size_t)q * R + (live ? ray : 0);
+  if (DIRECT) {
+    // whole waves leave together when none of their (ray, line)s was flagged
+    integ = integ && *flag != 0;
+    if (__ballot(integ) == 0) return;
+  }
+  bool bad = false;
+  const double dl = wla / (nl - 1);
+  const double lamc = double(0) / (nl - 1) * wla - wla / 2.;  // lambdaval of the first pixel (:200)
+  const double k4ln2 = 4. * log(2.);
+  double acc[CH];
+#pragma unroll
+  for (int c = 0; c < CH; ++c) acc[c] = 0.0;
+  int prev = -1;  // this lane's last resolved point: the warm start of the exact query
+  const int base = lane & ~(G - 1);
+  for (int kb = 0; kb < nz; kb += G) {
+    const int k = kb + q;
+    int r = -1;
+    if (live && k < nz) {
+      const double zz = double(k) * vg.deltaz + minz;             // :162
+      const double p0 = xx * cb * cl + yy * sl + zz * sb * cl;    // :165
+      const double p1 = -xx * cb * sl + yy * cl - zz * sb * sl;
+      const double p2 = -xx * sb + zz * cb;
+      SearchBox box;
+      box.t0 = (float)p0, box.t1 = (float)p1, box.t2 = (float)p2;  // :171
+      box.lo0 = (float)(p0 - md), box.lo1 = (float)(p1 - md), box.lo2 = (float)(p2 - md);  // :179
+      box.hi0 = (float)(p0 + md), box.hi1 = (float)(p1 + md), box.hi2 = (float)(p2 + md);
+      r = -2;
+      if (lk.enabled)
+        r = nn_lookup(pts, lslot, lcand, lk, box.t0, box.t1, box.t2, box.lo0, box.lo1, box.lo2, box.hi0, box.hi1,
+                      box.hi2);
+      if (r == -2) {
+        if (!(box.hi0 > g.bmin[0] && box.lo0 < g.bmax[0] && box.hi1 > g.bmin[1] && box.lo1 < g.bmax[1] &&
+              box.hi2 > g.bmin[2] && box.lo2 < g.bmax[2])) {
+          r = -1;  // no data point can be strictly inside the box
+        } else {
+          double best = INFINITY;
+          int bpos = -1, borig = 0x7fffffff;
+          if (prev >= 0) {
+            const float4 pq = pts[prev];
+            if (box.holds(pq)) {
+              const double d0 = (double)pq.x - (double)box.t0, d1 = (double)pq.y - (double)box.t1,
+                           d2 = (double)pq.z - (double)box.t2;
+              best = (d0 * d0 + d1 * d1) + d2 * d2;
+              bpos = prev;
+              borig = __float_as_int(pq.w);
+            }
+          }
+          r = bvh_query(pts, mpts, nlo, nhi, bv, box, best, bpos, borig);
+        }
+      }
+      if (r >= 0) prev = r;
+    }
+    // this lane's sample: the values write_sample would store
+    float f[4 * RS4];
+#pragma unroll
+    for (int c = 0; c < 4 * RS4; ++c) f[c] = 0.f;
+    if (r >= 0) {
+      const float4* rp = rec + (size_t)r * RS4;
+#pragma unroll
+      for (int c = 0; c < RS4; ++c) {
+        const float4 a = rp[c];
+        f[4 * c] = a.x, f[4 * c + 1] = a.y, f[4 * c + 2] = a.z, f[4 * c + 3] = a.w;
+      }
+    }
+    const float vls =
+        r >= 0 ? (float)(((0.0 + (sb * cl) * (double)f[0]) + (-sb * sl) * (double)f[1]) + cb * (double)f[2]) : 0.f;
+    const double st = r >= 0 ? sqrt((double)f[3]) : 0.0;
+#pragma unroll 1
+    for (int o = 0; o < G; ++o) {
+      if (kb + o >= nz) break;  // the same for every lane
+      const int src = base + o;
+      const float v = __shfl(vls, src, 64);
+      const double sto = __shfl(st, src, 64);
+      float pk = 0.f;
+#pragma unroll
+      for (int l = 0; l < (G < NL ? G : NL); ++l) {
+        const float t = __shfl(f[4 + l], src, 64);
+        if (l == q) pk = t;
+      }
+      if (!integ || pk == 0.f) continue;
+      // k_los_spectral's step for (peak, fwhm, vlos)
+      const double peak = pk, w = (double)(Kq > 0 ? (float)(Kq * sto) : 1.f);
+      const double mu = (double)v * l0c;
+      const double kap = k4ln2 / (w * w);
+      const double d0 = lamc - mu;
+      if (DIRECT) {
+#pragma unroll
+        for (int c = 0; c < CH; ++c) {
+          const double d = d0 + (double)c * dl;
+          acc[c] += peak * exp(-kap * d * d);
+        }
+        continue;
+      }
+      const double a0 = -kap * d0 * d0;
+      const double ar = -kap * (2.0 * d0 * dl + dl * dl);
+      const double aq = -2.0 * kap * dl * dl;
+      if (a0 > -600.0 && fabs(ar) < 100.0 && aq > -30.0) {
+        const double qq = exp(aq);
+        const double ra = exp(ar);
+        double rr = ra;
+        const double G0 = peak * exp(a0);
+        const double G1 = G0 * rr;
+        rr *= qq;
+        const double G2 = G1 * rr;
+        rr *= qq;
+        const double G3 = G2 * rr;
+        const double q2 = qq * qq, q4 = q2 * q2, q16 = (q4 * q4) * (q4 * q4);
+        const double ra2 = ra * ra;
+        const double R0 = (ra2 * ra2) * (q4 * q2);
+        const double R1 = R0 * q4, R2 = R1 * q4, R3 = R2 * q4;
+        double Gc[4] = {G0, G1, G2, G3};
+        double RR[4] = {R0, R1, R2, R3};
+#pragma unroll
+        for (int c0 = 0; c0 < CH; c0 += 4) {
+#pragma unroll
+          for (int jj = 0; jj < 4; ++jj) {
+            acc[c0 + jj] += Gc[jj];
+            Gc[jj] *= RR[jj];
+            RR[jj] *= q16;
+          }
+        }
+      } else {
+        bad = true;
+      }
+    }
+  }
+  if (!integ) return;
+  if (!DIRECT) {
+    *flag = bad ? 1 : 0;
+    if (bad) return;
+  }
+  const double sc = vg.scale;
+  float* o = out + (size_t)ray * nl;
+#pragma unroll
+  for (int c = 0; c < CH; ++c)
+    if (c < nl) o[c] = (float)(sc * acc[c]);
 }
 
 // nl == 1 (Imaging): intens += peak (fomo-nearestneighbour.cpp:217-225)
@@ -2215,8 +2377,8 @@ int pick_chunk(int nl, int* npass) {
 template <int CH>
 void launch_spectral_t(hipStream_t s, int R, int nz, const LineSet& ls, const float* vlos, const ViewGeom* geom,
                        int nl, int l0, int32_t* flag) {
-  // 128 rays per 256-thread block (each wave: 32 rays x 2 halves of the LOS)
-  dim3 grid((unsigned)((R + 127) / 128), ls.n);
+  // 256 rays per block, one lane per ray
+  dim3 grid((unsigned)((R + 255) / 256), ls.n);
   k_los_spectral<CH, false><<<grid, 256, 0, s>>>(R, nz, ls, vlos, geom, nl, l0, flag);
   k_los_spectral<CH, true><<<grid, 256, 0, s>>>(R, nz, ls, vlos, geom, nl, l0, flag);
 }
@@ -2243,6 +2405,51 @@ void enqueue_los(hipStream_t s, int R, int nz, const LineSet& ls, const float* v
       default: break;
     }
   }
+}
+
+// Fused NN + integration pass (k_nn_los) for spectral views with one lambda pass (2 <= nl <= 64)
+// and per-point records. Returns false when it does not apply (the caller runs enqueue_nn +
+// enqueue_los instead).
+template <int RS4, int G>
+void launch_nn_los_t(hipStream_t s, int ch, unsigned blocks, const Dataset& ds, const ViewGeom* geom, int nx, int ny,
+                     int nz, const float4* rec, const RecInfo& ri, const LineSet& ls, int nl, int32_t* flags) {
+  switch (ch) {
+#define FOMO_NNL_CH(C)                                                                                        \
+  case C:                                                                                                    \
+    k_nn_los<RS4, C, G, false><<<blocks, 256, 0, s>>>(                                                       \
+        ds.pts.as<float4>(), ds.grid, ds.lslot.as<float4>(), ds.lcand.as<float4>(), ds.lk, ds.mpts.as<float4>(), \
+        ds.nlo.as<float4>(), ds.nhi.as<float4>(), ds.bvh, geom, nx, ny, nz, rec, ri, ls, nl, flags);           \
+    k_nn_los<RS4, C, G, true><<<blocks, 256, 0, s>>>(                                                        \
+        ds.pts.as<float4>(), ds.grid, ds.lslot.as<float4>(), ds.lcand.as<float4>(), ds.lk, ds.mpts.as<float4>(), \
+        ds.nlo.as<float4>(), ds.nhi.as<float4>(), ds.bvh, geom, nx, ny, nz, rec, ri, ls, nl, flags);           \
+    break;
+    FOMO_NNL_CH(4) FOMO_NNL_CH(8) FOMO_NNL_CH(12) FOMO_NNL_CH(16) FOMO_NNL_CH(20) FOMO_NNL_CH(24) FOMO_NNL_CH(28)
+    FOMO_NNL_CH(32) FOMO_NNL_CH(36) FOMO_NNL_CH(40) FOMO_NNL_CH(44) FOMO_NNL_CH(48) FOMO_NNL_CH(52)
+    FOMO_NNL_CH(56) FOMO_NNL_CH(60) FOMO_NNL_CH(64)
+#undef FOMO_NNL_CH
+    default: break;
+  }
+}
+
+// flags: R * ls.n ints of scratch.
+bool enqueue_nn_los(hipStream_t s, const Dataset& ds, const ViewGeom* geom, int nx, int ny, int nz,
+                    const float4* rec, int rs4, const RecInfo& ri, const LineSet& ls, int nl, int32_t* flags) {
+  // several lines share each query; a single line is faster as one thread per sample (k_nn) and
+  // one lane per ray (k_los_spectral), which add the samples in the same order
+  if (!rec || nl < 2 || nl > 64 || ls.n < 2 || ls.n > kMaxLines) return false;
+  int npass = 1;
+  const int ch = pick_chunk(nl, &npass);
+  if (npass != 1) return false;
+  const int G = ls.n <= 2 ? 2 : (ls.n <= 4 ? 4 : 8);
+  if ((rs4 == 2 && G > 4) || (rs4 != 2 && rs4 != 4)) return false;
+  const int64_t R = (int64_t)nx * ny;
+  const int64_t rays_per_block = 4 * (64 / G);
+  const unsigned blocks = (unsigned)((R + rays_per_block - 1) / rays_per_block);
+  ProfScope ps("k_nn_los", s);
+  if (rs4 == 2 && G == 2) launch_nn_los_t<2, 2>(s, ch, blocks, ds, geom, nx, ny, nz, rec, ri, ls, nl, flags);
+  else if (rs4 == 2) launch_nn_los_t<2, 4>(s, ch, blocks, ds, geom, nx, ny, nz, rec, ri, ls, nl, flags);
+  else launch_nn_los_t<4, 8>(s, ch, blocks, ds, geom, nx, ny, nz, rec, ri, ls, nl, flags);
+  return true;
 }
 
 LineSet make_lineset(const std::vector<const float2*>& em, const std::vector<float*>& out,
@@ -2391,9 +2598,12 @@ int enqueue_render(hipStream_t s, const Dataset& ds, const Goft& gc, int mode, d
   } else if (mode == MODE_PROJ) {
     if (int rc = enqueue_projection(s, ds, gc, rb, nx, ny, nl, width)) return rc;
   } else if (mode == MODE_NN) {
-    enqueue_nn(s, ds, rb.geom, nx, ny, nz, rb.idx, rb.rayflag, gc.rec.as<float4>(), gc.rs4, 1, rb.aux, rb.sem, gc.ri);
     LineSet ls = make_lineset({rb.sem}, {rb.out}, {gc.lambda0}, width);
-    enqueue_los(s, (int)R, nz, ls, rb.aux, rb.geom, nl, rb.flag);
+    if (!enqueue_nn_los(s, ds, rb.geom, nx, ny, nz, gc.rec.as<float4>(), gc.rs4, gc.ri, ls, nl, rb.flag)) {
+      enqueue_nn(s, ds, rb.geom, nx, ny, nz, rb.idx, rb.rayflag, gc.rec.as<float4>(), gc.rs4, 1, rb.aux, rb.sem,
+                 gc.ri);
+      enqueue_los(s, (int)R, nz, ls, rb.aux, rb.geom, nl, rb.flag);
+    }
   } else {
     enqueue_nn(s, ds, rb.geom, nx, ny, nz, rb.idx, rb.rayflag);
     ProfScope ps("k_los_thomson", s);
@@ -3360,8 +3570,6 @@ static int enqueue_batch(Batch& bt, hipStream_t s, hipStream_t s2) {
     ViewGeom* geom = bt.geom.as<ViewGeom>() + v;
     if (s2 && v >= 2) HIPOK(hipStreamWaitEvent(s, bt.ev[nv + v - 2], 0));
     enqueue_view_finalize(s, MODE_NN, bt.l[v], bt.b[v], bt.nx, bt.ny, bt.nz, bt.dn, part, nbp, geom);
-    enqueue_nn(s, *ds, geom, bt.nx, bt.ny, bt.nz, bt.idx.as<int32_t>() + buf * S, bt.rayflag.as<uint8_t>() + buf * R,
-               bt.rec.as<float4>(), bt.rs4, L, svl, sem, bt.ri);
     std::vector<const float2*> ems;
     std::vector<float*> outs;
     std::vector<double> l0s;
@@ -3371,6 +3579,13 @@ static int enqueue_batch(Batch& bt, hipStream_t s, hipStream_t s2) {
       l0s.push_back(gs[gi]->lambda0);
     }
     LineSet ls = make_lineset(ems, outs, l0s, bt.width);
+    // one fused pass per view when it applies (then the views simply follow each other on s)
+    if (enqueue_nn_los(s, *ds, geom, bt.nx, bt.ny, bt.nz, bt.rec.as<float4>(), bt.rs4, bt.ri, ls, bt.nl, flag)) {
+      if (s2) HIPOK(hipEventRecord(bt.ev[nv + v], s));
+      continue;
+    }
+    enqueue_nn(s, *ds, geom, bt.nx, bt.ny, bt.nz, bt.idx.as<int32_t>() + buf * S, bt.rayflag.as<uint8_t>() + buf * R,
+               bt.rec.as<float4>(), bt.rs4, L, svl, sem, bt.ri);
     hipStream_t si = s;
     if (s2) {
       HIPOK(hipEventRecord(bt.ev[v], s));
