@@ -730,23 +730,22 @@ __global__ void k_clamp_counts(const int32_t* __restrict__ cnt, int64_t ncl, int
   pos[L] = max(cnt[L], 0);
 }
 
-// Fast path: the unconstrained nearest point from the target's lookup cell. Returns its sorted
-// position if it lies strictly inside the search box, else -2 (caller runs the ring search).
-__device__ __forceinline__ int nn_lookup(const float4* __restrict__ pts, const float4* __restrict__ lslot,
-                                         const float4* __restrict__ lcand, const LookupInfo& lk, float t0, float t1,
-                                         float t2, float lo0, float lo1, float lo2, float hi0, float hi1, float hi2) {
+// Lookup cell of a target (-1 outside the lookup grid).
+__device__ __forceinline__ int64_t lookup_cell(const LookupInfo& lk, float t0, float t1, float t2) {
   const double v0 = ((double)t0 - lk.o[0]) * lk.invh;
   const double v1 = ((double)t1 - lk.o[1]) * lk.invh;
   const double v2 = ((double)t2 - lk.o[2]) * lk.invh;
   if (!(v0 >= 0.0 && v0 < (double)lk.dims[0] && v1 >= 0.0 && v1 < (double)lk.dims[1] && v2 >= 0.0 &&
         v2 < (double)lk.dims[2]))
-    return -2;
-  const int64_t L = ((int64_t)(int)v2 * lk.dims[1] + (int)v1) * lk.dims[0] + (int)v0;
-  // the whole 64-byte record at once (one sector); the overflow list only for long lists
-  const float4* rp = lslot + (int64_t)kSlots * L;
-  float4 sl[kSlots];
-#pragma unroll
-  for (int q = 0; q < kSlots; ++q) sl[q] = rp[q];
+    return -1;
+  return ((int64_t)(int)v2 * lk.dims[1] + (int)v1) * lk.dims[0] + (int)v0;
+}
+
+// The nearest candidate of a lookup cell record sl (read from the overflow list for long lists),
+// if it lies strictly inside the search box; else -2.
+__device__ __forceinline__ int lookup_pick(const float4* __restrict__ pts, const float4* __restrict__ lcand,
+                                           const float4 (&sl)[kSlots], float t0, float t1, float t2, float lo0,
+                                           float lo1, float lo2, float hi0, float hi1, float hi2) {
   const int w0 = __float_as_int(sl[0].w);
   if (w0 == -1) return -2;
   // candidates hold their coordinates inline (x, y, z, sorted position); the original index
@@ -779,6 +778,21 @@ __device__ __forceinline__ int nn_lookup(const float4* __restrict__ pts, const f
   }
   if (lo0 < bq.x && bq.x < hi0 && lo1 < bq.y && bq.y < hi1 && lo2 < bq.z && bq.z < hi2) return bpos;
   return -2;
+}
+
+// Fast path: the unconstrained nearest point from the target's lookup cell. Returns its sorted
+// position if it lies strictly inside the search box, else -2 (the caller runs the exact query).
+__device__ __forceinline__ int nn_lookup(const float4* __restrict__ pts, const float4* __restrict__ lslot,
+                                         const float4* __restrict__ lcand, const LookupInfo& lk, float t0, float t1,
+                                         float t2, float lo0, float lo1, float lo2, float hi0, float hi1, float hi2) {
+  const int64_t L = lookup_cell(lk, t0, t1, t2);
+  if (L < 0) return -2;
+  // the whole 64-byte record at once (one sector); the overflow list only for long lists
+  const float4* rp = lslot + (int64_t)kSlots * L;
+  float4 sl[kSlots];
+#pragma unroll
+  for (int q = 0; q < kSlots; ++q) sl[q] = rp[q];
+  return lookup_pick(pts, lcand, sl, t0, t1, t2, lo0, lo1, lo2, hi0, hi1, hi2);
 }
 
 // ------------------------------------------------------------------------------------
@@ -1478,7 +1492,7 @@ __global__ void __launch_bounds__(256) k_nn_los(const float4* __restrict__ pts, 
   const double minx = vg.ext[0], maxx = vg.ext[1], miny = vg.ext[2], maxy = vg.ext[3], minz = vg.ext[4];
   const double xx = double(j) / (nx - 1) * (maxx - minx) + minx;  // fomo-nearestneighbour.cpp:152
   const double yy = double(i) / (ny - 1) * (maxy - miny) + miny;  // :153
-  const double cb = vg.cb, sb = vg.sb, cl = vg.cl, sl = vg.sl, md = vg.md;
+  const double cb = vg.cb, sb = vg.sb, cl = vg.cl, sl_ = vg.sl, md = vg.md;
   // line q's constants, selected without lane-dependent indexing of the kernel arguments
   double Kq = 0.0, wla = 0.0, l0c = 0.0;
   float* out = nullptr;
@@ -1506,59 +1520,90 @@ __global__ void __launch_bounds__(256) k_nn_los(const float4* __restrict__ pts, 
   for (int c = 0; c < CH; ++c) acc[c] = 0.0;
   int prev = -1;  // this lane's last resolved point: the warm start of the exact query
   const int base = lane & ~(G - 1);
-  for (int kb = 0; kb < nz; kb += G) {
-    const int k = kb + q;
-    int r = -1;
-    if (live && k < nz) {
-      const double zz = double(k) * vg.deltaz + minz;             // :162
-      const double p0 = xx * cb * cl + yy * sl + zz * sb * cl;    // :165
-      const double p1 = -xx * cb * sl + yy * cl - zz * sb * sl;
-      const double p2 = -xx * sb + zz * cb;
-      SearchBox box;
-      box.t0 = (float)p0, box.t1 = (float)p1, box.t2 = (float)p2;  // :171
-      box.lo0 = (float)(p0 - md), box.lo1 = (float)(p1 - md), box.lo2 = (float)(p2 - md);  // :179
-      box.hi0 = (float)(p0 + md), box.hi1 = (float)(p1 + md), box.hi2 = (float)(p2 + md);
-      r = -2;
-      if (lk.enabled)
-        r = nn_lookup(pts, lslot, lcand, lk, box.t0, box.t1, box.t2, box.lo0, box.lo1, box.lo2, box.hi0, box.hi1,
-                      box.hi2);
-      if (r == -2) {
-        if (!(box.hi0 > g.bmin[0] && box.lo0 < g.bmax[0] && box.hi1 > g.bmin[1] && box.lo1 < g.bmax[1] &&
-              box.hi2 > g.bmin[2] && box.lo2 < g.bmax[2])) {
-          r = -1;  // no data point can be strictly inside the box
-        } else {
-          double best = INFINITY;
-          int bpos = -1, borig = 0x7fffffff;
-          if (prev >= 0) {
-            const float4 pq = pts[prev];
-            if (box.holds(pq)) {
-              const double d0 = (double)pq.x - (double)box.t0, d1 = (double)pq.y - (double)box.t1,
-                           d2 = (double)pq.z - (double)box.t2;
-              best = (d0 * d0 + d1 * d1) + d2 * d2;
-              bpos = prev;
-              borig = __float_as_int(pq.w);
-            }
+  // Three-stage pipeline over groups of G samples: while group kb is integrated, the record of
+  // group kb + G (winner picked from its lookup cell record) and the lookup cell record of group
+  // kb + 2G are in flight.
+  // stage 1: target, search box and lookup cell record of this lane's sample k
+  const auto query_issue = [&](int k, SearchBox& box, float4 (&sl)[kSlots], bool& act, bool& cell) {
+    act = live && k < nz;
+    cell = false;
+    if (!act) return;
+    const double zz = double(k) * vg.deltaz + minz;             // :162
+    const double p0 = xx * cb * cl + yy * sl_ + zz * sb * cl;   // :165
+    const double p1 = -xx * cb * sl_ + yy * cl - zz * sb * sl_;
+    const double p2 = -xx * sb + zz * cb;
+    box.t0 = (float)p0, box.t1 = (float)p1, box.t2 = (float)p2;  // :171
+    box.lo0 = (float)(p0 - md), box.lo1 = (float)(p1 - md), box.lo2 = (float)(p2 - md);  // :179
+    box.hi0 = (float)(p0 + md), box.hi1 = (float)(p1 + md), box.hi2 = (float)(p2 + md);
+    if (!lk.enabled) return;
+    const int64_t L = lookup_cell(lk, box.t0, box.t1, box.t2);
+    if (L < 0) return;
+    cell = true;
+    const float4* rp = lslot + (int64_t)kSlots * L;
+#pragma unroll
+    for (int c = 0; c < kSlots; ++c) sl[c] = rp[c];
+  };
+  // stage 2: the winner (lookup, else the exact octree query), -1 for none
+  const auto query_resolve = [&](const SearchBox& box, const float4 (&sl)[kSlots], bool act, bool cell) -> int {
+    if (!act) return -1;
+    int r = cell ? lookup_pick(pts, lcand, sl, box.t0, box.t1, box.t2, box.lo0, box.lo1, box.lo2, box.hi0,
+                               box.hi1, box.hi2)
+                 : -2;
+    if (r == -2) {
+      if (!(box.hi0 > g.bmin[0] && box.lo0 < g.bmax[0] && box.hi1 > g.bmin[1] && box.lo1 < g.bmax[1] &&
+            box.hi2 > g.bmin[2] && box.lo2 < g.bmax[2])) {
+        r = -1;  // no data point can be strictly inside the box
+      } else {
+        double best = INFINITY;
+        int bpos = -1, borig = 0x7fffffff;
+        if (prev >= 0) {
+          const float4 pq = pts[prev];
+          if (box.holds(pq)) {
+            const double d0 = (double)pq.x - (double)box.t0, d1 = (double)pq.y - (double)box.t1,
+                         d2 = (double)pq.z - (double)box.t2;
+            best = (d0 * d0 + d1 * d1) + d2 * d2;
+            bpos = prev;
+            borig = __float_as_int(pq.w);
           }
-          r = bvh_query(pts, mpts, nlo, nhi, bv, box, best, bpos, borig);
         }
+        r = bvh_query(pts, mpts, nlo, nhi, bv, box, best, bpos, borig);
       }
-      if (r >= 0) prev = r;
     }
-    // this lane's sample: the values write_sample would store
-    float f[4 * RS4];
+    if (r >= 0) prev = r;
+    return r;
+  };
+  const auto record_issue = [&](int r, float (&f)[4 * RS4]) {
 #pragma unroll
     for (int c = 0; c < 4 * RS4; ++c) f[c] = 0.f;
-    if (r >= 0) {
-      const float4* rp = rec + (size_t)r * RS4;
+    if (r < 0) return;
+    const float4* rp = rec + (size_t)r * RS4;
 #pragma unroll
-      for (int c = 0; c < RS4; ++c) {
-        const float4 a = rp[c];
-        f[4 * c] = a.x, f[4 * c + 1] = a.y, f[4 * c + 2] = a.z, f[4 * c + 3] = a.w;
-      }
+    for (int c = 0; c < RS4; ++c) {
+      const float4 a = rp[c];
+      f[4 * c] = a.x, f[4 * c + 1] = a.y, f[4 * c + 2] = a.z, f[4 * c + 3] = a.w;
     }
-    const float vls =
-        r >= 0 ? (float)(((0.0 + (sb * cl) * (double)f[0]) + (-sb * sl) * (double)f[1]) + cb * (double)f[2]) : 0.f;
-    const double st = r >= 0 ? sqrt((double)f[3]) : 0.0;
+  };
+  SearchBox boxN;
+  float4 slN[kSlots];
+  bool actN, cellN;
+  float fC[4 * RS4];
+  query_issue(q, boxN, slN, actN, cellN);
+  int rC = query_resolve(boxN, slN, actN, cellN);
+  record_issue(rC, fC);
+  query_issue(G + q, boxN, slN, actN, cellN);
+  for (int kb = 0; kb < nz; kb += G) {
+    SearchBox boxA;
+    float4 slA[kSlots];
+    bool actA, cellA;
+    query_issue(kb + 2 * G + q, boxA, slA, actA, cellA);
+    const int rB = query_resolve(boxN, slN, actN, cellN);
+    float fB[4 * RS4];
+    record_issue(rB, fB);
+    // group kb: this lane's sample, the values write_sample would store
+    const float vls = rC >= 0 ? (float)(((0.0 + (sb * cl) * (double)fC[0]) + (-sb * sl_) * (double)fC[1]) +
+                                        cb * (double)fC[2])
+                              : 0.f;
+    const double st = rC >= 0 ? sqrt((double)fC[3]) : 0.0;
 #pragma unroll 1
     for (int o = 0; o < G; ++o) {
       if (kb + o >= nz) break;  // the same for every lane
@@ -1568,7 +1613,7 @@ __global__ void __launch_bounds__(256) k_nn_los(const float4* __restrict__ pts, 
       float pk = 0.f;
 #pragma unroll
       for (int l = 0; l < (G < NL ? G : NL); ++l) {
-        const float t = __shfl(f[4 + l], src, 64);
+        const float t = __shfl(fC[4 + l], src, 64);
         if (l == q) pk = t;
       }
       if (!integ || pk == 0.f) continue;
@@ -1617,6 +1662,15 @@ __global__ void __launch_bounds__(256) k_nn_los(const float4* __restrict__ pts, 
         bad = true;
       }
     }
+    // advance the pipeline
+    rC = rB;
+#pragma unroll
+    for (int c = 0; c < 4 * RS4; ++c) fC[c] = fB[c];
+    boxN = boxA;
+#pragma unroll
+    for (int c = 0; c < kSlots; ++c) slN[c] = slA[c];
+    actN = actA;
+    cellN = cellA;
   }
   if (!integ) return;
   if (!DIRECT) {
