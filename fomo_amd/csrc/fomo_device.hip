@@ -39,6 +39,7 @@
 #include "arena.h"
 #include "delaunay.h"
 #include "octree.h"
+#include "spectral_exp.h"
 
 // ------------------------------------------------------------------------------------
 // constants (GSL CODATA-2006 values used by fomo-CHIANTI.cpp:17-19)
@@ -1379,7 +1380,6 @@ __global__ void __launch_bounds__(256) k_los_spectral(int R, int nz, const LineS
   }
   const double dl = wla / (nl - 1);
   const double lamc = double(l0) / (nl - 1) * wla - wla / 2.;  // lambdaval of the first pixel of the pass (:200)
-  const double k4ln2 = 4. * log(2.);
   double acc[CH];
 #pragma unroll
   for (int c = 0; c < CH; ++c) acc[c] = 0.0;
@@ -1399,7 +1399,7 @@ __global__ void __launch_bounds__(256) k_los_spectral(int R, int nz, const LineS
     if (!live || e.x == 0.f) continue;
     const double peak = e.x, w = e.y;
     const double mu = (double)v * lam0_over_c;
-    const double kap = k4ln2 / (w * w);
+    const double kap = fomo::spectral_kappa(w);
     const double d0 = lamc - mu;
     if (DIRECT) {
 #pragma unroll
@@ -1414,10 +1414,13 @@ __global__ void __launch_bounds__(256) k_los_spectral(int R, int nz, const LineS
       if (a0 > -600.0 && fabs(ar) < 100.0 && aq > -30.0) {
         // stride-4 recurrence: four independent chains c = j, j+4, ...
         //   g_{c+4} = g_c R_c,  R_{c+4} = R_c q^16,  R_0 = r_0 r_1 r_2 r_3
-        const double q = exp(aq);
-        const double ra = exp(ar);
+        const double xa[3] = {aq, ar, a0};
+        double ex[3];
+        fomo::exp3(xa, ex);
+        const double q = ex[0];
+        const double ra = ex[1];
         double r = ra;
-        double G0 = peak * exp(a0);
+        double G0 = peak * ex[2];
         const double G1 = G0 * r;
         r *= q;
         const double G2 = G1 * r;
@@ -1514,7 +1517,6 @@ __global__ void __launch_bounds__(256) k_nn_los(const float4* __restrict__ pts, 
   bool bad = false;
   const double dl = wla / (nl - 1);
   const double lamc = double(0) / (nl - 1) * wla - wla / 2.;  // lambdaval of the first pixel (:200)
-  const double k4ln2 = 4. * log(2.);
   double acc[CH];
 #pragma unroll
   for (int c = 0; c < CH; ++c) acc[c] = 0.0;
@@ -1620,7 +1622,7 @@ __global__ void __launch_bounds__(256) k_nn_los(const float4* __restrict__ pts, 
       // k_los_spectral's step for (peak, fwhm, vlos)
       const double peak = pk, w = (double)(Kq > 0 ? (float)(Kq * sto) : 1.f);
       const double mu = (double)v * l0c;
-      const double kap = k4ln2 / (w * w);
+      const double kap = fomo::spectral_kappa(w);
       const double d0 = lamc - mu;
       if (DIRECT) {
 #pragma unroll
@@ -1634,10 +1636,13 @@ __global__ void __launch_bounds__(256) k_nn_los(const float4* __restrict__ pts, 
       const double ar = -kap * (2.0 * d0 * dl + dl * dl);
       const double aq = -2.0 * kap * dl * dl;
       if (a0 > -600.0 && fabs(ar) < 100.0 && aq > -30.0) {
-        const double qq = exp(aq);
-        const double ra = exp(ar);
+        const double xa[3] = {aq, ar, a0};
+        double ex[3];
+        fomo::exp3(xa, ex);
+        const double qq = ex[0];
+        const double ra = ex[1];
         double rr = ra;
-        const double G0 = peak * exp(a0);
+        const double G0 = peak * ex[2];
         const double G1 = G0 * rr;
         rr *= qq;
         const double G2 = G1 * rr;
