@@ -29,10 +29,14 @@ sys.path.insert(0, ROOT)
 
 C = 2.99792458e8
 PEAK_HBM = 8.0e12  # B/s, MI355X HBM3E
+PEAK_FP64 = 78.6e12  # FLOP/s, MI355X dense FP64 (vector and matrix peaks coincide)
+# Algorithmic FP64 work of the reference's integration per LOS sample x lambda pixel (W): the
+# Gaussian term and its accumulation, about 8 FLOP besides the exp (fomo-nearestneighbour.cpp:200)
+FLOP_PER_W = 8.0
 # HBM bytes per launch (read + write) of the default workload's kernels, from two rocprofv3 --pmc
 # runs (FETCH_SIZE x 2.000, WRITE_SIZE x 1.000, the factors a 1 GiB copy kernel gives):
 # profiles/batched_render_256cube_counters.md
-MEASURED_TRAFFIC = {"k_nn": 2.027e9, "k_los_spectral": 4.49e8}
+MEASURED_TRAFFIC = {"k_nn_los": 1.407e9}
 
 
 def log(*a):
@@ -57,7 +61,7 @@ def main():
     ap.add_argument("--autotune", action="store_true",
                     help="let the autotuner pick the index configuration first (for C2 the default one is as fast "
                          "inside the batch, where the NN pass shares the GPU with the integration)")
-    ap.add_argument("--workload", choices=["batched", "thomson", "serve", "tet"], default="batched")
+    ap.add_argument("--workload", choices=["batched", "thomson", "serve", "tet", "projection"], default="batched")
     ap.add_argument("--requests", type=int, default=1000, help="requests per arrival rate (serve workload)")
     ap.add_argument("--serve-requests", type=int, default=240, help="requests in the serving phase of the default run")
     ap.add_argument("--corona-m", type=int, default=645, help="corona grid points per axis (thomson workload)")
@@ -78,6 +82,8 @@ def main():
         return serve(F, args)
     if args.workload == "tet":
         return tet(F, args)
+    if args.workload == "projection":
+        return projection(F, args)
 
     m, nx, ny, nz, nl = args.m, args.nx, args.nx, args.nz, args.nl
     t0 = time.time()
@@ -154,15 +160,29 @@ def main():
     roof = None
     if cnt > 0:
         avg_s = tot / cnt / 1e3
-        ach = bytes_per[dom] / avg_s
         # HBM bytes per launch measured with rocprofv3 FETCH_SIZE / WRITE_SIZE (calibrated on a plain
         # copy kernel) for this exact workload: profiles/batched_render_256cube_counters.md
         default_cfg = (m, nx, nz, nl, len(ems)) == (256, 256, 128, 60, 4)
         traffic = MEASURED_TRAFFIC.get(dom) if default_cfg else None
-        roof = {"kernel": dom, "bound": "hbm", "achieved": ach / 1e9, "peak": PEAK_HBM / 1e9, "unit": "GB/s",
-                "frac": ach / PEAK_HBM, "traffic": traffic, "avg_launch_ms": avg_s * 1e3,
-                "algorithmic_bytes_per_launch": bytes_per[dom],
-                "share_of_step": {k: stats[k][1] / max(1e-9, sum(v[1] for v in stats.values())) for k in stats}}
+        share = {k: stats[k][1] / max(1e-9, sum(v[1] for v in stats.values())) for k in stats}
+        if dom == "k_nn_los":
+            # the fused pass is FP64-bound: roofline against the FP64 peak, with the algorithmic
+            # FLOPs of the integration it performs (all lines of one view per launch)
+            flops = FLOP_PER_W * S_view * nl * len(ems)
+            ach = flops / avg_s
+            roof = {"kernel": dom, "bound": "mfma", "what": "FP64 (vector and matrix peaks coincide)",
+                    "achieved": ach / 1e12, "peak": PEAK_FP64 / 1e12, "unit": "TFLOP/s", "frac": ach / PEAK_FP64,
+                    "traffic": traffic, "avg_launch_ms": avg_s * 1e3, "algorithmic_flops_per_launch": flops,
+                    "flop_per_W": FLOP_PER_W,
+                    "hbm": {"algorithmic_bytes_per_launch": bytes_per[dom],
+                            "measured_GBps": traffic / avg_s / 1e9 if traffic else None,
+                            "measured_frac": traffic / avg_s / PEAK_HBM if traffic else None},
+                    "share_of_step": share}
+        else:
+            ach = bytes_per[dom] / avg_s
+            roof = {"kernel": dom, "bound": "hbm", "achieved": ach / 1e9, "peak": PEAK_HBM / 1e9, "unit": "GB/s",
+                    "frac": ach / PEAK_HBM, "traffic": traffic, "avg_launch_ms": avg_s * 1e3,
+                    "algorithmic_bytes_per_launch": bytes_per[dom], "share_of_step": share}
 
     # ---- plain: one render() per (view, line), as FoMoObject::render does per call: emissivity
     #      recomputed, one view rendered, result copied to the host; the index stays resident
@@ -451,6 +471,79 @@ def tet(F, args):
            "delaunay": {"tets": ntets, "build_s": tbuild},
            "kernels_ms": {k: (v[1] / v[0] if v[0] else None) for k, v in st.items()},
            "nn_same_view_ms": dtn * 1e3}
+    print(json.dumps(out), flush=True)
+    em.close()
+    ds.close()
+    tab.close()
+    F.shutdown()
+
+
+def projection(F, args):
+    """Projection method at scale (fomo-projection.cpp:17-214): the 256^3 loop cube jittered by
+    +-0.3 cell (16.8M points) of the tetra workload, one view (l=0.3, b=1.2) of 512x512 pixels,
+    100 lambda pixels over 0.2 A (nz 256 only sets the path length). A step is one render, image
+    copied to the host. W = points x lambda pixels. The per-(pixel, lambda) sums are FP64-bound:
+    one exp per point and lambda pixel, in point order."""
+    m = args.m
+    nx = ny = args.nx if args.nx != 256 else 512
+    nz = args.nz if args.nz != 128 else 256
+    nl = args.nl if args.nl != 60 else 100
+    d = F.gen_loop_cube(m, seed=1, t_in=1.0e6, t_out=2.5e6, jitter=0.3, sausage=0.1)
+    n = d["x"].size
+    tmp = tempfile.mkdtemp()
+    path = F.gen_goft_table(os.path.join(tmp, "fe9.dat"), ion="fe_9", lambda0=171.0, atweight=56.0, peak_logt=5.9)
+    tab = F.Table.from_file(path)
+    lambda0, atw = 171.0, 56.0
+    width = 0.2 * C / lambda0
+    ds = F.Dataset(d["x"], d["y"], d["z"], [d["rho"], d["T"], d["vx"], d["vy"], d["vz"]])
+    em = F.Emission.chianti(ds, tab, lambda0, atw)
+    l, b = 0.3, 1.2
+    for _ in range(max(1, args.warmup)):
+        F.render_projection(em, l, b, nx, ny, nz, nl, width)
+    steps = max(1, args.steps)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        img, meta = F.render_projection(em, l, b, nx, ny, nz, nl, width)
+    dt = (time.perf_counter() - t0) / steps
+    W = float(n) * nl
+    F.profile_reset()
+    F.profile_enable(True)
+    F.render_projection(em, l, b, nx, ny, nz, nl, width)
+    F.profile_enable(False)
+    st = {k: F.profile_stats(k) for k in ("k_view_rotate_nn", "k_proj_keys", "proj_sort", "k_proj_spectral")}
+    log(f"projection render: {dt * 1e3:.1f} ms/view, {W / dt:.3e} point*lambda/s; kernels " + json.dumps(st))
+    cnt, tot = st["k_proj_spectral"]
+    roof = None
+    if cnt:
+        avg_s = tot / cnt / 1e3
+        flops = FLOP_PER_W * W
+        roof = {"kernel": "k_proj_spectral", "bound": "mfma", "what": "FP64 (vector and matrix peaks coincide)",
+                "achieved": flops / avg_s / 1e12, "peak": PEAK_FP64 / 1e12, "unit": "TFLOP/s",
+                "frac": flops / avg_s / PEAK_FP64, "traffic": None, "avg_launch_ms": avg_s * 1e3,
+                "algorithmic_flops_per_launch": flops, "flop_per_W": FLOP_PER_W}
+    cpu = None
+    if not args.no_cpu:
+        # oracle (reference float pipeline, sums in point order, one thread) on every 64th point
+        from oracle import oracle as O
+        otab = O.read_table(path)
+        sel = slice(0, n, 64)
+        pk, fw = O.emission(0, d["rho"][sel], d["T"][sel], otab)
+        t0 = time.perf_counter()
+        O.render_projection(0, d["x"][sel], d["y"][sel], d["z"][sel], pk, fw, d["vx"][sel], d["vy"][sel],
+                            d["vz"][sel], l, b, nx, ny, nz, nl, width, lambda0)
+        tc = time.perf_counter() - t0
+        ns = d["x"][sel].size
+        cpu = {"value": ns * nl / tc, "unit": "point*lambda/s", "cores": 1, "kind": "port",
+               "sample": f"every 64th point ({ns} points) of the same view and spectral grid, {tc:.2f}s; the "
+                         f"oracle's projection (reference float pipeline, point-order sums, one thread)"}
+        log(f"cpu: {ns * nl / tc:.3e} point*lambda/s on 1 thread")
+    out = {"metric": "points x lambda pixels per second, projection render", "value": W / dt,
+           "unit": "point*lambda/s", "n_gpus": 1, "steps": steps, "warmup": args.warmup, "ms_per_step": dt * 1e3,
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+           "config": {"workload": f"projection render, {m}^3 jittered loop cube ({n} points), 1 view, "
+                                  f"{nx}x{ny} image, nz {nz}, nl {nl}"},
+           "kernels_ms": {k: (v[1] / v[0] if v[0] else None) for k, v in st.items()},
+           "roofline": roof, "cpu_baseline": cpu}
     print(json.dumps(out), flush=True)
     em.close()
     ds.close()
