@@ -148,6 +148,36 @@ def test_narrow_lines_direct_path(engine, fe9_table):
     em.close(), T.close(), ds.close()
 
 
+@pytest.mark.parametrize("nlines,nl", [(3, 60), (4, 7), (5, 33), (8, 64)])
+def test_fused_batch_lines_equal_plain(engine, fe9_table, tmp_path, nlines, nl):
+    """Batches of 3-8 lines run the fused NN + integration pass (2, 4 or 8 lanes per ray); lines
+    with different lambda0 and atomic weight must each equal their single-line render bit for bit."""
+    F = engine
+    path, tab = fe9_table
+    lam = [171.0, 195.1, 211.3, 193.5, 284.2, 335.4, 94.0, 131.2][:nlines]
+    atw = [56.0, 56.0, 28.0, 56.0, 56.0, 56.0, 40.0, 56.0][:nlines]
+    d = F.gen_loop_cube(20, seed=4, jitter=0.2)
+    ds = _dataset(F, d)
+    tabs = [F.Table.from_file(F.gen_goft_table(str(tmp_path / f"l{i}.dat"), lambda0=lam[i], atweight=atw[i],
+                                               peak_logt=5.8 + 0.15 * i)) for i in range(nlines)]
+    ems = [F.Emission.chianti(ds, t, lam[i], atw[i]) for i, t in enumerate(tabs)]
+    ls, bs = [0.3, 2.1], [1.1, np.pi / 2]
+    nx, ny, nz = 26, 22, 33
+    width = 0.25 * C / 171.0
+    bt = F.Batch(ems, ls, bs, nx, ny, nz, nl, width)
+    bt.run()
+    out, meta = bt.read()
+    for v in range(len(ls)):
+        for g, e in enumerate(ems):
+            img, m = F.render_nn(e, ls[v], bs[v], nx, ny, nz, nl, width)
+            assert np.count_nonzero(img) > 0
+            assert np.array_equal(out[v, g].ravel(), img), f"view {v} line {g}"
+            assert np.array_equal(meta[v], m)
+    bt.close()
+    for o in ems + tabs + [ds]:
+        o.close()
+
+
 def test_batch_equals_plain(engine, fe9_table, tmp_path):
     """A batched graph (views x lines) gives bit-identical images to one render per view and line."""
     F = engine
