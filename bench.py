@@ -421,10 +421,12 @@ def serve(F, args):
 
 
 def tet(F, args):
-    """Tetra-path render at scale (the reference's CGAL method): a 256^3 loop cube jittered by
-    +-0.3 cell (16.8M points, unstructured Delaunay), 2.5 MK corona, 10 % sausage mode; one view
-    (l=0, b=pi/2) of 512x512 pixels, 256 LOS samples, 100 lambda pixels over 0.2 A. The Delaunay
-    triangulation (host, once per dataset) is timed separately; a step is one render."""
+    """Tetra-path render at scale, linear interpolation in the containing Delaunay tet (the Linear
+    method; the reference's CGAL method takes the tet's nearest vertex, timed beside it): a 256^3
+    loop cube jittered by +-0.3 cell (16.8M points, unstructured Delaunay), 2.5 MK corona, 10 %
+    sausage mode; one view (l=0, b=pi/2) of 512x512 pixels, 256 LOS samples, 100 lambda pixels over
+    0.2 A. The Delaunay triangulation (host, once per dataset) is timed separately; a step is one
+    render."""
     m = args.m
     nx = ny = args.nx if args.nx != 256 else 512
     nz = args.nz if args.nz != 128 else 256
@@ -445,32 +447,71 @@ def tet(F, args):
     em = F.Emission.chianti(ds, tab, lambda0, atw)
     l, b = 0.0, np.pi / 2
     for _ in range(max(1, args.warmup)):
-        F.render_tet(em, l, b, nx, ny, nz, nl, width)
+        F.render_tet(em, l, b, nx, ny, nz, nl, width, linear=True)
     steps = max(1, args.steps)
     t0 = time.perf_counter()
     for _ in range(steps):
-        img, meta = F.render_tet(em, l, b, nx, ny, nz, nl, width)
+        img, meta = F.render_tet(em, l, b, nx, ny, nz, nl, width, linear=True)
     dt = (time.perf_counter() - t0) / steps
     W = float(nx) * ny * nz * nl
     F.profile_reset()
     F.profile_enable(True)
-    F.render_tet(em, l, b, nx, ny, nz, nl, width)
+    F.render_tet(em, l, b, nx, ny, nz, nl, width, linear=True)
     F.profile_enable(False)
-    st = {k: F.profile_stats(k) for k in ("k_view_rotate_nn", "k_tet_walk", "k_los_spectral")}
-    log(f"tet render: {dt * 1e3:.1f} ms/view, {W / dt:.3e} W/s; kernels " + json.dumps(st))
+    st = {k: F.profile_stats(k) for k in ("k_view_rotate_nn", "k_tet_walk_linear", "k_los_spectral")}
+    log(f"linear tet render: {dt * 1e3:.1f} ms/view, {W / dt:.3e} W/s; kernels " + json.dumps(st))
+    # the CGAL method (nearest vertex of the tet) on the same view
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        F.render_tet(em, l, b, nx, ny, nz, nl, width)
+    dtc = (time.perf_counter() - t0) / steps
+    log(f"nearest-vertex tet render: {dtc * 1e3:.1f} ms/view")
     # same view, nearest-neighbour method, for comparison
     t0 = time.perf_counter()
     F.render_nn(em, l, b, nx, ny, nz, nl, width)
     dtn = time.perf_counter() - t0
-    out = {"metric": "LOS samples x lambda per second, tetra (CGAL-method) render", "value": W / dt,
+    # roofline of the walk: per sample the containing tet's record (80 B), the 4 vertex records it
+    # interpolates (4 x 32 B) and the per-sample outputs (12 B)
+    cnt, tot = st["k_tet_walk_linear"]
+    roof = None
+    if cnt:
+        avg_s = tot / cnt / 1e3
+        byt = float(nx) * ny * nz * (80 + 4 * 32 + 12)
+        roof = {"kernel": "k_tet_walk_linear", "bound": "hbm", "achieved": byt / avg_s / 1e9, "peak": PEAK_HBM / 1e9,
+                "unit": "GB/s", "frac": byt / avg_s / PEAK_HBM, "traffic": None, "avg_launch_ms": avg_s * 1e3,
+                "algorithmic_bytes_per_launch": byt}
+    cpu = None
+    if not args.no_cpu:
+        # oracle (numpy FP64 on scipy's Qhull triangulation, find_simplex per sample, linear
+        # interpolation) on a 64^3 cube of the same model, 128^2 x 64 samples x the same lambda grid;
+        # the triangulation is built first and not timed (as on the GPU)
+        from oracle import oracle as O
+        from scipy.spatial import Delaunay
+        ds_ = F.gen_loop_cube(64, seed=1, t_in=1.0e6, t_out=2.5e6, jitter=0.3, sausage=0.1)
+        otab = O.read_table(path)
+        pk, fw = O.emission(1, ds_["rho"], ds_["T"], otab)
+        tri = Delaunay(np.stack([ds_["x"], ds_["y"], ds_["z"]], 1).astype(np.float64))
+        cn, cz = 128, 64
+        t0 = time.perf_counter()
+        O.render_tet(ds_["x"], ds_["y"], ds_["z"], pk, fw, ds_["vx"], ds_["vy"], ds_["vz"], l, b, cn, cn, cz, nl,
+                     width, lambda0, linear=True, tri=tri)
+        tc = time.perf_counter() - t0
+        Wc = float(cn) * cn * cz * nl
+        cpu = {"value": Wc / tc, "unit": "LOS samples*lambda/s", "cores": 1, "kind": "port",
+               "sample": f"64^3 jittered loop cube (262144 points), {cn}^2 x {cz} samples x {nl} lambda pixels, "
+                         f"{tc:.2f}s: the oracle's linear tetra render (numpy FP64, scipy Qhull find_simplex), "
+                         f"triangulation not timed"}
+        log(f"cpu: {Wc / tc:.3e} W/s on 1 thread")
+    out = {"metric": "LOS samples x lambda per second, tetra linear-interpolation render", "value": W / dt,
            "unit": "LOS samples*lambda/s", "n_gpus": 1, "steps": steps, "warmup": args.warmup,
            "ms_per_step": dt * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
            "data": "synthetic",
-           "config": {"workload": f"tetra render, {m}^3 jittered loop cube ({d['x'].size} points), 1 view, "
-                                  f"{nx}x{ny} image, nz {nz}, nl {nl}"},
+           "config": {"workload": f"tetra linear-interpolation render, {m}^3 jittered loop cube ({d['x'].size} "
+                                  f"points), 1 view, {nx}x{ny} image, nz {nz}, nl {nl}"},
+           "nearest_vertex_ms": dtc * 1e3,
            "delaunay": {"tets": ntets, "build_s": tbuild},
            "kernels_ms": {k: (v[1] / v[0] if v[0] else None) for k, v in st.items()},
-           "nn_same_view_ms": dtn * 1e3}
+           "nn_same_view_ms": dtn * 1e3, "roofline": roof, "cpu_baseline": cpu}
     print(json.dumps(out), flush=True)
     em.close()
     ds.close()

@@ -206,14 +206,15 @@ def rel_l2(a, ref):
 
 
 def render_tet(x, y, z, peak, fwhm, vx, vy, vz, l, b, nx, ny, nz, nl, lambda_width, lambda0, dn=False,
-               linear=False):
+               linear=False, tri=None):
     """The reference's CGAL render (fomo-CGAL.cpp:75-307) restated in numpy, FP64, on an independent
     triangulation: scipy.spatial.Delaunay (Qhull) and its find_simplex for the point location
     (CGAL's locate, :194-206). Each sample takes the values of the nearest vertex of its tet
     (nearest_vertex_in_cell, :211-222), none outside the convex hull (:225-228); linear=True takes
     the barycentric interpolation instead (the "Linear" extension). Extents come from the NN
-    oracle's rotation (the same formula, :118-121). Returns (image [ny*nx*max(nl,1)],
-    vertex per sample [(iy*nx + jx)*nz + k], original index or -1, meta[10])."""
+    oracle's rotation (the same formula, :118-121). tri: a scipy Delaunay of these points to reuse
+    (the reference triangulates once per render() call, :33-73). Returns (image
+    [ny*nx*max(nl,1)], vertex per sample [(iy*nx + jx)*nz + k], original index or -1, meta[10])."""
     import math
 
     from scipy.spatial import Delaunay
@@ -221,7 +222,8 @@ def render_tet(x, y, z, peak, fwhm, vx, vy, vz, l, b, nx, ny, nz, nl, lambda_wid
     x, y, z, vx, vy, vz = map(_pf, (x, y, z, vx, vy, vz))
     peak, fwhm = _pd(peak), _pd(fwhm)
     pts = np.stack([x, y, z], 1).astype(np.float64)
-    tri = Delaunay(pts)
+    if tri is None:
+        tri = Delaunay(pts)
     minx, maxx, miny, maxy, minz, maxz = extents(x, y, z, l, b)
     cb, sb, cl, sl = math.cos(b), math.sin(b), math.cos(l), math.sin(l)
     jj = np.arange(nx, dtype=np.float64)[None, :, None]

@@ -178,6 +178,33 @@ def test_fused_batch_lines_equal_plain(engine, fe9_table, tmp_path, nlines, nl):
         o.close()
 
 
+@pytest.mark.parametrize("nl", [1, 100])
+def test_unfused_batch_equals_plain(engine, fe9_table, tmp_path, nl):
+    """Batches the fused pass does not take (imaging, or more than 64 lambda pixels: two lambda
+    passes) run k_nn + the integration kernels on two streams; they too equal single renders."""
+    F = engine
+    path, tab = fe9_table
+    d = F.gen_loop_cube(20, seed=6)
+    ds = _dataset(F, d)
+    tabs = [F.Table.from_file(path), F.Table.from_file(F.gen_goft_table(str(tmp_path / "b.dat"), peak_logt=6.3))]
+    ems = [F.Emission.chianti(ds, t, tab["lambda0"], tab["atweight"], imaging=(nl == 1)) for t in tabs]
+    ls, bs = [0.2, 1.4, 2.9], [np.pi / 2, 0.7, 1.2]
+    nx, ny, nz = 24, 20, 31
+    width = 0.2 * C / tab["lambda0"]
+    bt = F.Batch(ems, ls, bs, nx, ny, nz, nl, width)
+    bt.run()
+    out, meta = bt.read()
+    for v in range(len(ls)):
+        for g, e in enumerate(ems):
+            img, m = F.render_nn(e, ls[v], bs[v], nx, ny, nz, nl, width)
+            assert np.count_nonzero(img) > 0
+            assert np.array_equal(out[v, g].ravel(), img), f"view {v} line {g}"
+            assert np.array_equal(meta[v], m)
+    bt.close()
+    for o in ems + tabs + [ds]:
+        o.close()
+
+
 def test_batch_equals_plain(engine, fe9_table, tmp_path):
     """A batched graph (views x lines) gives bit-identical images to one render per view and line."""
     F = engine
