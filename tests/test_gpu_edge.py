@@ -1,0 +1,130 @@
+"""GPU edge cases of the nearest-neighbour render against the oracle: 2D datacubes (z = 0, as the
+reference sets it, fomo-nearestneighbour.cpp:80), face-on views of them (zero z extent: path length
+1, all samples of a ray at one position), datasets of 1 and 2 points (degenerate extents: an empty
+search box), duplicated points (exact distance ties: lowest original index) and zero densities.
+Bars: NN indices bit-exact; images within relative L2 1e-5 of the FP64 oracle."""
+import numpy as np
+import pytest
+
+import fomo_amd as F
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+C = 2.99792458e8
+TOL_IMG = 1e-5
+
+
+def _check(ds, d, tab, em, l, b, nx, ny, nz, nl, imaging=False):
+    width = 0.2 * C / tab["lambda0"]
+    idx, meta = F.nn_indices(ds, l, b, nx, ny, nz)
+    pk, fw = O.emission(1, d["rho"], d["T"], tab, imaging=imaging)
+    ref, ridx, rmeta, _ = O.render_nn(1, d["x"], d["y"], d["z"], pk, fw, d["vx"], d["vy"], d["vz"], l, b, nx, ny, nz,
+                                      nl, width, tab["lambda0"], want_idx=True)
+    assert np.array_equal(meta[:6], rmeta[:6])
+    assert np.array_equal(idx, ridx), f"{(idx != ridx).sum()} NN choices differ"
+    img, m2 = F.render_nn(em, l, b, nx, ny, nz, nl, width)
+    np.testing.assert_allclose(m2[7:10], rmeta[7:10], rtol=1e-12)
+    if np.count_nonzero(ref) == 0:
+        assert np.count_nonzero(img) == 0
+    else:
+        assert O.rel_l2(img, ref) <= TOL_IMG
+    return idx, img, ref
+
+
+def _fields(n, rng):
+    return {"rho": (1e9 * (1 + 0.3 * rng.random(n))).astype(np.float32),
+            "T": (1e6 * (1 + rng.random(n))).astype(np.float32),
+            "vx": (2e4 * rng.standard_normal(n)).astype(np.float32),
+            "vy": (2e4 * rng.standard_normal(n)).astype(np.float32),
+            "vz": (2e4 * rng.standard_normal(n)).astype(np.float32)}
+
+
+def _make(d, dim=3):
+    return F.Dataset(d["x"], d["y"], d["z"] if dim == 3 else None,
+                     [d["rho"], d["T"], d["vx"], d["vy"], d["vz"]], dim=dim)
+
+
+@pytest.mark.parametrize("l,b,nl", [(0.0, 0.0, 12), (0.0, np.pi / 2, 12), (0.7, 0.4, 1)])
+def test_2d_datacube(engine, fe9_table, l, b, nl):
+    path, tab = fe9_table
+    rng = np.random.default_rng(11)
+    m = 30
+    g = np.arange(m, dtype=np.float64) / (m - 1) * 10 - 5
+    yy, xx = np.meshgrid(g, g, indexing="ij")
+    d = {"x": (xx.ravel() + 0.1 * rng.uniform(-1, 1, m * m)).astype(np.float32),
+         "y": (yy.ravel() + 0.1 * rng.uniform(-1, 1, m * m)).astype(np.float32)}
+    d["z"] = np.zeros(m * m, np.float32)
+    d.update(_fields(m * m, rng))
+    ds = _make(d, dim=2)
+    T = F.Table.from_file(path)
+    em = F.Emission.chianti(ds, T, tab["lambda0"], tab["atweight"], imaging=(nl == 1))
+    try:
+        idx, img, ref = _check(ds, d, tab, em, l, b, 24, 20, 9, nl, imaging=(nl == 1))
+        assert (idx >= 0).any()
+    finally:
+        em.close(), T.close(), ds.close()
+
+
+@pytest.mark.parametrize("n", [1, 2, 5])
+def test_tiny_datasets(engine, fe9_table, n):
+    path, tab = fe9_table
+    rng = np.random.default_rng(20 + n)
+    d = {"x": rng.uniform(-1, 1, n).astype(np.float32), "y": rng.uniform(-1, 1, n).astype(np.float32),
+         "z": rng.uniform(-1, 1, n).astype(np.float32)}
+    d.update(_fields(n, rng))
+    ds = _make(d)
+    T = F.Table.from_file(path)
+    em = F.Emission.chianti(ds, T, tab["lambda0"], tab["atweight"])
+    try:
+        for (l, b) in [(0.0, np.pi / 2), (1.1, 0.6)]:
+            _check(ds, d, tab, em, l, b, 8, 6, 7, 10)
+    finally:
+        em.close(), T.close(), ds.close()
+
+
+def test_duplicated_points_tie_to_lowest_index(engine, fe9_table):
+    path, tab = fe9_table
+    rng = np.random.default_rng(31)
+    m = 12
+    g = (np.arange(m) / (m - 1) * 10 - 5).astype(np.float32)
+    zz, yy, xx = np.meshgrid(g, g, g, indexing="ij")
+    base = {"x": xx.ravel(), "y": yy.ravel(), "z": zz.ravel()}
+    n = base["x"].size
+    # every point twice, the copy first in index order half of the time
+    perm = rng.permutation(2 * n)
+    d = {k: np.concatenate([v, v])[perm] for k, v in base.items()}
+    d.update(_fields(2 * n, rng))
+    ds = _make(d)
+    T = F.Table.from_file(path)
+    em = F.Emission.chianti(ds, T, tab["lambda0"], tab["atweight"])
+    try:
+        idx, _, _ = _check(ds, d, tab, em, 0.0, np.pi / 2, m, m, m, 10)
+        # grid samples sit on grid points: each has two equidistant candidates
+        assert (idx >= 0).all()
+        twin = {}
+        for i, p in enumerate(perm):
+            twin.setdefault(p % n, []).append(i)
+        lowest = {min(v) for v in twin.values()}
+        assert set(np.unique(idx).tolist()) <= lowest
+    finally:
+        em.close(), T.close(), ds.close()
+
+
+def test_zero_density_points(engine, fe9_table):
+    path, tab = fe9_table
+    rng = np.random.default_rng(41)
+    n = 3000
+    d = {"x": rng.uniform(-4, 4, n).astype(np.float32), "y": rng.uniform(-4, 4, n).astype(np.float32),
+         "z": rng.uniform(-4, 4, n).astype(np.float32)}
+    d.update(_fields(n, rng))
+    d["rho"][::3] = 0.0  # log10 of zero density: the reference warns and the emission is 0
+    ds = _make(d)
+    T = F.Table.from_file(path)
+    em = F.Emission.chianti(ds, T, tab["lambda0"], tab["atweight"])
+    try:
+        pk_dev, _ = em.read()
+        assert (pk_dev[::3] == 0).all()
+        _check(ds, d, tab, em, 0.5, 1.0, 20, 18, 16, 12)
+    finally:
+        em.close(), T.close(), ds.close()

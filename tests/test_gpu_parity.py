@@ -148,6 +148,37 @@ def test_narrow_lines_direct_path(engine, fe9_table):
     em.close(), T.close(), ds.close()
 
 
+def test_fused_batch_direct_lines(engine, fe9_table):
+    """A batch whose lines include one much narrower than a lambda pixel: the fused pass flags that
+    line's rays and its DIRECT instantiation redoes them; every line equals its single-line render
+    bit for bit, and the narrow line matches the FP64 oracle."""
+    F = engine
+    path, tab = fe9_table
+    d = F.gen_loop_cube(16, seed=9)
+    ds = _dataset(F, d)
+    T = F.Table.from_file(path)
+    atws = [tab["atweight"], 5.6e5, tab["atweight"]]  # the middle line: thermal width / 100
+    ems = [F.Emission.chianti(ds, T, tab["lambda0"], a) for a in atws]
+    width = 0.2 * C / tab["lambda0"]
+    ls, bs = [0.3, 1.9], [0.8, np.pi / 2]
+    bt = F.Batch(ems, ls, bs, 20, 20, 20, 40, width)
+    bt.run()
+    out, _ = bt.read()
+    for v in range(2):
+        for g, e in enumerate(ems):
+            img, _ = F.render_nn(e, ls[v], bs[v], 20, 20, 20, 40, width)
+            assert np.array_equal(out[v, g].ravel(), img), f"view {v} line {g}"
+    heavy = dict(tab)
+    heavy["atweight"] = atws[1]
+    pk, fw = O.emission(1, d["rho"], d["T"], heavy)
+    ref, _, _, _ = O.render_nn(1, d["x"], d["y"], d["z"], pk, fw, d["vx"], d["vy"], d["vz"], ls[0], bs[0], 20, 20, 20,
+                               40, width, tab["lambda0"])
+    assert O.rel_l2(out[0, 1].ravel(), ref) <= TOL_IMG
+    bt.close()
+    for o in ems + [T, ds]:
+        o.close()
+
+
 @pytest.mark.parametrize("nlines,nl", [(3, 60), (4, 7), (5, 33), (8, 64)])
 def test_fused_batch_lines_equal_plain(engine, fe9_table, tmp_path, nlines, nl):
     """Batches of 3-8 lines run the fused NN + integration pass (2, 4 or 8 lanes per ray); lines
