@@ -625,11 +625,40 @@ def thomson(F, args):
     F.profile_enable(False)
     st = {k: F.profile_stats(k) for k in ("k_view_rotate_thomson", "k_nn", "k_nn_far", "k_los_thomson")}
     log("per-kernel (count, ms): " + json.dumps(st))
+    S = float(nx) * ny * nz
+    roof = None
+    if st["k_nn"][0] and st["k_nn_far"][0]:
+        # the NN pass (lookup + exact octree query for the samples the lookup cannot resolve):
+        # algorithmic bytes S*4 written + N*12 read per view; the integration: 16 B per sample
+        t_nn = (st["k_nn"][1] / st["k_nn"][0] + st["k_nn_far"][1] / st["k_nn_far"][0]) / 1e3
+        bnn = S * 4 + n * 12.0
+        roof = {"kernel": "k_nn + k_nn_far (NN pass)", "bound": "hbm", "achieved": bnn / t_nn / 1e9,
+                "peak": PEAK_HBM / 1e9, "unit": "GB/s", "frac": bnn / t_nn / PEAK_HBM, "traffic": None,
+                "avg_launch_ms": t_nn * 1e3, "algorithmic_bytes_per_launch": bnn,
+                "note": "the exact octree query of far-field samples is a chain of dependent node loads"}
+        if st["k_los_thomson"][0]:
+            t_int = st["k_los_thomson"][1] / st["k_los_thomson"][0] / 1e3
+            roof["integration"] = {"kernel": "k_los_thomson", "achieved": S * 16 / t_int / 1e9,
+                                   "frac": S * 16 / t_int / PEAK_HBM, "avg_launch_ms": t_int * 1e3}
+    cpu = None
+    if not args.no_cpu:
+        # oracle (C++, OpenMP, kd-tree) on a 160^3-grid corona, one view of 256^2 x 128 samples
+        from oracle import oracle as O
+        dc = F.gen_corona(160, seed=3, sigma=0.1)
+        tn, pn = O.thomson_emission(0, dc["x"], dc["y"], dc["z"], dc["rho"])
+        _, _, _, tim = O.render_thomson(0, dc["x"], dc["y"], dc["z"], tn, pn, ls[1], b, 256, 256, 128)
+        tc = float(tim[0] + tim[1] + tim[2])
+        cpu = {"value": 256 * 256 * 128 / tc, "unit": "LOS samples/s", "cores": O.num_threads(), "kind": "port",
+               "sample": f"corona on a 160^3 grid ({dc['x'].size} points), one view of 256^2 x 128 samples, "
+                         f"{tc:.2f}s (rotation {tim[0]:.2f}s, kd-tree {tim[1]:.2f}s, ray loop {tim[2]:.2f}s): the "
+                         f"oracle's Thomson render (reference float pipeline)"}
+        log(f"cpu: {256 * 256 * 128 / tc:.3e} samples/s on {O.num_threads()} threads")
     out = {"metric": "LOS samples per second, Thomson pB render (16 views)", "value": W / dt,
            "unit": "LOS samples/s", "n_gpus": 1, "steps": nv, "warmup": args.warmup, "ms_per_step": dt / nv * 1e3,
            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
            "config": {"workload": f"Thomson pB, corona {n} points, 16 views b=pi/2, {nx}x{ny} image, nz {nz}"},
-           "kernels_ms": {k: (v[1] / v[0] if v[0] else None) for k, v in st.items()}}
+           "kernels_ms": {k: (v[1] / v[0] if v[0] else None) for k, v in st.items()},
+           "roofline": roof, "cpu_baseline": cpu}
     print(json.dumps(out), flush=True)
     em.close()
     ds.close()
