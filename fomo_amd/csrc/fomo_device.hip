@@ -33,6 +33,7 @@
 #include <mutex>
 #include <string>
 #include <thread>
+#include <type_traits>
 #include <vector>
 
 #include "../../include/fomo_hip.h"
@@ -1362,6 +1363,41 @@ __global__ void k_idx_to_orig(const int32_t* __restrict__ idx, const int32_t* __
 // reference's order. A ray with a sample whose anchors would under/overflow is flagged and
 // left to the DIRECT instantiation (one exp per sample x lambda), launched right after.
 // ------------------------------------------------------------------------------------
+// The Gaussian line profile exp(-kap (d0 + c dl)^2) over lambda pixels c = 0, 1, ... by ratio
+// recurrence. profile_args: xa = {aq, ar, a0}, the exponents of the ratio of ratios, of the first
+// ratio (pixel 1 over pixel 0) and of pixel 0; false when the recurrence could leave the double
+// range (the caller then evaluates every pixel directly).
+__device__ __forceinline__ bool profile_args(double kap, double d0, double dl, double (&xa)[3]) {
+  const double a0 = -kap * d0 * d0;
+  const double ar = -kap * (2.0 * d0 * dl + dl * dl);
+  const double aq = -2.0 * kap * dl * dl;
+  xa[0] = aq, xa[1] = ar, xa[2] = a0;
+  return a0 > -600.0 && fabs(ar) < 100.0 && aq > -30.0;
+}
+
+// Stride-4 chains of the profile from ex = {q, r, g_0} = exp(xa): chain j holds the pixels
+// j, j+4, j+8, ..., starting at g[j] with ratio R[j]; one step is g *= R, R *= q16.
+//   g_{c+4} = g_c R_c,  R_{c+4} = R_c q^16,  R_0 = r_0 r_1 r_2 r_3 = r^4 q^6
+// Every kernel that integrates the profile builds its chains here, so that the same pixel gets
+// the same value in all of them.
+__device__ __forceinline__ void profile_chains(const double (&ex)[3], double (&g)[4], double (&R)[4], double& q16) {
+  const double q = ex[0], ra = ex[1];
+  double r = ra;
+  g[0] = ex[2];
+  g[1] = g[0] * r;
+  r *= q;
+  g[2] = g[1] * r;
+  r *= q;
+  g[3] = g[2] * r;
+  const double q2 = q * q, q4 = q2 * q2;
+  q16 = (q4 * q4) * (q4 * q4);
+  const double ra2 = ra * ra;
+  R[0] = (ra2 * ra2) * (q4 * q2);
+  R[1] = R[0] * q4;
+  R[2] = R[1] * q4;
+  R[3] = R[2] * q4;
+}
+
 template <int CH, bool DIRECT>
 __global__ void __launch_bounds__(256) k_los_spectral(int R, int nz, const LineSet ls, const float* __restrict__ vlos,
                                                       const ViewGeom* __restrict__ vgp, int nl, int l0,
@@ -1408,37 +1444,17 @@ __global__ void __launch_bounds__(256) k_los_spectral(int R, int nz, const LineS
         acc[c] += peak * exp(-kap * d * d);
       }
     } else {
-      const double a0 = -kap * d0 * d0;
-      const double ar = -kap * (2.0 * d0 * dl + dl * dl);
-      const double aq = -2.0 * kap * dl * dl;
-      if (a0 > -600.0 && fabs(ar) < 100.0 && aq > -30.0) {
-        // stride-4 recurrence: four independent chains c = j, j+4, ...
-        //   g_{c+4} = g_c R_c,  R_{c+4} = R_c q^16,  R_0 = r_0 r_1 r_2 r_3
-        const double xa[3] = {aq, ar, a0};
-        double ex[3];
+      double xa[3];
+      if (profile_args(kap, d0, dl, xa)) {
+        double ex[3], g[4], RR[4], q16;
         fomo::exp3(xa, ex);
-        const double q = ex[0];
-        const double ra = ex[1];
-        double r = ra;
-        double G0 = peak * ex[2];
-        const double G1 = G0 * r;
-        r *= q;
-        const double G2 = G1 * r;
-        r *= q;
-        const double G3 = G2 * r;
-        const double q2 = q * q, q4 = q2 * q2, q16 = (q4 * q4) * (q4 * q4);
-        // R_0 = r_0 r_1 r_2 r_3 = r^4 q^6 from the anchors (no fourth exp)
-        const double ra2 = ra * ra;
-        double R0 = (ra2 * ra2) * (q4 * q2);
-        double R1 = R0 * q4, R2 = R1 * q4, R3 = R2 * q4;
-        double G[4] = {G0, G1, G2, G3};
-        double RR[4] = {R0, R1, R2, R3};
+        profile_chains(ex, g, RR, q16);
 #pragma unroll
         for (int c0 = 0; c0 < CH; c0 += 4) {
 #pragma unroll
           for (int j = 0; j < 4; ++j) {
-            acc[c0 + j] += G[j];
-            G[j] *= RR[j];
+            acc[c0 + j] = fma(peak, g[j], acc[c0 + j]);
+            g[j] *= RR[j];
             RR[j] *= q16;
           }
         }
@@ -1461,21 +1477,60 @@ __global__ void __launch_bounds__(256) k_los_spectral(int R, int nz, const LineS
     if (l0 + c < nl) o[c] = (float)(sc * acc[c]);
 }
 
+// f(integral_constant<int, I>) for I = 0 .. N-1, unrolled at compile time
+template <int N, int I = 0, class F>
+__device__ __forceinline__ void static_for(F&& f) {
+  if constexpr (I < N) {
+    f(std::integral_constant<int, I>{});
+    static_for<N, I + 1>(f);
+  }
+}
+
+// Lane O of each aligned group of G lanes, to every lane of the group. G = 2, 4: a DPP quad
+// permutation (a VALU operand modifier, no LDS round trip; every lane of the quad must be
+// active); G = 8: ds_bpermute.
+template <int G, int O>
+__device__ __forceinline__ int group_bcast(int x) {
+  static_assert(O >= 0 && O < G, "lane outside the group");
+  if constexpr (G == 4) {
+    return __builtin_amdgcn_update_dpp(0, x, O * 0x55, 0xF, 0xF, false);
+  } else if constexpr (G == 2) {
+    return __builtin_amdgcn_update_dpp(0, x, O | (O << 2) | ((2 + O) << 4) | ((2 + O) << 6), 0xF, 0xF, false);
+  } else {
+    return __shfl(x, (int)(threadIdx.x & 63 & ~(G - 1)) + O, 64);
+  }
+}
+template <int G, int O>
+__device__ __forceinline__ float group_bcast(float x) {
+  return __int_as_float(group_bcast<G, O>(__float_as_int(x)));
+}
+template <int G, int O>
+__device__ __forceinline__ double group_bcast(double x) {
+  const int lo = group_bcast<G, O>(__double2loint(x)), hi = group_bcast<G, O>(__double2hiint(x));
+  return __hiloint2double(hi, lo);
+}
+
 // ------------------------------------------------------------------------------------
 // Fused NN query + spectral integration (k_nn, k_nn_far and k_los_spectral in one pass, for
-// 2 <= nl <= 64 and several lines). G lanes per ray (G = lines rounded up to 2, 4, 8); lane q
-// integrates line q. The G lanes of a ray query G consecutive samples at once (lane q: sample
-// kb + q; lookup grid, then the exact octree query), then hand each sample's (LOS velocity,
-// sqrt T, peaks) to all G lanes by cross-lane shuffles, and every lane adds the G samples in LOS
-// order with the integration step of k_los_spectral. The per-sample values never go through
-// HBM, and the memory-bound queries of some waves overlap the FP64 integration of others. Each
-// (ray, line) sees the same per-sample values in the same order as k_nn + k_los_spectral, so a
-// batch of L lines and L single-line renders give identical images. As in k_los_spectral, a
-// (ray, line) with a sample whose recurrence anchors would under- or overflow is flagged
-// (flags[line * R + ray]) and redone by the DIRECT instantiation, launched right after, with one
-// exp per sample and lambda pixel.
+// 2 <= nl <= 64 and several lines). G lanes per ray query G consecutive samples at once (lane q:
+// sample kb + q; lookup grid, then the exact octree query) and hand each sample's values to the
+// other lanes of the ray by cross-lane broadcasts, so the per-sample values never go through HBM
+// and the memory-bound queries of some waves overlap the FP64 integration of others. Two ways
+// to split the integration over the G lanes:
+//   SL == 0: lane q integrates line q (G = lines rounded up to 2, 4, 8) with the step of
+//            k_los_spectral, from the broadcast (LOS velocity, sqrt T, peak).
+//   SL > 0:  the lines share one profile (equal lambda0, lambda window and thermal-width factor,
+//            so their Gaussians differ only in the peak; lines beyond ls.n have peak 0). G = 4:
+//            lane q evaluates the anchors of its own sample, the ray's 4 lanes exchange them,
+//            and lane q runs the profile's chain q (pixels q, q+4, ...) of every sample, adding
+//            peak_l * g to line l's sums. The profile is evaluated once for all lines.
+// Both add, for each (ray, line, pixel), the same values in the same order as k_nn +
+// k_los_spectral (profile_chains, then fma(peak, g, sum)), so a batch of L lines and L
+// single-line renders give identical images. As in k_los_spectral, a (ray, line) with a sample
+// whose recurrence anchors would under- or overflow is flagged (flags[line * R + ray]) and redone
+// by the DIRECT instantiation, launched right after, with one exp per sample and lambda pixel.
 // ------------------------------------------------------------------------------------
-template <int RS4, int CH, int G, bool DIRECT>
+template <int RS4, int CH, int G, bool DIRECT, int SL>
 __global__ void __launch_bounds__(256) k_nn_los(const float4* __restrict__ pts, GridInfo g,
                                                 const float4* __restrict__ lslot, const float4* __restrict__ lcand,
                                                 LookupInfo lk, const float4* __restrict__ mpts,
@@ -1485,6 +1540,9 @@ __global__ void __launch_bounds__(256) k_nn_los(const float4* __restrict__ pts, 
                                                 int32_t* __restrict__ flags) {
   constexpr int NL = 4 * RS4 - 4;  // lines a record holds
   constexpr int RPW = 64 / G;      // rays per wave
+  static_assert(SL == 0 || (G == 4 && SL <= NL && CH % 4 == 0), "a shared profile runs 4 chains per ray");
+  constexpr int CJ = CH / 4;                  // pixels of one chain (SL > 0)
+  constexpr int NACC = SL ? SL * CJ : CH;     // SL > 0: acc[l * CJ + m] is line l, pixel 4 m + q
   const int lane = threadIdx.x & 63;
   const int q = lane & (G - 1);
   const int64_t R = (int64_t)nx * ny;
@@ -1496,32 +1554,47 @@ __global__ void __launch_bounds__(256) k_nn_los(const float4* __restrict__ pts, 
   const double xx = double(j) / (nx - 1) * (maxx - minx) + minx;  // fomo-nearestneighbour.cpp:152
   const double yy = double(i) / (ny - 1) * (maxy - miny) + miny;  // :153
   const double cb = vg.cb, sb = vg.sb, cl = vg.cl, sl_ = vg.sl, md = vg.md;
-  // line q's constants, selected without lane-dependent indexing of the kernel arguments
+  // the profile constants: line q's, selected without lane-dependent indexing of the kernel
+  // arguments, or (shared profile) line 0's
   double Kq = 0.0, wla = 0.0, l0c = 0.0;
   float* out = nullptr;
+  if constexpr (SL > 0) {
+    Kq = ri.K[0], wla = ls.wla[0], l0c = ls.l0c[0];
+  } else {
 #pragma unroll
-  for (int l = 0; l < kMaxLines; ++l)
-    if (l == q) {
-      Kq = ri.K[l];
-      wla = ls.wla[l];
-      l0c = ls.l0c[l];
-      out = ls.out[l];
-    }
-  bool integ = live && q < ls.n;
-  int32_t* const flag = flags + (size_t)q * R + (live ? ray : 0);
+    for (int l = 0; l < kMaxLines; ++l)
+      if (l == q) {
+        Kq = ri.K[l];
+        wla = ls.wla[l];
+        l0c = ls.l0c[l];
+        out = ls.out[l];
+      }
+  }
+  bool integ = live && (SL > 0 || q < ls.n);
+  int32_t* const flag = flags + (size_t)(SL > 0 ? 0 : q) * R + (live ? ray : 0);
+  // lines of this lane with a flagged (ray, line): bit l (shared profile), bit 0 (line q)
+  unsigned need = 0;
   if (DIRECT) {
+    if (integ) {
+      if constexpr (SL > 0) {
+#pragma unroll
+        for (int l = 0; l < SL; ++l)
+          if (l < ls.n && flag[(size_t)l * R] != 0) need |= 1u << l;
+      } else {
+        need = *flag != 0 ? 1u : 0u;
+      }
+    }
     // whole waves leave together when none of their (ray, line)s was flagged
-    integ = integ && *flag != 0;
+    integ = integ && need != 0;
     if (__ballot(integ) == 0) return;
   }
-  bool bad = false;
+  unsigned bad = 0;  // the lines (bits as in need) whose recurrence anchors left the range
   const double dl = wla / (nl - 1);
   const double lamc = double(0) / (nl - 1) * wla - wla / 2.;  // lambdaval of the first pixel (:200)
-  double acc[CH];
+  double acc[NACC];
 #pragma unroll
-  for (int c = 0; c < CH; ++c) acc[c] = 0.0;
+  for (int c = 0; c < NACC; ++c) acc[c] = 0.0;
   int prev = -1;  // this lane's last resolved point: the warm start of the exact query
-  const int base = lane & ~(G - 1);
   // Three-stage pipeline over groups of G samples: while group kb is integrated, the record of
   // group kb + G (winner picked from its lookup cell record) and the lookup cell record of group
   // kb + 2G are in flight.
@@ -1606,66 +1679,115 @@ __global__ void __launch_bounds__(256) k_nn_los(const float4* __restrict__ pts, 
                                         cb * (double)fC[2])
                               : 0.f;
     const double st = rC >= 0 ? sqrt((double)fC[3]) : 0.0;
-#pragma unroll 1
-    for (int o = 0; o < G; ++o) {
-      if (kb + o >= nz) break;  // the same for every lane
-      const int src = base + o;
-      const float v = __shfl(vls, src, 64);
-      const double sto = __shfl(st, src, 64);
-      float pk = 0.f;
+    // whole waves skip groups without a data point
+    if (__ballot(rC >= 0) == 0) {
+    } else if constexpr (SL == 0) {
+      // the G samples of the group in LOS order, each with k_los_spectral's step for line q
+      static_for<G>([&](auto oc) {
+        constexpr int o = decltype(oc)::value;
+        const float v = group_bcast<G, o>(vls);
+        const double sto = group_bcast<G, o>(st);
+        float pk = 0.f;
 #pragma unroll
-      for (int l = 0; l < (G < NL ? G : NL); ++l) {
-        const float t = __shfl(fC[4 + l], src, 64);
-        if (l == q) pk = t;
-      }
-      if (!integ || pk == 0.f) continue;
-      // k_los_spectral's step for (peak, fwhm, vlos)
-      const double peak = pk, w = (double)(Kq > 0 ? (float)(Kq * sto) : 1.f);
-      const double mu = (double)v * l0c;
-      const double kap = fomo::spectral_kappa(w);
-      const double d0 = lamc - mu;
-      if (DIRECT) {
-#pragma unroll
-        for (int c = 0; c < CH; ++c) {
-          const double d = d0 + (double)c * dl;
-          acc[c] += peak * exp(-kap * d * d);
+        for (int l = 0; l < (G < NL ? G : NL); ++l) {
+          const float t = group_bcast<G, o>(fC[4 + l]);
+          if (l == q) pk = t;
         }
-        continue;
-      }
-      const double a0 = -kap * d0 * d0;
-      const double ar = -kap * (2.0 * d0 * dl + dl * dl);
-      const double aq = -2.0 * kap * dl * dl;
-      if (a0 > -600.0 && fabs(ar) < 100.0 && aq > -30.0) {
-        const double xa[3] = {aq, ar, a0};
-        double ex[3];
+        const bool on = integ && pk != 0.f && kb + o < nz;
+        const double peak = pk, w = (double)(Kq > 0 ? (float)(Kq * sto) : 1.f);
+        const double mu = (double)v * l0c;
+        const double kap = fomo::spectral_kappa(w);
+        const double d0 = lamc - mu;
+        if constexpr (DIRECT) {
+          if (on) {
+#pragma unroll
+            for (int c = 0; c < CH; ++c) {
+              const double d = d0 + (double)c * dl;
+              acc[c] += peak * exp(-kap * d * d);
+            }
+          }
+          return;
+        }
+        if (!on) return;
+        double xa[3];
+        if (!profile_args(kap, d0, dl, xa)) {
+          bad = 1u;
+          return;
+        }
+        double ex[3], gc[4], RR[4], q16;
         fomo::exp3(xa, ex);
-        const double qq = ex[0];
-        const double ra = ex[1];
-        double rr = ra;
-        const double G0 = peak * ex[2];
-        const double G1 = G0 * rr;
-        rr *= qq;
-        const double G2 = G1 * rr;
-        rr *= qq;
-        const double G3 = G2 * rr;
-        const double q2 = qq * qq, q4 = q2 * q2, q16 = (q4 * q4) * (q4 * q4);
-        const double ra2 = ra * ra;
-        const double R0 = (ra2 * ra2) * (q4 * q2);
-        const double R1 = R0 * q4, R2 = R1 * q4, R3 = R2 * q4;
-        double Gc[4] = {G0, G1, G2, G3};
-        double RR[4] = {R0, R1, R2, R3};
+        profile_chains(ex, gc, RR, q16);
 #pragma unroll
         for (int c0 = 0; c0 < CH; c0 += 4) {
 #pragma unroll
           for (int jj = 0; jj < 4; ++jj) {
-            acc[c0 + jj] += Gc[jj];
-            Gc[jj] *= RR[jj];
+            acc[c0 + jj] = fma(peak, gc[jj], acc[c0 + jj]);
+            gc[jj] *= RR[jj];
             RR[jj] *= q16;
           }
         }
-      } else {
-        bad = true;
+      });
+    } else {
+      // the shared profile at this lane's own sample: its anchors (ex, ok), or for the direct
+      // pass its (kap, d0)
+      const double w = (double)(Kq > 0 ? (float)(Kq * st) : 1.f);
+      const double mu = (double)vls * l0c;
+      const double kapS = fomo::spectral_kappa(w);
+      const double d0S = lamc - mu;
+      double exS[3] = {1.0, 1.0, 1.0};
+      int okS = 0;
+      if constexpr (!DIRECT) {
+        double xa[3];
+        okS = profile_args(kapS, d0S, dl, xa) ? 1 : 0;
+        if (!okS) xa[0] = xa[1] = xa[2] = 0.0;  // finite chains; nothing is added from them
+        fomo::exp3(xa, exS);
       }
+      // the 4 samples of the group in LOS order; lane q runs chain q of each
+      static_for<4>([&](auto oc) {
+        constexpr int o = decltype(oc)::value;
+        float pk[SL];
+        unsigned on = 0;
+#pragma unroll
+        for (int l = 0; l < SL; ++l) {
+          pk[l] = group_bcast<4, o>(fC[4 + l]);
+          if (pk[l] != 0.f && l < ls.n) on |= 1u << l;
+        }
+        if (!integ || kb + o >= nz) on = 0;
+        if constexpr (DIRECT) {
+          const double kap = group_bcast<4, o>(kapS), d0 = group_bcast<4, o>(d0S);
+          const unsigned use = on & need;
+          if (use == 0) return;
+#pragma unroll
+          for (int m = 0; m < CJ; ++m) {
+            const double d = d0 + (double)(4 * m + q) * dl;
+            const double e = exp(-kap * d * d);
+#pragma unroll
+            for (int l = 0; l < SL; ++l)
+              if (use & (1u << l)) acc[l * CJ + m] += (double)pk[l] * e;
+          }
+          return;
+        }
+        if (group_bcast<4, o>(okS) == 0) {
+          bad |= on;
+          return;
+        }
+        if (on == 0) return;
+        const double ex[3] = {group_bcast<4, o>(exS[0]), group_bcast<4, o>(exS[1]), group_bcast<4, o>(exS[2])};
+        double gc[4], RR[4], q16;
+        profile_chains(ex, gc, RR, q16);
+        double gq = gc[0], Rq = RR[0];
+#pragma unroll
+        for (int c = 1; c < 4; ++c)
+          if (q == c) gq = gc[c], Rq = RR[c];
+        // a line with peak 0 adds fma(0, g, sum) = sum: g is finite here
+#pragma unroll
+        for (int m = 0; m < CJ; ++m) {
+#pragma unroll
+          for (int l = 0; l < SL; ++l) acc[l * CJ + m] = fma((double)pk[l], gq, acc[l * CJ + m]);
+          gq *= Rq;
+          Rq *= q16;
+        }
+      });
     }
     // advance the pipeline
     rC = rB;
@@ -1678,15 +1800,31 @@ __global__ void __launch_bounds__(256) k_nn_los(const float4* __restrict__ pts, 
     cellN = cellA;
   }
   if (!integ) return;
-  if (!DIRECT) {
-    *flag = bad ? 1 : 0;
-    if (bad) return;
-  }
   const double sc = vg.scale;
-  float* o = out + (size_t)ray * nl;
+  if constexpr (SL > 0) {
+    if (!DIRECT && q == 0) {
 #pragma unroll
-  for (int c = 0; c < CH; ++c)
-    if (c < nl) o[c] = (float)(sc * acc[c]);
+      for (int l = 0; l < SL; ++l)
+        if (l < ls.n) flag[(size_t)l * R] = (bad >> l) & 1u;
+    }
+#pragma unroll
+    for (int l = 0; l < SL; ++l) {
+      if (l >= ls.n || (DIRECT ? !(need & (1u << l)) : (bad & (1u << l)) != 0)) continue;
+      float* o = ls.out[l] + (size_t)ray * nl;
+#pragma unroll
+      for (int m = 0; m < CJ; ++m)
+        if (4 * m + q < nl) o[4 * m + q] = (float)(sc * acc[l * CJ + m]);
+    }
+  } else {
+    if (!DIRECT) {
+      *flag = bad ? 1 : 0;
+      if (bad) return;
+    }
+    float* o = out + (size_t)ray * nl;
+#pragma unroll
+    for (int c = 0; c < CH; ++c)
+      if (c < nl) o[c] = (float)(sc * acc[c]);
+  }
 }
 
 // nl == 1 (Imaging): intens += peak (fomo-nearestneighbour.cpp:217-225)
@@ -2469,16 +2607,16 @@ void enqueue_los(hipStream_t s, int R, int nz, const LineSet& ls, const float* v
 // Fused NN + integration pass (k_nn_los) for spectral views with one lambda pass (2 <= nl <= 64)
 // and per-point records. Returns false when it does not apply (the caller runs enqueue_nn +
 // enqueue_los instead).
-template <int RS4, int G>
+template <int RS4, int G, int SL>
 void launch_nn_los_t(hipStream_t s, int ch, unsigned blocks, const Dataset& ds, const ViewGeom* geom, int nx, int ny,
                      int nz, const float4* rec, const RecInfo& ri, const LineSet& ls, int nl, int32_t* flags) {
   switch (ch) {
 #define FOMO_NNL_CH(C)                                                                                        \
   case C:                                                                                                    \
-    k_nn_los<RS4, C, G, false><<<blocks, 256, 0, s>>>(                                                       \
+    k_nn_los<RS4, C, G, false, SL><<<blocks, 256, 0, s>>>(                                                   \
         ds.pts.as<float4>(), ds.grid, ds.lslot.as<float4>(), ds.lcand.as<float4>(), ds.lk, ds.mpts.as<float4>(), \
         ds.nlo.as<float4>(), ds.nhi.as<float4>(), ds.bvh, geom, nx, ny, nz, rec, ri, ls, nl, flags);           \
-    k_nn_los<RS4, C, G, true><<<blocks, 256, 0, s>>>(                                                        \
+    k_nn_los<RS4, C, G, true, SL><<<blocks, 256, 0, s>>>(                                                    \
         ds.pts.as<float4>(), ds.grid, ds.lslot.as<float4>(), ds.lcand.as<float4>(), ds.lk, ds.mpts.as<float4>(), \
         ds.nlo.as<float4>(), ds.nhi.as<float4>(), ds.bvh, geom, nx, ny, nz, rec, ri, ls, nl, flags);           \
     break;
@@ -2490,6 +2628,15 @@ void launch_nn_los_t(hipStream_t s, int ch, unsigned blocks, const Dataset& ds, 
   }
 }
 
+// True when the lines of ls have one line profile: the same lambda window and rest wavelength
+// (so the same Doppler shift) and the same thermal-width factor K (fwhm = K sqrt T), bit for bit.
+// Their Gaussians then differ only in the peak.
+bool shared_profile(const RecInfo& ri, const LineSet& ls) {
+  for (int l = 1; l < ls.n; ++l)
+    if (ri.K[l] != ri.K[0] || ls.wla[l] != ls.wla[0] || ls.l0c[l] != ls.l0c[0]) return false;
+  return true;
+}
+
 // flags: R * ls.n ints of scratch.
 bool enqueue_nn_los(hipStream_t s, const Dataset& ds, const ViewGeom* geom, int nx, int ny, int nz,
                     const float4* rec, int rs4, const RecInfo& ri, const LineSet& ls, int nl, int32_t* flags) {
@@ -2499,15 +2646,17 @@ bool enqueue_nn_los(hipStream_t s, const Dataset& ds, const ViewGeom* geom, int 
   int npass = 1;
   const int ch = pick_chunk(nl, &npass);
   if (npass != 1) return false;
-  const int G = ls.n <= 2 ? 2 : (ls.n <= 4 ? 4 : 8);
+  const bool shared = rs4 == 2 && ls.n <= 4 && shared_profile(ri, ls);
+  const int G = shared ? 4 : (ls.n <= 2 ? 2 : (ls.n <= 4 ? 4 : 8));
   if ((rs4 == 2 && G > 4) || (rs4 != 2 && rs4 != 4)) return false;
   const int64_t R = (int64_t)nx * ny;
   const int64_t rays_per_block = 4 * (64 / G);
   const unsigned blocks = (unsigned)((R + rays_per_block - 1) / rays_per_block);
   ProfScope ps("k_nn_los", s);
-  if (rs4 == 2 && G == 2) launch_nn_los_t<2, 2>(s, ch, blocks, ds, geom, nx, ny, nz, rec, ri, ls, nl, flags);
-  else if (rs4 == 2) launch_nn_los_t<2, 4>(s, ch, blocks, ds, geom, nx, ny, nz, rec, ri, ls, nl, flags);
-  else launch_nn_los_t<4, 8>(s, ch, blocks, ds, geom, nx, ny, nz, rec, ri, ls, nl, flags);
+  if (shared) launch_nn_los_t<2, 4, 4>(s, ch, blocks, ds, geom, nx, ny, nz, rec, ri, ls, nl, flags);
+  else if (rs4 == 2 && G == 2) launch_nn_los_t<2, 2, 0>(s, ch, blocks, ds, geom, nx, ny, nz, rec, ri, ls, nl, flags);
+  else if (rs4 == 2) launch_nn_los_t<2, 4, 0>(s, ch, blocks, ds, geom, nx, ny, nz, rec, ri, ls, nl, flags);
+  else launch_nn_los_t<4, 8, 0>(s, ch, blocks, ds, geom, nx, ny, nz, rec, ri, ls, nl, flags);
   return true;
 }
 

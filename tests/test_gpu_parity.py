@@ -209,6 +209,74 @@ def test_fused_batch_lines_equal_plain(engine, fe9_table, tmp_path, nlines, nl):
         o.close()
 
 
+@pytest.mark.parametrize("nlines,nl", [(2, 60), (3, 17), (4, 60), (4, 64), (4, 5)])
+def test_shared_profile_batch_equals_plain(engine, fe9_table, tmp_path, nlines, nl):
+    """Lines with one lambda0 and atomic weight (G(T) tables peaked at different temperatures, as
+    in a multi-table batch of one line) share their Gaussian profile: the fused pass evaluates it
+    once per sample for all lines, one stride-4 chain per lane. Each line must equal its
+    single-line render bit for bit, and match the FP64 oracle."""
+    F = engine
+    path, tab = fe9_table
+    d = F.gen_loop_cube(20, seed=12, jitter=0.2)
+    ds = _dataset(F, d)
+    peaks = [5.8, 6.2, 6.4, 6.9][:nlines]
+    tabs = [F.Table.from_file(F.gen_goft_table(str(tmp_path / f"s{i}.dat"), peak_logt=p)) for i, p in enumerate(peaks)]
+    ems = [F.Emission.chianti(ds, t, tab["lambda0"], tab["atweight"]) for t in tabs]
+    ls, bs = [0.3, 2.1, 1.0], [1.1, np.pi / 2, np.pi / 3]
+    nx, ny, nz = 26, 22, 35
+    width = 0.2 * C / tab["lambda0"]
+    bt = F.Batch(ems, ls, bs, nx, ny, nz, nl, width)
+    bt.run()
+    out, meta = bt.read()
+    for v in range(len(ls)):
+        for g, e in enumerate(ems):
+            img, m = F.render_nn(e, ls[v], bs[v], nx, ny, nz, nl, width)
+            assert np.count_nonzero(img) > 0
+            assert np.array_equal(out[v, g].ravel(), img), f"view {v} line {g}"
+            assert np.array_equal(meta[v], m)
+    last = O.read_table(str(tmp_path / f"s{nlines - 1}.dat"))
+    pk, fw = O.emission(1, d["rho"], d["T"], last)
+    ref, _, _, _ = O.render_nn(1, d["x"], d["y"], d["z"], pk, fw, d["vx"], d["vy"], d["vz"], ls[0], bs[0], nx, ny, nz,
+                               nl, width, tab["lambda0"])
+    assert O.rel_l2(out[0, nlines - 1].ravel(), ref) <= TOL_IMG
+    bt.close()
+    for o in ems + tabs + [ds]:
+        o.close()
+
+
+def test_shared_profile_direct_pass(engine, fe9_table, tmp_path):
+    """A shared profile much narrower than a lambda pixel: every flagged (ray, line) is redone by
+    the shared-profile DIRECT instantiation; lines still equal their single-line renders bit for
+    bit and match the FP64 oracle."""
+    F = engine
+    path, tab = fe9_table
+    d = F.gen_loop_cube(16, seed=9)
+    ds = _dataset(F, d)
+    tabs = [F.Table.from_file(path), F.Table.from_file(F.gen_goft_table(str(tmp_path / "n.dat"), peak_logt=6.3)),
+            F.Table.from_file(path)]
+    heavy = 5.6e5  # thermal width / 100
+    ems = [F.Emission.chianti(ds, t, tab["lambda0"], heavy) for t in tabs]
+    width = 0.2 * C / tab["lambda0"]
+    ls, bs = [0.3, 1.9], [0.8, np.pi / 2]
+    bt = F.Batch(ems, ls, bs, 20, 20, 20, 40, width)
+    bt.run()
+    out, _ = bt.read()
+    for v in range(2):
+        for g, e in enumerate(ems):
+            img, _ = F.render_nn(e, ls[v], bs[v], 20, 20, 20, 40, width)
+            assert np.array_equal(out[v, g].ravel(), img), f"view {v} line {g}"
+    assert np.array_equal(out[:, 0], out[:, 2])
+    hv = dict(tab)
+    hv["atweight"] = heavy
+    pk, fw = O.emission(1, d["rho"], d["T"], hv)
+    ref, _, _, _ = O.render_nn(1, d["x"], d["y"], d["z"], pk, fw, d["vx"], d["vy"], d["vz"], ls[1], bs[1], 20, 20, 20,
+                               40, width, tab["lambda0"])
+    assert O.rel_l2(out[1, 0].ravel(), ref) <= TOL_IMG
+    bt.close()
+    for o in ems + tabs + [ds]:
+        o.close()
+
+
 @pytest.mark.parametrize("nl", [1, 100])
 def test_unfused_batch_equals_plain(engine, fe9_table, tmp_path, nl):
     """Batches the fused pass does not take (imaging, or more than 64 lambda pixels: two lambda
