@@ -705,11 +705,13 @@ __global__ void k_lookup_headers(const int32_t* __restrict__ cnt, const int64_t*
   }
 }
 
-// pos = the overflow list length of each cell (cells with more than kSlots candidates)
+// pos = the room of each cell's overflow list (cells with more than kSlots candidates): its
+// length rounded up to whole 128-byte lines (8 candidates), so that a list of up to 8 candidates
+// is one line fetch (an L2 miss fetches an aligned 128-byte line)
 __global__ void k_overflow_counts(const int32_t* __restrict__ cnt, int64_t ncl, int32_t* __restrict__ pos) {
   const int64_t L = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (L >= ncl) return;
-  pos[L] = cnt[L] > kSlots ? cnt[L] : 0;
+  pos[L] = cnt[L] > kSlots ? (cnt[L] + 7) & ~7 : 0;
 }
 
 __global__ void k_count_negative(const int32_t* __restrict__ cnt, int64_t ncl, int* __restrict__ out) {
@@ -774,9 +776,18 @@ __device__ __forceinline__ int lookup_pick(const float4* __restrict__ pts, const
     for (int q = 0; q < kSlots; ++q)
       if (__float_as_int(sl[q].w) >= 0) consider(sl[q]);
   } else {
+    // four candidates per step: their loads are in flight together
     const float4* ov = lcand + __float_as_int(sl[0].x);
     const int n = __float_as_int(sl[0].y);
-    for (int i = 0; i < n; ++i) consider(ov[i]);
+    for (int i = 0; i < n; i += 4) {
+      float4 c4[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+        if (i + u < n) c4[u] = ov[i + u];
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+        if (i + u < n) consider(c4[u]);
+    }
   }
   if (lo0 < bq.x && bq.x < hi0 && lo1 < bq.y && bq.y < hi1 && lo2 < bq.z && bq.z < hi2) return bpos;
   return -2;
