@@ -125,9 +125,9 @@ struct GridInfo {
 // Candidate lookup grid: a finer grid over the data bbox whose cells list the only points that
 // can be the (unconstrained) nearest point of any target inside the cell.
 struct LookupInfo {
-  double o[3];      // origin (shifted off the cell-list lattice)
-  double h, invh;   // lookup cell edge
-  double eps;       // box expansion covering the rounding of a target's cell assignment
+  double o[3];               // origin (shifted off the cell-list lattice)
+  double h[3], invh[3];      // lookup cell edges (the x edge is up to twice the others)
+  double eps;                // box expansion covering the rounding of a target's cell assignment
   int dims[3];
   int enabled;
 };
@@ -528,12 +528,14 @@ __global__ void k_view_finalize(const double* __restrict__ partial, int np, int 
 // queries use the exact ring search.
 // ------------------------------------------------------------------------------------
 constexpr int kLookupCap = 24;
-// Lookup cell record: kSlots float4 (64 bytes) holding up to kSlots candidates (x, y, z, sorted
-// position as int bits; unused slots have w = -1). Slot 0 with w = -1: no candidates (the query
-// falls back to the exact search); w = -2: more than kSlots candidates, at lcand[x] .. (y of them,
-// x and y as int bits). One 64-byte record per query instead of a (offset, count) pair plus a
-// separate candidate list.
+// Lookup cell record: kRecSlots float4 (128 bytes, one L2 line: an L2 miss fetches an aligned
+// 128-byte line) holding up to kRecSlots candidates (x, y, z, sorted position as int bits; unused
+// slots have w = -1). Slot 0 with w = -1: no candidates (the query falls back to the exact
+// search); w = -2: more than kRecSlots candidates, at lcand[x] .. (y of them, x and y as int bits).
+// Queries load the first kSlots slots ahead of time and the rest, from the same line, only when
+// slot kSlots - 1 says that more follow (its w = -3 - sorted position).
 constexpr int kSlots = 4;
+constexpr int kRecSlots = 8;
 constexpr int kLookupMargin = 6;  // cell-list cells the lookup grid extends beyond the data bbox
 constexpr int kLookupRings = kLookupMargin + 2;
 
@@ -574,8 +576,8 @@ __global__ void k_lookup_build(const float4* __restrict__ pts, const int32_t* __
   double lo[3], hi[3], c[3], hw[3];
 #pragma unroll
   for (int a = 0; a < 3; ++a) {
-    lo[a] = lk.o[a] + (double)li[a] * lk.h - lk.eps;
-    hi[a] = lk.o[a] + (double)(li[a] + 1) * lk.h + lk.eps;
+    lo[a] = lk.o[a] + (double)li[a] * lk.h[a] - lk.eps;
+    hi[a] = lk.o[a] + (double)(li[a] + 1) * lk.h[a] + lk.eps;
     c[a] = 0.5 * (lo[a] + hi[a]);
     hw[a] = 0.5 * (hi[a] - lo[a]);
   }
@@ -633,7 +635,7 @@ __global__ void k_lookup_build(const float4* __restrict__ pts, const int32_t* __
     if (!FILL) cnt[L] = -1;
     return;
   }
-  const double Up = sqrt(U2) * (1.0 + 1e-9) + 1e-9 * lk.h;
+  const double Up = sqrt(U2) * (1.0 + 1e-9) + 1e-9 * lk.h[0];
   const double Up2 = Up * Up;
   int r0[3], r1[3];
 #pragma unroll
@@ -647,7 +649,7 @@ __global__ void k_lookup_build(const float4* __restrict__ pts, const int32_t* __
   }
   int n = 0;
   // small lists go to the cell's own record, longer ones to the overflow list
-  float4* const dst = FILL ? (cnt[L] <= kSlots ? slot + (int64_t)kSlots * L : cand + off[L]) : nullptr;
+  float4* const dst = FILL ? (cnt[L] <= kRecSlots ? slot + (int64_t)kRecSlots * L : cand + off[L]) : nullptr;
   for (int z = r0[2]; z <= r1[2]; ++z)
     for (int y = r0[1]; y <= r1[1]; ++y) {
       const int row = z * dxy + y * dx;
@@ -658,7 +660,12 @@ __global__ void k_lookup_build(const float4* __restrict__ pts, const int32_t* __
         for (int k = 0; k < nd && !dm; ++k)
           if (dpos[k] != p && dominated(c, hw, q, dom[k])) dm = true;
         if (dm) continue;
-        if (FILL) dst[n] = make_float4(q.x, q.y, q.z, __int_as_float(p));
+        // in a record with more than kSlots candidates, slot kSlots - 1 says that more follow
+        // (w = -3 - position)
+        if (FILL) {
+          const bool more = n == kSlots - 1 && cnt[L] > kSlots && cnt[L] <= kRecSlots;
+          dst[n] = make_float4(q.x, q.y, q.z, __int_as_float(more ? -3 - p : p));
+        }
         ++n;
         if (!FILL && n > kLookupCap) {
           cnt[L] = -1;
@@ -694,24 +701,24 @@ __global__ void k_lookup_headers(const int32_t* __restrict__ cnt, const int64_t*
   const int64_t L = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (L >= ncl) return;
   const int c = cnt[L];
-  float4* r = slot + (int64_t)kSlots * L;
+  float4* r = slot + (int64_t)kRecSlots * L;
   const float4 unused = make_float4(0.f, 0.f, 0.f, __int_as_float(-1));
   if (c <= 0) {
     r[0] = unused;
-  } else if (c > kSlots) {
+  } else if (c > kRecSlots) {
     r[0] = make_float4(__int_as_float((int)off[L]), __int_as_float(c), 0.f, __int_as_float(-2));
   } else {
-    for (int q = c; q < kSlots; ++q) r[q] = unused;
+    for (int q = c; q < kRecSlots; ++q) r[q] = unused;
   }
 }
 
-// pos = the room of each cell's overflow list (cells with more than kSlots candidates): its
+// pos = the room of each cell's overflow list (cells with more than kRecSlots candidates): its
 // length rounded up to whole 128-byte lines (8 candidates), so that a list of up to 8 candidates
 // is one line fetch (an L2 miss fetches an aligned 128-byte line)
 __global__ void k_overflow_counts(const int32_t* __restrict__ cnt, int64_t ncl, int32_t* __restrict__ pos) {
   const int64_t L = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (L >= ncl) return;
-  pos[L] = cnt[L] > kSlots ? (cnt[L] + 7) & ~7 : 0;
+  pos[L] = cnt[L] > kRecSlots ? (cnt[L] + 7) & ~7 : 0;
 }
 
 __global__ void k_count_negative(const int32_t* __restrict__ cnt, int64_t ncl, int* __restrict__ out) {
@@ -736,9 +743,9 @@ __global__ void k_clamp_counts(const int32_t* __restrict__ cnt, int64_t ncl, int
 
 // Lookup cell of a target (-1 outside the lookup grid).
 __device__ __forceinline__ int64_t lookup_cell(const LookupInfo& lk, float t0, float t1, float t2) {
-  const double v0 = ((double)t0 - lk.o[0]) * lk.invh;
-  const double v1 = ((double)t1 - lk.o[1]) * lk.invh;
-  const double v2 = ((double)t2 - lk.o[2]) * lk.invh;
+  const double v0 = ((double)t0 - lk.o[0]) * lk.invh[0];
+  const double v1 = ((double)t1 - lk.o[1]) * lk.invh[1];
+  const double v2 = ((double)t2 - lk.o[2]) * lk.invh[2];
   if (!(v0 >= 0.0 && v0 < (double)lk.dims[0] && v1 >= 0.0 && v1 < (double)lk.dims[1] && v2 >= 0.0 &&
         v2 < (double)lk.dims[2]))
     return -1;
@@ -748,7 +755,8 @@ __device__ __forceinline__ int64_t lookup_cell(const LookupInfo& lk, float t0, f
 // The nearest candidate of a lookup cell record sl (read from the overflow list for long lists),
 // if it lies strictly inside the search box; else -2.
 __device__ __forceinline__ int lookup_pick(const float4* __restrict__ pts, const float4* __restrict__ lcand,
-                                           const float4 (&sl)[kSlots], float t0, float t1, float t2, float lo0,
+                                           const float4* __restrict__ rp, const float4 (&sl)[kSlots], float t0,
+                                           float t1, float t2, float lo0,
                                            float lo1, float lo2, float hi0, float hi1, float hi2) {
   const int w0 = __float_as_int(sl[0].w);
   if (w0 == -1) return -2;
@@ -772,9 +780,21 @@ __device__ __forceinline__ int lookup_pick(const float4* __restrict__ pts, const
     }
   };
   if (w0 != -2) {
+    const int wl = __float_as_int(sl[kSlots - 1].w);
 #pragma unroll
-    for (int q = 0; q < kSlots; ++q)
+    for (int q = 0; q < kSlots - 1; ++q)
       if (__float_as_int(sl[q].w) >= 0) consider(sl[q]);
+    if (wl >= 0) consider(sl[kSlots - 1]);
+    if (wl <= -3) {
+      consider(make_float4(sl[kSlots - 1].x, sl[kSlots - 1].y, sl[kSlots - 1].z, __int_as_float(-3 - wl)));
+      // the rest of the record: the same 128-byte line as sl
+      float4 s2[kRecSlots - kSlots];
+#pragma unroll
+      for (int q = 0; q < kRecSlots - kSlots; ++q) s2[q] = rp[kSlots + q];
+#pragma unroll
+      for (int q = 0; q < kRecSlots - kSlots; ++q)
+        if (__float_as_int(s2[q].w) >= 0) consider(s2[q]);
+    }
   } else {
     // four candidates per step: their loads are in flight together
     const float4* ov = lcand + __float_as_int(sl[0].x);
@@ -800,12 +820,12 @@ __device__ __forceinline__ int nn_lookup(const float4* __restrict__ pts, const f
                                          float t2, float lo0, float lo1, float lo2, float hi0, float hi1, float hi2) {
   const int64_t L = lookup_cell(lk, t0, t1, t2);
   if (L < 0) return -2;
-  // the whole 64-byte record at once (one sector); the overflow list only for long lists
-  const float4* rp = lslot + (int64_t)kSlots * L;
+  // the first half of the 128-byte record; the rest and the overflow list only when taken
+  const float4* rp = lslot + (int64_t)kRecSlots * L;
   float4 sl[kSlots];
 #pragma unroll
   for (int q = 0; q < kSlots; ++q) sl[q] = rp[q];
-  return lookup_pick(pts, lcand, sl, t0, t1, t2, lo0, lo1, lo2, hi0, hi1, hi2);
+  return lookup_pick(pts, lcand, rp, sl, t0, t1, t2, lo0, lo1, lo2, hi0, hi1, hi2);
 }
 
 // ------------------------------------------------------------------------------------
@@ -1610,7 +1630,8 @@ __global__ void __launch_bounds__(256) k_nn_los(const float4* __restrict__ pts, 
   // group kb + G (winner picked from its lookup cell record) and the lookup cell record of group
   // kb + 2G are in flight.
   // stage 1: target, search box and lookup cell record of this lane's sample k
-  const auto query_issue = [&](int k, SearchBox& box, float4 (&sl)[kSlots], bool& act, bool& cell) {
+  const auto query_issue = [&](int k, SearchBox& box, const float4*& rp, float4 (&sl)[kSlots], bool& act,
+                               bool& cell) {
     act = live && k < nz;
     cell = false;
     if (!act) return;
@@ -1625,14 +1646,15 @@ __global__ void __launch_bounds__(256) k_nn_los(const float4* __restrict__ pts, 
     const int64_t L = lookup_cell(lk, box.t0, box.t1, box.t2);
     if (L < 0) return;
     cell = true;
-    const float4* rp = lslot + (int64_t)kSlots * L;
+    rp = lslot + (int64_t)kRecSlots * L;
 #pragma unroll
     for (int c = 0; c < kSlots; ++c) sl[c] = rp[c];
   };
   // stage 2: the winner (lookup, else the exact octree query), -1 for none
-  const auto query_resolve = [&](const SearchBox& box, const float4 (&sl)[kSlots], bool act, bool cell) -> int {
+  const auto query_resolve = [&](const SearchBox& box, const float4* rp, const float4 (&sl)[kSlots], bool act,
+                                 bool cell) -> int {
     if (!act) return -1;
-    int r = cell ? lookup_pick(pts, lcand, sl, box.t0, box.t1, box.t2, box.lo0, box.lo1, box.lo2, box.hi0,
+    int r = cell ? lookup_pick(pts, lcand, rp, sl, box.t0, box.t1, box.t2, box.lo0, box.lo1, box.lo2, box.hi0,
                                box.hi1, box.hi2)
                  : -2;
     if (r == -2) {
@@ -1670,19 +1692,21 @@ __global__ void __launch_bounds__(256) k_nn_los(const float4* __restrict__ pts, 
     }
   };
   SearchBox boxN;
+  const float4* rpN = lslot;
   float4 slN[kSlots];
   bool actN, cellN;
   float fC[4 * RS4];
-  query_issue(q, boxN, slN, actN, cellN);
-  int rC = query_resolve(boxN, slN, actN, cellN);
+  query_issue(q, boxN, rpN, slN, actN, cellN);
+  int rC = query_resolve(boxN, rpN, slN, actN, cellN);
   record_issue(rC, fC);
-  query_issue(G + q, boxN, slN, actN, cellN);
+  query_issue(G + q, boxN, rpN, slN, actN, cellN);
   for (int kb = 0; kb < nz; kb += G) {
     SearchBox boxA;
+    const float4* rpA = lslot;
     float4 slA[kSlots];
     bool actA, cellA;
-    query_issue(kb + 2 * G + q, boxA, slA, actA, cellA);
-    const int rB = query_resolve(boxN, slN, actN, cellN);
+    query_issue(kb + 2 * G + q, boxA, rpA, slA, actA, cellA);
+    const int rB = query_resolve(boxN, rpN, slN, actN, cellN);
     float fB[4 * RS4];
     record_issue(rB, fB);
     // group kb: this lane's sample, the values write_sample would store
@@ -1805,6 +1829,7 @@ __global__ void __launch_bounds__(256) k_nn_los(const float4* __restrict__ pts, 
 #pragma unroll
     for (int c = 0; c < 4 * RS4; ++c) fC[c] = fB[c];
     boxN = boxA;
+    rpN = rpA;
 #pragma unroll
     for (int c = 0; c < kSlots; ++c) slN[c] = slA[c];
     actN = actA;
@@ -1889,7 +1914,7 @@ struct Dataset {
   LookupInfo lk{};     // candidate lookup grid (lk.enabled==0: ring search only)
   int lookup_subdiv = 2;         // requested subdivision
   int lookup_active_subdiv = 0;  // subdivision actually built (0: none)
-  DevBuf lslot;        // kSlots float4 per lookup cell: inline candidates or an overflow header
+  DevBuf lslot;        // kRecSlots float4 per lookup cell: inline candidates or an overflow header
   DevBuf lcand;        // float4 overflow candidates (x, y, z, sorted position as int bits)
   BvhInfo bvh{};       // implicit BVH over Morton-ordered points (exact NN fallback)
   DevBuf mpts;         // float4 (x, y, z, sorted position) in Morton order
@@ -2220,34 +2245,41 @@ int build_lookup(Dataset& ds) {
   if (ds.lookup_subdiv <= 0) return FOMO_OK;
   hipStream_t s = g_ctx.stream;
   const GridInfo& g = ds.grid;
-  // subdivision: the requested one, reduced until the grid fits ~24 cells per point and 2^30
+  // subdivision: the requested one, reduced until the grid fits ~24 cells per point and 2^30.
+  // Cell edges: g.h / subdiv, and along x twice that (as long as it stays within g.h), so that a
+  // 128-byte record (8 slots) holds the candidates of structured data without overflow.
+  const auto edges = [&](int sd, double (&h)[3]) {
+    for (int a = 0; a < 3; ++a) h[a] = g.h / sd;
+    h[0] = g.h / std::max(1, sd / 2);
+  };
   int subdiv = ds.lookup_subdiv;
   for (; subdiv >= 1; --subdiv) {
-    double ncl_est = 1;
+    double h[3], ncl_est = 1;
+    edges(subdiv, h);
     for (int a = 0; a < 3; ++a) {
       const double mrg = (g.dims[a] > 1 ? kLookupMargin : 0) * g.h;
-      ncl_est *= std::floor(((double)g.bmax[a] - g.o[a] + 2 * mrg + 0.3 * g.h / subdiv) / (g.h / subdiv)) + 1;
+      ncl_est *= std::floor(((double)g.bmax[a] - g.o[a] + 2 * mrg + 0.3 * h[a]) / h[a]) + 1;
     }
     if (ncl_est <= 24.0 * ds.n + 65536 && ncl_est <= (double)(1LL << 30)) break;
   }
   if (subdiv < 1) return FOMO_OK;
   LookupInfo lk{};
-  lk.h = g.h / subdiv;
-  lk.invh = 1.0 / lk.h;
+  edges(subdiv, lk.h);
   double ext = 0, mag = 0;
   int64_t ncl = 1;
   for (int a = 0; a < 3; ++a) {
+    lk.invh[a] = 1.0 / lk.h[a];
     const double mrg = (g.dims[a] > 1 ? kLookupMargin : 0) * g.h;
-    lk.o[a] = g.o[a] - mrg - 0.3 * lk.h;
+    lk.o[a] = g.o[a] - mrg - 0.3 * lk.h[a];
     const double e = (double)g.bmax[a] - g.o[a];
-    lk.dims[a] = (int)std::floor((e + 2 * mrg + 0.3 * lk.h) / lk.h) + 1;
+    lk.dims[a] = (int)std::floor((e + 2 * mrg + 0.3 * lk.h[a]) / lk.h[a]) + 1;
     ncl *= lk.dims[a];
     ext = std::max(ext, e);
     mag = std::max(mag, std::max(std::fabs((double)g.bmin[a]), std::fabs((double)g.bmax[a])));
   }
   // keep the lookup grid within ~24 cells per point and 2^30 cells
   if (ncl > 24 * ds.n + 65536 || ncl > (1LL << 30)) return FOMO_OK;
-  lk.eps = 1e-7 * lk.h + 1e-12 * (mag + ext);
+  lk.eps = 1e-7 * lk.h[0] + 1e-12 * (mag + ext);
   lk.enabled = 1;
   // cells of the cell list with a point within kLookupRings cells
   DevBuf near0, near1;
@@ -2287,7 +2319,7 @@ int build_lookup(Dataset& ds) {
   }
   if (total >= (int64_t)INT32_MAX) return FOMO_OK;
   if (ds.lcand.alloc(sizeof(float4) * std::max<int64_t>(total, 1)) ||
-      ds.lslot.alloc(sizeof(float4) * kSlots * ncl)) {
+      ds.lslot.alloc(sizeof(float4) * kRecSlots * ncl)) {
     ds.lslot.release();
     ds.lcand.release();
     return FOMO_OK;
