@@ -1543,7 +1543,7 @@ __device__ __forceinline__ double group_bcast(double x) {
 
 // ------------------------------------------------------------------------------------
 // Fused NN query + spectral integration (k_nn, k_nn_far and k_los_spectral in one pass, for
-// 2 <= nl <= 64 and several lines). G lanes per ray query G consecutive samples at once (lane q:
+// 2 <= nl <= 64: several lines, or one line on views of at least kFusedMinRays rays). G lanes per ray query G consecutive samples at once (lane q:
 // sample kb + q; lookup grid, then the exact octree query) and hand each sample's values to the
 // other lanes of the ray by cross-lane broadcasts, so the per-sample values never go through HBM
 // and the memory-bound queries of some waves overlap the FP64 integration of others. Two ways
@@ -1551,7 +1551,8 @@ __device__ __forceinline__ double group_bcast(double x) {
 //   SL == 0: lane q integrates line q (G = lines rounded up to 2, 4, 8) with the step of
 //            k_los_spectral, from the broadcast (LOS velocity, sqrt T, peak).
 //   SL > 0:  the lines share one profile (equal lambda0, lambda window and thermal-width factor,
-//            so their Gaussians differ only in the peak; lines beyond ls.n have peak 0). G = 4:
+//            so their Gaussians differ only in the peak; lines beyond ls.n have peak 0), or there
+//            is one line (SL = 1). G = 4:
 //            lane q evaluates the anchors of its own sample, the ray's 4 lanes exchange them,
 //            and lane q runs the profile's chain q (pixels q, q+4, ...) of every sample, adding
 //            peak_l * g to line l's sums. The profile is evaluated once for all lines.
@@ -2680,19 +2681,32 @@ bool shared_profile(const RecInfo& ri, const LineSet& ls) {
   return true;
 }
 
-// flags: R * ls.n ints of scratch.
+// Rays below which a single-line view runs k_nn + k_los_spectral rather than the fused pass.
+constexpr int64_t kFusedMinRays = 65536;
+
+// flags: R * ls.n ints of scratch. False when the view is left to k_nn + the integration kernels.
 bool enqueue_nn_los(hipStream_t s, const Dataset& ds, const ViewGeom* geom, int nx, int ny, int nz,
                     const float4* rec, int rs4, const RecInfo& ri, const LineSet& ls, int nl, int32_t* flags) {
-  // several lines share each query; a single line is faster as one thread per sample (k_nn) and
-  // one lane per ray (k_los_spectral), which add the samples in the same order
-  if (!rec || nl < 2 || nl > 64 || ls.n < 2 || ls.n > kMaxLines) return false;
+  if (!rec || nl < 2 || ls.n < 1 || ls.n > kMaxLines) return false;
+  const int64_t R = (int64_t)nx * ny;
+  // one lambda pass (k_los_spectral's chunking: the same anchors)
+  if (nl > 64) return false;
   int npass = 1;
   const int ch = pick_chunk(nl, &npass);
   if (npass != 1) return false;
+  if (ls.n == 1) {
+    // One line: the 4 lanes of a ray run the profile's 4 chains. Small images have too few rays
+    // to fill the GPU this way; they run k_nn (one thread per sample) + k_los_spectral, which
+    // add the same values in the same order.
+    if (R < kFusedMinRays || rs4 != 2) return false;
+    ProfScope ps("k_nn_los", s);
+    launch_nn_los_t<2, 4, 1>(s, ch, (unsigned)((R + 63) / 64), ds, geom, nx, ny, nz, rec, ri, ls, nl, flags);
+    return true;
+  }
+  // several lines share each query
   const bool shared = rs4 == 2 && ls.n <= 4 && shared_profile(ri, ls);
   const int G = shared ? 4 : (ls.n <= 2 ? 2 : (ls.n <= 4 ? 4 : 8));
   if ((rs4 == 2 && G > 4) || (rs4 != 2 && rs4 != 4)) return false;
-  const int64_t R = (int64_t)nx * ny;
   const int64_t rays_per_block = 4 * (64 / G);
   const unsigned blocks = (unsigned)((R + rays_per_block - 1) / rays_per_block);
   ProfScope ps("k_nn_los", s);

@@ -277,6 +277,43 @@ def test_shared_profile_direct_pass(engine, fe9_table, tmp_path):
         o.close()
 
 
+@pytest.mark.parametrize("atw,nl", [(56.0, 60), (56.0, 7), (5.6e5, 40)])
+def test_single_line_fused_view(engine, fe9_table, tmp_path, atw, nl):
+    """A single-line view of 256 x 256 rays (the fused pass's threshold) runs the fused pass with
+    one line: the 4 lanes of a ray run the profile's 4 chains. It must equal the same line rendered
+    in a batch with a second line (one line per lane, or the shared profile) bit for bit, and match
+    the FP64 oracle; atomic weight 5.6e5 (thermal width / 100) takes the direct-exp pass."""
+    F = engine
+    path, tab = fe9_table
+    d = F.gen_loop_cube(14, seed=21, jitter=0.2)
+    ds = _dataset(F, d)
+    T = F.Table.from_file(path)
+    T2 = F.Table.from_file(F.gen_goft_table(str(tmp_path / "o.dat"), lambda0=193.5, atweight=56.0, peak_logt=6.2))
+    em = F.Emission.chianti(ds, T, tab["lambda0"], atw)
+    em_same = F.Emission.chianti(ds, T, tab["lambda0"], atw)          # shared profile with em
+    em_other = F.Emission.chianti(ds, T2, 193.5, 56.0)                 # a different profile
+    width = 0.2 * C / tab["lambda0"]
+    l, b, nx, ny, nz = 0.6, 1.0, 256, 256, 12
+    img, meta = F.render_nn(em, l, b, nx, ny, nz, nl, width)
+    for pair in ([em, em_same], [em, em_other]):
+        bt = F.Batch(pair, [l], [b], nx, ny, nz, nl, width)
+        bt.run()
+        out, bmeta = bt.read()
+        assert np.array_equal(out[0, 0].ravel(), img)
+        assert np.array_equal(bmeta[0], meta)
+        bt.close()
+    hv = dict(tab)
+    hv["atweight"] = atw
+    pk, fw = O.emission(1, d["rho"], d["T"], hv)
+    ref, _, rmeta, _ = O.render_nn(1, d["x"], d["y"], d["z"], pk, fw, d["vx"], d["vy"], d["vz"], l, b, nx, ny, nz, nl,
+                                   width, tab["lambda0"])
+    assert np.array_equal(meta[:7], rmeta[:7])
+    assert np.count_nonzero(ref) > 0
+    assert O.rel_l2(img, ref) <= TOL_IMG
+    for o in (em, em_same, em_other, T, T2, ds):
+        o.close()
+
+
 @pytest.mark.parametrize("nl", [1, 100])
 def test_unfused_batch_equals_plain(engine, fe9_table, tmp_path, nl):
     """Batches the fused pass does not take (imaging, or more than 64 lambda pixels: two lambda
