@@ -1562,7 +1562,7 @@ __device__ __forceinline__ double group_bcast(double x) {
 // whose recurrence anchors would under- or overflow is flagged (flags[line * R + ray]) and redone
 // by the DIRECT instantiation, launched right after, with one exp per sample and lambda pixel.
 // ------------------------------------------------------------------------------------
-template <int RS4, int CH, int G, bool DIRECT, int SL>
+template <int RS4, int CH, int G, bool DIRECT, int SL, int NP = 1>
 __global__ void __launch_bounds__(256) k_nn_los(const float4* __restrict__ pts, GridInfo g,
                                                 const float4* __restrict__ lslot, const float4* __restrict__ lcand,
                                                 LookupInfo lk, const float4* __restrict__ mpts,
@@ -1573,8 +1573,10 @@ __global__ void __launch_bounds__(256) k_nn_los(const float4* __restrict__ pts, 
   constexpr int NL = 4 * RS4 - 4;  // lines a record holds
   constexpr int RPW = 64 / G;      // rays per wave
   static_assert(SL == 0 || (G == 4 && SL <= NL && CH % 4 == 0), "a shared profile runs 4 chains per ray");
+  static_assert(NP == 1 || (SL > 0 && NP * SL <= 8), "several lambda passes only with a shared profile");
   constexpr int CJ = CH / 4;                  // pixels of one chain (SL > 0)
-  constexpr int NACC = SL ? SL * CJ : CH;     // SL > 0: acc[l * CJ + m] is line l, pixel 4 m + q
+  // SL > 0: acc[(p * SL + l) * CJ + m] is pass p, line l, pixel p * CH + 4 m + q
+  constexpr int NACC = SL ? NP * SL * CJ : CH;
   const int lane = threadIdx.x & 63;
   const int q = lane & (G - 1);
   const int64_t R = (int64_t)nx * ny;
@@ -1604,14 +1606,15 @@ __global__ void __launch_bounds__(256) k_nn_los(const float4* __restrict__ pts, 
   }
   bool integ = live && (SL > 0 || q < ls.n);
   int32_t* const flag = flags + (size_t)(SL > 0 ? 0 : q) * R + (live ? ray : 0);
-  // lines of this lane with a flagged (ray, line): bit l (shared profile), bit 0 (line q)
+  // flagged (ray, line, lambda pass)s of this lane: bit l * NP + p (shared profile; flags[l * R +
+  // ray] holds the passes' bits), bit 0 (line q)
   unsigned need = 0;
   if (DIRECT) {
     if (integ) {
       if constexpr (SL > 0) {
 #pragma unroll
         for (int l = 0; l < SL; ++l)
-          if (l < ls.n && flag[(size_t)l * R] != 0) need |= 1u << l;
+          if (l < ls.n) need |= ((unsigned)flag[(size_t)l * R] & ((1u << NP) - 1)) << (l * NP);
       } else {
         need = *flag != 0 ? 1u : 0u;
       }
@@ -1620,9 +1623,11 @@ __global__ void __launch_bounds__(256) k_nn_los(const float4* __restrict__ pts, 
     integ = integ && need != 0;
     if (__ballot(integ) == 0) return;
   }
-  unsigned bad = 0;  // the lines (bits as in need) whose recurrence anchors left the range
+  unsigned bad = 0;  // bits as in need: recurrence anchors out of range
   const double dl = wla / (nl - 1);
   const double lamc = double(0) / (nl - 1) * wla - wla / 2.;  // lambdaval of the first pixel (:200)
+  // lambdaval of the first pixel of pass p (k_los_spectral's l0 = p * CH)
+  const auto lamc_of = [&](int p) { return double(p * CH) / (nl - 1) * wla - wla / 2.; };
   double acc[NACC];
 #pragma unroll
   for (int c = 0; c < NACC; ++c) acc[c] = 0.0;
@@ -1764,19 +1769,24 @@ __global__ void __launch_bounds__(256) k_nn_los(const float4* __restrict__ pts, 
         }
       });
     } else {
-      // the shared profile at this lane's own sample: its anchors (ex, ok), or for the direct
-      // pass its (kap, d0)
+      // the shared profile at this lane's own sample, for each lambda pass p: its anchors (ex, ok
+      // bit p), or for the direct pass its (kap, d0)
       const double w = (double)(Kq > 0 ? (float)(Kq * st) : 1.f);
       const double mu = (double)vls * l0c;
       const double kapS = fomo::spectral_kappa(w);
-      const double d0S = lamc - mu;
-      double exS[3] = {1.0, 1.0, 1.0};
+      double d0S[NP], exS[NP][3];
       int okS = 0;
-      if constexpr (!DIRECT) {
-        double xa[3];
-        okS = profile_args(kapS, d0S, dl, xa) ? 1 : 0;
-        if (!okS) xa[0] = xa[1] = xa[2] = 0.0;  // finite chains; nothing is added from them
-        fomo::exp3(xa, exS);
+#pragma unroll
+      for (int p = 0; p < NP; ++p) {
+        d0S[p] = lamc_of(p) - mu;
+        exS[p][0] = exS[p][1] = exS[p][2] = 1.0;
+        if constexpr (!DIRECT) {
+          double xa[3];
+          const bool ok = profile_args(kapS, d0S[p], dl, xa);
+          okS |= ok ? 1 << p : 0;
+          if (!ok) xa[0] = xa[1] = xa[2] = 0.0;  // finite chains; nothing is added from them
+          fomo::exp3(xa, exS[p]);
+        }
       }
       // the 4 samples of the group in LOS order; lane q runs chain q of each
       static_for<4>([&](auto oc) {
@@ -1790,38 +1800,52 @@ __global__ void __launch_bounds__(256) k_nn_los(const float4* __restrict__ pts, 
         }
         if (!integ || kb + o >= nz) on = 0;
         if constexpr (DIRECT) {
-          const double kap = group_bcast<4, o>(kapS), d0 = group_bcast<4, o>(d0S);
-          const unsigned use = on & need;
-          if (use == 0) return;
+          const double kap = group_bcast<4, o>(kapS);
 #pragma unroll
-          for (int m = 0; m < CJ; ++m) {
-            const double d = d0 + (double)(4 * m + q) * dl;
-            const double e = exp(-kap * d * d);
+          for (int p = 0; p < NP; ++p) {
+            const double d0 = group_bcast<4, o>(d0S[p]);
+            unsigned use = 0;
 #pragma unroll
             for (int l = 0; l < SL; ++l)
-              if (use & (1u << l)) acc[l * CJ + m] += (double)pk[l] * e;
+              if ((on >> l) & (need >> (l * NP + p)) & 1u) use |= 1u << l;
+            if (use == 0) continue;
+#pragma unroll
+            for (int m = 0; m < CJ; ++m) {
+              const double d = d0 + (double)(4 * m + q) * dl;
+              const double e = exp(-kap * d * d);
+#pragma unroll
+              for (int l = 0; l < SL; ++l)
+                if (use & (1u << l)) acc[(p * SL + l) * CJ + m] += (double)pk[l] * e;
+            }
           }
           return;
         }
-        if (group_bcast<4, o>(okS) == 0) {
-          bad |= on;
-          return;
-        }
-        if (on == 0) return;
-        const double ex[3] = {group_bcast<4, o>(exS[0]), group_bcast<4, o>(exS[1]), group_bcast<4, o>(exS[2])};
-        double gc[4], RR[4], q16;
-        profile_chains(ex, gc, RR, q16);
-        double gq = gc[0], Rq = RR[0];
+        const int okb = group_bcast<4, o>(okS);
 #pragma unroll
-        for (int c = 1; c < 4; ++c)
-          if (q == c) gq = gc[c], Rq = RR[c];
-        // a line with peak 0 adds fma(0, g, sum) = sum: g is finite here
+        for (int p = 0; p < NP; ++p) {
+          if (!((okb >> p) & 1)) {
 #pragma unroll
-        for (int m = 0; m < CJ; ++m) {
+            for (int l = 0; l < SL; ++l)
+              if (on & (1u << l)) bad |= 1u << (l * NP + p);
+            continue;
+          }
+          if (on == 0) continue;
+          const double ex[3] = {group_bcast<4, o>(exS[p][0]), group_bcast<4, o>(exS[p][1]),
+                                group_bcast<4, o>(exS[p][2])};
+          double gc[4], RR[4], q16;
+          profile_chains(ex, gc, RR, q16);
+          double gq = gc[0], Rq = RR[0];
 #pragma unroll
-          for (int l = 0; l < SL; ++l) acc[l * CJ + m] = fma((double)pk[l], gq, acc[l * CJ + m]);
-          gq *= Rq;
-          Rq *= q16;
+          for (int c = 1; c < 4; ++c)
+            if (q == c) gq = gc[c], Rq = RR[c];
+          // a line with peak 0 adds fma(0, g, sum) = sum: g is finite here
+#pragma unroll
+          for (int m = 0; m < CJ; ++m) {
+#pragma unroll
+            for (int l = 0; l < SL; ++l) acc[(p * SL + l) * CJ + m] = fma((double)pk[l], gq, acc[(p * SL + l) * CJ + m]);
+            gq *= Rq;
+            Rq *= q16;
+          }
         }
       });
     }
@@ -1842,16 +1866,19 @@ __global__ void __launch_bounds__(256) k_nn_los(const float4* __restrict__ pts, 
     if (!DIRECT && q == 0) {
 #pragma unroll
       for (int l = 0; l < SL; ++l)
-        if (l < ls.n) flag[(size_t)l * R] = (bad >> l) & 1u;
+        if (l < ls.n) flag[(size_t)l * R] = (int)((bad >> (l * NP)) & ((1u << NP) - 1));
     }
 #pragma unroll
-    for (int l = 0; l < SL; ++l) {
-      if (l >= ls.n || (DIRECT ? !(need & (1u << l)) : (bad & (1u << l)) != 0)) continue;
-      float* o = ls.out[l] + (size_t)ray * nl;
+    for (int p = 0; p < NP; ++p)
 #pragma unroll
-      for (int m = 0; m < CJ; ++m)
-        if (4 * m + q < nl) o[4 * m + q] = (float)(sc * acc[l * CJ + m]);
-    }
+      for (int l = 0; l < SL; ++l) {
+        const unsigned bit = 1u << (l * NP + p);
+        if (l >= ls.n || (DIRECT ? !(need & bit) : (bad & bit) != 0)) continue;
+        float* o = ls.out[l] + (size_t)ray * nl + p * CH;
+#pragma unroll
+        for (int m = 0; m < CJ; ++m)
+          if (p * CH + 4 * m + q < nl) o[4 * m + q] = (float)(sc * acc[(p * SL + l) * CJ + m]);
+      }
   } else {
     if (!DIRECT) {
       *flag = bad ? 1 : 0;
@@ -2672,6 +2699,27 @@ void launch_nn_los_t(hipStream_t s, int ch, unsigned blocks, const Dataset& ds, 
   }
 }
 
+// One line in two lambda passes (65 <= nl <= 128), both in one launch: CH = k_los_spectral's chunk.
+void launch_nn_los_2pass(hipStream_t s, int ch, unsigned blocks, const Dataset& ds, const ViewGeom* geom, int nx,
+                         int ny, int nz, const float4* rec, const RecInfo& ri, const LineSet& ls, int nl,
+                         int32_t* flags) {
+  switch (ch) {
+#define FOMO_NNL2_CH(C)                                                                                       \
+  case C:                                                                                                    \
+    k_nn_los<2, C, 4, false, 1, 2><<<blocks, 256, 0, s>>>(                                                   \
+        ds.pts.as<float4>(), ds.grid, ds.lslot.as<float4>(), ds.lcand.as<float4>(), ds.lk, ds.mpts.as<float4>(), \
+        ds.nlo.as<float4>(), ds.nhi.as<float4>(), ds.bvh, geom, nx, ny, nz, rec, ri, ls, nl, flags);           \
+    k_nn_los<2, C, 4, true, 1, 2><<<blocks, 256, 0, s>>>(                                                    \
+        ds.pts.as<float4>(), ds.grid, ds.lslot.as<float4>(), ds.lcand.as<float4>(), ds.lk, ds.mpts.as<float4>(), \
+        ds.nlo.as<float4>(), ds.nhi.as<float4>(), ds.bvh, geom, nx, ny, nz, rec, ri, ls, nl, flags);           \
+    break;
+    FOMO_NNL2_CH(36) FOMO_NNL2_CH(40) FOMO_NNL2_CH(44) FOMO_NNL2_CH(48) FOMO_NNL2_CH(52) FOMO_NNL2_CH(56)
+    FOMO_NNL2_CH(60) FOMO_NNL2_CH(64)
+#undef FOMO_NNL2_CH
+    default: break;
+  }
+}
+
 // True when the lines of ls have one line profile: the same lambda window and rest wavelength
 // (so the same Doppler shift) and the same thermal-width factor K (fwhm = K sqrt T), bit for bit.
 // Their Gaussians then differ only in the peak.
@@ -2689,20 +2737,21 @@ bool enqueue_nn_los(hipStream_t s, const Dataset& ds, const ViewGeom* geom, int 
                     const float4* rec, int rs4, const RecInfo& ri, const LineSet& ls, int nl, int32_t* flags) {
   if (!rec || nl < 2 || ls.n < 1 || ls.n > kMaxLines) return false;
   const int64_t R = (int64_t)nx * ny;
-  // one lambda pass (k_los_spectral's chunking: the same anchors)
-  if (nl > 64) return false;
+  // k_los_spectral's lambda passes and their anchors
   int npass = 1;
   const int ch = pick_chunk(nl, &npass);
-  if (npass != 1) return false;
   if (ls.n == 1) {
-    // One line: the 4 lanes of a ray run the profile's 4 chains. Small images have too few rays
-    // to fill the GPU this way; they run k_nn (one thread per sample) + k_los_spectral, which
-    // add the same values in the same order.
-    if (R < kFusedMinRays || rs4 != 2) return false;
+    // One line: the 4 lanes of a ray run the profile's 4 chains, of up to two lambda passes.
+    // Small images have too few rays to fill the GPU this way; they run k_nn (one thread per
+    // sample) + k_los_spectral, which add the same values in the same order.
+    if (R < kFusedMinRays || rs4 != 2 || npass > 2) return false;
     ProfScope ps("k_nn_los", s);
-    launch_nn_los_t<2, 4, 1>(s, ch, (unsigned)((R + 63) / 64), ds, geom, nx, ny, nz, rec, ri, ls, nl, flags);
+    const unsigned blocks = (unsigned)((R + 63) / 64);
+    if (npass == 1) launch_nn_los_t<2, 4, 1>(s, ch, blocks, ds, geom, nx, ny, nz, rec, ri, ls, nl, flags);
+    else launch_nn_los_2pass(s, ch, blocks, ds, geom, nx, ny, nz, rec, ri, ls, nl, flags);
     return true;
   }
+  if (npass != 1) return false;
   // several lines share each query
   const bool shared = rs4 == 2 && ls.n <= 4 && shared_profile(ri, ls);
   const int G = shared ? 4 : (ls.n <= 2 ? 2 : (ls.n <= 4 ? 4 : 8));

@@ -277,12 +277,14 @@ def test_shared_profile_direct_pass(engine, fe9_table, tmp_path):
         o.close()
 
 
-@pytest.mark.parametrize("atw,nl", [(56.0, 60), (56.0, 7), (5.6e5, 40)])
+@pytest.mark.parametrize("atw,nl", [(56.0, 60), (56.0, 7), (5.6e5, 40), (56.0, 100), (5.6e5, 77)])
 def test_single_line_fused_view(engine, fe9_table, tmp_path, atw, nl):
     """A single-line view of 256 x 256 rays (the fused pass's threshold) runs the fused pass with
-    one line: the 4 lanes of a ray run the profile's 4 chains. It must equal the same line rendered
-    in a batch with a second line (one line per lane, or the shared profile) bit for bit, and match
-    the FP64 oracle; atomic weight 5.6e5 (thermal width / 100) takes the direct-exp pass."""
+    one line: the 4 lanes of a ray run the profile's 4 chains, of one lambda pass, or of two in one
+    launch for 65 <= nl <= 128. It must equal the same line rendered in a batch with a second line
+    (one line per lane, the shared profile, or, beyond 64 pixels, k_nn + two k_los_spectral passes)
+    bit for bit, and match the FP64 oracle; atomic weight 5.6e5 (thermal width / 100) takes the
+    direct-exp pass."""
     F = engine
     path, tab = fe9_table
     d = F.gen_loop_cube(14, seed=21, jitter=0.2)
