@@ -904,6 +904,36 @@ __device__ __forceinline__ int bvh_query(const float4* __restrict__ pts, const f
   return fomo::oct_query(pts, mpts, nlo, nhi, bv.nodes, sb, best, bpos, borig);
 }
 
+// Wave-cooperative oct_query: every lane with act set gets exactly oct_query's answer, and the
+// wave walks the tree once. A node is entered when any active lane could find a better point in
+// it; a lane's own test fails on every node inside a subtree it would have skipped (a child's box
+// lies inside its parent's), so each lane scans exactly the leaves of its own walk, in the same
+// order. The node index is wave-uniform, so the node records are read once per wave (scalar
+// loads) instead of once per lane; the 64 queries of a wave are neighbouring samples and enter
+// nearly the same nodes. All lanes of the wave that are still running must call it.
+__device__ int bvh_query_wave(const float4* __restrict__ pts, const float4* __restrict__ mpts,
+                              const float4* __restrict__ nlo, const float4* __restrict__ nhi, const BvhInfo& bv,
+                              const SearchBox& sb, bool act, double best, int bpos, int borig) {
+  if (bv.nodes <= 0) return bpos;
+  if (act && bpos < 0) fomo::oct_descend(pts, mpts, nlo, nhi, sb, best, bpos, borig);
+  const int n = (int)bv.nodes;
+  for (int e = 0; e < n;) {
+    e = __builtin_amdgcn_readfirstlane(e);
+    const float4 l = nlo[e];
+    const float4 h = nhi[e];
+    const int w = __float_as_int(l.w);
+    const bool in = act && fomo::oct_enter(l, h, sb, best);
+    const bool any = __ballot(in) != 0;
+    if (w < 0) {
+      if (in) fomo::leaf_scan(pts, mpts, __float_as_int(h.w), -w, sb, best, bpos, borig);
+      ++e;
+    } else {
+      e = any ? e + 1 : w;
+    }
+  }
+  return bpos;
+}
+
 // Per-sample outputs of the NN pass for the point at sorted position r (-1: none):
 // svlos[s] (LOS velocity, fomo-nearestneighbour.cpp:87-89) and sem[l*S + s] = (peak_l, fwhm_l),
 // zero when no point is inside the search box (:190-193).
@@ -1015,12 +1045,13 @@ __global__ void __launch_bounds__(256) k_nn_far(const float4* __restrict__ pts, 
   const int j = tx * 16 + (threadIdx.x & 15);
   const int i = ty * 16 + (threadIdx.x >> 4);
   const int k = blockIdx.y;
-  if (j >= nx || i >= ny) return;
   const size_t R = (size_t)nx * ny;
-  const size_t ray = (size_t)i * nx + j;
-  if (!rayflag[ray]) return;
+  const bool inside = j < nx && i < ny;
+  const size_t ray = inside ? (size_t)i * nx + j : 0;
   const size_t sidx = (size_t)k * R + ray;
-  if (idx[sidx] != -2) return;
+  // lanes without an unresolved sample still take part in the wave's walk (act = false)
+  const bool act = inside && rayflag[ray] && idx[sidx] == -2;
+  if (__ballot(act) == 0) return;
   const ViewGeom vg = *vgp;
   const double minx = vg.ext[0], maxx = vg.ext[1], miny = vg.ext[2], maxy = vg.ext[3], minz = vg.ext[4];
   const double xx = double(j) / (nx - 1) * (maxx - minx) + minx;  // fomo-nearestneighbour.cpp:152
@@ -1036,7 +1067,7 @@ __global__ void __launch_bounds__(256) k_nn_far(const float4* __restrict__ pts, 
   box.hi0 = (float)(p0 + md), box.hi1 = (float)(p1 + md), box.hi2 = (float)(p2 + md);
   double best = INFINITY;
   int bpos = -1, borig = 0x7fffffff;
-  if (k > 0) {
+  if (act && k > 0) {
     const int prev = idx[sidx - R];  // k_nn's result, or this pass's (any value >= 0 is a point)
     if (prev >= 0) {
       const float4 q = pts[prev];
@@ -1049,7 +1080,8 @@ __global__ void __launch_bounds__(256) k_nn_far(const float4* __restrict__ pts, 
       }
     }
   }
-  const int r = bvh_query(pts, mpts, nlo, nhi, bv, box, best, bpos, borig);
+  const int r = bvh_query_wave(pts, mpts, nlo, nhi, bv, box, act, best, bpos, borig);
+  if (!act) return;
   idx[sidx] = r;
   if constexpr (RS4 > 0) write_sample<RS4>(r, rec, ri, nlines, R * nz, sidx, cb, sb, cl, sl, svlos, sem);
 }

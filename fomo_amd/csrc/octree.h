@@ -92,46 +92,57 @@ FOMO_HD inline void leaf_scan(const A& pts, const A& mpts, int first, int count,
   }
 }
 
+// First bound of a query without one (bpos < 0): a greedy descent to the leaf whose box is
+// nearest at every level (among the children that overlap the search box), and that leaf's scan.
+template <class A>
+FOMO_HD inline void oct_descend(const A& pts, const A& mpts, const A& nlo, const A& nhi, const SearchBox& sb,
+                                double& best, int& bpos, int& borig) {
+  int e = 0;
+  int w = f2i(nlo[0].w);
+  while (w >= 0) {
+    int bc = -1;
+    double bm = INFINITY;
+    for (int c = e + 1; c < w;) {
+      const auto cl = nlo[c];
+      const auto ch = nhi[c];
+      if (sb.node_overlaps(cl, ch)) {
+        const double md = node_mindist2(cl, ch, sb.t0, sb.t1, sb.t2);
+        if (md < bm) {
+          bm = md;
+          bc = c;
+        }
+      }
+      const int cw = f2i(cl.w);
+      c = cw < 0 ? c + 1 : cw;
+    }
+    if (bc < 0) break;
+    e = bc;
+    w = f2i(nlo[e].w);
+  }
+  if (w < 0) leaf_scan(pts, mpts, f2i(nhi[e].w), -w, sb, best, bpos, borig);
+}
+
+// Whether node (l, h) can hold a point strictly inside the search box at distance <= best.
+template <class V>
+FOMO_HD inline bool oct_enter(const V& l, const V& h, const SearchBox& sb, double best) {
+  return sb.node_overlaps(l, h) && node_mindist2(l, h, sb.t0, sb.t1, sb.t2) <= best;
+}
+
 // Exact NN strictly inside the search box. (best, bpos, borig) may carry a known point strictly
-// inside the box as the starting bound; without one (bpos < 0) a greedy descent (nearest child
-// box at each level) supplies the first bound. Then a stackless preorder walk: a node whose box
-// cannot hold a point strictly inside the search box at distance <= best is skipped with its
+// inside the box as the starting bound; without one (bpos < 0) oct_descend supplies the first
+// bound. Then a stackless preorder walk: a node that oct_enter rejects is skipped with its
 // subtree. Returns the sorted position of the answer or -1.
 template <class A>
 FOMO_HD inline int oct_query(const A& pts, const A& mpts, const A& nlo, const A& nhi, int64_t nodes,
                              const SearchBox& sb, double best, int bpos, int borig) {
   if (nodes <= 0) return bpos;
-  if (bpos < 0) {
-    int e = 0;
-    int w = f2i(nlo[0].w);
-    while (w >= 0) {
-      int bc = -1;
-      double bm = INFINITY;
-      for (int c = e + 1; c < w;) {
-        const auto cl = nlo[c];
-        const auto ch = nhi[c];
-        if (sb.node_overlaps(cl, ch)) {
-          const double md = node_mindist2(cl, ch, sb.t0, sb.t1, sb.t2);
-          if (md < bm) {
-            bm = md;
-            bc = c;
-          }
-        }
-        const int cw = f2i(cl.w);
-        c = cw < 0 ? c + 1 : cw;
-      }
-      if (bc < 0) break;
-      e = bc;
-      w = f2i(nlo[e].w);
-    }
-    if (w < 0) leaf_scan(pts, mpts, f2i(nhi[e].w), -w, sb, best, bpos, borig);
-  }
+  if (bpos < 0) oct_descend(pts, mpts, nlo, nhi, sb, best, bpos, borig);
   const int n = (int)nodes;
   for (int e = 0; e < n;) {
     const auto l = nlo[e];
     const auto h = nhi[e];
     const int w = f2i(l.w);
-    const bool in = sb.node_overlaps(l, h) && node_mindist2(l, h, sb.t0, sb.t1, sb.t2) <= best;
+    const bool in = oct_enter(l, h, sb, best);
     if (w < 0) {
       if (in) leaf_scan(pts, mpts, f2i(h.w), -w, sb, best, bpos, borig);
       ++e;
