@@ -128,3 +128,55 @@ def test_zero_density_points(engine, fe9_table):
         _check(ds, d, tab, em, 0.5, 1.0, 20, 18, 16, 12)
     finally:
         em.close(), T.close(), ds.close()
+
+
+@pytest.mark.parametrize("nl", [2, 64, 65, 128])
+def test_single_line_fused_lambda_limits(engine, fe9_table, tmp_path, nl):
+    """The single-line fused pass at its lambda limits (2 and 64 pixels in one pass, 65 and 128 in
+    two) on a 256 x 256 view: equal to the same line in a two-line batch (one line per lane, or
+    k_nn + the integration passes beyond 64 pixels) bit for bit, and within 1e-5 of the FP64
+    oracle."""
+    path, tab = fe9_table
+    rng = np.random.default_rng(50 + nl)
+    n = 1500
+    d = {"x": rng.uniform(-3, 3, n).astype(np.float32), "y": rng.uniform(-3, 3, n).astype(np.float32),
+         "z": rng.uniform(-3, 3, n).astype(np.float32)}
+    d.update(_fields(n, rng))
+    ds = _make(d)
+    T = F.Table.from_file(path)
+    T2 = F.Table.from_file(F.gen_goft_table(str(tmp_path / "b.dat"), lambda0=193.5, atweight=56.0, peak_logt=6.2))
+    em = F.Emission.chianti(ds, T, tab["lambda0"], tab["atweight"])
+    em2 = F.Emission.chianti(ds, T2, 193.5, 56.0)
+    try:
+        l, b = 0.9, 0.7
+        _, _, ref = _check(ds, d, tab, em, l, b, 256, 256, 6, nl)
+        img, meta = F.render_nn(em, l, b, 256, 256, 6, nl, 0.2 * C / tab["lambda0"])
+        bt = F.Batch([em, em2], [l], [b], 256, 256, 6, nl, 0.2 * C / tab["lambda0"])
+        bt.run()
+        out, _ = bt.read()
+        bt.close()
+        assert np.array_equal(out[0, 0].ravel(), img)
+        assert np.count_nonzero(ref) > 0
+    finally:
+        em.close(), em2.close(), T.close(), T2.close(), ds.close()
+
+
+def test_single_line_fused_2d_datacube(engine, fe9_table):
+    """A 2D datacube (z = 0) seen obliquely on a 256 x 256 view: the single-line fused pass against
+    the FP64 oracle (NN indices bit-exact through the separate NN pass)."""
+    path, tab = fe9_table
+    rng = np.random.default_rng(61)
+    m = 40
+    g = np.arange(m, dtype=np.float64) / (m - 1) * 10 - 5
+    yy, xx = np.meshgrid(g, g, indexing="ij")
+    d = {"x": (xx.ravel() + 0.1 * rng.uniform(-1, 1, m * m)).astype(np.float32),
+         "y": (yy.ravel() + 0.1 * rng.uniform(-1, 1, m * m)).astype(np.float32)}
+    d["z"] = np.zeros(m * m, np.float32)
+    d.update(_fields(m * m, rng))
+    ds = _make(d, dim=2)
+    T = F.Table.from_file(path)
+    em = F.Emission.chianti(ds, T, tab["lambda0"], tab["atweight"])
+    try:
+        _check(ds, d, tab, em, 0.7, 0.4, 256, 256, 5, 30)
+    finally:
+        em.close(), T.close(), ds.close()
