@@ -57,14 +57,17 @@ struct SearchBox {
   }
 };
 
+// Squared distance from the target to the box [l, h] (l <= h on every axis). Per axis
+// max(l - t, t - h, 0): at most one of the two differences is positive, and it equals the
+// distance to the nearer face; written without branches, so a wave's lanes stay together.
 template <class V>
 FOMO_HD inline double node_mindist2(const V& l, const V& h, float t0, float t1, float t2) {
   double s = 0.0, d;
-  d = t0 < l.x ? (double)l.x - (double)t0 : (t0 > h.x ? (double)t0 - (double)h.x : 0.0);
+  d = std::fmax(std::fmax((double)l.x - (double)t0, (double)t0 - (double)h.x), 0.0);
   s = s + d * d;
-  d = t1 < l.y ? (double)l.y - (double)t1 : (t1 > h.y ? (double)t1 - (double)h.y : 0.0);
+  d = std::fmax(std::fmax((double)l.y - (double)t1, (double)t1 - (double)h.y), 0.0);
   s = s + d * d;
-  d = t2 < l.z ? (double)l.z - (double)t2 : (t2 > h.z ? (double)t2 - (double)h.z : 0.0);
+  d = std::fmax(std::fmax((double)l.z - (double)t2, (double)t2 - (double)h.z), 0.0);
   s = s + d * d;
   return s;
 }
