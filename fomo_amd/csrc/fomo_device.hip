@@ -1614,6 +1614,23 @@ __global__ void __launch_bounds__(256) k_nn_los(const float4* __restrict__ pts, 
   const int64_t R = (int64_t)nx * ny;
   const int64_t ray = ((int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) * RPW + lane / G;
   const bool live = ray < R;
+  int32_t* const flag = flags + (size_t)(SL > 0 ? 0 : q) * R + (live ? ray : 0);
+  // flagged (ray, line, lambda pass)s of this lane: bit l * NP + p (shared profile; flags[l * R +
+  // ray] holds the passes' bits), bit 0 (line q). The direct pass reads them first: usually none
+  // is set, and whole waves leave before any other work.
+  unsigned need = 0;
+  if (DIRECT) {
+    if (live && (SL > 0 || q < ls.n)) {
+      if constexpr (SL > 0) {
+#pragma unroll
+        for (int l = 0; l < SL; ++l)
+          if (l < ls.n) need |= ((unsigned)flag[(size_t)l * R] & ((1u << NP) - 1)) << (l * NP);
+      } else {
+        need = *flag != 0 ? 1u : 0u;
+      }
+    }
+    if (__ballot(need != 0) == 0) return;
+  }
   const int i = live ? (int)(ray / nx) : 0, j = live ? (int)(ray % nx) : 0;
   const ViewGeom vg = *vgp;
   const double minx = vg.ext[0], maxx = vg.ext[1], miny = vg.ext[2], maxy = vg.ext[3], minz = vg.ext[4];
@@ -1637,24 +1654,7 @@ __global__ void __launch_bounds__(256) k_nn_los(const float4* __restrict__ pts, 
       }
   }
   bool integ = live && (SL > 0 || q < ls.n);
-  int32_t* const flag = flags + (size_t)(SL > 0 ? 0 : q) * R + (live ? ray : 0);
-  // flagged (ray, line, lambda pass)s of this lane: bit l * NP + p (shared profile; flags[l * R +
-  // ray] holds the passes' bits), bit 0 (line q)
-  unsigned need = 0;
-  if (DIRECT) {
-    if (integ) {
-      if constexpr (SL > 0) {
-#pragma unroll
-        for (int l = 0; l < SL; ++l)
-          if (l < ls.n) need |= ((unsigned)flag[(size_t)l * R] & ((1u << NP) - 1)) << (l * NP);
-      } else {
-        need = *flag != 0 ? 1u : 0u;
-      }
-    }
-    // whole waves leave together when none of their (ray, line)s was flagged
-    integ = integ && need != 0;
-    if (__ballot(integ) == 0) return;
-  }
+  if (DIRECT) integ = integ && need != 0;
   unsigned bad = 0;  // bits as in need: recurrence anchors out of range
   const double dl = wla / (nl - 1);
   const double lamc = double(0) / (nl - 1) * wla - wla / 2.;  // lambdaval of the first pixel (:200)
