@@ -36,7 +36,7 @@ FLOP_PER_W = 8.0
 # HBM bytes per launch (read + write) of the default workload's kernels, from two rocprofv3 --pmc
 # runs (FETCH_SIZE x 2.000, WRITE_SIZE x 1.000, the factors a 1 GiB copy kernel gives):
 # profiles/batched_render_256cube_counters.md
-MEASURED_TRAFFIC = {"k_nn_los": 1.398e9}
+MEASURED_TRAFFIC = {"k_nn_los": 1.445e9}
 
 
 def log(*a):
