@@ -69,7 +69,7 @@ def main():
 
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1 "
-                                   "(multi-GPU sharding is described in DESIGN.md, not implemented)"}))
+                                   "(sharding over several GPUs is not implemented)"}))
         sys.exit(2)
 
     import fomo_amd as F

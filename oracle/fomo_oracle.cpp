@@ -24,7 +24,8 @@
 //   integration      : uniform slab, line integral and Doppler shift known answers.
 //   projection       : pixels of grid points in an axis view (exact), half-away-from-zero
 //                      rounding cases, point-order float sums, one-point Doppler profile.
-// Unpinned (no reference import possible): Boost R-tree tie order
+// Unpinned (the reference cannot be built here: its Boost.Geometry, GSL and CGAL dependencies
+// are absent): Boost R-tree tie order
 // (lowest index is used here), GSL constant versions (CODATA 2006 values used).
 
 #include <algorithm>

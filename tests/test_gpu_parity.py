@@ -1,7 +1,7 @@
 """GPU parity: the HIP engine (through the C ABI) against the CPU oracle.
 
 Bars: nearest-neighbour choices and view extents bit-exact; images within relative L2 1e-5 of
-the FP64 oracle; emissivity within float32 rounding of FP64.
+the FP64 oracle (the target tolerance); emissivity within float32 rounding of FP64.
 """
 import numpy as np
 import pytest

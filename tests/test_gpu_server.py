@@ -3,7 +3,7 @@
 Bars: a request's image and geometry are bit-identical whether it runs alone, in a mix of
 concurrently admitted requests of other shapes and methods, or through the synchronous
 renderer (the same kernels on private buffers; no cross-request reductions). One request of the
-mix is also checked against the FP64 oracle at the north-star tolerance (rel L2 1e-5).
+mix is also checked against the FP64 oracle at the target tolerance (rel L2 1e-5).
 Admission never exceeds the memory budget or the in-flight limit.
 """
 import numpy as np
