@@ -39,6 +39,7 @@
 #include "../../include/fomo_hip.h"
 #include "arena.h"
 #include "delaunay.h"
+#include "host_copy.h"
 #include "octree.h"
 #include "spectral_exp.h"
 
@@ -2066,6 +2067,9 @@ struct Ctx {
   std::map<std::string, ProfStat> prof;
   std::vector<std::pair<std::string, std::pair<hipEvent_t, hipEvent_t>>> pending;
   std::vector<hipEvent_t> evpool;
+  void* stage = nullptr;                         // pinned staging of large results (copy_out)
+  hipEvent_t stage_ev[2] = {nullptr, nullptr};
+  std::unique_ptr<fomo::HostCopyPool> copy_pool;
 };
 
 Ctx g_ctx;
@@ -2130,6 +2134,42 @@ int sync_stream() {
 }
 
 inline unsigned nblk(int64_t n, int bs = kBlock) { return (unsigned)((n + bs - 1) / bs); }
+
+// Device-to-host copy of a result enqueued on s. Results under 4 MiB are left to the runtime
+// (the caller synchronises). Larger ones go through two pinned 8 MiB chunks and are complete on
+// return: the device fills one chunk while host threads copy the other to the destination.
+constexpr int64_t kStageChunk = 8 << 20;
+constexpr int64_t kStageMin = 4 << 20;
+constexpr int kStageHelpers = 5;
+int copy_out(void* dst, const void* src, int64_t bytes, hipStream_t s) {
+  if (bytes <= 0) return FOMO_OK;
+  if (bytes < kStageMin) {
+    HIPOK(hipMemcpyAsync(dst, src, (size_t)bytes, hipMemcpyDeviceToHost, s));
+    return FOMO_OK;
+  }
+  if (!g_ctx.stage) {
+    HIPOK(hipHostMalloc(&g_ctx.stage, 2 * kStageChunk, hipHostMallocDefault));
+    for (auto& e : g_ctx.stage_ev) HIPOK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+  }
+  if (!g_ctx.copy_pool) g_ctx.copy_pool.reset(new fomo::HostCopyPool(kStageHelpers));
+  char* const st = static_cast<char*>(g_ctx.stage);
+  const int64_t n = (bytes + kStageChunk - 1) / kStageChunk;
+  const auto len_of = [&](int64_t c) { return std::min<int64_t>(kStageChunk, bytes - c * kStageChunk); };
+  const auto enq = [&](int64_t c) {
+    hipError_t e = hipMemcpyAsync(st + (c & 1) * kStageChunk, static_cast<const char*>(src) + c * kStageChunk,
+                                  (size_t)len_of(c), hipMemcpyDeviceToHost, s);
+    return e == hipSuccess ? hipEventRecord(g_ctx.stage_ev[c & 1], s) : e;
+  };
+  HIPOK(enq(0));
+  for (int64_t c = 0; c < n; ++c) {
+    // chunk c + 1 goes to the half that held chunk c - 1, copied out in the previous step
+    if (c + 1 < n) HIPOK(enq(c + 1));
+    HIPOK(hipEventSynchronize(g_ctx.stage_ev[c & 1]));
+    g_ctx.copy_pool->copy(static_cast<char*>(dst) + c * kStageChunk, st + (c & 1) * kStageChunk,
+                          (size_t)len_of(c));
+  }
+  return FOMO_OK;
+}
 
 // Implicit BVH over the dataset's points (Morton order of 10-bit quantised coordinates).
 int build_bvh(Dataset& ds) {
@@ -3386,6 +3426,13 @@ int fomo_shutdown(void) {
   g_ctx.tmpf.release();
   for (auto e : g_ctx.evpool) (void)hipEventDestroy(e);
   g_ctx.evpool.clear();
+  g_ctx.copy_pool.reset();
+  if (g_ctx.stage) (void)hipHostFree(g_ctx.stage);
+  g_ctx.stage = nullptr;
+  for (auto& e : g_ctx.stage_ev) {
+    if (e) (void)hipEventDestroy(e);
+    e = nullptr;
+  }
   (void)hipStreamDestroy(g_ctx.stream);
   (void)hipStreamDestroy(g_ctx.stream2);
   g_ctx.stream2 = nullptr;
@@ -3641,7 +3688,8 @@ static int render_one(int goft, int mode, double l, double b, int nx, int ny, in
   const RenderBufs rb{vs.partial.as<double>(), vs.geom.as<ViewGeom>(), vs.idx.as<int32_t>(), vs.aux.as<float>(),
                       vs.out.as<float>(),      vs.flag.as<int32_t>(),  vs.sem.as<float2>(),  vs.rayflag.as<uint8_t>()};
   if (int rc = enqueue_render(s, *ds, *gc, mode, l, b, nx, ny, nz, nl, width, dn, rb)) return rc;
-  if (out) HIPOK(hipMemcpyAsync(out, rb.out, sz[4], hipMemcpyDeviceToHost, s));
+  if (out)
+    if (int rc = copy_out(out, rb.out, sz[4], s)) return rc;
   ViewGeom hg;
   HIPOK(hipMemcpyAsync(&hg, rb.geom, sizeof(ViewGeom), hipMemcpyDeviceToHost, s));
   if (int rc = sync_stream()) return rc;
@@ -4056,7 +4104,8 @@ int fomo_batch_read(int id, float* out, double* meta) {
   if (!bt) return fail(FOMO_ERR_NOTFOUND, "unknown batch id");
   const int nv = (int)bt->l.size();
   const size_t total = (size_t)bt->nx * bt->ny * std::max(bt->nl, 1) * bt->gofts.size() * nv;
-  if (out) HIPOK(hipMemcpyAsync(out, bt->out.p, sizeof(float) * total, hipMemcpyDeviceToHost, g_ctx.stream));
+  if (out)
+    if (int rc = copy_out(out, bt->out.p, (int64_t)sizeof(float) * total, g_ctx.stream)) return rc;
   std::vector<ViewGeom> hg(nv);
   HIPOK(hipMemcpyAsync(hg.data(), bt->geom.p, sizeof(ViewGeom) * nv, hipMemcpyDeviceToHost, g_ctx.stream));
   HIPOK(hipStreamSynchronize(g_ctx.stream));
