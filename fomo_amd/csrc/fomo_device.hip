@@ -1348,7 +1348,48 @@ __global__ void k_proj_starts(const uint32_t* __restrict__ key, int64_t n, int64
   start[q] = (int32_t)lo;
 }
 
-// One thread per (pixel, lambda pixel); nl == 1 sums the peaks (Imaging, :148-155).
+// The Gaussian line profile exp(-kap (d0 + c dl)^2) over lambda pixels c = 0, 1, ... by ratio
+// recurrence. profile_args: xa = {aq, ar, a0}, the exponents of the ratio of ratios, of the first
+// ratio (pixel 1 over pixel 0) and of pixel 0; false when the recurrence could leave the double
+// range (the caller then evaluates every pixel directly).
+__device__ __forceinline__ bool profile_args(double kap, double d0, double dl, double (&xa)[3]) {
+  const double a0 = -kap * d0 * d0;
+  const double ar = -kap * (2.0 * d0 * dl + dl * dl);
+  const double aq = -2.0 * kap * dl * dl;
+  xa[0] = aq, xa[1] = ar, xa[2] = a0;
+  return a0 > -600.0 && fabs(ar) < 100.0 && aq > -30.0;
+}
+
+// Stride-4 chains of the profile from ex = {q, r, g_0} = exp(xa): chain j holds the pixels
+// j, j+4, j+8, ..., starting at g[j] with ratio R[j]; one step is g *= R, R *= q16.
+//   g_{c+4} = g_c R_c,  R_{c+4} = R_c q^16,  R_0 = r_0 r_1 r_2 r_3 = r^4 q^6
+// Every kernel that integrates the profile builds its chains here, so that the same pixel gets
+// the same value in all of them.
+__device__ __forceinline__ void profile_chains(const double (&ex)[3], double (&g)[4], double (&R)[4], double& q16) {
+  const double q = ex[0], ra = ex[1];
+  double r = ra;
+  g[0] = ex[2];
+  g[1] = g[0] * r;
+  r *= q;
+  g[2] = g[1] * r;
+  r *= q;
+  g[3] = g[2] * r;
+  const double q2 = q * q, q4 = q2 * q2;
+  q16 = (q4 * q4) * (q4 * q4);
+  const double ra2 = ra * ra;
+  R[0] = (ra2 * ra2) * (q4 * q2);
+  R[1] = R[0] * q4;
+  R[2] = R[1] * q4;
+  R[3] = R[2] * q4;
+}
+
+// One thread per (pixel, chunk of kProjChunk lambda pixels), adding the pixel's points in point
+// order; nl == 1: one thread per pixel summing the peaks (Imaging, :148-155). A point's Gaussian
+// over the chunk comes from the ratio recurrence (three exps per point and chunk,
+// profile_chains), and each term is added as the reference adds it: tmp = peak * g in double,
+// then acc = (float)(acc + tmp) (:139-144). A point whose anchors would leave the double range
+// takes the reference's exp for every lambda pixel.
+constexpr int kProjChunk = 16;
 template <int RS4>
 __global__ void __launch_bounds__(256) k_proj_spectral(const int32_t* __restrict__ start,
                                                        const int32_t* __restrict__ order,
@@ -1356,32 +1397,64 @@ __global__ void __launch_bounds__(256) k_proj_spectral(const int32_t* __restrict
                                                        const ViewGeom* __restrict__ vgp, int64_t R, int nl,
                                                        double wla, double lambda0, float* __restrict__ out) {
   const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  const int nlo = nl > 1 ? nl : 1;
-  if (t >= R * nlo) return;
-  const int64_t pix = t / nlo;
-  const int il = (int)(t - pix * nlo);
-  const int q0 = start[pix], q1 = start[pix + 1];
-  float acc = 0.f;
-  if (nl > 1) {
-    const double cb = vgp->cb, sb = vgp->sb, cl = vgp->cl, sl = vgp->sl;
-    const double u0 = sb * cl, u1 = -sb * sl, u2 = cb;                               // :34
-    const double lambdaval = static_cast<double>(il) / (nl - 1) * wla - wla / 2.;  // :138
-    const double ln2 = log(2.);
-    const double K = ri.K[0];
-    for (int q = q0; q < q1; ++q) {
-      const float4* rp = rec + (size_t)order[q] * RS4;
-      const float4 a = rp[0];  // vx, vy, vz, T
-      const float pk = rp[1].x;
-      const double losvel = ((0.0 + u0 * (double)a.x) + u1 * (double)a.y) + u2 * (double)a.z;  // :53-54
-      const double fw = K > 0 ? (double)(float)(K * sqrt((double)a.w)) : 1.0;
-      const double d = lambdaval - losvel / FOMO_C_LIGHT * lambda0;
-      const double tmp = (double)pk * exp((-(d * d)) / (fw * fw) * 4. * ln2);  // :139
-      acc = (float)((double)acc + tmp);                                           // :144
-    }
-  } else {
-    for (int q = q0; q < q1; ++q) acc = acc + rec[(size_t)order[q] * RS4 + 1].x;  // :154
+  if (nl <= 1) {
+    if (t >= R) return;
+    float acc = 0.f;
+    for (int q = start[t]; q < start[t + 1]; ++q) acc = acc + rec[(size_t)order[q] * RS4 + 1].x;  // :154
+    out[t] = (float)(vgp->scale * (double)acc);  // :200
+    return;
   }
-  out[t] = (float)(vgp->scale * (double)acc);  // :200
+  const int nch = (nl + kProjChunk - 1) / kProjChunk;
+  if (t >= R * nch) return;
+  const int64_t pix = t / nch;
+  const int c0 = (int)(t - pix * nch) * kProjChunk;
+  const int q0 = start[pix], q1 = start[pix + 1];
+  const double cb = vgp->cb, sb = vgp->sb, cl = vgp->cl, sl = vgp->sl;
+  const double u0 = sb * cl, u1 = -sb * sl, u2 = cb;                          // :34
+  const double dl = wla / (nl - 1);
+  const double lamc = static_cast<double>(c0) / (nl - 1) * wla - wla / 2.;  // lambdaval at c0 (:138)
+  const double ln2 = log(2.);
+  const double K = ri.K[0];
+  float acc[kProjChunk];
+#pragma unroll
+  for (int c = 0; c < kProjChunk; ++c) acc[c] = 0.f;
+  for (int q = q0; q < q1; ++q) {
+    const float4* rp = rec + (size_t)order[q] * RS4;
+    const float4 a = rp[0];  // vx, vy, vz, T
+    const float pk = rp[1].x;
+    if (pk == 0.f) continue;  // adds zeros
+    const double losvel = ((0.0 + u0 * (double)a.x) + u1 * (double)a.y) + u2 * (double)a.z;  // :53-54
+    const double fw = K > 0 ? (double)(float)(K * sqrt((double)a.w)) : 1.0;
+    const double mu = losvel / FOMO_C_LIGHT * lambda0;
+    double xa[3];
+    if (profile_args(fomo::spectral_kappa(fw), lamc - mu, dl, xa)) {
+      double ex[3], g[4], RR[4], q16;
+      fomo::exp3(xa, ex);
+      profile_chains(ex, g, RR, q16);
+#pragma unroll
+      for (int c4 = 0; c4 < kProjChunk; c4 += 4) {
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          const double tmp = (double)pk * g[j];
+          acc[c4 + j] = (float)((double)acc[c4 + j] + tmp);  // :144
+          g[j] *= RR[j];
+          RR[j] *= q16;
+        }
+      }
+    } else {
+#pragma unroll
+      for (int c = 0; c < kProjChunk; ++c) {
+        const double lambdaval = static_cast<double>(c0 + c) / (nl - 1) * wla - wla / 2.;  // :138
+        const double d = lambdaval - losvel / FOMO_C_LIGHT * lambda0;
+        const double tmp = (double)pk * exp((-(d * d)) / (fw * fw) * 4. * ln2);  // :139
+        acc[c] = (float)((double)acc[c] + tmp);                                  // :144
+      }
+    }
+  }
+  float* o = out + (size_t)pix * nl + c0;
+#pragma unroll
+  for (int c = 0; c < kProjChunk; ++c)
+    if (c0 + c < nl) o[c] = (float)(vgp->scale * (double)acc[c]);  // :200
 }
 
 // Per-point records for k_nn (sorted order): {vx, vy, vz, T, peak_0, peak_1, ...}, rs4 float4 each.
@@ -1427,41 +1500,6 @@ __global__ void k_idx_to_orig(const int32_t* __restrict__ idx, const int32_t* __
 // reference's order. A ray with a sample whose anchors would under/overflow is flagged and
 // left to the DIRECT instantiation (one exp per sample x lambda), launched right after.
 // ------------------------------------------------------------------------------------
-// The Gaussian line profile exp(-kap (d0 + c dl)^2) over lambda pixels c = 0, 1, ... by ratio
-// recurrence. profile_args: xa = {aq, ar, a0}, the exponents of the ratio of ratios, of the first
-// ratio (pixel 1 over pixel 0) and of pixel 0; false when the recurrence could leave the double
-// range (the caller then evaluates every pixel directly).
-__device__ __forceinline__ bool profile_args(double kap, double d0, double dl, double (&xa)[3]) {
-  const double a0 = -kap * d0 * d0;
-  const double ar = -kap * (2.0 * d0 * dl + dl * dl);
-  const double aq = -2.0 * kap * dl * dl;
-  xa[0] = aq, xa[1] = ar, xa[2] = a0;
-  return a0 > -600.0 && fabs(ar) < 100.0 && aq > -30.0;
-}
-
-// Stride-4 chains of the profile from ex = {q, r, g_0} = exp(xa): chain j holds the pixels
-// j, j+4, j+8, ..., starting at g[j] with ratio R[j]; one step is g *= R, R *= q16.
-//   g_{c+4} = g_c R_c,  R_{c+4} = R_c q^16,  R_0 = r_0 r_1 r_2 r_3 = r^4 q^6
-// Every kernel that integrates the profile builds its chains here, so that the same pixel gets
-// the same value in all of them.
-__device__ __forceinline__ void profile_chains(const double (&ex)[3], double (&g)[4], double (&R)[4], double& q16) {
-  const double q = ex[0], ra = ex[1];
-  double r = ra;
-  g[0] = ex[2];
-  g[1] = g[0] * r;
-  r *= q;
-  g[2] = g[1] * r;
-  r *= q;
-  g[3] = g[2] * r;
-  const double q2 = q * q, q4 = q2 * q2;
-  q16 = (q4 * q4) * (q4 * q4);
-  const double ra2 = ra * ra;
-  R[0] = (ra2 * ra2) * (q4 * q2);
-  R[1] = R[0] * q4;
-  R[2] = R[1] * q4;
-  R[3] = R[2] * q4;
-}
-
 template <int CH, bool DIRECT>
 __global__ void __launch_bounds__(256) k_los_spectral(int R, int nz, const LineSet ls, const float* __restrict__ vlos,
                                                       const ViewGeom* __restrict__ vgp, int nl, int l0,
@@ -2957,7 +2995,7 @@ int enqueue_projection(hipStream_t s, const Dataset& ds, const Goft& gc, const R
   }
   k_proj_starts<<<nblk(R + 1), kBlock, 0, s>>>(key_out, n, R, start);
   HIPOK(hipGetLastError());
-  const int64_t nt = R * std::max(nl, 1);
+  const int64_t nt = nl > 1 ? R * ((nl + kProjChunk - 1) / kProjChunk) : R;
   const double wla = width * gc.lambda0 / FOMO_C_LIGHT;  // fomo-projection.cpp:69
   ProfScope ps("k_proj_spectral", s);
 #define FOMO_PROJ_ARGS start, val_out, gc.rec.as<float4>(), gc.ri, rb.geom, R, nl, wla, gc.lambda0, rb.out
