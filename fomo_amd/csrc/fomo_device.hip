@@ -2181,13 +2181,18 @@ constexpr int64_t kStageMin = 4 << 20;
 constexpr int kStageHelpers = 5;
 int copy_out(void* dst, const void* src, int64_t bytes, hipStream_t s) {
   if (bytes <= 0) return FOMO_OK;
-  if (bytes < kStageMin) {
+  if (!g_ctx.stage && bytes >= kStageMin) {
+    // pinned staging once per context; without it the runtime's own path still works
+    if (hipHostMalloc(&g_ctx.stage, 2 * kStageChunk, hipHostMallocDefault) != hipSuccess) g_ctx.stage = nullptr;
+    for (auto& e : g_ctx.stage_ev)
+      if (g_ctx.stage && hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) {
+        (void)hipHostFree(g_ctx.stage);
+        g_ctx.stage = nullptr;
+      }
+  }
+  if (bytes < kStageMin || !g_ctx.stage) {
     HIPOK(hipMemcpyAsync(dst, src, (size_t)bytes, hipMemcpyDeviceToHost, s));
     return FOMO_OK;
-  }
-  if (!g_ctx.stage) {
-    HIPOK(hipHostMalloc(&g_ctx.stage, 2 * kStageChunk, hipHostMallocDefault));
-    for (auto& e : g_ctx.stage_ev) HIPOK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
   }
   if (!g_ctx.copy_pool) g_ctx.copy_pool.reset(new fomo::HostCopyPool(kStageHelpers));
   char* const st = static_cast<char*>(g_ctx.stage);
