@@ -917,16 +917,30 @@ __device__ int bvh_query_wave(const float4* __restrict__ pts, const float4* __re
                               const SearchBox& sb, bool act, double best, int bpos, int borig) {
   if (bv.nodes <= 0) return bpos;
   if (act && bpos < 0) fomo::oct_descend(pts, mpts, nlo, nhi, sb, best, bpos, borig);
+  // The node test in float: the squared box distance has a relative error below 1e-6 (exact
+  // differences of nearby floats, then three rounded squares and two rounded sums), so with a
+  // 1e-5 margin and the bound rounded up it enters every node oct_enter enters. Entering more
+  // nodes cannot change a lane's answer: its extra leaves only add points that lose to the
+  // nearest one.
+  const auto bound = [](double b) { return (float)b * 1.000001f; };
+  float bestf = bound(best);
   const int n = (int)bv.nodes;
   for (int e = 0; e < n;) {
     e = __builtin_amdgcn_readfirstlane(e);
     const float4 l = nlo[e];
     const float4 h = nhi[e];
     const int w = __float_as_int(l.w);
-    const bool in = act && fomo::oct_enter(l, h, sb, best);
+    const float d0 = fmaxf(fmaxf(l.x - sb.t0, sb.t0 - h.x), 0.f);
+    const float d1 = fmaxf(fmaxf(l.y - sb.t1, sb.t1 - h.y), 0.f);
+    const float d2 = fmaxf(fmaxf(l.z - sb.t2, sb.t2 - h.z), 0.f);
+    const float md = (d0 * d0 + d1 * d1) + d2 * d2;
+    const bool in = act && sb.node_overlaps(l, h) && md * 0.99999f <= bestf;
     const bool any = __ballot(in) != 0;
     if (w < 0) {
-      if (in) fomo::leaf_scan(pts, mpts, __float_as_int(h.w), -w, sb, best, bpos, borig);
+      if (in) {
+        fomo::leaf_scan(pts, mpts, __float_as_int(h.w), -w, sb, best, bpos, borig);
+        bestf = bound(best);
+      }
       ++e;
     } else {
       e = any ? e + 1 : w;
