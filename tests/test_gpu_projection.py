@@ -3,7 +3,8 @@
 Bars: the pixel of every point equal to the oracle's (integer work, bit-exact); given the same
 per-point emission (the device's, read back), the imaging image (nl == 1, float sums of peaks)
 bit-identical to the oracle's point-order float sums, and the spectroscopic image within
-relative L2 1e-6 of them (the device exp and glibc's may differ in the last bit of a term); and
+relative L2 1e-6 of them (the device takes each term's Gaussian from a ratio recurrence rather
+than one exp per term, so a term may differ in its last bits); and
 from the inputs alone, within relative L2 1e-5 of the FP64 oracle (FP64 emission, double sums).
 """
 import numpy as np
