@@ -406,6 +406,10 @@ constexpr int kViewsPerPass = 8;
 struct ViewTrigs {
   double cb[kViewsPerPass], sb[kViewsPerPass], cl[kViewsPerPass], sl[kViewsPerPass];
 };
+// SAMEB: every view of the launch has the same b, so the six products with cos b and sin b are
+// formed once per point (the same values and the same left-to-right order,
+// fomo-nearestneighbour.cpp:83-85).
+template <bool SAMEB>
 __global__ void __launch_bounds__(256) k_view_extents_multi(const float4* __restrict__ pts, int64_t n, ViewTrigs t,
                                                             int nv, double* __restrict__ partial, int64_t vstride) {
   double v[kViewsPerPass][6];
@@ -416,13 +420,15 @@ __global__ void __launch_bounds__(256) k_view_extents_multi(const float4* __rest
   for (int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < n; p += (int64_t)gridDim.x * blockDim.x) {
     const float4 q = pts[p];
     const double g0 = q.x, g1 = q.y, g2 = q.z;
+    const double gcb0 = g0 * t.cb[0], gcb1 = g1 * t.cb[0], gsb2 = g2 * t.sb[0];
+    const double gsb0 = g0 * t.sb[0], gsb1 = g1 * t.sb[0], gcb2 = g2 * t.cb[0];
 #pragma unroll
     for (int w = 0; w < kViewsPerPass; ++w) {
       if (w >= nv) break;
       const double cb = t.cb[w], sb = t.sb[w], cl = t.cl[w], sl = t.sl[w];
-      const double xr = g0 * cb * cl - g1 * cb * sl - g2 * sb;
+      const double xr = SAMEB ? gcb0 * cl - gcb1 * sl - gsb2 : g0 * cb * cl - g1 * cb * sl - g2 * sb;
       const double yr = g0 * sl + g1 * cl;
-      const double zr = g0 * sb * cl - g1 * sb * sl + g2 * cb;
+      const double zr = SAMEB ? gsb0 * cl - gsb1 * sl + gcb2 : g0 * sb * cl - g1 * sb * sl + g2 * cb;
       v[w][0] = fmin(v[w][0], xr); v[w][1] = fmax(v[w][1], xr);
       v[w][2] = fmin(v[w][2], yr); v[w][3] = fmax(v[w][3], yr);
       v[w][4] = fmin(v[w][4], zr); v[w][5] = fmax(v[w][5], zr);
@@ -2662,9 +2668,12 @@ unsigned enqueue_views_extents(hipStream_t s, const Dataset& ds, const std::vect
       t.cb[w] = cos(b[v0 + w]), t.sb[w] = sin(b[v0 + w]);
       t.cl[w] = cos(l[v0 + w]), t.sl[w] = sin(l[v0 + w]);
     }
+    bool sameb = true;
+    for (int w = 1; w < m; ++w) sameb = sameb && t.cb[w] == t.cb[0] && t.sb[w] == t.sb[0];
     ProfScope ps("k_view_extents_multi", s);
-    k_view_extents_multi<<<nb, kBlock, 0, s>>>(ds.pts.as<float4>(), ds.n, t, m,
-                                               partial + (int64_t)v0 * 6 * kMaxPartials, 6 * kMaxPartials);
+    double* const pv = partial + (int64_t)v0 * 6 * kMaxPartials;
+    if (sameb) k_view_extents_multi<true><<<nb, kBlock, 0, s>>>(ds.pts.as<float4>(), ds.n, t, m, pv, 6 * kMaxPartials);
+    else k_view_extents_multi<false><<<nb, kBlock, 0, s>>>(ds.pts.as<float4>(), ds.n, t, m, pv, 6 * kMaxPartials);
   }
   return nb;
 }
