@@ -568,14 +568,27 @@ __device__ __forceinline__ bool dominated(const double c[3], const double hw[3],
   return gmin > margin;
 }
 
-template <bool FILL>
+// MODE 0: count the candidates of every cell (cnt, -1 for cells left to the exact search);
+// MODE 1: write them, given those counts and the overflow lists' offsets (off);
+// MODE 2: one pass: write each record directly, cells with more than kRecSlots candidates taking
+// a block of kLookupCap entries of the overflow pool cand (pool_used: entries requested; a cell
+// whose block does not fit is left to the exact search).
+template <int MODE>
 __global__ void k_lookup_build(const float4* __restrict__ pts, const int32_t* __restrict__ cstart, GridInfo g,
                                LookupInfo lk, int64_t ncl, const uint8_t* __restrict__ near,
                                int32_t* __restrict__ cnt, const int64_t* __restrict__ off, float4* __restrict__ cand,
-                               float4* __restrict__ slot) {
+                               float4* __restrict__ slot, unsigned long long* __restrict__ pool_used,
+                               int64_t pool_cap) {
+  constexpr bool FILL = MODE == 1;
   const int64_t L = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (L >= ncl) return;
   if (FILL && cnt[L] <= 0) return;
+  float4* const rec = slot + (int64_t)kRecSlots * L;  // MODE 2
+  const float4 unused = make_float4(0.f, 0.f, 0.f, __int_as_float(-1));
+  const auto leave = [&]() {  // left to the exact search
+    if (MODE != 1) cnt[L] = -1;
+    if (MODE == 2) rec[0] = unused;
+  };
   const int lx = (int)(L % lk.dims[0]);
   const int ly = (int)((L / lk.dims[0]) % lk.dims[1]);
   const int lz = (int)(L / ((int64_t)lk.dims[0] * lk.dims[1]));
@@ -592,7 +605,7 @@ __global__ void k_lookup_build(const float4* __restrict__ pts, const int32_t* __
   const int dx = g.dims[0], dxy = g.dims[0] * g.dims[1];
   // no point within kLookupRings cells: leave the cell to the exact BVH query
   if (!near[(int64_t)cc2 * dxy + (int64_t)cc1 * dx + cc0]) {
-    if (!FILL) cnt[L] = -1;
+    leave();
     return;
   }
   // step 1: upper bound U and dominators
@@ -639,7 +652,7 @@ __global__ void k_lookup_build(const float4* __restrict__ pts, const int32_t* __
     if (found >= 0 && r >= found + 1) break;
   }
   if (nd == 0) {
-    if (!FILL) cnt[L] = -1;
+    leave();
     return;
   }
   const double Up = sqrt(U2) * (1.0 + 1e-9) + 1e-9 * lk.h[0];
@@ -651,10 +664,11 @@ __global__ void k_lookup_build(const float4* __restrict__ pts, const int32_t* __
     r1[a] = cell_coord(g, a, hi[a] + Up);
   }
   if ((int64_t)(r1[0] - r0[0] + 1) * (r1[1] - r0[1] + 1) * (r1[2] - r0[2] + 1) > 2197) {
-    if (!FILL) cnt[L] = -1;
+    leave();
     return;
   }
   int n = 0;
+  int64_t pbase = -1;  // MODE 2: this cell's block of the overflow pool
   // small lists go to the cell's own record, longer ones to the overflow list
   float4* const dst = FILL ? (cnt[L] <= kRecSlots ? slot + (int64_t)kRecSlots * L : cand + off[L]) : nullptr;
   for (int z = r0[2]; z <= r1[2]; ++z)
@@ -673,13 +687,44 @@ __global__ void k_lookup_build(const float4* __restrict__ pts, const int32_t* __
           const bool more = n == kSlots - 1 && cnt[L] > kSlots && cnt[L] <= kRecSlots;
           dst[n] = make_float4(q.x, q.y, q.z, __int_as_float(more ? -3 - p : p));
         }
+        if (MODE == 2) {
+          const float4 cq = make_float4(q.x, q.y, q.z, __int_as_float(p));
+          if (n < kRecSlots) {
+            rec[n] = cq;
+          } else {
+            if (n == kRecSlots) {
+              // the record overflows: a block of the pool (kLookupCap entries, whole 128-byte
+              // lines), starting with the candidates written so far
+              pbase = (int64_t)atomicAdd(pool_used, (unsigned long long)kLookupCap);
+              if (pbase + kLookupCap > pool_cap) {
+                leave();
+                return;
+              }
+              for (int i = 0; i < kRecSlots; ++i) cand[pbase + i] = rec[i];
+            }
+            if (n < kLookupCap) cand[pbase + n] = cq;
+          }
+        }
         ++n;
         if (!FILL && n > kLookupCap) {
-          cnt[L] = -1;
+          leave();
           return;
         }
       }
     }
+  if (MODE == 2) {
+    if (n == 0) {
+      rec[0] = unused;
+    } else if (n <= kRecSlots) {
+      for (int i = n; i < kRecSlots; ++i) rec[i] = unused;
+      if (n > kSlots) {
+        const float4 c3 = rec[kSlots - 1];
+        rec[kSlots - 1] = make_float4(c3.x, c3.y, c3.z, __int_as_float(-3 - __float_as_int(c3.w)));
+      }
+    } else {
+      rec[0] = make_float4(__int_as_float((int)pbase), __int_as_float(n), 0.f, __int_as_float(-2));
+    }
+  }
   if (!FILL) cnt[L] = n;
 }
 
@@ -702,7 +747,7 @@ __global__ void k_dilate_axis(const uint8_t* __restrict__ in, uint8_t* __restric
 }
 
 // Headers of the lookup cell records: the no-candidate and overflow markers, and w = -1 in the
-// slots a short list leaves unused (k_lookup_build<true> filled the others).
+// slots a short list leaves unused (the filling pass of the two-pass build wrote the others).
 __global__ void k_lookup_headers(const int32_t* __restrict__ cnt, const int64_t* __restrict__ off, int64_t ncl,
                                  float4* __restrict__ slot) {
   const int64_t L = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
@@ -2455,21 +2500,47 @@ int build_lookup(Dataset& ds) {
   DevBuf cnt, pos, off;
   if (cnt.alloc(sizeof(int32_t) * ncl) || pos.alloc(sizeof(int32_t) * ncl) || off.alloc(sizeof(int64_t) * ncl))
     return FOMO_OK;  // not enough memory: ring search only
-  {
-    ProfScope ps("k_lookup_build", s);
-    k_lookup_build<false><<<nblk(ncl), kBlock, 0, s>>>(ds.pts.as<float4>(), ds.cstart.as<int32_t>(), g, lk, ncl,
-                                                        near1.as<uint8_t>(), cnt.as<int32_t>(), nullptr, nullptr,
-                                                        nullptr);
-  }
-  // candidates in all (for the statistics), then the overflow lists' offsets
+  if (ds.lslot.alloc(sizeof(float4) * kRecSlots * ncl)) return FOMO_OK;
   size_t tmpb = 0;
   HIPOK(rocprim::exclusive_scan(nullptr, tmpb, pos.as<int32_t>(), off.as<int64_t>(), (int64_t)0, (size_t)ncl,
                                 rocprim::plus<int64_t>(), s));
   HIPOK(g_ctx.sorttmp.ensure(tmpb));
-  int64_t total = 0, ntotal = 0;
-  for (int pass = 0; pass < 2; ++pass) {
-    if (pass == 0) k_clamp_counts<<<nblk(ncl), kBlock, 0, s>>>(cnt.as<int32_t>(), ncl, pos.as<int32_t>());
-    else k_overflow_counts<<<nblk(ncl), kBlock, 0, s>>>(cnt.as<int32_t>(), ncl, pos.as<int32_t>());
+  // One pass, overflow lists bump-allocated from a pool with room for one cell in 8; if the pool
+  // cannot be had or runs out, the two-pass build (count, offsets, fill).
+  bool done = false;
+  {
+    const int64_t pool_cap = 3 * ncl;  // float4 entries: ncl / 8 blocks of kLookupCap
+    DevBuf pool, used;
+    if (pool_cap < (int64_t)INT32_MAX && !pool.alloc(sizeof(float4) * pool_cap) &&
+        !used.alloc(sizeof(unsigned long long))) {
+      HIPOK(hipMemsetAsync(used.p, 0, sizeof(unsigned long long), s));
+      {
+        ProfScope ps("k_lookup_build", s);
+        k_lookup_build<2><<<nblk(ncl), kBlock, 0, s>>>(ds.pts.as<float4>(), ds.cstart.as<int32_t>(), g, lk, ncl,
+                                                        near1.as<uint8_t>(), cnt.as<int32_t>(), nullptr,
+                                                        pool.as<float4>(), ds.lslot.as<float4>(),
+                                                        used.as<unsigned long long>(), pool_cap);
+      }
+      HIPOK(hipGetLastError());
+      unsigned long long pool_used = 0;
+      HIPOK(hipMemcpyAsync(&pool_used, used.p, sizeof pool_used, hipMemcpyDeviceToHost, s));
+      if (int rc = sync_stream()) return rc;
+      if ((int64_t)pool_used <= pool_cap && !ds.lcand.alloc(sizeof(float4) * std::max<int64_t>((int64_t)pool_used, 1))) {
+        if (pool_used)
+          HIPOK(hipMemcpyAsync(ds.lcand.p, pool.p, sizeof(float4) * pool_used, hipMemcpyDeviceToDevice, s));
+        done = true;
+      }
+    }
+  }
+  if (!done) {
+    {
+      ProfScope ps("k_lookup_build", s);
+      k_lookup_build<0><<<nblk(ncl), kBlock, 0, s>>>(ds.pts.as<float4>(), ds.cstart.as<int32_t>(), g, lk, ncl,
+                                                      near1.as<uint8_t>(), cnt.as<int32_t>(), nullptr, nullptr,
+                                                      nullptr, nullptr, 0);
+    }
+    // the overflow lists' offsets
+    k_overflow_counts<<<nblk(ncl), kBlock, 0, s>>>(cnt.as<int32_t>(), ncl, pos.as<int32_t>());
     HIPOK(hipGetLastError());
     HIPOK(rocprim::exclusive_scan(g_ctx.sorttmp.p, tmpb, pos.as<int32_t>(), off.as<int64_t>(), (int64_t)0,
                                   (size_t)ncl, rocprim::plus<int64_t>(), s));
@@ -2478,20 +2549,32 @@ int build_lookup(Dataset& ds) {
     HIPOK(hipMemcpyAsync(&last_off, off.as<int64_t>() + ncl - 1, sizeof(int64_t), hipMemcpyDeviceToHost, s));
     HIPOK(hipMemcpyAsync(&last_cnt, pos.as<int32_t>() + ncl - 1, sizeof(int32_t), hipMemcpyDeviceToHost, s));
     if (int rc = sync_stream()) return rc;
-    (pass == 0 ? ntotal : total) = last_off + last_cnt;
+    const int64_t total = last_off + last_cnt;
+    if (total >= (int64_t)INT32_MAX || ds.lcand.alloc(sizeof(float4) * std::max<int64_t>(total, 1))) {
+      ds.lslot.release();
+      ds.lcand.release();
+      return FOMO_OK;
+    }
+    k_lookup_build<1><<<nblk(ncl), kBlock, 0, s>>>(ds.pts.as<float4>(), ds.cstart.as<int32_t>(), g, lk, ncl,
+                                                    near1.as<uint8_t>(), cnt.as<int32_t>(), off.as<int64_t>(),
+                                                    ds.lcand.as<float4>(), ds.lslot.as<float4>(), nullptr, 0);
+    k_lookup_headers<<<nblk(ncl), kBlock, 0, s>>>(cnt.as<int32_t>(), off.as<int64_t>(), ncl, ds.lslot.as<float4>());
+    HIPOK(hipGetLastError());
   }
-  if (total >= (int64_t)INT32_MAX) return FOMO_OK;
-  if (ds.lcand.alloc(sizeof(float4) * std::max<int64_t>(total, 1)) ||
-      ds.lslot.alloc(sizeof(float4) * kRecSlots * ncl)) {
-    ds.lslot.release();
-    ds.lcand.release();
-    return FOMO_OK;
+  // candidates in all (statistics)
+  int64_t ntotal = 0;
+  {
+    k_clamp_counts<<<nblk(ncl), kBlock, 0, s>>>(cnt.as<int32_t>(), ncl, pos.as<int32_t>());
+    HIPOK(hipGetLastError());
+    HIPOK(rocprim::exclusive_scan(g_ctx.sorttmp.p, tmpb, pos.as<int32_t>(), off.as<int64_t>(), (int64_t)0,
+                                  (size_t)ncl, rocprim::plus<int64_t>(), s));
+    int64_t last_off = 0;
+    int32_t last_cnt = 0;
+    HIPOK(hipMemcpyAsync(&last_off, off.as<int64_t>() + ncl - 1, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+    HIPOK(hipMemcpyAsync(&last_cnt, pos.as<int32_t>() + ncl - 1, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    if (int rc = sync_stream()) return rc;
+    ntotal = last_off + last_cnt;
   }
-  k_lookup_build<true><<<nblk(ncl), kBlock, 0, s>>>(ds.pts.as<float4>(), ds.cstart.as<int32_t>(), g, lk, ncl,
-                                                     near1.as<uint8_t>(), cnt.as<int32_t>(), off.as<int64_t>(),
-                                                     ds.lcand.as<float4>(), ds.lslot.as<float4>());
-  k_lookup_headers<<<nblk(ncl), kBlock, 0, s>>>(cnt.as<int32_t>(), off.as<int64_t>(), ncl, ds.lslot.as<float4>());
-  HIPOK(hipGetLastError());
   int32_t nfb = 0;
   // number of cells left to the ring search
   int* dfb = reinterpret_cast<int*>(pos.p);
