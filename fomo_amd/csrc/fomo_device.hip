@@ -575,7 +575,7 @@ __device__ __forceinline__ bool dominated(const double c[3], const double hw[3],
 // whose block does not fit is left to the exact search).
 template <int MODE>
 __global__ void k_lookup_build(const float4* __restrict__ pts, const int32_t* __restrict__ cstart, GridInfo g,
-                               LookupInfo lk, int64_t ncl, const uint8_t* __restrict__ near,
+                               LookupInfo lk, int64_t ncl, const uint8_t* __restrict__ dist,
                                int32_t* __restrict__ cnt, const int64_t* __restrict__ off, float4* __restrict__ cand,
                                float4* __restrict__ slot, unsigned long long* __restrict__ pool_used,
                                int64_t pool_cap) {
@@ -603,8 +603,10 @@ __global__ void k_lookup_build(const float4* __restrict__ pts, const int32_t* __
   }
   const int cc0 = cell_coord(g, 0, c[0]), cc1 = cell_coord(g, 1, c[1]), cc2 = cell_coord(g, 2, c[2]);
   const int dx = g.dims[0], dxy = g.dims[0] * g.dims[1];
-  // no point within kLookupRings cells: leave the cell to the exact BVH query
-  if (!near[(int64_t)cc2 * dxy + (int64_t)cc1 * dx + cc0]) {
+  // no point within kLookupRings cells: leave the cell to the exact BVH query; rings closer than
+  // the distance to the nearest non-empty cell are empty and not visited
+  const int rfirst = dist[(int64_t)cc2 * dxy + (int64_t)cc1 * dx + cc0];
+  if (rfirst > kLookupRings) {
     leave();
     return;
   }
@@ -615,7 +617,7 @@ __global__ void k_lookup_build(const float4* __restrict__ pts, const int32_t* __
   double dkey[4];
   int nd = 0;
   int found = -1;
-  for (int r = 0; r <= kLookupRings; ++r) {
+  for (int r = rfirst; r <= kLookupRings; ++r) {
     for (int z = max(cc2 - r, 0); z <= min(cc2 + r, g.dims[2] - 1); ++z)
       for (int y = max(cc1 - r, 0); y <= min(cc1 + r, g.dims[1] - 1); ++y) {
         // cells at Chebyshev distance exactly r: the whole x-range on an edge row, else x = cc0 -/+ r
@@ -728,12 +730,13 @@ __global__ void k_lookup_build(const float4* __restrict__ pts, const int32_t* __
   if (!FILL) cnt[L] = n;
 }
 
-// near[c] = 1 when a non-empty cell lies within Chebyshev distance R of cell c (separable max filter)
-__global__ void k_occupancy(const int32_t* __restrict__ cstart, int64_t ncells, uint8_t* __restrict__ occ) {
+// dist[c] = Chebyshev distance (in cells) from cell c to the nearest non-empty cell, capped at
+// R + 1: a separable transform, per axis dist(c) = min over d of max(|d|, dist_prev(c + d)).
+__global__ void k_occupancy(const int32_t* __restrict__ cstart, int64_t ncells, int R, uint8_t* __restrict__ occ) {
   const int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (c < ncells) occ[c] = cstart[c + 1] > cstart[c] ? 1 : 0;
+  if (c < ncells) occ[c] = cstart[c + 1] > cstart[c] ? 0 : (uint8_t)(R + 1);
 }
-__global__ void k_dilate_axis(const uint8_t* __restrict__ in, uint8_t* __restrict__ out, GridInfo g, int axis, int R) {
+__global__ void k_dist_axis(const uint8_t* __restrict__ in, uint8_t* __restrict__ out, GridInfo g, int axis, int R) {
   const int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   const int64_t n = (int64_t)g.dims[0] * g.dims[1] * g.dims[2];
   if (c >= n) return;
@@ -741,9 +744,9 @@ __global__ void k_dilate_axis(const uint8_t* __restrict__ in, uint8_t* __restric
   const int coord = axis == 0 ? (int)(c % g.dims[0])
                               : (axis == 1 ? (int)((c / g.dims[0]) % g.dims[1]) : (int)(c / ((int64_t)g.dims[0] * g.dims[1])));
   const int lo = max(coord - R, 0) - coord, hi = min(coord + R, g.dims[axis] - 1) - coord;
-  uint8_t v = 0;
-  for (int d = lo; d <= hi && !v; ++d) v = in[c + d * stride];
-  out[c] = v;
+  int v = in[c];
+  for (int d = lo; d <= hi; ++d) v = min(v, max(abs(d), (int)in[c + d * stride]));
+  out[c] = (uint8_t)v;
 }
 
 // Headers of the lookup cell records: the no-candidate and overflow markers, and w = -1 in the
@@ -1155,6 +1158,211 @@ __global__ void __launch_bounds__(256) k_nn_far(const float4* __restrict__ pts, 
 __global__ void k_clear_u8(uint8_t* __restrict__ a, int64_t n) {
   const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (i < n) a[i] = 0;
+}
+
+// ------------------------------------------------------------------------------------
+// Far groups of the lookup grid. A lookup cell left to the exact query (no point within
+// kLookupRings cells, or too many candidates of its own) takes the candidate list of its group,
+// kFarGroup^3 cell-list cells, when that list is short. The list is built like a cell's, over the
+// group's box B: U = min over points of maxdist(B, q); every point with mindist(B, q) <= U that
+// none of the 4 points with the smallest maxdist dominates; then every survivor that another
+// survivor dominates is dropped as well (a dominated point is never the nearest to a target in
+// B, whichever point dominates it). Far from the data the Voronoi cells of the surface points
+// fan out, so a group of 4^3 cell-list cells keeps a dozen or so candidates, and the queries of
+// its samples need no octree walk. Points are enumerated with the octree, a wave of 64
+// neighbouring groups walking it together.
+// ------------------------------------------------------------------------------------
+constexpr int kFarGroup = 4;      // cell-list cells per group edge
+constexpr int kFarScratch = 128;  // survivors of the dominator test kept per group
+constexpr int kFarCap = 64;       // longest group list
+constexpr int kFarBlock = 64;     // groups per block: 4 x 4 x 4 neighbours, one wave
+
+struct FarGrid {
+  int gdim[3];   // groups per axis
+  int gcell[3];  // lookup cells per group edge
+  int bdim[3];   // blocks of 4 x 4 x 4 groups per axis
+};
+
+__device__ __forceinline__ double far_margin2(double u2, const LookupInfo& lk) {
+  const double u = sqrt(u2) * (1.0 + 1e-9) + 1e-9 * lk.h[0];
+  return u * u;
+}
+
+// Wave-cooperative walk over the nodes whose box comes within sqrt(r2) of the lane's box [lo, hi];
+// visit(first, count) runs on the leaves the lane's own test admits. A lane may lower r2 inside
+// visit (a negative r2 ends its part of the walk). All lanes of the wave must call it.
+template <class Visit>
+__device__ __forceinline__ void far_walk(const float4* __restrict__ nlo, const float4* __restrict__ nhi, int n,
+                                         const double (&lo)[3], const double (&hi)[3], const double& r2,
+                                         Visit&& visit) {
+  for (int e = 0; e < n;) {
+    e = __builtin_amdgcn_readfirstlane(e);
+    const float4 l = nlo[e];
+    const float4 h = nhi[e];
+    const int w = __float_as_int(l.w);
+    const double d0 = fmax(fmax((double)l.x - hi[0], lo[0] - (double)h.x), 0.0);
+    const double d1 = fmax(fmax((double)l.y - hi[1], lo[1] - (double)h.y), 0.0);
+    const double d2 = fmax(fmax((double)l.z - hi[2], lo[2] - (double)h.z), 0.0);
+    // a point of the node is no nearer to the box than the node's box is (margin: rounding)
+    const bool in = (d0 * d0 + d1 * d1) + d2 * d2 <= r2 * (1.0 + 1e-12);
+    if (w < 0) {
+      if (in) visit(__float_as_int(h.w), -w);
+      ++e;
+    } else {
+      e = __ballot(in) ? e + 1 : w;
+    }
+  }
+}
+
+// gflag[g] = 1 for the groups holding a lookup cell left to the exact query (cnt = -1)
+__global__ void k_far_flags(const int32_t* __restrict__ cnt, int64_t ncl, LookupInfo lk, FarGrid fg,
+                            uint8_t* __restrict__ gflag) {
+  const int64_t L = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (L >= ncl || cnt[L] != -1) return;
+  const int lx = (int)(L % lk.dims[0]);
+  const int ly = (int)((L / lk.dims[0]) % lk.dims[1]);
+  const int lz = (int)(L / ((int64_t)lk.dims[0] * lk.dims[1]));
+  const int64_t g = ((int64_t)(lz / fg.gcell[2]) * fg.gdim[1] + ly / fg.gcell[1]) * fg.gdim[0] + lx / fg.gcell[0];
+  gflag[g] = 1;
+}
+
+// Candidate lists of the flagged groups in blocks [b0, b0 + gridDim.x): ginfo[g] = (start in
+// fpool, count), count 0 when the list is too long or fpool is full. cbuf: kFarScratch entries
+// per lane of the launch.
+__global__ void __launch_bounds__(kFarBlock) k_far_groups(const float4* __restrict__ pts,
+                                                          const float4* __restrict__ mpts,
+                                                          const float4* __restrict__ nlo,
+                                                          const float4* __restrict__ nhi, int nodes, LookupInfo lk,
+                                                          FarGrid fg, int64_t b0, const uint8_t* __restrict__ gflag,
+                                                          float4* __restrict__ cbuf, float4* __restrict__ fpool,
+                                                          unsigned long long* __restrict__ fused, int64_t fcap,
+                                                          int2* __restrict__ ginfo) {
+  const int lane = threadIdx.x;
+  const int64_t b = b0 + blockIdx.x;
+  const int gi[3] = {(int)(b % fg.bdim[0]) * 4 + (lane & 3), (int)((b / fg.bdim[0]) % fg.bdim[1]) * 4 + ((lane >> 2) & 3),
+                     (int)(b / ((int64_t)fg.bdim[0] * fg.bdim[1])) * 4 + (lane >> 4)};
+  bool act = gi[0] < fg.gdim[0] && gi[1] < fg.gdim[1] && gi[2] < fg.gdim[2];
+  const int64_t g = act ? ((int64_t)gi[2] * fg.gdim[1] + gi[1]) * fg.gdim[0] + gi[0] : 0;
+  act = act && gflag[g];
+  if (__ballot(act) == 0) return;
+  double lo[3], hi[3], c[3], hw[3];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    const int i0 = gi[a] * fg.gcell[a], i1 = min(i0 + fg.gcell[a], lk.dims[a]);
+    lo[a] = lk.o[a] + (double)i0 * lk.h[a] - lk.eps;
+    hi[a] = lk.o[a] + (double)i1 * lk.h[a] + lk.eps;
+    c[a] = 0.5 * (lo[a] + hi[a]);
+    hw[a] = 0.5 * (hi[a] - lo[a]);
+  }
+  // first bound: the point a greedy descent from the box centre reaches
+  double U2 = INFINITY;
+  if (act) {
+    fomo::SearchBox sb;
+    sb.t0 = (float)c[0], sb.t1 = (float)c[1], sb.t2 = (float)c[2];
+    sb.lo0 = sb.lo1 = sb.lo2 = -INFINITY;
+    sb.hi0 = sb.hi1 = sb.hi2 = INFINITY;
+    double best = INFINITY;
+    int bpos = -1, borig = 0x7fffffff;
+    fomo::oct_descend(pts, mpts, nlo, nhi, sb, best, bpos, borig);
+    if (bpos >= 0) U2 = box_maxdist2(lo, hi, pts[bpos]);
+    act = bpos >= 0;
+  }
+  // pass 1: U and the 4 points with the smallest maxdist (any points would be sound dominators;
+  // these prune the most)
+  float4 dom[4];
+  int dpos[4];
+  double dkey[4];
+  int nd = 0;
+  double r2 = act ? far_margin2(U2, lk) : -1.0;
+  far_walk(nlo, nhi, nodes, lo, hi, r2, [&](int first, int count) {
+    for (int i = first; i < first + count; ++i) {
+      const float4 q = mpts[i];
+      if (box_mindist2(lo, hi, q) > r2) continue;
+      const double m2 = box_maxdist2(lo, hi, q);
+      U2 = fmin(U2, m2);
+      int k = nd < 4 ? nd++ : 4;
+      if (k == 4) {
+        if (m2 >= dkey[3]) continue;
+        k = 3;
+      }
+      while (k > 0 && dkey[k - 1] > m2) {
+        dkey[k] = dkey[k - 1];
+        dom[k] = dom[k - 1];
+        dpos[k] = dpos[k - 1];
+        --k;
+      }
+      dkey[k] = m2;
+      dom[k] = q;
+      dpos[k] = __float_as_int(q.w);
+      // beyond the 4th smallest maxdist a point neither lowers U nor joins the 4
+      r2 = far_margin2(nd == 4 ? dkey[3] : U2, lk);
+    }
+  });
+  // pass 2: every point within U (with margin) that none of the 4 dominates
+  const double Up2 = far_margin2(U2, lk);
+  float4* const cb = cbuf + ((int64_t)blockIdx.x * kFarBlock + lane) * kFarScratch;
+  int n = 0;
+  r2 = act ? Up2 : -1.0;
+  far_walk(nlo, nhi, nodes, lo, hi, r2, [&](int first, int count) {
+    for (int i = first; i < first + count && r2 >= 0.0; ++i) {
+      const float4 q = mpts[i];
+      if (box_mindist2(lo, hi, q) > Up2) continue;
+      const int p = __float_as_int(q.w);
+      bool dm = false;
+      for (int k = 0; k < nd && !dm; ++k)
+        if (dpos[k] != p && dominated(c, hw, q, dom[k])) dm = true;
+      if (dm) continue;
+      if (n == kFarScratch) {
+        r2 = -1.0;  // too many: the group stays with the exact query
+        n = kFarScratch + 1;
+        break;
+      }
+      cb[n++] = q;
+    }
+  });
+  if (!act) return;
+  int2 info = make_int2(0, 0);
+  if (n > 0 && n <= kFarScratch) {
+    // pass 3: drop the survivors another survivor dominates
+    uint64_t drop[kFarScratch / 64] = {};
+    int nk = n;
+    for (int a = 0; a < n; ++a) {
+      const float4 qa = cb[a];
+      for (int e = 0; e < n; ++e)
+        if (e != a && dominated(c, hw, qa, cb[e])) {
+          drop[a >> 6] |= 1ull << (a & 63);
+          --nk;
+          break;
+        }
+    }
+    if (nk > 0 && nk <= kFarCap) {
+      const int room = (nk + 7) & ~7;  // whole 128-byte lines
+      const int64_t pb = (int64_t)atomicAdd(fused, (unsigned long long)room);
+      if (pb + room <= fcap) {
+        int o = 0;
+        for (int a = 0; a < n; ++a)
+          if (!((drop[a >> 6] >> (a & 63)) & 1)) fpool[pb + o++] = cb[a];
+        info = make_int2((int)pb, nk);
+      }
+    }
+  }
+  ginfo[g] = info;
+}
+
+// Left cells of groups with a list: the record's header points at the group's list.
+__global__ void k_far_headers(int32_t* __restrict__ cnt, int64_t ncl, LookupInfo lk, FarGrid fg,
+                              const int2* __restrict__ ginfo, int64_t base, float4* __restrict__ slot) {
+  const int64_t L = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (L >= ncl || cnt[L] != -1) return;
+  const int lx = (int)(L % lk.dims[0]);
+  const int ly = (int)((L / lk.dims[0]) % lk.dims[1]);
+  const int lz = (int)(L / ((int64_t)lk.dims[0] * lk.dims[1]));
+  const int64_t g = ((int64_t)(lz / fg.gcell[2]) * fg.gdim[1] + ly / fg.gcell[1]) * fg.gdim[0] + lx / fg.gcell[0];
+  const int2 gi = ginfo[g];
+  if (gi.y <= 0) return;
+  slot[(int64_t)kRecSlots * L] =
+      make_float4(__int_as_float((int)(base + gi.x)), __int_as_float(gi.y), 0.f, __int_as_float(-2));
+  cnt[L] = gi.y;
 }
 
 // ------------------------------------------------------------------------------------
@@ -2445,6 +2653,65 @@ int build_tets(Dataset& ds) {
 // cell-list edge; 0 disables the fast path. The grid extends kLookupMargin cell-list cells beyond
 // the data bbox (so targets just outside the data, within a search box of them, take the fast
 // path too) and is shifted by 0.3 lookup cell off the cell lattice.
+// Far groups: candidate lists for the lookup cells left to the exact query (cnt = -1), one per
+// group of kFarGroup^3 cell-list cells, appended to lcand after its first nnear entries. Optional:
+// without the memory for it, or for groups with long lists, the cells keep the exact query.
+int build_far_groups(Dataset& ds, const LookupInfo& lk, int subdiv, int64_t ncl, int32_t* cnt, int64_t nnear) {
+  if (ds.bvh.nodes <= 0 || ds.bvh.nodes >= (int64_t)INT32_MAX) return FOMO_OK;
+  hipStream_t s = g_ctx.stream;
+  FarGrid fg{};
+  fg.gcell[0] = kFarGroup * std::max(1, subdiv / 2);  // lookup cells per cell-list cell along x
+  fg.gcell[1] = fg.gcell[2] = kFarGroup * subdiv;
+  int64_t ng = 1, nb = 1;
+  for (int a = 0; a < 3; ++a) {
+    fg.gdim[a] = (lk.dims[a] + fg.gcell[a] - 1) / fg.gcell[a];
+    fg.bdim[a] = (fg.gdim[a] + 3) / 4;
+    ng *= fg.gdim[a];
+    nb *= fg.bdim[a];
+  }
+  const int64_t chunk = std::min<int64_t>(nb, 4096);  // blocks per launch (scratch 512 MiB)
+  const int64_t fcap = std::min<int64_t>(16 * ng, (int64_t)INT32_MAX - 1 - nnear);
+  if (fcap <= 0) return FOMO_OK;
+  DevBuf gflag, ginfo, fcnt, fpool, cbuf;
+  if (gflag.alloc(ng) || ginfo.alloc(sizeof(int2) * ng) || fcnt.alloc(sizeof(unsigned long long)) ||
+      fpool.alloc(sizeof(float4) * fcap) || cbuf.alloc(sizeof(float4) * chunk * kFarBlock * kFarScratch)) {
+    (void)hipGetLastError();
+    return FOMO_OK;
+  }
+  HIPOK(hipMemsetAsync(gflag.p, 0, ng, s));
+  HIPOK(hipMemsetAsync(ginfo.p, 0, sizeof(int2) * ng, s));
+  HIPOK(hipMemsetAsync(fcnt.p, 0, sizeof(unsigned long long), s));
+  k_far_flags<<<nblk(ncl), kBlock, 0, s>>>(cnt, ncl, lk, fg, gflag.as<uint8_t>());
+  HIPOK(hipGetLastError());
+  {
+    ProfScope ps("k_far_groups", s);
+    for (int64_t b0 = 0; b0 < nb; b0 += chunk)
+      k_far_groups<<<(unsigned)std::min(chunk, nb - b0), kFarBlock, 0, s>>>(
+          ds.pts.as<float4>(), ds.mpts.as<float4>(), ds.nlo.as<float4>(), ds.nhi.as<float4>(), (int)ds.bvh.nodes, lk,
+          fg, b0, gflag.as<uint8_t>(), cbuf.as<float4>(), fpool.as<float4>(), fcnt.as<unsigned long long>(), fcap,
+          ginfo.as<int2>());
+  }
+  HIPOK(hipGetLastError());
+  unsigned long long used = 0;
+  HIPOK(hipMemcpyAsync(&used, fcnt.p, sizeof used, hipMemcpyDeviceToHost, s));
+  if (int rc = sync_stream()) return rc;
+  const int64_t nfar = std::min<int64_t>((int64_t)used, fcap);  // blocks past fcap were not written
+  if (nfar == 0) return FOMO_OK;
+  DevBuf lc;
+  if (lc.alloc(sizeof(float4) * (nnear + nfar))) {
+    (void)hipGetLastError();
+    return FOMO_OK;
+  }
+  if (nnear) HIPOK(hipMemcpyAsync(lc.p, ds.lcand.p, sizeof(float4) * nnear, hipMemcpyDeviceToDevice, s));
+  HIPOK(hipMemcpyAsync(lc.as<float4>() + nnear, fpool.p, sizeof(float4) * nfar, hipMemcpyDeviceToDevice, s));
+  k_far_headers<<<nblk(ncl), kBlock, 0, s>>>(cnt, ncl, lk, fg, ginfo.as<int2>(), nnear, ds.lslot.as<float4>());
+  HIPOK(hipGetLastError());
+  if (int rc = sync_stream()) return rc;
+  std::swap(ds.lcand.p, lc.p);
+  std::swap(ds.lcand.bytes, lc.bytes);
+  return FOMO_OK;
+}
+
 int build_lookup(Dataset& ds) {
   ds.lk = LookupInfo{};
   ds.lslot.release();
@@ -2489,13 +2756,13 @@ int build_lookup(Dataset& ds) {
   if (ncl > 24 * ds.n + 65536 || ncl > (1LL << 30)) return FOMO_OK;
   lk.eps = 1e-7 * lk.h[0] + 1e-12 * (mag + ext);
   lk.enabled = 1;
-  // cells of the cell list with a point within kLookupRings cells
+  // per cell of the cell list: Chebyshev distance to the nearest non-empty cell, capped
   DevBuf near0, near1;
   if (near0.alloc(ds.ncells) || near1.alloc(ds.ncells)) return FOMO_OK;
-  k_occupancy<<<nblk(ds.ncells), kBlock, 0, s>>>(ds.cstart.as<int32_t>(), ds.ncells, near0.as<uint8_t>());
-  k_dilate_axis<<<nblk(ds.ncells), kBlock, 0, s>>>(near0.as<uint8_t>(), near1.as<uint8_t>(), g, 0, kLookupRings);
-  k_dilate_axis<<<nblk(ds.ncells), kBlock, 0, s>>>(near1.as<uint8_t>(), near0.as<uint8_t>(), g, 1, kLookupRings);
-  k_dilate_axis<<<nblk(ds.ncells), kBlock, 0, s>>>(near0.as<uint8_t>(), near1.as<uint8_t>(), g, 2, kLookupRings);
+  k_occupancy<<<nblk(ds.ncells), kBlock, 0, s>>>(ds.cstart.as<int32_t>(), ds.ncells, kLookupRings, near0.as<uint8_t>());
+  k_dist_axis<<<nblk(ds.ncells), kBlock, 0, s>>>(near0.as<uint8_t>(), near1.as<uint8_t>(), g, 0, kLookupRings);
+  k_dist_axis<<<nblk(ds.ncells), kBlock, 0, s>>>(near1.as<uint8_t>(), near0.as<uint8_t>(), g, 1, kLookupRings);
+  k_dist_axis<<<nblk(ds.ncells), kBlock, 0, s>>>(near0.as<uint8_t>(), near1.as<uint8_t>(), g, 2, kLookupRings);
   HIPOK(hipGetLastError());
   DevBuf cnt, pos, off;
   if (cnt.alloc(sizeof(int32_t) * ncl) || pos.alloc(sizeof(int32_t) * ncl) || off.alloc(sizeof(int64_t) * ncl))
@@ -2508,6 +2775,7 @@ int build_lookup(Dataset& ds) {
   // One pass, overflow lists bump-allocated from a pool with room for one cell in 8; if the pool
   // cannot be had or runs out, the two-pass build (count, offsets, fill).
   bool done = false;
+  int64_t nnear = 0;  // entries of lcand in use
   {
     const int64_t pool_cap = 3 * ncl;  // float4 entries: ncl / 8 blocks of kLookupCap
     DevBuf pool, used;
@@ -2528,6 +2796,7 @@ int build_lookup(Dataset& ds) {
       if ((int64_t)pool_used <= pool_cap && !ds.lcand.alloc(sizeof(float4) * std::max<int64_t>((int64_t)pool_used, 1))) {
         if (pool_used)
           HIPOK(hipMemcpyAsync(ds.lcand.p, pool.p, sizeof(float4) * pool_used, hipMemcpyDeviceToDevice, s));
+        nnear = (int64_t)pool_used;
         done = true;
       }
     }
@@ -2560,7 +2829,9 @@ int build_lookup(Dataset& ds) {
                                                     ds.lcand.as<float4>(), ds.lslot.as<float4>(), nullptr, 0);
     k_lookup_headers<<<nblk(ncl), kBlock, 0, s>>>(cnt.as<int32_t>(), off.as<int64_t>(), ncl, ds.lslot.as<float4>());
     HIPOK(hipGetLastError());
+    nnear = total;
   }
+  if (int rc = build_far_groups(ds, lk, subdiv, ncl, cnt.as<int32_t>(), nnear)) return rc;
   // candidates in all (statistics)
   int64_t ntotal = 0;
   {

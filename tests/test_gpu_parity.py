@@ -437,3 +437,25 @@ def test_thomson_solar_disk_samples(engine):
     idx0, _ = F.nn_indices(ds, 0.9, 0.3, 64, 60, 48, thomson=True)
     assert np.array_equal(idx0, ridx)
     em.close(), ds.close()
+
+
+def test_far_group_lists_match_exact_query(engine):
+    """Lookup cells far from the data (beyond the shell, inside the disk hole) take their group's
+    candidate list; NN choices stay those of the exact octree query and of the oracle."""
+    F = engine
+    d = F.gen_corona(56, seed=5)
+    ds = F.Dataset(d["x"], d["y"], d["z"], [d["rho"]])
+    info = ds.lookup_info()
+    # without group lists every cell more than 8 cell-list cells from a point (the box corners
+    # beyond r = 3 and the middle of the hole) is left to the exact query
+    assert info["fallback_cells"] < 0.05 * info["cells"], info
+    views = [(0.0, np.pi / 2), (0.9, 0.3), (2.2, 0.0), (np.pi / 4, np.pi / 4)]
+    fast = [F.nn_indices(ds, l, b, 56, 52, 60, thomson=True)[0] for (l, b) in views]
+    ds.set_lookup(0)
+    exact = [F.nn_indices(ds, l, b, 56, 52, 60, thomson=True)[0] for (l, b) in views]
+    for (l, b), a, e in zip(views, fast, exact):
+        assert np.array_equal(a, e), f"view {(l, b)}: {(a != e).sum()} NN choices differ"
+    rt, rp = O.thomson_emission(1, d["x"], d["y"], d["z"], d["rho"])
+    ridx = O.render_thomson(1, d["x"], d["y"], d["z"], rt, rp, 0.9, 0.3, 56, 52, 60, want_idx=True)[1]
+    assert np.array_equal(fast[1], ridx)
+    ds.close()
