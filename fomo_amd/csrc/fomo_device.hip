@@ -545,6 +545,9 @@ constexpr int kSlots = 4;
 constexpr int kRecSlots = 8;
 constexpr int kLookupMargin = 6;  // cell-list cells the lookup grid extends beyond the data bbox
 constexpr int kLookupRings = kLookupMargin + 2;
+// cells farther than this from the nearest non-empty cell take their far group's list instead
+// of a list of their own (their own search box would span too many cells)
+constexpr int kLookupNearRings = 2;
 
 __device__ __forceinline__ double box_maxdist2(const double lo[3], const double hi[3], const float4 q) {
   const double a0 = fmax(fabs((double)q.x - lo[0]), fabs((double)q.x - hi[0]));
@@ -606,7 +609,7 @@ __global__ void k_lookup_build(const float4* __restrict__ pts, const int32_t* __
   // no point within kLookupRings cells: leave the cell to the exact BVH query; rings closer than
   // the distance to the nearest non-empty cell are empty and not visited
   const int rfirst = dist[(int64_t)cc2 * dxy + (int64_t)cc1 * dx + cc0];
-  if (rfirst > kLookupRings) {
+  if (rfirst > kLookupNearRings) {
     leave();
     return;
   }
@@ -1165,16 +1168,16 @@ __global__ void k_clear_u8(uint8_t* __restrict__ a, int64_t n) {
 // kLookupRings cells, or too many candidates of its own) takes the candidate list of its group,
 // kFarGroup^3 cell-list cells, when that list is short. The list is built like a cell's, over the
 // group's box B: U = min over points of maxdist(B, q); every point with mindist(B, q) <= U that
-// none of the 4 points with the smallest maxdist dominates; then every survivor that another
-// survivor dominates is dropped as well (a dominated point is never the nearest to a target in
-// B, whichever point dominates it). Far from the data the Voronoi cells of the surface points
-// fan out, so a group of 4^3 cell-list cells keeps a dozen or so candidates, and the queries of
-// its samples need no octree walk. Points are enumerated with the octree, a wave of 64
+// none of the 4 points with the smallest maxdist dominates and no other such point dominates (a
+// dominated point is never the nearest to a target in B, whichever point dominates it). Far from
+// the data the Voronoi cells of the surface points fan out, so a group of 4^3 cell-list cells
+// keeps a dozen or so candidates, and the queries of its samples need no octree walk. Inside a
+// concave hole they converge instead: near its middle the lists grow past kFarCap. Points are enumerated with the octree, a wave of 64
 // neighbouring groups walking it together.
 // ------------------------------------------------------------------------------------
 constexpr int kFarGroup = 4;      // cell-list cells per group edge
-constexpr int kFarScratch = 128;  // survivors of the dominator test kept per group
-constexpr int kFarCap = 64;       // longest group list
+constexpr int kFarCap = 256;      // longest group list
+constexpr int kFarVisit = 65536;  // points within U a group may examine (more: exact query)
 constexpr int kFarBlock = 64;     // groups per block: 4 x 4 x 4 neighbours, one wave
 
 struct FarGrid {
@@ -1227,8 +1230,8 @@ __global__ void k_far_flags(const int32_t* __restrict__ cnt, int64_t ncl, Lookup
 }
 
 // Candidate lists of the flagged groups in blocks [b0, b0 + gridDim.x): ginfo[g] = (start in
-// fpool, count), count 0 when the list is too long or fpool is full. cbuf: kFarScratch entries
-// per lane of the launch.
+// fpool, count), count 0 when the list is too long or fpool is full. cbuf: kFarCap entries per
+// lane of the launch.
 __global__ void __launch_bounds__(kFarBlock) k_far_groups(const float4* __restrict__ pts,
                                                           const float4* __restrict__ mpts,
                                                           const float4* __restrict__ nlo,
@@ -1298,23 +1301,36 @@ __global__ void __launch_bounds__(kFarBlock) k_far_groups(const float4* __restri
       r2 = far_margin2(nd == 4 ? dkey[3] : U2, lk);
     }
   });
-  // pass 2: every point within U (with margin) that none of the 4 dominates
+  // pass 2: the points within U (with margin) that none of the 4 dominates, pruned pairwise as
+  // they come: a point a kept one dominates is skipped, and a new point drops the kept ones it
+  // dominates, so the list never holds more than its final size plus the points still to lose
   const double Up2 = far_margin2(U2, lk);
-  float4* const cb = cbuf + ((int64_t)blockIdx.x * kFarBlock + lane) * kFarScratch;
-  int n = 0;
+  float4* const cb = cbuf + ((int64_t)blockIdx.x * kFarBlock + lane) * kFarCap;
+  int n = 0, seen = 0;
   r2 = act ? Up2 : -1.0;
   far_walk(nlo, nhi, nodes, lo, hi, r2, [&](int first, int count) {
     for (int i = first; i < first + count && r2 >= 0.0; ++i) {
       const float4 q = mpts[i];
       if (box_mindist2(lo, hi, q) > Up2) continue;
+      if (++seen > kFarVisit) {
+        r2 = -1.0;  // the middle of a concave hole: every point of its rim is near
+        n = kFarCap + 1;
+        break;
+      }
       const int p = __float_as_int(q.w);
       bool dm = false;
       for (int k = 0; k < nd && !dm; ++k)
         if (dpos[k] != p && dominated(c, hw, q, dom[k])) dm = true;
+      for (int e = 0; e < n && !dm; ++e)
+        if (dominated(c, hw, q, cb[e])) dm = true;
       if (dm) continue;
-      if (n == kFarScratch) {
+      for (int e = 0; e < n;) {
+        if (dominated(c, hw, cb[e], q)) cb[e] = cb[--n];
+        else ++e;
+      }
+      if (n == kFarCap) {
         r2 = -1.0;  // too many: the group stays with the exact query
-        n = kFarScratch + 1;
+        n = kFarCap + 1;
         break;
       }
       cb[n++] = q;
@@ -1322,28 +1338,12 @@ __global__ void __launch_bounds__(kFarBlock) k_far_groups(const float4* __restri
   });
   if (!act) return;
   int2 info = make_int2(0, 0);
-  if (n > 0 && n <= kFarScratch) {
-    // pass 3: drop the survivors another survivor dominates
-    uint64_t drop[kFarScratch / 64] = {};
-    int nk = n;
-    for (int a = 0; a < n; ++a) {
-      const float4 qa = cb[a];
-      for (int e = 0; e < n; ++e)
-        if (e != a && dominated(c, hw, qa, cb[e])) {
-          drop[a >> 6] |= 1ull << (a & 63);
-          --nk;
-          break;
-        }
-    }
-    if (nk > 0 && nk <= kFarCap) {
-      const int room = (nk + 7) & ~7;  // whole 128-byte lines
-      const int64_t pb = (int64_t)atomicAdd(fused, (unsigned long long)room);
-      if (pb + room <= fcap) {
-        int o = 0;
-        for (int a = 0; a < n; ++a)
-          if (!((drop[a >> 6] >> (a & 63)) & 1)) fpool[pb + o++] = cb[a];
-        info = make_int2((int)pb, nk);
-      }
+  if (n > 0 && n <= kFarCap) {
+    const int room = (n + 7) & ~7;  // whole 128-byte lines
+    const int64_t pb = (int64_t)atomicAdd(fused, (unsigned long long)room);
+    if (pb + room <= fcap) {
+      for (int a = 0; a < n; ++a) fpool[pb + a] = cb[a];
+      info = make_int2((int)pb, n);
     }
   }
   ginfo[g] = info;
@@ -2669,12 +2669,12 @@ int build_far_groups(Dataset& ds, const LookupInfo& lk, int subdiv, int64_t ncl,
     ng *= fg.gdim[a];
     nb *= fg.bdim[a];
   }
-  const int64_t chunk = std::min<int64_t>(nb, 4096);  // blocks per launch (scratch 512 MiB)
-  const int64_t fcap = std::min<int64_t>(16 * ng, (int64_t)INT32_MAX - 1 - nnear);
+  const int64_t chunk = std::min<int64_t>(nb, 2048);  // blocks per launch (scratch 512 MiB)
+  const int64_t fcap = std::min<int64_t>(32 * ng, (int64_t)INT32_MAX - 1 - nnear);
   if (fcap <= 0) return FOMO_OK;
   DevBuf gflag, ginfo, fcnt, fpool, cbuf;
   if (gflag.alloc(ng) || ginfo.alloc(sizeof(int2) * ng) || fcnt.alloc(sizeof(unsigned long long)) ||
-      fpool.alloc(sizeof(float4) * fcap) || cbuf.alloc(sizeof(float4) * chunk * kFarBlock * kFarScratch)) {
+      fpool.alloc(sizeof(float4) * fcap) || cbuf.alloc(sizeof(float4) * chunk * kFarBlock * kFarCap)) {
     (void)hipGetLastError();
     return FOMO_OK;
   }
