@@ -1191,13 +1191,40 @@ __device__ __forceinline__ double far_margin2(double u2, const LookupInfo& lk) {
   return u * u;
 }
 
+// Node [l, h] dominated by p over the box (center c, half widths hw): every point q of the node
+// is farther than p from every t in the box, min over t, q of |t - q|^2 - |t - p|^2 > 0. The
+// expression separates by axis into d(t)^2 - (t - p)^2 with d the distance from t to the node's
+// interval: linear in t outside that interval and concave inside it, so its minimum over the
+// box's interval lies at the interval's ends or at the node's ends between them. Relative margin
+// 1e-10 as in dominated().
+__device__ __forceinline__ bool node_dominated(const double c[3], const double hw[3], const float4 l, const float4 h,
+                                               const float4 p) {
+  const float ln[3] = {l.x, l.y, l.z}, hn[3] = {h.x, h.y, h.z}, pp[3] = {p.x, p.y, p.z};
+  double s = 0.0, scale = 0.0;
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    const double nl = (double)ln[a] - c[a], nh = (double)hn[a] - c[a], pa = (double)pp[a] - c[a];
+    const auto f = [&](double t) {
+      const double d = fmax(fmax(nl - t, t - nh), 0.0);
+      return d * d - (t - pa) * (t - pa);
+    };
+    double m = fmin(f(-hw[a]), f(hw[a]));
+    if (nl > -hw[a] && nl < hw[a]) m = fmin(m, f(nl));
+    if (nh > -hw[a] && nh < hw[a]) m = fmin(m, f(nh));
+    s += m;
+    scale += 4.0 * (hw[a] * hw[a] + nl * nl + nh * nh + pa * pa);
+  }
+  return s > 1e-10 * scale;
+}
+
 // Wave-cooperative walk over the nodes whose box comes within sqrt(r2) of the lane's box [lo, hi];
-// visit(first, count) runs on the leaves the lane's own test admits. A lane may lower r2 inside
-// visit (a negative r2 ends its part of the walk). All lanes of the wave must call it.
-template <class Visit>
+// admit(l, h) may reject a node the distance test lets in; visit(first, count) runs on the leaves
+// the lane's own tests admit. A lane may lower r2 inside visit (a negative r2 ends its part of the
+// walk). All lanes of the wave must call it.
+template <class Admit, class Visit>
 __device__ __forceinline__ void far_walk(const float4* __restrict__ nlo, const float4* __restrict__ nhi, int n,
                                          const double (&lo)[3], const double (&hi)[3], const double& r2,
-                                         Visit&& visit) {
+                                         Admit&& admit, Visit&& visit) {
   for (int e = 0; e < n;) {
     e = __builtin_amdgcn_readfirstlane(e);
     const float4 l = nlo[e];
@@ -1207,7 +1234,7 @@ __device__ __forceinline__ void far_walk(const float4* __restrict__ nlo, const f
     const double d1 = fmax(fmax((double)l.y - hi[1], lo[1] - (double)h.y), 0.0);
     const double d2 = fmax(fmax((double)l.z - hi[2], lo[2] - (double)h.z), 0.0);
     // a point of the node is no nearer to the box than the node's box is (margin: rounding)
-    const bool in = (d0 * d0 + d1 * d1) + d2 * d2 <= r2 * (1.0 + 1e-12);
+    const bool in = (d0 * d0 + d1 * d1) + d2 * d2 <= r2 * (1.0 + 1e-12) && admit(l, h);
     if (w < 0) {
       if (in) visit(__float_as_int(h.w), -w);
       ++e;
@@ -1277,7 +1304,7 @@ __global__ void __launch_bounds__(kFarBlock) k_far_groups(const float4* __restri
   double dkey[4];
   int nd = 0;
   double r2 = act ? far_margin2(U2, lk) : -1.0;
-  far_walk(nlo, nhi, nodes, lo, hi, r2, [&](int first, int count) {
+  far_walk(nlo, nhi, nodes, lo, hi, r2, [](const float4&, const float4&) { return true; }, [&](int first, int count) {
     for (int i = first; i < first + count; ++i) {
       const float4 q = mpts[i];
       if (box_mindist2(lo, hi, q) > r2) continue;
@@ -1301,14 +1328,19 @@ __global__ void __launch_bounds__(kFarBlock) k_far_groups(const float4* __restri
       r2 = far_margin2(nd == 4 ? dkey[3] : U2, lk);
     }
   });
-  // pass 2: the points within U (with margin) that none of the 4 dominates, pruned pairwise as
-  // they come: a point a kept one dominates is skipped, and a new point drops the kept ones it
+  // pass 2: the points within U (with margin) that none of the 4 dominates (whole subtrees one of
+  // them dominates are skipped), pruned pairwise as they come: a point a kept one dominates is skipped, and a new point drops the kept ones it
   // dominates, so the list never holds more than its final size plus the points still to lose
   const double Up2 = far_margin2(U2, lk);
   float4* const cb = cbuf + ((int64_t)blockIdx.x * kFarBlock + lane) * kFarCap;
   int n = 0, seen = 0;
   r2 = act ? Up2 : -1.0;
-  far_walk(nlo, nhi, nodes, lo, hi, r2, [&](int first, int count) {
+  const auto admit = [&](const float4& l, const float4& h) {
+    for (int k = 0; k < nd; ++k)
+      if (node_dominated(c, hw, l, h, dom[k])) return false;
+    return true;
+  };
+  far_walk(nlo, nhi, nodes, lo, hi, r2, admit, [&](int first, int count) {
     for (int i = first; i < first + count && r2 >= 0.0; ++i) {
       const float4 q = mpts[i];
       if (box_mindist2(lo, hi, q) > Up2) continue;
