@@ -40,6 +40,7 @@
 #include "arena.h"
 #include "delaunay.h"
 #include "host_copy.h"
+#include "lookup_geom.h"
 #include "octree.h"
 #include "spectral_exp.h"
 
@@ -549,27 +550,10 @@ constexpr int kLookupRings = kLookupMargin + 2;
 // of a list of their own (their own search box would span too many cells)
 constexpr int kLookupNearRings = 2;
 
-__device__ __forceinline__ double box_maxdist2(const double lo[3], const double hi[3], const float4 q) {
-  const double a0 = fmax(fabs((double)q.x - lo[0]), fabs((double)q.x - hi[0]));
-  const double a1 = fmax(fabs((double)q.y - lo[1]), fabs((double)q.y - hi[1]));
-  const double a2 = fmax(fabs((double)q.z - lo[2]), fabs((double)q.z - hi[2]));
-  return a0 * a0 + a1 * a1 + a2 * a2;
-}
-__device__ __forceinline__ double box_mindist2(const double lo[3], const double hi[3], const float4 q) {
-  const double a0 = fmax(0.0, fmax(lo[0] - (double)q.x, (double)q.x - hi[0]));
-  const double a1 = fmax(0.0, fmax(lo[1] - (double)q.y, (double)q.y - hi[1]));
-  const double a2 = fmax(0.0, fmax(lo[2] - (double)q.z, (double)q.z - hi[2]));
-  return a0 * a0 + a1 * a1 + a2 * a2;
-}
-// q dominated by p over the box (center c, half widths hw): every t in the box is closer to p
-__device__ __forceinline__ bool dominated(const double c[3], const double hw[3], const float4 q, const float4 p) {
-  const double u0 = (double)q.x - c[0], u1 = (double)q.y - c[1], u2 = (double)q.z - c[2];
-  const double w0 = (double)p.x - c[0], w1 = (double)p.y - c[1], w2 = (double)p.z - c[2];
-  const double uu = u0 * u0 + u1 * u1 + u2 * u2, ww = w0 * w0 + w1 * w1 + w2 * w2;
-  const double gmin = (uu - ww) - 2.0 * (hw[0] * fabs(u0 - w0) + hw[1] * fabs(u1 - w1) + hw[2] * fabs(u2 - w2));
-  const double margin = 1e-10 * (uu + ww + 4.0 * (hw[0] * hw[0] + hw[1] * hw[1] + hw[2] * hw[2]));
-  return gmin > margin;
-}
+using fomo::box_maxdist2;
+using fomo::box_mindist2;
+using fomo::dominated;
+using fomo::node_dominated;
 
 // MODE 0: count the candidates of every cell (cnt, -1 for cells left to the exact search);
 // MODE 1: write them, given those counts and the overflow lists' offsets (off);
@@ -1189,32 +1173,6 @@ struct FarGrid {
 __device__ __forceinline__ double far_margin2(double u2, const LookupInfo& lk) {
   const double u = sqrt(u2) * (1.0 + 1e-9) + 1e-9 * lk.h[0];
   return u * u;
-}
-
-// Node [l, h] dominated by p over the box (center c, half widths hw): every point q of the node
-// is farther than p from every t in the box, min over t, q of |t - q|^2 - |t - p|^2 > 0. The
-// expression separates by axis into d(t)^2 - (t - p)^2 with d the distance from t to the node's
-// interval: linear in t outside that interval and concave inside it, so its minimum over the
-// box's interval lies at the interval's ends or at the node's ends between them. Relative margin
-// 1e-10 as in dominated().
-__device__ __forceinline__ bool node_dominated(const double c[3], const double hw[3], const float4 l, const float4 h,
-                                               const float4 p) {
-  const float ln[3] = {l.x, l.y, l.z}, hn[3] = {h.x, h.y, h.z}, pp[3] = {p.x, p.y, p.z};
-  double s = 0.0, scale = 0.0;
-#pragma unroll
-  for (int a = 0; a < 3; ++a) {
-    const double nl = (double)ln[a] - c[a], nh = (double)hn[a] - c[a], pa = (double)pp[a] - c[a];
-    const auto f = [&](double t) {
-      const double d = fmax(fmax(nl - t, t - nh), 0.0);
-      return d * d - (t - pa) * (t - pa);
-    };
-    double m = fmin(f(-hw[a]), f(hw[a]));
-    if (nl > -hw[a] && nl < hw[a]) m = fmin(m, f(nl));
-    if (nh > -hw[a] && nh < hw[a]) m = fmin(m, f(nh));
-    s += m;
-    scale += 4.0 * (hw[a] * hw[a] + nl * nl + nh * nh + pa * pa);
-  }
-  return s > 1e-10 * scale;
 }
 
 // Wave-cooperative walk over the nodes whose box comes within sqrt(r2) of the lane's box [lo, hi];
