@@ -1148,16 +1148,17 @@ __global__ void k_clear_u8(uint8_t* __restrict__ a, int64_t n) {
 }
 
 // ------------------------------------------------------------------------------------
-// Far groups of the lookup grid. A lookup cell left to the exact query (no point within
-// kLookupRings cells, or too many candidates of its own) takes the candidate list of its group,
-// kFarGroup^3 cell-list cells, when that list is short. The list is built like a cell's, over the
-// group's box B: U = min over points of maxdist(B, q); every point with mindist(B, q) <= U that
-// none of the 4 points with the smallest maxdist dominates and no other such point dominates (a
-// dominated point is never the nearest to a target in B, whichever point dominates it). Far from
-// the data the Voronoi cells of the surface points fan out, so a group of 4^3 cell-list cells
-// keeps a dozen or so candidates, and the queries of its samples need no octree walk. Inside a
-// concave hole they converge instead: near its middle the lists grow past kFarCap. Points are enumerated with the octree, a wave of 64
-// neighbouring groups walking it together.
+// Far groups of the lookup grid. A lookup cell left to the exact query (more than
+// kLookupNearRings cells from any point, or too many candidates of its own) takes the candidate
+// list of its group, kFarGroup^3 cell-list cells, when that list is short. The list is built like
+// a cell's, over the group's box B: U = the smallest maxdist(B, q) among the points nearest to B's
+// centre; every point with mindist(B, q) <= U that none of those 4 points dominates and no other
+// such point dominates (a dominated point is never the nearest to a target in B, whichever point
+// dominates it). Far from the data the Voronoi cells of the surface points fan out, so a group of
+// 4^3 cell-list cells keeps a dozen or so candidates, and the queries of its samples need no
+// octree walk. Inside a concave hole they converge instead: near its middle the lists grow past
+// kFarCap. Points are enumerated with the octree, a wave of 64 neighbouring groups walking it
+// together.
 // ------------------------------------------------------------------------------------
 constexpr int kFarGroup = 4;      // cell-list cells per group edge
 constexpr int kFarCap = 256;      // longest group list
@@ -1243,52 +1244,57 @@ __global__ void __launch_bounds__(kFarBlock) k_far_groups(const float4* __restri
     hw[a] = 0.5 * (hi[a] - lo[a]);
   }
   // first bound: the point a greedy descent from the box centre reaches
-  double U2 = INFINITY;
+  double U2 = INFINITY, dd = INFINITY;
   if (act) {
     fomo::SearchBox sb;
     sb.t0 = (float)c[0], sb.t1 = (float)c[1], sb.t2 = (float)c[2];
     sb.lo0 = sb.lo1 = sb.lo2 = -INFINITY;
     sb.hi0 = sb.hi1 = sb.hi2 = INFINITY;
-    double best = INFINITY;
     int bpos = -1, borig = 0x7fffffff;
-    fomo::oct_descend(pts, mpts, nlo, nhi, sb, best, bpos, borig);
+    fomo::oct_descend(pts, mpts, nlo, nhi, sb, dd, bpos, borig);
     if (bpos >= 0) U2 = box_maxdist2(lo, hi, pts[bpos]);
     act = bpos >= 0;
   }
-  // pass 1: U and the 4 points with the smallest maxdist (any points would be sound dominators;
-  // these prune the most)
+  // pass 1: the 4 points nearest to the centre, within one cell edge beyond the descent's point
+  // (a walk like a 4-NN query), and U = the smallest maxdist among them. Any point's maxdist bounds
+  // U and any points are sound dominators; the nearest to the centre prune the most.
   float4 dom[4];
   int dpos[4];
   double dkey[4];
   int nd = 0;
-  double r2 = act ? far_margin2(U2, lk) : -1.0;
-  far_walk(nlo, nhi, nodes, lo, hi, r2, [](const float4&, const float4&) { return true; }, [&](int first, int count) {
+  const double ctr[3] = {c[0], c[1], c[2]};  // a box of no extent: the walk's test is the distance to c
+  double r2 = -1.0;
+  if (act) {
+    const double r0 = sqrt(dd) + lk.h[0];
+    r2 = r0 * r0;
+  }
+  far_walk(nlo, nhi, nodes, ctr, ctr, r2, [](const float4&, const float4&) { return true; }, [&](int first, int count) {
     for (int i = first; i < first + count; ++i) {
       const float4 q = mpts[i];
-      if (box_mindist2(lo, hi, q) > r2) continue;
-      const double m2 = box_maxdist2(lo, hi, q);
-      U2 = fmin(U2, m2);
+      const double d2 = box_mindist2(ctr, ctr, q);
+      if (d2 > r2) continue;
+      U2 = fmin(U2, box_maxdist2(lo, hi, q));
       int k = nd < 4 ? nd++ : 4;
       if (k == 4) {
-        if (m2 >= dkey[3]) continue;
+        if (d2 >= dkey[3]) continue;
         k = 3;
       }
-      while (k > 0 && dkey[k - 1] > m2) {
+      while (k > 0 && dkey[k - 1] > d2) {
         dkey[k] = dkey[k - 1];
         dom[k] = dom[k - 1];
         dpos[k] = dpos[k - 1];
         --k;
       }
-      dkey[k] = m2;
+      dkey[k] = d2;
       dom[k] = q;
       dpos[k] = __float_as_int(q.w);
-      // beyond the 4th smallest maxdist a point neither lowers U nor joins the 4
-      r2 = far_margin2(nd == 4 ? dkey[3] : U2, lk);
+      if (nd == 4) r2 = fmin(r2, dkey[3]);
     }
   });
   // pass 2: the points within U (with margin) that none of the 4 dominates (whole subtrees one of
-  // them dominates are skipped), pruned pairwise as they come: a point a kept one dominates is skipped, and a new point drops the kept ones it
-  // dominates, so the list never holds more than its final size plus the points still to lose
+  // them dominates are skipped), pruned pairwise as they come: a point a kept one dominates is
+  // skipped, and a new point drops the kept ones it dominates, so the list never holds more than
+  // its final size plus the points still to lose
   const double Up2 = far_margin2(U2, lk);
   float4* const cb = cbuf + ((int64_t)blockIdx.x * kFarBlock + lane) * kFarCap;
   int n = 0, seen = 0;
