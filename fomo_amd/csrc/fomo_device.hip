@@ -545,10 +545,12 @@ constexpr int kLookupCap = 24;
 constexpr int kSlots = 4;
 constexpr int kRecSlots = 8;
 constexpr int kLookupMargin = 6;  // cell-list cells the lookup grid extends beyond the data bbox
-constexpr int kLookupRings = kLookupMargin + 2;
 // cells farther than this from the nearest non-empty cell take their far group's list instead
 // of a list of their own (their own search box would span too many cells)
 constexpr int kLookupNearRings = 2;
+// cap of the cell list's distance transform: the ring search of a cell at distance d visits rings
+// d and d + 1
+constexpr int kLookupDistCap = kLookupNearRings + 1;
 
 using fomo::box_maxdist2;
 using fomo::box_mindist2;
@@ -590,8 +592,8 @@ __global__ void k_lookup_build(const float4* __restrict__ pts, const int32_t* __
   }
   const int cc0 = cell_coord(g, 0, c[0]), cc1 = cell_coord(g, 1, c[1]), cc2 = cell_coord(g, 2, c[2]);
   const int dx = g.dims[0], dxy = g.dims[0] * g.dims[1];
-  // no point within kLookupRings cells: leave the cell to the exact BVH query; rings closer than
-  // the distance to the nearest non-empty cell are empty and not visited
+  // no point within kLookupNearRings cells: leave the cell to its far group or the exact query;
+  // rings closer than the distance to the nearest non-empty cell are empty and not visited
   const int rfirst = dist[(int64_t)cc2 * dxy + (int64_t)cc1 * dx + cc0];
   if (rfirst > kLookupNearRings) {
     leave();
@@ -604,7 +606,7 @@ __global__ void k_lookup_build(const float4* __restrict__ pts, const int32_t* __
   double dkey[4];
   int nd = 0;
   int found = -1;
-  for (int r = rfirst; r <= kLookupRings; ++r) {
+  for (int r = rfirst; r <= kLookupDistCap; ++r) {
     for (int z = max(cc2 - r, 0); z <= min(cc2 + r, g.dims[2] - 1); ++z)
       for (int y = max(cc1 - r, 0); y <= min(cc1 + r, g.dims[1] - 1); ++y) {
         // cells at Chebyshev distance exactly r: the whole x-range on an edge row, else x = cc0 -/+ r
@@ -2755,10 +2757,10 @@ int build_lookup(Dataset& ds) {
   // per cell of the cell list: Chebyshev distance to the nearest non-empty cell, capped
   DevBuf near0, near1;
   if (near0.alloc(ds.ncells) || near1.alloc(ds.ncells)) return FOMO_OK;
-  k_occupancy<<<nblk(ds.ncells), kBlock, 0, s>>>(ds.cstart.as<int32_t>(), ds.ncells, kLookupRings, near0.as<uint8_t>());
-  k_dist_axis<<<nblk(ds.ncells), kBlock, 0, s>>>(near0.as<uint8_t>(), near1.as<uint8_t>(), g, 0, kLookupRings);
-  k_dist_axis<<<nblk(ds.ncells), kBlock, 0, s>>>(near1.as<uint8_t>(), near0.as<uint8_t>(), g, 1, kLookupRings);
-  k_dist_axis<<<nblk(ds.ncells), kBlock, 0, s>>>(near0.as<uint8_t>(), near1.as<uint8_t>(), g, 2, kLookupRings);
+  k_occupancy<<<nblk(ds.ncells), kBlock, 0, s>>>(ds.cstart.as<int32_t>(), ds.ncells, kLookupDistCap, near0.as<uint8_t>());
+  k_dist_axis<<<nblk(ds.ncells), kBlock, 0, s>>>(near0.as<uint8_t>(), near1.as<uint8_t>(), g, 0, kLookupDistCap);
+  k_dist_axis<<<nblk(ds.ncells), kBlock, 0, s>>>(near1.as<uint8_t>(), near0.as<uint8_t>(), g, 1, kLookupDistCap);
+  k_dist_axis<<<nblk(ds.ncells), kBlock, 0, s>>>(near0.as<uint8_t>(), near1.as<uint8_t>(), g, 2, kLookupDistCap);
   HIPOK(hipGetLastError());
   DevBuf cnt, pos, off;
   if (cnt.alloc(sizeof(int32_t) * ncl) || pos.alloc(sizeof(int32_t) * ncl) || off.alloc(sizeof(int64_t) * ncl))
