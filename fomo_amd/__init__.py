@@ -219,6 +219,8 @@ class Dataset:
         _check(lib().fomo_dataset_set_lookup(self.id, subdiv))
 
     def lookup_info(self):
+        """Candidate lookup grid: active subdivision (0 = off), cells, candidates summed over the
+        cells (a far group's shared list once per cell using it), cells left to the exact query."""
         sd = ctypes.c_int()
         cells, cand, fb = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64()
         _check(lib().fomo_dataset_lookup_info(self.id, ctypes.byref(sd), ctypes.byref(cells), ctypes.byref(cand),

@@ -59,7 +59,8 @@ int fomo_dataset_reindex(int id, double cell_scale);
  * 0 disables it, leaving the exact ring search over the cell list. Results are identical either
  * way; only speed and memory change. */
 int fomo_dataset_set_lookup(int id, int subdiv);
-/* Active subdivision (0 = disabled), lookup cells, stored candidates, cells left to the ring search. */
+/* Active subdivision (0 = disabled), lookup cells, candidates summed over the cells (a far group's
+   shared list counts once per cell that uses it), cells left to the exact octree query. */
 int fomo_dataset_lookup_info(int id, int* subdiv, int64_t* cells, int64_t* candidates, int64_t* fallback_cells);
 
 /* ---------------------------------------------------------------------------------------
