@@ -132,6 +132,10 @@ struct LookupInfo {
   double eps;                // box expansion covering the rounding of a target's cell assignment
   int dims[3];
   int enabled;
+  // outer groups: candidate lists for targets beyond the grid (ohdr = nullptr: none)
+  double oo[3], oinvh[3];    // origin and inverse edges of the outer group lattice
+  int odims[3];
+  const int2* ohdr;          // per outer group: (start in lcand, count), count 0 for none
 };
 
 // Start grid of the tetra walk (a finite tet near each cell centre) and the points' bounding box,
@@ -857,13 +861,30 @@ __device__ __forceinline__ int lookup_pick(const float4* __restrict__ pts, const
   return -2;
 }
 
+// Targets beyond the lookup grid: the list of their outer group, if it has one (else -2).
+__device__ __forceinline__ int outer_lookup(const float4* __restrict__ pts, const float4* __restrict__ lcand,
+                                            const LookupInfo& lk, float t0, float t1, float t2, float lo0,
+                                            float lo1, float lo2, float hi0, float hi1, float hi2) {
+  const double v0 = ((double)t0 - lk.oo[0]) * lk.oinvh[0];
+  const double v1 = ((double)t1 - lk.oo[1]) * lk.oinvh[1];
+  const double v2 = ((double)t2 - lk.oo[2]) * lk.oinvh[2];
+  if (!(v0 >= 0.0 && v0 < (double)lk.odims[0] && v1 >= 0.0 && v1 < (double)lk.odims[1] && v2 >= 0.0 &&
+        v2 < (double)lk.odims[2]))
+    return -2;
+  const int2 hd = lk.ohdr[((int64_t)(int)v2 * lk.odims[1] + (int)v1) * lk.odims[0] + (int)v0];
+  if (hd.y <= 0) return -2;
+  const float4 h4 = make_float4(__int_as_float(hd.x), __int_as_float(hd.y), 0.f, __int_as_float(-2));
+  const float4 sl[kSlots] = {h4, h4, h4, h4};  // an overflow header: only slot 0 is read
+  return lookup_pick(pts, lcand, nullptr, sl, t0, t1, t2, lo0, lo1, lo2, hi0, hi1, hi2);
+}
+
 // Fast path: the unconstrained nearest point from the target's lookup cell. Returns its sorted
 // position if it lies strictly inside the search box, else -2 (the caller runs the exact query).
 __device__ __forceinline__ int nn_lookup(const float4* __restrict__ pts, const float4* __restrict__ lslot,
                                          const float4* __restrict__ lcand, const LookupInfo& lk, float t0, float t1,
                                          float t2, float lo0, float lo1, float lo2, float hi0, float hi1, float hi2) {
   const int64_t L = lookup_cell(lk, t0, t1, t2);
-  if (L < 0) return -2;
+  if (L < 0) return lk.ohdr ? outer_lookup(pts, lcand, lk, t0, t1, t2, lo0, lo1, lo2, hi0, hi1, hi2) : -2;
   // the first half of the 128-byte record; the rest and the overflow list only when taken
   const float4* rp = lslot + (int64_t)kRecSlots * L;
   float4 sl[kSlots];
@@ -1171,6 +1192,8 @@ struct FarGrid {
   int gdim[3];   // groups per axis
   int gcell[3];  // lookup cells per group edge
   int bdim[3];   // blocks of 4 x 4 x 4 groups per axis
+  int ioff[3];   // lookup cell index of group 0's low corner (negative for outer groups)
+  int clip;      // boxes end at the grid's last cell (groups of the grid) or not (outer groups)
 };
 
 __device__ __forceinline__ double far_margin2(double u2, const LookupInfo& lk) {
@@ -1239,7 +1262,8 @@ __global__ void __launch_bounds__(kFarBlock) k_far_groups(const float4* __restri
   double lo[3], hi[3], c[3], hw[3];
 #pragma unroll
   for (int a = 0; a < 3; ++a) {
-    const int i0 = gi[a] * fg.gcell[a], i1 = min(i0 + fg.gcell[a], lk.dims[a]);
+    const int i0 = gi[a] * fg.gcell[a] + fg.ioff[a];
+    const int i1 = fg.clip ? min(i0 + fg.gcell[a], lk.dims[a]) : i0 + fg.gcell[a];
     lo[a] = lk.o[a] + (double)i0 * lk.h[a] - lk.eps;
     hi[a] = lk.o[a] + (double)i1 * lk.h[a] + lk.eps;
     c[a] = 0.5 * (lo[a] + hi[a]);
@@ -1345,6 +1369,29 @@ __global__ void __launch_bounds__(kFarBlock) k_far_groups(const float4* __restri
     }
   }
   ginfo[g] = info;
+}
+
+// Outer groups: gflag[g] = 1 for the groups reaching beyond the lookup grid.
+__global__ void k_outer_flags(LookupInfo lk, FarGrid fg, int64_t ng, uint8_t* __restrict__ gflag) {
+  const int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (g >= ng) return;
+  const int gi[3] = {(int)(g % fg.gdim[0]), (int)((g / fg.gdim[0]) % fg.gdim[1]),
+                     (int)(g / ((int64_t)fg.gdim[0] * fg.gdim[1]))};
+  bool inside = true;
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    const int i0 = gi[a] * fg.gcell[a] + fg.ioff[a];
+    inside = inside && i0 >= 0 && i0 + fg.gcell[a] <= lk.dims[a];
+  }
+  gflag[g] = inside ? 0 : 1;
+}
+
+// Outer group headers: (start in lcand, count) from the lists' places in the far pool.
+__global__ void k_outer_headers(const int2* __restrict__ ginfo, int64_t ng, int64_t base, int2* __restrict__ ohdr) {
+  const int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (g >= ng) return;
+  const int2 gi = ginfo[g];
+  ohdr[g] = gi.y > 0 ? make_int2((int)(base + gi.x), gi.y) : make_int2(0, 0);
 }
 
 // Left cells of groups with a list: the record's header points at the group's list.
@@ -2286,6 +2333,7 @@ struct Dataset {
   int lookup_active_subdiv = 0;  // subdivision actually built (0: none)
   DevBuf lslot;        // kRecSlots float4 per lookup cell: inline candidates or an overflow header
   DevBuf lcand;        // float4 overflow candidates (x, y, z, sorted position as int bits)
+  DevBuf ohdr;         // int2 per outer group of the lookup grid (lk.ohdr)
   BvhInfo bvh{};       // implicit BVH over Morton-ordered points (exact NN fallback)
   DevBuf mpts;         // float4 (x, y, z, sorted position) in Morton order
   DevBuf nlo, nhi;     // node boxes, all levels
@@ -2654,40 +2702,62 @@ int build_tets(Dataset& ds) {
 // Far groups: candidate lists for the lookup cells left to the exact query (cnt = -1), one per
 // group of kFarGroup^3 cell-list cells, appended to lcand after its first nnear entries. Optional:
 // without the memory for it, or for groups with long lists, the cells keep the exact query.
-int build_far_groups(Dataset& ds, const LookupInfo& lk, int subdiv, int64_t ncl, int32_t* cnt, int64_t nnear) {
+int build_far_groups(Dataset& ds, LookupInfo& lk, int subdiv, int64_t ncl, int32_t* cnt, int64_t nnear) {
   if (ds.bvh.nodes <= 0 || ds.bvh.nodes >= (int64_t)INT32_MAX) return FOMO_OK;
   hipStream_t s = g_ctx.stream;
   FarGrid fg{};
   fg.gcell[0] = kFarGroup * std::max(1, subdiv / 2);  // lookup cells per cell-list cell along x
   fg.gcell[1] = fg.gcell[2] = kFarGroup * subdiv;
-  int64_t ng = 1, nb = 1;
+  fg.clip = 1;
+  // outer groups: twice the edge, over a fifth of the grid's extent beyond each face
+  FarGrid og{};
+  int64_t ng = 1, nb = 1, ong = 1, onb = 1;
   for (int a = 0; a < 3; ++a) {
     fg.gdim[a] = (lk.dims[a] + fg.gcell[a] - 1) / fg.gcell[a];
     fg.bdim[a] = (fg.gdim[a] + 3) / 4;
     ng *= fg.gdim[a];
     nb *= fg.bdim[a];
+    og.gcell[a] = 2 * fg.gcell[a];
+    const int across = (lk.dims[a] + og.gcell[a] - 1) / og.gcell[a];
+    const int ext = lk.dims[a] > 1 ? (across + 4) / 5 : 0;
+    og.ioff[a] = -ext * og.gcell[a];
+    og.gdim[a] = across + 2 * ext;
+    og.bdim[a] = (og.gdim[a] + 3) / 4;
+    ong *= og.gdim[a];
+    onb *= og.bdim[a];
   }
-  const int64_t chunk = std::min<int64_t>(nb, 2048);  // blocks per launch (scratch 512 MiB)
-  const int64_t fcap = std::min<int64_t>(32 * ng, (int64_t)INT32_MAX - 1 - nnear);
+  const bool outer = ong <= ng;  // the outer lattice is coarser: about a third of the groups
+  const int64_t chunk = std::min<int64_t>(std::max(nb, onb), 2048);  // blocks per launch (scratch 512 MiB)
+  const int64_t fcap = std::min<int64_t>(32 * (ng + (outer ? ong : 0)), (int64_t)INT32_MAX - 1 - nnear);
   if (fcap <= 0) return FOMO_OK;
-  DevBuf gflag, ginfo, fcnt, fpool, cbuf;
+  DevBuf gflag, ginfo, fcnt, fpool, cbuf, oflag, oinfo;
   if (gflag.alloc(ng) || ginfo.alloc(sizeof(int2) * ng) || fcnt.alloc(sizeof(unsigned long long)) ||
       fpool.alloc(sizeof(float4) * fcap) || cbuf.alloc(sizeof(float4) * chunk * kFarBlock * kFarCap)) {
     (void)hipGetLastError();
     return FOMO_OK;
   }
+  const bool do_outer = outer && !oflag.alloc(ong) && !oinfo.alloc(sizeof(int2) * ong);
+  if (!do_outer) (void)hipGetLastError();
   HIPOK(hipMemsetAsync(gflag.p, 0, ng, s));
   HIPOK(hipMemsetAsync(ginfo.p, 0, sizeof(int2) * ng, s));
   HIPOK(hipMemsetAsync(fcnt.p, 0, sizeof(unsigned long long), s));
   k_far_flags<<<nblk(ncl), kBlock, 0, s>>>(cnt, ncl, lk, fg, gflag.as<uint8_t>());
   HIPOK(hipGetLastError());
+  const auto groups = [&](const FarGrid& G, int64_t nblocks, DevBuf& flag, DevBuf& info) {
+    for (int64_t b0 = 0; b0 < nblocks; b0 += chunk)
+      k_far_groups<<<(unsigned)std::min(chunk, nblocks - b0), kFarBlock, 0, s>>>(
+          ds.pts.as<float4>(), ds.mpts.as<float4>(), ds.nlo.as<float4>(), ds.nhi.as<float4>(), (int)ds.bvh.nodes, lk,
+          G, b0, flag.as<uint8_t>(), cbuf.as<float4>(), fpool.as<float4>(), fcnt.as<unsigned long long>(), fcap,
+          info.as<int2>());
+  };
   {
     ProfScope ps("k_far_groups", s);
-    for (int64_t b0 = 0; b0 < nb; b0 += chunk)
-      k_far_groups<<<(unsigned)std::min(chunk, nb - b0), kFarBlock, 0, s>>>(
-          ds.pts.as<float4>(), ds.mpts.as<float4>(), ds.nlo.as<float4>(), ds.nhi.as<float4>(), (int)ds.bvh.nodes, lk,
-          fg, b0, gflag.as<uint8_t>(), cbuf.as<float4>(), fpool.as<float4>(), fcnt.as<unsigned long long>(), fcap,
-          ginfo.as<int2>());
+    groups(fg, nb, gflag, ginfo);
+    if (do_outer) {
+      HIPOK(hipMemsetAsync(oinfo.p, 0, sizeof(int2) * ong, s));
+      k_outer_flags<<<nblk(ong), kBlock, 0, s>>>(lk, og, ong, oflag.as<uint8_t>());
+      groups(og, onb, oflag, oinfo);
+    }
   }
   HIPOK(hipGetLastError());
   unsigned long long used = 0;
@@ -2696,23 +2766,35 @@ int build_far_groups(Dataset& ds, const LookupInfo& lk, int subdiv, int64_t ncl,
   const int64_t nfar = std::min<int64_t>((int64_t)used, fcap);  // blocks past fcap were not written
   if (nfar == 0) return FOMO_OK;
   DevBuf lc;
-  if (lc.alloc(sizeof(float4) * (nnear + nfar))) {
+  if (lc.alloc(sizeof(float4) * (nnear + nfar)) || (do_outer && ds.ohdr.alloc(sizeof(int2) * ong))) {
     (void)hipGetLastError();
+    ds.ohdr.release();
     return FOMO_OK;
   }
   if (nnear) HIPOK(hipMemcpyAsync(lc.p, ds.lcand.p, sizeof(float4) * nnear, hipMemcpyDeviceToDevice, s));
   HIPOK(hipMemcpyAsync(lc.as<float4>() + nnear, fpool.p, sizeof(float4) * nfar, hipMemcpyDeviceToDevice, s));
   k_far_headers<<<nblk(ncl), kBlock, 0, s>>>(cnt, ncl, lk, fg, ginfo.as<int2>(), nnear, ds.lslot.as<float4>());
+  if (do_outer)
+    k_outer_headers<<<nblk(ong), kBlock, 0, s>>>(oinfo.as<int2>(), ong, nnear, ds.ohdr.as<int2>());
   HIPOK(hipGetLastError());
   if (int rc = sync_stream()) return rc;
   std::swap(ds.lcand.p, lc.p);
   std::swap(ds.lcand.bytes, lc.bytes);
+  if (do_outer) {
+    for (int a = 0; a < 3; ++a) {
+      lk.oo[a] = lk.o[a] + (double)og.ioff[a] * lk.h[a];
+      lk.oinvh[a] = 1.0 / ((double)og.gcell[a] * lk.h[a]);
+      lk.odims[a] = og.gdim[a];
+    }
+    lk.ohdr = ds.ohdr.as<int2>();
+  }
   return FOMO_OK;
 }
 
 int build_lookup(Dataset& ds) {
   ds.lk = LookupInfo{};
   ds.lslot.release();
+  ds.ohdr.release();
   ds.lcand.release();
   ds.ncand = ds.nlcells = ds.nfallback = 0;
   if (ds.lookup_subdiv <= 0) return FOMO_OK;
