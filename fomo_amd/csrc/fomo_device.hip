@@ -861,17 +861,24 @@ __device__ __forceinline__ int lookup_pick(const float4* __restrict__ pts, const
   return -2;
 }
 
-// Targets beyond the lookup grid: the list of their outer group, if it has one (else -2).
-__device__ __forceinline__ int outer_lookup(const float4* __restrict__ pts, const float4* __restrict__ lcand,
-                                            const LookupInfo& lk, float t0, float t1, float t2, float lo0,
-                                            float lo1, float lo2, float hi0, float hi1, float hi2) {
+// Outer group of a target beyond the lookup grid (-1 outside the outer lattice).
+__device__ __forceinline__ int64_t outer_group(const LookupInfo& lk, float t0, float t1, float t2) {
   const double v0 = ((double)t0 - lk.oo[0]) * lk.oinvh[0];
   const double v1 = ((double)t1 - lk.oo[1]) * lk.oinvh[1];
   const double v2 = ((double)t2 - lk.oo[2]) * lk.oinvh[2];
   if (!(v0 >= 0.0 && v0 < (double)lk.odims[0] && v1 >= 0.0 && v1 < (double)lk.odims[1] && v2 >= 0.0 &&
         v2 < (double)lk.odims[2]))
-    return -2;
-  const int2 hd = lk.ohdr[((int64_t)(int)v2 * lk.odims[1] + (int)v1) * lk.odims[0] + (int)v0];
+    return -1;
+  return ((int64_t)(int)v2 * lk.odims[1] + (int)v1) * lk.odims[0] + (int)v0;
+}
+
+// Targets beyond the lookup grid: the list of their outer group, if it has one (else -2).
+__device__ __forceinline__ int outer_lookup(const float4* __restrict__ pts, const float4* __restrict__ lcand,
+                                            const LookupInfo& lk, float t0, float t1, float t2, float lo0,
+                                            float lo1, float lo2, float hi0, float hi1, float hi2) {
+  const int64_t og = outer_group(lk, t0, t1, t2);
+  if (og < 0) return -2;
+  const int2 hd = lk.ohdr[og];
   if (hd.y <= 0) return -2;
   const float4 h4 = make_float4(__int_as_float(hd.x), __int_as_float(hd.y), 0.f, __int_as_float(-2));
   const float4 sl[kSlots] = {h4, h4, h4, h4};  // an overflow header: only slot 0 is read
@@ -1068,7 +1075,12 @@ __device__ __forceinline__ void nn_sample(const float4* __restrict__ pts, const 
   const double p1 = -xx * cb * sl + yy * cl - zz * sb * sl;
   const double p2 = -xx * sb + zz * cb;
   int r = -2;
-  if (lk.enabled) {
+  const double md0 = vg.md;
+  // a box that misses the data's bounding box holds no point: the lookup is skipped (r = -1 below)
+  const bool may_hold = (float)(p0 + md0) > g.bmin[0] && (float)(p0 - md0) < g.bmax[0] &&
+                        (float)(p1 + md0) > g.bmin[1] && (float)(p1 - md0) < g.bmax[1] &&
+                        (float)(p2 + md0) > g.bmin[2] && (float)(p2 - md0) < g.bmax[2];
+  if (lk.enabled && may_hold) {
     const double md = vg.md;
     r = nn_lookup(pts, lslot, lcand, lk, (float)p0, (float)p1, (float)p2, (float)(p0 - md), (float)(p1 - md),
                   (float)(p2 - md), (float)(p0 + md), (float)(p1 + md), (float)(p2 + md));
