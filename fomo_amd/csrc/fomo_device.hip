@@ -136,6 +136,9 @@ struct LookupInfo {
   double oo[3], oinvh[3];    // origin and inverse edges of the outer group lattice
   int odims[3];
   const int2* ohdr;          // per outer group: (start in lcand, count), count 0 for none
+  // far groups whose cells all use the group's list (ghdr = nullptr: none): read before the record
+  int gcell[3], gdims[3];    // lookup cells per group edge, groups per axis
+  const int2* ghdr;          // per group: (start in lcand, count), count 0 when a cell has its own
 };
 
 // Start grid of the tetra walk (a finite tet near each cell centre) and the points' bounding box,
@@ -892,6 +895,19 @@ __device__ __forceinline__ int nn_lookup(const float4* __restrict__ pts, const f
                                          float t2, float lo0, float lo1, float lo2, float hi0, float hi1, float hi2) {
   const int64_t L = lookup_cell(lk, t0, t1, t2);
   if (L < 0) return lk.ohdr ? outer_lookup(pts, lcand, lk, t0, t1, t2, lo0, lo1, lo2, hi0, hi1, hi2) : -2;
+  if (lk.ghdr) {
+    // a group whose cells all point at its list: its 8-byte header instead of the cell's record
+    const int lx = (int)(L % lk.dims[0]);
+    const int ly = (int)((L / lk.dims[0]) % lk.dims[1]);
+    const int lz = (int)(L / ((int64_t)lk.dims[0] * lk.dims[1]));
+    const int2 hd = lk.ghdr[((int64_t)(lz / lk.gcell[2]) * lk.gdims[1] + ly / lk.gcell[1]) * lk.gdims[0] +
+                            lx / lk.gcell[0]];
+    if (hd.y > 0) {
+      const float4 h4 = make_float4(__int_as_float(hd.x), __int_as_float(hd.y), 0.f, __int_as_float(-2));
+      const float4 sg[kSlots] = {h4, h4, h4, h4};  // an overflow header: only slot 0 is read
+      return lookup_pick(pts, lcand, nullptr, sg, t0, t1, t2, lo0, lo1, lo2, hi0, hi1, hi2);
+    }
+  }
   // the first half of the 128-byte record; the rest and the overflow list only when taken
   const float4* rp = lslot + (int64_t)kRecSlots * L;
   float4 sl[kSlots];
@@ -1240,16 +1256,30 @@ __device__ __forceinline__ void far_walk(const float4* __restrict__ nlo, const f
   }
 }
 
-// gflag[g] = 1 for the groups holding a lookup cell left to the exact query (cnt = -1)
+// gflag[g] = 1 for the groups holding a lookup cell left to the exact query (cnt = -1), gown[g] = 1
+// for those holding a cell with a list of its own
 __global__ void k_far_flags(const int32_t* __restrict__ cnt, int64_t ncl, LookupInfo lk, FarGrid fg,
-                            uint8_t* __restrict__ gflag) {
+                            uint8_t* __restrict__ gflag, uint8_t* __restrict__ gown) {
   const int64_t L = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (L >= ncl || cnt[L] != -1) return;
+  if (L >= ncl) return;
+  const bool left = cnt[L] == -1;
   const int lx = (int)(L % lk.dims[0]);
   const int ly = (int)((L / lk.dims[0]) % lk.dims[1]);
   const int lz = (int)(L / ((int64_t)lk.dims[0] * lk.dims[1]));
   const int64_t g = ((int64_t)(lz / fg.gcell[2]) * fg.gdim[1] + ly / fg.gcell[1]) * fg.gdim[0] + lx / fg.gcell[0];
-  gflag[g] = 1;
+  if (left) gflag[g] = 1;
+  else gown[g] = 1;
+}
+
+// Headers of the groups whose cells all take the group's list; nfull counts them.
+__global__ void k_group_headers(const int2* __restrict__ ginfo, const uint8_t* __restrict__ gown, int64_t ng,
+                                int64_t base, int2* __restrict__ ghdr, unsigned long long* __restrict__ nfull) {
+  const int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (g >= ng) return;
+  const int2 gi = ginfo[g];
+  const bool full = gi.y > 0 && !gown[g];
+  ghdr[g] = full ? make_int2((int)(base + gi.x), gi.y) : make_int2(0, 0);
+  if (full) atomicAdd(nfull, 1ull);
 }
 
 // Candidate lists of the flagged groups in blocks [b0, b0 + gridDim.x): ginfo[g] = (start in
@@ -2346,6 +2376,7 @@ struct Dataset {
   DevBuf lslot;        // kRecSlots float4 per lookup cell: inline candidates or an overflow header
   DevBuf lcand;        // float4 overflow candidates (x, y, z, sorted position as int bits)
   DevBuf ohdr;         // int2 per outer group of the lookup grid (lk.ohdr)
+  DevBuf ghdr;         // int2 per far group whose cells all take its list (lk.ghdr)
   BvhInfo bvh{};       // implicit BVH over Morton-ordered points (exact NN fallback)
   DevBuf mpts;         // float4 (x, y, z, sorted position) in Morton order
   DevBuf nlo, nhi;     // node boxes, all levels
@@ -2742,8 +2773,8 @@ int build_far_groups(Dataset& ds, LookupInfo& lk, int subdiv, int64_t ncl, int32
   const int64_t chunk = std::min<int64_t>(std::max(nb, onb), 2048);  // blocks per launch (scratch 512 MiB)
   const int64_t fcap = std::min<int64_t>(32 * (ng + (outer ? ong : 0)), (int64_t)INT32_MAX - 1 - nnear);
   if (fcap <= 0) return FOMO_OK;
-  DevBuf gflag, ginfo, fcnt, fpool, cbuf, oflag, oinfo;
-  if (gflag.alloc(ng) || ginfo.alloc(sizeof(int2) * ng) || fcnt.alloc(sizeof(unsigned long long)) ||
+  DevBuf gflag, gown, ginfo, fcnt, fpool, cbuf, oflag, oinfo;
+  if (gflag.alloc(ng) || gown.alloc(ng) || ginfo.alloc(sizeof(int2) * ng) || fcnt.alloc(sizeof(unsigned long long)) ||
       fpool.alloc(sizeof(float4) * fcap) || cbuf.alloc(sizeof(float4) * chunk * kFarBlock * kFarCap)) {
     (void)hipGetLastError();
     return FOMO_OK;
@@ -2751,9 +2782,10 @@ int build_far_groups(Dataset& ds, LookupInfo& lk, int subdiv, int64_t ncl, int32
   const bool do_outer = outer && !oflag.alloc(ong) && !oinfo.alloc(sizeof(int2) * ong);
   if (!do_outer) (void)hipGetLastError();
   HIPOK(hipMemsetAsync(gflag.p, 0, ng, s));
+  HIPOK(hipMemsetAsync(gown.p, 0, ng, s));
   HIPOK(hipMemsetAsync(ginfo.p, 0, sizeof(int2) * ng, s));
   HIPOK(hipMemsetAsync(fcnt.p, 0, sizeof(unsigned long long), s));
-  k_far_flags<<<nblk(ncl), kBlock, 0, s>>>(cnt, ncl, lk, fg, gflag.as<uint8_t>());
+  k_far_flags<<<nblk(ncl), kBlock, 0, s>>>(cnt, ncl, lk, fg, gflag.as<uint8_t>(), gown.as<uint8_t>());
   HIPOK(hipGetLastError());
   const auto groups = [&](const FarGrid& G, int64_t nblocks, DevBuf& flag, DevBuf& info) {
     for (int64_t b0 = 0; b0 < nblocks; b0 += chunk)
@@ -2788,8 +2820,19 @@ int build_far_groups(Dataset& ds, LookupInfo& lk, int subdiv, int64_t ncl, int32
   k_far_headers<<<nblk(ncl), kBlock, 0, s>>>(cnt, ncl, lk, fg, ginfo.as<int2>(), nnear, ds.lslot.as<float4>());
   if (do_outer)
     k_outer_headers<<<nblk(ong), kBlock, 0, s>>>(oinfo.as<int2>(), ong, nnear, ds.ohdr.as<int2>());
+  // the group table, kept only when some group's cells all take its list
+  unsigned long long nfull = 0;
+  if (!ds.ghdr.alloc(sizeof(int2) * ng)) {
+    HIPOK(hipMemsetAsync(fcnt.p, 0, sizeof(unsigned long long), s));
+    k_group_headers<<<nblk(ng), kBlock, 0, s>>>(ginfo.as<int2>(), gown.as<uint8_t>(), ng, nnear, ds.ghdr.as<int2>(),
+                                                fcnt.as<unsigned long long>());
+    HIPOK(hipMemcpyAsync(&nfull, fcnt.p, sizeof nfull, hipMemcpyDeviceToHost, s));
+  } else {
+    (void)hipGetLastError();
+  }
   HIPOK(hipGetLastError());
   if (int rc = sync_stream()) return rc;
+  if (nfull == 0) ds.ghdr.release();
   std::swap(ds.lcand.p, lc.p);
   std::swap(ds.lcand.bytes, lc.bytes);
   if (do_outer) {
@@ -2800,6 +2843,13 @@ int build_far_groups(Dataset& ds, LookupInfo& lk, int subdiv, int64_t ncl, int32
     }
     lk.ohdr = ds.ohdr.as<int2>();
   }
+  if (ds.ghdr.p) {
+    for (int a = 0; a < 3; ++a) {
+      lk.gcell[a] = fg.gcell[a];
+      lk.gdims[a] = fg.gdim[a];
+    }
+    lk.ghdr = ds.ghdr.as<int2>();
+  }
   return FOMO_OK;
 }
 
@@ -2807,6 +2857,7 @@ int build_lookup(Dataset& ds) {
   ds.lk = LookupInfo{};
   ds.lslot.release();
   ds.ohdr.release();
+  ds.ghdr.release();
   ds.lcand.release();
   ds.ncand = ds.nlcells = ds.nfallback = 0;
   if (ds.lookup_subdiv <= 0) return FOMO_OK;
