@@ -19,7 +19,9 @@
 //   goft             : numpy.interp along log T at the nearest log n row; brute force.
 //   linefwhm/emission: closed forms in FP64.
 //   thomson factors  : mpmath evaluation of the closed form.
-//   rotation/samples : identity view (image = grid), exact replay.
+//   rotation/samples : identity view (image = grid); at oblique views, scipy's elementary
+//                      rotations, the transposed matrix for the samples, Doppler centroids,
+//                      Thomson's matrix blend and sin^2 chi limits, planted-mistake rebuilds.
 //   NN query         : brute-force argmin on small inputs; scipy cKDTree fixtures.
 //   integration      : uniform slab, line integral and Doppler shift known answers.
 //   projection       : pixels of grid points in an axis view (exact), half-away-from-zero
@@ -177,11 +179,96 @@ struct Extents {
   double minx, maxx, miny, maxy, minz, maxz;
 };
 
-// fomo-nearestneighbour.cpp:83-85 rotation (left-associative FP64 from float coordinates).
+// ---------------------------------------------------------------------------
+// View geometry. Every render function below takes its rotation, LOS unit vector, sample
+// positions and search half-width from these helpers, and orc_geometry exports them, so the
+// independent checks of tests/test_oracle_geometry.py pin exactly the code the renders run.
+// All of it is left-associative FP64 (no FMA), as the reference evaluates it.
+// ---------------------------------------------------------------------------
+
+// fomo-nearestneighbour.cpp:83-85 rotation of a data point (FP64 from float coordinates).
 inline void rotate_point(double g0, double g1, double g2, double l, double b, double* xr, double* yr, double* zr) {
   *xr = g0 * cos(b) * cos(l) - g1 * cos(b) * sin(l) - g2 * sin(b);
   *yr = g0 * sin(l) + g1 * cos(l);
   *zr = g0 * sin(b) * cos(l) - g1 * sin(b) * sin(l) + g2 * cos(b);
+}
+
+// fomo-nearestneighbour.cpp:87-90 LOS unit vector and inner_product (starting from 0.0).
+inline void los_unit(double l, double b, double u[3]) {
+  u[0] = sin(b) * cos(l);
+  u[1] = -sin(b) * sin(l);
+  u[2] = cos(b);
+}
+inline double los_velocity(const double u[3], double vx, double vy, double vz) {
+  return ((0.0 + u[0] * vx) + u[1] * vy) + u[2] * vz;
+}
+
+// De-rotation of the LOS sample (x, y, z) of the image frame back to the data frame:
+// fomo-nearestneighbour.cpp:165, fomo-thomson.cpp:295 (unclamped cos_b/sin_b, :275-276),
+// fomo-CGAL.cpp:192.
+inline void derotate_sample(double x, double y, double z, double l, double b, double* p0, double* p1, double* p2) {
+  *p0 = x * cos(b) * cos(l) + y * sin(l) + z * sin(b) * cos(l);
+  *p1 = -x * cos(b) * sin(l) + y * cos(l) - z * sin(b) * sin(l);
+  *p2 = -x * sin(b) + z * cos(b);
+}
+
+// Image-frame coordinates of ray (i, j) and sample k. kind 0: NearestNeighbour / Thomson,
+// z = k*deltaz + minz with deltaz = (maxz-minz)/(nz-1) unless nz == 1
+// (fomo-nearestneighbour.cpp:141-162, fomo-thomson.cpp:264-288); kind 2: CGAL,
+// z = k/(nz-1)*(maxz-minz) + minz (fomo-CGAL.cpp:187-189; minz when nz == 1, where the
+// reference's 0/0 is undefined).
+inline void sample_xyz(int kind, const double* e, int nx, int ny, int nz, int i, int j, int k,
+                       double* x, double* y, double* z) {
+  *x = double(j) / (nx - 1) * (e[1] - e[0]) + e[0];
+  *y = double(i) / (ny - 1) * (e[3] - e[2]) + e[2];
+  if (kind == 2) {
+    *z = nz > 1 ? double(k) / (nz - 1) * (e[5] - e[4]) + e[4] : e[4];
+  } else {
+    double deltaz = e[5] - e[4];
+    if (nz != 1) deltaz /= (nz - 1);
+    *z = double(k) * deltaz + e[4];
+  }
+}
+
+// fomo-nearestneighbour.cpp:138: search half-width of the NearestNeighbour render.
+inline double nn_maxdistance(const double* e, int nx, int ny) {
+  return std::max((e[1] - e[0]) / (nx - 1), (e[3] - e[2]) / (ny - 1)) / .3;
+}
+
+// fomo-thomson.cpp:139-199: blended rotation of a data point (full rotation with eps-clamped
+// cos_b/sin_b, sigmoid-blended into the pure z rotation near the pole) and sin^2 chi from the
+// blended position and the clamped LOS unit vector.
+inline void thomson_rotate(double g0, double g1, double g2, double l, double b, double* xa, double* ya,
+                           double* za, double* s2) {
+  const double epsilon = 1e-6;  // :139-152
+  double cos_b = cos(b), sin_b = sin(b), cos_l = cos(l), sin_l = sin(l);
+  if (std::abs(cos_b) < epsilon) cos_b = epsilon;
+  if (std::abs(sin_b) < epsilon) sin_b = epsilon;
+  double threshold = 1.309, steepness = 20.0;  // :173-176
+  double blend = 1.0 / (1.0 + exp(-steepness * (std::fabs(b) - threshold)));
+  double full_x = g0 * cos_b * cos_l - g1 * cos_b * sin_l - g2 * sin_b;  // :179-181
+  double full_y = g0 * sin_l + g1 * cos_l;
+  double full_z = g0 * sin_b * cos_l - g1 * sin_b * sin_l + g2 * cos_b;
+  double pole_x = g0 * cos_l - g1 * sin_l;  // :184-186
+  double pole_y = g0 * sin_l + g1 * cos_l;
+  double pole_z = g2;
+  *xa = full_x * (1.0 - blend) + pole_x * blend;  // :189-191
+  *ya = full_y * (1.0 - blend) + pole_y * blend;
+  *za = full_z * (1.0 - blend) + pole_z * blend;
+  double unit[3] = {sin_b * cos(l), -sin_b * sin(l), cos_b};  // :194
+  double ru = ((0.0 + unit[0] * *xa) + unit[1] * *ya) + unit[2] * *za;
+  double rr = ((0.0 + *xa * *xa) + *ya * *ya) + *za * *za;
+  *s2 = 1. - std::pow(ru, 2) / rr;  // :198
+}
+
+// fomo-thomson.cpp:249-260: search half-width of the Thomson render, a sigmoid blend in |b|.
+inline double thomson_maxdistance(const double* e, int nx, int ny, double b) {
+  const double minx = e[0], maxx = e[1], miny = e[2], maxy = e[3];
+  double blending_factor = 1.0 / (1.0 + exp(-10 * (std::fabs(b) - 0.1745)));
+  double equator_formula = std::max(0.5 * std::sqrt(std::pow((maxx - minx) / (nx - 1), 2) + std::pow((maxy - miny) / (ny - 1), 2)),
+                                    0.1 * (maxx - minx) / (nx - 1));
+  double away_from_equator_formula = std::max((maxx - minx) / (nx - 1), (maxy - miny) / (ny - 1)) / 0.005;
+  return equator_formula * (1.0 - blending_factor) + away_from_equator_formula * blending_factor;
 }
 
 }  // namespace
@@ -423,12 +510,10 @@ int orc_render_nn(int mode, int64_t n, const float* x, const float* y, const flo
   std::vector<double> losvel(n);
   double ext[6];
   orc_extents(n, x, y, z, l, b, ext);
+  double unit[3];
+  los_unit(l, b, unit);
 #pragma omp parallel for schedule(static)
-  for (int64_t i = 0; i < n; ++i) {
-    double unit[3] = {sin(b) * cos(l), -sin(b) * sin(l), cos(b)};
-    double v[3] = {vx[i], vy[i], vz[i]};
-    losvel[i] = ((0.0 + unit[0] * v[0]) + unit[1] * v[1]) + unit[2] * v[2];
-  }
+  for (int64_t i = 0; i < n; ++i) losvel[i] = los_velocity(unit, vx[i], vy[i], vz[i]);
 #ifdef _OPENMP
   double t1 = omp_get_wtime();
 #else
@@ -443,9 +528,7 @@ int orc_render_nn(int mode, int64_t n, const float* x, const float* y, const flo
 #endif
   const double minx = ext[0], maxx = ext[1], miny = ext[2], maxy = ext[3], minz = ext[4], maxz = ext[5];
   const double lambda_width_in_A = lambda_width * lambda0 / kSpeedOfLight;  // :116
-  const double maxdistance = std::max((maxx - minx) / (nx - 1), (maxy - miny) / (ny - 1)) / .3;  // :138
-  double deltaz = (maxz - minz);
-  if (nz != 1) deltaz /= (nz - 1);  // :142-143
+  const double maxdistance = nn_maxdistance(ext, nx, ny);  // :138
   const int nlo = nl > 1 ? nl : 1;
   std::vector<float> peakf, fwhmf;
   if (mode == 0) {
@@ -456,16 +539,13 @@ int orc_render_nn(int mode, int64_t n, const float* x, const float* y, const flo
 #pragma omp parallel for schedule(dynamic, 1) collapse(2)
   for (int i = row0; i < row1; ++i)
     for (int j = 0; j < nx; ++j) {
-      double xx = double(j) / (nx - 1) * (maxx - minx) + minx;  // :152
-      double yy = double(i) / (ny - 1) * (maxy - miny) + miny;  // :153
       std::vector<float> accf(nlo, 0.f);
       std::vector<double> accd(nlo, 0.0);
       double intpolpeak = 0, intpolfwhm = 0, intpollosvel = 0;
       for (int k = 0; k < nz; ++k) {
-        double zz = double(k) * deltaz + minz;  // :162
-        double p0 = xx * cos(b) * cos(l) + yy * sin(l) + zz * sin(b) * cos(l);  // :165
-        double p1 = -xx * cos(b) * sin(l) + yy * cos(l) - zz * sin(b) * sin(l);
-        double p2 = -xx * sin(b) + zz * cos(b);
+        double xx, yy, zz, p0, p1, p2;
+        sample_xyz(0, ext, nx, ny, nz, i, j, k, &xx, &yy, &zz);  // :152-162
+        derotate_sample(xx, yy, zz, l, b, &p0, &p1, &p2);        // :165
         float t[3] = {(float)p0, (float)p1, (float)p2};  // :171
         float blo[3] = {(float)(p0 - maxdistance), (float)(p1 - maxdistance), (float)(p2 - maxdistance)};  // :179
         float bhi[3] = {(float)(p0 + maxdistance), (float)(p1 + maxdistance), (float)(p2 + maxdistance)};
@@ -555,7 +635,8 @@ int orc_render_projection(int mode, int64_t n, const float* x, const float* y, c
   const size_t ntot = (size_t)nx * ny * nlo;
   std::vector<float> accf(mode == 0 ? ntot : 0, 0.f);
   std::vector<double> accd(mode == 0 ? 0 : ntot, 0.0);
-  const double unit[3] = {sin(b) * cos(l), -sin(b) * sin(l), cos(b)};  // :34
+  double unit[3];
+  los_unit(l, b, unit);  // :34
   for (int64_t k = 0; k < n; ++k) {
     double xr, yr, zr;
     rotate_point(x[k], y[k], z[k], l, b, &xr, &yr, &zr);  // :50-52
@@ -563,8 +644,7 @@ int orc_render_projection(int mode, int64_t n, const float* x, const float* y, c
     const int i = proj_index(yr, miny, maxy, ny);
     if (out_pix) out_pix[k] = i * nx + j;
     if (nl > 1) {
-      const double v[3] = {vx[k], vy[k], vz[k]};
-      const double losvel = ((0.0 + unit[0] * v[0]) + unit[1] * v[1]) + unit[2] * v[2];  // :53-55
+      const double losvel = los_velocity(unit, vx[k], vy[k], vz[k]);  // :53-55
       const double pk = mode == 0 ? (double)(float)peak[k] : peak[k];
       const double fw = mode == 0 ? (double)(float)fwhm[k] : fwhm[k];
       for (int il = 0; il < nl; ++il) {
@@ -614,26 +694,8 @@ int orc_render_thomson(int mode, int64_t n, const float* x, const float* y, cons
   double mnx = INFINITY, mxx = -INFINITY, mny = INFINITY, mxy = -INFINITY, mnz = INFINITY, mxz = -INFINITY;
 #pragma omp parallel for reduction(min : mnx, mny, mnz) reduction(max : mxx, mxy, mxz)
   for (int64_t i = 0; i < n; ++i) {
-    const double epsilon = 1e-6;  // :139-152
-    double cos_b = cos(b), sin_b = sin(b), cos_l = cos(l), sin_l = sin(l);
-    if (std::abs(cos_b) < epsilon) cos_b = epsilon;
-    if (std::abs(sin_b) < epsilon) sin_b = epsilon;
-    double g0 = x[i], g1 = y[i], g2 = z[i];
-    double threshold = 1.309, steepness = 20.0;  // :173-176
-    double blend = 1.0 / (1.0 + exp(-steepness * (std::fabs(b) - threshold)));
-    double full_x = g0 * cos_b * cos_l - g1 * cos_b * sin_l - g2 * sin_b;  // :179-181
-    double full_y = g0 * sin_l + g1 * cos_l;
-    double full_z = g0 * sin_b * cos_l - g1 * sin_b * sin_l + g2 * cos_b;
-    double pole_x = g0 * cos_l - g1 * sin_l;  // :184-186
-    double pole_y = g0 * sin_l + g1 * cos_l;
-    double pole_z = g2;
-    double xa = full_x * (1.0 - blend) + pole_x * blend;  // :189-191
-    double ya = full_y * (1.0 - blend) + pole_y * blend;
-    double za = full_z * (1.0 - blend) + pole_z * blend;
-    double unit[3] = {sin_b * cos(l), -sin_b * sin(l), cos_b};  // :194
-    double ru = ((0.0 + unit[0] * xa) + unit[1] * ya) + unit[2] * za;
-    double rr = ((0.0 + xa * xa) + ya * ya) + za * za;
-    sin2chi[i] = 1. - std::pow(ru, 2) / rr;  // :198
+    double xa, ya, za;
+    thomson_rotate(x[i], y[i], z[i], l, b, &xa, &ya, &za, &sin2chi[i]);  // :136-199
     mnx = std::min(mnx, xa); mxx = std::max(mxx, xa);
     mny = std::min(mny, ya); mxy = std::max(mxy, ya);
     mnz = std::min(mnz, za); mxz = std::max(mxz, za);
@@ -651,14 +713,8 @@ int orc_render_thomson(int mode, int64_t n, const float* x, const float* y, cons
   double t2 = 0;
 #endif
   const double minx = mnx, maxx = mxx, miny = mny, maxy = mxy, minz = mnz, maxz = mxz;
-  // :249-260 blended maxdistance
-  double blending_factor = 1.0 / (1.0 + exp(-10 * (std::fabs(b) - 0.1745)));
-  double equator_formula = std::max(0.5 * std::sqrt(std::pow((maxx - minx) / (nx - 1), 2) + std::pow((maxy - miny) / (ny - 1), 2)),
-                                    0.1 * (maxx - minx) / (nx - 1));
-  double away_from_equator_formula = std::max((maxx - minx) / (nx - 1), (maxy - miny) / (ny - 1)) / 0.005;
-  double maxdistance = equator_formula * (1.0 - blending_factor) + away_from_equator_formula * blending_factor;
-  double deltaz = (maxz - minz);
-  if (nz != 1) deltaz /= (nz - 1);
+  const double ext[6] = {mnx, mxx, mny, mxy, mnz, mxz};
+  const double maxdistance = thomson_maxdistance(ext, nx, ny, b);  // :249-260
   std::vector<float> totf, polf;
   if (mode == 0) {
     totf.resize(n);
@@ -668,16 +724,12 @@ int orc_render_thomson(int mode, int64_t n, const float* x, const float* y, cons
 #pragma omp parallel for schedule(dynamic, 1) collapse(2)
   for (int i = 0; i < ny; ++i)
     for (int j = 0; j < nx; ++j) {
-      double cos_b = cos(b), sin_b = sin(b);  // :275-276 (not clamped)
-      double xx = double(j) / (nx - 1) * (maxx - minx) + minx;
-      double yy = double(i) / (ny - 1) * (maxy - miny) + miny;
       float accf = 0.f;
       double accd = 0.0;
       for (int k = 0; k < nz; ++k) {
-        double zz = double(k) * deltaz + minz;
-        double p0 = xx * cos_b * cos(l) + yy * sin(l) + zz * sin_b * cos(l);  // :295
-        double p1 = -xx * cos_b * sin(l) + yy * cos(l) - zz * sin_b * sin(l);
-        double p2 = -xx * sin_b + zz * cos_b;
+        double xx, yy, zz, p0, p1, p2;
+        sample_xyz(0, ext, nx, ny, nz, i, j, k, &xx, &yy, &zz);  // :282-288
+        derotate_sample(xx, yy, zz, l, b, &p0, &p1, &p2);        // :295 (cos_b, sin_b not clamped, :275-276)
         float t[3] = {(float)p0, (float)p1, (float)p2};
         float blo[3] = {(float)(p0 - maxdistance), (float)(p1 - maxdistance), (float)(p2 - maxdistance)};  // :308
         float bhi[3] = {(float)(p0 + maxdistance), (float)(p1 + maxdistance), (float)(p2 + maxdistance)};
@@ -714,6 +766,57 @@ int orc_render_thomson(int mode, int64_t n, const float* x, const float* y, cons
   }
   if (timing) { timing[0] = t1 - t0; timing[1] = t2 - t1; timing[2] = t3 - t2; }
   return 0;
+}
+
+// ---------------------------------------------------------------------------
+// View geometry, exported so that tests can check the helpers every render above runs.
+// orc_rotate: kind 0 = the rotation of fomo-nearestneighbour.cpp:83-85 (also CGAL :118-121 and
+//   Projection :50-52); kind 1 = Thomson's blended rotation (fomo-thomson.cpp:136-191), with
+//   s2 (optional) receiving sin^2 chi (:193-198).
+// orc_los_velocity: LOS velocity of fomo-nearestneighbour.cpp:87-90.
+// orc_view_samples: data-frame position of every LOS sample of a view with extents ext6,
+//   layout ((i*nx + j)*nz + k); kind 0 = NearestNeighbour/Thomson z spacing, 2 = CGAL.
+// orc_maxdistance: kind 0 = fomo-nearestneighbour.cpp:138, 1 = fomo-thomson.cpp:249-260.
+// ---------------------------------------------------------------------------
+void orc_rotate(int kind, int64_t n, const float* x, const float* y, const float* z, double l, double b,
+                double* xr, double* yr, double* zr, double* s2) {
+  for (int64_t i = 0; i < n; ++i) {
+    if (kind == 1) {
+      double s;
+      thomson_rotate(x[i], y[i], z[i], l, b, &xr[i], &yr[i], &zr[i], &s);
+      if (s2) s2[i] = s;
+    } else {
+      rotate_point(x[i], y[i], z[i], l, b, &xr[i], &yr[i], &zr[i]);
+    }
+  }
+}
+
+void orc_rotate_d(int64_t n, const double* x, const double* y, const double* z, double l, double b, double* xr,
+                  double* yr, double* zr) {
+  for (int64_t i = 0; i < n; ++i) rotate_point(x[i], y[i], z[i], l, b, &xr[i], &yr[i], &zr[i]);
+}
+
+void orc_los_velocity(int64_t n, const float* vx, const float* vy, const float* vz, double l, double b,
+                      double* out) {
+  double unit[3];
+  los_unit(l, b, unit);
+  for (int64_t i = 0; i < n; ++i) out[i] = los_velocity(unit, vx[i], vy[i], vz[i]);
+}
+
+void orc_view_samples(int kind, const double* ext6, double l, double b, int nx, int ny, int nz, double* p0,
+                      double* p1, double* p2) {
+  for (int i = 0; i < ny; ++i)
+    for (int j = 0; j < nx; ++j)
+      for (int k = 0; k < nz; ++k) {
+        double xx, yy, zz;
+        sample_xyz(kind, ext6, nx, ny, nz, i, j, k, &xx, &yy, &zz);
+        const int64_t s = ((int64_t)i * nx + j) * nz + k;
+        derotate_sample(xx, yy, zz, l, b, &p0[s], &p1[s], &p2[s]);
+      }
+}
+
+double orc_maxdistance(int kind, const double* ext6, int nx, int ny, double b) {
+  return kind == 1 ? thomson_maxdistance(ext6, nx, ny, b) : nn_maxdistance(ext6, nx, ny);
 }
 
 int orc_num_threads(void) {

@@ -29,39 +29,57 @@ def lib():
     if _LIB is None:
         if not os.path.exists(LIB_PATH):
             build()
-        L = ctypes.CDLL(LIB_PATH)
-        L.orc_table_read.restype = ctypes.c_int
-        L.orc_table_read.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int, _d, _d, ctypes.c_char_p,
-                                     ctypes.c_int, _ip, _ip, _f, _f, _f]
-        L.orc_emission.restype = ctypes.c_int64
-        L.orc_emission.argtypes = [ctypes.c_int, ctypes.c_int64, _f, _f, ctypes.c_int, ctypes.c_int, _f, _f, _f,
-                                   ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_double, _d, _d]
-        L.orc_thomson_factor.restype = None
-        L.orc_thomson_factor.argtypes = [ctypes.c_int, ctypes.c_double, _d, _d]
-        L.orc_thomson_emission.restype = None
-        L.orc_thomson_emission.argtypes = [ctypes.c_int, ctypes.c_int64, _f, _f, _f, _f, _d, _d]
-        L.orc_extents.restype = None
-        L.orc_extents.argtypes = [ctypes.c_int64, _f, _f, _f, ctypes.c_double, ctypes.c_double, _d]
-        for nm in ("orc_nn_bruteforce", "orc_nn_kdtree"):
-            fn = getattr(L, nm)
-            fn.restype = None
-            fn.argtypes = [ctypes.c_int64, _f, _f, _f, ctypes.c_int64, _d, _d, _d, ctypes.c_double, _i]
-        L.orc_render_nn.restype = ctypes.c_int
-        L.orc_render_nn.argtypes = [ctypes.c_int, ctypes.c_int64, _f, _f, _f, _d, _d, _f, _f, _f, ctypes.c_double,
-                                    ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
-                                    ctypes.c_double, ctypes.c_double, ctypes.c_int, _d, _i, _d, _d, ctypes.c_int,
-                                    ctypes.c_int]
-        L.orc_render_thomson.restype = ctypes.c_int
-        L.orc_render_thomson.argtypes = [ctypes.c_int, ctypes.c_int64, _f, _f, _f, _d, _d, ctypes.c_double,
-                                         ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.c_int, _d, _i, _d, _d]
-        L.orc_render_projection.restype = ctypes.c_int
-        L.orc_render_projection.argtypes = [ctypes.c_int, ctypes.c_int64, _f, _f, _f, _d, _d, _f, _f, _f,
-                                            ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_int,
-                                            ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double,
-                                            ctypes.c_int, _d, _i, _d]
-        L.orc_num_threads.restype = ctypes.c_int
-        _LIB = L
+        _LIB = load(LIB_PATH)
     return _LIB
+
+
+def load(path):
+    """Open an oracle library at `path` with its entry points typed (tests also load variants
+    built from edited copies of the source, to show their checks catch planted mistakes)."""
+    L = ctypes.CDLL(path)
+    L.orc_table_read.restype = ctypes.c_int
+    L.orc_table_read.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int, _d, _d, ctypes.c_char_p,
+                                 ctypes.c_int, _ip, _ip, _f, _f, _f]
+    L.orc_emission.restype = ctypes.c_int64
+    L.orc_emission.argtypes = [ctypes.c_int, ctypes.c_int64, _f, _f, ctypes.c_int, ctypes.c_int, _f, _f, _f,
+                               ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_double, _d, _d]
+    L.orc_thomson_factor.restype = None
+    L.orc_thomson_factor.argtypes = [ctypes.c_int, ctypes.c_double, _d, _d]
+    L.orc_thomson_emission.restype = None
+    L.orc_thomson_emission.argtypes = [ctypes.c_int, ctypes.c_int64, _f, _f, _f, _f, _d, _d]
+    L.orc_extents.restype = None
+    L.orc_extents.argtypes = [ctypes.c_int64, _f, _f, _f, ctypes.c_double, ctypes.c_double, _d]
+    for nm in ("orc_nn_bruteforce", "orc_nn_kdtree"):
+        fn = getattr(L, nm)
+        fn.restype = None
+        fn.argtypes = [ctypes.c_int64, _f, _f, _f, ctypes.c_int64, _d, _d, _d, ctypes.c_double, _i]
+    L.orc_render_nn.restype = ctypes.c_int
+    L.orc_render_nn.argtypes = [ctypes.c_int, ctypes.c_int64, _f, _f, _f, _d, _d, _f, _f, _f, ctypes.c_double,
+                                ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                ctypes.c_double, ctypes.c_double, ctypes.c_int, _d, _i, _d, _d, ctypes.c_int,
+                                ctypes.c_int]
+    L.orc_render_thomson.restype = ctypes.c_int
+    L.orc_render_thomson.argtypes = [ctypes.c_int, ctypes.c_int64, _f, _f, _f, _d, _d, ctypes.c_double,
+                                     ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.c_int, _d, _i, _d, _d]
+    L.orc_render_projection.restype = ctypes.c_int
+    L.orc_render_projection.argtypes = [ctypes.c_int, ctypes.c_int64, _f, _f, _f, _d, _d, _f, _f, _f,
+                                        ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_int,
+                                        ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double,
+                                        ctypes.c_int, _d, _i, _d]
+    L.orc_num_threads.restype = ctypes.c_int
+    L.orc_rotate.restype = None
+    L.orc_rotate.argtypes = [ctypes.c_int, ctypes.c_int64, _f, _f, _f, ctypes.c_double, ctypes.c_double, _d, _d,
+                             _d, _d]
+    L.orc_rotate_d.restype = None
+    L.orc_rotate_d.argtypes = [ctypes.c_int64, _d, _d, _d, ctypes.c_double, ctypes.c_double, _d, _d, _d]
+    L.orc_los_velocity.restype = None
+    L.orc_los_velocity.argtypes = [ctypes.c_int64, _f, _f, _f, ctypes.c_double, ctypes.c_double, _d]
+    L.orc_view_samples.restype = None
+    L.orc_view_samples.argtypes = [ctypes.c_int, _d, ctypes.c_double, ctypes.c_double, ctypes.c_int,
+                                   ctypes.c_int, ctypes.c_int, _d, _d, _d]
+    L.orc_maxdistance.restype = ctypes.c_double
+    L.orc_maxdistance.argtypes = [ctypes.c_int, _d, ctypes.c_int, ctypes.c_int, ctypes.c_double]
+    return L
 
 
 def _pf(a):
@@ -127,6 +145,50 @@ def extents(x, y, z, l, b):
     lib().orc_extents(x.size, P(x, ctypes.c_float), P(y, ctypes.c_float), P(z, ctypes.c_float), l, b,
                       P(e, ctypes.c_double))
     return e
+
+
+def rotate(x, y, z, l, b, thomson=False):
+    """Rotated coordinates of data points as the renders compute them. thomson=True: the blended
+    rotation of fomo-thomson.cpp:136-191, returning sin^2 chi as a fourth array."""
+    x, y, z = _pf(x), _pf(y), _pf(z)
+    n = x.size
+    xr, yr, zr, s2 = np.zeros(n), np.zeros(n), np.zeros(n), np.zeros(n)
+    lib().orc_rotate(1 if thomson else 0, n, P(x, ctypes.c_float), P(y, ctypes.c_float), P(z, ctypes.c_float), l,
+                     b, P(xr, ctypes.c_double), P(yr, ctypes.c_double), P(zr, ctypes.c_double),
+                     P(s2, ctypes.c_double))
+    return (xr, yr, zr, s2) if thomson else (xr, yr, zr)
+
+
+def rotate_d(x, y, z, l, b):
+    """The NearestNeighbour rotation applied to double coordinates (to rotate samples back)."""
+    x, y, z = _pd(x), _pd(y), _pd(z)
+    n = x.size
+    xr, yr, zr = np.zeros(n), np.zeros(n), np.zeros(n)
+    lib().orc_rotate_d(n, P(x, ctypes.c_double), P(y, ctypes.c_double), P(z, ctypes.c_double), l, b,
+                       P(xr, ctypes.c_double), P(yr, ctypes.c_double), P(zr, ctypes.c_double))
+    return xr, yr, zr
+
+
+def los_velocity(vx, vy, vz, l, b):
+    vx, vy, vz = _pf(vx), _pf(vy), _pf(vz)
+    out = np.zeros(vx.size)
+    lib().orc_los_velocity(vx.size, P(vx, ctypes.c_float), P(vy, ctypes.c_float), P(vz, ctypes.c_float), l, b,
+                           P(out, ctypes.c_double))
+    return out
+
+
+def view_samples(ext6, l, b, nx, ny, nz, cgal=False):
+    """Data-frame positions of every LOS sample, each of shape (ny, nx, nz)."""
+    e = _pd(ext6)
+    p = [np.zeros(nx * ny * nz) for _ in range(3)]
+    lib().orc_view_samples(2 if cgal else 0, P(e, ctypes.c_double), l, b, nx, ny, nz,
+                           *[P(a, ctypes.c_double) for a in p])
+    return tuple(a.reshape(ny, nx, nz) for a in p)
+
+
+def maxdistance(ext6, nx, ny, b=0.0, thomson=False):
+    e = _pd(ext6)
+    return lib().orc_maxdistance(1 if thomson else 0, P(e, ctypes.c_double), nx, ny, b)
 
 
 def nn_query(x, y, z, px, py, pz, md, method="kdtree"):
@@ -225,23 +287,13 @@ def render_tet(x, y, z, peak, fwhm, vx, vy, vz, l, b, nx, ny, nz, nl, lambda_wid
     if tri is None:
         tri = Delaunay(pts)
     minx, maxx, miny, maxy, minz, maxz = extents(x, y, z, l, b)
-    cb, sb, cl, sl = math.cos(b), math.sin(b), math.cos(l), math.sin(l)
-    jj = np.arange(nx, dtype=np.float64)[None, :, None]
-    ii = np.arange(ny, dtype=np.float64)[:, None, None]
-    kk = np.arange(nz, dtype=np.float64)[None, None, :]
-    xx = jj / (nx - 1) * (maxx - minx) + minx                                      # :187
-    yy = ii / (ny - 1) * (maxy - miny) + miny                                      # :188
-    zz = kk / (nz - 1) * (maxz - minz) + minz if nz > 1 else np.full_like(kk, minz)  # :189
     shape = (ny, nx, nz)
-    px = np.broadcast_to(xx * cb * cl + yy * sl + zz * sb * cl, shape)             # :192
-    py = np.broadcast_to(-xx * cb * sl + yy * cl - zz * sb * sl, shape)
-    pz = np.broadcast_to(-xx * sb + zz * cb, shape)
+    px, py, pz = view_samples([minx, maxx, miny, maxy, minz, maxz], l, b, nx, ny, nz, cgal=True)  # :187-192
     q = np.stack([px.ravel(), py.ravel(), pz.ravel()], 1)
     s = tri.find_simplex(q)
     inside = s >= 0
     verts = tri.simplices[np.where(inside, s, 0)]
-    ux, uy, uz = sb * cl, -sb * sl, cb                                             # :110
-    vlos_pt = ((0.0 + ux * vx.astype(np.float64)) + uy * vy.astype(np.float64)) + uz * vz.astype(np.float64)
+    vlos_pt = los_velocity(vx, vy, vz, l, b)                                       # :110
     if not linear:
         d = np.stack([((pts[verts[:, m], 0] - q[:, 0]) ** 2 + (pts[verts[:, m], 1] - q[:, 1]) ** 2)
                       + (pts[verts[:, m], 2] - q[:, 2]) ** 2 for m in range(4)], 1)
