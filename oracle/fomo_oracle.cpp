@@ -181,7 +181,8 @@ struct Extents {
 
 // ---------------------------------------------------------------------------
 // View geometry. Every render function below takes its rotation, LOS unit vector, sample
-// positions and search half-width from these helpers, and orc_geometry exports them, so the
+// positions and search half-width from these helpers, and orc_rotate, orc_los_velocity,
+// orc_view_samples and orc_maxdistance export them, so the
 // independent checks of tests/test_oracle_geometry.py pin exactly the code the renders run.
 // All of it is left-associative FP64 (no FMA), as the reference evaluates it.
 // ---------------------------------------------------------------------------
