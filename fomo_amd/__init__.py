@@ -11,7 +11,7 @@ import numpy as np
 
 __all__ = [
     "lib", "FomoError", "init", "shutdown", "device_info", "Dataset", "Table", "Emission", "Batch",
-    "render_nn", "render_thomson", "nn_indices", "Server", "METHOD_NN", "METHOD_THOMSON", "METHOD_TET", "METHOD_LINEAR", "METHOD_PROJECTION", "delaunay", "render_tet",
+    "render_nn", "render_nn_noalloc", "render_thomson", "nn_indices", "Server", "METHOD_NN", "METHOD_THOMSON", "METHOD_TET", "METHOD_LINEAR", "METHOD_PROJECTION", "delaunay", "render_tet",
     "autotune", "tet_vertices", "render_projection", "projection_pixels", "gen_loop_cube", "gen_goft_table", "gen_corona",
     "read_table", "profile_enable", "profile_reset", "profile_stats", "LIB_PATH",
 ]
@@ -52,6 +52,9 @@ _SIGS = {
     "fomo_goft_read": (ctypes.c_int, [ctypes.c_int, _c_flt_p, _c_flt_p]),
     "fomo_render_nn": (ctypes.c_int, [ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_int,
                                       ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_int, _c_flt_p, _c_dbl_p]),
+    "fomo_render_nn_noalloc": (ctypes.c_int, [ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_int,
+                                              ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_int,
+                                              ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
     "fomo_render_thomson": (ctypes.c_int, [ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_int,
                                            ctypes.c_int, ctypes.c_int, _c_flt_p, _c_dbl_p]),
     "fomo_nn_indices": (ctypes.c_int, [ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_int,
@@ -325,6 +328,13 @@ def render_nn(em, l, b, nx, ny, nz, nl, lambda_width, dn=False, out=True):
     _check(lib().fomo_render_nn(em.id, l, b, nx, ny, nz, nl, lambda_width, 1 if dn else 0,
                                 _ptr(img, ctypes.c_float) if out else None, _ptr(meta, ctypes.c_double)))
     return img, meta
+
+
+def render_nn_noalloc(em, l, b, nx, ny, nz, nl, lambda_width, out_dev, meta_dev=None, stream=None, dn=False):
+    """Allocation-free render_nn: enqueues on `stream` (a hipStream_t as an int, None = the
+    library's stream) into device buffers given as integer addresses, and returns at once."""
+    _check(lib().fomo_render_nn_noalloc(em.id, l, b, nx, ny, nz, nl, lambda_width, 1 if dn else 0, out_dev,
+                                        meta_dev, stream))
 
 
 def render_thomson(em, l, b, nx, ny, nz):

@@ -57,6 +57,12 @@
 #define FOMO_RE 2.8179e-13
 #define FOMO_LUM 2.8e10
 
+// Scratch of the allocation-free NN render (fomo_render_nn_noalloc): one array per GPU, held
+// until the process ends. In host code `scratch` names a host-side stand-in, not device memory;
+// fomo_init takes the device address with hipGetSymbolAddress (Ctx::noalloc) and host code uses
+// only that address.
+static __device__ __align__(256) unsigned char scratch[FOMO_NOALLOC_SCRATCH_BYTES];
+
 namespace {
 
 thread_local std::string g_err;
@@ -1196,6 +1202,16 @@ __global__ void __launch_bounds__(256) k_nn_far(const float4* __restrict__ pts, 
 __global__ void k_clear_u8(uint8_t* __restrict__ a, int64_t n) {
   const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (i < n) a[i] = 0;
+}
+
+// The 10 meta doubles of a view (extents, half-width, path length, scale, apix) into device memory.
+__global__ void k_geom_meta(const ViewGeom* __restrict__ g, double* __restrict__ meta) {
+  const int t = threadIdx.x;
+  if (t < 6) meta[t] = g->ext[t];
+  else if (t == 6) meta[6] = g->md;
+  else if (t == 7) meta[7] = g->pathlength;
+  else if (t == 8) meta[8] = g->scale;
+  else if (t == 9) meta[9] = g->apix;
 }
 
 // ------------------------------------------------------------------------------------
@@ -2467,6 +2483,7 @@ struct Ctx {
   std::map<std::string, ProfStat> prof;
   std::vector<std::pair<std::string, std::pair<hipEvent_t, hipEvent_t>>> pending;
   std::vector<hipEvent_t> evpool;
+  void* noalloc = nullptr;                       // device address of `scratch` on this device
   void* stage = nullptr;                         // pinned staging of large results (copy_out)
   hipEvent_t stage_ev[2] = {nullptr, nullptr};
   std::unique_ptr<fomo::HostCopyPool> copy_pool;
@@ -3367,6 +3384,17 @@ bool shared_profile(const RecInfo& ri, const LineSet& ls) {
 // Rays below which a single-line view runs k_nn + k_los_spectral rather than the fused pass.
 constexpr int64_t kFusedMinRays = 65536;
 
+// True when a single-line spectral view of R rays runs the fused pass (k_nn_los): the 4 lanes of
+// a ray run the profile's 4 chains, of up to two lambda passes. Small images have too few rays to
+// fill the GPU this way; they run k_nn (one thread per sample) + k_los_spectral, which add the
+// same values in the same order.
+bool fused_single_line(const float4* rec, int rs4, int64_t R, int nl) {
+  if (!rec || nl < 2) return false;
+  int npass = 1;
+  (void)pick_chunk(nl, &npass);
+  return R >= kFusedMinRays && rs4 == 2 && npass <= 2;
+}
+
 // flags: R * ls.n ints of scratch. False when the view is left to k_nn + the integration kernels.
 bool enqueue_nn_los(hipStream_t s, const Dataset& ds, const ViewGeom* geom, int nx, int ny, int nz,
                     const float4* rec, int rs4, const RecInfo& ri, const LineSet& ls, int nl, int32_t* flags) {
@@ -3376,10 +3404,7 @@ bool enqueue_nn_los(hipStream_t s, const Dataset& ds, const ViewGeom* geom, int 
   int npass = 1;
   const int ch = pick_chunk(nl, &npass);
   if (ls.n == 1) {
-    // One line: the 4 lanes of a ray run the profile's 4 chains, of up to two lambda passes.
-    // Small images have too few rays to fill the GPU this way; they run k_nn (one thread per
-    // sample) + k_los_spectral, which add the same values in the same order.
-    if (R < kFusedMinRays || rs4 != 2 || npass > 2) return false;
+    if (!fused_single_line(rec, rs4, R, nl)) return false;
     ProfScope ps("k_nn_los", s);
     const unsigned blocks = (unsigned)((R + 63) / 64);
     if (npass == 1) launch_nn_los_t<2, 4, 1>(s, ch, blocks, ds, geom, nx, ny, nz, rec, ri, ls, nl, flags);
@@ -3928,6 +3953,51 @@ int method_mode(int method, int* mode) {
   }
 }
 
+// Allocation-free NN render: the kernels of fomo_render_nn with their per-view buffers carved
+// from the `scratch` array (at Ctx::noalloc) instead of the grow-only context scratch, the image
+// written to the caller's device buffer, everything enqueued on s without waiting. Views whose
+// buffers do not fit the array are rejected. Nothing is allocated: no hipMalloc/hipFree, no
+// events (kernel profiling must be off), no library routine that allocates.
+int render_nn_noalloc(int goft, double l, double b, int nx, int ny, int nz, int nl, double width, int dn,
+                      float* out, double* meta, hipStream_t s) {
+  if (int rc = require_init()) return rc;
+  if (!g_ctx.noalloc) return fail(FOMO_ERR_STATE, "the scratch array has no device address");
+  if (g_ctx.profile)
+    return fail(FOMO_ERR_STATE, "kernel profiling records events; turn it off for the allocation-free render");
+  if (!out) return fail(FOMO_ERR_ARG, "null output");
+  Goft* gc = nullptr;
+  Dataset* ds = nullptr;
+  if (int rc = resolve_render(goft, MODE_NN, nx, ny, nz, nl, &gc, &ds)) return rc;
+  const int64_t R = (int64_t)nx * ny, S = R * nz;
+  int64_t sz[kRenderBufs];
+  render_sizes(MODE_NN, ds->n, R, S, nl, sz);
+  sz[4] = 0;  // the image is the caller's
+  if (fused_single_line(gc->rec.as<float4>(), gc->rs4, R, nl)) sz[2] = sz[3] = sz[6] = sz[7] = 0;  // no per-sample buffers
+  int64_t off[kRenderBufs], total = 0;
+  for (int k = 0; k < kRenderBufs; ++k) {
+    off[k] = total;
+    total += (sz[k] + 255) & ~(int64_t)255;
+  }
+  if (total > (int64_t)FOMO_NOALLOC_SCRATCH_BYTES) {
+    char msg[160];
+    std::snprintf(msg, sizeof msg, "view needs %lld bytes of scratch, more than the %lld of the allocation-free render",
+                  (long long)total, (long long)FOMO_NOALLOC_SCRATCH_BYTES);
+    return fail(FOMO_ERR_ARG, msg);
+  }
+  char* const base = static_cast<char*>(g_ctx.noalloc);
+  const auto at = [&](int k) { return sz[k] ? static_cast<void*>(base + off[k]) : nullptr; };
+  const RenderBufs rb{static_cast<double*>(at(0)),  static_cast<ViewGeom*>(at(1)), static_cast<int32_t*>(at(2)),
+                      static_cast<float*>(at(3)),   out,                           static_cast<int32_t*>(at(5)),
+                      static_cast<float2*>(at(6)), static_cast<uint8_t*>(at(7))};
+  // the ray flags must read zero on entry; the array keeps whatever the last call left
+  if (rb.rayflag) k_clear_u8<<<nblk(R), kBlock, 0, s>>>(rb.rayflag, R);
+  HIPOK(hipGetLastError());
+  if (int rc = enqueue_render(s, *ds, *gc, MODE_NN, l, b, nx, ny, nz, nl, width, dn, rb)) return rc;
+  if (meta) k_geom_meta<<<1, 64, 0, s>>>(rb.geom, meta);
+  HIPOK(hipGetLastError());
+  return FOMO_OK;
+}
+
 }  // namespace
 
 // ======================================================================================
@@ -3951,6 +4021,10 @@ int fomo_init(int device) {
   HIPOK(hipGetDeviceProperties(&g_ctx.prop, device));
   HIPOK(hipStreamCreateWithFlags(&g_ctx.stream, hipStreamNonBlocking));
   HIPOK(hipStreamCreateWithFlags(&g_ctx.stream2, hipStreamNonBlocking));
+  // device address of the allocation-free render's scratch array on this device
+  void* sp = nullptr;
+  HIPOK(hipGetSymbolAddress(&sp, HIP_SYMBOL(scratch)));
+  g_ctx.noalloc = sp;
   g_ctx.device = device;
   g_ctx.init = true;
   return FOMO_OK;
@@ -4281,6 +4355,13 @@ int fomo_render_projection(int goft, double l, double b, int nx, int ny, int nz,
                            float* out, double* meta) {
   std::lock_guard<std::recursive_mutex> lk(g_mu);
   return render_one(goft, MODE_PROJ, l, b, nx, ny, nz, nl, width, dn, out, meta);
+}
+
+int fomo_render_nn_noalloc(int goft, double l, double b, int nx, int ny, int nz, int nl, double width, int dn,
+                           float* out_dev, double* meta_dev, void* stream) {
+  std::lock_guard<std::recursive_mutex> lk(g_mu);
+  hipStream_t s = stream ? static_cast<hipStream_t>(stream) : g_ctx.stream;
+  return render_nn_noalloc(goft, l, b, nx, ny, nz, nl, width, dn, out_dev, meta_dev, s);
 }
 
 int fomo_projection_pixels(int dataset, double l, double b, int nx, int ny, int nz, int32_t* out_pixel,

@@ -120,6 +120,24 @@ int fomo_render_tet(int goft, double l, double b, int nx, int ny, int nz, int nl
 int fomo_render_projection(int goft, double l, double b, int nx, int ny, int nz, int nl,
                            double lambda_width, int dn, float* out, double* meta);
 
+/* Allocation-free variant of fomo_render_nn. No counterpart in the reference: it is for callers
+ * that cannot allocate device memory (code under HIP graph capture, tight serving loops, hosts
+ * that forbid hipMalloc). It runs the same kernels as fomo_render_nn and gives the same image
+ * bit for bit, but takes its per-view buffers from one static device array of
+ * FOMO_NOALLOC_SCRATCH_BYTES per GPU (held until the process ends) and calls no hipMalloc,
+ * hipFree or allocating library routine. Views whose buffers exceed the array fail with
+ * FOMO_ERR_ARG: a single-line view of at least 65,536 rays needs about 5 bytes per ray, any
+ * other view about 16 bytes per LOS sample. Kernel profiling (fomo_profile_enable) must be off.
+ * out_dev: device buffer of ny*nx*max(nl,1) floats (RenderCube layout); meta_dev (optional):
+ * device buffer of the 10 meta doubles. The work is enqueued on `stream` (a hipStream_t; NULL =
+ * the library's stream) and the call returns without waiting: synchronize that stream before
+ * reading out_dev. Every call on one GPU shares its array, so calls whose work overlaps on the
+ * device race; serialize them on one stream. The emission cube and its dataset must outlive the
+ * enqueued work. */
+#define FOMO_NOALLOC_SCRATCH_BYTES (64ll << 20)
+int fomo_render_nn_noalloc(int goft, double l, double b, int nx, int ny, int nz, int nl, double lambda_width,
+                           int dn, float* out_dev, double* meta_dev, void* stream);
+
 /* The projection's pixel iy*nx + jx of every point (original order) for one view. For parity
  * checks of the pixel assignment (:129-131). */
 int fomo_projection_pixels(int dataset, double l, double b, int nx, int ny, int nz, int32_t* out_pixel,
