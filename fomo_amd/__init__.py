@@ -40,6 +40,7 @@ _SIGS = {
     "fomo_dataset_reindex": (ctypes.c_int, [ctypes.c_int, ctypes.c_double]),
     "fomo_dataset_set_lookup": (ctypes.c_int, [ctypes.c_int, ctypes.c_int]),
     "fomo_dataset_lookup_info": (ctypes.c_int, [ctypes.c_int, _c_int_p, _c_i64_p, _c_i64_p, _c_i64_p]),
+    "fomo_dataset_lookup_groups": (ctypes.c_int, [ctypes.c_int, _c_i64_p]),
     "fomo_table_read": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int, _c_dbl_p, _c_dbl_p,
                                        ctypes.c_char_p, ctypes.c_int, _c_int_p, _c_int_p, _c_flt_p, _c_flt_p,
                                        _c_flt_p]),
@@ -94,6 +95,7 @@ _SIGS = {
     "fomo_batch_sync": (ctypes.c_int, [ctypes.c_int]),
     "fomo_batch_profile": (ctypes.c_int, [ctypes.c_int]),
     "fomo_batch_read": (ctypes.c_int, [ctypes.c_int, _c_flt_p, _c_dbl_p]),
+    "fomo_batch_read_view": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int, _c_flt_p, _c_dbl_p]),
     "fomo_batch_destroy": (ctypes.c_int, [ctypes.c_int]),
     "fomo_profile_enable": (ctypes.c_int, [ctypes.c_int]),
     "fomo_profile_reset": (ctypes.c_int, []),
@@ -229,6 +231,13 @@ class Dataset:
         _check(lib().fomo_dataset_lookup_info(self.id, ctypes.byref(sd), ctypes.byref(cells), ctypes.byref(cand),
                                               ctypes.byref(fb)))
         return dict(subdiv=sd.value, cells=cells.value, candidates=cand.value, fallback_cells=fb.value)
+
+    def lookup_groups(self):
+        """Far and outer groups of the lookup grid and how many hold a shared candidate list."""
+        c = np.zeros(5, np.int64)
+        _check(lib().fomo_dataset_lookup_groups(self.id, _ptr(c, ctypes.c_int64)))
+        return dict(far_groups=int(c[0]), far_lists=int(c[1]), full_groups=int(c[2]), outer_groups=int(c[3]),
+                    outer_lists=int(c[4]))
 
     def triangulate(self):
         """Build the Delaunay triangulation for the tetra path; returns (tets, seconds)."""
@@ -432,6 +441,13 @@ class Batch:
         out = np.zeros(self.shape, np.float32)
         meta = np.zeros((self.shape[0], 10), np.float64)
         _check(lib().fomo_batch_read(self.id, _ptr(out, ctypes.c_float), _ptr(meta, ctypes.c_double)))
+        return out, meta
+
+    def read_view(self, view, line):
+        """One image (ny*nx*max(nl,1) floats) and its view's meta."""
+        out = np.zeros(self.shape[2] * self.shape[3] * self.shape[4], np.float32)
+        meta = np.zeros(10, np.float64)
+        _check(lib().fomo_batch_read_view(self.id, view, line, _ptr(out, ctypes.c_float), _ptr(meta, ctypes.c_double)))
         return out, meta
 
     def close(self):

@@ -2397,6 +2397,9 @@ struct Dataset {
   DevBuf mpts;         // float4 (x, y, z, sorted position) in Morton order
   DevBuf nlo, nhi;     // node boxes, all levels
   int64_t ncand = 0, nlcells = 0, nfallback = 0;
+  // far and outer groups of the lookup grid: groups, groups with a candidate list, far groups
+  // whose cells all take the list (group headers)
+  int64_t far_groups = 0, far_lists = 0, full_groups = 0, outer_groups = 0, outer_lists = 0;
   // tetra path: Delaunay triangulation of the points, built on the host on first use
   int64_t ntets = 0;      // 0: not built
   DevBuf trec;            // 4 float4 per tet: vertex {x, y, z, sorted position as int bits (-1 = infinite)}
@@ -2848,8 +2851,20 @@ int build_far_groups(Dataset& ds, LookupInfo& lk, int subdiv, int64_t ncl, int32
     (void)hipGetLastError();
   }
   HIPOK(hipGetLastError());
+  // group counts for fomo_dataset_lookup_groups
+  {
+    std::vector<int2> hi((size_t)ng), ho(do_outer ? (size_t)ong : 0);
+    HIPOK(hipMemcpyAsync(hi.data(), ginfo.p, sizeof(int2) * ng, hipMemcpyDeviceToHost, s));
+    if (do_outer) HIPOK(hipMemcpyAsync(ho.data(), oinfo.p, sizeof(int2) * ong, hipMemcpyDeviceToHost, s));
+    if (int rc = sync_stream()) return rc;
+    ds.far_groups = ng;
+    ds.far_lists = std::count_if(hi.begin(), hi.end(), [](const int2& g) { return g.y > 0; });
+    ds.outer_groups = do_outer ? ong : 0;
+    ds.outer_lists = std::count_if(ho.begin(), ho.end(), [](const int2& g) { return g.y > 0; });
+  }
   if (int rc = sync_stream()) return rc;
   if (nfull == 0) ds.ghdr.release();
+  ds.full_groups = (int64_t)nfull;
   std::swap(ds.lcand.p, lc.p);
   std::swap(ds.lcand.bytes, lc.bytes);
   if (do_outer) {
@@ -2877,6 +2892,7 @@ int build_lookup(Dataset& ds) {
   ds.ghdr.release();
   ds.lcand.release();
   ds.ncand = ds.nlcells = ds.nfallback = 0;
+  ds.far_groups = ds.far_lists = ds.full_groups = ds.outer_groups = ds.outer_lists = 0;
   if (ds.lookup_subdiv <= 0) return FOMO_OK;
   hipStream_t s = g_ctx.stream;
   const GridInfo& g = ds.grid;
@@ -4169,6 +4185,20 @@ int fomo_dataset_lookup_info(int id, int* subdiv, int64_t* cells, int64_t* candi
   return FOMO_OK;
 }
 
+int fomo_dataset_lookup_groups(int id, int64_t* counts5) {
+  std::lock_guard<std::recursive_mutex> lk(g_mu);
+  if (int rc = require_init()) return rc;
+  Dataset* ds = find_obj(g_ctx.datasets, id);
+  if (!ds) return fail(FOMO_ERR_NOTFOUND, "unknown dataset id");
+  if (!counts5) return fail(FOMO_ERR_ARG, "null output");
+  counts5[0] = ds->far_groups;
+  counts5[1] = ds->far_lists;
+  counts5[2] = ds->full_groups;
+  counts5[3] = ds->outer_groups;
+  counts5[4] = ds->outer_lists;
+  return FOMO_OK;
+}
+
 int fomo_dataset_reindex(int id, double cell_scale) {
   std::lock_guard<std::recursive_mutex> lk(g_mu);
   if (int rc = require_init()) return rc;
@@ -4761,6 +4791,24 @@ int fomo_batch_read(int id, float* out, double* meta) {
       meta[10 * v + 8] = hg[v].scale;
       meta[10 * v + 9] = hg[v].apix;
     }
+  return FOMO_OK;
+}
+
+int fomo_batch_read_view(int id, int view, int line, float* out, double* meta) {
+  std::lock_guard<std::recursive_mutex> lk(g_mu);
+  if (int rc = require_init()) return rc;
+  Batch* bt = find_obj(g_ctx.batches, id);
+  if (!bt) return fail(FOMO_ERR_NOTFOUND, "unknown batch id");
+  const int nv = (int)bt->l.size(), ng = (int)bt->gofts.size();
+  if (view < 0 || view >= nv || line < 0 || line >= ng) return fail(FOMO_ERR_ARG, "view or line out of range");
+  const size_t img = (size_t)bt->nx * bt->ny * std::max(bt->nl, 1);
+  const float* src = bt->out.as<float>() + ((size_t)view * ng + line) * img;
+  if (out)
+    if (int rc = copy_out(out, src, (int64_t)sizeof(float) * img, g_ctx.stream)) return rc;
+  ViewGeom hg;
+  HIPOK(hipMemcpyAsync(&hg, bt->geom.as<ViewGeom>() + view, sizeof(ViewGeom), hipMemcpyDeviceToHost, g_ctx.stream));
+  HIPOK(hipStreamSynchronize(g_ctx.stream));
+  if (meta) geom_to_meta(hg, meta);
   return FOMO_OK;
 }
 

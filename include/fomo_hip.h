@@ -63,6 +63,12 @@ int fomo_dataset_set_lookup(int id, int subdiv);
    shared list counts once per cell that uses it), cells left to the exact octree query. */
 int fomo_dataset_lookup_info(int id, int* subdiv, int64_t* cells, int64_t* candidates, int64_t* fallback_cells);
 
+/* Far and outer groups of the lookup grid (targets more than 2 cells from any point, and targets
+ * beyond the grid, whose cells read a group's shared candidate list). counts5: far groups, far
+ * groups with a list, far groups whose cells all take their list (group headers), outer groups,
+ * outer groups with a list. All 0 when the dataset has none. */
+int fomo_dataset_lookup_groups(int id, int64_t* counts5);
+
 /* ---------------------------------------------------------------------------------------
  * G(T,n) tables. The host-side parse of readgoftfromchianti (fomo-CHIANTI.cpp:143-218) is
  * fomo_table_read; fomo_table_create uploads a parsed table (nt log T values, nrho log n
@@ -184,6 +190,9 @@ int fomo_batch_sync(int batch);
 int fomo_batch_profile(int batch);
 /* out: [view][goft][ny][nx][nl] floats; meta: nviews*10 doubles (either may be NULL). */
 int fomo_batch_read(int batch, float* out, double* meta);
+/* One image of a batch: view `view`, emission cube number `line` of the batch (ny*nx*max(nl,1)
+ * floats), and that view's 10 meta doubles (either may be NULL). */
+int fomo_batch_read_view(int batch, int view, int line, float* out, double* meta);
 int fomo_batch_destroy(int batch);
 
 /* ---------------------------------------------------------------------------------------

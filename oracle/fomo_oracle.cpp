@@ -137,9 +137,12 @@ struct KdTree {
     return s;
   }
 
-  int32_t query(const float t[3], const float blo[3], const float bhi[3], double* outd = nullptr) const {
+  // tie (optional): set when another point in the box lies at exactly the nearest distance.
+  int32_t query(const float t[3], const float blo[3], const float bhi[3], double* outd = nullptr,
+                bool* tie = nullptr) const {
     double best = std::numeric_limits<double>::infinity();
     int32_t bi = -1;
+    bool tied = false;
     if (nodes.empty()) return -1;
     int32_t stack[128];
     int sp = 0;
@@ -160,7 +163,8 @@ struct KdTree {
                 blo[2] < q[2] && q[2] < bhi[2]))
             continue;
           double d = dist2(q, t);
-          if (d < best || (d == best && k < bi)) { best = d; bi = k; }
+          if (d < best) { best = d; bi = k; tied = false; }
+          else if (d == best) { tied = true; if (k < bi) bi = k; }
         }
       } else {
         const Node& L = nodes[nd.left];
@@ -171,6 +175,7 @@ struct KdTree {
       }
     }
     if (outd) *outd = best;
+    if (tie) *tie = tied;
     return bi;
   }
 };
@@ -487,19 +492,25 @@ void orc_nn_kdtree(int64_t n, const float* x, const float* y, const float* z, in
 }
 
 // ---------------------------------------------------------------------------
-// A7-A11 + A14: nearestneighbourinterpolation, fomo-nearestneighbour.cpp:31-281.
-// out: ny*nx*max(nl,1) intensities (index (i*nx+j)*nl+il), already scaled.
+// A7-A11 + A14: nearestneighbourinterpolation, fomo-nearestneighbour.cpp:31-281, for nlines
+// emission cubes of one dataset at once (the reference renders each in its own call; the NN
+// query does not depend on the line, so one query per sample serves every line).
+// peak, fwhm: nlines * n values (line-major). out: nlines * ny*nx*max(nl,1) intensities (line
+// major, index (i*nx+j)*nl+il within a line), already scaled.
 // out_idx (optional): ny*nx*nz nearest original indices (-1 = none), index (i*nx+j)*nz+k.
+// out_tie (optional): ny*nx*nz flags, 1 where another point in the box lies at exactly the
+// nearest distance (the chosen index then rests on the tie rule, lowest index here).
 // meta (optional, 10 doubles): minx,maxx,miny,maxy,minz,maxz,maxdistance,pathlength,scale,apix.
 // dn!=0 applies the DN/arcsec branch (:252-265). Returns 0, or <0 on bad sizes.
 // timing (optional, 3 doubles): seconds for rotate+extents, index build, ray loop.
 // row0,row1: render only image rows [row0,row1) (row1<=0: all rows); other rows are untouched.
 // ---------------------------------------------------------------------------
-int orc_render_nn(int mode, int64_t n, const float* x, const float* y, const float* z,
-                  const double* peak, const double* fwhm, const float* vx, const float* vy, const float* vz,
-                  double l, double b, int nx, int ny, int nz, int nl, double lambda_width, double lambda0,
-                  int dn, double* out, int32_t* out_idx, double* meta, double* timing, int row0, int row1) {
-  if (nx < 2 || ny < 2 || nz < 1 || nl < 1 || n < 1) return -1;
+int orc_render_nn_multi(int mode, int64_t n, const float* x, const float* y, const float* z, int nlines,
+                        const double* peak, const double* fwhm, const float* vx, const float* vy, const float* vz,
+                        double l, double b, int nx, int ny, int nz, int nl, double lambda_width, double lambda0,
+                        int dn, double* out, int32_t* out_idx, uint8_t* out_tie, double* meta, double* timing,
+                        int row0, int row1) {
+  if (nx < 2 || ny < 2 || nz < 1 || nl < 1 || n < 1 || nlines < 1) return -1;
   if (row1 <= 0 || row1 > ny) row1 = ny;
   if (row0 < 0) row0 = 0;
 #ifdef _OPENMP
@@ -531,18 +542,28 @@ int orc_render_nn(int mode, int64_t n, const float* x, const float* y, const flo
   const double lambda_width_in_A = lambda_width * lambda0 / kSpeedOfLight;  // :116
   const double maxdistance = nn_maxdistance(ext, nx, ny);  // :138
   const int nlo = nl > 1 ? nl : 1;
+  const int64_t img = (int64_t)nx * ny * nlo;
   std::vector<float> peakf, fwhmf;
   if (mode == 0) {
-    peakf.resize(n);
-    fwhmf.resize(n);
-    for (int64_t i = 0; i < n; ++i) { peakf[i] = (float)peak[i]; fwhmf[i] = (float)fwhm[i]; }
+    peakf.resize((size_t)n * nlines);
+    fwhmf.resize((size_t)n * nlines);
+    for (int64_t i = 0; i < n * nlines; ++i) { peakf[i] = (float)peak[i]; fwhmf[i] = (float)fwhm[i]; }
   }
+  double pathlength = (maxz - minz) / (nz - 1);  // :235-267 scaling
+  if ((maxz == minz) || (nz == 1)) pathlength = 1.;
+  double apix = 1.;
+  if (dn) {
+    float dx = (maxx - minx) / (nx - 1), dy = (maxy - miny) / (ny - 1);
+    apix = (dx / kMmPerArcsec) * (dy / kMmPerArcsec) * pow(kPi / 180. / 3600., 2);
+  }
+  const double sc = pathlength * 1e8 * apix;
 #pragma omp parallel for schedule(dynamic, 1) collapse(2)
   for (int i = row0; i < row1; ++i)
     for (int j = 0; j < nx; ++j) {
-      std::vector<float> accf(nlo, 0.f);
-      std::vector<double> accd(nlo, 0.0);
-      double intpolpeak = 0, intpolfwhm = 0, intpollosvel = 0;
+      std::vector<float> accf((size_t)nlo * nlines, 0.f);
+      std::vector<double> accd((size_t)nlo * nlines, 0.0);
+      std::vector<double> intpolpeak(nlines, 0.0), intpolfwhm(nlines, 0.0);
+      double intpollosvel = 0;
       for (int k = 0; k < nz; ++k) {
         double xx, yy, zz, p0, p1, p2;
         sample_xyz(0, ext, nx, ny, nz, i, j, k, &xx, &yy, &zz);  // :152-162
@@ -550,42 +571,46 @@ int orc_render_nn(int mode, int64_t n, const float* x, const float* y, const flo
         float t[3] = {(float)p0, (float)p1, (float)p2};  // :171
         float blo[3] = {(float)(p0 - maxdistance), (float)(p1 - maxdistance), (float)(p2 - maxdistance)};  // :179
         float bhi[3] = {(float)(p0 + maxdistance), (float)(p1 + maxdistance), (float)(p2 + maxdistance)};
-        int32_t nearest = kd.query(t, blo, bhi);  // :181
-        if (out_idx) out_idx[((int64_t)i * nx + j) * nz + k] = nearest;
-        if (nearest >= 0) {  // :183-193
-          intpolpeak = mode == 0 ? (double)peakf[nearest] : peak[nearest];
-          intpolfwhm = mode == 0 ? (double)fwhmf[nearest] : fwhm[nearest];
-          intpollosvel = losvel[nearest];
-        } else {
-          intpolpeak = 0;
-        }
-        if (nl > 1) {  // :195-215
-          for (int il = 0; il < nl; ++il) {
-            double lambdaval = double(il) / (nl - 1) * lambda_width_in_A - lambda_width_in_A / 2.;
-            double tempintens = intpolpeak
-                ? intpolpeak * exp(-pow(lambdaval - intpollosvel / kSpeedOfLight * lambda0, 2) / pow(intpolfwhm, 2) * 4. * log(2.))
-                : 0;
-            if (mode == 0) accf[il] = (float)((double)accf[il] + tempintens);
-            else accd[il] += tempintens;
+        bool tie = false;
+        int32_t nearest = kd.query(t, blo, bhi, nullptr, out_tie ? &tie : nullptr);  // :181
+        const int64_t sidx = ((int64_t)i * nx + j) * nz + k;
+        if (out_idx) out_idx[sidx] = nearest;
+        if (out_tie) out_tie[sidx] = tie ? 1 : 0;
+        for (int g = 0; g < nlines; ++g) {
+          const int64_t q = (int64_t)g * n + nearest;
+          if (nearest >= 0) {  // :183-193
+            intpolpeak[g] = mode == 0 ? (double)peakf[q] : peak[q];
+            intpolfwhm[g] = mode == 0 ? (double)fwhmf[q] : fwhm[q];
+          } else {
+            intpolpeak[g] = 0;
           }
-        } else {  // :217-225
-          if (mode == 0) accf[0] = (float)((double)accf[0] + intpolpeak);
-          else accd[0] += intpolpeak;
+        }
+        if (nearest >= 0) intpollosvel = losvel[nearest];
+        for (int g = 0; g < nlines; ++g) {
+          float* af = mode == 0 ? &accf[(size_t)g * nlo] : nullptr;
+          double* ad = mode == 0 ? nullptr : &accd[(size_t)g * nlo];
+          if (nl > 1) {  // :195-215
+            for (int il = 0; il < nl; ++il) {
+              double lambdaval = double(il) / (nl - 1) * lambda_width_in_A - lambda_width_in_A / 2.;
+              double tempintens = intpolpeak[g]
+                  ? intpolpeak[g] * exp(-pow(lambdaval - intpollosvel / kSpeedOfLight * lambda0, 2) /
+                                        pow(intpolfwhm[g], 2) * 4. * log(2.))
+                  : 0;
+              if (mode == 0) af[il] = (float)((double)af[il] + tempintens);
+              else ad[il] += tempintens;
+            }
+          } else {  // :217-225
+            if (mode == 0) af[0] = (float)((double)af[0] + intpolpeak[g]);
+            else ad[0] += intpolpeak[g];
+          }
         }
       }
-      // :235-267 scaling
-      double pathlength = (maxz - minz) / (nz - 1);
-      if ((maxz == minz) || (nz == 1)) pathlength = 1.;
-      double apix = 1.;
-      if (dn) {
-        float dx = (maxx - minx) / (nx - 1), dy = (maxy - miny) / (ny - 1);
-        apix = (dx / kMmPerArcsec) * (dy / kMmPerArcsec) * pow(kPi / 180. / 3600., 2);
-      }
-      const double sc = pathlength * 1e8 * apix;
-      for (int il = 0; il < nlo; ++il) {
-        int64_t ind = ((int64_t)i * nx + j) * nlo + il;
-        out[ind] = mode == 0 ? (double)(float)(sc * (double)accf[il]) : sc * accd[il];
-      }
+      for (int g = 0; g < nlines; ++g)
+        for (int il = 0; il < nlo; ++il) {
+          const int64_t ind = (int64_t)g * img + ((int64_t)i * nx + j) * nlo + il;
+          const size_t a = (size_t)g * nlo + il;
+          out[ind] = mode == 0 ? (double)(float)(sc * (double)accf[a]) : sc * accd[a];
+        }
     }
 #ifdef _OPENMP
   double t3 = omp_get_wtime();
@@ -593,18 +618,20 @@ int orc_render_nn(int mode, int64_t n, const float* x, const float* y, const flo
   double t3 = 0;
 #endif
   if (meta) {
-    double pathlength = (maxz - minz) / (nz - 1);
-    if ((maxz == minz) || (nz == 1)) pathlength = 1.;
-    double apix = 1.;
-    if (dn) {
-      float dx = (maxx - minx) / (nx - 1), dy = (maxy - miny) / (ny - 1);
-      apix = (dx / kMmPerArcsec) * (dy / kMmPerArcsec) * pow(kPi / 180. / 3600., 2);
-    }
-    double m[10] = {minx, maxx, miny, maxy, minz, maxz, maxdistance, pathlength, pathlength * 1e8 * apix, apix};
+    double m[10] = {minx, maxx, miny, maxy, minz, maxz, maxdistance, pathlength, sc, apix};
     std::memcpy(meta, m, sizeof(m));
   }
   if (timing) { timing[0] = t1 - t0; timing[1] = t2 - t1; timing[2] = t3 - t2; }
   return 0;
+}
+
+// One emission cube: orc_render_nn_multi with nlines = 1.
+int orc_render_nn(int mode, int64_t n, const float* x, const float* y, const float* z,
+                  const double* peak, const double* fwhm, const float* vx, const float* vy, const float* vz,
+                  double l, double b, int nx, int ny, int nz, int nl, double lambda_width, double lambda0,
+                  int dn, double* out, int32_t* out_idx, double* meta, double* timing, int row0, int row1) {
+  return orc_render_nn_multi(mode, n, x, y, z, 1, peak, fwhm, vx, vy, vz, l, b, nx, ny, nz, nl, lambda_width, lambda0,
+                             dn, out, out_idx, nullptr, meta, timing, row0, row1);
 }
 
 // ---------------------------------------------------------------------------
@@ -680,12 +707,15 @@ int orc_render_projection(int mode, int64_t n, const float* x, const float* y, c
 
 // ---------------------------------------------------------------------------
 // A12: thomsoninterpolation, fomo-thomson.cpp:94-367 (pB, nearest of a k=5 query).
-// out: ny*nx pB image (scaled). out_idx / meta as in orc_render_nn.
+// out: ny*nx pB image (scaled). out_idx, out_tie, meta and rows as in orc_render_nn_multi.
 // ---------------------------------------------------------------------------
 int orc_render_thomson(int mode, int64_t n, const float* x, const float* y, const float* z,
                        const double* totn, const double* poln, double l, double b, int nx, int ny, int nz,
-                       double* out, int32_t* out_idx, double* meta, double* timing) {
+                       double* out, int32_t* out_idx, uint8_t* out_tie, double* meta, double* timing, int row0,
+                       int row1) {
   if (nx < 2 || ny < 2 || nz < 1 || n < 1) return -1;
+  if (row1 <= 0 || row1 > ny) row1 = ny;
+  if (row0 < 0) row0 = 0;
 #ifdef _OPENMP
   double t0 = omp_get_wtime();
 #else
@@ -723,7 +753,7 @@ int orc_render_thomson(int mode, int64_t n, const float* x, const float* y, cons
     for (int64_t i = 0; i < n; ++i) { totf[i] = (float)totn[i]; polf[i] = (float)poln[i]; }
   }
 #pragma omp parallel for schedule(dynamic, 1) collapse(2)
-  for (int i = 0; i < ny; ++i)
+  for (int i = row0; i < row1; ++i)
     for (int j = 0; j < nx; ++j) {
       float accf = 0.f;
       double accd = 0.0;
@@ -734,8 +764,10 @@ int orc_render_thomson(int mode, int64_t n, const float* x, const float* y, cons
         float t[3] = {(float)p0, (float)p1, (float)p2};
         float blo[3] = {(float)(p0 - maxdistance), (float)(p1 - maxdistance), (float)(p2 - maxdistance)};  // :308
         float bhi[3] = {(float)(p0 + maxdistance), (float)(p1 + maxdistance), (float)(p2 + maxdistance)};
-        int32_t nearest = kd.query(t, blo, bhi);
+        bool tie = false;
+        int32_t nearest = kd.query(t, blo, bhi, nullptr, out_tie ? &tie : nullptr);
         if (out_idx) out_idx[((int64_t)i * nx + j) * nz + k] = nearest;
+        if (out_tie) out_tie[((int64_t)i * nx + j) * nz + k] = tie ? 1 : 0;
         double intpoltot = 0, intpolpol = 0, intpollos = 0;
         if (nearest >= 0) {
           intpoltot = mode == 0 ? (double)totf[nearest] : totn[nearest];

@@ -18,6 +18,7 @@ _f = ctypes.POINTER(ctypes.c_float)
 _d = ctypes.POINTER(ctypes.c_double)
 _i = ctypes.POINTER(ctypes.c_int32)
 _ip = ctypes.POINTER(ctypes.c_int)
+_u8 = ctypes.POINTER(ctypes.c_uint8)
 
 
 def build():
@@ -60,7 +61,13 @@ def load(path):
                                 ctypes.c_int]
     L.orc_render_thomson.restype = ctypes.c_int
     L.orc_render_thomson.argtypes = [ctypes.c_int, ctypes.c_int64, _f, _f, _f, _d, _d, ctypes.c_double,
-                                     ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.c_int, _d, _i, _d, _d]
+                                     ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.c_int, _d, _i, _u8, _d, _d,
+                                     ctypes.c_int, ctypes.c_int]
+    L.orc_render_nn_multi.restype = ctypes.c_int
+    L.orc_render_nn_multi.argtypes = [ctypes.c_int, ctypes.c_int64, _f, _f, _f, ctypes.c_int, _d, _d, _f, _f, _f,
+                                      ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                      ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_int, _d, _i, _u8, _d,
+                                      _d, ctypes.c_int, ctypes.c_int]
     L.orc_render_projection.restype = ctypes.c_int
     L.orc_render_projection.argtypes = [ctypes.c_int, ctypes.c_int64, _f, _f, _f, _d, _d, _f, _f, _f,
                                         ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_int,
@@ -221,19 +228,50 @@ def render_nn(mode, x, y, z, peak, fwhm, vx, vy, vz, l, b, nx, ny, nz, nl, lambd
     return out, idx, meta, tim
 
 
-def render_thomson(mode, x, y, z, totn, poln, l, b, nx, ny, nz, want_idx=False):
+def render_nn_multi(mode, x, y, z, peaks, fwhms, vx, vy, vz, l, b, nx, ny, nz, nl, lambda_width, lambda0,
+                    dn=False, rows=None):
+    """Several emission cubes (lists of per-point peak and fwhm arrays) of one dataset, one NN query
+    per sample for all. Returns (images [nlines, ny*nx*max(nl,1)], idx [ny*nx*nz], tie flags
+    [ny*nx*nz], meta[10], timing[3]); rows=(r0, r1) renders only those image rows."""
+    x, y, z, vx, vy, vz = map(_pf, (x, y, z, vx, vy, vz))
+    L = len(peaks)
+    pk = _pd(np.concatenate([np.asarray(p, np.float64) for p in peaks]))
+    fw = _pd(np.concatenate([np.asarray(f, np.float64) for f in fwhms]))
+    out = np.zeros(L * nx * ny * max(nl, 1))
+    idx = np.zeros(nx * ny * nz, np.int32)
+    tie = np.zeros(nx * ny * nz, np.uint8)
+    meta = np.zeros(10)
+    tim = np.zeros(3)
+    rc = lib().orc_render_nn_multi(mode, x.size, P(x, ctypes.c_float), P(y, ctypes.c_float), P(z, ctypes.c_float), L,
+                                   P(pk, ctypes.c_double), P(fw, ctypes.c_double), P(vx, ctypes.c_float),
+                                   P(vy, ctypes.c_float), P(vz, ctypes.c_float), l, b, nx, ny, nz, nl, lambda_width,
+                                   lambda0, 1 if dn else 0, P(out, ctypes.c_double), P(idx, ctypes.c_int32),
+                                   P(tie, ctypes.c_uint8), P(meta, ctypes.c_double), P(tim, ctypes.c_double),
+                                   rows[0] if rows else 0, rows[1] if rows else 0)
+    if rc != 0:
+        raise ValueError(f"oracle render_nn_multi rejected the arguments ({rc})")
+    return out.reshape(L, -1), idx, tie, meta, tim
+
+
+def render_thomson(mode, x, y, z, totn, poln, l, b, nx, ny, nz, want_idx=False, rows=None, want_tie=False):
+    """Returns (image [ny*nx], idx [ny*nx*nz] or None, meta[10], timing[3]); with want_tie also the
+    per-sample tie flags. rows=(r0, r1) renders only those image rows (the rest stay 0)."""
     x, y, z = map(_pf, (x, y, z))
     totn, poln = _pd(totn), _pd(poln)
     out = np.zeros(nx * ny)
     idx = np.zeros(nx * ny * nz, np.int32) if want_idx else None
+    tie = np.zeros(nx * ny * nz, np.uint8) if want_tie else None
     meta = np.zeros(10)
     tim = np.zeros(3)
     rc = lib().orc_render_thomson(mode, x.size, P(x, ctypes.c_float), P(y, ctypes.c_float), P(z, ctypes.c_float),
                                   P(totn, ctypes.c_double), P(poln, ctypes.c_double), l, b, nx, ny, nz,
                                   P(out, ctypes.c_double), P(idx, ctypes.c_int32) if want_idx else None,
-                                  P(meta, ctypes.c_double), P(tim, ctypes.c_double))
+                                  P(tie, ctypes.c_uint8) if want_tie else None, P(meta, ctypes.c_double),
+                                  P(tim, ctypes.c_double), rows[0] if rows else 0, rows[1] if rows else 0)
     if rc != 0:
         raise ValueError(f"oracle render_thomson rejected the arguments ({rc})")
+    if want_tie:
+        return out, idx, meta, tim, tie
     return out, idx, meta, tim
 
 
@@ -268,14 +306,15 @@ def rel_l2(a, ref):
 
 
 def render_tet(x, y, z, peak, fwhm, vx, vy, vz, l, b, nx, ny, nz, nl, lambda_width, lambda0, dn=False,
-               linear=False, tri=None):
+               linear=False, tri=None, rows=None):
     """The reference's CGAL render (fomo-CGAL.cpp:75-307) restated in numpy, FP64, on an independent
     triangulation: scipy.spatial.Delaunay (Qhull) and its find_simplex for the point location
     (CGAL's locate, :194-206). Each sample takes the values of the nearest vertex of its tet
     (nearest_vertex_in_cell, :211-222), none outside the convex hull (:225-228); linear=True takes
     the barycentric interpolation instead (the "Linear" extension). Extents come from the NN
     oracle's rotation (the same formula, :118-121). tri: a scipy Delaunay of these points to reuse
-    (the reference triangulates once per render() call, :33-73). Returns (image
+    (the reference triangulates once per render() call, :33-73). rows=(r0, r1): only those image
+    rows are rendered (the others stay 0 in the image and -1 in the vertices). Returns (image
     [ny*nx*max(nl,1)], vertex per sample [(iy*nx + jx)*nz + k], original index or -1, meta[10])."""
     import math
 
@@ -287,8 +326,10 @@ def render_tet(x, y, z, peak, fwhm, vx, vy, vz, l, b, nx, ny, nz, nl, lambda_wid
     if tri is None:
         tri = Delaunay(pts)
     minx, maxx, miny, maxy, minz, maxz = extents(x, y, z, l, b)
-    shape = (ny, nx, nz)
-    px, py, pz = view_samples([minx, maxx, miny, maxy, minz, maxz], l, b, nx, ny, nz, cgal=True)  # :187-192
+    r0, r1 = rows if rows else (0, ny)
+    shape = (r1 - r0, nx, nz)
+    px, py, pz = (a[r0:r1] for a in view_samples([minx, maxx, miny, maxy, minz, maxz], l, b, nx, ny, nz,
+                                                 cgal=True))  # :187-192
     q = np.stack([px.ravel(), py.ravel(), pz.ravel()], 1)
     s = tri.find_simplex(q)
     inside = s >= 0
@@ -320,9 +361,9 @@ def render_tet(x, y, z, peak, fwhm, vx, vy, vz, l, b, nx, ny, nz, nl, lambda_wid
             lam = il / (nl - 1) * wla - wla / 2.0                                  # :237
             with np.errstate(over="ignore", invalid="ignore", divide="ignore"):
                 t = pk * np.exp(-((lam - vl / C * lambda0) ** 2) / fw ** 2 * 4.0 * math.log(2.0))
-            img[:, :, il] = np.where(pk != 0, t, 0.0).sum(2)
+            img[r0:r1, :, il] = np.where(pk != 0, t, 0.0).sum(2)
     else:
-        img[:, :, 0] = pk.sum(2)
+        img[r0:r1, :, 0] = pk.sum(2)
     pathlength = (maxz - minz) / (nz - 1) if nz > 1 else 1.0                      # :263-271
     if maxz == minz:
         pathlength = 1.0
@@ -333,4 +374,6 @@ def render_tet(x, y, z, peak, fwhm, vx, vy, vz, l, b, nx, ny, nz, nl, lambda_wid
         apix = (float(dxf) / 0.715) * (float(dyf) / 0.715) * (math.pi / 180.0 / 3600.0) ** 2
     scale = pathlength * 1e8 * apix
     meta = np.array([minx, maxx, miny, maxy, minz, maxz, 0.0, pathlength, scale, apix])
-    return (img * scale).reshape(-1), vert.reshape(ny, nx, nz).reshape(-1), meta
+    vfull = np.full((ny, nx, nz), -1, np.int32)
+    vfull[r0:r1] = vert.reshape(shape)
+    return (img * scale).reshape(-1), vfull.reshape(-1), meta

@@ -449,13 +449,30 @@ def test_far_group_lists_match_exact_query(engine):
     # without group lists every cell more than 8 cell-list cells from a point (the box corners
     # beyond r = 3 and the middle of the hole) is left to the exact query
     assert info["fallback_cells"] < 0.05 * info["cells"], info
+    # the group paths this test covers are built: far groups whose cells all read the group's
+    # header (no per-cell record), and outer groups for targets beyond the lookup grid
+    groups = ds.lookup_groups()
+    print("lookup groups:", groups, flush=True)
+    assert groups["far_lists"] > 0 and groups["full_groups"] > 0, groups
+    assert groups["outer_groups"] > 0 and groups["outer_lists"] > 0, groups
     views = [(0.0, np.pi / 2), (0.9, 0.3), (2.2, 0.0), (np.pi / 4, np.pi / 4)]
+    # the rotated views place LOS samples beyond the lookup grid (the data box plus 6 cell-list
+    # cells on each side), where the outer groups answer
+    n_, dims, _ = ds.info()
+    lo = np.array([d["x"].min(), d["y"].min(), d["z"].min()], np.float64)
+    hi = np.array([d["x"].max(), d["y"].max(), d["z"].max()], np.float64)
+    margin = 7.0 * (hi - lo) / (np.array(dims) - 1)
+    ext = O.extents(d["x"], d["y"], d["z"], np.pi / 4, np.pi / 4)
+    p = np.stack([a.ravel() for a in O.view_samples(ext, np.pi / 4, np.pi / 4, 56, 52, 60)], 1)
+    beyond = np.any((p < lo - margin) | (p > hi + margin), 1)
+    assert beyond.sum() > 1000, beyond.sum()
     fast = [F.nn_indices(ds, l, b, 56, 52, 60, thomson=True)[0] for (l, b) in views]
     ds.set_lookup(0)
     exact = [F.nn_indices(ds, l, b, 56, 52, 60, thomson=True)[0] for (l, b) in views]
     for (l, b), a, e in zip(views, fast, exact):
         assert np.array_equal(a, e), f"view {(l, b)}: {(a != e).sum()} NN choices differ"
     rt, rp = O.thomson_emission(1, d["x"], d["y"], d["z"], d["rho"])
-    ridx = O.render_thomson(1, d["x"], d["y"], d["z"], rt, rp, 0.9, 0.3, 56, 52, 60, want_idx=True)[1]
-    assert np.array_equal(fast[1], ridx)
+    for k in (1, 3):
+        ridx = O.render_thomson(1, d["x"], d["y"], d["z"], rt, rp, *views[k], 56, 52, 60, want_idx=True)[1]
+        assert np.array_equal(fast[k], ridx)
     ds.close()

@@ -63,3 +63,49 @@ def test_matches_scipy_ckdtree_when_box_is_large():
     np.testing.assert_allclose(d_got, dist[:, 0] ** 2, rtol=1e-12)
     unique = dist[:, 1] > dist[:, 0] * (1 + 1e-9)
     assert np.array_equal(got[unique], idx[unique, 0])
+
+
+def test_multi_line_render_equals_single_line_renders_and_flags_ties():
+    """render_nn_multi shares one query per sample between lines: each line's image equals its own
+    render_nn, and the tie flags mark exactly the samples where a second point in the box lies at
+    the nearest distance (a structured grid seen along an axis has many)."""
+    C = 2.99792458e8
+    m = 9
+    a = np.linspace(-4, 4, m).astype(np.float32)
+    zz, yy, xx = np.meshgrid(a, a, a, indexing="ij")
+    x, y, z = xx.ravel(), yy.ravel(), zz.ravel()
+    n = x.size
+    rng = np.random.default_rng(3)
+    peaks = [rng.uniform(0.5, 2, n) for _ in range(3)]
+    fwhms = [rng.uniform(0.01, 0.03, n) for _ in range(3)]
+    v = [rng.normal(0, 2e4, n).astype(np.float32) for _ in range(3)]
+    width = 0.2 * C / 171.0
+    for l, b, mode in ((0.0, np.pi / 2, 1), (0.7, 1.1, 0)):
+        nx, ny, nz, nl = 13, 11, 17, 12
+        imgs, idx, tie, meta, _ = O.render_nn_multi(mode, x, y, z, peaks, fwhms, *v, l, b, nx, ny, nz, nl, width,
+                                                    171.0, rows=(2, 9))
+        for g in range(3):
+            ref, ridx, rmeta, _ = O.render_nn(mode, x, y, z, peaks[g], fwhms[g], *v, l, b, nx, ny, nz, nl, width, 171.0,
+                                              want_idx=True, rows=(2, 9))
+            assert np.array_equal(imgs[g], ref)
+            band = ridx.reshape(ny, nx * nz)[2:9]
+            assert np.array_equal(idx.reshape(ny, nx * nz)[2:9], band)
+            assert np.array_equal(meta, rmeta)
+        # tie flags by brute force over the band's samples
+        p = O.view_samples(meta[:6], l, b, nx, ny, nz)
+        sel = np.zeros((ny, nx, nz), bool)
+        sel[2:9] = True
+        pts = np.stack([x, y, z], 1).astype(np.float64)
+        md = meta[6]
+        want = []
+        for q in zip(p[0][sel], p[1][sel], p[2][sel]):
+            t = np.array(q, np.float32).astype(np.float64)
+            lo = np.array([np.float32(c - md) for c in q], np.float64)
+            hi = np.array([np.float32(c + md) for c in q], np.float64)
+            inb = np.all((pts > lo) & (pts < hi), 1)
+            d = (((pts[:, 0] - t[0]) ** 2 + (pts[:, 1] - t[1]) ** 2) + (pts[:, 2] - t[2]) ** 2)[inb]
+            want.append(inb.any() and (d == d.min()).sum() > 1)
+        got = tie.reshape(ny, nx, nz)[sel].astype(bool)
+        assert np.array_equal(got, np.array(want))
+        if l == 0.0:
+            assert got.sum() > 0
