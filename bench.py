@@ -8,10 +8,18 @@ x 4 spectral lines (synthetic G(T) tables peaked at log T = 5.8, 6.2, 6.4, 6.9, 
 W per step = 64*4*256*256*128*60 = 1.29e11 LOS-sample x lambda evaluations. Inputs (datacube,
 cell list, emission cubes) are resident in HBM before the timed region; results stay on the device.
 
-Also measured: the plain path (one render per view and line, each re-computing the emissivity,
-result copied to the host, as FoMoObject::render does per call; the spatial index stays resident),
-the index build time, per-kernel durations from HIP events around each kernel of an eager pass of
-the same work (roofline), and the CPU oracle (reference float pipeline) on a bounded sample.
+Also measured, beside the headline:
+- a self-check: one oblique view of the batch (all 4 lines) on a band of rows against the FP64
+  oracle (NN choices bit for bit with exact-distance ties counted apart, images within 1e-5);
+- the batched step plus the copy of all its images to the host;
+- the plain path (one render per view and line, each re-computing the emissivity, result copied
+  to the host, as FoMoObject::render does per call), with the spatial index resident and with it
+  rebuilt before every render (the reference rebuilds its R-tree per view);
+- the index build time and per-kernel durations from HIP events around each kernel of an eager
+  pass of the same work (roofline);
+- serving: the mixed request stream of the serving configuration over resident cubes (this one
+  among them), saturated and at 0.5, 0.8 and 0.95 of that rate, p50/p99 latency;
+- the CPU oracle (reference float pipeline) on a bounded sample, median of 3 runs.
 
 Prints ONE JSON line (see the README of the harness contract).
 """
@@ -62,8 +70,9 @@ def main():
                     help="let the autotuner pick the index configuration first (for C2 the default one is as fast "
                          "inside the batch, where the NN pass shares the GPU with the integration)")
     ap.add_argument("--workload", choices=["batched", "thomson", "serve", "tet", "projection"], default="batched")
-    ap.add_argument("--requests", type=int, default=1000, help="requests per arrival rate (serve workload)")
-    ap.add_argument("--serve-requests", type=int, default=240, help="requests in the serving phase of the default run")
+    ap.add_argument("--requests", type=int, default=10000, help="requests per arrival rate (serve workload)")
+    ap.add_argument("--serve-requests", type=int, default=10000,
+                    help="requests per arrival rate in the serving phase of the default run (0: no serving phase)")
     ap.add_argument("--corona-m", type=int, default=645, help="corona grid points per axis (thomson workload)")
     args = ap.parse_args()
 
@@ -189,6 +198,31 @@ def main():
                     "frac": ach / PEAK_HBM, "traffic": traffic, "avg_launch_ms": avg_s * 1e3,
                     "algorithmic_bytes_per_launch": bytes_per[dom], "share_of_step": share}
 
+    # ---- self-check: one oblique view of the batch, all lines, on a band of rows, against the FP64
+    #      oracle (NN choices of the same view bit-exact, ties counted; images within 1e-5)
+    check = None
+    if not args.no_cpu:
+        check = self_check(F, d, paths, ds, bt, nv // 2 + 3, ls, bs, nx, ny, nz, nl, width, lambda0, args.cpu_rows)
+
+    # ---- batched with the images copied to the host: the same step plus one read of every image
+    # (the first read also faults in the fresh host buffer; the second reuses it, as a consumer
+    # that keeps its output buffer does)
+    cube = np.empty(bt.shape, np.float32)
+    t0 = time.perf_counter()
+    bt.read(cube)
+    t_first = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    bt.read(cube)
+    t_copy = time.perf_counter() - t0
+    del cube
+    nbytes = float(nv * len(ems)) * R * nl * 4
+    with_copy = {"ms_per_step": ms_step + t_copy * 1e3, "value": W_step / (dt / args.steps + t_copy),
+                 "copy_ms": t_copy * 1e3, "copy_GBps": nbytes / t_copy / 1e9,
+                 "first_copy_ms": t_first * 1e3, "bytes": nbytes,
+                 "what": "batched step + device-to-host copy of all its images into a reused host buffer"}
+    log(f"batched + copy of all {with_copy['bytes'] / 2**30:.1f} GiB of images to the host: "
+        f"{with_copy['ms_per_step']:.1f} ms/step, {with_copy['value']:.3e} W/s")
+
     # ---- plain: one render() per (view, line), as FoMoObject::render does per call: emissivity
     #      recomputed, one view rendered, result copied to the host; the index stays resident
     nplain = min(args.plain_renders, nv * len(ems))
@@ -202,41 +236,68 @@ def main():
     W_render = float(nx) * ny * nz * nl
     plain_value = W_render * nplain / dtp
     log(f"plain: {dtp / nplain * 1e3:.2f} ms/render, {plain_value:.3e} W/s")
-    # index (cell list + candidate lookup grid) build time, once per dataset (the batch graph holds
-    # pointers into the index, so it goes first)
+    # the batch graph holds pointers into the index, so it goes before any re-index
     bt.close()
+    # ---- plain with the spatial index rebuilt for every render, as the reference rebuilds its
+    #      R-tree per view (fomo-nearestneighbour.cpp:93-100)
+    nrb = 2
+    t0 = time.perf_counter()
+    for r in range(nrb):
+        v, g = r // len(ems), r % len(ems)
+        ds.reindex(tune["cell_scale"] if tune else args.cell_scale)
+        e = F.Emission.chianti(ds, tabs[g], lambda0, atw)
+        img, _ = F.render_nn(e, ls[v], bs[v], nx, ny, nz, nl, width)
+        e.close()
+    dtr = time.perf_counter() - t0
+    plain_rebuild = {"value": W_render * nrb / dtr, "ms_per_render": dtr / nrb * 1e3, "renders_timed": nrb,
+                     "what": "one render per view and line with the spatial index rebuilt first"}
+    log(f"plain with index rebuild: {dtr / nrb * 1e3:.1f} ms/render")
+    # index (cell list + candidate lookup grid) build time, once per dataset
     t0 = time.perf_counter()
     ds.reindex(tune["cell_scale"] if tune else args.cell_scale)
     t_index = time.perf_counter() - t0
     linfo = ds.lookup_info()
     log(f"index rebuild: {t_index * 1e3:.1f} ms, lookup {linfo}")
 
-    # ---- serving: requests of mixed sizes on the resident cube, Poisson arrivals at 0.8 x the
-    #      saturated request rate; latency = submit -> image in the caller's buffer -------------
-    emi = F.Emission.chianti(ds, tabs[0], lambda0, atw, imaging=True)
-    rng = np.random.default_rng(7)
-    targets = [{"spec": ems[0], "imag": emi, "weight": 1.0, "name": f"{m}^3 loop"}]
-    serving = serve_measure(F, targets, width, rng, args.serve_requests, [0.8])
-    emi.close()
+    # ---- serving: the request mix of the serving configuration (NN, CGAL/Linear tetra and Thomson
+    #      requests over resident cubes, this 256^3 cube among them), Poisson arrivals at 0.5, 0.8
+    #      and 0.95 x the saturated request rate; latency = submit -> image in the caller's buffer
+    serving = None
+    if args.serve_requests > 0:
+        emi = F.Emission.chianti(ds, tabs[1 % len(tabs)], lambda0, atw, imaging=True)
+        targets, objs = c4_targets(F, paths[1 % len(paths)], lambda0, atw,
+                                   resident_256=(ems[1 % len(ems)], emi, f"{m}^3 loop (NN)"))
+        rng = np.random.default_rng(11)
+        serving = serve_measure(F, targets, width, rng, args.serve_requests, [0.5, 0.8, 0.95])
+        for o in objs:
+            o.close()
+        emi.close()
 
-    # ---- CPU baseline: reference float pipeline (oracle), 1 view x 1 line, row sample ---------
+    # ---- CPU baseline: reference float pipeline (oracle), 1 view x 1 line, row sample, 3 runs
     cpu = None
     if not args.no_cpu:
         from oracle import oracle as O
         tab = O.read_table(paths[0])
-        t0 = time.perf_counter()
-        pk, fw = O.emission(0, d["rho"], d["T"], tab)
-        t_em = time.perf_counter() - t0
         rows = (ny // 2 - args.cpu_rows // 2, ny // 2 + args.cpu_rows // 2)
-        _, _, _, tim = O.render_nn(0, d["x"], d["y"], d["z"], pk, fw, d["vx"], d["vy"], d["vz"], ls[1], bs[1], nx, ny,
-                                   nz, nl, width, lambda0, rows=rows)
-        t_view = tim[0] + tim[1] + tim[2] * ny / (rows[1] - rows[0])
-        cpu_value = W_render / (t_view + t_em)
+        runs = []
+        for _ in range(3):
+            t0 = time.perf_counter()
+            pk, fw = O.emission(0, d["rho"], d["T"], tab)
+            t_em = time.perf_counter() - t0
+            _, _, _, tim = O.render_nn(0, d["x"], d["y"], d["z"], pk, fw, d["vx"], d["vy"], d["vz"], ls[1], bs[1], nx,
+                                       ny, nz, nl, width, lambda0, rows=rows)
+            runs.append((tim[0] + tim[1] + tim[2] * ny / (rows[1] - rows[0]) + t_em, tim, t_em))
+        runs.sort(key=lambda r: r[0])
+        t_view, tim, t_em = runs[1]
+        cpu_value = W_render / t_view
         cpu = {"value": cpu_value, "unit": "LOS samples*lambda/s", "cores": O.num_threads(), "kind": "port",
-               "sample": f"1 view (l=pi/32, b=pi/2) x 1 line of the same workload; rotate {tim[0]:.2f}s + kd-tree "
-                         f"{tim[1]:.2f}s + ray loop over {rows[1] - rows[0]} of {ny} image rows {tim[2]:.2f}s "
-                         f"(scaled to all rows) + emissivity {t_em:.2f}s; reference float pipeline (oracle mode 0)"}
-        log(f"cpu: {cpu_value:.3e} W/s on {O.num_threads()} threads")
+               "cpu": cpu_model(),
+               "sample": f"1 view (l=pi/32, b=pi/2) x 1 line of the same workload, median of 3 runs "
+                         f"({', '.join(f'{r[0]:.2f}s' for r in runs)} per view); the median run: rotate {tim[0]:.2f}s "
+                         f"+ kd-tree {tim[1]:.2f}s + ray loop over {rows[1] - rows[0]} of {ny} image rows "
+                         f"{tim[2]:.2f}s (scaled to all rows) + emissivity {t_em:.2f}s; reference float pipeline "
+                         f"(oracle mode 0)"}
+        log(f"cpu: {cpu_value:.3e} W/s on {O.num_threads()} threads ({cpu['cpu']})")
 
     out = {
         "metric": "LOS samples x lambda per second, batched render (64 views x 4 lines in one HIP graph)",
@@ -255,13 +316,17 @@ def main():
                                f"{nx}x{ny} image, nz {nz}, nl {nl}",
                    "points": n, "views": nv, "lines": len(ems), "image": [nx, ny], "z_pixels": nz,
                    "lambda_pixels": nl, "W_per_step": W_step},
+        "batched_with_copy": with_copy,
         "plain": {"value": plain_value, "ms_per_render": dtp / nplain * 1e3, "renders_timed": nplain,
                   "what": "one render per view and line: emissivity + render + copy to host, index resident"},
+        "plain_rebuild_index": plain_rebuild,
+        "batched_vs_plain": value / plain_value,
+        "batched_with_copy_vs_plain": with_copy["value"] / plain_value,
         "index_build_ms": t_index * 1e3,
         "copy_probe_GBps": copy_gbps,
         "lookup": linfo,
         "autotune": tune,
-        "batched_vs_plain": value / plain_value,
+        "self_check": check,
         "roofline": roof,
         "cpu_baseline": cpu,
         "serving": serving,
@@ -271,6 +336,51 @@ def main():
         e.close()
     ds.close()
     F.shutdown()
+
+
+def cpu_model():
+    """CPU model name and the cores this process may use."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return f"{model}; {len(os.sched_getaffinity(0))} cores available"
+
+
+def self_check(F, d, paths, ds, bt, v, ls, bs, nx, ny, nz, nl, width, lambda0, nrows):
+    """One view of the batch (all its lines) on nrows image rows against the FP64 oracle: the
+    relative L2 error of every line's band, and the NN choices of the same view (F.nn_indices)
+    bit for bit, with the samples whose nearest distance is an exact tie counted apart."""
+    from oracle import oracle as O
+    t0 = time.perf_counter()
+    rows = (ny // 2 - nrows // 2, ny // 2 + nrows // 2)
+    nlines = bt.shape[1]
+    em_o = [O.emission(1, d["rho"], d["T"], O.read_table(p)) for p in paths[:nlines]]
+    imgs, ridx, tie, rmeta, _ = O.render_nn_multi(1, d["x"], d["y"], d["z"], [e[0] for e in em_o],
+                                                  [e[1] for e in em_o], d["vx"], d["vy"], d["vz"], ls[v], bs[v], nx,
+                                                  ny, nz, nl, width, lambda0, rows=rows)
+    idx, _ = F.nn_indices(ds, ls[v], bs[v], nx, ny, nz)
+    gi, oi = idx.reshape(ny, -1)[rows[0]:rows[1]], ridx.reshape(ny, -1)[rows[0]:rows[1]]
+    ti = tie.reshape(ny, -1)[rows[0]:rows[1]].astype(bool)
+    errs = []
+    meta_ok = True
+    for g in range(nlines):
+        img, meta = bt.read_view(v, g)
+        meta_ok = meta_ok and bool(np.array_equal(meta[:7], rmeta[:7]))
+        errs.append(O.rel_l2(img.reshape(ny, -1)[rows[0]:rows[1]], imgs[g].reshape(ny, -1)[rows[0]:rows[1]]))
+    res = {"view": v, "l": ls[v], "b": bs[v], "rows": list(rows), "lines": nlines, "samples": int(gi.size),
+           "nn_mismatch": int((gi != oi).sum()), "nn_mismatch_non_tie": int(((gi != oi) & ~ti).sum()),
+           "tie_samples": int(ti.sum()), "empty_box_samples": int((oi < 0).sum()),
+           "rel_l2_per_line": errs, "rel_l2_max": max(errs), "tolerance": 1e-5, "meta_equal": meta_ok,
+           "ok": bool((gi == oi).all() and max(errs) <= 1e-5 and meta_ok), "seconds": time.perf_counter() - t0}
+    log(f"self-check: view {v}, rows {rows}: NN mismatches {res['nn_mismatch']} ({res['tie_samples']} tie samples), "
+        f"max rel L2 {res['rel_l2_max']:.2e}, ok={res['ok']}")
+    return res
 
 
 # Request mix of the serving workload: image sizes x LOS samples x lambda pixels drawn uniformly.
@@ -308,24 +418,57 @@ def req_work(r):
     return float(r[4]) * r[5] * r[6] * max(r[7], 1)
 
 
+class _Buffers:
+    """Reusable float32 output buffers by size, so that a long request stream holds only the images
+    of the requests that are still outstanding."""
+
+    def __init__(self):
+        self.free = {}
+
+    def get(self, n):
+        lst = self.free.get(n)
+        return lst.pop() if lst else np.empty(n, np.float32)
+
+    def put(self, a):
+        self.free.setdefault(a.size, []).append(a)
+
+
+def _out_size(r):
+    return r[4] * r[5] * max(r[7], 1)
+
+
 def serve_measure(F, targets, width, rng, count, loads, device_bytes=24 << 30, host_bytes=2 << 30,
-                  max_inflight=8, nstreams=4):
-    """Saturated throughput (all requests queued at once), the same requests one synchronous render
-    at a time, then open-loop Poisson arrivals at each load x the saturated rate: p50/p99 latency."""
+                  max_inflight=8, nstreams=4, window=256):
+    """Saturated throughput (requests kept `window` deep in the queue), the same requests one
+    synchronous render at a time, then open-loop Poisson arrivals at each load x the saturated rate:
+    p50/p99 latency from submit to the image in the caller's buffer."""
+    import collections
     sv = F.Server(device_bytes=device_bytes, host_bytes=host_bytes, max_inflight=max_inflight, nstreams=nstreams)
+    bufs = _Buffers()
+
+    def submit(r):
+        out = bufs.get(_out_size(r))
+        return sv.submit(*r[:8], lambda_width=r[8], out=out)
+
+    def finish(t, timeout=-1.0):
+        img, _ = sv.result(t, timeout)
+        bufs.put(img)
+
     try:
         warm = draw_requests(F, targets, width, rng, 8)
-        for r in warm:
-            sv.submit(*r[:8], lambda_width=r[8])
-        sv.wait_all(300)
+        for t in [submit(r) for r in warm]:
+            finish(t, 300)
         reqs = draw_requests(F, targets, width, rng, count)
         W = sum(req_work(r) for r in reqs)
+        q = collections.deque()
         t0 = time.perf_counter()
-        tks = [sv.submit(*r[:8], lambda_width=r[8]) for r in reqs]
-        sv.wait_all(600)
+        for r in reqs:
+            if len(q) >= window:
+                finish(q.popleft(), 600)
+            q.append(submit(r))
+        while q:
+            finish(q.popleft(), 600)
         dt_sat = time.perf_counter() - t0
-        for t in tks:
-            sv.result(t, 1)
         sat_rps = count / dt_sat
         log(f"serving saturated: {count} requests in {dt_sat:.2f}s, {sat_rps:.1f} req/s, {W / dt_sat:.3e} W/s")
         t0 = time.perf_counter()
@@ -343,23 +486,27 @@ def serve_measure(F, targets, width, rng, count, loads, device_bytes=24 << 30, h
             rate = load * sat_rps
             reqs = draw_requests(F, targets, width, rng, count)
             arr = np.cumsum(rng.exponential(1.0 / rate, count))
-            tks = []
+            tks, q = [], collections.deque()
             t0 = time.perf_counter()
             for a, r in zip(arr, reqs):
+                # hand back the buffers of finished requests, oldest first
+                while q and sv.poll(q[0]) == 2:
+                    finish(q.popleft(), 0)
                 d = a - (time.perf_counter() - t0)
                 if d > 0:
                     time.sleep(d)
-                tks.append(sv.submit(*r[:8], lambda_width=r[8]))
-            sv.wait_all(900)
+                t = submit(r)
+                tks.append(t)
+                q.append(t)
+            while q:
+                finish(q.popleft(), 900)
             span = time.perf_counter() - t0
             lat = np.array([sv.times(t)[2] - sv.times(t)[0] for t in tks]) * 1e3
-            for t in tks:
-                sv.result(t, 1)
             Wl = sum(req_work(r) for r in reqs)
             run = {"load": load, "arrival_rate_req_s": rate, "requests": count, "p50_ms": float(np.percentile(lat, 50)),
                    "p99_ms": float(np.percentile(lat, 99)), "mean_ms": float(lat.mean()),
                    "max_ms": float(lat.max()), "throughput_W_s": Wl / span}
-            log(f"serving at {load:.2f} x saturation ({rate:.1f} req/s): p50 {run['p50_ms']:.2f} ms, "
+            log(f"serving at {load:.2f} x saturation ({rate:.1f} req/s, {count} requests): p50 {run['p50_ms']:.2f} ms, "
                 f"p99 {run['p99_ms']:.2f} ms, {Wl / span:.3e} W/s")
             runs.append(run)
         st = sv.stats()
@@ -367,6 +514,7 @@ def serve_measure(F, targets, width, rng, count, loads, device_bytes=24 << 30, h
         sv.close()
     return {"mix": " + ".join(t["name"] for t in targets) + "; image {64,128,256,512}^2 x nz {64,128,256} x "
                    "nl {1,30,100} uniform, l ~ U(0,2pi), b ~ U(0.1,pi/2)",
+            "requests_per_rate": count, "saturation_window": window,
             "saturated_req_s": sat_rps, "saturated_W_s": W / dt_sat, "sequential_W_s": W / dt_seq,
             "served_vs_sequential": dt_seq / dt_sat, "max_inflight": max_inflight, "streams": nstreams,
             "device_budget_GiB": device_bytes / 2**30, "device_peak_GiB": st["device_peak"] / 2**30,
@@ -375,18 +523,19 @@ def serve_measure(F, targets, width, rng, count, loads, device_bytes=24 << 30, h
             "p99_ms": runs[0]["p99_ms"] if len(runs) == 1 else None}
 
 
-def serve(F, args):
-    """Serving workload: resident 128^3 and 256^3 loop cubes (a spectroscopic and an imaging
-    emission cube each) for nearest-neighbour requests, a triangulated jittered 128^3 cube for
-    tetra (CGAL and linear) requests and a corona for Thomson requests; requests drawn uniformly
-    from the size mix, Poisson arrivals at 0.5, 0.8 and 0.95 x the saturated request rate."""
-    tmp = tempfile.mkdtemp()
-    path = F.gen_goft_table(os.path.join(tmp, "fe9.dat"), ion="fe_9", lambda0=171.0, atweight=56.0, peak_logt=6.2)
-    tab = F.Table.from_file(path)
-    lambda0, atw = 171.0, 56.0
-    width = 0.2 * C / lambda0
-    objs, targets = [], []
+def c4_targets(F, table_path, lambda0, atw, resident_256=None):
+    """Resident objects of the serving configuration: 128^3 and 256^3 loop cubes (a spectroscopic
+    and an imaging emission cube each) for nearest-neighbour requests, a triangulated jittered 128^3
+    cube for tetra (CGAL and Linear) requests and a corona for Thomson requests. resident_256:
+    (spectroscopic, imaging, name) of a 256^3 cube already resident, used instead of a new one.
+    Returns (targets, objects to close)."""
+    tab = F.Table.from_file(table_path)
+    objs, targets = [tab], []
     for m, seed in ((128, 4), (256, 2)):
+        if m == 256 and resident_256 is not None:
+            es, ei, name = resident_256
+            targets.append({"spec": es, "imag": ei, "weight": 0.35, "name": name})
+            continue
         d = F.gen_loop_cube(m, seed=seed)
         ds = F.Dataset(d["x"], d["y"], d["z"], [d["rho"], d["T"], d["vx"], d["vy"], d["vz"]])
         es = F.Emission.chianti(ds, tab, lambda0, atw)
@@ -403,12 +552,24 @@ def serve(F, args):
     objs += [es, ei, dst]
     targets.append({"spec": es, "imag": ei, "tet": True, "weight": 0.2,
                     "name": "jittered 128^3 loop (CGAL / Linear tetra path)"})
-    c = F.gen_corona(args.corona_m if args.corona_m != 645 else 128, seed=3, sigma=0.1)
+    c = F.gen_corona(128, seed=3, sigma=0.1)
     dsc = F.Dataset(c["x"], c["y"], c["z"], [c["rho"]])
     et = F.Emission.thomson(dsc)
     objs += [et, dsc]
     targets.append({"thomson": et, "weight": 0.10, "name": f"corona {c['x'].size} pts (Thomson)"})
     log("resident: " + ", ".join(t["name"] for t in targets))
+    # close in reverse order of creation (emission cubes before their datasets)
+    return targets, objs[::-1]
+
+
+def serve(F, args):
+    """Serving workload: the resident objects of c4_targets; requests drawn uniformly from the size
+    mix, Poisson arrivals at 0.5, 0.8 and 0.95 x the saturated request rate."""
+    tmp = tempfile.mkdtemp()
+    path = F.gen_goft_table(os.path.join(tmp, "fe9.dat"), ion="fe_9", lambda0=171.0, atweight=56.0, peak_logt=6.2)
+    lambda0, atw = 171.0, 56.0
+    width = 0.2 * C / lambda0
+    targets, objs = c4_targets(F, path, lambda0, atw)
     rng = np.random.default_rng(11)
     res = serve_measure(F, targets, width, rng, args.requests, [0.5, 0.8, 0.95])
     mid = [r for r in res["runs"] if r["load"] == 0.8][0]
@@ -421,7 +582,6 @@ def serve(F, args):
     print(json.dumps(out), flush=True)
     for o in objs:
         o.close()
-    tab.close()
     F.shutdown()
 
 

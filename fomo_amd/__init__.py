@@ -437,8 +437,12 @@ class Batch:
         """One eager pass of the batch's kernels with HIP events around each launch."""
         _check(lib().fomo_batch_profile(self.id))
 
-    def read(self):
-        out = np.zeros(self.shape, np.float32)
+    def read(self, out=None):
+        """All images [view][line][ny][nx][nl] (into `out` when given) and the views' meta."""
+        if out is None:
+            out = np.zeros(self.shape, np.float32)
+        elif out.shape != self.shape or out.dtype != np.float32 or not out.flags.c_contiguous:
+            raise ValueError(f"out must be a contiguous float32 array of shape {self.shape}")
         meta = np.zeros((self.shape[0], 10), np.float64)
         _check(lib().fomo_batch_read(self.id, _ptr(out, ctypes.c_float), _ptr(meta, ctypes.c_double)))
         return out, meta
