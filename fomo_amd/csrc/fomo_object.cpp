@@ -1,11 +1,14 @@
 // fomo_object.cpp — the FoMo C++ API (include/FoMo.h) on top of the MI355X engine (fomo_hip.h).
 //
-// Host containers keep the reference's value semantics; FoMoObject::render() uploads the datacube
-// once (re-uploading only after push_back_datapoint/setdata), computes the emission on the GPU,
-// renders every (l,b) view on the GPU and builds the RenderCube grids on the host exactly as
-// fomo-nearestneighbour.cpp:233-279 / fomo-thomson.cpp:348-364 do. Error behaviour follows the
-// reference facade: missing tables, a wrong instrument table and unknown methods print to stderr
-// and exit(EXIT_FAILURE); the C ABI below returns codes instead.
+// Host containers keep the reference's value semantics. FoMoObject::render() uploads the datacube
+// once (again only after push_back_datapoint/setdata), computes the emission on the GPU, renders
+// the (l,b) views on the GPU (the NearestNeighbour views of one call together, in one captured
+// batch) and builds the RenderCube grids on the host exactly as fomo-nearestneighbour.cpp:233-279
+// / fomo-thomson.cpp:348-364 do, while a background thread writes the previous view's files. The
+// emission cube stays on the device: the GoftCube is read back only when readgoftcube() asks for it
+// (or before the datacube changes). Error behaviour follows the reference facade: missing tables,
+// a wrong instrument table and unknown methods print to stderr and exit(EXIT_FAILURE); the C ABI
+// below returns codes instead.
 
 #include <sys/stat.h>
 
@@ -23,6 +26,7 @@
 
 #include "../../include/FoMo.h"
 #include "../../include/fomo_hip.h"
+#include "fomo_writer.h"
 
 namespace {
 
@@ -43,17 +47,60 @@ void engine() {
   ok = true;
 }
 
-// Device-side state of one FoMoObject (kept out of the public header so its layout matches).
-struct ObjState {
-  int dataset = -1;
-  bool dirty = true;
+// A dataset or emission cube on the device, destroyed with its last owner.
+struct DeviceDataset {
+  int id = -1;
+  ~DeviceDataset() {
+    if (id >= 0) fomo_dataset_destroy(id);
+  }
 };
-std::mutex g_state_mu;
-std::map<const FoMo::FoMoObject*, ObjState> g_state;
+struct DeviceEmission {
+  int id = -1;
+  std::shared_ptr<DeviceDataset> ds;  // released after the emission cube
+  ~DeviceEmission() {
+    if (id >= 0) fomo_goft_destroy(id);
+  }
+};
 
-ObjState& state_of(const FoMo::FoMoObject* o) {
-  std::lock_guard<std::mutex> lk(g_state_mu);
-  return g_state[o];
+}  // namespace
+
+struct FoMo::FoMoDeviceState {
+  std::shared_ptr<DeviceDataset> dataset;    // upload of the datacube; null when none or stale
+  std::shared_ptr<DeviceEmission> emission;  // emission cube of the last render()
+  bool goft_pending = false;                 // goftcube holds only the metadata of `emission`
+  bool thomson = false;
+  std::vector<std::string> goft_units;
+};
+
+namespace {
+
+// The state of one object alone: copies of a FoMoObject share it until one of them changes it.
+FoMo::FoMoDeviceState& own_state(std::shared_ptr<FoMo::FoMoDeviceState>& d) {
+  if (!d) d = std::make_shared<FoMo::FoMoDeviceState>();
+  else if (d.use_count() > 1) d = std::make_shared<FoMo::FoMoDeviceState>(*d);
+  return *d;
+}
+
+// Fill the GoftCube of the last render() from the device (emissionfromdatacube's output,
+// fomo-CHIANTI.cpp:296-321, or thomsonfromdatacube's, fomo-thomson.cpp:60-92): the datacube's
+// grid, the two emission variables in the original point order, and for a spectral emission the
+// velocity variables.
+void materialize(FoMo::FoMoDeviceState& st, const FoMo::DataCube& dc, FoMo::GoftCube& gc) {
+  if (!st.goft_pending) return;
+  st.goft_pending = false;
+  const int64_t n = dc.readngrid();
+  std::vector<float> a(n), c(n);
+  if (fomo_goft_read(st.emission->id, a.data(), c.data()) != FOMO_OK)
+    die(std::string("Error: emission read-back failed: ") + fomo_last_error());
+  FoMo::tgrid g = dc.readgrid();
+  FoMo::tvars v{std::move(a), std::move(c)};
+  if (st.thomson) {
+    gc.setdata(g, v, nullptr);
+    return;
+  }
+  for (int i = 2; i < dc.readnvars(); ++i) v.push_back(dc.readvar(i));
+  std::vector<std::string> units = st.goft_units;
+  gc.setdata(g, v, &units);
 }
 
 struct TableEntry {
@@ -365,19 +412,15 @@ void FoMo::RenderCube::readangles(double& lo, double& bo) {
 // =====================================================================================
 FoMo::FoMoObject::FoMoObject(const int indim) : datacube(indim), goftcube(datacube), rendering(goftcube) {}
 
-FoMo::FoMoObject::~FoMoObject() {
-  std::lock_guard<std::mutex> lk(g_state_mu);
-  auto it = g_state.find(this);
-  if (it != g_state.end()) {
-    if (it->second.dataset >= 0) fomo_dataset_destroy(it->second.dataset);
-    g_state.erase(it);
-  }
-}
+FoMo::FoMoObject::~FoMoObject() {}
 
 void FoMo::FoMoObject::setrendermethod(const std::string m) { rendering.setrendermethod(m); }
 std::string FoMo::FoMoObject::readrendermethod() { return rendering.readrendermethod(); }
 FoMo::DataCube FoMo::FoMoObject::readdatacube() { return datacube; }
-FoMo::GoftCube FoMo::FoMoObject::readgoftcube() { return goftcube; }
+FoMo::GoftCube FoMo::FoMoObject::readgoftcube() {
+  if (device && device->goft_pending) materialize(own_state(device), datacube, goftcube);
+  return goftcube;
+}
 FoMo::RenderCube FoMo::FoMoObject::readrendering() { return rendering; }
 void FoMo::FoMoObject::setabundfile(const std::string s) {
   goftcube.setabundfile(s);
@@ -403,13 +446,17 @@ void FoMo::FoMoObject::push_back_datapoint(std::vector<double> coordinate, std::
                                            std::vector<std::string>* unitvec) {
   // the reference ignores unitvec here (fomo-object.cpp:139-141)
   (void)unitvec;
+  FoMoDeviceState& st = own_state(device);
+  materialize(st, datacube, goftcube);  // the last render's GoftCube keeps the data it was made from
   datacube.push_back(coordinate, variables);
-  state_of(this).dirty = true;
+  st.dataset.reset();
 }
 
 void FoMo::FoMoObject::setdata(tgrid& ingrid, tvars& indata, std::vector<std::string>* unitvec) {
+  FoMoDeviceState& st = own_state(device);
+  materialize(st, datacube, goftcube);
   datacube.setdata(ingrid, indata, unitvec);
-  state_of(this).dirty = true;
+  st.dataset.reset();
 }
 
 void FoMo::FoMoObject::setobservationtype(FoMoObservationType t) { rendering.setobservationtype(t); }
@@ -445,11 +492,8 @@ void FoMo::FoMoObject::setwriteoutdeletefiles(const bool v) {
 
 namespace {
 
-int upload_dataset(const FoMo::DataCube& dc, ObjState& st) {
+std::shared_ptr<DeviceDataset> upload_dataset(const FoMo::DataCube& dc) {
   engine();
-  if (!st.dirty && st.dataset >= 0) return st.dataset;
-  if (st.dataset >= 0) fomo_dataset_destroy(st.dataset);
-  st.dataset = -1;
   const int dim = dc.readdim();
   const int nv = dc.readnvars();
   const int64_t n = dc.readngrid();
@@ -461,14 +505,15 @@ int upload_dataset(const FoMo::DataCube& dc, ObjState& st) {
     vars[i] = dc.readvar(i);
     vp[i] = vars[i].data();
   }
-  int id = -1;
+  auto d = std::make_shared<DeviceDataset>();
   int rc = fomo_dataset_create(n, dim == 2 ? 2 : 3, g[0].data(), g[1].data(), dim >= 3 ? g[2].data() : nullptr, nv,
-                               vp.data(), 0.0, &id);
+                               vp.data(), 0.0, &d->id);
   if (rc != FOMO_OK) die(std::string("Error: dataset upload failed: ") + fomo_last_error());
-  st.dataset = id;
-  st.dirty = false;
-  return id;
+  return d;
 }
+
+// Views of one NearestNeighbour render() batched together: at most this much image per batch.
+constexpr double kBatchImageBytes = 2.0 * (1 << 30);
 
 }  // namespace
 
@@ -488,25 +533,22 @@ void FoMo::FoMoObject::render(const std::vector<double> lvec, const std::vector<
     die("Error: unknown rendering method.");
   }
   const std::bitset<noptions> woptions = goftcube.getwriteoptions();
-  ObjState& st = state_of(this);
-  const int ds = upload_dataset(datacube, st);
+  FoMoDeviceState& st = own_state(device);
+  if (!st.dataset) st.dataset = upload_dataset(datacube);
+  const int ds = st.dataset->id;
   const int dim = datacube.readdim();
-  const int64_t n = datacube.readngrid();
 
-  // ---- emission (the GoftCube) -------------------------------------------------------------
-  int goft = -1;
-  FoMo::GoftCube tmpgoft(datacube);
+  // ---- emission (the GoftCube): stays on the device, read back by readgoftcube() ------------
+  auto em = std::make_shared<DeviceEmission>();
+  em->ds = st.dataset;
+  FoMo::GoftCube meta(dim);  // the GoftCube's metadata; its data arrive on demand
   std::string emunit;
+  std::vector<std::string> units;
   if (thomson) {
-    if (fomo_emission_thomson(ds, &goft) != FOMO_OK) die(std::string("Error: ") + fomo_last_error());
-    std::vector<float> a(n), c(n);
-    fomo_goft_read(goft, a.data(), c.data());
-    FoMo::tgrid g = datacube.readgrid();
-    FoMo::tvars v{a, c};
-    tmpgoft.setdata(g, v, nullptr);
-    tmpgoft.setchiantifile("thomson scattering");
-    tmpgoft.setabundfile("thomson scattering");
-    tmpgoft.setlambda0(0.);
+    if (fomo_emission_thomson(ds, &em->id) != FOMO_OK) die(std::string("Error: ") + fomo_last_error());
+    meta.setchiantifile("thomson scattering");
+    meta.setabundfile("thomson scattering");
+    meta.setlambda0(0.);
   } else {
     const std::string cfile = rendering.readchiantifile();
     const TableEntry& te = load_table(cfile);
@@ -517,120 +559,159 @@ void FoMo::FoMoObject::render(const std::vector<double> lvec, const std::vector<
     if (!imaging && afile != std::string("/empty"))
       abundratio = abund_from_file(afile, te.ion) / abund_from_file(default_abund_path(cfile), te.ion);
     if (datacube.readnvars() < 2) die("Error: the datacube needs at least n and T.");
-    if (fomo_emission_chianti(ds, te.id, te.lambda0, te.atweight, imaging ? 1 : 0, abundratio, &goft) != FOMO_OK)
+    if (fomo_emission_chianti(ds, te.id, te.lambda0, te.atweight, imaging ? 1 : 0, abundratio, &em->id) != FOMO_OK)
       die(std::string("Error: ") + fomo_last_error());
-    std::vector<float> pk(n), fw(n);
-    fomo_goft_read(goft, pk.data(), fw.data());
-    FoMo::tgrid g = datacube.readgrid();
-    FoMo::tvars v{pk, fw};
     std::vector<std::string> dunits = datacube.readunit();
-    std::vector<std::string> units;
     for (int i = 0; i < dim; ++i) units.push_back(i < (int)dunits.size() ? dunits[i] : "");
     units.push_back(te.unit);
     units.push_back("\\AA{}");
-    for (int i = 2; i < datacube.readnvars(); ++i) {
-      v.push_back(datacube.readvar(i));
-      units.push_back(dim + i < (int)dunits.size() ? dunits[dim + i] : "");
-    }
-    tmpgoft.setdata(g, v, &units);
-    tmpgoft.setchiantifile(cfile);
-    tmpgoft.setabundfile(afile);
-    tmpgoft.setlambda0(te.lambda0);
+    for (int i = 2; i < datacube.readnvars(); ++i) units.push_back(dim + i < (int)dunits.size() ? dunits[dim + i] : "");
+    meta.setchiantifile(cfile);
+    meta.setabundfile(afile);
+    meta.setlambda0(te.lambda0);
     emunit = te.unit;
   }
-  goftcube = tmpgoft;
-  goftcube.setwriteoptions(woptions);
+  meta.setwriteoptions(woptions);
+  st.emission = em;
+  st.goft_pending = true;
+  st.thomson = thomson;
+  st.goft_units = units;
+  goftcube = meta;
+  const int goft = em->id;
   const double lambda0 = goftcube.readlambda0();
 
-  // metadata-only cube for the per-view results (the big point grid is not needed there)
-  FoMo::GoftCube meta(dim);
-  meta.setchiantifile(goftcube.readchiantifile());
-  meta.setabundfile(goftcube.readabundfile());
-  meta.setlambda0(lambda0);
-  meta.setwriteoptions(woptions);
-
-  FoMo::RenderCube tmprender(meta);
   const bool dn = emunit.find("DN") != std::string::npos;
   const int nlo = std::max(nl, 1);
-  std::vector<float> img((size_t)nx * ny * nlo);
+  const size_t npix = (size_t)nx * ny * nlo;
+  std::vector<std::pair<double, double>> views;
   for (double lv : lvec)
-    for (double bv : bvec) {
-      double m[10];
-      int rc = thomson ? fomo_render_thomson(goft, lv, bv, nx, ny, nz, img.data(), m)
-               : tet   ? fomo_render_tet(goft, lv, bv, nx, ny, nz, nl, width, dn ? 1 : 0, method == "Linear" ? 1 : 0,
-                                         img.data(), m)
-               : proj  ? fomo_render_projection(goft, lv, bv, nx, ny, nz, nl, width, dn ? 1 : 0, img.data(), m)
-                       : fomo_render_nn(goft, lv, bv, nx, ny, nz, nl, width, dn ? 1 : 0, img.data(), m);
-      if (rc != FOMO_OK) die(std::string("Error: render failed: ") + fomo_last_error());
-      const double minx = m[0], maxx = m[1], miny = m[2], maxy = m[3];
-      FoMo::RenderCube rc3(meta);
-      FoMo::tgrid newgrid;
-      FoMo::tvars newdata;
-      if (!thomson) {
-        // fomo-nearestneighbour.cpp:124-128, 152-153, 200-211, 233-279 (the CGAL path builds the same
-        // grids, fomo-CGAL.cpp:187-188, 210-218, 262-285, and so does the projection,
-        // fomo-projection.cpp:76-118, 161-212, whose view cubes also say "NearestNeighbour", :203)
-        const size_t ng3 = (size_t)nx * ny * nlo;
-        FoMo::tcoord xv(ng3), yv(ng3), lv3(nl > 1 ? ng3 : 0);
-        const double wla = width * lambda0 / kC;
-        for (int i = 0; i < ny; ++i) {
-          const double y = double(i) / (ny - 1) * (maxy - miny) + miny;
-          for (int j = 0; j < nx; ++j) {
-            const double x = double(j) / (nx - 1) * (maxx - minx) + minx;
-            for (int il = 0; il < nlo; ++il) {
-              const size_t ind = ((size_t)i * nx + j) * nlo + il;
-              xv[ind] = (float)x;
-              yv[ind] = (float)y;
-              if (nl > 1) lv3[ind] = (float)(double(il) / (nl - 1) * wla - wla / 2. + lambda0);
-            }
+    for (double bv : bvec) views.emplace_back(lv, bv);
+  if (views.empty()) {
+    // no view: the reference keeps RenderCube(goftcube) (fomo-nearestneighbour.cpp:287)
+    materialize(st, datacube, goftcube);
+    FoMo::RenderCube r(goftcube);
+    r.setrendermethod(method);
+    r.setobservationtype(rendering.readobservationtype());
+    rendering = r;
+    return;
+  }
+
+  // One view's RenderCube, built on the host from its image and meta (extents, scaling).
+  const auto make_cube = [&](const std::vector<float>& img, const double* m, double lv, double bv) {
+    const double minx = m[0], maxx = m[1], miny = m[2], maxy = m[3];
+    FoMo::RenderCube rc3(meta);
+    FoMo::tgrid newgrid;
+    FoMo::tvars newdata;
+    if (!thomson) {
+      // fomo-nearestneighbour.cpp:124-128, 152-153, 200-211, 233-279 (the CGAL path builds the same
+      // grids, fomo-CGAL.cpp:187-188, 210-218, 262-285, and so does the projection,
+      // fomo-projection.cpp:76-118, 161-212, whose view cubes also say "NearestNeighbour", :203)
+      FoMo::tcoord xv(npix), yv(npix), lv3(nl > 1 ? npix : 0);
+      const double wla = width * lambda0 / kC;
+      for (int i = 0; i < ny; ++i) {
+        const double y = double(i) / (ny - 1) * (maxy - miny) + miny;
+        for (int j = 0; j < nx; ++j) {
+          const double x = double(j) / (nx - 1) * (maxx - minx) + minx;
+          for (int il = 0; il < nlo; ++il) {
+            const size_t ind = ((size_t)i * nx + j) * nlo + il;
+            xv[ind] = (float)x;
+            yv[ind] = (float)y;
+            if (nl > 1) lv3[ind] = (float)(double(il) / (nl - 1) * wla - wla / 2. + lambda0);
           }
         }
-        std::vector<std::string> units{"Mm", "Mm"};
-        if (nl > 1) units.push_back("\\AA{}");
-        units.push_back("erg cm^{-2} s^{-1} \\AA{}^{-1}");
-        if (dn) {
-          units[0] = "arcsec";
-          xv = FoMo::operator*(1. / kMmPerArcsec, xv);
-          units[1] = "arcsec";
-          yv = FoMo::operator*(1. / kMmPerArcsec, yv);
-          units.back() = "DN s^{-1} pixel^{-1}";
-        }
-        newgrid.push_back(std::move(xv));
-        newgrid.push_back(std::move(yv));
-        if (nl > 1) newgrid.push_back(std::move(lv3));
-        newdata.push_back(img);
-        rc3.setdata(newgrid, newdata, &units);
-        rc3.setrendermethod(tet ? method : std::string("NearestNeighbour"));
-        rc3.setresolution(nx, ny, nz, nl, width);
-        rc3.setobservationtype(nl == 1 ? FoMo::Imaging : FoMo::Spectroscopic);
-      } else {
-        // fomo-thomson.cpp:228-232, 333-334, 348-364
-        FoMo::tcoord xv((size_t)nx * ny), yv((size_t)nx * ny);
-        for (int i = 0; i < ny; ++i) {
-          const double y = double(i) / (ny - 1) * (maxy - miny) + miny;
-          for (int j = 0; j < nx; ++j) {
-            xv[(size_t)i * nx + j] = (float)(double(j) / (nx - 1) * (maxx - minx) + minx);
-            yv[(size_t)i * nx + j] = (float)y;
-          }
-        }
-        newgrid.push_back(std::move(xv));
-        newgrid.push_back(std::move(yv));
-        newdata.push_back(std::vector<float>(img.begin(), img.begin() + (size_t)nx * ny));
-        std::vector<std::string> units{"Mm", "Mm", "pB"};
-        rc3.setdata(newgrid, newdata, &units);
-        rc3.setrendermethod("Thomson");
-        rc3.setresolution(nx, ny, nz, 1, 100000.);
-        rc3.setobservationtype(FoMo::Imaging);
       }
-      rc3.setangles(lv, bv);
-      // fomo-nearestneighbour.cpp:294-302: one file per view, even when outfile is ""
-      rc3.writegoftcube(outfile + "l" + angle_tag(lv) + "b" + angle_tag(bv) + ".txt");
-      tmprender = rc3;
+      std::vector<std::string> vunits{"Mm", "Mm"};
+      if (nl > 1) vunits.push_back("\\AA{}");
+      vunits.push_back("erg cm^{-2} s^{-1} \\AA{}^{-1}");
+      if (dn) {
+        vunits[0] = "arcsec";
+        xv = FoMo::operator*(1. / kMmPerArcsec, xv);
+        vunits[1] = "arcsec";
+        yv = FoMo::operator*(1. / kMmPerArcsec, yv);
+        vunits.back() = "DN s^{-1} pixel^{-1}";
+      }
+      newgrid.push_back(std::move(xv));
+      newgrid.push_back(std::move(yv));
+      if (nl > 1) newgrid.push_back(std::move(lv3));
+      newdata.push_back(img);
+      rc3.setdata(newgrid, newdata, &vunits);
+      rc3.setrendermethod(tet ? method : std::string("NearestNeighbour"));
+      rc3.setresolution(nx, ny, nz, nl, width);
+      rc3.setobservationtype(nl == 1 ? FoMo::Imaging : FoMo::Spectroscopic);
+    } else {
+      // fomo-thomson.cpp:228-232, 333-334, 348-364
+      FoMo::tcoord xv((size_t)nx * ny), yv((size_t)nx * ny);
+      for (int i = 0; i < ny; ++i) {
+        const double y = double(i) / (ny - 1) * (maxy - miny) + miny;
+        for (int j = 0; j < nx; ++j) {
+          xv[(size_t)i * nx + j] = (float)(double(j) / (nx - 1) * (maxx - minx) + minx);
+          yv[(size_t)i * nx + j] = (float)y;
+        }
+      }
+      newgrid.push_back(std::move(xv));
+      newgrid.push_back(std::move(yv));
+      newdata.push_back(std::vector<float>(img.begin(), img.begin() + (size_t)nx * ny));
+      std::vector<std::string> tunits{"Mm", "Mm", "pB"};
+      rc3.setdata(newgrid, newdata, &tunits);
+      rc3.setrendermethod("Thomson");
+      rc3.setresolution(nx, ny, nz, 1, 100000.);
+      rc3.setobservationtype(FoMo::Imaging);
     }
-  fomo_goft_destroy(goft);
-  tmprender.setrendermethod(method);
-  tmprender.setobservationtype(rendering.readobservationtype());
-  rendering = tmprender;
+    rc3.setangles(lv, bv);
+    return rc3;
+  };
+
+  // Per-view files (fomo-nearestneighbour.cpp:294-302: one per view, even when outfile is "") are
+  // written by a background thread while the next view is rendered and built.
+  FoMo::RenderCube last(meta);
+  {
+    fomo::CubeWriter writer(2);
+    const auto emit = [&](const std::vector<float>& img, const double* m, size_t v) {
+      const double lv = views[v].first, bv = views[v].second;
+      FoMo::RenderCube rc3 = make_cube(img, m, lv, bv);
+      if (v + 1 == views.size()) last = rc3;
+      writer.submit(std::move(rc3), outfile + "l" + angle_tag(lv) + "b" + angle_tag(bv) + ".txt");
+    };
+    std::vector<float> img(npix);
+    double m[10];
+    const bool batched = !thomson && !tet && !proj && views.size() > 1;
+    if (batched) {
+      // all NearestNeighbour views in captured batches of at most kBatchImageBytes of images;
+      // a batch gives each view the image fomo_render_nn gives it
+      const size_t per = std::max<size_t>(1, (size_t)(kBatchImageBytes / (4.0 * npix)));
+      for (size_t v0 = 0; v0 < views.size(); v0 += per) {
+        const size_t nv = std::min(per, views.size() - v0);
+        std::vector<double> ls(nv), bs(nv);
+        for (size_t k = 0; k < nv; ++k) ls[k] = views[v0 + k].first, bs[k] = views[v0 + k].second;
+        int bt = -1;
+        if (fomo_batch_create(1, &goft, (int)nv, ls.data(), bs.data(), nx, ny, nz, nl, width, dn ? 1 : 0, &bt) !=
+                FOMO_OK ||
+            fomo_batch_run(bt) != FOMO_OK)
+          die(std::string("Error: render failed: ") + fomo_last_error());
+        for (size_t k = 0; k < nv; ++k) {
+          if (fomo_batch_read_view(bt, (int)k, 0, img.data(), m) != FOMO_OK)
+            die(std::string("Error: render failed: ") + fomo_last_error());
+          emit(img, m, v0 + k);
+        }
+        fomo_batch_destroy(bt);
+      }
+    } else {
+      for (size_t v = 0; v < views.size(); ++v) {
+        const double lv = views[v].first, bv = views[v].second;
+        int rc = thomson ? fomo_render_thomson(goft, lv, bv, nx, ny, nz, img.data(), m)
+                 : tet   ? fomo_render_tet(goft, lv, bv, nx, ny, nz, nl, width, dn ? 1 : 0,
+                                           method == "Linear" ? 1 : 0, img.data(), m)
+                 : proj  ? fomo_render_projection(goft, lv, bv, nx, ny, nz, nl, width, dn ? 1 : 0, img.data(), m)
+                         : fomo_render_nn(goft, lv, bv, nx, ny, nz, nl, width, dn ? 1 : 0, img.data(), m);
+        if (rc != FOMO_OK) die(std::string("Error: render failed: ") + fomo_last_error());
+        emit(img, m, v);
+      }
+    }
+    writer.drain();
+  }
+  last.setrendermethod(method);
+  last.setobservationtype(rendering.readobservationtype());
+  rendering = last;
 }
 
 void FoMo::FoMoObject::render(const double l, const double b) {

@@ -8,6 +8,7 @@
 #define FOMO_H
 
 #include <bitset>
+#include <memory>
 #include <string>
 #include <vector>
 
@@ -114,6 +115,9 @@ class RenderCube : public GoftCube {
   FoMoObservationType readobservationtype();
 };
 
+// Device-side state of a FoMoObject (defined in the library).
+struct FoMoDeviceState;
+
 // Main object: datacube -> goftcube -> rendering (FoMo.h:166-218).
 class FoMoObject {
  protected:
@@ -121,6 +125,11 @@ class FoMoObject {
   FoMo::DataCube datacube;
   FoMo::GoftCube goftcube;
   FoMo::RenderCube rendering;
+
+ private:
+  // Not in the reference: the datacube's upload to the GPU and the emission cube of the last
+  // render(), shared by copies of the object until one of them changes its data.
+  std::shared_ptr<FoMoDeviceState> device;
 
  public:
   FoMoObject(const int = 3);

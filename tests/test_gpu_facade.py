@@ -1,5 +1,6 @@
 """GPU: the FoMo.h C++ API end to end (push_back_datapoint ... render ... writegoftcube), checked
 against the C ABI render and the FP64 oracle, through the binary file it writes."""
+import gzip
 import os
 import subprocess
 
@@ -8,7 +9,7 @@ import pytest
 
 import fomo_amd as F
 from oracle import oracle as O
-from tests.fomo_reader import read_dat
+from tests.fomo_reader import read_dat, read_txt
 
 pytestmark = pytest.mark.gpu
 
@@ -79,3 +80,48 @@ def test_facade_projection_render_file(api_bin, fe9_table, tmp_path):
     assert O.rel_l2(got["cols"][3], ref) <= 1e-6
     lam = (np.arange(nl) / (nl - 1) * (width * 171.0 / C) - width * 171.0 / C / 2 + 171.0).astype(np.float32)
     assert np.array_equal(got["cols"][2].reshape(ny, nx, nl)[3, 4, :], lam)
+
+
+def test_facade_multiview_batch_text_gzip_and_goftcube(engine, api_bin, fe9_table, tmp_path):
+    """render(lvec, bvec) with several NearestNeighbour views runs them as one batch and writes each
+    view's files from a background thread: every view's image equals the single-view C ABI render
+    of that view bit for bit; the text (the reference's default format), binary and gzip files
+    agree; the GoftCube read after the render (also from a copy of the object made before it was
+    read) holds the emission computed on the device."""
+    path, tab = fe9_table
+    m, nx, ny, nz, nl = 20, 32, 24, 30, 30
+    ls, bs = [0.3, 1.1], [1.0, 0.4]
+    prefix = str(tmp_path / "multi.")
+    r = subprocess.run([api_bin, path, prefix, "NearestNeighbour", str(m), str(nx), str(ny), str(nz), str(nl),
+                        ",".join(map(str, ls)), ",".join(map(str, bs)), "0111"], capture_output=True, text=True,
+                       timeout=300)
+    assert r.returncode == 0, r.stderr
+    d = F.gen_loop_cube(m, seed=1)
+    ds = F.Dataset(d["x"], d["y"], d["z"], [d["rho"], d["T"], d["vx"], d["vy"], d["vz"]])
+    T = F.Table.from_file(path)
+    em = F.Emission.chianti(ds, T, tab["lambda0"], tab["atweight"])
+    width = 0.2 * C / 171.0
+    try:
+        for l in ls:
+            for b in bs:
+                tag = f"l{round(l / np.pi * 180):03d}b{round(b / np.pi * 180):03d}"
+                dat = read_dat(prefix + tag + ".dat")
+                img, _ = F.render_nn(em, l, b, nx, ny, nz, nl, width)
+                assert np.array_equal(dat["cols"][3], img), f"view {tag}: batched facade image differs"
+                txt = read_txt(prefix + tag + ".txt")
+                assert (txt["dim"], txt["ng"], txt["nvars"]) == (dat["dim"], dat["ng"], dat["nvars"])
+                assert txt["units"] == dat["units"] and txt["chiantifile"] == dat["chiantifile"]
+                np.testing.assert_allclose(txt["data"].T.astype(np.float32), dat["cols"], rtol=1.2e-7, atol=1e-45)
+                for ext in (".dat", ".txt"):
+                    with gzip.open(prefix + tag + ext + ".gz", "rb") as f:
+                        assert f.read() == open(prefix + tag + ext, "rb").read()
+        gl = [ln.split() for ln in r.stdout.splitlines() if ln.startswith("goft ")]
+        assert len(gl) == 2 and gl[0] == gl[1], r.stdout
+        pk, fw = em.read()
+        assert int(gl[0][1]) == m ** 3 and int(gl[0][2]) == 5
+        assert abs(float(gl[0][3]) / pk.astype(np.float64).sum() - 1) < 1e-12
+        assert abs(float(gl[0][4]) / fw.astype(np.float64).sum() - 1) < 1e-12
+    finally:
+        em.close()
+        T.close()
+        ds.close()
