@@ -49,6 +49,7 @@ _SIGS = {
     "fomo_emission_chianti": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double,
                                              ctypes.c_int, ctypes.c_double, _c_int_p]),
     "fomo_emission_thomson": (ctypes.c_int, [ctypes.c_int, _c_int_p]),
+    "fomo_emission_given": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_double, _c_int_p]),
     "fomo_goft_destroy": (ctypes.c_int, [ctypes.c_int]),
     "fomo_goft_read": (ctypes.c_int, [ctypes.c_int, _c_flt_p, _c_flt_p]),
     "fomo_render_nn": (ctypes.c_int, [ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_int,
@@ -70,6 +71,7 @@ _SIGS = {
     "fomo_server_poll": (ctypes.c_int, [ctypes.c_int, ctypes.c_int64, _c_int_p]),
     "fomo_server_wait": (ctypes.c_int, [ctypes.c_int, ctypes.c_int64, ctypes.c_double, _c_int_p]),
     "fomo_server_wait_all": (ctypes.c_int, [ctypes.c_int, ctypes.c_double]),
+    "fomo_server_cancel": (ctypes.c_int, [ctypes.c_int, ctypes.c_int64]),
     "fomo_server_times": (ctypes.c_int, [ctypes.c_int, ctypes.c_int64, _c_dbl_p]),
     "fomo_server_stats": (ctypes.c_int, [ctypes.c_int, _c_dbl_p]),
     "fomo_delaunay": (ctypes.c_int, [ctypes.c_int64, _c_flt_p, _c_flt_p, _c_flt_p, ctypes.c_int64, _c_i64_p,
@@ -318,6 +320,14 @@ class Emission:
         _check(lib().fomo_emission_thomson(ds.id, ctypes.byref(out)))
         return cls(out.value, ds.n)
 
+    @classmethod
+    def given(cls, ds, lambda0=0.0, thomson=False):
+        """The emission of a dataset holding a computed GoftCube: variables (peak, fwhm, vx, vy, vz)
+        of a line at lambda0, or (n tot, n pol) with thomson=True."""
+        out = ctypes.c_int()
+        _check(lib().fomo_emission_given(ds.id, 1 if thomson else 0, lambda0, ctypes.byref(out)))
+        return cls(out.value, ds.n)
+
     def read(self):
         a = np.zeros(self.n, np.float32)
         b = np.zeros(self.n, np.float32)
@@ -513,6 +523,16 @@ class Server:
         st = ctypes.c_int()
         _check(lib().fomo_server_wait(self.id, ticket, float(timeout), ctypes.byref(st)))
         return self._bufs.pop(ticket)
+
+    def cancel(self, ticket):
+        """Cancel a queued request. True when it was cancelled, False when it had already been
+        admitted or had finished."""
+        rc = lib().fomo_server_cancel(self.id, ticket)
+        if rc == -5:
+            return False
+        _check(rc)
+        self._bufs.pop(ticket, None)
+        return True
 
     def wait_all(self, timeout=-1.0):
         _check(lib().fomo_server_wait_all(self.id, float(timeout)))

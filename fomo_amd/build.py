@@ -1,7 +1,8 @@
 """Build recipe for the gfx950 engine library and the FoMo C++ facade.
 
 Produces, in-tree:
-  fomo_amd/libfomo_hip.so  HIP kernels + C ABI (include/fomo_hip.h)
+  fomo_amd/libfomo_hip.so  HIP kernels + C ABI (include/fomo_hip.h); fomo_device.hip and the eight
+                           k_nn_los instantiation units of fomo_nnlos_part.hip compile in parallel
   fomo_amd/libFoMo.so      FoMo.h C++ API (FoMoObject & co.) on top of libfomo_hip.so
 
 Run `python -m fomo_amd.build` or call build_all(). hipcc cross-compiles without a GPU.
@@ -33,14 +34,32 @@ def _run(cmd):
     subprocess.check_call(cmd)
 
 
+NNLOS_GROUPS = 8  # translation units of fomo_nnlos_part.hip (k_nn_los instantiations)
+
+
+def _run_all(cmds):
+    """Run compile commands in parallel; raise if any fails."""
+    procs = []
+    for cmd in cmds:
+        print("+", " ".join(cmd), flush=True)
+        procs.append((cmd, subprocess.Popen(cmd)))
+    bad = [cmd for cmd, p in procs if p.wait() != 0]
+    if bad:
+        raise subprocess.CalledProcessError(1, bad[0])
+
+
 def build_engine():
     os.makedirs(BUILD, exist_ok=True)
     dev_o = os.path.join(BUILD, "fomo_device.o")
     host_o = os.path.join(BUILD, "fomo_host.o")
-    _run([HIPCC] + HIP_FLAGS + ["-c", os.path.join(CSRC, "fomo_device.hip"), "-o", dev_o])
-    _run(["g++"] + HOST_FLAGS + ["-c", os.path.join(CSRC, "fomo_host.cpp"), "-o", host_o])
+    part_o = [os.path.join(BUILD, f"fomo_nnlos_part{g}.o") for g in range(NNLOS_GROUPS)]
+    cmds = [[HIPCC] + HIP_FLAGS + ["-c", os.path.join(CSRC, "fomo_device.hip"), "-o", dev_o],
+            ["g++"] + HOST_FLAGS + ["-c", os.path.join(CSRC, "fomo_host.cpp"), "-o", host_o]]
+    cmds += [[HIPCC] + HIP_FLAGS + [f"-DFOMO_NNLOS_GROUP={g}", "-c", os.path.join(CSRC, "fomo_nnlos_part.hip"), "-o",
+                                    part_o[g]] for g in range(NNLOS_GROUPS)]
+    _run_all(cmds)
     out = os.path.join(PKG, "libfomo_hip.so")
-    _run([HIPCC, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", out, dev_o, host_o, "-pthread"])
+    _run([HIPCC, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", out, dev_o, host_o] + part_o + ["-pthread"])
     return out
 
 

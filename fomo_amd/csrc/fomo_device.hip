@@ -43,19 +43,11 @@
 #include "lookup_geom.h"
 #include "octree.h"
 #include "spectral_exp.h"
+#include "fomo_common.h"
 
 // ------------------------------------------------------------------------------------
 // constants (GSL CODATA-2006 values used by fomo-CHIANTI.cpp:17-19)
 // ------------------------------------------------------------------------------------
-#define FOMO_C_LIGHT 2.99792458e8
-#define FOMO_K_BOLTZ 1.3806504e-23
-#define FOMO_M_PROTON 1.672621637e-27
-#define FOMO_ALPHA 1.0645
-#define FOMO_MM_PER_ARCSEC 0.715
-#define FOMO_RSUN 695.7
-#define FOMO_U_LIMB 0.58
-#define FOMO_RE 2.8179e-13
-#define FOMO_LUM 2.8e10
 
 // Scratch of the allocation-free NN render (fomo_render_nn_noalloc): one array per GPU, held
 // until the process ends. In host code `scratch` names a host-side stand-in, not device memory;
@@ -64,6 +56,8 @@
 static __device__ __align__(256) unsigned char scratch[FOMO_NOALLOC_SCRATCH_BYTES];
 
 namespace {
+
+using namespace fomo_dev;
 
 thread_local std::string g_err;
 std::recursive_mutex g_mu;
@@ -116,93 +110,6 @@ struct DevBuf {
   }
   template <class T> T* as() const { return reinterpret_cast<T*>(p); }
 };
-
-// ------------------------------------------------------------------------------------
-// shared structs (host + device)
-// ------------------------------------------------------------------------------------
-struct GridInfo {
-  double o[3];      // origin (data bbox minimum)
-  double h;         // cell edge
-  double invh;      // 1/h
-  double slack;     // conservative margin on cell-boundary distances
-  float bmin[3], bmax[3];
-  int dims[3];
-  int pad;
-};
-
-// Candidate lookup grid: a finer grid over the data bbox whose cells list the only points that
-// can be the (unconstrained) nearest point of any target inside the cell.
-struct LookupInfo {
-  double o[3];               // origin (shifted off the cell-list lattice)
-  double h[3], invh[3];      // lookup cell edges (the x edge is up to twice the others)
-  double eps;                // box expansion covering the rounding of a target's cell assignment
-  int dims[3];
-  int enabled;
-  // outer groups: candidate lists for targets beyond the grid (ohdr = nullptr: none)
-  double oo[3], oinvh[3];    // origin and inverse edges of the outer group lattice
-  int odims[3];
-  const int2* ohdr;          // per outer group: (start in lcand, count), count 0 for none
-  // far groups whose cells all use the group's list (ghdr = nullptr: none): read before the record
-  int gcell[3], gdims[3];    // lookup cells per group edge, groups per axis
-  const int2* ghdr;          // per group: (start in lcand, count), count 0 when a cell has its own
-};
-
-// Start grid of the tetra walk (a finite tet near each cell centre) and the points' bounding box,
-// which contains the convex hull: a sample outside the box is outside the hull without a walk.
-struct TetGrid {
-  double o[3], h, invh;
-  double lo[3], hi[3];
-  int dims[3];
-};
-
-// Per-view geometry, filled on the device by k_view_finalize.
-struct ViewGeom {
-  double cb, sb, cl, sl;          // trig of b and l for the LOS samples (host glibc)
-  double ext[6];                  // minx,maxx,miny,maxy,minz,maxz of rotated points
-  double md;                      // search-box half width
-  double deltaz;                  // LOS sample spacing
-  double pathlength, apix, scale; // output scaling
-};
-
-enum { MODE_NN = 0, MODE_THOMSON = 1, MODE_TET = 2, MODE_LINEAR = 3, MODE_PROJ = 4 };
-
-// Spectral lines rendered together from one NN result (blockIdx.y selects the line).
-constexpr int kMaxLines = 8;
-// Per-point records for the NN passes: {vx, vy, vz, T} then the peak of every line. A line's
-// fwhm is not stored: it is recomputed exactly as the emission kernel computed it,
-// (float)(K * sqrt((double)T)), or 1 for imaging cubes (K == 0) (fomo-CHIANTI.cpp:220-225, :284-285).
-struct RecInfo {
-  double K[kMaxLines];
-};
-
-struct LineSet {
-  const float2* em[kMaxLines];  // (peak, fwhm): per sorted point (record build) or per sample (K5)
-  float* out[kMaxLines];        // R*nl output per line
-  double wla[kMaxLines];        // lambda window width in Angstrom
-  double l0c[kMaxLines];        // lambda0 / c
-  int n;
-};
-
-constexpr int kBlock = 256;
-constexpr int kMaxPartials = 2048;  // rotate kernel grid cap (grid-stride beyond)
-
-// ------------------------------------------------------------------------------------
-// small device helpers
-// ------------------------------------------------------------------------------------
-__device__ __forceinline__ int cell_coord(const GridInfo& g, int a, double v) {
-  double c = floor((v - g.o[a]) * g.invh);
-  c = fmin(fmax(c, 0.0), (double)(g.dims[a] - 1));
-  return (int)c;
-}
-
-template <class T> __device__ __forceinline__ T wave_min(T v) {
-  for (int off = 32; off > 0; off >>= 1) v = fmin(v, __shfl_xor(v, off, 64));
-  return v;
-}
-template <class T> __device__ __forceinline__ T wave_max(T v) {
-  for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
-  return v;
-}
 
 // Block-wide min/max of 6 doubles (vals[0,2,4] minima, [1,3,5] maxima) -> out (6 doubles).
 __device__ void block_minmax6(double v[6], double* out) {
@@ -548,28 +455,6 @@ __global__ void k_view_finalize(const double* __restrict__ partial, int np, int 
 // Cells with no point within 3 coarse rings, or more than cap candidates, are marked -1 and their
 // queries use the exact ring search.
 // ------------------------------------------------------------------------------------
-constexpr int kLookupCap = 24;
-// Lookup cell record: kRecSlots float4 (128 bytes, one L2 line: an L2 miss fetches an aligned
-// 128-byte line) holding up to kRecSlots candidates (x, y, z, sorted position as int bits; unused
-// slots have w = -1). Slot 0 with w = -1: no candidates (the query falls back to the exact
-// search); w = -2: more than kRecSlots candidates, at lcand[x] .. (y of them, x and y as int bits).
-// Queries load the first kSlots slots ahead of time and the rest, from the same line, only when
-// slot kSlots - 1 says that more follow (its w = -3 - sorted position).
-constexpr int kSlots = 4;
-constexpr int kRecSlots = 8;
-constexpr int kLookupMargin = 6;  // cell-list cells the lookup grid extends beyond the data bbox
-// cells farther than this from the nearest non-empty cell take their far group's list instead
-// of a list of their own (their own search box would span too many cells)
-constexpr int kLookupNearRings = 2;
-// cap of the cell list's distance transform: the ring search of a cell at distance d visits rings
-// d and d + 1
-constexpr int kLookupDistCap = kLookupNearRings + 1;
-
-using fomo::box_maxdist2;
-using fomo::box_mindist2;
-using fomo::dominated;
-using fomo::node_dominated;
-
 // MODE 0: count the candidates of every cell (cnt, -1 for cells left to the exact search);
 // MODE 1: write them, given those counts and the overflow lists' offsets (off);
 // MODE 2: one pass: write each record directly, cells with more than kRecSlots candidates taking
@@ -798,130 +683,6 @@ __global__ void k_clamp_counts(const int32_t* __restrict__ cnt, int64_t ncl, int
   pos[L] = max(cnt[L], 0);
 }
 
-// Lookup cell of a target (-1 outside the lookup grid).
-__device__ __forceinline__ int64_t lookup_cell(const LookupInfo& lk, float t0, float t1, float t2) {
-  const double v0 = ((double)t0 - lk.o[0]) * lk.invh[0];
-  const double v1 = ((double)t1 - lk.o[1]) * lk.invh[1];
-  const double v2 = ((double)t2 - lk.o[2]) * lk.invh[2];
-  if (!(v0 >= 0.0 && v0 < (double)lk.dims[0] && v1 >= 0.0 && v1 < (double)lk.dims[1] && v2 >= 0.0 &&
-        v2 < (double)lk.dims[2]))
-    return -1;
-  return ((int64_t)(int)v2 * lk.dims[1] + (int)v1) * lk.dims[0] + (int)v0;
-}
-
-// The nearest candidate of a lookup cell record sl (read from the overflow list for long lists),
-// if it lies strictly inside the search box; else -2.
-__device__ __forceinline__ int lookup_pick(const float4* __restrict__ pts, const float4* __restrict__ lcand,
-                                           const float4* __restrict__ rp, const float4 (&sl)[kSlots], float t0,
-                                           float t1, float t2, float lo0,
-                                           float lo1, float lo2, float hi0, float hi1, float hi2) {
-  const int w0 = __float_as_int(sl[0].w);
-  if (w0 == -1) return -2;
-  // candidates hold their coordinates inline (x, y, z, sorted position); the original index
-  // (tie-break) is read from pts only on an exact distance tie
-  double best = INFINITY;
-  int bpos = -1;
-  float4 bq = make_float4(0.f, 0.f, 0.f, 0.f);
-  const auto consider = [&](const float4 q) {
-    const double d0 = (double)q.x - (double)t0;
-    const double d1 = (double)q.y - (double)t1;
-    const double d2 = (double)q.z - (double)t2;
-    const double d = (d0 * d0 + d1 * d1) + d2 * d2;
-    const int p = __float_as_int(q.w);
-    bool take = d < best;
-    if (d == best && bpos >= 0) take = __float_as_int(pts[p].w) < __float_as_int(pts[bpos].w);
-    if (take) {
-      best = d;
-      bpos = p;
-      bq = q;
-    }
-  };
-  if (w0 != -2) {
-    const int wl = __float_as_int(sl[kSlots - 1].w);
-#pragma unroll
-    for (int q = 0; q < kSlots - 1; ++q)
-      if (__float_as_int(sl[q].w) >= 0) consider(sl[q]);
-    if (wl >= 0) consider(sl[kSlots - 1]);
-    if (wl <= -3) {
-      consider(make_float4(sl[kSlots - 1].x, sl[kSlots - 1].y, sl[kSlots - 1].z, __int_as_float(-3 - wl)));
-      // the rest of the record: the same 128-byte line as sl
-      float4 s2[kRecSlots - kSlots];
-#pragma unroll
-      for (int q = 0; q < kRecSlots - kSlots; ++q) s2[q] = rp[kSlots + q];
-#pragma unroll
-      for (int q = 0; q < kRecSlots - kSlots; ++q)
-        if (__float_as_int(s2[q].w) >= 0) consider(s2[q]);
-    }
-  } else {
-    // four candidates per step: their loads are in flight together
-    const float4* ov = lcand + __float_as_int(sl[0].x);
-    const int n = __float_as_int(sl[0].y);
-    for (int i = 0; i < n; i += 4) {
-      float4 c4[4];
-#pragma unroll
-      for (int u = 0; u < 4; ++u)
-        if (i + u < n) c4[u] = ov[i + u];
-#pragma unroll
-      for (int u = 0; u < 4; ++u)
-        if (i + u < n) consider(c4[u]);
-    }
-  }
-  if (lo0 < bq.x && bq.x < hi0 && lo1 < bq.y && bq.y < hi1 && lo2 < bq.z && bq.z < hi2) return bpos;
-  return -2;
-}
-
-// Outer group of a target beyond the lookup grid (-1 outside the outer lattice).
-__device__ __forceinline__ int64_t outer_group(const LookupInfo& lk, float t0, float t1, float t2) {
-  const double v0 = ((double)t0 - lk.oo[0]) * lk.oinvh[0];
-  const double v1 = ((double)t1 - lk.oo[1]) * lk.oinvh[1];
-  const double v2 = ((double)t2 - lk.oo[2]) * lk.oinvh[2];
-  if (!(v0 >= 0.0 && v0 < (double)lk.odims[0] && v1 >= 0.0 && v1 < (double)lk.odims[1] && v2 >= 0.0 &&
-        v2 < (double)lk.odims[2]))
-    return -1;
-  return ((int64_t)(int)v2 * lk.odims[1] + (int)v1) * lk.odims[0] + (int)v0;
-}
-
-// Targets beyond the lookup grid: the list of their outer group, if it has one (else -2).
-__device__ __forceinline__ int outer_lookup(const float4* __restrict__ pts, const float4* __restrict__ lcand,
-                                            const LookupInfo& lk, float t0, float t1, float t2, float lo0,
-                                            float lo1, float lo2, float hi0, float hi1, float hi2) {
-  const int64_t og = outer_group(lk, t0, t1, t2);
-  if (og < 0) return -2;
-  const int2 hd = lk.ohdr[og];
-  if (hd.y <= 0) return -2;
-  const float4 h4 = make_float4(__int_as_float(hd.x), __int_as_float(hd.y), 0.f, __int_as_float(-2));
-  const float4 sl[kSlots] = {h4, h4, h4, h4};  // an overflow header: only slot 0 is read
-  return lookup_pick(pts, lcand, nullptr, sl, t0, t1, t2, lo0, lo1, lo2, hi0, hi1, hi2);
-}
-
-// Fast path: the unconstrained nearest point from the target's lookup cell. Returns its sorted
-// position if it lies strictly inside the search box, else -2 (the caller runs the exact query).
-__device__ __forceinline__ int nn_lookup(const float4* __restrict__ pts, const float4* __restrict__ lslot,
-                                         const float4* __restrict__ lcand, const LookupInfo& lk, float t0, float t1,
-                                         float t2, float lo0, float lo1, float lo2, float hi0, float hi1, float hi2) {
-  const int64_t L = lookup_cell(lk, t0, t1, t2);
-  if (L < 0) return lk.ohdr ? outer_lookup(pts, lcand, lk, t0, t1, t2, lo0, lo1, lo2, hi0, hi1, hi2) : -2;
-  if (lk.ghdr) {
-    // a group whose cells all point at its list: its 8-byte header instead of the cell's record
-    const int lx = (int)(L % lk.dims[0]);
-    const int ly = (int)((L / lk.dims[0]) % lk.dims[1]);
-    const int lz = (int)(L / ((int64_t)lk.dims[0] * lk.dims[1]));
-    const int2 hd = lk.ghdr[((int64_t)(lz / lk.gcell[2]) * lk.gdims[1] + ly / lk.gcell[1]) * lk.gdims[0] +
-                            lx / lk.gcell[0]];
-    if (hd.y > 0) {
-      const float4 h4 = make_float4(__int_as_float(hd.x), __int_as_float(hd.y), 0.f, __int_as_float(-2));
-      const float4 sg[kSlots] = {h4, h4, h4, h4};  // an overflow header: only slot 0 is read
-      return lookup_pick(pts, lcand, nullptr, sg, t0, t1, t2, lo0, lo1, lo2, hi0, hi1, hi2);
-    }
-  }
-  // the first half of the 128-byte record; the rest and the overflow list only when taken
-  const float4* rp = lslot + (int64_t)kRecSlots * L;
-  float4 sl[kSlots];
-#pragma unroll
-  for (int q = 0; q < kSlots; ++q) sl[q] = rp[q];
-  return lookup_pick(pts, lcand, rp, sl, t0, t1, t2, lo0, lo1, lo2, hi0, hi1, hi2);
-}
-
 // ------------------------------------------------------------------------------------
 // Octree over the points in Morton order (exact fallback of the NN query), in depth-first
 // preorder. Every node covers a contiguous range of the Morton-sorted points; an internal node
@@ -935,11 +696,6 @@ __device__ __forceinline__ int nn_lookup(const float4* __restrict__ pts, const f
 // inside the box, ties to the lowest original index (the reference's R-tree query,
 // fomo-nearestneighbour.cpp:179-183, fomo-thomson.cpp:310).
 // ------------------------------------------------------------------------------------
-struct BvhInfo {
-  int64_t nodes;  // 0: no tree
-  int64_t n;      // points
-};
-
 __global__ void k_morton_keys(const float4* __restrict__ pts, int64_t n, GridInfo g, uint32_t* __restrict__ key,
                               int32_t* __restrict__ val) {
   const int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
@@ -989,59 +745,6 @@ __global__ void k_oct_boxes(const int32_t* __restrict__ ids, int64_t cnt, const 
   nhi[i] = make_float4(h.x, h.y, h.z, b0.w);
 }
 
-using fomo::SearchBox;
-
-__device__ __forceinline__ int bvh_query(const float4* __restrict__ pts, const float4* __restrict__ mpts,
-                                         const float4* __restrict__ nlo, const float4* __restrict__ nhi,
-                                         const BvhInfo& bv, const SearchBox& sb, double best = INFINITY,
-                                         int bpos = -1, int borig = 0x7fffffff) {
-  return fomo::oct_query(pts, mpts, nlo, nhi, bv.nodes, sb, best, bpos, borig);
-}
-
-// Wave-cooperative oct_query: every lane with act set gets exactly oct_query's answer, and the
-// wave walks the tree once. A node is entered when any active lane could find a better point in
-// it; a lane's own test fails on every node inside a subtree it would have skipped (a child's box
-// lies inside its parent's), so each lane scans exactly the leaves of its own walk, in the same
-// order. The node index is wave-uniform, so the node records are read once per wave (scalar
-// loads) instead of once per lane; the 64 queries of a wave are neighbouring samples and enter
-// nearly the same nodes. All lanes of the wave that are still running must call it.
-__device__ int bvh_query_wave(const float4* __restrict__ pts, const float4* __restrict__ mpts,
-                              const float4* __restrict__ nlo, const float4* __restrict__ nhi, const BvhInfo& bv,
-                              const SearchBox& sb, bool act, double best, int bpos, int borig) {
-  if (bv.nodes <= 0) return bpos;
-  if (act && bpos < 0) fomo::oct_descend(pts, mpts, nlo, nhi, sb, best, bpos, borig);
-  // The node test in float: the squared box distance has a relative error below 1e-6 (exact
-  // differences of nearby floats, then three rounded squares and two rounded sums), so with a
-  // 1e-5 margin and the bound rounded up it enters every node oct_enter enters. Entering more
-  // nodes cannot change a lane's answer: its extra leaves only add points that lose to the
-  // nearest one.
-  const auto bound = [](double b) { return (float)b * 1.000001f; };
-  float bestf = bound(best);
-  const int n = (int)bv.nodes;
-  for (int e = 0; e < n;) {
-    e = __builtin_amdgcn_readfirstlane(e);
-    const float4 l = nlo[e];
-    const float4 h = nhi[e];
-    const int w = __float_as_int(l.w);
-    const float d0 = fmaxf(fmaxf(l.x - sb.t0, sb.t0 - h.x), 0.f);
-    const float d1 = fmaxf(fmaxf(l.y - sb.t1, sb.t1 - h.y), 0.f);
-    const float d2 = fmaxf(fmaxf(l.z - sb.t2, sb.t2 - h.z), 0.f);
-    const float md = (d0 * d0 + d1 * d1) + d2 * d2;
-    const bool in = act && sb.node_overlaps(l, h) && md * 0.99999f <= bestf;
-    const bool any = __ballot(in) != 0;
-    if (w < 0) {
-      if (in) {
-        fomo::leaf_scan(pts, mpts, __float_as_int(h.w), -w, sb, best, bpos, borig);
-        bestf = bound(best);
-      }
-      ++e;
-    } else {
-      e = any ? e + 1 : w;
-    }
-  }
-  return bpos;
-}
-
 // Per-sample outputs of the NN pass for the point at sorted position r (-1: none):
 // svlos[s] (LOS velocity, fomo-nearestneighbour.cpp:87-89) and sem[l*S + s] = (peak_l, fwhm_l),
 // zero when no point is inside the search box (:190-193).
@@ -1066,12 +769,12 @@ __device__ __forceinline__ void write_sample(int r, const float4* __restrict__ r
   // unit vector (sin b cos l, -sin b sin l, cos b)
   svlos[sidx] = r >= 0 ? (float)(((0.0 + (sb * cl) * (double)f[0]) + (-sb * sl) * (double)f[1]) + cb * (double)f[2])
                        : 0.f;
-  const double st = r >= 0 ? sqrt((double)f[3]) : 0.0;
+  const double st = r >= 0 ? rec_st(ri.K[0], f[3]) : 0.0;
   // a record of RS4 float4 holds 4*RS4-4 lines after {v, T}
 #pragma unroll
   for (int l = 0; l < 4 * RS4 - 4; ++l)
     if (l < nlines) {
-      const float fw = r < 0 ? 0.f : (ri.K[l] > 0 ? (float)(ri.K[l] * st) : 1.f);
+      const float fw = r < 0 ? 0.f : rec_fw(ri.K[l], st);
       sem[(size_t)l * S + sidx] = make_float2(f[4 + l], fw);
     }
 }
@@ -1661,11 +1364,11 @@ __global__ void __launch_bounds__(256) k_tet_walk(const float4* __restrict__ tre
             f[4 * qq] = a.x, f[4 * qq + 1] = a.y, f[4 * qq + 2] = a.z, f[4 * qq + 3] = a.w;
           }
           vl += wm * (((0.0 + (sb * cl) * (double)f[0]) + (-sb * sl) * (double)f[1]) + cb * (double)f[2]);
-          const double st = sqrt((double)f[3]);
+          const double st = rec_st(ri.K[0], f[3]);
 #pragma unroll
           for (int l = 0; l < NL; ++l) {
             pk[l] += wm * (double)f[4 + l];
-            fw[l] += wm * (double)(ri.K[l] > 0 ? (float)(ri.K[l] * st) : 1.f);
+            fw[l] += wm * (double)rec_fw(ri.K[l], st);
           }
         }
         svlos[sidx] = (float)vl;
@@ -1724,41 +1427,6 @@ __global__ void k_proj_starts(const uint32_t* __restrict__ key, int64_t n, int64
   start[q] = (int32_t)lo;
 }
 
-// The Gaussian line profile exp(-kap (d0 + c dl)^2) over lambda pixels c = 0, 1, ... by ratio
-// recurrence. profile_args: xa = {aq, ar, a0}, the exponents of the ratio of ratios, of the first
-// ratio (pixel 1 over pixel 0) and of pixel 0; false when the recurrence could leave the double
-// range (the caller then evaluates every pixel directly).
-__device__ __forceinline__ bool profile_args(double kap, double d0, double dl, double (&xa)[3]) {
-  const double a0 = -kap * d0 * d0;
-  const double ar = -kap * (2.0 * d0 * dl + dl * dl);
-  const double aq = -2.0 * kap * dl * dl;
-  xa[0] = aq, xa[1] = ar, xa[2] = a0;
-  return a0 > -600.0 && fabs(ar) < 100.0 && aq > -30.0;
-}
-
-// Stride-4 chains of the profile from ex = {q, r, g_0} = exp(xa): chain j holds the pixels
-// j, j+4, j+8, ..., starting at g[j] with ratio R[j]; one step is g *= R, R *= q16.
-//   g_{c+4} = g_c R_c,  R_{c+4} = R_c q^16,  R_0 = r_0 r_1 r_2 r_3 = r^4 q^6
-// Every kernel that integrates the profile builds its chains here, so that the same pixel gets
-// the same value in all of them.
-__device__ __forceinline__ void profile_chains(const double (&ex)[3], double (&g)[4], double (&R)[4], double& q16) {
-  const double q = ex[0], ra = ex[1];
-  double r = ra;
-  g[0] = ex[2];
-  g[1] = g[0] * r;
-  r *= q;
-  g[2] = g[1] * r;
-  r *= q;
-  g[3] = g[2] * r;
-  const double q2 = q * q, q4 = q2 * q2;
-  q16 = (q4 * q4) * (q4 * q4);
-  const double ra2 = ra * ra;
-  R[0] = (ra2 * ra2) * (q4 * q2);
-  R[1] = R[0] * q4;
-  R[2] = R[1] * q4;
-  R[3] = R[2] * q4;
-}
-
 // One thread per (pixel, chunk of kProjChunk lambda pixels), adding the pixel's points in point
 // order; nl == 1: one thread per pixel summing the peaks (Imaging, :148-155). A point's Gaussian
 // over the chunk comes from the ratio recurrence (three exps per point and chunk,
@@ -1800,7 +1468,7 @@ __global__ void __launch_bounds__(256) k_proj_spectral(const int32_t* __restrict
     const float pk = rp[1].x;
     if (pk == 0.f) continue;  // adds zeros
     const double losvel = ((0.0 + u0 * (double)a.x) + u1 * (double)a.y) + u2 * (double)a.z;  // :53-54
-    const double fw = K > 0 ? (double)(float)(K * sqrt((double)a.w)) : 1.0;
+    const double fw = (double)rec_fw(K, rec_st(K, a.w));
     const double mu = losvel / FOMO_C_LIGHT * lambda0;
     double xa[3];
     if (profile_args(fomo::spectral_kappa(fw), lamc - mu, dl, xa)) {
@@ -1834,6 +1502,12 @@ __global__ void __launch_bounds__(256) k_proj_spectral(const int32_t* __restrict
 }
 
 // Per-point records for k_nn (sorted order): {vx, vy, vz, T, peak_0, peak_1, ...}, rs4 float4 each.
+// Emission pair of every sorted point from two of its variables (a given GoftCube's).
+__global__ void k_pack_em(const float* __restrict__ a, const float* __restrict__ b, int64_t n, float2* __restrict__ em) {
+  const int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (p < n) em[p] = make_float2(a[p], b[p]);
+}
+
 __global__ void k_build_rec(const float4* __restrict__ vel, const float* __restrict__ T, const LineSet ls, int64_t n,
                             int rs4, float4* __restrict__ rec) {
   const int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
@@ -1953,389 +1627,6 @@ __global__ void __launch_bounds__(256) k_los_spectral(int R, int nz, const LineS
 #pragma unroll
   for (int c = 0; c < CH; ++c)
     if (l0 + c < nl) o[c] = (float)(sc * acc[c]);
-}
-
-// f(integral_constant<int, I>) for I = 0 .. N-1, unrolled at compile time
-template <int N, int I = 0, class F>
-__device__ __forceinline__ void static_for(F&& f) {
-  if constexpr (I < N) {
-    f(std::integral_constant<int, I>{});
-    static_for<N, I + 1>(f);
-  }
-}
-
-// Lane O of each aligned group of G lanes, to every lane of the group. G = 2, 4: a DPP quad
-// permutation (a VALU operand modifier, no LDS round trip; every lane of the quad must be
-// active); G = 8: ds_bpermute.
-template <int G, int O>
-__device__ __forceinline__ int group_bcast(int x) {
-  static_assert(O >= 0 && O < G, "lane outside the group");
-  if constexpr (G == 4) {
-    return __builtin_amdgcn_update_dpp(0, x, O * 0x55, 0xF, 0xF, false);
-  } else if constexpr (G == 2) {
-    return __builtin_amdgcn_update_dpp(0, x, O | (O << 2) | ((2 + O) << 4) | ((2 + O) << 6), 0xF, 0xF, false);
-  } else {
-    return __shfl(x, (int)(threadIdx.x & 63 & ~(G - 1)) + O, 64);
-  }
-}
-template <int G, int O>
-__device__ __forceinline__ float group_bcast(float x) {
-  return __int_as_float(group_bcast<G, O>(__float_as_int(x)));
-}
-template <int G, int O>
-__device__ __forceinline__ double group_bcast(double x) {
-  const int lo = group_bcast<G, O>(__double2loint(x)), hi = group_bcast<G, O>(__double2hiint(x));
-  return __hiloint2double(hi, lo);
-}
-
-// ------------------------------------------------------------------------------------
-// Fused NN query + spectral integration (k_nn, k_nn_far and k_los_spectral in one pass, for
-// 2 <= nl <= 64: several lines, or one line on views of at least kFusedMinRays rays). G lanes per ray query G consecutive samples at once (lane q:
-// sample kb + q; lookup grid, then the exact octree query) and hand each sample's values to the
-// other lanes of the ray by cross-lane broadcasts, so the per-sample values never go through HBM
-// and the memory-bound queries of some waves overlap the FP64 integration of others. Two ways
-// to split the integration over the G lanes:
-//   SL == 0: lane q integrates line q (G = lines rounded up to 2, 4, 8) with the step of
-//            k_los_spectral, from the broadcast (LOS velocity, sqrt T, peak).
-//   SL > 0:  the lines share one profile (equal lambda0, lambda window and thermal-width factor,
-//            so their Gaussians differ only in the peak; lines beyond ls.n have peak 0), or there
-//            is one line (SL = 1). G = 4:
-//            lane q evaluates the anchors of its own sample, the ray's 4 lanes exchange them,
-//            and lane q runs the profile's chain q (pixels q, q+4, ...) of every sample, adding
-//            peak_l * g to line l's sums. The profile is evaluated once for all lines.
-// Both add, for each (ray, line, pixel), the same values in the same order as k_nn +
-// k_los_spectral (profile_chains, then fma(peak, g, sum)), so a batch of L lines and L
-// single-line renders give identical images. As in k_los_spectral, a (ray, line) with a sample
-// whose recurrence anchors would under- or overflow is flagged (flags[line * R + ray]) and redone
-// by the DIRECT instantiation, launched right after, with one exp per sample and lambda pixel.
-// ------------------------------------------------------------------------------------
-template <int RS4, int CH, int G, bool DIRECT, int SL, int NP = 1>
-__global__ void __launch_bounds__(256) k_nn_los(const float4* __restrict__ pts, GridInfo g,
-                                                const float4* __restrict__ lslot, const float4* __restrict__ lcand,
-                                                LookupInfo lk, const float4* __restrict__ mpts,
-                                                const float4* __restrict__ nlo, const float4* __restrict__ nhi,
-                                                BvhInfo bv, const ViewGeom* __restrict__ vgp, int nx, int ny, int nz,
-                                                const float4* __restrict__ rec, RecInfo ri, const LineSet ls, int nl,
-                                                int32_t* __restrict__ flags) {
-  constexpr int NL = 4 * RS4 - 4;  // lines a record holds
-  constexpr int RPW = 64 / G;      // rays per wave
-  static_assert(SL == 0 || (G == 4 && SL <= NL && CH % 4 == 0), "a shared profile runs 4 chains per ray");
-  static_assert(NP == 1 || (SL > 0 && NP * SL <= 8), "several lambda passes only with a shared profile");
-  constexpr int CJ = CH / 4;                  // pixels of one chain (SL > 0)
-  // SL > 0: acc[(p * SL + l) * CJ + m] is pass p, line l, pixel p * CH + 4 m + q
-  constexpr int NACC = SL ? NP * SL * CJ : CH;
-  const int lane = threadIdx.x & 63;
-  const int q = lane & (G - 1);
-  const int64_t R = (int64_t)nx * ny;
-  const int64_t ray = ((int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) * RPW + lane / G;
-  const bool live = ray < R;
-  int32_t* const flag = flags + (size_t)(SL > 0 ? 0 : q) * R + (live ? ray : 0);
-  // flagged (ray, line, lambda pass)s of this lane: bit l * NP + p (shared profile; flags[l * R +
-  // ray] holds the passes' bits), bit 0 (line q). The direct pass reads them first: usually none
-  // is set, and whole waves leave before any other work.
-  unsigned need = 0;
-  if (DIRECT) {
-    if (live && (SL > 0 || q < ls.n)) {
-      if constexpr (SL > 0) {
-#pragma unroll
-        for (int l = 0; l < SL; ++l)
-          if (l < ls.n) need |= ((unsigned)flag[(size_t)l * R] & ((1u << NP) - 1)) << (l * NP);
-      } else {
-        need = *flag != 0 ? 1u : 0u;
-      }
-    }
-    if (__ballot(need != 0) == 0) return;
-  }
-  const int i = live ? (int)(ray / nx) : 0, j = live ? (int)(ray % nx) : 0;
-  const ViewGeom vg = *vgp;
-  const double minx = vg.ext[0], maxx = vg.ext[1], miny = vg.ext[2], maxy = vg.ext[3], minz = vg.ext[4];
-  const double xx = double(j) / (nx - 1) * (maxx - minx) + minx;  // fomo-nearestneighbour.cpp:152
-  const double yy = double(i) / (ny - 1) * (maxy - miny) + miny;  // :153
-  const double cb = vg.cb, sb = vg.sb, cl = vg.cl, sl_ = vg.sl, md = vg.md;
-  // the profile constants: line q's, selected without lane-dependent indexing of the kernel
-  // arguments, or (shared profile) line 0's
-  double Kq = 0.0, wla = 0.0, l0c = 0.0;
-  float* out = nullptr;
-  if constexpr (SL > 0) {
-    Kq = ri.K[0], wla = ls.wla[0], l0c = ls.l0c[0];
-  } else {
-#pragma unroll
-    for (int l = 0; l < kMaxLines; ++l)
-      if (l == q) {
-        Kq = ri.K[l];
-        wla = ls.wla[l];
-        l0c = ls.l0c[l];
-        out = ls.out[l];
-      }
-  }
-  bool integ = live && (SL > 0 || q < ls.n);
-  if (DIRECT) integ = integ && need != 0;
-  unsigned bad = 0;  // bits as in need: recurrence anchors out of range
-  const double dl = wla / (nl - 1);
-  const double lamc = double(0) / (nl - 1) * wla - wla / 2.;  // lambdaval of the first pixel (:200)
-  // lambdaval of the first pixel of pass p (k_los_spectral's l0 = p * CH)
-  const auto lamc_of = [&](int p) { return double(p * CH) / (nl - 1) * wla - wla / 2.; };
-  double acc[NACC];
-#pragma unroll
-  for (int c = 0; c < NACC; ++c) acc[c] = 0.0;
-  int prev = -1;  // this lane's last resolved point: the warm start of the exact query
-  // Three-stage pipeline over groups of G samples: while group kb is integrated, the record of
-  // group kb + G (winner picked from its lookup cell record) and the lookup cell record of group
-  // kb + 2G are in flight.
-  // stage 1: target, search box and lookup cell record of this lane's sample k
-  const auto query_issue = [&](int k, SearchBox& box, const float4*& rp, float4 (&sl)[kSlots], bool& act,
-                               bool& cell) {
-    act = live && k < nz;
-    cell = false;
-    if (!act) return;
-    const double zz = double(k) * vg.deltaz + minz;             // :162
-    const double p0 = xx * cb * cl + yy * sl_ + zz * sb * cl;   // :165
-    const double p1 = -xx * cb * sl_ + yy * cl - zz * sb * sl_;
-    const double p2 = -xx * sb + zz * cb;
-    box.t0 = (float)p0, box.t1 = (float)p1, box.t2 = (float)p2;  // :171
-    box.lo0 = (float)(p0 - md), box.lo1 = (float)(p1 - md), box.lo2 = (float)(p2 - md);  // :179
-    box.hi0 = (float)(p0 + md), box.hi1 = (float)(p1 + md), box.hi2 = (float)(p2 + md);
-    if (!lk.enabled) return;
-    const int64_t L = lookup_cell(lk, box.t0, box.t1, box.t2);
-    if (L < 0) return;
-    cell = true;
-    rp = lslot + (int64_t)kRecSlots * L;
-#pragma unroll
-    for (int c = 0; c < kSlots; ++c) sl[c] = rp[c];
-  };
-  // stage 2: the winner (lookup, else the exact octree query), -1 for none
-  const auto query_resolve = [&](const SearchBox& box, const float4* rp, const float4 (&sl)[kSlots], bool act,
-                                 bool cell) -> int {
-    if (!act) return -1;
-    int r = cell ? lookup_pick(pts, lcand, rp, sl, box.t0, box.t1, box.t2, box.lo0, box.lo1, box.lo2, box.hi0,
-                               box.hi1, box.hi2)
-                 : -2;
-    if (r == -2) {
-      if (!(box.hi0 > g.bmin[0] && box.lo0 < g.bmax[0] && box.hi1 > g.bmin[1] && box.lo1 < g.bmax[1] &&
-            box.hi2 > g.bmin[2] && box.lo2 < g.bmax[2])) {
-        r = -1;  // no data point can be strictly inside the box
-      } else {
-        double best = INFINITY;
-        int bpos = -1, borig = 0x7fffffff;
-        if (prev >= 0) {
-          const float4 pq = pts[prev];
-          if (box.holds(pq)) {
-            const double d0 = (double)pq.x - (double)box.t0, d1 = (double)pq.y - (double)box.t1,
-                         d2 = (double)pq.z - (double)box.t2;
-            best = (d0 * d0 + d1 * d1) + d2 * d2;
-            bpos = prev;
-            borig = __float_as_int(pq.w);
-          }
-        }
-        r = bvh_query(pts, mpts, nlo, nhi, bv, box, best, bpos, borig);
-      }
-    }
-    if (r >= 0) prev = r;
-    return r;
-  };
-  const auto record_issue = [&](int r, float (&f)[4 * RS4]) {
-#pragma unroll
-    for (int c = 0; c < 4 * RS4; ++c) f[c] = 0.f;
-    if (r < 0) return;
-    const float4* rp = rec + (size_t)r * RS4;
-#pragma unroll
-    for (int c = 0; c < RS4; ++c) {
-      const float4 a = rp[c];
-      f[4 * c] = a.x, f[4 * c + 1] = a.y, f[4 * c + 2] = a.z, f[4 * c + 3] = a.w;
-    }
-  };
-  SearchBox boxN;
-  const float4* rpN = lslot;
-  float4 slN[kSlots];
-  bool actN, cellN;
-  float fC[4 * RS4];
-  query_issue(q, boxN, rpN, slN, actN, cellN);
-  int rC = query_resolve(boxN, rpN, slN, actN, cellN);
-  record_issue(rC, fC);
-  query_issue(G + q, boxN, rpN, slN, actN, cellN);
-  for (int kb = 0; kb < nz; kb += G) {
-    SearchBox boxA;
-    const float4* rpA = lslot;
-    float4 slA[kSlots];
-    bool actA, cellA;
-    query_issue(kb + 2 * G + q, boxA, rpA, slA, actA, cellA);
-    const int rB = query_resolve(boxN, rpN, slN, actN, cellN);
-    float fB[4 * RS4];
-    record_issue(rB, fB);
-    // group kb: this lane's sample, the values write_sample would store
-    const float vls = rC >= 0 ? (float)(((0.0 + (sb * cl) * (double)fC[0]) + (-sb * sl_) * (double)fC[1]) +
-                                        cb * (double)fC[2])
-                              : 0.f;
-    const double st = rC >= 0 ? sqrt((double)fC[3]) : 0.0;
-    // whole waves skip groups without a data point
-    if (__ballot(rC >= 0) == 0) {
-    } else if constexpr (SL == 0) {
-      // the G samples of the group in LOS order, each with k_los_spectral's step for line q
-      static_for<G>([&](auto oc) {
-        constexpr int o = decltype(oc)::value;
-        const float v = group_bcast<G, o>(vls);
-        const double sto = group_bcast<G, o>(st);
-        float pk = 0.f;
-#pragma unroll
-        for (int l = 0; l < (G < NL ? G : NL); ++l) {
-          const float t = group_bcast<G, o>(fC[4 + l]);
-          if (l == q) pk = t;
-        }
-        const bool on = integ && pk != 0.f && kb + o < nz;
-        const double peak = pk, w = (double)(Kq > 0 ? (float)(Kq * sto) : 1.f);
-        const double mu = (double)v * l0c;
-        const double kap = fomo::spectral_kappa(w);
-        const double d0 = lamc - mu;
-        if constexpr (DIRECT) {
-          if (on) {
-#pragma unroll
-            for (int c = 0; c < CH; ++c) {
-              const double d = d0 + (double)c * dl;
-              acc[c] += peak * exp(-kap * d * d);
-            }
-          }
-          return;
-        }
-        if (!on) return;
-        double xa[3];
-        if (!profile_args(kap, d0, dl, xa)) {
-          bad = 1u;
-          return;
-        }
-        double ex[3], gc[4], RR[4], q16;
-        fomo::exp3(xa, ex);
-        profile_chains(ex, gc, RR, q16);
-#pragma unroll
-        for (int c0 = 0; c0 < CH; c0 += 4) {
-#pragma unroll
-          for (int jj = 0; jj < 4; ++jj) {
-            acc[c0 + jj] = fma(peak, gc[jj], acc[c0 + jj]);
-            gc[jj] *= RR[jj];
-            RR[jj] *= q16;
-          }
-        }
-      });
-    } else {
-      // the shared profile at this lane's own sample, for each lambda pass p: its anchors (ex, ok
-      // bit p), or for the direct pass its (kap, d0)
-      const double w = (double)(Kq > 0 ? (float)(Kq * st) : 1.f);
-      const double mu = (double)vls * l0c;
-      const double kapS = fomo::spectral_kappa(w);
-      double d0S[NP], exS[NP][3];
-      int okS = 0;
-#pragma unroll
-      for (int p = 0; p < NP; ++p) {
-        d0S[p] = lamc_of(p) - mu;
-        exS[p][0] = exS[p][1] = exS[p][2] = 1.0;
-        if constexpr (!DIRECT) {
-          double xa[3];
-          const bool ok = profile_args(kapS, d0S[p], dl, xa);
-          okS |= ok ? 1 << p : 0;
-          if (!ok) xa[0] = xa[1] = xa[2] = 0.0;  // finite chains; nothing is added from them
-          fomo::exp3(xa, exS[p]);
-        }
-      }
-      // the 4 samples of the group in LOS order; lane q runs chain q of each
-      static_for<4>([&](auto oc) {
-        constexpr int o = decltype(oc)::value;
-        float pk[SL];
-        unsigned on = 0;
-#pragma unroll
-        for (int l = 0; l < SL; ++l) {
-          pk[l] = group_bcast<4, o>(fC[4 + l]);
-          if (pk[l] != 0.f && l < ls.n) on |= 1u << l;
-        }
-        if (!integ || kb + o >= nz) on = 0;
-        if constexpr (DIRECT) {
-          const double kap = group_bcast<4, o>(kapS);
-#pragma unroll
-          for (int p = 0; p < NP; ++p) {
-            const double d0 = group_bcast<4, o>(d0S[p]);
-            unsigned use = 0;
-#pragma unroll
-            for (int l = 0; l < SL; ++l)
-              if ((on >> l) & (need >> (l * NP + p)) & 1u) use |= 1u << l;
-            if (use == 0) continue;
-#pragma unroll
-            for (int m = 0; m < CJ; ++m) {
-              const double d = d0 + (double)(4 * m + q) * dl;
-              const double e = exp(-kap * d * d);
-#pragma unroll
-              for (int l = 0; l < SL; ++l)
-                if (use & (1u << l)) acc[(p * SL + l) * CJ + m] += (double)pk[l] * e;
-            }
-          }
-          return;
-        }
-        const int okb = group_bcast<4, o>(okS);
-#pragma unroll
-        for (int p = 0; p < NP; ++p) {
-          if (!((okb >> p) & 1)) {
-#pragma unroll
-            for (int l = 0; l < SL; ++l)
-              if (on & (1u << l)) bad |= 1u << (l * NP + p);
-            continue;
-          }
-          if (on == 0) continue;
-          const double ex[3] = {group_bcast<4, o>(exS[p][0]), group_bcast<4, o>(exS[p][1]),
-                                group_bcast<4, o>(exS[p][2])};
-          double gc[4], RR[4], q16;
-          profile_chains(ex, gc, RR, q16);
-          double gq = gc[0], Rq = RR[0];
-#pragma unroll
-          for (int c = 1; c < 4; ++c)
-            if (q == c) gq = gc[c], Rq = RR[c];
-          // a line with peak 0 adds fma(0, g, sum) = sum: g is finite here
-#pragma unroll
-          for (int m = 0; m < CJ; ++m) {
-#pragma unroll
-            for (int l = 0; l < SL; ++l) acc[(p * SL + l) * CJ + m] = fma((double)pk[l], gq, acc[(p * SL + l) * CJ + m]);
-            gq *= Rq;
-            Rq *= q16;
-          }
-        }
-      });
-    }
-    // advance the pipeline
-    rC = rB;
-#pragma unroll
-    for (int c = 0; c < 4 * RS4; ++c) fC[c] = fB[c];
-    boxN = boxA;
-    rpN = rpA;
-#pragma unroll
-    for (int c = 0; c < kSlots; ++c) slN[c] = slA[c];
-    actN = actA;
-    cellN = cellA;
-  }
-  if (!integ) return;
-  const double sc = vg.scale;
-  if constexpr (SL > 0) {
-    if (!DIRECT && q == 0) {
-#pragma unroll
-      for (int l = 0; l < SL; ++l)
-        if (l < ls.n) flag[(size_t)l * R] = (int)((bad >> (l * NP)) & ((1u << NP) - 1));
-    }
-#pragma unroll
-    for (int p = 0; p < NP; ++p)
-#pragma unroll
-      for (int l = 0; l < SL; ++l) {
-        const unsigned bit = 1u << (l * NP + p);
-        if (l >= ls.n || (DIRECT ? !(need & bit) : (bad & bit) != 0)) continue;
-        float* o = ls.out[l] + (size_t)ray * nl + p * CH;
-#pragma unroll
-        for (int m = 0; m < CJ; ++m)
-          if (p * CH + 4 * m + q < nl) o[4 * m + q] = (float)(sc * acc[(p * SL + l) * CJ + m]);
-      }
-  } else {
-    if (!DIRECT) {
-      *flag = bad ? 1 : 0;
-      if (bad) return;
-    }
-    float* o = out + (size_t)ray * nl;
-#pragma unroll
-    for (int c = 0; c < CH; ++c)
-      if (c < nl) o[c] = (float)(sc * acc[c]);
-  }
 }
 
 // nl == 1 (Imaging): intens += peak (fomo-nearestneighbour.cpp:217-225)
@@ -3286,6 +2577,9 @@ int enqueue_tet(hipStream_t s, const Dataset& ds, const ViewGeom* geom, int nx, 
 int build_records(hipStream_t s, const Dataset& ds, const std::vector<const Goft*>& gcs, DevBuf& rec, int* rs4,
                   RecInfo* ri) {
   if (ds.nvars < 2) return fail(FOMO_ERR_STATE, "records need the temperature variable");
+  for (const Goft* g : gcs)
+    if ((g->K < 0) != (gcs[0]->K < 0))
+      return fail(FOMO_ERR_ARG, "emission cubes made from a given GoftCube cannot share a render with computed ones");
   *rs4 = rec_width((int)gcs.size());
   if (rec.ensure(sizeof(float4) * (*rs4) * ds.n)) return fail(FOMO_ERR_NOMEM, "device allocation failed (records)");
   LineSet ls{};
@@ -3344,46 +2638,21 @@ void enqueue_los(hipStream_t s, int R, int nz, const LineSet& ls, const float* v
 }
 
 // Fused NN + integration pass (k_nn_los) for spectral views with one lambda pass (2 <= nl <= 64)
-// and per-point records. Returns false when it does not apply (the caller runs enqueue_nn +
-// enqueue_los instead).
-template <int RS4, int G, int SL>
-void launch_nn_los_t(hipStream_t s, int ch, unsigned blocks, const Dataset& ds, const ViewGeom* geom, int nx, int ny,
-                     int nz, const float4* rec, const RecInfo& ri, const LineSet& ls, int nl, int32_t* flags) {
+// and per-point records (two passes for one line with 65 <= nl <= 128). The instantiations are
+// compiled in fomo_nnlos_part.hip; this picks the one for lambda chunk ch.
+void launch_nn_los(int variant, int ch, unsigned blocks, hipStream_t s, const Dataset& ds, const ViewGeom* geom,
+                   int nx, int ny, int nz, const float4* rec, const RecInfo& ri, const LineSet& ls, int nl,
+                   int32_t* flags) {
+  const NnLosArgs a{ds.pts.as<float4>(), ds.grid, ds.lslot.as<float4>(), ds.lcand.as<float4>(), ds.lk,
+                    ds.mpts.as<float4>(), ds.nlo.as<float4>(), ds.nhi.as<float4>(), ds.bvh, geom, nx, ny, nz, rec,
+                    ri, ls, nl, flags};
   switch (ch) {
-#define FOMO_NNL_CH(C)                                                                                        \
-  case C:                                                                                                    \
-    k_nn_los<RS4, C, G, false, SL><<<blocks, 256, 0, s>>>(                                                   \
-        ds.pts.as<float4>(), ds.grid, ds.lslot.as<float4>(), ds.lcand.as<float4>(), ds.lk, ds.mpts.as<float4>(), \
-        ds.nlo.as<float4>(), ds.nhi.as<float4>(), ds.bvh, geom, nx, ny, nz, rec, ri, ls, nl, flags);           \
-    k_nn_los<RS4, C, G, true, SL><<<blocks, 256, 0, s>>>(                                                    \
-        ds.pts.as<float4>(), ds.grid, ds.lslot.as<float4>(), ds.lcand.as<float4>(), ds.lk, ds.mpts.as<float4>(), \
-        ds.nlo.as<float4>(), ds.nhi.as<float4>(), ds.bvh, geom, nx, ny, nz, rec, ri, ls, nl, flags);           \
-    break;
+#define FOMO_NNL_CH(C) \
+  case C: nn_los_launch<C>(variant, blocks, s, a); break;
     FOMO_NNL_CH(4) FOMO_NNL_CH(8) FOMO_NNL_CH(12) FOMO_NNL_CH(16) FOMO_NNL_CH(20) FOMO_NNL_CH(24) FOMO_NNL_CH(28)
     FOMO_NNL_CH(32) FOMO_NNL_CH(36) FOMO_NNL_CH(40) FOMO_NNL_CH(44) FOMO_NNL_CH(48) FOMO_NNL_CH(52)
     FOMO_NNL_CH(56) FOMO_NNL_CH(60) FOMO_NNL_CH(64)
 #undef FOMO_NNL_CH
-    default: break;
-  }
-}
-
-// One line in two lambda passes (65 <= nl <= 128), both in one launch: CH = k_los_spectral's chunk.
-void launch_nn_los_2pass(hipStream_t s, int ch, unsigned blocks, const Dataset& ds, const ViewGeom* geom, int nx,
-                         int ny, int nz, const float4* rec, const RecInfo& ri, const LineSet& ls, int nl,
-                         int32_t* flags) {
-  switch (ch) {
-#define FOMO_NNL2_CH(C)                                                                                       \
-  case C:                                                                                                    \
-    k_nn_los<2, C, 4, false, 1, 2><<<blocks, 256, 0, s>>>(                                                   \
-        ds.pts.as<float4>(), ds.grid, ds.lslot.as<float4>(), ds.lcand.as<float4>(), ds.lk, ds.mpts.as<float4>(), \
-        ds.nlo.as<float4>(), ds.nhi.as<float4>(), ds.bvh, geom, nx, ny, nz, rec, ri, ls, nl, flags);           \
-    k_nn_los<2, C, 4, true, 1, 2><<<blocks, 256, 0, s>>>(                                                    \
-        ds.pts.as<float4>(), ds.grid, ds.lslot.as<float4>(), ds.lcand.as<float4>(), ds.lk, ds.mpts.as<float4>(), \
-        ds.nlo.as<float4>(), ds.nhi.as<float4>(), ds.bvh, geom, nx, ny, nz, rec, ri, ls, nl, flags);           \
-    break;
-    FOMO_NNL2_CH(36) FOMO_NNL2_CH(40) FOMO_NNL2_CH(44) FOMO_NNL2_CH(48) FOMO_NNL2_CH(52) FOMO_NNL2_CH(56)
-    FOMO_NNL2_CH(60) FOMO_NNL2_CH(64)
-#undef FOMO_NNL2_CH
     default: break;
   }
 }
@@ -3423,8 +2692,8 @@ bool enqueue_nn_los(hipStream_t s, const Dataset& ds, const ViewGeom* geom, int 
     if (!fused_single_line(rec, rs4, R, nl)) return false;
     ProfScope ps("k_nn_los", s);
     const unsigned blocks = (unsigned)((R + 63) / 64);
-    if (npass == 1) launch_nn_los_t<2, 4, 1>(s, ch, blocks, ds, geom, nx, ny, nz, rec, ri, ls, nl, flags);
-    else launch_nn_los_2pass(s, ch, blocks, ds, geom, nx, ny, nz, rec, ri, ls, nl, flags);
+    launch_nn_los(npass == 1 ? kNnLosSingle : kNnLosSingle2, ch, blocks, s, ds, geom, nx, ny, nz, rec, ri, ls, nl,
+                  flags);
     return true;
   }
   if (npass != 1) return false;
@@ -3435,10 +2704,8 @@ bool enqueue_nn_los(hipStream_t s, const Dataset& ds, const ViewGeom* geom, int 
   const int64_t rays_per_block = 4 * (64 / G);
   const unsigned blocks = (unsigned)((R + rays_per_block - 1) / rays_per_block);
   ProfScope ps("k_nn_los", s);
-  if (shared) launch_nn_los_t<2, 4, 4>(s, ch, blocks, ds, geom, nx, ny, nz, rec, ri, ls, nl, flags);
-  else if (rs4 == 2 && G == 2) launch_nn_los_t<2, 2, 0>(s, ch, blocks, ds, geom, nx, ny, nz, rec, ri, ls, nl, flags);
-  else if (rs4 == 2) launch_nn_los_t<2, 4, 0>(s, ch, blocks, ds, geom, nx, ny, nz, rec, ri, ls, nl, flags);
-  else launch_nn_los_t<4, 8, 0>(s, ch, blocks, ds, geom, nx, ny, nz, rec, ri, ls, nl, flags);
+  const int variant = shared ? kNnLosShared : (rs4 == 2 && G == 2) ? kNnLosLines2 : rs4 == 2 ? kNnLosLines4 : kNnLosLines8;
+  launch_nn_los(variant, ch, blocks, s, ds, geom, nx, ny, nz, rec, ri, ls, nl, flags);
   return true;
 }
 
@@ -4273,6 +3540,33 @@ int fomo_emission_chianti(int dataset, int table, double lambda0, double atweigh
   return FOMO_OK;
 }
 
+int fomo_emission_given(int dataset, int kind, double lambda0, int* out_goft) {
+  std::lock_guard<std::recursive_mutex> lk(g_mu);
+  if (int rc = require_init()) return rc;
+  Dataset* ds = find_obj(g_ctx.datasets, dataset);
+  if (!ds) return fail(FOMO_ERR_NOTFOUND, "unknown dataset id");
+  if (kind != 0 && kind != 1) return fail(FOMO_ERR_ARG, "kind must be 0 (line) or 1 (Thomson)");
+  if (ds->nvars < 2) return fail(FOMO_ERR_ARG, "a GoftCube holds at least two variables");
+  if (!out_goft) return fail(FOMO_ERR_ARG, "null output id");
+  std::unique_ptr<Goft> gc(new Goft());
+  gc->dataset = dataset;
+  gc->kind = kind;
+  gc->lambda0 = kind == 0 ? lambda0 : 0.0;
+  if (gc->em.alloc(sizeof(float2) * ds->n)) return fail(FOMO_ERR_NOMEM, "device allocation failed");
+  k_pack_em<<<nblk(ds->n), kBlock, 0, g_ctx.stream>>>(ds->svars[0]->as<float>(), ds->svars[1]->as<float>(), ds->n,
+                                                       gc->em.as<float2>());
+  HIPOK(hipGetLastError());
+  if (kind == 0) {
+    gc->K = -1.0;  // the records' T slot (variable 1) holds the fwhm
+    if (int rc = build_records(g_ctx.stream, *ds, {gc.get()}, gc->rec, &gc->rs4, &gc->ri)) return rc;
+  }
+  if (int rc = sync_stream()) return rc;
+  int id = g_ctx.next_id++;
+  g_ctx.gofts[id] = std::move(gc);
+  *out_goft = id;
+  return FOMO_OK;
+}
+
 int fomo_emission_thomson(int dataset, int* out_goft) {
   std::lock_guard<std::recursive_mutex> lk(g_mu);
   if (int rc = require_init()) return rc;
@@ -4992,6 +4286,20 @@ int fomo_server_wait(int server, int64_t ticket, double timeout_s, int* state) {
   if (state) *state = t.state;
   if (t.state < 0) return fail(t.state, t.err);
   if (t.state != kTicketDone) return fail(FOMO_ERR_STATE, "timed out");
+  return FOMO_OK;
+}
+
+int fomo_server_cancel(int server, int64_t ticket) {
+  auto sv = find_server(server);
+  if (!sv) return fail(FOMO_ERR_NOTFOUND, "unknown server id");
+  std::lock_guard<std::mutex> lk(sv->mu);
+  if (ticket < 0 || ticket >= (int64_t)sv->tickets.size()) return fail(FOMO_ERR_NOTFOUND, "unknown ticket");
+  Ticket& t = sv->tickets[ticket];
+  if (t.state != kTicketQueued) return fail(FOMO_ERR_STATE, "the request is no longer queued");
+  sv->pending.erase(std::find(sv->pending.begin(), sv->pending.end(), ticket));
+  server_fail_ticket(*sv, t, FOMO_ERR_CANCELLED, "cancelled before it was admitted");
+  sv->cv_done.notify_all();
+  sv->cv_work.notify_one();  // a request it held back may fit now
   return FOMO_OK;
 }
 

@@ -81,12 +81,15 @@ int fomo_table_read(const char* path, char* ion, int ion_len, double* lambda0, d
     if (!(in >> f)) return FOMO_ERR_IO;
     tv[i] = (float)f;
   }
+  // rows as the reference reads them (fomo-CHIANTI.cpp:178-189): a row cut short at the end of
+  // the file keeps its missing values as 0 (a failed extraction stores 0); only a row count that
+  // does not match the header is an error (:192-197)
   double r;
   while (in >> r) {
     rows.push_back((float)r);
     for (int i = 0; i < nt; ++i) {
-      double f;
-      if (!(in >> f)) return FOMO_ERR_IO;
+      double f = 0;
+      in >> f;
       vals.push_back((float)f);
     }
   }

@@ -1,0 +1,64 @@
+// fomo_nnlos_part.hip — instantiations of the fused NN + integration kernel k_nn_los for one
+// group of lambda chunk sizes CH. The build compiles this file once per group
+// (-DFOMO_NNLOS_GROUP=0..7, two CH values each, one of them at least 36 for the two-pass variant)
+// so that the 176 instantiations compile in parallel.
+#include "fomo_common.h"
+
+#ifndef FOMO_NNLOS_GROUP
+#error "compile with -DFOMO_NNLOS_GROUP=<0..7>"
+#endif
+
+namespace fomo_dev {
+
+namespace {
+
+template <int RS4, int CH, int G, int SL, int NP>
+void launch_pair(unsigned blocks, hipStream_t s, const NnLosArgs& a) {
+  k_nn_los<RS4, CH, G, false, SL, NP><<<blocks, 256, 0, s>>>(a.pts, a.g, a.lslot, a.lcand, a.lk, a.mpts, a.nlo, a.nhi,
+                                                             a.bvh, a.geom, a.nx, a.ny, a.nz, a.rec, a.ri, a.ls, a.nl,
+                                                             a.flags);
+  k_nn_los<RS4, CH, G, true, SL, NP><<<blocks, 256, 0, s>>>(a.pts, a.g, a.lslot, a.lcand, a.lk, a.mpts, a.nlo, a.nhi,
+                                                            a.bvh, a.geom, a.nx, a.ny, a.nz, a.rec, a.ri, a.ls, a.nl,
+                                                            a.flags);
+}
+
+}  // namespace
+
+template <int CH>
+void nn_los_launch(int variant, unsigned blocks, hipStream_t s, const NnLosArgs& a) {
+  switch (variant) {
+    case kNnLosSingle: launch_pair<2, CH, 4, 1, 1>(blocks, s, a); break;
+    case kNnLosShared: launch_pair<2, CH, 4, 4, 1>(blocks, s, a); break;
+    case kNnLosLines2: launch_pair<2, CH, 2, 0, 1>(blocks, s, a); break;
+    case kNnLosLines4: launch_pair<2, CH, 4, 0, 1>(blocks, s, a); break;
+    case kNnLosLines8: launch_pair<4, CH, 8, 0, 1>(blocks, s, a); break;
+    case kNnLosSingle2:
+      if constexpr (CH >= 36) launch_pair<2, CH, 4, 1, 2>(blocks, s, a);
+      break;
+    default: break;
+  }
+}
+
+#define FOMO_NNLOS_INST(C) template void nn_los_launch<C>(int, unsigned, hipStream_t, const NnLosArgs&);
+#if FOMO_NNLOS_GROUP == 0
+FOMO_NNLOS_INST(4) FOMO_NNLOS_INST(64)
+#elif FOMO_NNLOS_GROUP == 1
+FOMO_NNLOS_INST(8) FOMO_NNLOS_INST(60)
+#elif FOMO_NNLOS_GROUP == 2
+FOMO_NNLOS_INST(12) FOMO_NNLOS_INST(56)
+#elif FOMO_NNLOS_GROUP == 3
+FOMO_NNLOS_INST(16) FOMO_NNLOS_INST(52)
+#elif FOMO_NNLOS_GROUP == 4
+FOMO_NNLOS_INST(20) FOMO_NNLOS_INST(48)
+#elif FOMO_NNLOS_GROUP == 5
+FOMO_NNLOS_INST(24) FOMO_NNLOS_INST(44)
+#elif FOMO_NNLOS_GROUP == 6
+FOMO_NNLOS_INST(28) FOMO_NNLOS_INST(40)
+#elif FOMO_NNLOS_GROUP == 7
+FOMO_NNLOS_INST(32) FOMO_NNLOS_INST(36)
+#else
+#error "FOMO_NNLOS_GROUP must be 0..7"
+#endif
+#undef FOMO_NNLOS_INST
+
+}  // namespace fomo_dev

@@ -117,11 +117,20 @@ long file_mtime(const std::string& p) {
   return (long)st.st_mtime;
 }
 
-const TableEntry& load_table(const std::string& path) {
+// The G(T) table at path, parsed and uploaded once (again when the file changes). An imaging
+// render needs an instrument table, whose ion field is the channel (fomo-CHIANTI.cpp:278-282);
+// every failure here ends the program before the GPU is initialised.
+const TableEntry& load_table(const std::string& path, bool imaging) {
+  const auto check_ion = [&](const std::string& ion, double l0) {
+    if (imaging && std::atoi(ion.c_str()) != int(l0 + 0.5)) die("GOFT table is not correct!");
+  };
   long mt = file_mtime(path);
   if (mt < 0) die("Error: no CHIANTI G(T) file exists: " + path);
   auto it = g_tables.find(path);
-  if (it != g_tables.end() && it->second.mtime == mt) return it->second;
+  if (it != g_tables.end() && it->second.mtime == mt) {
+    check_ion(it->second.ion, it->second.lambda0);
+    return it->second;
+  }
   char ion[256], unit[1024];
   double l0 = 0, aw = 0;
   int nt = 0, nr = 0;
@@ -132,6 +141,7 @@ const TableEntry& load_table(const std::string& path) {
   if (fomo_table_read(path.c_str(), ion, sizeof ion, &l0, &aw, unit, sizeof unit, &nt, &nr, lt.data(), ln.data(),
                       g.data()) != FOMO_OK)
     die("Error: CHIANTI table not read in properly!");
+  check_ion(ion, l0);
   engine();
   TableEntry e;
   if (it != g_tables.end() && it->second.id >= 0) fomo_table_destroy(it->second.id);
@@ -533,6 +543,12 @@ void FoMo::FoMoObject::render(const std::vector<double> lvec, const std::vector<
     die("Error: unknown rendering method.");
   }
   const std::bitset<noptions> woptions = goftcube.getwriteoptions();
+  // the G(T) table first, as emissionfromdatacube reads it before anything else
+  // (fomo-CHIANTI.cpp:244): a missing or malformed table, or an imaging table of another
+  // instrument (:278-282), ends the program before the GPU is used
+  const TableEntry* te = nullptr;
+  const bool imaging = rendering.readobservationtype() == Imaging;
+  if (!thomson) te = &load_table(rendering.readchiantifile(), imaging);
   FoMoDeviceState& st = own_state(device);
   if (!st.dataset) st.dataset = upload_dataset(datacube);
   const int ds = st.dataset->id;
@@ -551,25 +567,23 @@ void FoMo::FoMoObject::render(const std::vector<double> lvec, const std::vector<
     meta.setlambda0(0.);
   } else {
     const std::string cfile = rendering.readchiantifile();
-    const TableEntry& te = load_table(cfile);
-    const bool imaging = rendering.readobservationtype() == Imaging;
-    if (imaging && std::atoi(te.ion.c_str()) != int(te.lambda0 + 0.5)) die("GOFT table is not correct!");
+    const TableEntry& tab = *te;
     double abundratio = 1.0;
     const std::string afile = rendering.readabundfile();
     if (!imaging && afile != std::string("/empty"))
-      abundratio = abund_from_file(afile, te.ion) / abund_from_file(default_abund_path(cfile), te.ion);
+      abundratio = abund_from_file(afile, tab.ion) / abund_from_file(default_abund_path(cfile), tab.ion);
     if (datacube.readnvars() < 2) die("Error: the datacube needs at least n and T.");
-    if (fomo_emission_chianti(ds, te.id, te.lambda0, te.atweight, imaging ? 1 : 0, abundratio, &em->id) != FOMO_OK)
+    if (fomo_emission_chianti(ds, tab.id, tab.lambda0, tab.atweight, imaging ? 1 : 0, abundratio, &em->id) != FOMO_OK)
       die(std::string("Error: ") + fomo_last_error());
     std::vector<std::string> dunits = datacube.readunit();
     for (int i = 0; i < dim; ++i) units.push_back(i < (int)dunits.size() ? dunits[i] : "");
-    units.push_back(te.unit);
+    units.push_back(tab.unit);
     units.push_back("\\AA{}");
     for (int i = 2; i < datacube.readnvars(); ++i) units.push_back(dim + i < (int)dunits.size() ? dunits[dim + i] : "");
     meta.setchiantifile(cfile);
     meta.setabundfile(afile);
-    meta.setlambda0(te.lambda0);
-    emunit = te.unit;
+    meta.setlambda0(tab.lambda0);
+    emunit = tab.unit;
   }
   meta.setwriteoptions(woptions);
   st.emission = em;

@@ -27,6 +27,7 @@ extern "C" {
 #define FOMO_ERR_NOMEM (-4)    /* device allocation failed or over budget */
 #define FOMO_ERR_STATE (-5)    /* not initialised / wrong object kind */
 #define FOMO_ERR_IO (-6)       /* file missing or malformed */
+#define FOMO_ERR_CANCELLED (-7) /* server request cancelled before it ran */
 
 /* Message for the last failing call on this thread ("" if none). */
 const char* fomo_last_error(void);
@@ -91,6 +92,13 @@ int fomo_table_destroy(int id);
 int fomo_emission_chianti(int dataset, int table, double lambda0, double atweight, int imaging,
                           double abundratio, int* out_goft);
 int fomo_emission_thomson(int dataset, int* out_goft);
+/* Emission cube of a dataset that holds a GoftCube already computed on the host: variables 0 and
+ * 1 are (peak, fwhm) of a line at lambda0 (kind 0; variables 2..4 the velocity), or (n tot,
+ * n pol) for Thomson scattering (kind 1). This is the input of the reference's RenderWith*
+ * functions (FoMo-internal.h:14-30, e.g. RenderWithNearestNeighbour, fomo-nearestneighbour.cpp:
+ * 284-306); the renders use the given values as emissionfromdatacube / thomsonfromdatacube would
+ * have produced them. A cube of this kind cannot share a batch with cubes computed from T. */
+int fomo_emission_given(int dataset, int kind, double lambda0, int* out_goft);
 int fomo_goft_destroy(int id);
 /* Copy the two per-point emission variables back, in the dataset's original point order. */
 int fomo_goft_read(int goft, float* var0, float* var1);
@@ -233,6 +241,10 @@ int fomo_server_poll(int server, int64_t ticket, int* state);
  * the ticket's own error code when it failed. */
 int fomo_server_wait(int server, int64_t ticket, double timeout_s, int* state);
 int fomo_server_wait_all(int server, double timeout_s);
+/* Cancel a queued request: it leaves the queue and its ticket fails with FOMO_ERR_CANCELLED; its
+ * out and meta are not written. A request already admitted (running) or finished cannot be
+ * cancelled (FOMO_ERR_STATE). Cancelled requests count as failed in fomo_server_stats. */
+int fomo_server_cancel(int server, int64_t ticket);
 /* Seconds since the server was created: submitted, admitted (-1 if queued), finished (-1). */
 int fomo_server_times(int server, int64_t ticket, double* times3);
 /* 12 doubles: submitted, completed, failed, queued, running, peak running, device bytes in use,

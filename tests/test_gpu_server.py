@@ -153,3 +153,35 @@ def test_objects_in_use_cannot_be_destroyed(engine, fe9_table):
     finally:
         sv.close()
         T.close(), ds.close()
+
+
+def test_cancel_queued_requests(scene):
+    """Queued requests can be cancelled: they never run, their tickets fail with
+    FOMO_ERR_CANCELLED and their output buffers stay untouched; requests already admitted or
+    finished cannot be cancelled, and the others still equal the synchronous render."""
+    w = 0.2 * C / scene["tab"]["lambda0"]
+    req = (scene["em"], F.METHOD_NN, 0.7, 1.1, 512, 512, 128, 30)
+    ref, _ = F.render_nn(scene["em"], 0.7, 1.1, 512, 512, 128, 30, w)
+    sv = F.Server(device_bytes=2 << 30, host_bytes=1 << 30, max_inflight=1, nstreams=1)
+    try:
+        outs = [np.full(512 * 512 * 30, -7.0, np.float32) for _ in range(24)]
+        tks = [sv.submit(*req, lambda_width=w, out=o) for o in outs]
+        cancelled = [sv.cancel(t) for t in tks[12:]]
+        # one request runs at a time, each for milliseconds: the last twelve are still queued
+        assert all(cancelled), cancelled
+        assert sv.cancel(tks[12]) is False  # already cancelled
+        for t in tks[12:]:
+            with pytest.raises(F.FomoError) as ei:
+                sv.result(t, 60)
+            assert ei.value.code == -7
+            assert sv.poll(t) == -7
+        for t, o in zip(tks[:12], outs[:12]):
+            img, _ = sv.result(t, 60)
+            assert np.array_equal(img, ref)
+        assert sv.cancel(tks[0]) is False  # finished
+        for o in outs[12:]:
+            assert np.all(o == -7.0)
+        st = sv.stats()
+        assert st["completed"] == 12 and st["failed"] == 12 and st["queued"] == 0
+    finally:
+        sv.close()
