@@ -143,10 +143,13 @@ def main():
     ms_step = dt / args.steps * 1e3
     log(f"batched: {ms_step:.2f} ms/step, {value:.3e} W/s")
 
-    # ---- device copy probe (plain float4 copy of 1 GiB: 1 GiB read + 1 GiB written) --------------
+    # ---- HBM probes: a streaming copy of 1 GiB (1 GiB read + 1 GiB written) and 16 M random
+    #      128-byte lines of a 4 GiB buffer (the NN passes' access pattern) ------------------------
     copy_ms = F.copy_probe(1 << 30, 10)
     copy_gbps = 2 * (1 << 30) / (copy_ms * 1e-3) / 1e9
-    log(f"copy probe: {copy_ms:.3f} ms per 1 GiB copy, {copy_gbps:.0f} GB/s read+write")
+    gather_ms = F.gather_probe(4 << 30, 16 << 20, 10)
+    gather_gbps = (16 << 20) * 128 / (gather_ms * 1e-3) / 1e9
+    log(f"HBM probes: copy {copy_gbps:.0f} GB/s read+write, random 128-byte lines {gather_gbps:.0f} GB/s")
 
     # ---- per-kernel timing: HIP events around each launch of an eager pass of the same work ----
     F.profile_reset()
@@ -187,7 +190,9 @@ def main():
                     "avg_launch_ms": avg_s * 1e3, "algorithmic_bytes_per_launch": alg,
                     "measured": {"GBps": traffic / avg_s / 1e9 if traffic else None,
                                  "frac_of_peak": traffic / avg_s / PEAK_HBM if traffic else None,
-                                 "frac_of_copy_probe": traffic / avg_s / 1e9 / copy_gbps if traffic else None},
+                                 "frac_of_copy_probe": traffic / avg_s / 1e9 / copy_gbps if traffic else None,
+                                 "frac_of_random_line_probe": traffic / avg_s / 1e9 / gather_gbps if traffic
+                                 else None},
                     "fp64": {"algorithmic_flops_per_launch": flops, "flop_per_W": FLOP_PER_W,
                              "achieved_TFLOPs": flops / avg_s / 1e12, "peak_TFLOPs": PEAK_FP64 / 1e12,
                              "frac": flops / avg_s / PEAK_FP64},
@@ -324,6 +329,7 @@ def main():
         "batched_with_copy_vs_plain": with_copy["value"] / plain_value,
         "index_build_ms": t_index * 1e3,
         "copy_probe_GBps": copy_gbps,
+        "random_line_probe_GBps": gather_gbps,
         "lookup": linfo,
         "autotune": tune,
         "self_check": check,

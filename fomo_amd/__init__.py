@@ -36,9 +36,14 @@ _SIGS = {
     "fomo_dataset_create": (ctypes.c_int, [ctypes.c_int64, ctypes.c_int, _c_flt_p, _c_flt_p, _c_flt_p, ctypes.c_int,
                                            ctypes.POINTER(_c_flt_p), ctypes.c_double, _c_int_p]),
     "fomo_dataset_destroy": (ctypes.c_int, [ctypes.c_int]),
+    "fomo_dataset_begin": (ctypes.c_int, [ctypes.c_int64, ctypes.c_int, ctypes.c_int, _c_int_p]),
+    "fomo_dataset_append": (ctypes.c_int, [ctypes.c_int, ctypes.c_int64, ctypes.c_int64, _c_flt_p, _c_flt_p, _c_flt_p,
+                                           ctypes.POINTER(_c_flt_p)]),
+    "fomo_dataset_commit": (ctypes.c_int, [ctypes.c_int, ctypes.c_double]),
     "fomo_dataset_info": (ctypes.c_int, [ctypes.c_int, _c_i64_p, _c_int_p, _c_int_p]),
     "fomo_dataset_reindex": (ctypes.c_int, [ctypes.c_int, ctypes.c_double]),
     "fomo_dataset_set_lookup": (ctypes.c_int, [ctypes.c_int, ctypes.c_int]),
+    "fomo_dataset_set_ray_tile": (ctypes.c_int, [ctypes.c_int, ctypes.c_int]),
     "fomo_dataset_lookup_info": (ctypes.c_int, [ctypes.c_int, _c_int_p, _c_i64_p, _c_i64_p, _c_i64_p]),
     "fomo_dataset_lookup_groups": (ctypes.c_int, [ctypes.c_int, _c_i64_p]),
     "fomo_table_read": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int, _c_dbl_p, _c_dbl_p,
@@ -103,6 +108,7 @@ _SIGS = {
     "fomo_profile_reset": (ctypes.c_int, []),
     "fomo_profile_stats": (ctypes.c_int, [ctypes.c_char_p, _c_i64_p, _c_dbl_p]),
     "fomo_copy_probe": (ctypes.c_int, [ctypes.c_int64, ctypes.c_int, _c_dbl_p]),
+    "fomo_gather_probe": (ctypes.c_int, [ctypes.c_int64, ctypes.c_int64, ctypes.c_int, _c_dbl_p]),
     "fomo_gen_loop_cube": (ctypes.c_int, [ctypes.c_int, ctypes.c_uint64, ctypes.c_double, ctypes.c_double,
                                           ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double,
                                           _c_flt_p, _c_flt_p, _c_flt_p, _c_flt_p, _c_flt_p, _c_flt_p, _c_flt_p,
@@ -174,9 +180,16 @@ def profile_reset():
 
 
 def copy_probe(nbytes, reps=10):
-    """Mean ms of a plain device copy of nbytes (reads nbytes, writes nbytes)."""
+    """Mean ms of a streaming device copy of nbytes (reads nbytes, writes nbytes)."""
     ms = ctypes.c_double()
     _check(lib().fomo_copy_probe(int(nbytes), reps, ctypes.byref(ms)))
+    return ms.value
+
+
+def gather_probe(nbytes, lines, reps=10):
+    """Mean ms of reading `lines` random 128-byte lines of an nbytes buffer."""
+    ms = ctypes.c_double()
+    _check(lib().fomo_gather_probe(int(nbytes), int(lines), reps, ctypes.byref(ms)))
     return ms.value
 
 
@@ -214,12 +227,34 @@ class Dataset:
         self.n = n
         self._keep = None
 
+    @classmethod
+    def streamed(cls, n, chunks, dim=3, nvars=0, cell_scale=0.0):
+        """A dataset filled by the streaming ingest: chunks yields (offset, x, y, z, [vars]) with
+        float32 arrays of one length; every point must be covered once."""
+        out = ctypes.c_int()
+        _check(lib().fomo_dataset_begin(int(n), dim, nvars, ctypes.byref(out)))
+        self = cls.__new__(cls)
+        self.id, self.n, self._keep = out.value, int(n), None
+        for off, x, y, z, variables in chunks:
+            xs, xp = _f32(x)
+            ys, yp = _f32(y)
+            zs, zp = _f32(z) if dim == 3 else (None, None)
+            vs = [_f32(v) for v in variables]
+            arr = (_c_flt_p * max(1, len(vs)))(*[v[1] for v in vs])
+            _check(lib().fomo_dataset_append(self.id, int(off), xs.size, xp, yp, zp, arr))
+        _check(lib().fomo_dataset_commit(self.id, cell_scale))
+        return self
+
     def info(self):
         n = ctypes.c_int64()
         dims = (ctypes.c_int * 3)()
         mx = ctypes.c_int()
         _check(lib().fomo_dataset_info(self.id, ctypes.byref(n), dims, ctypes.byref(mx)))
         return n.value, tuple(dims), mx.value
+
+    def set_ray_tile(self, width):
+        """Ray tile width (8, 16, 32 or 64 rays) of the NN passes' blocks. Results do not change."""
+        _check(lib().fomo_dataset_set_ray_tile(self.id, width))
 
     def set_lookup(self, subdiv):
         """Candidate lookup grid subdivision (0 = ring search only). Results do not change."""
@@ -364,14 +399,15 @@ def render_thomson(em, l, b, nx, ny, nz):
 
 
 def autotune(ems, l, b, nx, ny, nz, reps=3):
-    """Pick the fastest index configuration (cell scale x lookup subdivision) for views like this one
-    of the given emission cube(s) (a batch's lines); the dataset keeps it. Returns
-    dict(cell_scale, subdiv, ms, ms_before)."""
+    """Pick the fastest index configuration (cell scale x lookup subdivision x ray tile) for views
+    like this one of the given emission cube(s) (a batch's lines); the dataset keeps it. Returns
+    dict(cell_scale, subdiv, ms, ms_before, tile)."""
     ems = ems if isinstance(ems, (list, tuple)) else [ems]
     ids = (ctypes.c_int * len(ems))(*[e.id for e in ems])
-    r = np.zeros(4, np.float64)
+    r = np.zeros(5, np.float64)
     _check(lib().fomo_autotune(len(ems), ids, l, b, nx, ny, nz, reps, _ptr(r, ctypes.c_double)))
-    return {"cell_scale": float(r[0]), "subdiv": int(r[1]), "ms": float(r[2]), "ms_before": float(r[3])}
+    return {"cell_scale": float(r[0]), "subdiv": int(r[1]), "ms": float(r[2]), "ms_before": float(r[3]),
+            "tile": int(r[4])}
 
 
 def render_tet(em, l, b, nx, ny, nz, nl, lambda_width, dn=False, linear=False):

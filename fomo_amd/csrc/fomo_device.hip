@@ -779,7 +779,7 @@ __device__ __forceinline__ void write_sample(int r, const float4* __restrict__ r
     }
 }
 
-// One thread per LOS sample. Block = 16x16 rays at one k (blockIdx.y = k).
+// One thread per LOS sample. Block = one ray tile at one k (blockIdx.y = k).
 // RS4 > 0: per-point records of RS4 float4 {vx,vy,vz, peak_0,fwhm_0, ..., peak_{L-1},fwhm_{L-1}}
 // are gathered once for the chosen point, and the kernel writes coalesced per-sample arrays:
 // svlos[s] (LOS velocity, fomo-nearestneighbour.cpp:87-89) and sem[l*S + s] = (peak_l, fwhm_l),
@@ -829,17 +829,25 @@ __device__ __forceinline__ void nn_sample(const float4* __restrict__ pts, const 
 }
 
 
-// One thread per LOS sample: block = 16x16 rays of one LOS plane (blockIdx.y = k).
+// Ray tile of a k_nn / k_nn_far block: 256 rays of one LOS plane, 2^tlog wide and 256 / 2^tlog
+// high (tlog 3..6: 8x32 .. 64x4), so each wave of 64 lanes covers 2^tlog x 64 / 2^tlog rays.
+__device__ __forceinline__ void ray_of_thread(int nx, int tlog, int* i, int* j) {
+  const int tw = 1 << tlog, th = 256 >> tlog;
+  const int tilesx = (nx + tw - 1) >> tlog;
+  *j = (int)(blockIdx.x % tilesx) * tw + (int)(threadIdx.x & (tw - 1));
+  *i = (int)(blockIdx.x / tilesx) * th + (int)(threadIdx.x >> tlog);
+}
+
+// One thread per LOS sample: block = one ray tile of one LOS plane (blockIdx.y = k).
 template <int RS4>
 __global__ void __launch_bounds__(256) k_nn(const float4* __restrict__ pts, GridInfo g,
                                             const float4* __restrict__ lslot, const float4* __restrict__ lcand,
                                             LookupInfo lk, const ViewGeom* __restrict__ vgp, int nx, int ny, int nz,
                                             int32_t* __restrict__ idx, const float4* __restrict__ rec, RecInfo ri,
                                             int nlines, float* __restrict__ svlos, float2* __restrict__ sem,
-                                            uint8_t* __restrict__ rayflag) {
-  const int tilesx = (nx + 15) >> 4;
-  const int j = (blockIdx.x % tilesx) * 16 + (threadIdx.x & 15);
-  const int i = (blockIdx.x / tilesx) * 16 + (threadIdx.x >> 4);
+                                            uint8_t* __restrict__ rayflag, int tlog) {
+  int i, j;
+  ray_of_thread(nx, tlog, &i, &j);
   if (j >= nx || i >= ny) return;
   nn_sample<RS4>(pts, g, lslot, lcand, lk, *vgp, nx, ny, nz, i, j, (int)blockIdx.y, idx, rec, ri, nlines, svlos,
                  sem, rayflag);
@@ -855,11 +863,9 @@ __global__ void __launch_bounds__(256) k_nn_far(const float4* __restrict__ pts, 
                                                 BvhInfo bv, const ViewGeom* __restrict__ vgp, int nx, int ny, int nz,
                                                 int32_t* __restrict__ idx, const float4* __restrict__ rec, RecInfo ri,
                                                 int nlines, float* __restrict__ svlos, float2* __restrict__ sem,
-                                                const uint8_t* __restrict__ rayflag) {
-  const int tilesx = (nx + 15) >> 4;
-  const int tx = blockIdx.x % tilesx, ty = blockIdx.x / tilesx;
-  const int j = tx * 16 + (threadIdx.x & 15);
-  const int i = ty * 16 + (threadIdx.x >> 4);
+                                                const uint8_t* __restrict__ rayflag, int tlog) {
+  int i, j;
+  ray_of_thread(nx, tlog, &i, &j);
   const int k = blockIdx.y;
   const size_t R = (size_t)nx * ny;
   const bool inside = j < nx && i < ny;
@@ -1691,6 +1697,7 @@ struct Dataset {
   // far and outer groups of the lookup grid: groups, groups with a candidate list, far groups
   // whose cells all take the list (group headers)
   int64_t far_groups = 0, far_lists = 0, full_groups = 0, outer_groups = 0, outer_lists = 0;
+  int tile_log2 = 4;  // ray tile of k_nn / k_nn_far blocks: 2^tile_log2 x 256 / 2^tile_log2 (autotuned)
   // tetra path: Delaunay triangulation of the points, built on the host on first use
   int64_t ntets = 0;      // 0: not built
   DevBuf trec;            // 4 float4 per tet: vertex {x, y, z, sorted position as int bits (-1 = infinite)}
@@ -1767,6 +1774,13 @@ struct Ctx {
   hipDeviceProp_t prop{};
   int next_id = 1;
   std::map<int, std::unique_ptr<Dataset>> datasets;
+  // datasets being filled by fomo_dataset_append: the object and the point ranges written so far
+  struct Ingest {
+    std::unique_ptr<Dataset> ds;
+    int dim = 3;
+    std::vector<std::pair<int64_t, int64_t>> written;  // (offset, count)
+  };
+  std::map<int, Ingest> ingests;
   std::map<int, std::unique_ptr<Table>> tables;
   std::map<int, std::unique_ptr<Goft>> gofts;
   std::map<int, std::unique_ptr<Batch>> batches;
@@ -2512,14 +2526,15 @@ int rec_width(int nlines) {
 void enqueue_nn(hipStream_t s, const Dataset& ds, const ViewGeom* geom, int nx, int ny, int nz, int32_t* idx,
                 uint8_t* rayflag, const float4* rec = nullptr, int rs4 = 0, int nlines = 0, float* svlos = nullptr,
                 float2* sem = nullptr, const RecInfo& ri = RecInfo{}) {
-  const int tiles = ((nx + 15) / 16) * ((ny + 15) / 16);
+  const int tlog = ds.tile_log2, tw = 1 << tlog, th = 256 >> tlog;
+  const int tiles = ((nx + tw - 1) / tw) * ((ny + th - 1) / th);
   dim3 grid(tiles, nz);
   const int64_t R = (int64_t)nx * ny;
   {
     ProfScope ps("k_nn", s);
 #define FOMO_NN_ARGS                                                                                         \
   ds.pts.as<float4>(), ds.grid, ds.lslot.as<float4>(), ds.lcand.as<float4>(), ds.lk, geom, nx, ny, nz, idx, rec, \
-      ri, nlines, svlos, sem, rayflag
+      ri, nlines, svlos, sem, rayflag, tlog
     if (!rec) k_nn<0><<<grid, 256, 0, s>>>(FOMO_NN_ARGS);
     else if (rs4 == 2) k_nn<2><<<grid, 256, 0, s>>>(FOMO_NN_ARGS);
     else if (rs4 == 4) k_nn<4><<<grid, 256, 0, s>>>(FOMO_NN_ARGS);
@@ -2530,7 +2545,7 @@ void enqueue_nn(hipStream_t s, const Dataset& ds, const ViewGeom* geom, int nx, 
     ProfScope ps("k_nn_far", s);
 #define FOMO_NNF_ARGS                                                                                         \
   ds.pts.as<float4>(), ds.mpts.as<float4>(), ds.nlo.as<float4>(), ds.nhi.as<float4>(), ds.bvh, geom, nx, ny, \
-      nz, idx, rec, ri, nlines, svlos, sem, rayflag
+      nz, idx, rec, ri, nlines, svlos, sem, rayflag, tlog
     if (!rec) k_nn_far<0><<<grid, 256, 0, s>>>(FOMO_NNF_ARGS);
     else if (rs4 == 2) k_nn_far<2><<<grid, 256, 0, s>>>(FOMO_NNF_ARGS);
     else if (rs4 == 4) k_nn_far<4><<<grid, 256, 0, s>>>(FOMO_NNF_ARGS);
@@ -3407,10 +3422,83 @@ int fomo_dataset_create(int64_t n, int dim, const float* x, const float* y, cons
   return FOMO_OK;
 }
 
+int fomo_dataset_begin(int64_t n, int dim, int nvars, int* out_id) {
+  std::lock_guard<std::recursive_mutex> lk(g_mu);
+  if (int rc = require_init()) return rc;
+  if (n < 1 || n > 2147483647LL) return fail(FOMO_ERR_ARG, "point count must be in [1, 2^31)");
+  if (dim != 2 && dim != 3) return fail(FOMO_ERR_ARG, "dim must be 2 or 3");
+  if (nvars < 0 || nvars > 16) return fail(FOMO_ERR_ARG, "bad nvars");
+  if (!out_id) return fail(FOMO_ERR_ARG, "null output id");
+  Ctx::Ingest in;
+  in.ds.reset(new Dataset());
+  in.ds->n = n;
+  in.ds->nvars = nvars;
+  in.dim = dim;
+  for (int a = 0; a < 3; ++a)
+    if (in.ds->raw[a].alloc(sizeof(float) * n) != hipSuccess) return fail(FOMO_ERR_NOMEM, "device allocation failed");
+  if (dim == 2) HIPOK(hipMemsetAsync(in.ds->raw[2].p, 0, sizeof(float) * n, g_ctx.stream));
+  for (int v = 0; v < nvars; ++v) {
+    in.ds->rawvars.emplace_back(new DevBuf());
+    if (in.ds->rawvars.back()->alloc(sizeof(float) * n) != hipSuccess)
+      return fail(FOMO_ERR_NOMEM, "device allocation failed");
+  }
+  if (int rc = sync_stream()) return rc;
+  const int id = g_ctx.next_id++;
+  g_ctx.ingests[id] = std::move(in);
+  *out_id = id;
+  return FOMO_OK;
+}
+
+int fomo_dataset_append(int id, int64_t offset, int64_t count, const float* x, const float* y, const float* z,
+                        const float* const* vars) {
+  std::lock_guard<std::recursive_mutex> lk(g_mu);
+  if (int rc = require_init()) return rc;
+  auto it = g_ctx.ingests.find(id);
+  if (it == g_ctx.ingests.end()) return fail(FOMO_ERR_NOTFOUND, "no dataset of that id is being filled");
+  Ctx::Ingest& in = it->second;
+  Dataset& ds = *in.ds;
+  if (count < 0 || offset < 0 || offset + count > ds.n) return fail(FOMO_ERR_ARG, "chunk outside [0, n)");
+  if (count == 0) return FOMO_OK;
+  if (!x || !y || (in.dim == 3 && !z) || (ds.nvars > 0 && !vars)) return fail(FOMO_ERR_ARG, "null pointer");
+  for (int v = 0; v < ds.nvars; ++v)
+    if (!vars[v]) return fail(FOMO_ERR_ARG, "null variable pointer");
+  hipStream_t s = g_ctx.stream;
+  const size_t off = sizeof(float) * (size_t)offset, len = sizeof(float) * (size_t)count;
+  const float* src[3] = {x, y, z};
+  for (int a = 0; a < in.dim; ++a)
+    HIPOK(hipMemcpyAsync(static_cast<char*>(ds.raw[a].p) + off, src[a], len, hipMemcpyHostToDevice, s));
+  for (int v = 0; v < ds.nvars; ++v)
+    HIPOK(hipMemcpyAsync(static_cast<char*>(ds.rawvars[v]->p) + off, vars[v], len, hipMemcpyHostToDevice, s));
+  if (int rc = sync_stream()) return rc;  // the caller's chunk may be reused at once
+  in.written.emplace_back(offset, count);
+  return FOMO_OK;
+}
+
+int fomo_dataset_commit(int id, double cell_scale) {
+  std::lock_guard<std::recursive_mutex> lk(g_mu);
+  if (int rc = require_init()) return rc;
+  auto it = g_ctx.ingests.find(id);
+  if (it == g_ctx.ingests.end()) return fail(FOMO_ERR_NOTFOUND, "no dataset of that id is being filled");
+  Ctx::Ingest& in = it->second;
+  // every point written exactly once
+  std::sort(in.written.begin(), in.written.end());
+  int64_t next = 0;
+  for (const auto& w : in.written) {
+    if (w.first != next) return fail(FOMO_ERR_STATE, w.first > next ? "points missing before commit" : "overlapping chunks");
+    next = w.first + w.second;
+  }
+  if (next != in.ds->n) return fail(FOMO_ERR_STATE, "points missing before commit");
+  if (int rc = build_index(*in.ds, cell_scale)) return rc;
+  g_ctx.datasets[id] = std::move(in.ds);
+  g_ctx.ingests.erase(it);
+  return FOMO_OK;
+}
+
 int fomo_dataset_destroy(int id) {
   std::lock_guard<std::recursive_mutex> lk(g_mu);
   if (int rc = require_init()) return rc;
   (void)hipStreamSynchronize(g_ctx.stream);
+  if (g_ctx.ingests.erase(id)) return FOMO_OK;  // a dataset still being filled
   Dataset* ds = find_obj(g_ctx.datasets, id);
   if (!ds) return fail(FOMO_ERR_NOTFOUND, "unknown dataset id");
   if (ds->inuse.load() > 0) return fail(FOMO_ERR_STATE, "dataset is in use by a batch or a server request");
@@ -3426,6 +3514,19 @@ int fomo_dataset_info(int id, int64_t* n, int* dims3, int* maxcount) {
   if (n) *n = ds->n;
   if (dims3) for (int a = 0; a < 3; ++a) dims3[a] = ds->grid.dims[a];
   if (maxcount) *maxcount = ds->maxcount;
+  return FOMO_OK;
+}
+
+int fomo_dataset_set_ray_tile(int id, int width) {
+  std::lock_guard<std::recursive_mutex> lk(g_mu);
+  if (int rc = require_init()) return rc;
+  Dataset* ds = find_obj(g_ctx.datasets, id);
+  if (!ds) return fail(FOMO_ERR_NOTFOUND, "unknown dataset id");
+  if (ds->inuse.load() > 0) return fail(FOMO_ERR_STATE, "dataset is in use by a batch or a server request");
+  int tl = 3;
+  while (tl <= 6 && (1 << tl) != width) ++tl;
+  if (tl > 6) return fail(FOMO_ERR_ARG, "ray tile width must be 8, 16, 32 or 64");
+  ds->tile_log2 = tl;
   return FOMO_OK;
 }
 
@@ -3781,28 +3882,34 @@ int fomo_autotune(int ngoft, const int* gofts, double l, double b, int nx, int n
   };
   const double scales[] = {0.7, 1.0, 1.4};
   const int subdivs[] = {1, 2, 3};
+  const int tlogs[] = {3, 4, 5, 6};  // ray tiles 8x32, 16x16, 32x8, 64x4
   const double cs0 = ds->cell_scale;
-  const int sd0 = ds->lookup_subdiv;
+  const int sd0 = ds->lookup_subdiv, tl0 = ds->tile_log2;
   double base = 0, best = 1e300, bscale = cs0;
-  int bsub = sd0, rc = FOMO_OK;
+  int bsub = sd0, btl = tl0, rc = FOMO_OK;
   if ((rc = time_nn(&base))) goto done;
   for (double sc : scales)
     for (int sd : subdivs) {
       ds->lookup_subdiv = sd;
       if ((rc = reindex_dataset(dsid, *ds, sc))) goto done;
       if (ds->lookup_active_subdiv != sd && sd > 1) continue;  // did not fit: same as a smaller one
-      double ms = 0;
-      if ((rc = time_nn(&ms))) goto done;
-      if (ms < best) best = ms, bscale = sc, bsub = sd;
+      for (int tl : tlogs) {
+        ds->tile_log2 = tl;
+        double ms = 0;
+        if ((rc = time_nn(&ms))) goto done;
+        if (ms < best) best = ms, bscale = sc, bsub = sd, btl = tl;
+      }
     }
-  if (!(best < 0.97 * base)) best = base, bscale = cs0, bsub = sd0;
+  if (!(best < 0.97 * base)) best = base, bscale = cs0, bsub = sd0, btl = tl0;
   ds->lookup_subdiv = bsub;
+  ds->tile_log2 = btl;
   rc = reindex_dataset(dsid, *ds, bscale);
 done:
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   if (rc) {
     ds->lookup_subdiv = sd0;
+    ds->tile_log2 = tl0;
     (void)reindex_dataset(dsid, *ds, cs0);
     return rc;
   }
@@ -3811,6 +3918,7 @@ done:
     result[1] = bsub;
     result[2] = best;
     result[3] = base;
+    result[4] = (double)(1 << btl);
   }
   return FOMO_OK;
 }
@@ -4117,16 +4225,57 @@ int fomo_batch_destroy(int id) {
   return FOMO_OK;
 }
 
-// Plain streaming copy (float4), for calibrating hardware byte counters and as a bandwidth probe.
+// Bandwidth probes. k_copy_f4: a streaming copy in one pass over the buffer, each thread moving
+// 4 float4 (stride n/4) with nontemporal loads and stores, so 4 loads per thread are in flight
+// (a grid-stride loop with one float4 per thread in flight reaches about 80 % of this rate).
+// k_gather_lines: random aligned 128-byte lines, 8 lanes per line and 2 lines per lane group in
+// flight, the access pattern of lookup-record and point-record gathers. Both are in bounds by
+// construction: every index is below n (copy) or the buffer's line count (gather).
+typedef float fomo_v4f __attribute__((ext_vector_type(4)));
 __global__ void k_copy_f4(const float4* __restrict__ a, float4* __restrict__ b, int64_t n) {
-  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
-    b[i] = a[i];
+  const int64_t stride = (n + 3) / 4;
+  const int64_t i0 = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i0 >= stride) return;
+  const fomo_v4f* src = reinterpret_cast<const fomo_v4f*>(a);
+  fomo_v4f* dst = reinterpret_cast<fomo_v4f*>(b);
+  fomo_v4f v[4];
+#pragma unroll
+  for (int u = 0; u < 4; ++u)
+    if (i0 + u * stride < n) v[u] = __builtin_nontemporal_load(src + i0 + u * stride);
+#pragma unroll
+  for (int u = 0; u < 4; ++u)
+    if (i0 + u * stride < n) __builtin_nontemporal_store(v[u], dst + i0 + u * stride);
+}
+
+__device__ __forceinline__ uint64_t probe_mix(uint64_t x) {
+  x ^= x >> 33;
+  x *= 0xff51afd7ed558ccdull;
+  x ^= x >> 33;
+  x *= 0xc4ceb9fe1a85ec53ull;
+  x ^= x >> 33;
+  return x;
+}
+
+__global__ void k_gather_lines(const float4* __restrict__ a, int64_t nlines, int64_t nitems, uint64_t seed,
+                               float* __restrict__ out) {
+  const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const int64_t item = t >> 3;
+  const int sub = (int)(t & 7);
+  float4 v[2];
+#pragma unroll
+  for (int r = 0; r < 2; ++r) {
+    const int64_t it = 2 * item + r;
+    const int64_t line = it < nitems ? (int64_t)(probe_mix((uint64_t)it + seed) % (uint64_t)nlines) : 0;
+    v[r] = a[line * 8 + sub];
+  }
+  const float s = v[0].x + v[1].w;
+  if (s == 1.2345e30f) out[0] = s;  // keeps the loads; never true for the zeroed buffer
 }
 
 int fomo_copy_probe(int64_t bytes, int reps, double* ms_per_copy) {
   std::lock_guard<std::recursive_mutex> lk(g_mu);
   if (int rc = require_init()) return rc;
-  if (bytes < 16 || reps < 1) return fail(FOMO_ERR_ARG, "bytes >= 16 and reps >= 1 required");
+  if (bytes < 64 || reps < 1) return fail(FOMO_ERR_ARG, "bytes >= 64 and reps >= 1 required");
   const int64_t n = bytes / 16;
   DevBuf a, b;
   if (a.alloc(n * 16) || b.alloc(n * 16)) return fail(FOMO_ERR_NOMEM, "device allocation failed");
@@ -4134,7 +4283,7 @@ int fomo_copy_probe(int64_t bytes, int reps, double* ms_per_copy) {
   hipEvent_t e0, e1;
   HIPOK(hipEventCreate(&e0));
   HIPOK(hipEventCreate(&e1));
-  const unsigned nb = (unsigned)std::min<int64_t>((n + 255) / 256, 8192);
+  const unsigned nb = (unsigned)(((n + 3) / 4 + 255) / 256);
   k_copy_f4<<<nb, 256, 0, g_ctx.stream>>>(a.as<float4>(), b.as<float4>(), n);  // warm
   HIPOK(hipEventRecord(e0, g_ctx.stream));
   for (int r = 0; r < reps; ++r) k_copy_f4<<<nb, 256, 0, g_ctx.stream>>>(a.as<float4>(), b.as<float4>(), n);
@@ -4145,6 +4294,33 @@ int fomo_copy_probe(int64_t bytes, int reps, double* ms_per_copy) {
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   if (ms_per_copy) *ms_per_copy = ms / reps;
+  return FOMO_OK;
+}
+
+int fomo_gather_probe(int64_t bytes, int64_t lines, int reps, double* ms_per_gather) {
+  std::lock_guard<std::recursive_mutex> lk(g_mu);
+  if (int rc = require_init()) return rc;
+  if (bytes < 128 || lines < 2 || reps < 1) return fail(FOMO_ERR_ARG, "bytes >= 128, lines >= 2, reps >= 1 required");
+  const int64_t nlines = bytes / 128;
+  const int64_t items = lines & ~(int64_t)1;
+  DevBuf a, o;
+  if (a.alloc(nlines * 128) || o.alloc(64)) return fail(FOMO_ERR_NOMEM, "device allocation failed");
+  HIPOK(hipMemsetAsync(a.p, 0, nlines * 128, g_ctx.stream));
+  hipEvent_t e0, e1;
+  HIPOK(hipEventCreate(&e0));
+  HIPOK(hipEventCreate(&e1));
+  const unsigned nb = (unsigned)((items / 2 * 8 + 255) / 256);
+  k_gather_lines<<<nb, 256, 0, g_ctx.stream>>>(a.as<float4>(), nlines, items, 0, o.as<float>());  // warm
+  HIPOK(hipEventRecord(e0, g_ctx.stream));
+  for (int r = 0; r < reps; ++r)
+    k_gather_lines<<<nb, 256, 0, g_ctx.stream>>>(a.as<float4>(), nlines, items, (uint64_t)(r + 1) << 40, o.as<float>());
+  HIPOK(hipEventRecord(e1, g_ctx.stream));
+  HIPOK(hipEventSynchronize(e1));
+  float ms = 0;
+  HIPOK(hipEventElapsedTime(&ms, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (ms_per_gather) *ms_per_gather = ms / reps;
   return FOMO_OK;
 }
 

@@ -51,6 +51,19 @@ int fomo_device_info(char* name, int name_len, double* hbm_bytes, int* compute_u
  * ------------------------------------------------------------------------------------- */
 int fomo_dataset_create(int64_t n, int dim, const float* x, const float* y, const float* z,
                         int nvars, const float* const* vars, double cell_scale, int* out_id);
+/* Streaming ingest for loaders (no counterpart in the reference, where a loader fills a DataCube
+ * point by point with push_back_datapoint or whole vectors with setdata, fomo-datacube.cpp:205-279,
+ * and the render copies it again): fomo_dataset_begin creates an empty dataset of n points on the
+ * device, fomo_dataset_append copies a chunk of `count` points starting at point `offset` (SoA
+ * float32, z ignored when dim == 2) straight to device memory before it returns, in any order and
+ * size, and fomo_dataset_commit builds the spatial index once every point has been written
+ * exactly once (FOMO_ERR_STATE otherwise). Until the commit the id is known only to these calls
+ * and fomo_dataset_destroy; afterwards the dataset is the one fomo_dataset_create makes from the
+ * same points. */
+int fomo_dataset_begin(int64_t n, int dim, int nvars, int* out_id);
+int fomo_dataset_append(int id, int64_t offset, int64_t count, const float* x, const float* y, const float* z,
+                        const float* const* vars);
+int fomo_dataset_commit(int id, double cell_scale);
 int fomo_dataset_destroy(int id);
 /* n points, cells per axis, maximum points in one cell. */
 int fomo_dataset_info(int id, int64_t* n, int* grid_dims3, int* max_cell_count);
@@ -60,6 +73,9 @@ int fomo_dataset_reindex(int id, double cell_scale);
  * 0 disables it, leaving the exact ring search over the cell list. Results are identical either
  * way; only speed and memory change. */
 int fomo_dataset_set_lookup(int id, int subdiv);
+/* Ray tile of the NN passes' blocks: width 8, 16 (default), 32 or 64 rays, 256 rays per block.
+ * Speed only; results do not change. */
+int fomo_dataset_set_ray_tile(int id, int width);
 /* Active subdivision (0 = disabled), lookup cells, candidates summed over the cells (a far group's
    shared list counts once per cell that uses it), cells left to the exact octree query. */
 int fomo_dataset_lookup_info(int id, int* subdiv, int64_t* cells, int64_t* candidates, int64_t* fallback_cells);
@@ -158,12 +174,14 @@ int fomo_projection_pixels(int dataset, double l, double b, int nx, int ny, int 
                            double* meta);
 
 /* Autotuner (no counterpart in the reference): for the dataset of the ngoft emission cubes (one
- * dataset, one kind; the lines of a batch), times the nearest-neighbour passes of one
- * representative view (l, b, nx x ny x nz; Thomson geometry for a Thomson cube), gathering the
- * same per-point records a batch of these cubes gathers, over cell-list scales {0.7, 1, 1.4} x
- * lookup subdivisions {1, 2, 3}, `reps` timed repetitions each. The dataset keeps the fastest
- * configuration when it beats the one on entry by more than 3 %, else the one on entry. result
- * (optional, 4 doubles): chosen cell scale, subdivision, its ms per view, the ms per view on entry.
+ * dataset, one kind; the lines of a batch), times the nearest-neighbour passes (k_nn + k_nn_far)
+ * of one representative view (l, b, nx x ny x nz; Thomson geometry for a Thomson cube), gathering
+ * the same per-point records a batch of these cubes gathers, over cell-list scales {0.7, 1, 1.4} x
+ * lookup subdivisions {1, 2, 3} x ray tiles of those passes' blocks {8x32, 16x16, 32x8, 64x4}
+ * (a wave of 64 neighbouring rays walks the octree together, so the tile also shapes the walks),
+ * `reps` timed repetitions each. The dataset keeps the fastest configuration when it beats the
+ * one on entry by more than 3 %, else the one on entry. result (optional, 5 doubles): chosen cell
+ * scale, subdivision, its ms per view, the ms per view on entry, the chosen tile width in rays.
  * Render results do not depend on the configuration (NN choices are exact for every one). */
 int fomo_autotune(int ngoft, const int* gofts, double l, double b, int nx, int ny, int nz, int reps,
                   double* result);
@@ -259,10 +277,14 @@ int fomo_server_stats(int server, double* stats12);
 int fomo_profile_enable(int on);
 int fomo_profile_reset(void);
 int fomo_profile_stats(const char* kernel, int64_t* count, double* total_ms);
-/* Device bandwidth probe: copies `bytes` between two device buffers `reps` times with a plain
- * float4 streaming kernel (k_copy_f4); returns the mean milliseconds per copy. Used to calibrate
- * hardware byte counters (a copy of B bytes moves B read + B written). */
+/* Device bandwidth probes (no counterpart in the reference). fomo_copy_probe copies `bytes`
+ * between two device buffers `reps` times with a one-pass nontemporal float4 streaming kernel
+ * (k_copy_f4); mean milliseconds per copy (a copy of B bytes moves B read + B written; also the
+ * calibration of hardware byte counters). fomo_gather_probe reads `lines` random aligned 128-byte
+ * lines of a buffer of `bytes` bytes, `reps` times (k_gather_lines, the access pattern of the
+ * NN passes' record gathers); mean milliseconds per launch. */
 int fomo_copy_probe(int64_t bytes, int reps, double* ms_per_copy);
+int fomo_gather_probe(int64_t bytes, int64_t lines, int reps, double* ms_per_gather);
 
 /* ---------------------------------------------------------------------------------------
  * Synthetic inputs (host-only, deterministic from the seed; no GPU needed).

@@ -180,3 +180,47 @@ def test_single_line_fused_2d_datacube(engine, fe9_table):
         _check(ds, d, tab, em, 0.7, 0.4, 256, 256, 5, 30)
     finally:
         em.close(), T.close(), ds.close()
+
+
+def test_streamed_ingest_equals_bulk_upload(engine, fe9_table):
+    """The loaders' streaming ingest (fomo_dataset_begin / append / commit) gives the dataset that
+    one bulk upload gives: chunks of uneven sizes appended out of order render bit-identical
+    images and meta; incomplete or overlapping chunks are refused at commit, out-of-range
+    chunks at append, and an uncommitted dataset cannot be used."""
+    import ctypes
+    F = engine
+    path, tab = fe9_table
+    d = F.gen_loop_cube(30, seed=13, jitter=0.2)
+    names = ("rho", "T", "vx", "vy", "vz")
+    n = d["x"].size
+    rng = np.random.default_rng(3)
+    cuts = np.sort(rng.choice(np.arange(1, n), 9, replace=False))
+    bounds = list(zip(np.r_[0, cuts], np.r_[cuts, n]))
+    rng.shuffle(bounds)
+    chunks = [(a, d["x"][a:b], d["y"][a:b], d["z"][a:b], [d[k][a:b] for k in names]) for a, b in bounds]
+    ds_s = F.Dataset.streamed(n, chunks, nvars=5)
+    ds_b = F.Dataset(d["x"], d["y"], d["z"], [d[k] for k in names])
+    T = F.Table.from_file(path)
+    ems = [F.Emission.chianti(ds, T, tab["lambda0"], tab["atweight"]) for ds in (ds_s, ds_b)]
+    w = 0.2 * 2.99792458e8 / tab["lambda0"]
+    for l, b in [(0.3, 1.0), (2.0, 0.4)]:
+        a, ma = F.render_nn(ems[0], l, b, 40, 36, 44, 24, w)
+        c, mc = F.render_nn(ems[1], l, b, 40, 36, 44, 24, w)
+        assert np.array_equal(a, c) and np.array_equal(ma, mc)
+    for e in ems:
+        e.close()
+    L = F.lib()
+    out = ctypes.c_int()
+    assert L.fomo_dataset_begin(100, 3, 0, ctypes.byref(out)) == 0
+    x = np.zeros(60, np.float32)
+    fp = x.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+    assert L.fomo_dataset_append(out.value, 50, 60, fp, fp, fp, None) == -1  # past n
+    assert L.fomo_dataset_append(out.value, 0, 60, fp, fp, fp, None) == 0
+    assert L.fomo_dataset_commit(out.value, 0.0) == -5  # 40 points missing
+    assert L.fomo_dataset_append(out.value, 50, 50, fp, fp, fp, None) == 0
+    assert L.fomo_dataset_commit(out.value, 0.0) == -5  # points 50..59 written twice
+    assert L.fomo_dataset_info(out.value, None, None, None) == -3  # not usable before its commit
+    assert L.fomo_dataset_destroy(out.value) == 0
+    T.close()
+    ds_s.close()
+    ds_b.close()
