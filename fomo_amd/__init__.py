@@ -44,6 +44,7 @@ _SIGS = {
     "fomo_dataset_reindex": (ctypes.c_int, [ctypes.c_int, ctypes.c_double]),
     "fomo_dataset_set_lookup": (ctypes.c_int, [ctypes.c_int, ctypes.c_int]),
     "fomo_dataset_set_ray_tile": (ctypes.c_int, [ctypes.c_int, ctypes.c_int]),
+    "fomo_dataset_set_far_budget": (ctypes.c_int, [ctypes.c_int, ctypes.c_int]),
     "fomo_dataset_lookup_info": (ctypes.c_int, [ctypes.c_int, _c_int_p, _c_i64_p, _c_i64_p, _c_i64_p]),
     "fomo_dataset_lookup_groups": (ctypes.c_int, [ctypes.c_int, _c_i64_p]),
     "fomo_table_read": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int, _c_dbl_p, _c_dbl_p,
@@ -256,6 +257,11 @@ class Dataset:
         """Ray tile width (8, 16, 32 or 64 rays) of the NN passes' blocks. Results do not change."""
         _check(lib().fomo_dataset_set_ray_tile(self.id, width))
 
+    def set_far_budget(self, steps):
+        """Node steps of an exact query's walk before it moves to the parallel range walk (0 = never).
+        Results do not change."""
+        _check(lib().fomo_dataset_set_far_budget(self.id, steps))
+
     def set_lookup(self, subdiv):
         """Candidate lookup grid subdivision (0 = ring search only). Results do not change."""
         _check(lib().fomo_dataset_set_lookup(self.id, subdiv))
@@ -448,7 +454,7 @@ def tet_vertices(ds, l, b, nx, ny, nz):
 
 def nn_indices(ds, l, b, nx, ny, nz, thomson=False):
     idx = np.zeros(nx * ny * nz, np.int32)
-    meta = np.zeros(10, np.float64)
+    meta = np.zeros(12, np.float64)
     _check(lib().fomo_nn_indices(ds.id, l, b, nx, ny, nz, 1 if thomson else 0, _ptr(idx, ctypes.c_int32),
                                  _ptr(meta, ctypes.c_double)))
     return idx, meta

@@ -291,47 +291,108 @@ __device__ __forceinline__ int bvh_query(const float4* __restrict__ pts, const f
   return fomo::oct_query(pts, mpts, nlo, nhi, bv.nodes, sb, best, bpos, borig);
 }
 
-// Wave-cooperative oct_query: every lane with act set gets exactly oct_query's answer, and the
-// wave walks the tree once. A node is entered when any active lane could find a better point in
-// it; a lane's own test fails on every node inside a subtree it would have skipped (a child's box
-// lies inside its parent's), so each lane scans exactly the leaves of its own walk, in the same
-// order. The node index is wave-uniform, so the node records are read once per wave (scalar
-// loads) instead of once per lane; the 64 queries of a wave are neighbouring samples and enter
-// nearly the same nodes. All lanes of the wave that are still running must call it.
-__device__ int bvh_query_wave(const float4* __restrict__ pts, const float4* __restrict__ mpts,
-                              const float4* __restrict__ nlo, const float4* __restrict__ nhi, const BvhInfo& bv,
-                              const SearchBox& sb, bool act, double best, int bpos, int borig) {
-  if (bv.nodes <= 0) return bpos;
-  if (act && bpos < 0) fomo::oct_descend(pts, mpts, nlo, nhi, sb, best, bpos, borig);
-  // The node test in float: the squared box distance has a relative error below 1e-6 (exact
-  // differences of nearby floats, then three rounded squares and two rounded sums), so with a
-  // 1e-5 margin and the bound rounded up it enters every node oct_enter enters. Entering more
-  // nodes cannot change a lane's answer: its extra leaves only add points that lose to the
-  // nearest one.
-  const auto bound = [](double b) { return (float)b * 1.000001f; };
-  float bestf = bound(best);
-  const int n = (int)bv.nodes;
-  for (int e = 0; e < n;) {
+// Whether the box [l, h] can hold a point strictly inside the search box at a squared distance
+// <= bestf (the running best rounded up, bvh_bound). The test is in float: the squared box
+// distance has a relative error below 1e-6 (exact differences of nearby floats, then three
+// rounded squares and two rounded sums), so with a 1e-5 margin and the bound rounded up it
+// admits every box oct_enter admits. Admitting more boxes cannot change an answer: their extra
+// leaves only add points that lose to the nearest one.
+__device__ __forceinline__ bool box_may_improve(const float4& l, const float4& h, const SearchBox& sb, float bestf) {
+  const float d0 = fmaxf(fmaxf(l.x - sb.t0, sb.t0 - h.x), 0.f);
+  const float d1 = fmaxf(fmaxf(l.y - sb.t1, sb.t1 - h.y), 0.f);
+  const float d2 = fmaxf(fmaxf(l.z - sb.t2, sb.t2 - h.z), 0.f);
+  const float md = (d0 * d0 + d1 * d1) + d2 * d2;
+  return sb.node_overlaps(l, h) && md * 0.99999f <= bestf;
+}
+__device__ __forceinline__ float bvh_bound(double best) { return (float)best * 1.000001f; }
+
+// Wave-cooperative walk of the preorder node range [e0, e1) of an n-node tree, from the running
+// best (best, bpos, borig; borig < 0: not read yet). Every lane with act set ends with the nearest
+// point strictly inside its box among the leaves of the range, the running best included: a leaf
+// in the range is either reached or inside the subtree of a node the walk tested and rejected, so
+// any contiguous range is a complete walk of its own leaves. A node is entered when any active
+// lane could find a better point in it; a lane's own test fails on every node inside a subtree
+// it would have skipped (a child's box lies inside its parent's), so each lane scans exactly the
+// leaves of its own walk, in the same order. The node index is wave-uniform, so the node records
+// are read once per wave instead of once per lane; the 64 queries of a wave are neighbouring
+// samples and enter nearly the same nodes. Every lane of the wave must call it (the node window
+// and the leaf points are loaded by all 64 lanes and read back with lane reads).
+// budget > 0: the walk stops after that many node steps and returns true (wave-uniform) with
+// the running best so far; a restart from node e0 with it finishes the query.
+__device__ bool bvh_walk_wave(const float4* __restrict__ pts, const float4* __restrict__ mpts,
+                              const float4* __restrict__ nlo, const float4* __restrict__ nhi, int n, int e0, int e1,
+                              const SearchBox& sb, bool act, double& best, int& bpos, int& borig, int budget) {
+  float bestf = bvh_bound(best);
+  const int lane = (int)(threadIdx.x & 63);
+  // The walk only moves forward (e + 1 or a skip pointer past the subtree), so node records are
+  // read 64 at a time, one per lane, into a window; a step reads its node from the window with
+  // lane reads instead of a dependent load. The window moves to e when e leaves it.
+  int base = e0 - 64, steps = 0;
+  float4 wl = make_float4(0.f, 0.f, 0.f, 0.f), wh = wl;
+  const auto pick = [](float v, int o) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), o));
+  };
+  for (int e = e0; e < e1;) {
     e = __builtin_amdgcn_readfirstlane(e);
-    const float4 l = nlo[e];
-    const float4 h = nhi[e];
+    if (budget > 0 && ++steps > budget) return true;
+    if (e - base >= 64) {
+      base = e;
+      const int me = min(e + lane, n - 1);
+      wl = nlo[me];
+      wh = nhi[me];
+    }
+    const int o = e - base;
+    const float4 l = make_float4(pick(wl.x, o), pick(wl.y, o), pick(wl.z, o), pick(wl.w, o));
+    const float4 h = make_float4(pick(wh.x, o), pick(wh.y, o), pick(wh.z, o), pick(wh.w, o));
     const int w = __float_as_int(l.w);
-    const float d0 = fmaxf(fmaxf(l.x - sb.t0, sb.t0 - h.x), 0.f);
-    const float d1 = fmaxf(fmaxf(l.y - sb.t1, sb.t1 - h.y), 0.f);
-    const float d2 = fmaxf(fmaxf(l.z - sb.t2, sb.t2 - h.z), 0.f);
-    const float md = (d0 * d0 + d1 * d1) + d2 * d2;
-    const bool in = act && sb.node_overlaps(l, h) && md * 0.99999f <= bestf;
+    const bool in = act && box_may_improve(l, h, sb, bestf);
     const bool any = __ballot(in) != 0;
     if (w < 0) {
-      if (in) {
-        fomo::leaf_scan(pts, mpts, __float_as_int(h.w), -w, sb, best, bpos, borig);
-        bestf = bound(best);
+      if (any) {
+        // the leaf's points (at most 8) in one load, lane u & 7 holding point u; then every
+        // entering lane scans them in leaf order, as leaf_scan does
+        const int first = __float_as_int(h.w), cnt = -w;
+        const float4 q8 = mpts[first + min(lane & 7, cnt - 1)];
+        for (int u = 0; u < cnt; ++u) {
+          const float4 q = make_float4(pick(q8.x, u), pick(q8.y, u), pick(q8.z, u), pick(q8.w, u));
+          if (!in || !sb.holds(q)) continue;
+          const double e0 = (double)q.x - (double)sb.t0;
+          const double e1 = (double)q.y - (double)sb.t1;
+          const double e2 = (double)q.z - (double)sb.t2;
+          const double d = (e0 * e0 + e1 * e1) + e2 * e2;
+          const int p = __float_as_int(q.w);
+          if (d < best) {
+            best = d;
+            bpos = p;
+            borig = -1;
+          } else if (d == best && bpos >= 0) {
+            // borig of the running best is read from pts only when a tie needs it (-1: unread)
+            if (borig < 0) borig = __float_as_int(pts[bpos].w);
+            const int oi = __float_as_int(pts[p].w);
+            if (oi < borig) {
+              bpos = p;
+              borig = oi;
+            }
+          }
+        }
+        bestf = bvh_bound(best);
       }
       ++e;
     } else {
       e = any ? e + 1 : w;
     }
   }
+  return false;
+}
+
+// Wave-cooperative oct_query: every lane with act set gets exactly oct_query's answer (see
+// bvh_walk_wave; all lanes of the wave must call it).
+__device__ int bvh_query_wave(const float4* __restrict__ pts, const float4* __restrict__ mpts,
+                              const float4* __restrict__ nlo, const float4* __restrict__ nhi, const BvhInfo& bv,
+                              const SearchBox& sb, bool act, double best, int bpos, int borig) {
+  if (bv.nodes <= 0) return bpos;
+  if (act && bpos < 0) fomo::oct_descend(pts, mpts, nlo, nhi, sb, best, bpos, borig);
+  bvh_walk_wave(pts, mpts, nlo, nhi, (int)bv.nodes, 0, (int)bv.nodes, sb, act, best, bpos, borig, 0);
   return bpos;
 }
 

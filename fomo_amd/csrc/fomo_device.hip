@@ -845,7 +845,8 @@ __global__ void __launch_bounds__(256) k_nn(const float4* __restrict__ pts, Grid
                                             LookupInfo lk, const ViewGeom* __restrict__ vgp, int nx, int ny, int nz,
                                             int32_t* __restrict__ idx, const float4* __restrict__ rec, RecInfo ri,
                                             int nlines, float* __restrict__ svlos, float2* __restrict__ sem,
-                                            uint8_t* __restrict__ rayflag, int tlog) {
+                                            uint8_t* __restrict__ rayflag, int tlog, unsigned* __restrict__ dcount) {
+  if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) *dcount = 0;  // k_nn_far's deep queue
   int i, j;
   ray_of_thread(nx, tlog, &i, &j);
   if (j >= nx || i >= ny) return;
@@ -853,28 +854,8 @@ __global__ void __launch_bounds__(256) k_nn(const float4* __restrict__ pts, Grid
                  sem, rayflag);
 }
 
-// Second NN pass over the samples k_nn left unresolved (-2): exact octree query. Same grid as
-// k_nn; rays without such samples exit at once (rayflag, cleared afterwards by k_clear_u8).
-// The starting bound is the previous sample's point on the ray when k_nn resolved it and it lies
-// strictly inside this sample's box, else a greedy descent.
-template <int RS4>
-__global__ void __launch_bounds__(256) k_nn_far(const float4* __restrict__ pts, const float4* __restrict__ mpts,
-                                                const float4* __restrict__ nlo, const float4* __restrict__ nhi,
-                                                BvhInfo bv, const ViewGeom* __restrict__ vgp, int nx, int ny, int nz,
-                                                int32_t* __restrict__ idx, const float4* __restrict__ rec, RecInfo ri,
-                                                int nlines, float* __restrict__ svlos, float2* __restrict__ sem,
-                                                const uint8_t* __restrict__ rayflag, int tlog) {
-  int i, j;
-  ray_of_thread(nx, tlog, &i, &j);
-  const int k = blockIdx.y;
-  const size_t R = (size_t)nx * ny;
-  const bool inside = j < nx && i < ny;
-  const size_t ray = inside ? (size_t)i * nx + j : 0;
-  const size_t sidx = (size_t)k * R + ray;
-  // lanes without an unresolved sample still take part in the wave's walk (act = false)
-  const bool act = inside && rayflag[ray] && idx[sidx] == -2;
-  if (__ballot(act) == 0) return;
-  const ViewGeom vg = *vgp;
+// Search box of LOS sample (i, j, k) of a view (fomo-nearestneighbour.cpp:152-183).
+__device__ __forceinline__ SearchBox sample_box(const ViewGeom& vg, int nx, int ny, int i, int j, int k) {
   const double minx = vg.ext[0], maxx = vg.ext[1], miny = vg.ext[2], maxy = vg.ext[3], minz = vg.ext[4];
   const double xx = double(j) / (nx - 1) * (maxx - minx) + minx;  // fomo-nearestneighbour.cpp:152
   const double yy = double(i) / (ny - 1) * (maxy - miny) + miny;  // :153
@@ -887,6 +868,128 @@ __global__ void __launch_bounds__(256) k_nn_far(const float4* __restrict__ pts, 
   box.t0 = (float)p0, box.t1 = (float)p1, box.t2 = (float)p2;
   box.lo0 = (float)(p0 - md), box.lo1 = (float)(p1 - md), box.lo2 = (float)(p2 - md);
   box.hi0 = (float)(p0 + md), box.hi1 = (float)(p1 + md), box.hi2 = (float)(p2 + md);
+  return box;
+}
+
+__device__ __forceinline__ double box_dist2(const SearchBox& box, const float4& q) {
+  const double d0 = (double)q.x - (double)box.t0, d1 = (double)q.y - (double)box.t1, d2 = (double)q.z - (double)box.t2;
+  return (d0 * d0 + d1 * d1) + d2 * d2;
+}
+
+// Long exact queries. A k_nn_far wave whose walk passes the far budget of node steps (targets
+// deep in a hole of the data, nearly equidistant from a large part of it, take up to ~7e4) hands
+// its samples to a queue instead. The tree's preorder node array is cut into kDeepSegs equal
+// ranges, each with the bounding box of its leaves' points (Dataset::segbox). k_nn_deep puts
+// kDeepWaves waves on every queued wave: they take its ranges one at a time from a shared counter,
+// skip a range whose box cannot improve any lane's running best (started from the queued best
+// so far) and walk the others, each keeping its own running best across its ranges.
+// k_nn_deep_merge then takes every sample's nearest point over the waves (distance, then lowest
+// original index: the order of the query, so the result is the query's). With a full queue the
+// wave finishes its walk in place.
+constexpr int kFarBudget = 2048;  // default; fomo_dataset_set_far_budget
+constexpr int kDeepSegs = 1024;
+constexpr int kDeepWaves = 32;    // waves per queued wave: kDeepWaves / 4 blocks of 256
+constexpr int kDeepBlocks = 2048;
+struct DeepQueue {
+  unsigned* count;      // queued waves (may pass cap; the queue holds the first cap)
+  int64_t* sidx;        // cap x 64 samples (-1: lane without one)
+  int32_t* seed;        // cap x 64: the running best when the wave stopped (-1: none yet)
+  unsigned* next;       // cap: next range of each queued wave (zeroed when it is queued)
+  int32_t* part;        // cap x kDeepWaves x 64: each deep wave's nearest point
+  const float4* segbox; // kDeepSegs range boxes: lo[], then hi[]
+  int cap;
+  int budget;           // node steps before a wave is queued (0: never)
+};
+// Queue capacity for a view of S samples.
+int nn_deep_cap(int64_t S) { return (int)std::min<int64_t>(2048, std::max<int64_t>(16, S >> 18)); }
+int64_t nn_deep_off(int64_t R) { return (R + 255) & ~(int64_t)255; }
+int64_t round256(int64_t b) { return (b + 255) & ~(int64_t)255; }
+// Bytes of the NN passes' scratch of a view: R ray flags, then the deep queue.
+int64_t nn_scratch_bytes(int64_t R, int64_t S) {
+  const int64_t cap = nn_deep_cap(S);
+  return nn_deep_off(R) + 256 + round256(cap * 64 * (int64_t)sizeof(int64_t)) +
+         round256(cap * 64 * (int64_t)sizeof(int32_t)) + round256(cap * (int64_t)sizeof(unsigned)) +
+         cap * kDeepWaves * 64 * (int64_t)sizeof(int32_t);
+}
+DeepQueue deep_queue(uint8_t* scratch, int64_t R, int64_t S, int budget, const float4* segbox) {
+  DeepQueue q;
+  char* b = reinterpret_cast<char*>(scratch) + nn_deep_off(R);
+  const int64_t cap = nn_deep_cap(S);
+  q.cap = (int)cap;
+  q.budget = segbox ? budget : 0;
+  q.segbox = segbox;
+  q.count = reinterpret_cast<unsigned*>(b);
+  b += 256;
+  q.sidx = reinterpret_cast<int64_t*>(b);
+  b += round256(cap * 64 * (int64_t)sizeof(int64_t));
+  q.seed = reinterpret_cast<int32_t*>(b);
+  b += round256(cap * 64 * (int64_t)sizeof(int32_t));
+  q.next = reinterpret_cast<unsigned*>(b);
+  b += round256(cap * (int64_t)sizeof(unsigned));
+  q.part = reinterpret_cast<int32_t*>(b);
+  return q;
+}
+
+// Preorder node range g of kDeepSegs over an n-node tree.
+__device__ __forceinline__ void deep_range(int n, unsigned g, int* e0, int* e1) {
+  *e0 = (int)((int64_t)n * g / kDeepSegs);
+  *e1 = (int)((int64_t)n * (g + 1) / kDeepSegs);
+}
+
+// Bounding box of the points of the leaves in each node range (empty range: lo = +inf, hi = -inf,
+// which overlaps no search box). One block per range.
+__global__ void __launch_bounds__(256) k_seg_boxes(const float4* __restrict__ nlo, const float4* __restrict__ nhi,
+                                                   int n, float4* __restrict__ segbox) {
+  __shared__ float red[6][256];
+  int e0, e1;
+  deep_range(n, blockIdx.x, &e0, &e1);
+  float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+  for (int e = e0 + (int)threadIdx.x; e < e1; e += blockDim.x) {
+    const float4 l = nlo[e];
+    if (__float_as_int(l.w) >= 0) continue;  // internal node
+    const float4 h = nhi[e];
+    lo[0] = fminf(lo[0], l.x), lo[1] = fminf(lo[1], l.y), lo[2] = fminf(lo[2], l.z);
+    hi[0] = fmaxf(hi[0], h.x), hi[1] = fmaxf(hi[1], h.y), hi[2] = fmaxf(hi[2], h.z);
+  }
+  for (int a = 0; a < 3; ++a) red[a][threadIdx.x] = lo[a], red[3 + a][threadIdx.x] = hi[a];
+  __syncthreads();
+  for (int st = 128; st > 0; st >>= 1) {
+    if ((int)threadIdx.x < st)
+      for (int a = 0; a < 3; ++a) {
+        red[a][threadIdx.x] = fminf(red[a][threadIdx.x], red[a][threadIdx.x + st]);
+        red[3 + a][threadIdx.x] = fmaxf(red[3 + a][threadIdx.x], red[3 + a][threadIdx.x + st]);
+      }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    segbox[blockIdx.x] = make_float4(red[0][0], red[1][0], red[2][0], 0.f);
+    segbox[kDeepSegs + blockIdx.x] = make_float4(red[3][0], red[4][0], red[5][0], 0.f);
+  }
+}
+
+// Second NN pass over the samples k_nn left unresolved (-2): exact octree query. Same grid as
+// k_nn; rays without such samples exit at once (rayflag, cleared afterwards by k_clear_u8).
+// The starting bound is the previous sample's point on the ray when k_nn resolved it and it lies
+// strictly inside this sample's box, else a greedy descent.
+template <int RS4>
+__global__ void __launch_bounds__(256) k_nn_far(const float4* __restrict__ pts, const float4* __restrict__ mpts,
+                                                const float4* __restrict__ nlo, const float4* __restrict__ nhi,
+                                                BvhInfo bv, const ViewGeom* __restrict__ vgp, int nx, int ny, int nz,
+                                                int32_t* __restrict__ idx, const float4* __restrict__ rec, RecInfo ri,
+                                                int nlines, float* __restrict__ svlos, float2* __restrict__ sem,
+                                                const uint8_t* __restrict__ rayflag, int tlog, DeepQueue dq) {
+  int i, j;
+  ray_of_thread(nx, tlog, &i, &j);
+  const int k = blockIdx.y;
+  const size_t R = (size_t)nx * ny;
+  const bool inside = j < nx && i < ny;
+  const size_t ray = inside ? (size_t)i * nx + j : 0;
+  const size_t sidx = (size_t)k * R + ray;
+  // lanes without an unresolved sample still take part in the wave's walk (act = false)
+  const bool act = inside && rayflag[ray] && idx[sidx] == -2;
+  if (__ballot(act) == 0) return;
+  const ViewGeom vg = *vgp;
+  const SearchBox box = sample_box(vg, nx, ny, inside ? i : 0, inside ? j : 0, k);
   double best = INFINITY;
   int bpos = -1, borig = 0x7fffffff;
   if (act && k > 0) {
@@ -894,18 +997,114 @@ __global__ void __launch_bounds__(256) k_nn_far(const float4* __restrict__ pts, 
     if (prev >= 0) {
       const float4 q = pts[prev];
       if (box.holds(q)) {
-        const double d0 = (double)q.x - (double)box.t0, d1 = (double)q.y - (double)box.t1,
-                     d2 = (double)q.z - (double)box.t2;
-        best = (d0 * d0 + d1 * d1) + d2 * d2;
+        best = box_dist2(box, q);
         bpos = prev;
         borig = __float_as_int(q.w);
       }
     }
   }
-  const int r = bvh_query_wave(pts, mpts, nlo, nhi, bv, box, act, best, bpos, borig);
+  if (bv.nodes > 0) {
+    const int n = (int)bv.nodes;
+    if (act && bpos < 0) fomo::oct_descend(pts, mpts, nlo, nhi, box, best, bpos, borig);
+    if (bvh_walk_wave(pts, mpts, nlo, nhi, n, 0, n, box, act, best, bpos, borig, dq.budget)) {
+      const int lane = (int)(threadIdx.x & 63);
+      unsigned job = 0;
+      if (lane == 0) {
+        job = atomicAdd(dq.count, 1u);
+        if (job < (unsigned)dq.cap) dq.next[job] = 0;
+      }
+      job = (unsigned)__shfl((int)job, 0, 64);
+      if (job < (unsigned)dq.cap) {
+        dq.sidx[(size_t)job * 64 + lane] = act ? (int64_t)sidx : -1;
+        dq.seed[(size_t)job * 64 + lane] = act ? bpos : -1;
+        return;  // idx stays -2 until k_nn_deep_merge
+      }
+      bvh_walk_wave(pts, mpts, nlo, nhi, n, 0, n, box, act, best, bpos, borig, 0);
+    }
+  }
   if (!act) return;
-  idx[sidx] = r;
-  if constexpr (RS4 > 0) write_sample<RS4>(r, rec, ri, nlines, R * nz, sidx, cb, sb, cl, sl, svlos, sem);
+  idx[sidx] = bpos;
+  if constexpr (RS4 > 0)
+    write_sample<RS4>(bpos, rec, ri, nlines, R * nz, sidx, vg.cb, vg.sb, vg.cl, vg.sl, svlos, sem);
+}
+
+// kDeepWaves waves per queued wave (kDeepWaves / 4 blocks, each block one (queued wave, block)
+// pair at a time); they share the queued wave's ranges through its counter.
+__global__ void __launch_bounds__(256) k_nn_deep(const float4* __restrict__ pts, const float4* __restrict__ mpts,
+                                                 const float4* __restrict__ nlo, const float4* __restrict__ nhi,
+                                                 BvhInfo bv, const ViewGeom* __restrict__ vgp, int nx, int ny,
+                                                 DeepQueue dq) {
+  constexpr unsigned kPer = kDeepWaves / 4;
+  const unsigned jobs = min(*dq.count, (unsigned)dq.cap);
+  const unsigned pairs = jobs * kPer;
+  if (blockIdx.x >= pairs) return;
+  const int lane = (int)(threadIdx.x & 63);
+  const ViewGeom vg = *vgp;
+  const int64_t R = (int64_t)nx * ny;
+  const int n = (int)bv.nodes;
+  for (unsigned p = blockIdx.x; p < pairs; p += gridDim.x) {
+    const unsigned job = p / kPer, slot = (p % kPer) * 4 + (threadIdx.x >> 6);
+    const int64_t sidx = dq.sidx[(size_t)job * 64 + lane];
+    const bool act = sidx >= 0;
+    const int64_t sa = act ? sidx : 0, ray = sa % R;
+    const SearchBox box = sample_box(vg, nx, ny, (int)(ray / nx), (int)(ray % nx), (int)(sa / R));
+    double best = INFINITY;
+    int bpos = act ? dq.seed[(size_t)job * 64 + lane] : -1, borig = 0x7fffffff;
+    if (bpos >= 0) {
+      const float4 q = pts[bpos];
+      best = box_dist2(box, q);
+      borig = __float_as_int(q.w);
+    }
+    for (;;) {
+      unsigned g = 0;
+      if (lane == 0) g = atomicAdd(&dq.next[job], 1u);
+      g = (unsigned)__shfl((int)g, 0, 64);
+      if (g >= (unsigned)kDeepSegs) break;
+      const float4 l = dq.segbox[g], h = dq.segbox[kDeepSegs + g];
+      if (__ballot(act && box_may_improve(l, h, box, bvh_bound(best))) == 0) continue;
+      int e0, e1;
+      deep_range(n, g, &e0, &e1);
+      bvh_walk_wave(pts, mpts, nlo, nhi, n, e0, e1, box, act, best, bpos, borig, 0);
+    }
+    dq.part[((size_t)job * kDeepWaves + slot) * 64 + lane] = bpos;
+  }
+}
+
+// Every queued sample's nearest point over the deep waves; written like k_nn_far writes its samples.
+template <int RS4>
+__global__ void __launch_bounds__(256) k_nn_deep_merge(const float4* __restrict__ pts,
+                                                       const ViewGeom* __restrict__ vgp, int nx, int ny, int nz,
+                                                       int32_t* __restrict__ idx, const float4* __restrict__ rec,
+                                                       RecInfo ri, int nlines, float* __restrict__ svlos,
+                                                       float2* __restrict__ sem, DeepQueue dq) {
+  const unsigned jobs = min(*dq.count, (unsigned)dq.cap);
+  const ViewGeom vg = *vgp;
+  const int64_t R = (int64_t)nx * ny;
+  for (size_t u = blockIdx.x * (size_t)blockDim.x + threadIdx.x; u < (size_t)jobs * 64;
+       u += (size_t)gridDim.x * blockDim.x) {
+    const int64_t sidx = dq.sidx[u];
+    if (sidx < 0) continue;
+    const size_t job = u / 64, lane = u % 64;
+    const int64_t ray = sidx % R;
+    const SearchBox box = sample_box(vg, nx, ny, (int)(ray / nx), (int)(ray % nx), (int)(sidx / R));
+    int bpos = -1, borig = 0x7fffffff;
+    double best = INFINITY;
+    for (int g = 0; g < kDeepWaves; ++g) {
+      const int p = dq.part[(job * kDeepWaves + g) * 64 + lane];
+      if (p < 0) continue;
+      const float4 q = pts[p];
+      const double d = box_dist2(box, q);
+      const int o = __float_as_int(q.w);
+      if (d < best || (d == best && o < borig)) {
+        best = d;
+        bpos = p;
+        borig = o;
+      }
+    }
+    idx[sidx] = bpos;
+    if constexpr (RS4 > 0)
+      write_sample<RS4>(bpos, rec, ri, nlines, (size_t)R * nz, (size_t)sidx, vg.cb, vg.sb, vg.cl, vg.sl, svlos, sem);
+  }
 }
 
 __global__ void k_clear_u8(uint8_t* __restrict__ a, int64_t n) {
@@ -1693,10 +1892,12 @@ struct Dataset {
   BvhInfo bvh{};       // implicit BVH over Morton-ordered points (exact NN fallback)
   DevBuf mpts;         // float4 (x, y, z, sorted position) in Morton order
   DevBuf nlo, nhi;     // node boxes, all levels
+  DevBuf segbox;       // kDeepSegs bounding boxes of the node ranges of k_nn_deep (lo[], then hi[])
   int64_t ncand = 0, nlcells = 0, nfallback = 0;
   // far and outer groups of the lookup grid: groups, groups with a candidate list, far groups
   // whose cells all take the list (group headers)
   int64_t far_groups = 0, far_lists = 0, full_groups = 0, outer_groups = 0, outer_lists = 0;
+  int far_budget = kFarBudget;  // node steps of a k_nn_far walk before the deep pass (0: never)
   int tile_log2 = 4;  // ray tile of k_nn / k_nn_far blocks: 2^tile_log2 x 256 / 2^tile_log2 (autotuned)
   // tetra path: Delaunay triangulation of the points, built on the host on first use
   int64_t ntets = 0;      // 0: not built
@@ -1733,7 +1934,7 @@ struct ViewScratch {
   DevBuf out;      // float R*nl
   DevBuf flag;     // int32 R (rays needing the direct-exp pass)
   DevBuf sem;      // float2 S: per-sample (peak, fwhm)
-  DevBuf rayflag;  // uint8 R: rays with samples for the BVH pass (kept all-zero between uses)
+  DevBuf rayflag;  // nn_scratch_bytes: R ray flags for the BVH pass (kept all-zero between uses), deep queue
 };
 
 struct Batch {
@@ -1744,7 +1945,7 @@ struct Batch {
   double width = 0;
   DevBuf partial, geom, aux, out, flag;
   DevBuf idx;      // per-sample sorted positions (double-buffered)
-  DevBuf rayflag;  // uint8 R per buffer, all-zero between uses
+  DevBuf rayflag;  // nn_scratch_bytes per buffer: ray flags all-zero between uses, deep queue
   DevBuf rec;  // per-point records {vel, T, all lines' peaks}
   RecInfo ri{};
   DevBuf sem;  // per-sample (peak, fwhm), nlines x S
@@ -1958,6 +2159,8 @@ int build_bvh(Dataset& ds) {
       k_oct_boxes<<<nblk(cnt), kBlock, 0, s>>>(dids.as<int32_t>() + doff[d], cnt, ds.mpts.as<float4>(),
                                               ds.nlo.as<float4>(), ds.nhi.as<float4>());
   }
+  if (ds.segbox.ensure(sizeof(float4) * 2 * kDeepSegs)) return fail(FOMO_ERR_NOMEM, "device allocation failed (octree)");
+  k_seg_boxes<<<kDeepSegs, 256, 0, s>>>(ds.nlo.as<float4>(), ds.nhi.as<float4>(), (int)nodes, ds.segbox.as<float4>());
   HIPOK(hipGetLastError());
   if (int rc = sync_stream()) return rc;
   ds.bvh.nodes = nodes;
@@ -2522,7 +2725,8 @@ int rec_width(int nlines) {
 // NN query for every sample of a view (idx: S ints, always written). With rec: per-sample vlos
 // and emissivities (nlines) too. Two passes: k_nn (lookup grid) and k_nn_far (exact octree query
 // for the rest).
-// rayflag: R bytes, zero on entry and on exit (set by k_nn, cleared by k_clear_u8).
+// rayflag: nn_scratch_bytes(R, S) bytes: R ray flags, zero on entry and on exit (set by k_nn,
+// cleared by k_clear_u8), then k_nn_far's deep queue (its count zeroed by k_nn).
 void enqueue_nn(hipStream_t s, const Dataset& ds, const ViewGeom* geom, int nx, int ny, int nz, int32_t* idx,
                 uint8_t* rayflag, const float4* rec = nullptr, int rs4 = 0, int nlines = 0, float* svlos = nullptr,
                 float2* sem = nullptr, const RecInfo& ri = RecInfo{}) {
@@ -2530,11 +2734,12 @@ void enqueue_nn(hipStream_t s, const Dataset& ds, const ViewGeom* geom, int nx, 
   const int tiles = ((nx + tw - 1) / tw) * ((ny + th - 1) / th);
   dim3 grid(tiles, nz);
   const int64_t R = (int64_t)nx * ny;
+  const DeepQueue dq = deep_queue(rayflag, R, R * nz, ds.far_budget, ds.segbox.as<float4>());
   {
     ProfScope ps("k_nn", s);
 #define FOMO_NN_ARGS                                                                                         \
   ds.pts.as<float4>(), ds.grid, ds.lslot.as<float4>(), ds.lcand.as<float4>(), ds.lk, geom, nx, ny, nz, idx, rec, \
-      ri, nlines, svlos, sem, rayflag, tlog
+      ri, nlines, svlos, sem, rayflag, tlog, dq.count
     if (!rec) k_nn<0><<<grid, 256, 0, s>>>(FOMO_NN_ARGS);
     else if (rs4 == 2) k_nn<2><<<grid, 256, 0, s>>>(FOMO_NN_ARGS);
     else if (rs4 == 4) k_nn<4><<<grid, 256, 0, s>>>(FOMO_NN_ARGS);
@@ -2545,12 +2750,24 @@ void enqueue_nn(hipStream_t s, const Dataset& ds, const ViewGeom* geom, int nx, 
     ProfScope ps("k_nn_far", s);
 #define FOMO_NNF_ARGS                                                                                         \
   ds.pts.as<float4>(), ds.mpts.as<float4>(), ds.nlo.as<float4>(), ds.nhi.as<float4>(), ds.bvh, geom, nx, ny, \
-      nz, idx, rec, ri, nlines, svlos, sem, rayflag, tlog
+      nz, idx, rec, ri, nlines, svlos, sem, rayflag, tlog, dq
     if (!rec) k_nn_far<0><<<grid, 256, 0, s>>>(FOMO_NNF_ARGS);
     else if (rs4 == 2) k_nn_far<2><<<grid, 256, 0, s>>>(FOMO_NNF_ARGS);
     else if (rs4 == 4) k_nn_far<4><<<grid, 256, 0, s>>>(FOMO_NNF_ARGS);
     else k_nn_far<8><<<grid, 256, 0, s>>>(FOMO_NNF_ARGS);
 #undef FOMO_NNF_ARGS
+  }
+  {
+    ProfScope ps("k_nn_deep", s);
+    k_nn_deep<<<kDeepBlocks, 256, 0, s>>>(ds.pts.as<float4>(), ds.mpts.as<float4>(), ds.nlo.as<float4>(),
+                                           ds.nhi.as<float4>(), ds.bvh, geom, nx, ny, dq);
+    const unsigned mblocks = (unsigned)((dq.cap * 64 + 255) / 256);
+#define FOMO_NND_ARGS ds.pts.as<float4>(), geom, nx, ny, nz, idx, rec, ri, nlines, svlos, sem, dq
+    if (!rec) k_nn_deep_merge<0><<<mblocks, 256, 0, s>>>(FOMO_NND_ARGS);
+    else if (rs4 == 2) k_nn_deep_merge<2><<<mblocks, 256, 0, s>>>(FOMO_NND_ARGS);
+    else if (rs4 == 4) k_nn_deep_merge<4><<<mblocks, 256, 0, s>>>(FOMO_NND_ARGS);
+    else k_nn_deep_merge<8><<<mblocks, 256, 0, s>>>(FOMO_NND_ARGS);
+#undef FOMO_NND_ARGS
     k_clear_u8<<<nblk(R), kBlock, 0, s>>>(rayflag, R);
   }
 }
@@ -2754,7 +2971,7 @@ struct RenderBufs {
   float* out;        // R * max(nl,1)
   int32_t* flag;     // NN: R rays needing the direct-exp pass
   float2* sem;       // NN: per-sample (peak, fwhm), S
-  uint8_t* rayflag;  // R, zero on entry (left zero)
+  uint8_t* rayflag;  // nn_scratch_bytes: R ray flags, zero on entry (left zero), and the deep queue
 };
 constexpr int kRenderBufs = 8;
 
@@ -2800,7 +3017,7 @@ void render_sizes(int mode, int64_t n, int64_t R, int64_t S, int nl, int64_t sz[
   sz[4] = (int64_t)sizeof(float) * R * std::max(nl, 1);
   sz[5] = mode != MODE_THOMSON ? (int64_t)sizeof(int32_t) * R : 0;
   sz[6] = mode != MODE_THOMSON ? (int64_t)sizeof(float2) * S : 0;
-  sz[7] = R;
+  sz[7] = nn_scratch_bytes(R, S);
 }
 
 // Argument and object checks shared by the synchronous renders and the server.
@@ -3530,6 +3747,17 @@ int fomo_dataset_set_ray_tile(int id, int width) {
   return FOMO_OK;
 }
 
+int fomo_dataset_set_far_budget(int id, int steps) {
+  std::lock_guard<std::recursive_mutex> lk(g_mu);
+  if (int rc = require_init()) return rc;
+  Dataset* ds = find_obj(g_ctx.datasets, id);
+  if (!ds) return fail(FOMO_ERR_NOTFOUND, "unknown dataset id");
+  if (ds->inuse.load() > 0) return fail(FOMO_ERR_STATE, "dataset is in use by a batch or a server request");
+  if (steps < 0) return fail(FOMO_ERR_ARG, "far budget must be >= 0");
+  ds->far_budget = steps;
+  return FOMO_OK;
+}
+
 int fomo_dataset_set_lookup(int id, int subdiv) {
   std::lock_guard<std::recursive_mutex> lk(g_mu);
   if (int rc = require_init()) return rc;
@@ -3850,7 +4078,7 @@ int fomo_autotune(int ngoft, const int* gofts, double l, double b, int nx, int n
   HIPOK(vs.idx.ensure(sizeof(int32_t) * S));
   HIPOK(vs.aux.ensure(sizeof(float) * std::max<int64_t>(ds->n, S)));
   HIPOK(vs.sem.ensure(sizeof(float2) * S * ngoft));
-  HIPOK(vs.rayflag.ensure_zeroed(R, s));
+  HIPOK(vs.rayflag.ensure_zeroed(nn_scratch_bytes(R, S), s));
   hipEvent_t e0, e1;
   HIPOK(hipEventCreate(&e0));
   HIPOK(hipEventCreate(&e1));
@@ -3985,7 +4213,7 @@ int fomo_nn_indices(int dataset, double l, double b, int nx, int ny, int nz, int
   HIPOK(vs.geom.ensure(sizeof(ViewGeom)));
   HIPOK(vs.idx.ensure(sizeof(int32_t) * S));
   HIPOK(vs.aux.ensure(sizeof(float) * std::max<int64_t>(ds->n, S)));
-  HIPOK(vs.rayflag.ensure_zeroed(R, s));
+  HIPOK(vs.rayflag.ensure_zeroed(nn_scratch_bytes(R, S), s));
   ViewGeom* geom = vs.geom.as<ViewGeom>();
   enqueue_view_geom(s, *ds, mode, l, b, nx, ny, nz, 0, vs.aux.as<float>(), vs.partial.as<double>(), geom);
   enqueue_nn(s, *ds, geom, nx, ny, nz, vs.idx.as<int32_t>(), vs.rayflag.as<uint8_t>());
@@ -3995,6 +4223,9 @@ int fomo_nn_indices(int dataset, double l, double b, int nx, int ny, int nz, int
   HIPOK(hipMemcpyAsync(out_idx, tmp, sizeof(int32_t) * S, hipMemcpyDeviceToHost, s));
   ViewGeom hg;
   HIPOK(hipMemcpyAsync(&hg, geom, sizeof(ViewGeom), hipMemcpyDeviceToHost, s));
+  const DeepQueue dq = deep_queue(vs.rayflag.as<uint8_t>(), R, S, ds->far_budget, ds->segbox.as<float4>());
+  unsigned queued = 0;
+  HIPOK(hipMemcpyAsync(&queued, dq.count, sizeof(unsigned), hipMemcpyDeviceToHost, s));
   if (int rc = sync_stream()) return rc;
   if (meta) {
     for (int a = 0; a < 6; ++a) meta[a] = hg.ext[a];
@@ -4002,6 +4233,8 @@ int fomo_nn_indices(int dataset, double l, double b, int nx, int ny, int nz, int
     meta[7] = hg.pathlength;
     meta[8] = hg.scale;
     meta[9] = hg.apix;
+    meta[10] = (double)queued;  // waves that reached the far budget (the first cap went to k_nn_deep)
+    meta[11] = (double)dq.cap;
   }
   return FOMO_OK;
 }
@@ -4054,7 +4287,7 @@ static int enqueue_batch(Batch& bt, hipStream_t s, hipStream_t s2) {
       if (s2) HIPOK(hipEventRecord(bt.ev[nv + v], s));
       continue;
     }
-    enqueue_nn(s, *ds, geom, bt.nx, bt.ny, bt.nz, bt.idx.as<int32_t>() + buf * S, bt.rayflag.as<uint8_t>() + buf * R,
+    enqueue_nn(s, *ds, geom, bt.nx, bt.ny, bt.nz, bt.idx.as<int32_t>() + buf * S, bt.rayflag.as<uint8_t>() + buf * nn_scratch_bytes(R, S),
                bt.rec.as<float4>(), bt.rs4, L, svl, sem, bt.ri);
     hipStream_t si = s;
     if (s2) {
@@ -4111,11 +4344,11 @@ int fomo_batch_create(int ngoft, const int* gofts, int nviews, const double* l, 
   const int nlo = std::max(nl, 1);
   if (bt->partial.alloc(sizeof(double) * 6 * kMaxPartials * nviews) || bt->geom.alloc(sizeof(ViewGeom) * nviews) ||
       bt->sem.alloc(sizeof(float2) * S * ngoft * 2) || bt->aux.alloc(sizeof(float) * S * 2) ||
-      bt->idx.alloc(sizeof(int32_t) * S * 2) || bt->rayflag.alloc(R * 2) ||
+      bt->idx.alloc(sizeof(int32_t) * S * 2) || bt->rayflag.alloc(nn_scratch_bytes(R, S) * 2) ||
       bt->flag.alloc(sizeof(int32_t) * R * kMaxLines * 2) ||
       bt->out.alloc(sizeof(float) * (size_t)R * nlo * ngoft * nviews))
     return fail(FOMO_ERR_NOMEM, "device allocation failed for batch");
-  HIPOK(hipMemsetAsync(bt->rayflag.p, 0, R * 2, g_ctx.stream));
+  HIPOK(hipMemsetAsync(bt->rayflag.p, 0, nn_scratch_bytes(R, S) * 2, g_ctx.stream));
   {
     std::vector<const Goft*> cg;
     for (int gid : bt->gofts) cg.push_back(find_obj(g_ctx.gofts, gid));

@@ -76,6 +76,10 @@ int fomo_dataset_set_lookup(int id, int subdiv);
 /* Ray tile of the NN passes' blocks: width 8, 16 (default), 32 or 64 rays, 256 rays per block.
  * Speed only; results do not change. */
 int fomo_dataset_set_ray_tile(int id, int width);
+/* Node steps an exact octree query of the NN passes takes (per wave of 64 samples) before its
+ * samples move to a pass that walks 1024 node ranges of the tree with 32 waves in parallel and
+ * merges them (default 2048; 0: never). Speed only; results do not change. */
+int fomo_dataset_set_far_budget(int id, int steps);
 /* Active subdivision (0 = disabled), lookup cells, candidates summed over the cells (a far group's
    shared list counts once per cell that uses it), cells left to the exact octree query. */
 int fomo_dataset_lookup_info(int id, int* subdiv, int64_t* cells, int64_t* candidates, int64_t* fallback_cells);
@@ -197,7 +201,10 @@ int fomo_tet_vertices(int dataset, double l, double b, int nx, int ny, int nz, i
 
 /* Nearest-neighbour choice of every LOS sample of one view (original point indices, -1 = none),
  * layout (iy*nx + jx)*nz + k. thomson!=0 uses the Thomson geometry (fomo-thomson.cpp:136-317).
- * For parity checks of the spatial index against the reference's R-tree query. */
+ * For parity checks of the spatial index against the reference's R-tree query. meta: 12 doubles,
+ * the 10 of a render, then the waves of 64 samples whose exact query reached the far budget
+ * (fomo_dataset_set_far_budget) and the number of them the parallel range walk takes (the rest
+ * finish their own walk). */
 int fomo_nn_indices(int dataset, double l, double b, int nx, int ny, int nz, int thomson,
                     int32_t* out_idx, double* meta);
 
