@@ -1898,7 +1898,7 @@ struct Dataset {
   // whose cells all take the list (group headers)
   int64_t far_groups = 0, far_lists = 0, full_groups = 0, outer_groups = 0, outer_lists = 0;
   int far_budget = kFarBudget;  // node steps of a k_nn_far walk before the deep pass (0: never)
-  int tile_log2 = 4;  // ray tile of k_nn / k_nn_far blocks: 2^tile_log2 x 256 / 2^tile_log2 (autotuned)
+  int tile_log2 = 3;  // ray tile of k_nn / k_nn_far blocks: 2^tile_log2 x 256 / 2^tile_log2 (autotuned)
   // tetra path: Delaunay triangulation of the points, built on the host on first use
   int64_t ntets = 0;      // 0: not built
   DevBuf trec;            // 4 float4 per tet: vertex {x, y, z, sorted position as int bits (-1 = infinite)}

@@ -73,7 +73,7 @@ int fomo_dataset_reindex(int id, double cell_scale);
  * 0 disables it, leaving the exact ring search over the cell list. Results are identical either
  * way; only speed and memory change. */
 int fomo_dataset_set_lookup(int id, int subdiv);
-/* Ray tile of the NN passes' blocks: width 8, 16 (default), 32 or 64 rays, 256 rays per block.
+/* Ray tile of the NN passes' blocks: width 8 (default), 16, 32 or 64 rays, 256 rays per block.
  * Speed only; results do not change. */
 int fomo_dataset_set_ray_tile(int id, int width);
 /* Node steps an exact octree query of the NN passes takes (per wave of 64 samples) before its

@@ -23,7 +23,7 @@ def test_autotune_keeps_results(engine, fe9_table):
     r = F.autotune(em, 0.3, 1.2, 48, 40, 50, reps=2)
     assert r["cell_scale"] in (0.7, 1.0, 1.4) and r["subdiv"] in (1, 2, 3) and r["tile"] in (8, 16, 32, 64)
     assert 0 < r["ms"] <= r["ms_before"]
-    if (r["cell_scale"], r["subdiv"], r["tile"]) != (1.0, 2, 16):
+    if (r["cell_scale"], r["subdiv"], r["tile"]) != (1.0, 2, 8):
         assert r["ms"] < 0.97 * r["ms_before"]
     after = [F.render_nn(em, l, b, nx, ny, nz, nl, width) for (l, b, nx, ny, nz, nl) in views]
     for (a, ma), (c, mc) in zip(before, after):
