@@ -45,6 +45,7 @@ _SIGS = {
     "fomo_dataset_set_lookup": (ctypes.c_int, [ctypes.c_int, ctypes.c_int]),
     "fomo_dataset_set_ray_tile": (ctypes.c_int, [ctypes.c_int, ctypes.c_int]),
     "fomo_dataset_set_far_budget": (ctypes.c_int, [ctypes.c_int, ctypes.c_int]),
+    "fomo_dataset_tet_stats": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(ctypes.c_int64)]),
     "fomo_dataset_lookup_info": (ctypes.c_int, [ctypes.c_int, _c_int_p, _c_i64_p, _c_i64_p, _c_i64_p]),
     "fomo_dataset_lookup_groups": (ctypes.c_int, [ctypes.c_int, _c_i64_p]),
     "fomo_table_read": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int, _c_dbl_p, _c_dbl_p,
@@ -257,6 +258,13 @@ class Dataset:
         """Ray tile width (8, 16, 32 or 64 rays) of the NN passes' blocks. Results do not change."""
         _check(lib().fomo_dataset_set_ray_tile(self.id, width))
 
+    def tet_stats(self):
+        """Tetra path counts since the triangulation was built: samples walked again with exact
+        orientation tests, and samples left unresolved."""
+        out = (ctypes.c_int64 * 2)()
+        _check(lib().fomo_dataset_tet_stats(self.id, out))
+        return {"exact_rewalks": int(out[0]), "unresolved": int(out[1])}
+
     def set_far_budget(self, steps):
         """Node steps of an exact query's walk before it moves to the parallel range walk (0 = never).
         Results do not change."""
@@ -447,7 +455,7 @@ def projection_pixels(ds, l, b, nx, ny, nz=2):
 
 def tet_vertices(ds, l, b, nx, ny, nz):
     out = np.zeros(nx * ny * nz, np.int32)
-    meta = np.zeros(10, np.float64)
+    meta = np.zeros(12, np.float64)
     _check(lib().fomo_tet_vertices(ds.id, l, b, nx, ny, nz, _ptr(out, ctypes.c_int32), _ptr(meta, ctypes.c_double)))
     return out, meta
 

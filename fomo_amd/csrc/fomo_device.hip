@@ -42,6 +42,7 @@
 #include "host_copy.h"
 #include "lookup_geom.h"
 #include "octree.h"
+#include "orient_exact.h"
 #include "spectral_exp.h"
 #include "fomo_common.h"
 
@@ -1386,6 +1387,7 @@ __global__ void k_far_headers(int32_t* __restrict__ cnt, int64_t ncl, LookupInfo
 // sample on a shared face stays in the tet the walk reached first.
 // ------------------------------------------------------------------------------------
 constexpr int kTetMaxSteps = 1 << 16;
+constexpr int kTetRedoBlocks = 64;  // k_tet_redo grid (grid-stride over the redo list)
 constexpr int64_t kTetThreads = 1 << 18;  // segments per launch worth splitting rays for
 constexpr int kTetMaxSegment = 16;          // longest run of LOS samples one thread walks
 
@@ -1434,20 +1436,98 @@ __global__ void k_tet_records(const int4* __restrict__ tv, const float4* __restr
   }
 }
 
+// The outputs of one sample from its tet q (vertex records) or outside (no tet): the nearest
+// vertex of the cell (first of equals in the cell's vertex order, fomo-CGAL.cpp:211-222), or with
+// LINEAR the barycentric interpolation of the vertex values; vert (when given) gets the chosen
+// vertex (sorted position, -1 outside).
+template <int RS4, bool LINEAR>
+__device__ __forceinline__ void tet_emit(const float4 (&q)[4], bool outside, double px, double py, double pz,
+                                         size_t sidx, size_t S, double cb, double sb, double cl, double sl,
+                                         const float4* __restrict__ rec, const RecInfo& ri, int nlines,
+                                         float* __restrict__ svlos, float2* __restrict__ sem,
+                                         int32_t* __restrict__ vert) {
+  if (outside) {
+    if (vert) vert[sidx] = -1;
+    if constexpr (RS4 > 0) write_sample<RS4>(-1, rec, ri, nlines, S, sidx, cb, sb, cl, sl, svlos, sem);
+    return;
+  }
+  if constexpr (!LINEAR) {
+    // nearest vertex of the cell (first of equals in the cell's vertex order)
+    int best = 0;
+    double bd = INFINITY;
+#pragma unroll
+    for (int m = 0; m < 4; ++m) {
+      const double d0 = (double)q[m].x - px, d1 = (double)q[m].y - py, d2 = (double)q[m].z - pz;
+      const double d = (d0 * d0 + d1 * d1) + d2 * d2;
+      if (d < bd) {
+        bd = d;
+        best = m;
+      }
+    }
+    const int r = __float_as_int(best == 0 ? q[0].w : (best == 1 ? q[1].w : (best == 2 ? q[2].w : q[3].w)));
+    if (vert) vert[sidx] = r;
+    if constexpr (RS4 > 0) write_sample<RS4>(r, rec, ri, nlines, S, sidx, cb, sb, cl, sl, svlos, sem);
+  } else {
+    // barycentric weights: volume of the tet with vertex m replaced by the sample, over the volume
+    double w[4], tot = 0.0;
+    (void)orient_d(q[2], q[1], q[3], px, py, pz, &w[0]);
+    (void)orient_d(q[0], q[2], q[3], px, py, pz, &w[1]);
+    (void)orient_d(q[1], q[0], q[3], px, py, pz, &w[2]);
+    (void)orient_d(q[0], q[1], q[2], px, py, pz, &w[3]);
+#pragma unroll
+    for (int f = 0; f < 4; ++f) {
+      w[f] = fmax(w[f], 0.0);
+      tot += w[f];
+    }
+    if (vert) vert[sidx] = __float_as_int(q[0].w);
+    if constexpr (RS4 > 0) {
+      constexpr int NL = 4 * RS4 - 4;  // lines in a record
+      double vl = 0.0, pk[NL], fw[NL];
+#pragma unroll
+      for (int l = 0; l < NL; ++l) pk[l] = fw[l] = 0.0;
+#pragma unroll
+      for (int m = 0; m < 4; ++m) {
+        const double wm = tot > 0 ? w[m] / tot : 0.25;
+        const float4* rp = rec + (size_t)__float_as_int(q[m].w) * RS4;
+        float f[4 * RS4];
+#pragma unroll
+        for (int qq = 0; qq < RS4; ++qq) {
+          const float4 a = rp[qq];
+          f[4 * qq] = a.x, f[4 * qq + 1] = a.y, f[4 * qq + 2] = a.z, f[4 * qq + 3] = a.w;
+        }
+        vl += wm * (((0.0 + (sb * cl) * (double)f[0]) + (-sb * sl) * (double)f[1]) + cb * (double)f[2]);
+        const double st = rec_st(ri.K[0], f[3]);
+#pragma unroll
+        for (int l = 0; l < NL; ++l) {
+          pk[l] += wm * (double)f[4 + l];
+          fw[l] += wm * (double)rec_fw(ri.K[l], st);
+        }
+      }
+      svlos[sidx] = (float)vl;
+#pragma unroll
+      for (int l = 0; l < NL; ++l)
+        if (l < nlines) sem[(size_t)l * S + sidx] = make_float2((float)pk[l], (float)fw[l]);
+    }
+  }
+}
+
 // One thread per segment of seglen samples of a ray: the walk is a chain of dependent loads, so
 // small views split their rays to keep enough walks in flight. Each segment's first sample starts
 // from the start grid, the others from the previous sample's tet. A warm start on a hull tet whose
 // face does not exclude the sample (a ray entering the hull) restarts from the start grid too, so
 // no walk crosses the hull surface from where the ray left it. Writes per-sample svlos / sem like
 // k_nn (RS4 > 0) and, when vert != null, the chosen vertex (sorted position, -1 outside) per
-// sample in k-major order.
+// sample in k-major order. A sample whose final tet had an undecided test, or whose walk reached
+// the step cap, goes to the redo list (redo[0]: entries, redo[1]: samples beyond redo_cap, then
+// the sample indices), which k_tet_redo walks again with exact tests.
 template <int RS4, bool LINEAR>
 __global__ void __launch_bounds__(256) k_tet_walk(const float4* __restrict__ trec, const int4* __restrict__ tn,
                                                   const int32_t* __restrict__ tstart, TetGrid tg,
                                                   const ViewGeom* __restrict__ vgp, int nx, int ny, int nz,
                                                   int seglen, const float4* __restrict__ rec, RecInfo ri, int nlines,
                                                   float* __restrict__ svlos, float2* __restrict__ sem,
-                                                  int32_t* __restrict__ vert) {
+                                                  int32_t* __restrict__ vert, uint32_t* __restrict__ redo,
+                                                  int redo_cap) {
   const int64_t R = (int64_t)nx * ny;
   const int64_t tid = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   const int nseg = (nz + seglen - 1) / seglen;
@@ -1476,6 +1556,7 @@ __global__ void __launch_bounds__(256) k_tet_walk(const float4* __restrict__ tre
                    pz > tg.hi[2];
     float4 q[4];
     int4 nb;
+    bool flag = false;  // an undecided test at the end of the walk, or the step cap
     if (!outside) {
       const bool warm = t >= 0;
       if (!warm) t = tet_start(tg, tstart, px, py, pz);
@@ -1503,85 +1584,139 @@ __global__ void __launch_bounds__(256) k_tet_walk(const float4* __restrict__ tre
           }
           t = (steps == 0 && warm) ? tet_start(tg, tstart, px, py, pz) : ivec(nb, inf);
         } else {
-          // faces opposite v0..v3: (2,1,3), (0,2,3), (1,0,3), (0,1,2)
-          int next = -1;
-          if (orient_d(q[2], q[1], q[3], px, py, pz) < 0) next = nb.x;
-          else if (orient_d(q[0], q[2], q[3], px, py, pz) < 0) next = nb.y;
-          else if (orient_d(q[1], q[0], q[3], px, py, pz) < 0) next = nb.z;
-          else if (orient_d(q[0], q[1], q[2], px, py, pz) < 0) next = nb.w;
-          if (next < 0) break;
+          // faces opposite v0..v3: (2,1,3), (0,2,3), (1,0,3), (0,1,2); the walk stops in a tet no
+          // face of which has the sample strictly beyond it, which is certain only when no test
+          // there was undecided (0)
+          int next = -1, o;
+          bool und = false;
+          if ((o = orient_d(q[2], q[1], q[3], px, py, pz)) < 0) {
+            next = nb.x;
+          } else {
+            und = o == 0;
+            if ((o = orient_d(q[0], q[2], q[3], px, py, pz)) < 0) {
+              next = nb.y;
+            } else {
+              und |= o == 0;
+              if ((o = orient_d(q[1], q[0], q[3], px, py, pz)) < 0) {
+                next = nb.z;
+              } else {
+                und |= o == 0;
+                if ((o = orient_d(q[0], q[1], q[2], px, py, pz)) < 0) next = nb.w;
+                else und |= o == 0;
+              }
+            }
+          }
+          if (next < 0) {
+            flag = und;
+            break;
+          }
           t = next;
         }
         if (++steps > kTetMaxSteps) {  // never expected (the walk on a Delaunay triangulation ends)
           outside = true;
+          flag = true;
           t = -1;
           break;
         }
       }
     }
-    if (outside) {
-      if (vert) vert[sidx] = -1;
-      if constexpr (RS4 > 0) write_sample<RS4>(-1, rec, ri, nlines, S, sidx, cb, sb, cl, sl, svlos, sem);
-      continue;
+    if (flag) {
+      // the filtered tests could not place the sample: k_tet_redo walks it again exactly
+      const unsigned e = atomicAdd(&redo[0], 1u);
+      if (e < (unsigned)redo_cap) redo[2 + e] = (uint32_t)sidx;
+      else atomicAdd(&redo[1], 1u);  // left as the filtered walk placed it; counted as unresolved
     }
-    if constexpr (!LINEAR) {
-      // nearest vertex of the cell (first of equals in the cell's vertex order)
-      int best = 0;
-      double bd = INFINITY;
-#pragma unroll
-      for (int m = 0; m < 4; ++m) {
-        const double d0 = (double)q[m].x - px, d1 = (double)q[m].y - py, d2 = (double)q[m].z - pz;
-        const double d = (d0 * d0 + d1 * d1) + d2 * d2;
-        if (d < bd) {
-          bd = d;
-          best = m;
-        }
+    tet_emit<RS4, LINEAR>(q, outside, px, py, pz, sidx, S, cb, sb, cl, sl, rec, ri, nlines, svlos, sem, vert);
+  }
+}
+
+
+// The samples of the redo list, walked again from the start grid with exact orientation tests
+// (fomo::orient_exact_fp); a sample on a face plane (exact 0) stays in the first tet the walk
+// reaches, as in k_tet_walk. stats[0] += samples walked here, stats[1] += samples left unresolved
+// (beyond the list, or at the step cap, which an exact walk on a Delaunay triangulation does not
+// reach); redo[1] holds this view's unresolved count afterwards.
+__device__ __forceinline__ int orient_x(const float4& a, const float4& b, const float4& c, double px, double py,
+                                        double pz) {
+  const double A[3] = {a.x, a.y, a.z}, B[3] = {b.x, b.y, b.z}, Cc[3] = {c.x, c.y, c.z}, P[3] = {px, py, pz};
+  return fomo::orient_exact_fp(A, B, Cc, P);
+}
+
+template <int RS4, bool LINEAR>
+__global__ void __launch_bounds__(64) k_tet_redo(const float4* __restrict__ trec, const int4* __restrict__ tn,
+                                                 const int32_t* __restrict__ tstart, TetGrid tg,
+                                                 const ViewGeom* __restrict__ vgp, int nx, int ny, int nz,
+                                                 const float4* __restrict__ rec, RecInfo ri, int nlines,
+                                                 float* __restrict__ svlos, float2* __restrict__ sem,
+                                                 int32_t* __restrict__ vert, uint32_t* __restrict__ redo, int redo_cap,
+                                                 unsigned long long* __restrict__ stats) {
+  const unsigned cnt = min(redo[0], (unsigned)redo_cap);
+  const int64_t R = (int64_t)nx * ny;
+  const size_t S = (size_t)R * nz;
+  const ViewGeom vg = *vgp;
+  const double minx = vg.ext[0], maxx = vg.ext[1], miny = vg.ext[2], maxy = vg.ext[3], minz = vg.ext[4],
+               maxz = vg.ext[5];
+  const double cb = vg.cb, sb = vg.sb, cl = vg.cl, sl = vg.sl;
+  unsigned unresolved = 0;
+  for (unsigned e = blockIdx.x * blockDim.x + threadIdx.x; e < cnt; e += gridDim.x * blockDim.x) {
+    const size_t sidx = redo[2 + e];
+    const int64_t ray = (int64_t)(sidx % (size_t)R);
+    const int k = (int)(sidx / (size_t)R);
+    const int j = (int)(ray % nx), i = (int)(ray / nx);
+    const double x = double(j) / (nx - 1) * (maxx - minx) + minx;  // fomo-CGAL.cpp:187
+    const double y = double(i) / (ny - 1) * (maxy - miny) + miny;  // :188
+    const double z = nz > 1 ? double(k) / (nz - 1) * (maxz - minz) + minz : minz;  // :189
+    const double px = x * cb * cl + y * sl + z * sb * cl;  // :192
+    const double py = -x * cb * sl + y * cl - z * sb * sl;
+    const double pz = -x * sb + z * cb;
+    bool outside = false;
+    float4 q[4];
+    int t = tet_start(tg, tstart, px, py, pz);
+    for (int steps = 0;; ++steps) {
+      if (steps > kTetMaxSteps) {
+        outside = true;
+        ++unresolved;
+        break;
       }
-      const int r = __float_as_int(best == 0 ? q[0].w : (best == 1 ? q[1].w : (best == 2 ? q[2].w : q[3].w)));
-      if (vert) vert[sidx] = r;
-      if constexpr (RS4 > 0) write_sample<RS4>(r, rec, ri, nlines, S, sidx, cb, sb, cl, sl, svlos, sem);
-    } else {
-      // barycentric weights: volume of the tet with vertex m replaced by the sample, over the volume
-      double w[4], tot = 0.0;
-      (void)orient_d(q[2], q[1], q[3], px, py, pz, &w[0]);
-      (void)orient_d(q[0], q[2], q[3], px, py, pz, &w[1]);
-      (void)orient_d(q[1], q[0], q[3], px, py, pz, &w[2]);
-      (void)orient_d(q[0], q[1], q[2], px, py, pz, &w[3]);
+      const float4* tr = trec + (size_t)t * 4;
+      q[0] = tr[0], q[1] = tr[1], q[2] = tr[2], q[3] = tr[3];
+      const int4 nb = tn[t];
+      int inf = -1;
 #pragma unroll
-      for (int f = 0; f < 4; ++f) {
-        w[f] = fmax(w[f], 0.0);
-        tot += w[f];
-      }
-      if (vert) vert[sidx] = __float_as_int(q[0].w);
-      if constexpr (RS4 > 0) {
-        constexpr int NL = 4 * RS4 - 4;  // lines in a record
-        double vl = 0.0, pk[NL], fw[NL];
-#pragma unroll
-        for (int l = 0; l < NL; ++l) pk[l] = fw[l] = 0.0;
-#pragma unroll
-        for (int m = 0; m < 4; ++m) {
-          const double wm = tot > 0 ? w[m] / tot : 0.25;
-          const float4* rp = rec + (size_t)__float_as_int(q[m].w) * RS4;
-          float f[4 * RS4];
-#pragma unroll
-          for (int qq = 0; qq < RS4; ++qq) {
-            const float4 a = rp[qq];
-            f[4 * qq] = a.x, f[4 * qq + 1] = a.y, f[4 * qq + 2] = a.z, f[4 * qq + 3] = a.w;
-          }
-          vl += wm * (((0.0 + (sb * cl) * (double)f[0]) + (-sb * sl) * (double)f[1]) + cb * (double)f[2]);
-          const double st = rec_st(ri.K[0], f[3]);
-#pragma unroll
-          for (int l = 0; l < NL; ++l) {
-            pk[l] += wm * (double)f[4 + l];
-            fw[l] += wm * (double)rec_fw(ri.K[l], st);
-          }
+      for (int m = 0; m < 4; ++m)
+        if (__float_as_int(q[m].w) < 0) inf = m;
+      if (inf >= 0) {
+        int o;
+        switch (inf) {
+          case 0: o = orient_x(q[2], q[1], q[3], px, py, pz); break;
+          case 1: o = orient_x(q[0], q[2], q[3], px, py, pz); break;
+          case 2: o = orient_x(q[1], q[0], q[3], px, py, pz); break;
+          default: o = orient_x(q[0], q[1], q[2], px, py, pz); break;
         }
-        svlos[sidx] = (float)vl;
-#pragma unroll
-        for (int l = 0; l < NL; ++l)
-          if (l < nlines) sem[(size_t)l * S + sidx] = make_float2((float)pk[l], (float)fw[l]);
+        if (o > 0) {
+          outside = true;
+          break;
+        }
+        t = ivec(nb, inf);
+      } else {
+        int next = -1;
+        if (orient_x(q[2], q[1], q[3], px, py, pz) < 0) next = nb.x;
+        else if (orient_x(q[0], q[2], q[3], px, py, pz) < 0) next = nb.y;
+        else if (orient_x(q[1], q[0], q[3], px, py, pz) < 0) next = nb.z;
+        else if (orient_x(q[0], q[1], q[2], px, py, pz) < 0) next = nb.w;
+        if (next < 0) break;
+        t = next;
       }
     }
+    tet_emit<RS4, LINEAR>(q, outside, px, py, pz, sidx, S, cb, sb, cl, sl, rec, ri, nlines, svlos, sem, vert);
+  }
+  if (unresolved) {
+    atomicAdd(&redo[1], unresolved);
+    atomicAdd(&stats[1], (unsigned long long)unresolved);
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    atomicAdd(&stats[0], (unsigned long long)cnt);
+    atomicAdd(&stats[1], (unsigned long long)(redo[0] > (unsigned)redo_cap ? redo[0] - redo_cap : 0));
   }
 }
 
@@ -1904,6 +2039,7 @@ struct Dataset {
   DevBuf trec;            // 4 float4 per tet: vertex {x, y, z, sorted position as int bits (-1 = infinite)}
   DevBuf tn;              // int4 per tet: neighbour opposite each vertex
   DevBuf tstart;          // int32 per start-grid cell: a finite tet near the cell centre
+  DevBuf tetstats;        // uint64 {samples walked again with exact tests, samples left unresolved}
   TetGrid tk{};           // start grid geometry
   double tet_build_s = 0;
   std::atomic<int> inuse{0};  // batches and queued/running server requests reading this dataset
@@ -2247,12 +2383,14 @@ int build_tets(Dataset& ds) {
   }
   DevBuf tv;
   if (tv.alloc(sizeof(int4) * hv.size()) || ds.trec.alloc(4 * sizeof(float4) * hv.size()) ||
-      ds.tn.alloc(sizeof(int4) * hn.size()) || ds.tstart.alloc(sizeof(int32_t) * ng)) {
+      ds.tn.alloc(sizeof(int4) * hn.size()) || ds.tstart.alloc(sizeof(int32_t) * ng) ||
+      ds.tetstats.ensure(2 * sizeof(unsigned long long))) {
     ds.trec.release();
     ds.tn.release();
     ds.tstart.release();
     return fail(FOMO_ERR_NOMEM, "device allocation failed (tets)");
   }
+  HIPOK(hipMemsetAsync(ds.tetstats.p, 0, 2 * sizeof(unsigned long long), s));
   HIPOK(hipMemcpyAsync(tv.p, hv.data(), sizeof(int4) * hv.size(), hipMemcpyHostToDevice, s));
   HIPOK(hipMemcpyAsync(ds.tn.p, hn.data(), sizeof(int4) * hn.size(), hipMemcpyHostToDevice, s));
   HIPOK(hipMemcpyAsync(ds.tstart.p, start.data(), sizeof(int32_t) * ng, hipMemcpyHostToDevice, s));
@@ -2774,10 +2912,21 @@ void enqueue_nn(hipStream_t s, const Dataset& ds, const ViewGeom* geom, int nx, 
 
 // Tetra path for every sample of a view: per-sample vlos / emissivities like enqueue_nn, and the
 // chosen vertex (sorted position, -1 outside the hull) in vert when given.
+// Entries of a view's redo list (k_tet_walk) held in tet_redo_bytes.
+constexpr int64_t kTetRedoMax = 1 << 16;
+int tet_redo_cap(int64_t S) {
+  return S > (int64_t)UINT32_MAX ? 0 : (int)std::max<int64_t>(0, std::min<int64_t>(S - 2, kTetRedoMax));
+}
+int64_t tet_redo_bytes(int64_t S) { return (int64_t)sizeof(uint32_t) * (2 + tet_redo_cap(S)); }
+
+// redo: tet_redo_bytes(R * nz) of scratch for the samples the filtered walk cannot place.
 int enqueue_tet(hipStream_t s, const Dataset& ds, const ViewGeom* geom, int nx, int ny, int nz, bool linear,
-                const float4* rec, int rs4, const RecInfo& ri, int nlines, float* svlos, float2* sem, int32_t* vert) {
+                const float4* rec, int rs4, const RecInfo& ri, int nlines, float* svlos, float2* sem, int32_t* vert,
+                uint32_t* redo) {
   if (ds.ntets <= 0) return fail(FOMO_ERR_STATE, "dataset has no triangulation (fomo_dataset_triangulate)");
   const int64_t R = (int64_t)nx * ny;
+  const int redo_cap = tet_redo_cap(R * nz);
+  k_clear_u8<<<1, 64, 0, s>>>(reinterpret_cast<uint8_t*>(redo), 8);
   // split rays into segments only as far as needed to keep kTetThreads walks in flight
   const int64_t want = (R * nz + kTetThreads - 1) / kTetThreads;
   const int seglen = (int)std::max<int64_t>(1, std::min<int64_t>({want, (int64_t)kTetMaxSegment, (int64_t)nz}));
@@ -2786,20 +2935,30 @@ int enqueue_tet(hipStream_t s, const Dataset& ds, const ViewGeom* geom, int nx, 
   ProfScope ps(linear ? "k_tet_walk_linear" : "k_tet_walk", s);
 #define FOMO_TET_ARGS                                                                                        \
   ds.trec.as<float4>(), ds.tn.as<int4>(), ds.tstart.as<int32_t>(), ds.tk, geom, nx, ny, nz, seglen, rec, ri, \
-      nlines, svlos, sem, vert
+      nlines, svlos, sem, vert, redo, redo_cap
+#define FOMO_REDO_ARGS                                                                                    \
+  ds.trec.as<float4>(), ds.tn.as<int4>(), ds.tstart.as<int32_t>(), ds.tk, geom, nx, ny, nz, rec, ri, nlines, \
+      svlos, sem, vert, redo, redo_cap, ds.tetstats.as<unsigned long long>()
+#define FOMO_TET_LAUNCH(RS, LIN)                                       \
+  do {                                                                 \
+    k_tet_walk<RS, LIN><<<nb, 256, 0, s>>>(FOMO_TET_ARGS);             \
+    k_tet_redo<RS, LIN><<<kTetRedoBlocks, 64, 0, s>>>(FOMO_REDO_ARGS); \
+  } while (0)
   if (!rec) {
     if (linear) return fail(FOMO_ERR_ARG, "linear interpolation needs per-point records");
-    k_tet_walk<0, false><<<nb, 256, 0, s>>>(FOMO_TET_ARGS);
+    FOMO_TET_LAUNCH(0, false);
   } else if (rs4 == 2) {
-    if (linear) k_tet_walk<2, true><<<nb, 256, 0, s>>>(FOMO_TET_ARGS);
-    else k_tet_walk<2, false><<<nb, 256, 0, s>>>(FOMO_TET_ARGS);
+    if (linear) FOMO_TET_LAUNCH(2, true);
+    else FOMO_TET_LAUNCH(2, false);
   } else if (rs4 == 4) {
-    if (linear) k_tet_walk<4, true><<<nb, 256, 0, s>>>(FOMO_TET_ARGS);
-    else k_tet_walk<4, false><<<nb, 256, 0, s>>>(FOMO_TET_ARGS);
+    if (linear) FOMO_TET_LAUNCH(4, true);
+    else FOMO_TET_LAUNCH(4, false);
   } else {
-    if (linear) k_tet_walk<8, true><<<nb, 256, 0, s>>>(FOMO_TET_ARGS);
-    else k_tet_walk<8, false><<<nb, 256, 0, s>>>(FOMO_TET_ARGS);
+    if (linear) FOMO_TET_LAUNCH(8, true);
+    else FOMO_TET_LAUNCH(8, false);
   }
+#undef FOMO_TET_LAUNCH
+#undef FOMO_REDO_ARGS
 #undef FOMO_TET_ARGS
   HIPOK(hipGetLastError());
   return FOMO_OK;
@@ -3079,8 +3238,9 @@ int enqueue_render(hipStream_t s, const Dataset& ds, const Goft& gc, int mode, d
   enqueue_view_geom(s, ds, mode == MODE_THOMSON ? MODE_THOMSON : MODE_NN, l, b, nx, ny, nz, dn, rb.aux, rb.partial,
                     rb.geom);
   if (mode == MODE_TET || mode == MODE_LINEAR) {
+    // the tetra path leaves rb.idx (S ints) unused: it holds the view's redo list
     if (int rc = enqueue_tet(s, ds, rb.geom, nx, ny, nz, mode == MODE_LINEAR, gc.rec.as<float4>(), gc.rs4, gc.ri, 1,
-                             rb.aux, rb.sem, nullptr))
+                             rb.aux, rb.sem, nullptr, reinterpret_cast<uint32_t*>(rb.idx)))
       return rc;
     LineSet ls = make_lineset({rb.sem}, {rb.out}, {gc.lambda0}, width);
     enqueue_los(s, (int)R, nz, ls, rb.aux, rb.geom, nl, rb.flag);
@@ -3747,6 +3907,22 @@ int fomo_dataset_set_ray_tile(int id, int width) {
   return FOMO_OK;
 }
 
+int fomo_dataset_tet_stats(int id, int64_t* out2) {
+  std::lock_guard<std::recursive_mutex> lk(g_mu);
+  if (int rc = require_init()) return rc;
+  if (!out2) return fail(FOMO_ERR_ARG, "null output");
+  Dataset* ds = find_obj(g_ctx.datasets, id);
+  if (!ds) return fail(FOMO_ERR_NOTFOUND, "unknown dataset id");
+  out2[0] = out2[1] = 0;
+  if (!ds->tetstats.p) return FOMO_OK;
+  unsigned long long v[2];
+  HIPOK(hipMemcpyAsync(v, ds->tetstats.p, sizeof(v), hipMemcpyDeviceToHost, g_ctx.stream));
+  if (int rc = sync_stream()) return rc;
+  out2[0] = (int64_t)v[0];
+  out2[1] = (int64_t)v[1];
+  return FOMO_OK;
+}
+
 int fomo_dataset_set_far_budget(int id, int steps) {
   std::lock_guard<std::recursive_mutex> lk(g_mu);
   if (int rc = require_init()) return rc;
@@ -4183,8 +4359,10 @@ int fomo_tet_vertices(int dataset, double l, double b, int nx, int ny, int nz, i
   HIPOK(vs.aux.ensure(sizeof(float) * std::max<int64_t>(ds->n, S)));
   ViewGeom* geom = vs.geom.as<ViewGeom>();
   enqueue_view_geom(s, *ds, MODE_NN, l, b, nx, ny, nz, 0, vs.aux.as<float>(), vs.partial.as<double>(), geom);
+  HIPOK(vs.sem.ensure(tet_redo_bytes(S)));
+  uint32_t* redo = vs.sem.as<uint32_t>();
   if (int rc = enqueue_tet(s, *ds, geom, nx, ny, nz, false, nullptr, 0, RecInfo{}, 0, nullptr, nullptr,
-                           vs.idx.as<int32_t>()))
+                           vs.idx.as<int32_t>(), redo))
     return rc;
   int32_t* tmp = reinterpret_cast<int32_t*>(vs.aux.p);
   k_idx_to_orig<<<nblk(S), kBlock, 0, s>>>(vs.idx.as<int32_t>(), ds->perm.as<int32_t>(), R, nz, tmp);
@@ -4192,8 +4370,14 @@ int fomo_tet_vertices(int dataset, double l, double b, int nx, int ny, int nz, i
   HIPOK(hipMemcpyAsync(out_vertex, tmp, sizeof(int32_t) * S, hipMemcpyDeviceToHost, s));
   ViewGeom hg;
   HIPOK(hipMemcpyAsync(&hg, geom, sizeof(ViewGeom), hipMemcpyDeviceToHost, s));
+  uint32_t rc2[2] = {0, 0};
+  HIPOK(hipMemcpyAsync(rc2, redo, sizeof(rc2), hipMemcpyDeviceToHost, s));
   if (int rc = sync_stream()) return rc;
-  if (meta) geom_to_meta(hg, meta);
+  if (meta) {
+    geom_to_meta(hg, meta);
+    meta[10] = (double)std::min<uint32_t>(rc2[0], (uint32_t)tet_redo_cap(S));  // walked again exactly
+    meta[11] = (double)rc2[1];                                                 // left unresolved
+  }
   return FOMO_OK;
 }
 

@@ -710,6 +710,8 @@ void FoMo::FoMoObject::render(const std::vector<double> lvec, const std::vector<
         fomo_batch_destroy(bt);
       }
     } else {
+      int64_t tet0[2] = {0, 0};
+      if (tet) fomo_dataset_tet_stats(ds, tet0);
       for (size_t v = 0; v < views.size(); ++v) {
         const double lv = views[v].first, bv = views[v].second;
         int rc = thomson ? fomo_render_thomson(goft, lv, bv, nx, ny, nz, img.data(), m)
@@ -720,6 +722,10 @@ void FoMo::FoMoObject::render(const std::vector<double> lvec, const std::vector<
         if (rc != FOMO_OK) die(std::string("Error: render failed: ") + fomo_last_error());
         emit(img, m, v);
       }
+      int64_t tet1[2] = {0, 0};
+      if (tet && fomo_dataset_tet_stats(ds, tet1) == FOMO_OK && tet1[1] > tet0[1])
+        std::cerr << "Warning: " << (tet1[1] - tet0[1])
+                  << " LOS samples could not be located exactly in the triangulation" << std::endl;
     }
     writer.drain();
   }

@@ -80,6 +80,13 @@ int fomo_dataset_set_ray_tile(int id, int width);
  * samples move to a pass that walks 1024 node ranges of the tree with 32 waves in parallel and
  * merges them (default 2048; 0: never). Speed only; results do not change. */
 int fomo_dataset_set_far_budget(int id, int steps);
+/* Tetra path counts since the triangulation was built: out2[0] = LOS samples whose filtered FP64
+ * walk could not place them (an orientation test within its error bound at the tet where the walk
+ * ended, or the walk's step cap) and that were walked again with exact orientation tests;
+ * out2[1] = samples left unresolved (beyond a view's redo list of 65536, or at the exact walk's
+ * step cap): those keep the filtered walk's tet, or none at the cap. Both 0 before the first tetra
+ * render. Reads a snapshot while other streams may still render. */
+int fomo_dataset_tet_stats(int id, int64_t* out2);
 /* Active subdivision (0 = disabled), lookup cells, candidates summed over the cells (a far group's
    shared list counts once per cell that uses it), cells left to the exact octree query. */
 int fomo_dataset_lookup_info(int id, int* subdiv, int64_t* cells, int64_t* candidates, int64_t* fallback_cells);
@@ -195,7 +202,9 @@ int fomo_autotune(int ngoft, const int* gofts, double l, double b, int nx, int n
 int fomo_dataset_triangulate(int dataset, int64_t* ntets, double* seconds);
 
 /* The tetra path's vertex choice for every LOS sample of one view (original point indices, -1 =
- * outside the convex hull), layout (iy*nx + jx)*nz + k. For parity checks. */
+ * outside the convex hull), layout (iy*nx + jx)*nz + k. For parity checks. meta: 12 doubles, the
+ * 10 of a render, then this view's samples walked again with exact orientation tests and those
+ * left unresolved (see fomo_dataset_tet_stats). */
 int fomo_tet_vertices(int dataset, double l, double b, int nx, int ny, int nz, int32_t* out_vertex,
                       double* meta);
 

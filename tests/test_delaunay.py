@@ -79,6 +79,43 @@ def test_predicates_match_exact_rationals(tmp_path):
     assert 0 in want and 1 in want and -1 in want  # the degenerate and both signs are exercised
 
 
+def test_tetra_walk_exact_orient_matches_rationals(tmp_path):
+    """The tetra walk's exact orientation test (float vertices, double sample point; the fallback
+    for samples the filtered FP64 test cannot place) against exact rational arithmetic: points
+    exactly on a face plane, a few ulps off it, with coordinates of mixed magnitudes (differences
+    of floats that do not fit one double), and random ones."""
+    exe = str(tmp_path / "pred_test")
+    subprocess.run(["g++", "-O2", "-std=c++17", "-ffp-contract=off", "-Wall", "-Werror", "-o", exe,
+                    os.path.join(ROOT, "tests", "cpp", "pred_test.cpp")], check=True)
+    rng = np.random.default_rng(11)
+    cases, want = [], []
+    for t in range(600):
+        scale = 10.0 ** rng.integers(-6, 7) if t % 3 == 0 else 10.0
+        a, b, c = (rng.normal(size=(3, 3)) * scale).astype(np.float32).astype(np.float64)
+        if t % 5 == 0:
+            a[0] = np.float32(rng.normal() * 1e-12)  # coordinates whose float differences overflow a double
+        if t % 4 == 0:
+            # exactly on the plane: a lattice plane z = const with float vertices and a double point
+            zc = float(np.float32(rng.normal() * scale))
+            a[2] = b[2] = c[2] = zc
+            d = np.array([rng.normal() * scale, rng.normal() * scale, zc])
+        else:
+            u, v = rng.random(2)
+            d = a + u * (b - a) + v * (c - a)
+            if t % 4 == 1:
+                d = d + np.spacing(d) * rng.integers(-3, 4, 3)
+            elif t % 4 == 3:
+                d = rng.normal(size=3) * scale
+        pts = [a, b, c, d]
+        cases.append("F " + " ".join(repr(float(x)) for q in pts for x in q))
+        want.append(_exact_orient(pts))
+    r = subprocess.run([exe], input="\n".join(cases) + "\n", capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    got = [int(v) for v in r.stdout.split()]
+    assert got == want
+    assert want.count(0) > 50 and 1 in want and -1 in want
+
+
 @pytest.mark.parametrize("n,seed", [(10, 1), (300, 2), (20000, 3)])
 def test_general_position_equals_qhull(n, seed):
     p = np.random.default_rng(seed).random((n, 3)).astype(np.float32)

@@ -90,3 +90,57 @@ def test_tet_server_equals_sync(loop):
         assert np.array_equal(res[k][0], sync[k % len(reqs)][0])
     # the tetra image differs from the nearest-neighbour one of the same view (different method)
     assert not np.array_equal(sync[0][0], sync[3][0])
+
+
+def _orient_q(a, b, c, d):
+    """Exact sign of det[b - a; c - a; d - a] in rationals."""
+    from fractions import Fraction
+    a, b, c, d = ([Fraction(float(v)) for v in q] for q in (a, b, c, d))
+    m = [[b[i] - a[i] for i in range(3)], [c[i] - a[i] for i in range(3)], [d[i] - a[i] for i in range(3)]]
+    det = (m[0][0] * (m[1][1] * m[2][2] - m[1][2] * m[2][1]) - m[0][1] * (m[1][0] * m[2][2] - m[1][2] * m[2][0])
+           + m[0][2] * (m[1][0] * m[2][1] - m[1][1] * m[2][0]))
+    return (det > 0) - (det < 0)
+
+
+def test_samples_on_faces_are_placed_exactly(engine):
+    """A structured grid seen along an axis at half the grid spacing puts LOS samples exactly on
+    tet faces and edges, where the filtered FP64 orientation test cannot decide. Those samples are
+    walked again with exact tests (counted in the view meta and the dataset's counts; none left
+    unresolved), and every sample's vertex is a nearest vertex of a tet that contains the sample
+    exactly (closed), checked in rationals on the host builder's copy of the same triangulation
+    (Delaunay with symbolic perturbation is unique). Which of several such tets the walk reports
+    is not fixed by the reference (CGAL's locate returns one of the cells incident to the face)."""
+    d = F.gen_loop_cube(8, seed=2)
+    ds = F.Dataset(d["x"], d["y"], d["z"], [d["rho"], d["T"], d["vx"], d["vy"], d["vz"]])
+    ds.triangulate()
+    assert ds.tet_stats() == {"exact_rewalks": 0, "unresolved": 0}
+    l, b, nx, ny, nz = 0.0, np.pi / 2, 15, 15, 9
+    vert, meta = F.tet_vertices(ds, l, b, nx, ny, nz)
+    print(f"structured 8^3, axis view: {int(meta[10])} samples walked again exactly, {int(meta[11])} unresolved",
+          flush=True)
+    assert meta[10] > 0 and meta[11] == 0
+    st = ds.tet_stats()
+    assert st["exact_rewalks"] == int(meta[10]) and st["unresolved"] == 0
+    P = np.stack([d["x"], d["y"], d["z"]], 1).astype(np.float64)
+    tets = F.delaunay(d["x"], d["y"], d["z"])
+    inc = [[] for _ in range(P.shape[0])]
+    for t, tv in enumerate(tets):
+        for v in tv:
+            inc[v].append(t)
+    px, py, pz = (a.ravel() for a in O.view_samples(meta[:6], l, b, nx, ny, nz, cgal=True))
+    assert (vert >= 0).mean() > 0.3
+    for s in np.flatnonzero(vert >= 0):
+        p = np.array([px[s], py[s], pz[s]])
+        v = int(vert[s])
+        ok = False
+        for t in inc[v]:
+            q = P[tets[t]]
+            o = _orient_q(*q)
+            sub = [_orient_q(*[p if m == i else q[m] for m in range(4)]) for i in range(4)]
+            if all(si * o >= 0 for si in sub):
+                dist = ((q - p) ** 2).sum(1)
+                if dist[list(tets[t]).index(v)] <= dist.min():
+                    ok = True
+                    break
+        assert ok, f"sample {s} at {p}: vertex {v} is not a nearest vertex of a tet that contains it"
+    ds.close()
