@@ -42,9 +42,9 @@ PEAK_FP64 = 78.6e12  # FLOP/s, MI355X dense FP64 (vector and matrix peaks coinci
 # Gaussian term and its accumulation, about 8 FLOP besides the exp (fomo-nearestneighbour.cpp:200)
 FLOP_PER_W = 8.0
 # HBM bytes per launch (read + write) of the default workload's kernels, from two rocprofv3 --pmc
-# runs (FETCH_SIZE x 2.000, WRITE_SIZE x 1.000, the factors a 1 GiB copy kernel gives):
-# profiles/batched_render_256cube_counters.md
-MEASURED_TRAFFIC = {"k_nn_los": 1.445e9}
+# runs (FETCH_SIZE x 2.000, WRITE_SIZE x 1.000, the factors a 1 GiB copy kernel gives), per index
+# kind of the cube: profiles/batched_render_256cube_counters.md
+MEASURED_TRAFFIC = {"scattered": {"k_nn_los": 1.445e9}, "rectilinear": {}}
 
 
 def log(*a):
@@ -109,7 +109,10 @@ def main():
     t0 = time.time()
     ds = F.Dataset(d["x"], d["y"], d["z"], [d["rho"], d["T"], d["vx"], d["vy"], d["vz"]], cell_scale=args.cell_scale)
     n, dims, maxc = ds.info()
-    log(f"dataset upload + index: {time.time() - t0:.2f}s, grid {dims}, max cell occupancy {maxc}")
+    rect = ds.rect_axes()
+    index_kind = "rectilinear" if rect else "scattered"
+    log(f"dataset upload + index: {time.time() - t0:.2f}s, {index_kind} index "
+        f"({'axes ' + str(rect) if rect else f'grid {dims}, max cell occupancy {maxc}'})")
     ems = [F.Emission.chianti(ds, t, lambda0, atw) for t in tabs]
 
     nv = args.nviews
@@ -177,7 +180,7 @@ def main():
         # HBM bytes per launch measured with rocprofv3 FETCH_SIZE / WRITE_SIZE (calibrated on a plain
         # copy kernel) for this exact workload: profiles/batched_render_256cube_counters.md
         default_cfg = (m, nx, nz, nl, len(ems)) == (256, 256, 128, 60, 4)
-        traffic = MEASURED_TRAFFIC.get(dom) if default_cfg else None
+        traffic = MEASURED_TRAFFIC[index_kind].get(dom) if default_cfg else None
         share = {k: stats[k][1] / max(1e-9, sum(v[1] for v in stats.values())) for k in stats}
         if dom == "k_nn_los":
             # the fused pass is bound by the 128-byte lines its NN queries fetch at random (lookup
@@ -264,6 +267,35 @@ def main():
     linfo = ds.lookup_info()
     log(f"index rebuild: {t_index * 1e3:.1f} ms, lookup {linfo}")
 
+    # ---- the same batched step on the scattered index (cell list + candidate lookup grid), the one
+    #      every dataset that is not a product grid uses; the headline runs on the index the cube
+    #      gets by default
+    scattered = None
+    if rect:
+        t0 = time.perf_counter()
+        ds.set_rectilinear(False)
+        t_scat = time.perf_counter() - t0
+        bt2 = F.Batch(ems, ls, bs, nx, ny, nz, nl, width)
+        for _ in range(2):
+            bt2.run()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            bt2.launch()
+        bt2.sync()
+        dt2 = time.perf_counter() - t0
+        F.profile_reset()
+        bt2.profile()
+        c2, tot2 = F.profile_stats("k_nn_los")
+        bt2.close()
+        scattered = {"ms_per_step": dt2 / args.steps * 1e3, "value": W_step * args.steps / dt2,
+                     "index_build_ms": t_scat * 1e3, "k_nn_los_ms": tot2 / c2 if c2 else None,
+                     "lookup": ds.lookup_info(),
+                     "what": "the batched step with the cube indexed as scattered points (its product-grid axes "
+                             "not used)"}
+        log(f"scattered index: {scattered['ms_per_step']:.2f} ms/step, index {t_scat * 1e3:.0f} ms, k_nn_los "
+            f"{scattered['k_nn_los_ms']} ms")
+        ds.set_rectilinear(True)
+
     # ---- serving: the request mix of the serving configuration (NN, CGAL/Linear tetra and Thomson
     #      requests over resident cubes, this 256^3 cube among them), Poisson arrivals at 0.5, 0.8
     #      and 0.95 x the saturated request rate; latency = submit -> image in the caller's buffer
@@ -330,7 +362,9 @@ def main():
         "index_build_ms": t_index * 1e3,
         "copy_probe_GBps": copy_gbps,
         "random_line_probe_GBps": gather_gbps,
+        "index": {"kind": index_kind, "axes": list(rect) if rect else None},
         "lookup": linfo,
+        "scattered_index": scattered,
         "autotune": tune,
         "self_check": check,
         "roofline": roof,

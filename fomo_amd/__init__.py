@@ -48,6 +48,8 @@ _SIGS = {
     "fomo_dataset_tet_stats": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(ctypes.c_int64)]),
     "fomo_dataset_lookup_info": (ctypes.c_int, [ctypes.c_int, _c_int_p, _c_i64_p, _c_i64_p, _c_i64_p]),
     "fomo_dataset_lookup_groups": (ctypes.c_int, [ctypes.c_int, _c_i64_p]),
+    "fomo_dataset_set_rectilinear": (ctypes.c_int, [ctypes.c_int, ctypes.c_int]),
+    "fomo_dataset_rect_info": (ctypes.c_int, [ctypes.c_int, _c_int_p]),
     "fomo_table_read": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int, _c_dbl_p, _c_dbl_p,
                                        ctypes.c_char_p, ctypes.c_int, _c_int_p, _c_int_p, _c_flt_p, _c_flt_p,
                                        _c_flt_p]),
@@ -282,6 +284,17 @@ class Dataset:
         _check(lib().fomo_dataset_lookup_info(self.id, ctypes.byref(sd), ctypes.byref(cells), ctypes.byref(cand),
                                               ctypes.byref(fb)))
         return dict(subdiv=sd.value, cells=cells.value, candidates=cand.value, fallback_cells=fb.value)
+
+    def set_rectilinear(self, on):
+        """Index a product-grid dataset by its axes (on, the default) or as scattered points (off).
+        Results do not change."""
+        _check(lib().fomo_dataset_set_rectilinear(self.id, 1 if on else 0))
+
+    def rect_axes(self):
+        """Values per axis of a rectilinear dataset's grid, or None for a scattered one."""
+        a = (ctypes.c_int * 3)()
+        _check(lib().fomo_dataset_rect_info(self.id, a))
+        return tuple(a) if a[0] else None
 
     def lookup_groups(self):
         """Far and outer groups of the lookup grid and how many hold a shared candidate list."""

@@ -1,8 +1,9 @@
 """Build recipe for the gfx950 engine library and the FoMo C++ facade.
 
 Produces, in-tree:
-  fomo_amd/libfomo_hip.so  HIP kernels + C ABI (include/fomo_hip.h); fomo_device.hip and the eight
-                           k_nn_los instantiation units of fomo_nnlos_part.hip compile in parallel
+  fomo_amd/libfomo_hip.so  HIP kernels + C ABI (include/fomo_hip.h); fomo_device.hip and the sixteen
+                           k_nn_los instantiation units of fomo_nnlos_part.hip (eight lambda chunk
+                           groups x two index kinds) compile in parallel
   fomo_amd/libFoMo.so      FoMo.h C++ API (FoMoObject & co.) on top of libfomo_hip.so
 
 Run `python -m fomo_amd.build` or call build_all(). hipcc cross-compiles without a GPU.
@@ -34,7 +35,7 @@ def _run(cmd):
     subprocess.check_call(cmd)
 
 
-NNLOS_GROUPS = 8  # translation units of fomo_nnlos_part.hip (k_nn_los instantiations)
+NNLOS_GROUPS = 8  # lambda chunk groups of fomo_nnlos_part.hip (k_nn_los instantiations), per index kind
 
 
 def _run_all(cmds):
@@ -52,11 +53,13 @@ def build_engine():
     os.makedirs(BUILD, exist_ok=True)
     dev_o = os.path.join(BUILD, "fomo_device.o")
     host_o = os.path.join(BUILD, "fomo_host.o")
-    part_o = [os.path.join(BUILD, f"fomo_nnlos_part{g}.o") for g in range(NNLOS_GROUPS)]
+    parts = [(g, r) for g in range(NNLOS_GROUPS) for r in (0, 1)]
+    part_o = [os.path.join(BUILD, f"fomo_nnlos_part{g}_{r}.o") for g, r in parts]
     cmds = [[HIPCC] + HIP_FLAGS + ["-c", os.path.join(CSRC, "fomo_device.hip"), "-o", dev_o],
             ["g++"] + HOST_FLAGS + ["-c", os.path.join(CSRC, "fomo_host.cpp"), "-o", host_o]]
-    cmds += [[HIPCC] + HIP_FLAGS + [f"-DFOMO_NNLOS_GROUP={g}", "-c", os.path.join(CSRC, "fomo_nnlos_part.hip"), "-o",
-                                    part_o[g]] for g in range(NNLOS_GROUPS)]
+    cmds += [[HIPCC] + HIP_FLAGS + [f"-DFOMO_NNLOS_GROUP={g}", f"-DFOMO_NNLOS_RECT={r}", "-c",
+                                    os.path.join(CSRC, "fomo_nnlos_part.hip"), "-o", o]
+             for (g, r), o in zip(parts, part_o)]
     _run_all(cmds)
     out = os.path.join(PKG, "libfomo_hip.so")
     _run([HIPCC, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", out, dev_o, host_o] + part_o + ["-pthread"])

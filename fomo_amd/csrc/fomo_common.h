@@ -11,6 +11,7 @@
 
 #include "lookup_geom.h"
 #include "octree.h"
+#include "rect_geom.h"
 #include "spectral_exp.h"
 
 // Physical constants (fomo-CHIANTI.cpp:16-19, fomo-thomson.cpp:31-34, FoMo-internal.h:4; GSL
@@ -55,6 +56,15 @@ struct LookupInfo {
   // far groups whose cells all use the group's list (ghdr = nullptr: none): read before the record
   int gcell[3], gdims[3];    // lookup cells per group edge, groups per axis
   const int2* ghdr;          // per group: (start in lcand, count), count 0 when a cell has its own
+  // rectilinear datasets (rect != 0; no candidate grid, enabled == 0): the points are the product
+  // grid of the axis values rv[a], and a point's sorted position is its grid position
+  // i + n0 (j + n1 k) (rect_geom.h)
+  int rect;
+  int rn[3], rnb[3];         // values and bins per axis
+  double ro[3], rinv[3];     // bin origin and inverse bin width per axis
+  double rgap2[3];           // square of the smallest gap between consecutive values, rounded down
+  const float* rv[3];        // axis values, ascending
+  const float4* rbin[3];     // bin tables: {index as int bits, v[index], v[index + 1], v[index + 2]}
 };
 
 // Start grid of the tetra walk (a finite tet near each cell centre) and the points' bounding box,
@@ -247,11 +257,37 @@ __device__ __forceinline__ int outer_lookup(const float4* __restrict__ pts, cons
   return lookup_pick(pts, lcand, nullptr, sl, t0, t1, t2, lo0, lo1, lo2, hi0, hi1, hi2);
 }
 
+// Rectilinear datasets, first step: the bin table entries of the target (one 16-byte read per
+// axis, from tables small enough to stay in the caches).
+__device__ __forceinline__ void rect_issue(const LookupInfo& lk, float t0, float t1, float t2, float4 (&e)[kSlots]) {
+  e[0] = lk.rbin[0][fomo::rect_bin(lk.ro[0], lk.rinv[0], lk.rnb[0], t0)];
+  e[1] = lk.rbin[1][fomo::rect_bin(lk.ro[1], lk.rinv[1], lk.rnb[1], t1)];
+  e[2] = lk.rbin[2][fomo::rect_bin(lk.ro[2], lk.rinv[2], lk.rnb[2], t2)];
+}
+
+// Rectilinear datasets, second step: the nearest point's sorted position if it lies strictly
+// inside the search box, else -2 (rect_pick).
+__device__ __forceinline__ int rect_resolve(const float4* __restrict__ pts, const LookupInfo& lk,
+                                            const float4 (&e)[kSlots], const fomo::SearchBox& box) {
+  return fomo::rect_pick(
+      e[0], e[1], e[2], lk.rn[0], lk.rn[1], lk.rn[2], lk.rgap2[0], lk.rgap2[1], lk.rgap2[2], box,
+      [&](int a, int k) { return a == 0 ? lk.rv[0][k] : (a == 1 ? lk.rv[1][k] : lk.rv[2][k]); },
+      [&](int64_t gp) { return __float_as_int(pts[gp].w); });
+}
+
 // Fast path: the unconstrained nearest point from the target's lookup cell. Returns its sorted
 // position if it lies strictly inside the search box, else -2 (the caller runs the exact query).
 __device__ __forceinline__ int nn_lookup(const float4* __restrict__ pts, const float4* __restrict__ lslot,
                                          const float4* __restrict__ lcand, const LookupInfo& lk, float t0, float t1,
                                          float t2, float lo0, float lo1, float lo2, float hi0, float hi1, float hi2) {
+  if (lk.rect) {
+    float4 e[kSlots];
+    rect_issue(lk, t0, t1, t2, e);
+    fomo::SearchBox box;
+    box.t0 = t0, box.t1 = t1, box.t2 = t2;
+    box.lo0 = lo0, box.lo1 = lo1, box.lo2 = lo2, box.hi0 = hi0, box.hi1 = hi1, box.hi2 = hi2;
+    return rect_resolve(pts, lk, e, box);
+  }
   const int64_t L = lookup_cell(lk, t0, t1, t2);
   if (L < 0) return lk.ohdr ? outer_lookup(pts, lcand, lk, t0, t1, t2, lo0, lo1, lo2, hi0, hi1, hi2) : -2;
   if (lk.ghdr) {
@@ -487,7 +523,7 @@ __device__ __forceinline__ double group_bcast(double x) {
 // whose recurrence anchors would under- or overflow is flagged (flags[line * R + ray]) and redone
 // by the DIRECT instantiation, launched right after, with one exp per sample and lambda pixel.
 // ------------------------------------------------------------------------------------
-template <int RS4, int CH, int G, bool DIRECT, int SL, int NP = 1>
+template <int RS4, int CH, int G, bool DIRECT, int SL, int NP = 1, bool RECT = false>
 __global__ void __launch_bounds__(256) k_nn_los(const float4* __restrict__ pts, GridInfo g,
                                                 const float4* __restrict__ lslot, const float4* __restrict__ lcand,
                                                 LookupInfo lk, const float4* __restrict__ mpts,
@@ -574,6 +610,13 @@ __global__ void __launch_bounds__(256) k_nn_los(const float4* __restrict__ pts, 
     box.t0 = (float)p0, box.t1 = (float)p1, box.t2 = (float)p2;  // :171
     box.lo0 = (float)(p0 - md), box.lo1 = (float)(p1 - md), box.lo2 = (float)(p2 - md);  // :179
     box.hi0 = (float)(p0 + md), box.hi1 = (float)(p1 + md), box.hi2 = (float)(p2 + md);
+    if constexpr (RECT) {
+      // rectilinear dataset: the bin table entries of the target's three axes
+      cell = true;
+      rect_issue(lk, box.t0, box.t1, box.t2, sl);
+      sl[3] = make_float4(0.f, 0.f, 0.f, 0.f);
+      return;
+    }
     if (!lk.enabled) return;
     const int64_t L = lookup_cell(lk, box.t0, box.t1, box.t2);
     if (L < 0) return;
@@ -586,9 +629,14 @@ __global__ void __launch_bounds__(256) k_nn_los(const float4* __restrict__ pts, 
   const auto query_resolve = [&](const SearchBox& box, const float4* rp, const float4 (&sl)[kSlots], bool act,
                                  bool cell) -> int {
     if (!act) return -1;
-    int r = cell ? lookup_pick(pts, lcand, rp, sl, box.t0, box.t1, box.t2, box.lo0, box.lo1, box.lo2, box.hi0,
-                               box.hi1, box.hi2)
-                 : -2;
+    int r = -2;
+    if (cell) {
+      if constexpr (RECT)
+        r = rect_resolve(pts, lk, sl, box);
+      else
+        r = lookup_pick(pts, lcand, rp, sl, box.t0, box.t1, box.t2, box.lo0, box.lo1, box.lo2, box.hi0, box.hi1,
+                        box.hi2);
+    }
     if (r == -2) {
       if (!(box.hi0 > g.bmin[0] && box.lo0 < g.bmax[0] && box.hi1 > g.bmin[1] && box.lo1 < g.bmax[1] &&
             box.hi2 > g.bmin[2] && box.lo2 < g.bmax[2])) {
@@ -852,8 +900,9 @@ enum NnLosVariant {
 };
 
 // Launches the variant with lambda chunk CH, first with DIRECT = false, then the DIRECT pass for
-// the flagged rays. Explicitly instantiated in fomo_nnlos_part.hip for CH = 4, 8, ..., 64.
-template <int CH>
+// the flagged rays; RECT: the dataset is rectilinear (lk.rect), else it has a candidate lookup
+// grid or none. Explicitly instantiated in fomo_nnlos_part.hip for CH = 4, 8, ..., 64.
+template <int CH, bool RECT>
 void nn_los_launch(int variant, unsigned blocks, hipStream_t s, const NnLosArgs& a);
 
 }  // namespace fomo_dev

@@ -18,6 +18,7 @@
 #include <cstring>
 #include <rocprim/device/device_radix_sort.hpp>
 #include <rocprim/device/device_scan.hpp>
+#include <rocprim/device/device_select.hpp>
 
 #include <algorithm>
 #include <atomic>
@@ -152,6 +153,33 @@ __global__ void k_reduce_partials(const double* __restrict__ partial, int np, do
 #pragma unroll
     for (int a = 0; a < 6; ++a) v[a] = (a & 1) ? fmax(v[a], partial[6 * i + a]) : fmin(v[a], partial[6 * i + a]);
   block_minmax6(v, out6);
+}
+
+// Grid position of every point on a candidate product grid (axis values v0, v1, v2 ascending):
+// perm[gid] = original index. Sets *bad when a coordinate is not an axis value or two points
+// share a grid position (perm must be all -1 on entry).
+__device__ __forceinline__ int axis_index(const float* __restrict__ v, int n, float c) {
+  int a = 0, b = n;  // first index with v[idx] >= c
+  while (a < b) {
+    const int m = (a + b) >> 1;
+    if (v[m] < c) a = m + 1;
+    else b = m;
+  }
+  return a < n && v[a] == c ? a : -1;
+}
+__global__ void k_rect_map(const float* __restrict__ x, const float* __restrict__ y, const float* __restrict__ z,
+                           int64_t n, const float* __restrict__ v0, const float* __restrict__ v1,
+                           const float* __restrict__ v2, int n0, int n1, int n2, int32_t* __restrict__ perm,
+                           int* __restrict__ bad) {
+  for (int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < n; p += (int64_t)gridDim.x * blockDim.x) {
+    const int i = axis_index(v0, n0, x[p]), j = axis_index(v1, n1, y[p]), k = axis_index(v2, n2, z[p]);
+    if (i < 0 || j < 0 || k < 0) {
+      atomicOr(bad, 1);
+      continue;
+    }
+    const int64_t gid = i + (int64_t)n0 * (j + (int64_t)n1 * k);
+    if (atomicCAS(&perm[gid], -1, (int32_t)p) != -1) atomicOr(bad, 1);
+  }
 }
 
 __global__ void k_cell_keys(const float* __restrict__ x, const float* __restrict__ y, const float* __restrict__ z,
@@ -806,7 +834,7 @@ __device__ __forceinline__ void nn_sample(const float4* __restrict__ pts, const 
   const bool may_hold = (float)(p0 + md0) > g.bmin[0] && (float)(p0 - md0) < g.bmax[0] &&
                         (float)(p1 + md0) > g.bmin[1] && (float)(p1 - md0) < g.bmax[1] &&
                         (float)(p2 + md0) > g.bmin[2] && (float)(p2 - md0) < g.bmax[2];
-  if (lk.enabled && may_hold) {
+  if ((lk.enabled || lk.rect) && may_hold) {
     const double md = vg.md;
     r = nn_lookup(pts, lslot, lcand, lk, (float)p0, (float)p1, (float)p2, (float)(p0 - md), (float)(p1 - md),
                   (float)(p2 - md), (float)(p0 + md), (float)(p1 + md), (float)(p2 + md));
@@ -2028,6 +2056,15 @@ struct Dataset {
   DevBuf mpts;         // float4 (x, y, z, sorted position) in Morton order
   DevBuf nlo, nhi;     // node boxes, all levels
   DevBuf segbox;       // kDeepSegs bounding boxes of the node ranges of k_nn_deep (lo[], then hi[])
+  // rectilinear datasets: the points form a product grid of distinct axis values, each
+  // combination once; the sorted order is then the grid order and no cell list or candidate grid
+  // is built (the NN query reads the axes: rect_geom.h)
+  bool allow_rect = true;  // detect product grids when indexing (fomo_dataset_set_rectilinear)
+  bool rect = false;
+  int rect_n[3] = {0, 0, 0}, rect_nb[3] = {0, 0, 0};
+  double rect_o[3] = {0, 0, 0}, rect_inv[3] = {0, 0, 0}, rect_gap2[3] = {0, 0, 0};
+  DevBuf rv;           // axis values, the three axes one after another
+  DevBuf rbin;         // float4 bin tables, the three axes one after another
   int64_t ncand = 0, nlcells = 0, nfallback = 0;
   // far and outer groups of the lookup grid: groups, groups with a candidate list, far groups
   // whose cells all take the list (group headers)
@@ -2540,6 +2577,23 @@ int build_lookup(Dataset& ds) {
   ds.ncand = ds.nlcells = ds.nfallback = 0;
   ds.far_groups = ds.far_lists = ds.full_groups = ds.outer_groups = ds.outer_lists = 0;
   if (ds.lookup_subdiv <= 0) return FOMO_OK;
+  if (ds.rect) {
+    LookupInfo& lk = ds.lk;
+    lk.rect = 1;
+    int64_t vo = 0, bo = 0;
+    for (int a = 0; a < 3; ++a) {
+      lk.rn[a] = ds.rect_n[a];
+      lk.rnb[a] = ds.rect_nb[a];
+      lk.ro[a] = ds.rect_o[a];
+      lk.rinv[a] = ds.rect_inv[a];
+      lk.rgap2[a] = ds.rect_gap2[a];
+      lk.rv[a] = ds.rv.as<float>() + vo;
+      lk.rbin[a] = ds.rbin.as<float4>() + bo;
+      vo += ds.rect_n[a];
+      bo += ds.rect_nb[a];
+    }
+    return FOMO_OK;
+  }
   hipStream_t s = g_ctx.stream;
   const GridInfo& g = ds.grid;
   // subdivision: the requested one, reduced until the grid fits ~24 cells per point and 2^30.
@@ -2683,7 +2737,132 @@ int build_lookup(Dataset& ds) {
   return FOMO_OK;
 }
 
-// Build (or rebuild) the cell-list index of a dataset on the device.
+// Rectilinear detection. The distinct values of each axis (device sort + unique); when their
+// counts multiply to n, every point's grid position (k_rect_map). On success *ok is set,
+// ds.perm[gid] holds the original index of the point at grid position gid (the sorted order is
+// the grid order), and the axis values and bin tables are on the device. Otherwise the dataset is
+// indexed as scattered points.
+int build_rect(Dataset& ds, bool* ok) {
+  *ok = false;
+  ds.rect = false;
+  ds.rv.release();
+  ds.rbin.release();
+  const int64_t n = ds.n;
+  if (!ds.allow_rect || n < 2) return FOMO_OK;
+  hipStream_t s = g_ctx.stream;
+  HIPOK(g_ctx.sortkeys[0].ensure(sizeof(uint32_t) * n));
+  HIPOK(g_ctx.sortkeys[1].ensure(sizeof(uint32_t) * n));
+  float* sorted = g_ctx.sortkeys[0].as<float>();
+  float* uniq = g_ctx.sortkeys[1].as<float>();
+  DevBuf dcount;
+  if (dcount.alloc(sizeof(unsigned int))) return fail(FOMO_ERR_NOMEM, "device allocation failed (rectilinear test)");
+  std::vector<float> vals[3];
+  int64_t prod = 1;
+  for (int a = 0; a < 3; ++a) {
+    size_t tb = 0, tu = 0;
+    HIPOK(rocprim::radix_sort_keys(nullptr, tb, ds.raw[a].as<float>(), sorted, (size_t)n, 0, 32, s));
+    HIPOK(rocprim::unique(nullptr, tu, sorted, uniq, dcount.as<unsigned int>(), (size_t)n,
+                          rocprim::equal_to<float>(), s));
+    HIPOK(g_ctx.sorttmp.ensure(std::max(tb, tu)));
+    HIPOK(rocprim::radix_sort_keys(g_ctx.sorttmp.p, tb, ds.raw[a].as<float>(), sorted, (size_t)n, 0, 32, s));
+    HIPOK(rocprim::unique(g_ctx.sorttmp.p, tu, sorted, uniq, dcount.as<unsigned int>(), (size_t)n,
+                          rocprim::equal_to<float>(), s));
+    unsigned int na = 0;
+    HIPOK(hipMemcpyAsync(&na, dcount.p, sizeof(na), hipMemcpyDeviceToHost, s));
+    if (int rc = sync_stream()) return rc;
+    prod *= (int64_t)na;
+    if (na == 0 || n % (int64_t)na != 0 || prod > n) return FOMO_OK;
+    vals[a].resize(na);
+    HIPOK(hipMemcpyAsync(vals[a].data(), uniq, sizeof(float) * na, hipMemcpyDeviceToHost, s));
+    if (int rc = sync_stream()) return rc;
+    for (float f : vals[a])
+      if (!std::isfinite(f)) return FOMO_OK;
+  }
+  if (prod != n) return FOMO_OK;
+  // axis tables
+  int64_t nv = 0, nbt = 0;
+  for (int a = 0; a < 3; ++a) {
+    ds.rect_n[a] = (int)vals[a].size();
+    ds.rect_nb[a] = fomo::kRectBinsPerValue * ds.rect_n[a];
+    nv += ds.rect_n[a];
+    nbt += ds.rect_nb[a];
+  }
+  std::vector<float> hv;
+  std::vector<float4> hb(nbt);
+  hv.reserve(nv);
+  int64_t boff = 0;
+  for (int a = 0; a < 3; ++a) {
+    hv.insert(hv.end(), vals[a].begin(), vals[a].end());
+    fomo::rect_axis_table(vals[a].data(), ds.rect_n[a], hb.data() + boff, &ds.rect_o[a], &ds.rect_inv[a],
+                          &ds.rect_gap2[a]);
+    boff += ds.rect_nb[a];
+  }
+  if (ds.rv.alloc(sizeof(float) * nv) || ds.rbin.alloc(sizeof(float4) * nbt))
+    return fail(FOMO_ERR_NOMEM, "device allocation failed (rectilinear axes)");
+  HIPOK(hipMemcpyAsync(ds.rv.p, hv.data(), sizeof(float) * nv, hipMemcpyHostToDevice, s));
+  HIPOK(hipMemcpyAsync(ds.rbin.p, hb.data(), sizeof(float4) * nbt, hipMemcpyHostToDevice, s));
+  // grid positions: every point on an axis value, no two on one position
+  HIPOK(ds.perm.ensure(sizeof(int32_t) * n));
+  HIPOK(hipMemsetAsync(ds.perm.p, 0xff, sizeof(int32_t) * n, s));
+  int* bad = reinterpret_cast<int*>(dcount.p);
+  HIPOK(hipMemsetAsync(bad, 0, sizeof(int), s));
+  const float* v = ds.rv.as<float>();
+  k_rect_map<<<nblk(n), kBlock, 0, s>>>(ds.raw[0].as<float>(), ds.raw[1].as<float>(), ds.raw[2].as<float>(), n, v,
+                                        v + ds.rect_n[0], v + ds.rect_n[0] + ds.rect_n[1], ds.rect_n[0],
+                                        ds.rect_n[1], ds.rect_n[2], ds.perm.as<int32_t>(), bad);
+  HIPOK(hipGetLastError());
+  int hbad = 1;
+  HIPOK(hipMemcpyAsync(&hbad, bad, sizeof(int), hipMemcpyDeviceToHost, s));
+  if (int rc = sync_stream()) return rc;
+  if (hbad) {
+    ds.rv.release();
+    ds.rbin.release();
+    return FOMO_OK;
+  }
+  ds.rect = true;
+  *ok = true;
+  return FOMO_OK;
+}
+
+// Cell list of a scattered dataset: points sorted by the key of their cell of ds.grid (x fastest),
+// ds.perm = sorted position -> original index, ds.cstart = first sorted position of every cell.
+int build_cell_list(Dataset& ds) {
+  hipStream_t s = g_ctx.stream;
+  const int64_t n = ds.n;
+  const GridInfo& g = ds.grid;
+  // keys
+  HIPOK(g_ctx.sortkeys[0].ensure(sizeof(uint32_t) * n));
+  HIPOK(g_ctx.sortkeys[1].ensure(sizeof(uint32_t) * n));
+  HIPOK(g_ctx.sortvals.ensure(sizeof(int32_t) * n));
+  HIPOK(ds.perm.ensure(sizeof(int32_t) * n));
+  {
+    ProfScope ps("k_cell_keys", s);
+    k_cell_keys<<<nblk(n), kBlock, 0, s>>>(ds.raw[0].as<float>(), ds.raw[1].as<float>(), ds.raw[2].as<float>(), n, g,
+                                           g_ctx.sortkeys[0].as<uint32_t>(), g_ctx.sortvals.as<int32_t>());
+  }
+  HIPOK(hipGetLastError());
+  int endbit = 1;
+  while (endbit < 32 && ((int64_t)1 << endbit) < ds.ncells) endbit++;
+  size_t tmpb = 0;
+  HIPOK(rocprim::radix_sort_pairs(nullptr, tmpb, g_ctx.sortkeys[0].as<uint32_t>(), g_ctx.sortkeys[1].as<uint32_t>(),
+                                  g_ctx.sortvals.as<int32_t>(), ds.perm.as<int32_t>(), (size_t)n, 0, endbit, s));
+  HIPOK(g_ctx.sorttmp.ensure(tmpb));
+  {
+    ProfScope ps("radix_sort", s);
+    HIPOK(rocprim::radix_sort_pairs(g_ctx.sorttmp.p, tmpb, g_ctx.sortkeys[0].as<uint32_t>(),
+                                    g_ctx.sortkeys[1].as<uint32_t>(), g_ctx.sortvals.as<int32_t>(),
+                                    ds.perm.as<int32_t>(), (size_t)n, 0, endbit, s));
+  }
+  HIPOK(ds.cstart.ensure(sizeof(int32_t) * (ds.ncells + 1)));
+  {
+    ProfScope ps("k_cell_start", s);
+    k_cell_start<<<nblk(n + 1), kBlock, 0, s>>>(g_ctx.sortkeys[1].as<uint32_t>(), n, ds.ncells, ds.cstart.as<int32_t>());
+  }
+  return FOMO_OK;
+}
+
+// Build (or rebuild) the index of a dataset on the device: rectilinear axes when the points form
+// a product grid, else the cell list and candidate lookup grid; then the octree.
 int build_index(Dataset& ds, double cell_scale) {
   hipStream_t s = g_ctx.stream;
   const int64_t n = ds.n;
@@ -2740,33 +2919,14 @@ int build_index(Dataset& ds, double cell_scale) {
   g.invh = 1.0 / h;
   g.slack = 1e-6 * h;
   ds.ncells = (int64_t)g.dims[0] * g.dims[1] * g.dims[2];
-  // keys
-  HIPOK(g_ctx.sortkeys[0].ensure(sizeof(uint32_t) * n));
-  HIPOK(g_ctx.sortkeys[1].ensure(sizeof(uint32_t) * n));
-  HIPOK(g_ctx.sortvals.ensure(sizeof(int32_t) * n));
-  HIPOK(ds.perm.ensure(sizeof(int32_t) * n));
-  {
-    ProfScope ps("k_cell_keys", s);
-    k_cell_keys<<<nblk(n), kBlock, 0, s>>>(ds.raw[0].as<float>(), ds.raw[1].as<float>(), ds.raw[2].as<float>(), n, g,
-                                           g_ctx.sortkeys[0].as<uint32_t>(), g_ctx.sortvals.as<int32_t>());
-  }
-  HIPOK(hipGetLastError());
-  int endbit = 1;
-  while (endbit < 32 && ((int64_t)1 << endbit) < ds.ncells) endbit++;
-  size_t tmpb = 0;
-  HIPOK(rocprim::radix_sort_pairs(nullptr, tmpb, g_ctx.sortkeys[0].as<uint32_t>(), g_ctx.sortkeys[1].as<uint32_t>(),
-                                  g_ctx.sortvals.as<int32_t>(), ds.perm.as<int32_t>(), (size_t)n, 0, endbit, s));
-  HIPOK(g_ctx.sorttmp.ensure(tmpb));
-  {
-    ProfScope ps("radix_sort", s);
-    HIPOK(rocprim::radix_sort_pairs(g_ctx.sorttmp.p, tmpb, g_ctx.sortkeys[0].as<uint32_t>(),
-                                    g_ctx.sortkeys[1].as<uint32_t>(), g_ctx.sortvals.as<int32_t>(),
-                                    ds.perm.as<int32_t>(), (size_t)n, 0, endbit, s));
-  }
-  HIPOK(ds.cstart.ensure(sizeof(int32_t) * (ds.ncells + 1)));
-  {
-    ProfScope ps("k_cell_start", s);
-    k_cell_start<<<nblk(n + 1), kBlock, 0, s>>>(g_ctx.sortkeys[1].as<uint32_t>(), n, ds.ncells, ds.cstart.as<int32_t>());
+  bool rect = false;
+  if (int rc2 = build_rect(ds, &rect)) return rc2;
+  if (rect) {
+    // no cell list: the sorted order is the grid order (ds.perm from build_rect)
+    ds.ncells = 0;
+    ds.cstart.release();
+  } else if (int rc2 = build_cell_list(ds)) {
+    return rc2;
   }
   HIPOK(ds.pts.ensure(sizeof(float4) * n));
   k_permute_points<<<nblk(n), kBlock, 0, s>>>(ds.raw[0].as<float>(), ds.raw[1].as<float>(), ds.raw[2].as<float>(),
@@ -2784,13 +2944,15 @@ int build_index(Dataset& ds, double cell_scale) {
                                           ds.rawvars[4]->as<float>(), ds.perm.as<int32_t>(), n, ds.vel.as<float4>());
   }
   HIPOK(hipGetLastError());
-  // maximum cell occupancy
-  int* mx = reinterpret_cast<int*>(part);
-  HIPOK(hipMemsetAsync(mx, 0, sizeof(int), s));
-  k_cell_maxcount<<<nblk(ds.ncells), kBlock, 0, s>>>(ds.cstart.as<int32_t>(), ds.ncells, mx);
-  HIPOK(hipGetLastError());
-  int hmx = 0;
-  HIPOK(hipMemcpyAsync(&hmx, mx, sizeof(int), hipMemcpyDeviceToHost, s));
+  // maximum cell occupancy (one point per grid position on rectilinear datasets)
+  int hmx = 1;
+  if (!rect) {
+    int* mx = reinterpret_cast<int*>(part);
+    HIPOK(hipMemsetAsync(mx, 0, sizeof(int), s));
+    k_cell_maxcount<<<nblk(ds.ncells), kBlock, 0, s>>>(ds.cstart.as<int32_t>(), ds.ncells, mx);
+    HIPOK(hipGetLastError());
+    HIPOK(hipMemcpyAsync(&hmx, mx, sizeof(int), hipMemcpyDeviceToHost, s));
+  }
   rc = sync_stream();
   if (rc) return rc;
   ds.maxcount = hmx;
@@ -3038,8 +3200,13 @@ void launch_nn_los(int variant, int ch, unsigned blocks, hipStream_t s, const Da
                     ds.mpts.as<float4>(), ds.nlo.as<float4>(), ds.nhi.as<float4>(), ds.bvh, geom, nx, ny, nz, rec,
                     ri, ls, nl, flags};
   switch (ch) {
-#define FOMO_NNL_CH(C) \
-  case C: nn_los_launch<C>(variant, blocks, s, a); break;
+#define FOMO_NNL_CH(C)                              \
+  case C:                                           \
+    if (ds.lk.rect)                                 \
+      nn_los_launch<C, true>(variant, blocks, s, a);  \
+    else                                            \
+      nn_los_launch<C, false>(variant, blocks, s, a); \
+    break;
     FOMO_NNL_CH(4) FOMO_NNL_CH(8) FOMO_NNL_CH(12) FOMO_NNL_CH(16) FOMO_NNL_CH(20) FOMO_NNL_CH(24) FOMO_NNL_CH(28)
     FOMO_NNL_CH(32) FOMO_NNL_CH(36) FOMO_NNL_CH(40) FOMO_NNL_CH(44) FOMO_NNL_CH(48) FOMO_NNL_CH(52)
     FOMO_NNL_CH(56) FOMO_NNL_CH(60) FOMO_NNL_CH(64)
@@ -3957,6 +4124,26 @@ int fomo_dataset_lookup_info(int id, int* subdiv, int64_t* cells, int64_t* candi
   return FOMO_OK;
 }
 
+int fomo_dataset_set_rectilinear(int id, int on) {
+  std::lock_guard<std::recursive_mutex> lk(g_mu);
+  if (int rc = require_init()) return rc;
+  Dataset* ds = find_obj(g_ctx.datasets, id);
+  if (!ds) return fail(FOMO_ERR_NOTFOUND, "unknown dataset id");
+  if (ds->inuse.load() > 0) return fail(FOMO_ERR_STATE, "dataset is in use by a batch or a server request");
+  ds->allow_rect = on != 0;
+  return reindex_dataset(id, *ds, ds->cell_scale);
+}
+
+int fomo_dataset_rect_info(int id, int* axes3) {
+  std::lock_guard<std::recursive_mutex> lk(g_mu);
+  if (int rc = require_init()) return rc;
+  Dataset* ds = find_obj(g_ctx.datasets, id);
+  if (!ds) return fail(FOMO_ERR_NOTFOUND, "unknown dataset id");
+  if (!axes3) return fail(FOMO_ERR_ARG, "null output");
+  for (int a = 0; a < 3; ++a) axes3[a] = ds->rect ? ds->rect_n[a] : 0;
+  return FOMO_OK;
+}
+
 int fomo_dataset_lookup_groups(int id, int64_t* counts5) {
   std::lock_guard<std::recursive_mutex> lk(g_mu);
   if (int rc = require_init()) return rc;
@@ -4284,8 +4471,9 @@ int fomo_autotune(int ngoft, const int* gofts, double l, double b, int nx, int n
     *ms = t / reps;
     return FOMO_OK;
   };
-  const double scales[] = {0.7, 1.0, 1.4};
-  const int subdivs[] = {1, 2, 3};
+  // the cell scale and lookup subdivision do not apply to a rectilinear dataset's index
+  const std::vector<double> scales = ds->rect ? std::vector<double>{ds->cell_scale} : std::vector<double>{0.7, 1.0, 1.4};
+  const std::vector<int> subdivs = ds->rect ? std::vector<int>{ds->lookup_subdiv} : std::vector<int>{1, 2, 3};
   const int tlogs[] = {3, 4, 5, 6};  // ray tiles 8x32, 16x16, 32x8, 64x4
   const double cs0 = ds->cell_scale;
   const int sd0 = ds->lookup_subdiv, tl0 = ds->tile_log2;
@@ -4296,7 +4484,7 @@ int fomo_autotune(int ngoft, const int* gofts, double l, double b, int nx, int n
     for (int sd : subdivs) {
       ds->lookup_subdiv = sd;
       if ((rc = reindex_dataset(dsid, *ds, sc))) goto done;
-      if (ds->lookup_active_subdiv != sd && sd > 1) continue;  // did not fit: same as a smaller one
+      if (!ds->rect && ds->lookup_active_subdiv != sd && sd > 1) continue;  // did not fit: same as a smaller one
       for (int tl : tlogs) {
         ds->tile_log2 = tl;
         double ms = 0;

@@ -1,31 +1,38 @@
 // fomo_nnlos_part.hip — instantiations of the fused NN + integration kernel k_nn_los for one
-// group of lambda chunk sizes CH. The build compiles this file once per group
-// (-DFOMO_NNLOS_GROUP=0..7, two CH values each, one of them at least 36 for the two-pass variant)
-// so that the 176 instantiations compile in parallel.
+// group of lambda chunk sizes CH and one index kind. The build compiles this file once per group
+// and kind (-DFOMO_NNLOS_GROUP=0..7, two CH values each, one of them at least 36 for the two-pass
+// variant; -DFOMO_NNLOS_RECT=0 for candidate lookup grids, 1 for rectilinear datasets) so that the
+// 352 instantiations compile in parallel.
 #include "fomo_common.h"
 
 #ifndef FOMO_NNLOS_GROUP
 #error "compile with -DFOMO_NNLOS_GROUP=<0..7>"
+#endif
+#ifndef FOMO_NNLOS_RECT
+#error "compile with -DFOMO_NNLOS_RECT=<0|1>"
 #endif
 
 namespace fomo_dev {
 
 namespace {
 
+constexpr bool kRect = FOMO_NNLOS_RECT != 0;
+
 template <int RS4, int CH, int G, int SL, int NP>
 void launch_pair(unsigned blocks, hipStream_t s, const NnLosArgs& a) {
-  k_nn_los<RS4, CH, G, false, SL, NP><<<blocks, 256, 0, s>>>(a.pts, a.g, a.lslot, a.lcand, a.lk, a.mpts, a.nlo, a.nhi,
-                                                             a.bvh, a.geom, a.nx, a.ny, a.nz, a.rec, a.ri, a.ls, a.nl,
-                                                             a.flags);
-  k_nn_los<RS4, CH, G, true, SL, NP><<<blocks, 256, 0, s>>>(a.pts, a.g, a.lslot, a.lcand, a.lk, a.mpts, a.nlo, a.nhi,
-                                                            a.bvh, a.geom, a.nx, a.ny, a.nz, a.rec, a.ri, a.ls, a.nl,
-                                                            a.flags);
+  k_nn_los<RS4, CH, G, false, SL, NP, kRect><<<blocks, 256, 0, s>>>(a.pts, a.g, a.lslot, a.lcand, a.lk, a.mpts, a.nlo,
+                                                                    a.nhi, a.bvh, a.geom, a.nx, a.ny, a.nz, a.rec,
+                                                                    a.ri, a.ls, a.nl, a.flags);
+  k_nn_los<RS4, CH, G, true, SL, NP, kRect><<<blocks, 256, 0, s>>>(a.pts, a.g, a.lslot, a.lcand, a.lk, a.mpts, a.nlo,
+                                                                   a.nhi, a.bvh, a.geom, a.nx, a.ny, a.nz, a.rec,
+                                                                   a.ri, a.ls, a.nl, a.flags);
 }
 
 }  // namespace
 
-template <int CH>
+template <int CH, bool RECT>
 void nn_los_launch(int variant, unsigned blocks, hipStream_t s, const NnLosArgs& a) {
+  static_assert(RECT == kRect, "instantiated in the translation unit of its index kind");
   switch (variant) {
     case kNnLosSingle: launch_pair<2, CH, 4, 1, 1>(blocks, s, a); break;
     case kNnLosShared: launch_pair<2, CH, 4, 4, 1>(blocks, s, a); break;
@@ -39,7 +46,7 @@ void nn_los_launch(int variant, unsigned blocks, hipStream_t s, const NnLosArgs&
   }
 }
 
-#define FOMO_NNLOS_INST(C) template void nn_los_launch<C>(int, unsigned, hipStream_t, const NnLosArgs&);
+#define FOMO_NNLOS_INST(C) template void nn_los_launch<C, kRect>(int, unsigned, hipStream_t, const NnLosArgs&);
 #if FOMO_NNLOS_GROUP == 0
 FOMO_NNLOS_INST(4) FOMO_NNLOS_INST(64)
 #elif FOMO_NNLOS_GROUP == 1

@@ -91,6 +91,19 @@ int fomo_dataset_tet_stats(int id, int64_t* out2);
    shared list counts once per cell that uses it), cells left to the exact octree query. */
 int fomo_dataset_lookup_info(int id, int* subdiv, int64_t* cells, int64_t* candidates, int64_t* fallback_cells);
 
+/* Rectilinear datasets. When the points form a product grid (x, y and z each take a set of
+ * distinct values, and every combination occurs exactly once: structured simulation output), the
+ * index is the three sorted axes instead of the cell list and candidate lookup grid. A target's
+ * nearest point then combines the axis values that bracket it (exact ties are resolved as by every
+ * other query: squared float-coordinate distances in FP64, then the lowest original index), and
+ * the exact octree query answers the rest. Detection is on by default; on = 0 re-indexes the
+ * dataset as scattered points, on = 1 detects again. Results are identical either way; speed and
+ * memory change (no 128-byte lookup record per sample, no lookup grid). Replaces, like the rest of
+ * the index, the per-view R-tree build of fomo-nearestneighbour.cpp:93-100. */
+int fomo_dataset_set_rectilinear(int id, int on);
+/* Values per axis of a rectilinear dataset's grid (axes3), all 0 for a scattered one. */
+int fomo_dataset_rect_info(int id, int* axes3);
+
 /* Far and outer groups of the lookup grid (targets more than 2 cells from any point, and targets
  * beyond the grid, whose cells read a group's shared candidate list). counts5: far groups, far
  * groups with a list, far groups whose cells all take their list (group headers), outer groups,

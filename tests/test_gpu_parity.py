@@ -396,10 +396,14 @@ def test_thomson_small(engine):
 
 @pytest.mark.parametrize("jitter", [0.0, 0.3])
 def test_lookup_fast_path_matches_ring_search(engine, jitter):
-    """The candidate lookup grid changes speed only: NN choices equal the ring search and the oracle."""
+    """The candidate lookup grid changes speed only: NN choices equal the ring search and the oracle.
+    The structured cube (jitter 0) would be indexed by its axes; it is indexed as scattered points
+    here, so that its lookup grid is tested too."""
     F = engine
     d = F.gen_loop_cube(36, seed=21, jitter=jitter)
     ds = _dataset(F, d)
+    ds.set_rectilinear(False)
+    assert ds.rect_axes() is None
     info = ds.lookup_info()
     assert info["subdiv"] == 2 and info["cells"] > 0
     assert info["fallback_cells"] < 0.05 * info["cells"], info
