@@ -60,11 +60,12 @@ struct LookupInfo {
   // grid of the axis values rv[a], and a point's sorted position is its grid position
   // i + n0 (j + n1 k) (rect_geom.h)
   int rect;
-  int rn[3], rnb[3];         // values and bins per axis
-  double ro[3], rinv[3];     // bin origin and inverse bin width per axis
-  double rgap2[3];           // square of the smallest gap between consecutive values, rounded down
-  const float* rv[3];        // axis values, ascending
-  const float4* rbin[3];     // bin tables: {index as int bits, v[index], v[index + 1], v[index + 2]}
+  int rn[3];                 // values per axis (bins: kRectBinsPerValue times as many)
+  float ro[3], rinv[3];      // bin origin and inverse bin width per axis
+  double rdmax;              // largest nearest distance without a possible tie two values away
+  const float* rv;           // axis values, ascending, the three axes one after another
+  const float4* rbin;        // bin tables {index as int bits, v[index], v[index + 1], v[index + 2]},
+                             // the three axes one after another
 };
 
 // Start grid of the tetra walk (a finite tet near each cell centre) and the points' bounding box,
@@ -260,9 +261,11 @@ __device__ __forceinline__ int outer_lookup(const float4* __restrict__ pts, cons
 // Rectilinear datasets, first step: the bin table entries of the target (one 16-byte read per
 // axis, from tables small enough to stay in the caches).
 __device__ __forceinline__ void rect_issue(const LookupInfo& lk, float t0, float t1, float t2, float4 (&e)[kSlots]) {
-  e[0] = lk.rbin[0][fomo::rect_bin(lk.ro[0], lk.rinv[0], lk.rnb[0], t0)];
-  e[1] = lk.rbin[1][fomo::rect_bin(lk.ro[1], lk.rinv[1], lk.rnb[1], t1)];
-  e[2] = lk.rbin[2][fomo::rect_bin(lk.ro[2], lk.rinv[2], lk.rnb[2], t2)];
+  constexpr int B = fomo::kRectBinsPerValue;
+  const int nb0 = B * lk.rn[0], nb1 = B * lk.rn[1], nb2 = B * lk.rn[2];
+  e[0] = lk.rbin[fomo::rect_bin(lk.ro[0], lk.rinv[0], nb0, t0)];
+  e[1] = lk.rbin[nb0 + fomo::rect_bin(lk.ro[1], lk.rinv[1], nb1, t1)];
+  e[2] = lk.rbin[nb0 + nb1 + fomo::rect_bin(lk.ro[2], lk.rinv[2], nb2, t2)];
 }
 
 // Rectilinear datasets, second step: the nearest point's sorted position if it lies strictly
@@ -270,8 +273,8 @@ __device__ __forceinline__ void rect_issue(const LookupInfo& lk, float t0, float
 __device__ __forceinline__ int rect_resolve(const float4* __restrict__ pts, const LookupInfo& lk,
                                             const float4 (&e)[kSlots], const fomo::SearchBox& box) {
   return fomo::rect_pick(
-      e[0], e[1], e[2], lk.rn[0], lk.rn[1], lk.rn[2], lk.rgap2[0], lk.rgap2[1], lk.rgap2[2], box,
-      [&](int a, int k) { return a == 0 ? lk.rv[0][k] : (a == 1 ? lk.rv[1][k] : lk.rv[2][k]); },
+      e[0], e[1], e[2], lk.rn[0], lk.rn[1], lk.rn[2], lk.rdmax, box,
+      [&](int a, int k) { return lk.rv[(a > 0 ? lk.rn[0] : 0) + (a > 1 ? lk.rn[1] : 0) + k]; },
       [&](int64_t gp) { return __float_as_int(pts[gp].w); });
 }
 
@@ -469,6 +472,14 @@ __device__ __forceinline__ void profile_chains(const double (&ex)[3], double (&g
 }
 
 
+// Orders the LDS accesses of one wave around it: what its lanes wrote before is visible to the
+// reads of any lane after it (no code motion across, LDS operations complete in order).
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // f(integral_constant<int, I>) for I = 0 .. N-1, unrolled at compile time
 template <int N, int I = 0, class F>
 __device__ __forceinline__ void static_for(F&& f) {
@@ -541,6 +552,13 @@ __global__ void __launch_bounds__(256) k_nn_los(const float4* __restrict__ pts, 
   const int lane = threadIdx.x & 63;
   const int q = lane & (G - 1);
   const int64_t R = (int64_t)nx * ny;
+  // SL > 0 (not DIRECT): every lane writes, for its own sample, the starts of the profile's 4
+  // chains (g, R) and q16 of each lambda pass, the lines' peaks and the pass bits to its LDS slot;
+  // lane q of the ray then reads chain q of each of the ray's 4 samples. These are the values of
+  // profile_chains at the owner's anchors, as every lane computing them itself would get.
+  constexpr bool kXch = SL > 0 && !DIRECT;
+  constexpr int kSlot = NP * 80 + 32;  // per pass {(g, R) x 4, q16, pad}, then peaks (float4), bits
+  __shared__ __align__(16) unsigned char xch[kXch ? 256 * kSlot : 16];
   const int64_t ray = ((int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) * RPW + lane / G;
   const bool live = ray < R;
   int32_t* const flag = flags + (size_t)(SL > 0 ? 0 : q) * R + (live ? ray : 0);
@@ -594,9 +612,10 @@ __global__ void __launch_bounds__(256) k_nn_los(const float4* __restrict__ pts, 
 #pragma unroll
   for (int c = 0; c < NACC; ++c) acc[c] = 0.0;
   int prev = -1;  // this lane's last resolved point: the warm start of the exact query
-  // Three-stage pipeline over groups of G samples: while group kb is integrated, the record of
-  // group kb + G (winner picked from its lookup cell record) and the lookup cell record of group
-  // kb + 2G are in flight.
+  // Pipeline over groups of G samples. While group kb is integrated, with a lookup grid: the point
+  // record of group kb + G (winner picked from its lookup cell record) and the lookup cell record
+  // of group kb + 2G are in flight; on a rectilinear dataset: the point records of groups kb + G
+  // and kb + 2G and the bin entries of group kb + 3G.
   // stage 1: target, search box and lookup cell record of this lane's sample k
   const auto query_issue = [&](int k, SearchBox& box, const float4*& rp, float4 (&sl)[kSlots], bool& act,
                                bool& cell) {
@@ -671,24 +690,42 @@ __global__ void __launch_bounds__(256) k_nn_los(const float4* __restrict__ pts, 
       f[4 * c] = a.x, f[4 * c + 1] = a.y, f[4 * c + 2] = a.z, f[4 * c + 3] = a.w;
     }
   };
+  // Rectilinear datasets (bin entries in cache, a small query state) keep two point records in
+  // flight; with a lookup grid the larger query state (the cell record) leaves room for one.
+  constexpr bool kDeep = RECT;
   SearchBox boxN;
   const float4* rpN = lslot;
   float4 slN[kSlots];
   bool actN, cellN;
-  float fC[4 * RS4];
+  float fC[4 * RS4], fB[4 * RS4];
   query_issue(q, boxN, rpN, slN, actN, cellN);
   int rC = query_resolve(boxN, rpN, slN, actN, cellN);
   record_issue(rC, fC);
   query_issue(G + q, boxN, rpN, slN, actN, cellN);
+  int rB = -1;
+  if constexpr (kDeep) {
+    rB = query_resolve(boxN, rpN, slN, actN, cellN);
+    record_issue(rB, fB);
+    query_issue(2 * G + q, boxN, rpN, slN, actN, cellN);
+  }
   for (int kb = 0; kb < nz; kb += G) {
     SearchBox boxA;
     const float4* rpA = lslot;
     float4 slA[kSlots];
-    bool actA, cellA;
-    query_issue(kb + 2 * G + q, boxA, rpA, slA, actA, cellA);
-    const int rB = query_resolve(boxN, rpN, slN, actN, cellN);
-    float fB[4 * RS4];
-    record_issue(rB, fB);
+    bool actA = false, cellA = false;
+    int r2 = -1;
+    float f2[4 * RS4];
+    if constexpr (kDeep) {
+      // group kb + 2G: winner from the bin entries issued at the end of the last iteration, and
+      // its point record, two groups ahead of the integration
+      r2 = query_resolve(boxN, rpN, slN, actN, cellN);
+      record_issue(r2, f2);
+    } else {
+      // group kb + 2G's lookup cell record, group kb + G's winner and point record
+      query_issue(kb + 2 * G + q, boxA, rpA, slA, actA, cellA);
+      rB = query_resolve(boxN, rpN, slN, actN, cellN);
+      record_issue(rB, fB);
+    }
     // group kb: this lane's sample, the values write_sample would store
     const float vls = rC >= 0 ? (float)(((0.0 + (sb * cl) * (double)fC[0]) + (-sb * sl_) * (double)fC[1]) +
                                         cb * (double)fC[2])
@@ -762,16 +799,44 @@ __global__ void __launch_bounds__(256) k_nn_los(const float4* __restrict__ pts, 
           fomo::exp3(xa, exS[p]);
         }
       }
+      if constexpr (kXch) {
+        unsigned char* my = xch + threadIdx.x * kSlot;
+        wave_sync();  // the reads of the previous group are done
+#pragma unroll
+        for (int p = 0; p < NP; ++p) {
+          double gc[4], RR[4], q16;
+          profile_chains(exS[p], gc, RR, q16);
+          double2* gr = reinterpret_cast<double2*>(my + p * 80);
+#pragma unroll
+          for (int c = 0; c < 4; ++c) gr[c] = make_double2(gc[c], RR[c]);
+          *reinterpret_cast<double*>(my + p * 80 + 64) = q16;
+        }
+        float pw[4] = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int l = 0; l < SL; ++l) pw[l] = fC[4 + l];
+        *reinterpret_cast<float4*>(my + NP * 80) = make_float4(pw[0], pw[1], pw[2], pw[3]);
+        *reinterpret_cast<int*>(my + NP * 80 + 16) = okS;
+        wave_sync();
+      }
       // the 4 samples of the group in LOS order; lane q runs chain q of each
       static_for<4>([&](auto oc) {
         constexpr int o = decltype(oc)::value;
+        // lane o of the ray's quad: the owner of sample kb + o
+        const unsigned char* src = xch + (kXch ? (((unsigned)threadIdx.x & ~3u) | (unsigned)o) * kSlot : 0);
         float pk[SL];
         unsigned on = 0;
+        if constexpr (kXch) {
+          const float4 p4 = *reinterpret_cast<const float4*>(src + NP * 80);
+          const float pw[4] = {p4.x, p4.y, p4.z, p4.w};
 #pragma unroll
-        for (int l = 0; l < SL; ++l) {
-          pk[l] = group_bcast<4, o>(fC[4 + l]);
-          if (pk[l] != 0.f && l < ls.n) on |= 1u << l;
+          for (int l = 0; l < SL; ++l) pk[l] = pw[l];
+        } else {
+#pragma unroll
+          for (int l = 0; l < SL; ++l) pk[l] = group_bcast<4, o>(fC[4 + l]);
         }
+#pragma unroll
+        for (int l = 0; l < SL; ++l)
+          if (pk[l] != 0.f && l < ls.n) on |= 1u << l;
         if (!integ || kb + o >= nz) on = 0;
         if constexpr (DIRECT) {
           const double kap = group_bcast<4, o>(kapS);
@@ -794,7 +859,7 @@ __global__ void __launch_bounds__(256) k_nn_los(const float4* __restrict__ pts, 
           }
           return;
         }
-        const int okb = group_bcast<4, o>(okS);
+        const int okb = *reinterpret_cast<const int*>(src + NP * 80 + 16);
 #pragma unroll
         for (int p = 0; p < NP; ++p) {
           if (!((okb >> p) & 1)) {
@@ -804,14 +869,9 @@ __global__ void __launch_bounds__(256) k_nn_los(const float4* __restrict__ pts, 
             continue;
           }
           if (on == 0) continue;
-          const double ex[3] = {group_bcast<4, o>(exS[p][0]), group_bcast<4, o>(exS[p][1]),
-                                group_bcast<4, o>(exS[p][2])};
-          double gc[4], RR[4], q16;
-          profile_chains(ex, gc, RR, q16);
-          double gq = gc[0], Rq = RR[0];
-#pragma unroll
-          for (int c = 1; c < 4; ++c)
-            if (q == c) gq = gc[c], Rq = RR[c];
+          const double2 gr = *reinterpret_cast<const double2*>(src + p * 80 + 16 * q);
+          const double q16 = *reinterpret_cast<const double*>(src + p * 80 + 64);
+          double gq = gr.x, Rq = gr.y;
           // a line with peak 0 adds fma(0, g, sum) = sum: g is finite here
 #pragma unroll
           for (int m = 0; m < CJ; ++m) {
@@ -824,15 +884,27 @@ __global__ void __launch_bounds__(256) k_nn_los(const float4* __restrict__ pts, 
       });
     }
     // advance the pipeline
-    rC = rB;
+    if constexpr (kDeep) {
+      // the bin entries of group kb + 3G in flight during the next integration
+      query_issue(kb + 3 * G + q, boxN, rpN, slN, actN, cellN);
+      rC = rB;
+      rB = r2;
 #pragma unroll
-    for (int c = 0; c < 4 * RS4; ++c) fC[c] = fB[c];
-    boxN = boxA;
-    rpN = rpA;
+      for (int c = 0; c < 4 * RS4; ++c) {
+        fC[c] = fB[c];
+        fB[c] = f2[c];
+      }
+    } else {
+      rC = rB;
 #pragma unroll
-    for (int c = 0; c < kSlots; ++c) slN[c] = slA[c];
-    actN = actA;
-    cellN = cellA;
+      for (int c = 0; c < 4 * RS4; ++c) fC[c] = fB[c];
+      boxN = boxA;
+      rpN = rpA;
+#pragma unroll
+      for (int c = 0; c < kSlots; ++c) slN[c] = slA[c];
+      actN = actA;
+      cellN = cellA;
+    }
   }
   if (!integ) return;
   const double sc = vg.scale;

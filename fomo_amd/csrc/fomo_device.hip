@@ -2061,10 +2061,12 @@ struct Dataset {
   // is built (the NN query reads the axes: rect_geom.h)
   bool allow_rect = true;  // detect product grids when indexing (fomo_dataset_set_rectilinear)
   bool rect = false;
-  int rect_n[3] = {0, 0, 0}, rect_nb[3] = {0, 0, 0};
-  double rect_o[3] = {0, 0, 0}, rect_inv[3] = {0, 0, 0}, rect_gap2[3] = {0, 0, 0};
+  int rect_n[3] = {0, 0, 0};
+  float rect_o[3] = {0, 0, 0}, rect_inv[3] = {0, 0, 0};
+  double rect_gap2[3] = {0, 0, 0};
   DevBuf rv;           // axis values, the three axes one after another
   DevBuf rbin;         // float4 bin tables, the three axes one after another
+  DevBuf corners;      // float4 x 8: the grid's corner points (their rotated extents are the grid's)
   int64_t ncand = 0, nlcells = 0, nfallback = 0;
   // far and outer groups of the lookup grid: groups, groups with a candidate list, far groups
   // whose cells all take the list (group headers)
@@ -2580,18 +2582,14 @@ int build_lookup(Dataset& ds) {
   if (ds.rect) {
     LookupInfo& lk = ds.lk;
     lk.rect = 1;
-    int64_t vo = 0, bo = 0;
     for (int a = 0; a < 3; ++a) {
       lk.rn[a] = ds.rect_n[a];
-      lk.rnb[a] = ds.rect_nb[a];
       lk.ro[a] = ds.rect_o[a];
       lk.rinv[a] = ds.rect_inv[a];
-      lk.rgap2[a] = ds.rect_gap2[a];
-      lk.rv[a] = ds.rv.as<float>() + vo;
-      lk.rbin[a] = ds.rbin.as<float4>() + bo;
-      vo += ds.rect_n[a];
-      bo += ds.rect_nb[a];
     }
+    lk.rdmax = fomo::rect_dmax(ds.rect_gap2);
+    lk.rv = ds.rv.as<float>();
+    lk.rbin = ds.rbin.as<float4>();
     return FOMO_OK;
   }
   hipStream_t s = g_ctx.stream;
@@ -2747,6 +2745,7 @@ int build_rect(Dataset& ds, bool* ok) {
   ds.rect = false;
   ds.rv.release();
   ds.rbin.release();
+  ds.corners.release();
   const int64_t n = ds.n;
   if (!ds.allow_rect || n < 2) return FOMO_OK;
   hipStream_t s = g_ctx.stream;
@@ -2783,9 +2782,8 @@ int build_rect(Dataset& ds, bool* ok) {
   int64_t nv = 0, nbt = 0;
   for (int a = 0; a < 3; ++a) {
     ds.rect_n[a] = (int)vals[a].size();
-    ds.rect_nb[a] = fomo::kRectBinsPerValue * ds.rect_n[a];
     nv += ds.rect_n[a];
-    nbt += ds.rect_nb[a];
+    nbt += fomo::kRectBinsPerValue * ds.rect_n[a];
   }
   std::vector<float> hv;
   std::vector<float4> hb(nbt);
@@ -2795,12 +2793,20 @@ int build_rect(Dataset& ds, bool* ok) {
     hv.insert(hv.end(), vals[a].begin(), vals[a].end());
     fomo::rect_axis_table(vals[a].data(), ds.rect_n[a], hb.data() + boff, &ds.rect_o[a], &ds.rect_inv[a],
                           &ds.rect_gap2[a]);
-    boff += ds.rect_nb[a];
+    boff += fomo::kRectBinsPerValue * ds.rect_n[a];
   }
-  if (ds.rv.alloc(sizeof(float) * nv) || ds.rbin.alloc(sizeof(float4) * nbt))
+  // the 8 corners of the grid: a view's rotated coordinate ((x c0 - y c1) - z c2 and the like, in
+  // the kernels' FP64 order) is monotone in each coordinate, since rounding is monotone, so its
+  // smallest and largest values over the grid are taken at corners
+  std::vector<float4> hc(8);
+  for (int c = 0; c < 8; ++c)
+    hc[c] = make_float4((c & 1) ? vals[0].back() : vals[0].front(), (c & 2) ? vals[1].back() : vals[1].front(),
+                        (c & 4) ? vals[2].back() : vals[2].front(), 0.f);
+  if (ds.rv.alloc(sizeof(float) * nv) || ds.rbin.alloc(sizeof(float4) * nbt) || ds.corners.alloc(sizeof(float4) * 8))
     return fail(FOMO_ERR_NOMEM, "device allocation failed (rectilinear axes)");
   HIPOK(hipMemcpyAsync(ds.rv.p, hv.data(), sizeof(float) * nv, hipMemcpyHostToDevice, s));
   HIPOK(hipMemcpyAsync(ds.rbin.p, hb.data(), sizeof(float4) * nbt, hipMemcpyHostToDevice, s));
+  HIPOK(hipMemcpyAsync(ds.corners.p, hc.data(), sizeof(float4) * 8, hipMemcpyHostToDevice, s));
   // grid positions: every point on an axis value, no two on one position
   HIPOK(ds.perm.ensure(sizeof(int32_t) * n));
   HIPOK(hipMemsetAsync(ds.perm.p, 0xff, sizeof(int32_t) * n, s));
@@ -2817,6 +2823,7 @@ int build_rect(Dataset& ds, bool* ok) {
   if (hbad) {
     ds.rv.release();
     ds.rbin.release();
+    ds.corners.release();
     return FOMO_OK;
   }
   ds.rect = true;
@@ -2975,12 +2982,16 @@ void enqueue_view_finalize(hipStream_t s, int mode, double l, double b, int nx, 
 
 void enqueue_view_geom(hipStream_t s, const Dataset& ds, int mode, double l, double b, int nx, int ny, int nz, int dn,
                        float* aux, double* partial, ViewGeom* geom) {
-  const unsigned nb = std::min<unsigned>(nblk(ds.n), kMaxPartials);
+  unsigned nb = std::min<unsigned>(nblk(ds.n), kMaxPartials);
   const double cb = cos(b), sb = sin(b), cl = cos(l), sl = sin(l);
   if (mode == MODE_NN) {
     ProfScope ps("k_view_rotate_nn", s);
     (void)aux;
-    k_view_rotate_nn<<<nb, kBlock, 0, s>>>(ds.pts.as<float4>(), nullptr, ds.n, cb, sb, cl, sl, nullptr, partial);
+    // a rectilinear dataset's extents are those of its 8 corners
+    const int64_t np = ds.rect ? 8 : ds.n;
+    if (ds.rect) nb = 1;
+    k_view_rotate_nn<<<nb, kBlock, 0, s>>>(ds.rect ? ds.corners.as<float4>() : ds.pts.as<float4>(), nullptr, np, cb,
+                                           sb, cl, sl, nullptr, partial);
   } else {
     const double eps = 1e-6;  // fomo-thomson.cpp:139-152
     double cbc = cb, sbc = sb;
@@ -2997,7 +3008,10 @@ void enqueue_view_geom(hipStream_t s, const Dataset& ds, int mode, double l, dou
 // view v at partial + v * 6 * kMaxPartials. Returns the number of partials per view.
 unsigned enqueue_views_extents(hipStream_t s, const Dataset& ds, const std::vector<double>& l,
                                const std::vector<double>& b, double* partial) {
-  const unsigned nb = std::min<unsigned>(nblk(ds.n), kMaxPartials);
+  // a rectilinear dataset's extents are those of its 8 corners
+  const float4* P = ds.rect ? ds.corners.as<float4>() : ds.pts.as<float4>();
+  const int64_t np = ds.rect ? 8 : ds.n;
+  const unsigned nb = std::min<unsigned>(nblk(np), kMaxPartials);
   const int nv = (int)l.size();
   for (int v0 = 0; v0 < nv; v0 += kViewsPerPass) {
     ViewTrigs t{};
@@ -3010,8 +3024,8 @@ unsigned enqueue_views_extents(hipStream_t s, const Dataset& ds, const std::vect
     for (int w = 1; w < m; ++w) sameb = sameb && t.cb[w] == t.cb[0] && t.sb[w] == t.sb[0];
     ProfScope ps("k_view_extents_multi", s);
     double* const pv = partial + (int64_t)v0 * 6 * kMaxPartials;
-    if (sameb) k_view_extents_multi<true><<<nb, kBlock, 0, s>>>(ds.pts.as<float4>(), ds.n, t, m, pv, 6 * kMaxPartials);
-    else k_view_extents_multi<false><<<nb, kBlock, 0, s>>>(ds.pts.as<float4>(), ds.n, t, m, pv, 6 * kMaxPartials);
+    if (sameb) k_view_extents_multi<true><<<nb, kBlock, 0, s>>>(P, np, t, m, pv, 6 * kMaxPartials);
+    else k_view_extents_multi<false><<<nb, kBlock, 0, s>>>(P, np, t, m, pv, 6 * kMaxPartials);
   }
   return nb;
 }

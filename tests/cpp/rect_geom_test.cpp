@@ -77,7 +77,8 @@ int main() {
     std::iota(orig.begin(), orig.end(), 0);
     std::shuffle(orig.begin(), orig.end(), rng);
     std::vector<F4> bins[3];
-    double o[3], inv[3], gap2[3];
+    float o[3], inv[3];
+    double gap2[3];
     int nb[3];
     for (int a = 0; a < 3; ++a) {
       nb[a] = fomo::kRectBinsPerValue * n[a];
@@ -156,15 +157,14 @@ int main() {
       const int64_t expect = in ? bg : -2;
       F4 e[3];
       for (int a = 0; a < 3; ++a) e[a] = bins[a][fomo::rect_bin(o[a], inv[a], nb[a], t[a])];
-      const int r = fomo::rect_pick(e[0], e[1], e[2], n[0], n[1], n[2], gap2[0], gap2[1], gap2[2], box, ld, og);
+      const int r = fomo::rect_pick(e[0], e[1], e[2], n[0], n[1], n[2], fomo::rect_dmax(gap2), box, ld, og);
       ++queries;
       inside += in;
       ties += nbest > 1;
       if (r == expect) continue;
       if (r == -2) {
         ++fallbacks;
-        const bool allowed = !(gap2[0] > fomo::kRectGapTol * best && gap2[1] > fomo::kRectGapTol * best &&
-                               gap2[2] > fomo::kRectGapTol * best);
+        const bool allowed = !(best <= fomo::rect_dmax(gap2));
         if (!allowed && ++fails <= 10)
           std::printf("FAIL set %d query %d: fallback without a tiny gap (expected %lld)\n", set, qn,
                       (long long)expect);
