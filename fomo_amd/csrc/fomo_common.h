@@ -691,41 +691,11 @@ __global__ void __launch_bounds__(256) k_nn_los(const float4* __restrict__ pts, 
     }
   };
   // Rectilinear datasets (bin entries in cache, a small query state) keep two point records in
-  // flight; with a lookup grid the larger query state (the cell record) leaves room for one.
-  constexpr bool kDeep = RECT;
-  SearchBox boxN;
-  const float4* rpN = lslot;
-  float4 slN[kSlots];
-  bool actN, cellN;
-  float fC[4 * RS4], fB[4 * RS4];
-  query_issue(q, boxN, rpN, slN, actN, cellN);
-  int rC = query_resolve(boxN, rpN, slN, actN, cellN);
-  record_issue(rC, fC);
-  query_issue(G + q, boxN, rpN, slN, actN, cellN);
-  int rB = -1;
-  if constexpr (kDeep) {
-    rB = query_resolve(boxN, rpN, slN, actN, cellN);
-    record_issue(rB, fB);
-    query_issue(2 * G + q, boxN, rpN, slN, actN, cellN);
-  }
-  for (int kb = 0; kb < nz; kb += G) {
-    SearchBox boxA;
-    const float4* rpA = lslot;
-    float4 slA[kSlots];
-    bool actA = false, cellA = false;
-    int r2 = -1;
-    float f2[4 * RS4];
-    if constexpr (kDeep) {
-      // group kb + 2G: winner from the bin entries issued at the end of the last iteration, and
-      // its point record, two groups ahead of the integration
-      r2 = query_resolve(boxN, rpN, slN, actN, cellN);
-      record_issue(r2, f2);
-    } else {
-      // group kb + 2G's lookup cell record, group kb + G's winner and point record
-      query_issue(kb + 2 * G + q, boxA, rpA, slA, actA, cellA);
-      rB = query_resolve(boxN, rpN, slN, actN, cellN);
-      record_issue(rB, fB);
-    }
+  // flight; with a lookup grid the larger query state (the cell record) leaves room for one, and
+  // so does the direct pass (rare rays, one exp per pixel), whose code is kept compact.
+  constexpr bool kDeep = RECT && !DIRECT;
+  // the integration of group kb from its point records fC (winner rC)
+  const auto integrate = [&](const int kb, const float (&fC)[4 * RS4], const int rC) {
     // group kb: this lane's sample, the values write_sample would store
     const float vls = rC >= 0 ? (float)(((0.0 + (sb * cl) * (double)fC[0]) + (-sb * sl_) * (double)fC[1]) +
                                         cb * (double)fC[2])
@@ -883,18 +853,55 @@ __global__ void __launch_bounds__(256) k_nn_los(const float4* __restrict__ pts, 
         }
       });
     }
-    // advance the pipeline
-    if constexpr (kDeep) {
-      // the bin entries of group kb + 3G in flight during the next integration
+  };
+  SearchBox boxN;
+  const float4* rpN = lslot;
+  float4 slN[kSlots];
+  bool actN, cellN;
+  if constexpr (kDeep) {
+    // Three record sets in fixed roles, unrolled by three so that no set is copied while its load
+    // is in flight: step g integrates group g from `cur` and loads group g + 2G's record into
+    // `ahead` (the set group g - G used), then issues group g + 3G's bin entries.
+    float fa[4 * RS4], fb[4 * RS4], fc[4 * RS4];
+    int ra, rb, rc = -1;
+    query_issue(q, boxN, rpN, slN, actN, cellN);
+    ra = query_resolve(boxN, rpN, slN, actN, cellN);
+    record_issue(ra, fa);
+    query_issue(G + q, boxN, rpN, slN, actN, cellN);
+    rb = query_resolve(boxN, rpN, slN, actN, cellN);
+    record_issue(rb, fb);
+    query_issue(2 * G + q, boxN, rpN, slN, actN, cellN);
+    const auto step = [&](const int kb, const float (&cur)[4 * RS4], const int rcur, float (&ahead)[4 * RS4],
+                          int& rahead) {
+      rahead = query_resolve(boxN, rpN, slN, actN, cellN);
+      record_issue(rahead, ahead);
+      integrate(kb, cur, rcur);
       query_issue(kb + 3 * G + q, boxN, rpN, slN, actN, cellN);
-      rC = rB;
-      rB = r2;
-#pragma unroll
-      for (int c = 0; c < 4 * RS4; ++c) {
-        fC[c] = fB[c];
-        fB[c] = f2[c];
-      }
-    } else {
+    };
+    for (int kb = 0; kb < nz; kb += 3 * G) {
+      step(kb, fa, ra, fc, rc);
+      if (kb + G >= nz) break;
+      step(kb + G, fb, rb, fa, ra);
+      if (kb + 2 * G >= nz) break;
+      step(kb + 2 * G, fc, rc, fb, rb);
+    }
+  } else {
+    // group kb + 2G's lookup cell record and group kb + G's point record in flight while group kb
+    // is integrated
+    float fC[4 * RS4], fB[4 * RS4];
+    query_issue(q, boxN, rpN, slN, actN, cellN);
+    int rC = query_resolve(boxN, rpN, slN, actN, cellN);
+    record_issue(rC, fC);
+    query_issue(G + q, boxN, rpN, slN, actN, cellN);
+    for (int kb = 0; kb < nz; kb += G) {
+      SearchBox boxA;
+      const float4* rpA = lslot;
+      float4 slA[kSlots];
+      bool actA, cellA;
+      query_issue(kb + 2 * G + q, boxA, rpA, slA, actA, cellA);
+      const int rB = query_resolve(boxN, rpN, slN, actN, cellN);
+      record_issue(rB, fB);
+      integrate(kb, fC, rC);
       rC = rB;
 #pragma unroll
       for (int c = 0; c < 4 * RS4; ++c) fC[c] = fB[c];

@@ -3254,7 +3254,8 @@ bool fused_single_line(const float4* rec, int rs4, int64_t R, int nl) {
 
 // flags: R * ls.n ints of scratch. False when the view is left to k_nn + the integration kernels.
 bool enqueue_nn_los(hipStream_t s, const Dataset& ds, const ViewGeom* geom, int nx, int ny, int nz,
-                    const float4* rec, int rs4, const RecInfo& ri, const LineSet& ls, int nl, int32_t* flags) {
+                    const float4* rec, int rs4, const RecInfo& ri, const LineSet& ls, int nl, int32_t* flags,
+                    bool launch = true) {
   if (!rec || nl < 2 || ls.n < 1 || ls.n > kMaxLines) return false;
   const int64_t R = (int64_t)nx * ny;
   // k_los_spectral's lambda passes and their anchors
@@ -3262,6 +3263,7 @@ bool enqueue_nn_los(hipStream_t s, const Dataset& ds, const ViewGeom* geom, int 
   const int ch = pick_chunk(nl, &npass);
   if (ls.n == 1) {
     if (!fused_single_line(rec, rs4, R, nl)) return false;
+    if (!launch) return true;
     ProfScope ps("k_nn_los", s);
     const unsigned blocks = (unsigned)((R + 63) / 64);
     launch_nn_los(npass == 1 ? kNnLosSingle : kNnLosSingle2, ch, blocks, s, ds, geom, nx, ny, nz, rec, ri, ls, nl,
@@ -3273,6 +3275,7 @@ bool enqueue_nn_los(hipStream_t s, const Dataset& ds, const ViewGeom* geom, int 
   const bool shared = rs4 == 2 && ls.n <= 4 && shared_profile(ri, ls);
   const int G = shared ? 4 : (ls.n <= 2 ? 2 : (ls.n <= 4 ? 4 : 8));
   if ((rs4 == 2 && G > 4) || (rs4 != 2 && rs4 != 4)) return false;
+  if (!launch) return true;
   const int64_t rays_per_block = 4 * (64 / G);
   const unsigned blocks = (unsigned)((R + rays_per_block - 1) / rays_per_block);
   ProfScope ps("k_nn_los", s);
@@ -4648,6 +4651,17 @@ static int enqueue_batch(Batch& bt, hipStream_t s, hipStream_t s2) {
   const size_t outv = (size_t)R * std::max(bt.nl, 1);
   const int L = (int)gs.size();
   if (s2 && (int)bt.ev.size() < 2 * nv) return fail(FOMO_ERR_STATE, "batch events missing");
+  // whether the views take the fused pass (the same for every view: one resolution and line set)
+  bool fused = false;
+  {
+    std::vector<const float2*> ems0(L, nullptr);
+    std::vector<float*> outs0(L, nullptr);
+    std::vector<double> l0s0;
+    for (int gi = 0; gi < L; ++gi) l0s0.push_back(gs[gi]->lambda0);
+    const LineSet ls0 = make_lineset(ems0, outs0, l0s0, bt.width);
+    fused = enqueue_nn_los(s, *ds, nullptr, bt.nx, bt.ny, bt.nz, bt.rec.as<float4>(), bt.rs4, bt.ri, ls0, bt.nl,
+                           nullptr, false);
+  }
   // extents of all views: one pass over the points per kViewsPerPass views
   const unsigned nbp = enqueue_views_extents(s, *ds, bt.l, bt.b, bt.partial.as<double>());
   for (int v = 0; v < nv; ++v) {
@@ -4657,7 +4671,8 @@ static int enqueue_batch(Batch& bt, hipStream_t s, hipStream_t s2) {
     int32_t* flag = bt.flag.as<int32_t>() + (size_t)buf * R * kMaxLines;
     double* part = bt.partial.as<double>() + (size_t)v * 6 * kMaxPartials;
     ViewGeom* geom = bt.geom.as<ViewGeom>() + v;
-    if (s2 && v >= 2) HIPOK(hipStreamWaitEvent(s, bt.ev[nv + v - 2], 0));
+    // the unfused path's per-sample buffers of view v - 2 are free once its integration is done
+    if (s2 && v >= 2 && !fused) HIPOK(hipStreamWaitEvent(s, bt.ev[nv + v - 2], 0));
     enqueue_view_finalize(s, MODE_NN, bt.l[v], bt.b[v], bt.nx, bt.ny, bt.nz, bt.dn, part, nbp, geom);
     std::vector<const float2*> ems;
     std::vector<float*> outs;
@@ -4668,9 +4683,19 @@ static int enqueue_batch(Batch& bt, hipStream_t s, hipStream_t s2) {
       l0s.push_back(gs[gi]->lambda0);
     }
     LineSet ls = make_lineset(ems, outs, l0s, bt.width);
-    // one fused pass per view when it applies (then the views simply follow each other on s)
-    if (enqueue_nn_los(s, *ds, geom, bt.nx, bt.ny, bt.nz, bt.rec.as<float4>(), bt.rs4, bt.ri, ls, bt.nl, flag)) {
-      if (s2) HIPOK(hipEventRecord(bt.ev[nv + v], s));
+    // one fused pass per view when it applies; odd views on the second stream, so that a view's
+    // waves start while the previous one drains (the views' flags alternate between two buffers,
+    // each used by the views of one stream)
+    if (enqueue_nn_los(s, *ds, geom, bt.nx, bt.ny, bt.nz, bt.rec.as<float4>(), bt.rs4, bt.ri, ls, bt.nl, flag,
+                       false)) {
+      hipStream_t sv = s;
+      if (s2 && (v & 1)) {
+        HIPOK(hipEventRecord(bt.ev[v], s));
+        HIPOK(hipStreamWaitEvent(s2, bt.ev[v], 0));
+        sv = s2;
+      }
+      enqueue_nn_los(sv, *ds, geom, bt.nx, bt.ny, bt.nz, bt.rec.as<float4>(), bt.rs4, bt.ri, ls, bt.nl, flag);
+      if (s2) HIPOK(hipEventRecord(bt.ev[nv + v], sv));
       continue;
     }
     enqueue_nn(s, *ds, geom, bt.nx, bt.ny, bt.nz, bt.idx.as<int32_t>() + buf * S, bt.rayflag.as<uint8_t>() + buf * nn_scratch_bytes(R, S),
@@ -4684,7 +4709,12 @@ static int enqueue_batch(Batch& bt, hipStream_t s, hipStream_t s2) {
     enqueue_los(si, (int)R, bt.nz, ls, svl, geom, bt.nl, flag);
     if (s2) HIPOK(hipEventRecord(bt.ev[nv + v], s2));
   }
-  if (s2) HIPOK(hipStreamWaitEvent(s, bt.ev[2 * nv - 1], 0));  // join
+  // join: the last view's work on the second stream (views alternate there when fused)
+  if (s2) {
+    HIPOK(hipStreamWaitEvent(s, bt.ev[2 * nv - 1], 0));
+    const int last_odd = (nv & 1) ? nv - 2 : nv - 1;
+    if (fused && last_odd >= 1) HIPOK(hipStreamWaitEvent(s, bt.ev[nv + last_odd], 0));
+  }
   return FOMO_OK;
 }
 
