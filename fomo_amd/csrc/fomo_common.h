@@ -541,7 +541,7 @@ __global__ void __launch_bounds__(256) k_nn_los(const float4* __restrict__ pts, 
                                                 const float4* __restrict__ nlo, const float4* __restrict__ nhi,
                                                 BvhInfo bv, const ViewGeom* __restrict__ vgp, int nx, int ny, int nz,
                                                 const float4* __restrict__ rec, RecInfo ri, const LineSet ls, int nl,
-                                                int32_t* __restrict__ flags) {
+                                                int32_t* __restrict__ flags, int64_t out_vs, int64_t flag_vs) {
   constexpr int NL = 4 * RS4 - 4;  // lines a record holds
   constexpr int RPW = 64 / G;      // rays per wave
   static_assert(SL == 0 || (G == 4 && SL <= NL && CH % 4 == 0), "a shared profile runs 4 chains per ray");
@@ -561,7 +561,11 @@ __global__ void __launch_bounds__(256) k_nn_los(const float4* __restrict__ pts, 
   __shared__ __align__(16) unsigned char xch[kXch ? 256 * kSlot : 16];
   const int64_t ray = ((int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) * RPW + lane / G;
   const bool live = ray < R;
-  int32_t* const flag = flags + (size_t)(SL > 0 ? 0 : q) * R + (live ? ray : 0);
+  // view blockIdx.y of the launch: its geometry vgp[view], outputs ls.out[l] + view * out_vs,
+  // ray flags flags + view * flag_vs
+  const int view = (int)blockIdx.y;
+  const int64_t ooff = (int64_t)view * out_vs;
+  int32_t* const flag = flags + (int64_t)view * flag_vs + (size_t)(SL > 0 ? 0 : q) * R + (live ? ray : 0);
   // flagged (ray, line, lambda pass)s of this lane: bit l * NP + p (shared profile; flags[l * R +
   // ray] holds the passes' bits), bit 0 (line q). The direct pass reads them first: usually none
   // is set, and whole waves leave before any other work.
@@ -579,7 +583,7 @@ __global__ void __launch_bounds__(256) k_nn_los(const float4* __restrict__ pts, 
     if (__ballot(need != 0) == 0) return;
   }
   const int i = live ? (int)(ray / nx) : 0, j = live ? (int)(ray % nx) : 0;
-  const ViewGeom vg = *vgp;
+  const ViewGeom vg = vgp[view];
   const double minx = vg.ext[0], maxx = vg.ext[1], miny = vg.ext[2], maxy = vg.ext[3], minz = vg.ext[4];
   const double xx = double(j) / (nx - 1) * (maxx - minx) + minx;  // fomo-nearestneighbour.cpp:152
   const double yy = double(i) / (ny - 1) * (maxy - miny) + miny;  // :153
@@ -598,7 +602,7 @@ __global__ void __launch_bounds__(256) k_nn_los(const float4* __restrict__ pts, 
         Kq = ri.K[l];
         wla = ls.wla[l];
         l0c = ls.l0c[l];
-        out = ls.out[l];
+        out = ls.out[l] + ooff;
       }
   }
   bool integ = live && (SL > 0 || q < ls.n);
@@ -927,7 +931,7 @@ __global__ void __launch_bounds__(256) k_nn_los(const float4* __restrict__ pts, 
       for (int l = 0; l < SL; ++l) {
         const unsigned bit = 1u << (l * NP + p);
         if (l >= ls.n || (DIRECT ? !(need & bit) : (bad & bit) != 0)) continue;
-        float* o = ls.out[l] + (size_t)ray * nl + p * CH;
+        float* o = ls.out[l] + ooff + (size_t)ray * nl + p * CH;
 #pragma unroll
         for (int m = 0; m < CJ; ++m)
           if (p * CH + 4 * m + q < nl) o[4 * m + q] = (float)(sc * acc[(p * SL + l) * CJ + m]);
@@ -966,6 +970,9 @@ struct NnLosArgs {
   LineSet ls;
   int nl;
   int32_t* flags;
+  int nviews;      // views of one launch (blockIdx.y): geom[v], outputs + v * out_vs, flags + v * flag_vs
+  int64_t out_vs;  // floats between two views' outputs of one line
+  int64_t flag_vs; // ints between two views' ray flags
 };
 
 // k_nn_los<RS4, CH, G, DIRECT, SL, NP> variants the host launches, as (RS4, G, SL, NP).

@@ -431,8 +431,8 @@ __global__ void k_view_rotate_thomson(const float4* __restrict__ pts, int64_t n,
 // Reduce the per-block extents and derive the view geometry (one block).
 // NN: maxdistance fomo-nearestneighbour.cpp:138, deltaz :142-143, scale :235-267.
 // Thomson: maxdistance fomo-thomson.cpp:249-260, scale :350-359.
-__global__ void k_view_finalize(const double* __restrict__ partial, int np, int mode, int nx, int ny, int nz, int dn,
-                                double cb, double sb, double cl, double sl, double bf, ViewGeom* __restrict__ out) {
+__device__ void view_finalize(const double* __restrict__ partial, int np, int mode, int nx, int ny, int nz, int dn,
+                              double cb, double sb, double cl, double sl, double bf, ViewGeom* __restrict__ out) {
   __shared__ double ext[6];
   double v[6] = {INFINITY, -INFINITY, INFINITY, -INFINITY, INFINITY, -INFINITY};
   for (int i = threadIdx.x; i < np; i += blockDim.x)
@@ -472,6 +472,21 @@ __global__ void k_view_finalize(const double* __restrict__ partial, int np, int 
   g.apix = apix;
   g.scale = pl * 1e8 * apix;
   *out = g;
+}
+
+__global__ void k_view_finalize(const double* __restrict__ partial, int np, int mode, int nx, int ny, int nz, int dn,
+                                double cb, double sb, double cl, double sl, double bf, ViewGeom* __restrict__ out) {
+  view_finalize(partial, np, mode, nx, ny, nz, dn, cb, sb, cl, sl, bf, out);
+}
+
+// The views of a batch in one launch: block v finalizes view v from its partials
+// (partial + v * pstride) and its host trig (trig[5 v ..]: cos b, sin b, cos l, sin l, blend).
+__global__ void k_view_finalize_views(const double* __restrict__ partial, int64_t pstride, int np, int mode, int nx,
+                                      int ny, int nz, int dn, const double* __restrict__ trig,
+                                      ViewGeom* __restrict__ out) {
+  const int v = (int)blockIdx.x;
+  const double* t = trig + 5 * v;
+  view_finalize(partial + v * pstride, np, mode, nx, ny, nz, dn, t[0], t[1], t[2], t[3], t[4], out + v);
 }
 
 // ------------------------------------------------------------------------------------
@@ -2122,6 +2137,8 @@ struct Batch {
   DevBuf idx;      // per-sample sorted positions (double-buffered)
   DevBuf rayflag;  // nn_scratch_bytes per buffer: ray flags all-zero between uses, deep queue
   DevBuf rec;  // per-point records {vel, T, all lines' peaks}
+  DevBuf trig;  // 5 doubles per view (view_trig), for the views' finalize in one launch
+  bool fused = false;  // the views take the fused pass, all in one launch
   RecInfo ri{};
   DevBuf sem;  // per-sample (peak, fwhm), nlines x S
   int rs4 = 0;
@@ -2973,11 +2990,18 @@ template <class T> T* find_obj(std::map<int, std::unique_ptr<T>>& m, int id) {
 }
 
 // Enqueue the per-view rotate + finalize kernels (mode NN or Thomson) on stream s.
+// Host trig of a view (glibc): cos b, sin b, cos l, sin l and the Thomson blend of the search
+// half-width, 1/(1+exp(-10(|b|-0.1745))) (fomo-thomson.cpp:249).
+void view_trig(double l, double b, double t[5]) {
+  t[0] = cos(b), t[1] = sin(b), t[2] = cos(l), t[3] = sin(l);
+  t[4] = 1.0 / (1.0 + exp(-10 * (std::fabs(b) - 0.1745)));
+}
+
 void enqueue_view_finalize(hipStream_t s, int mode, double l, double b, int nx, int ny, int nz, int dn,
                            const double* partial, unsigned nb, ViewGeom* geom) {
-  const double cb = cos(b), sb = sin(b), cl = cos(l), sl = sin(l);
-  const double bf = 1.0 / (1.0 + exp(-10 * (std::fabs(b) - 0.1745)));  // fomo-thomson.cpp:249
-  k_view_finalize<<<1, kBlock, 0, s>>>(partial, nb, mode, nx, ny, nz, dn, cb, sb, cl, sl, bf, geom);
+  double t[5];
+  view_trig(l, b, t);
+  k_view_finalize<<<1, kBlock, 0, s>>>(partial, nb, mode, nx, ny, nz, dn, t[0], t[1], t[2], t[3], t[4], geom);
 }
 
 void enqueue_view_geom(hipStream_t s, const Dataset& ds, int mode, double l, double b, int nx, int ny, int nz, int dn,
@@ -3209,10 +3233,10 @@ void enqueue_los(hipStream_t s, int R, int nz, const LineSet& ls, const float* v
 // compiled in fomo_nnlos_part.hip; this picks the one for lambda chunk ch.
 void launch_nn_los(int variant, int ch, unsigned blocks, hipStream_t s, const Dataset& ds, const ViewGeom* geom,
                    int nx, int ny, int nz, const float4* rec, const RecInfo& ri, const LineSet& ls, int nl,
-                   int32_t* flags) {
+                   int32_t* flags, int nviews, int64_t out_vs, int64_t flag_vs) {
   const NnLosArgs a{ds.pts.as<float4>(), ds.grid, ds.lslot.as<float4>(), ds.lcand.as<float4>(), ds.lk,
                     ds.mpts.as<float4>(), ds.nlo.as<float4>(), ds.nhi.as<float4>(), ds.bvh, geom, nx, ny, nz, rec,
-                    ri, ls, nl, flags};
+                    ri, ls, nl, flags, nviews, out_vs, flag_vs};
   switch (ch) {
 #define FOMO_NNL_CH(C)                              \
   case C:                                           \
@@ -3253,9 +3277,12 @@ bool fused_single_line(const float4* rec, int rs4, int64_t R, int nl) {
 }
 
 // flags: R * ls.n ints of scratch. False when the view is left to k_nn + the integration kernels.
+// The fused pass of nviews views of one resolution (geom[v]; outputs ls.out[l] + v * out_vs; ray
+// flags flags + v * flag_vs, R * kMaxLines each) in one launch, when it applies to them (false:
+// nothing enqueued). launch = false only answers whether it applies.
 bool enqueue_nn_los(hipStream_t s, const Dataset& ds, const ViewGeom* geom, int nx, int ny, int nz,
                     const float4* rec, int rs4, const RecInfo& ri, const LineSet& ls, int nl, int32_t* flags,
-                    bool launch = true) {
+                    bool launch = true, int nviews = 1, int64_t out_vs = 0, int64_t flag_vs = 0) {
   if (!rec || nl < 2 || ls.n < 1 || ls.n > kMaxLines) return false;
   const int64_t R = (int64_t)nx * ny;
   // k_los_spectral's lambda passes and their anchors
@@ -3267,7 +3294,7 @@ bool enqueue_nn_los(hipStream_t s, const Dataset& ds, const ViewGeom* geom, int 
     ProfScope ps("k_nn_los", s);
     const unsigned blocks = (unsigned)((R + 63) / 64);
     launch_nn_los(npass == 1 ? kNnLosSingle : kNnLosSingle2, ch, blocks, s, ds, geom, nx, ny, nz, rec, ri, ls, nl,
-                  flags);
+                  flags, nviews, out_vs, flag_vs);
     return true;
   }
   if (npass != 1) return false;
@@ -3280,7 +3307,7 @@ bool enqueue_nn_los(hipStream_t s, const Dataset& ds, const ViewGeom* geom, int 
   const unsigned blocks = (unsigned)((R + rays_per_block - 1) / rays_per_block);
   ProfScope ps("k_nn_los", s);
   const int variant = shared ? kNnLosShared : (rs4 == 2 && G == 2) ? kNnLosLines2 : rs4 == 2 ? kNnLosLines4 : kNnLosLines8;
-  launch_nn_los(variant, ch, blocks, s, ds, geom, nx, ny, nz, rec, ri, ls, nl, flags);
+  launch_nn_los(variant, ch, blocks, s, ds, geom, nx, ny, nz, rec, ri, ls, nl, flags, nviews, out_vs, flag_vs);
   return true;
 }
 
@@ -4651,19 +4678,32 @@ static int enqueue_batch(Batch& bt, hipStream_t s, hipStream_t s2) {
   const size_t outv = (size_t)R * std::max(bt.nl, 1);
   const int L = (int)gs.size();
   if (s2 && (int)bt.ev.size() < 2 * nv) return fail(FOMO_ERR_STATE, "batch events missing");
-  // whether the views take the fused pass (the same for every view: one resolution and line set)
-  bool fused = false;
-  {
-    std::vector<const float2*> ems0(L, nullptr);
-    std::vector<float*> outs0(L, nullptr);
-    std::vector<double> l0s0;
-    for (int gi = 0; gi < L; ++gi) l0s0.push_back(gs[gi]->lambda0);
-    const LineSet ls0 = make_lineset(ems0, outs0, l0s0, bt.width);
-    fused = enqueue_nn_los(s, *ds, nullptr, bt.nx, bt.ny, bt.nz, bt.rec.as<float4>(), bt.rs4, bt.ri, ls0, bt.nl,
-                           nullptr, false);
-  }
   // extents of all views: one pass over the points per kViewsPerPass views
   const unsigned nbp = enqueue_views_extents(s, *ds, bt.l, bt.b, bt.partial.as<double>());
+  if (bt.fused) {
+    // every view's geometry in one launch, then every view's fused pass in one launch (blockIdx.y =
+    // view): no launch gaps or per-view tails between the views
+    k_view_finalize_views<<<nv, kBlock, 0, s>>>(bt.partial.as<double>(), 6 * kMaxPartials, (int)nbp, MODE_NN, bt.nx,
+                                                bt.ny, bt.nz, bt.dn, bt.trig.as<double>(), bt.geom.as<ViewGeom>());
+    HIPOK(hipGetLastError());
+    constexpr int kMaxViewsPerLaunch = 65535;  // grid y extent
+    for (int v0 = 0; v0 < nv; v0 += kMaxViewsPerLaunch) {
+      const int nvl = std::min(kMaxViewsPerLaunch, nv - v0);
+      std::vector<const float2*> ems(L, nullptr);
+      std::vector<float*> outs;
+      std::vector<double> l0s;
+      for (int gi = 0; gi < L; ++gi) {
+        outs.push_back(bt.out.as<float>() + ((size_t)v0 * L + gi) * outv);
+        l0s.push_back(gs[gi]->lambda0);
+      }
+      const LineSet ls = make_lineset(ems, outs, l0s, bt.width);
+      if (!enqueue_nn_los(s, *ds, bt.geom.as<ViewGeom>() + v0, bt.nx, bt.ny, bt.nz, bt.rec.as<float4>(), bt.rs4, bt.ri,
+                          ls, bt.nl, bt.flag.as<int32_t>() + (size_t)v0 * R * kMaxLines, true, nvl,
+                          (int64_t)L * (int64_t)outv, R * kMaxLines))
+        return fail(FOMO_ERR_STATE, "batch: the fused pass no longer applies");
+    }
+    return FOMO_OK;
+  }
   for (int v = 0; v < nv; ++v) {
     const int buf = s2 ? (v & 1) : 0;
     float* svl = bt.aux.as<float>() + buf * S;
@@ -4671,8 +4711,8 @@ static int enqueue_batch(Batch& bt, hipStream_t s, hipStream_t s2) {
     int32_t* flag = bt.flag.as<int32_t>() + (size_t)buf * R * kMaxLines;
     double* part = bt.partial.as<double>() + (size_t)v * 6 * kMaxPartials;
     ViewGeom* geom = bt.geom.as<ViewGeom>() + v;
-    // the unfused path's per-sample buffers of view v - 2 are free once its integration is done
-    if (s2 && v >= 2 && !fused) HIPOK(hipStreamWaitEvent(s, bt.ev[nv + v - 2], 0));
+    // the per-sample buffers of view v - 2 are free once its integration is done
+    if (s2 && v >= 2) HIPOK(hipStreamWaitEvent(s, bt.ev[nv + v - 2], 0));
     enqueue_view_finalize(s, MODE_NN, bt.l[v], bt.b[v], bt.nx, bt.ny, bt.nz, bt.dn, part, nbp, geom);
     std::vector<const float2*> ems;
     std::vector<float*> outs;
@@ -4683,21 +4723,6 @@ static int enqueue_batch(Batch& bt, hipStream_t s, hipStream_t s2) {
       l0s.push_back(gs[gi]->lambda0);
     }
     LineSet ls = make_lineset(ems, outs, l0s, bt.width);
-    // one fused pass per view when it applies; odd views on the second stream, so that a view's
-    // waves start while the previous one drains (the views' flags alternate between two buffers,
-    // each used by the views of one stream)
-    if (enqueue_nn_los(s, *ds, geom, bt.nx, bt.ny, bt.nz, bt.rec.as<float4>(), bt.rs4, bt.ri, ls, bt.nl, flag,
-                       false)) {
-      hipStream_t sv = s;
-      if (s2 && (v & 1)) {
-        HIPOK(hipEventRecord(bt.ev[v], s));
-        HIPOK(hipStreamWaitEvent(s2, bt.ev[v], 0));
-        sv = s2;
-      }
-      enqueue_nn_los(sv, *ds, geom, bt.nx, bt.ny, bt.nz, bt.rec.as<float4>(), bt.rs4, bt.ri, ls, bt.nl, flag);
-      if (s2) HIPOK(hipEventRecord(bt.ev[nv + v], sv));
-      continue;
-    }
     enqueue_nn(s, *ds, geom, bt.nx, bt.ny, bt.nz, bt.idx.as<int32_t>() + buf * S, bt.rayflag.as<uint8_t>() + buf * nn_scratch_bytes(R, S),
                bt.rec.as<float4>(), bt.rs4, L, svl, sem, bt.ri);
     hipStream_t si = s;
@@ -4709,12 +4734,7 @@ static int enqueue_batch(Batch& bt, hipStream_t s, hipStream_t s2) {
     enqueue_los(si, (int)R, bt.nz, ls, svl, geom, bt.nl, flag);
     if (s2) HIPOK(hipEventRecord(bt.ev[nv + v], s2));
   }
-  // join: the last view's work on the second stream (views alternate there when fused)
-  if (s2) {
-    HIPOK(hipStreamWaitEvent(s, bt.ev[2 * nv - 1], 0));
-    const int last_odd = (nv & 1) ? nv - 2 : nv - 1;
-    if (fused && last_odd >= 1) HIPOK(hipStreamWaitEvent(s, bt.ev[nv + last_odd], 0));
-  }
+  if (s2) HIPOK(hipStreamWaitEvent(s, bt.ev[2 * nv - 1], 0));  // join
   return FOMO_OK;
 }
 
@@ -4758,19 +4778,36 @@ int fomo_batch_create(int ngoft, const int* gofts, int nviews, const double* l, 
   bt->nx = nx; bt->ny = ny; bt->nz = nz; bt->nl = nl; bt->dn = dn; bt->width = width;
   const int64_t R = (int64_t)nx * ny, S = R * nz;
   const int nlo = std::max(nl, 1);
-  if (bt->partial.alloc(sizeof(double) * 6 * kMaxPartials * nviews) || bt->geom.alloc(sizeof(ViewGeom) * nviews) ||
-      bt->sem.alloc(sizeof(float2) * S * ngoft * 2) || bt->aux.alloc(sizeof(float) * S * 2) ||
-      bt->idx.alloc(sizeof(int32_t) * S * 2) || bt->rayflag.alloc(nn_scratch_bytes(R, S) * 2) ||
-      bt->flag.alloc(sizeof(int32_t) * R * kMaxLines * 2) ||
-      bt->out.alloc(sizeof(float) * (size_t)R * nlo * ngoft * nviews))
-    return fail(FOMO_ERR_NOMEM, "device allocation failed for batch");
-  HIPOK(hipMemsetAsync(bt->rayflag.p, 0, nn_scratch_bytes(R, S) * 2, g_ctx.stream));
   {
     std::vector<const Goft*> cg;
     for (int gid : bt->gofts) cg.push_back(find_obj(g_ctx.gofts, gid));
     if (int rc = build_records(g_ctx.stream, *ds, cg, bt->rec, &bt->rs4, &bt->ri)) return rc;
     if (int rc = sync_stream()) return rc;
   }
+  // whether the views take the fused pass (the same for all: one resolution and line set)
+  {
+    std::vector<const float2*> ems0(ngoft, nullptr);
+    std::vector<float*> outs0(ngoft, nullptr);
+    std::vector<double> l0s;
+    for (int gid : bt->gofts) l0s.push_back(find_obj(g_ctx.gofts, gid)->lambda0);
+    const LineSet ls0 = make_lineset(ems0, outs0, l0s, width);
+    bt->fused = enqueue_nn_los(g_ctx.stream, *ds, nullptr, nx, ny, nz, bt->rec.as<float4>(), bt->rs4, bt->ri, ls0, nl,
+                               nullptr, false);
+  }
+  // the fused pass keeps ray flags per view and no per-sample buffers; the per-view passes
+  // double-buffer both
+  const bool fz = bt->fused;
+  std::vector<double> trig(5 * (size_t)nviews);
+  for (int v = 0; v < nviews; ++v) view_trig(l[v], b[v], trig.data() + 5 * v);
+  if (bt->partial.alloc(sizeof(double) * 6 * kMaxPartials * nviews) || bt->geom.alloc(sizeof(ViewGeom) * nviews) ||
+      bt->trig.alloc(sizeof(double) * trig.size()) ||
+      (!fz && (bt->sem.alloc(sizeof(float2) * S * ngoft * 2) || bt->aux.alloc(sizeof(float) * S * 2) ||
+               bt->idx.alloc(sizeof(int32_t) * S * 2) || bt->rayflag.alloc(nn_scratch_bytes(R, S) * 2))) ||
+      bt->flag.alloc(sizeof(int32_t) * R * kMaxLines * (fz ? nviews : 2)) ||
+      bt->out.alloc(sizeof(float) * (size_t)R * nlo * ngoft * nviews))
+    return fail(FOMO_ERR_NOMEM, "device allocation failed for batch");
+  HIPOK(hipMemcpyAsync(bt->trig.p, trig.data(), sizeof(double) * trig.size(), hipMemcpyHostToDevice, g_ctx.stream));
+  if (!fz) HIPOK(hipMemsetAsync(bt->rayflag.p, 0, nn_scratch_bytes(R, S) * 2, g_ctx.stream));
   for (int i = 0; i < 2 * nviews; ++i) {
     hipEvent_t e;
     HIPOK(hipEventCreateWithFlags(&e, hipEventDisableTiming));

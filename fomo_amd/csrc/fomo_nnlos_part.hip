@@ -20,12 +20,13 @@ constexpr bool kRect = FOMO_NNLOS_RECT != 0;
 
 template <int RS4, int CH, int G, int SL, int NP>
 void launch_pair(unsigned blocks, hipStream_t s, const NnLosArgs& a) {
-  k_nn_los<RS4, CH, G, false, SL, NP, kRect><<<blocks, 256, 0, s>>>(a.pts, a.g, a.lslot, a.lcand, a.lk, a.mpts, a.nlo,
-                                                                    a.nhi, a.bvh, a.geom, a.nx, a.ny, a.nz, a.rec,
-                                                                    a.ri, a.ls, a.nl, a.flags);
-  k_nn_los<RS4, CH, G, true, SL, NP, kRect><<<blocks, 256, 0, s>>>(a.pts, a.g, a.lslot, a.lcand, a.lk, a.mpts, a.nlo,
-                                                                   a.nhi, a.bvh, a.geom, a.nx, a.ny, a.nz, a.rec,
-                                                                   a.ri, a.ls, a.nl, a.flags);
+  const dim3 grid(blocks, (unsigned)a.nviews);
+  k_nn_los<RS4, CH, G, false, SL, NP, kRect><<<grid, 256, 0, s>>>(a.pts, a.g, a.lslot, a.lcand, a.lk, a.mpts, a.nlo,
+                                                                  a.nhi, a.bvh, a.geom, a.nx, a.ny, a.nz, a.rec, a.ri,
+                                                                  a.ls, a.nl, a.flags, a.out_vs, a.flag_vs);
+  k_nn_los<RS4, CH, G, true, SL, NP, kRect><<<grid, 256, 0, s>>>(a.pts, a.g, a.lslot, a.lcand, a.lk, a.mpts, a.nlo,
+                                                                 a.nhi, a.bvh, a.geom, a.nx, a.ny, a.nz, a.rec, a.ri,
+                                                                 a.ls, a.nl, a.flags, a.out_vs, a.flag_vs);
 }
 
 }  // namespace
