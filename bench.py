@@ -41,10 +41,14 @@ PEAK_FP64 = 78.6e12  # FLOP/s, MI355X dense FP64 (vector and matrix peaks coinci
 # Algorithmic FP64 work of the reference's integration per LOS sample x lambda pixel (W): the
 # Gaussian term and its accumulation, about 8 FLOP besides the exp (fomo-nearestneighbour.cpp:200)
 FLOP_PER_W = 8.0
-# HBM bytes per launch (read + write) of the default workload's kernels, from two rocprofv3 --pmc
-# runs (FETCH_SIZE x 2.000, WRITE_SIZE x 1.000, the factors a 1 GiB copy kernel gives), per index
-# kind of the cube: profiles/batched_render_256cube_counters.md
-MEASURED_TRAFFIC = {"scattered": {"k_nn_los": 1.445e9}, "rectilinear": {}}
+# HBM bytes per view (read + write) of the default workload's fused pass, from two rocprofv3 --pmc
+# runs (FETCH_SIZE x 2.000, WRITE_SIZE x 1.000, the factors a 1 GiB copy kernel gives) divided by
+# the views of a launch, per index kind of the cube: profiles/batched_render_256cube_counters.md
+MEASURED_TRAFFIC = {"scattered": {"k_nn_los": 1.445e9}, "rectilinear": {"k_nn_los": 5.371e8}}
+# FP64 work the fused pass executes per view (FLOP: 2 per FMA, 1 per MUL/ADD/transcendental,
+# from SQ_INSTS_VALU_{FMA,MUL,ADD,TRANS}_F64 x 64 lanes) on the rectilinear default cube, beside the
+# reference formula's count: profiles/batched_render_256cube_counters.md
+EXECUTED_FP64 = {"rectilinear": {"k_nn_los": 3.690e9}}
 
 
 def log(*a):
@@ -162,8 +166,12 @@ def main():
                                              "k_los_spectral")}
     stats = {k: v for k, v in stats.items() if v[0] > 0}
     log("per-kernel stats (count, total ms): " + json.dumps(stats))
+    # views per launch: the batch runs every view's fused pass in one launch (two profile passes)
+    vpl = {k: 1 for k in stats}
+    if "k_nn_los" in stats:
+        vpl["k_nn_los"] = max(1, round(2 * nv / stats["k_nn_los"][0]))
     bytes_per = {
-        # algorithmic bytes per launch
+        # algorithmic bytes per view
         "k_nn": S_view * 4 + n * 16 + (dims[0] * dims[1] * dims[2] + 1) * 4,
         # fused NN + integration: what the unfused pipeline moves for the same work, the NN pass
         # (S * 4 written, N * 12 read) plus each line's integration (S * 16 gathered, R * nl * 4
@@ -177,34 +185,43 @@ def main():
     roof = None
     if cnt > 0:
         avg_s = tot / cnt / 1e3
+        views = vpl.get(dom, 1)
         # HBM bytes per launch measured with rocprofv3 FETCH_SIZE / WRITE_SIZE (calibrated on a plain
         # copy kernel) for this exact workload: profiles/batched_render_256cube_counters.md
         default_cfg = (m, nx, nz, nl, len(ems)) == (256, 256, 128, 60, 4)
         traffic = MEASURED_TRAFFIC[index_kind].get(dom) if default_cfg else None
+        traffic = traffic * views if traffic else None
         share = {k: stats[k][1] / max(1e-9, sum(v[1] for v in stats.values())) for k in stats}
         if dom == "k_nn_los":
             # the fused pass is bound by the 128-byte lines its NN queries fetch at random (lookup
             # cell records, winners' records); its FP64 integration is reported beside, with the
             # algorithmic FLOPs of the reference's formula
-            alg = bytes_per[dom]
-            flops = FLOP_PER_W * S_view * nl * len(ems)
+            alg = bytes_per[dom] * views
+            flops = FLOP_PER_W * S_view * nl * len(ems) * views
+            exe = EXECUTED_FP64.get(index_kind, {}).get(dom) if default_cfg else None
+            exe = exe * views if exe else None
             roof = {"kernel": dom, "bound": "hbm", "achieved": alg / avg_s / 1e9, "peak": PEAK_HBM / 1e9,
                     "unit": "GB/s", "frac": alg / avg_s / PEAK_HBM, "traffic": traffic,
-                    "avg_launch_ms": avg_s * 1e3, "algorithmic_bytes_per_launch": alg,
+                    "avg_launch_ms": avg_s * 1e3, "views_per_launch": views, "ms_per_view": avg_s * 1e3 / views,
+                    "algorithmic_bytes_per_launch": alg, "algorithmic_bytes_per_view": bytes_per[dom],
                     "measured": {"GBps": traffic / avg_s / 1e9 if traffic else None,
                                  "frac_of_peak": traffic / avg_s / PEAK_HBM if traffic else None,
                                  "frac_of_copy_probe": traffic / avg_s / 1e9 / copy_gbps if traffic else None,
                                  "frac_of_random_line_probe": traffic / avg_s / 1e9 / gather_gbps if traffic
                                  else None},
-                    "fp64": {"algorithmic_flops_per_launch": flops, "flop_per_W": FLOP_PER_W,
-                             "achieved_TFLOPs": flops / avg_s / 1e12, "peak_TFLOPs": PEAK_FP64 / 1e12,
-                             "frac": flops / avg_s / PEAK_FP64},
+                    "fp64": {"reference_formula_flops_per_launch": flops, "flop_per_W": FLOP_PER_W,
+                             "reference_formula_TFLOPs": flops / avg_s / 1e12,
+                             "executed_flops_per_launch": exe,
+                             "executed_TFLOPs": exe / avg_s / 1e12 if exe else None,
+                             "peak_TFLOPs": PEAK_FP64 / 1e12,
+                             "frac": exe / avg_s / PEAK_FP64 if exe else None},
                     "share_of_step": share}
         else:
-            ach = bytes_per[dom] / avg_s
+            ach = bytes_per[dom] * views / avg_s
             roof = {"kernel": dom, "bound": "hbm", "achieved": ach / 1e9, "peak": PEAK_HBM / 1e9, "unit": "GB/s",
                     "frac": ach / PEAK_HBM, "traffic": traffic, "avg_launch_ms": avg_s * 1e3,
-                    "algorithmic_bytes_per_launch": bytes_per[dom], "share_of_step": share}
+                    "views_per_launch": views, "algorithmic_bytes_per_launch": bytes_per[dom] * views,
+                    "share_of_step": share}
 
     # ---- self-check: one oblique view of the batch, all lines, on a band of rows, against the FP64
     #      oracle (NN choices of the same view bit-exact, ties counted; images within 1e-5)
@@ -288,12 +305,12 @@ def main():
         c2, tot2 = F.profile_stats("k_nn_los")
         bt2.close()
         scattered = {"ms_per_step": dt2 / args.steps * 1e3, "value": W_step * args.steps / dt2,
-                     "index_build_ms": t_scat * 1e3, "k_nn_los_ms": tot2 / c2 if c2 else None,
+                     "index_build_ms": t_scat * 1e3, "k_nn_los_ms_per_view": tot2 / nv if c2 else None,
                      "lookup": ds.lookup_info(),
                      "what": "the batched step with the cube indexed as scattered points (its product-grid axes "
                              "not used)"}
         log(f"scattered index: {scattered['ms_per_step']:.2f} ms/step, index {t_scat * 1e3:.0f} ms, k_nn_los "
-            f"{scattered['k_nn_los_ms']} ms")
+            f"{scattered['k_nn_los_ms_per_view']} ms per view")
         ds.set_rectilinear(True)
 
     # ---- serving: the request mix of the serving configuration (NN, CGAL/Linear tetra and Thomson

@@ -534,14 +534,19 @@ __device__ __forceinline__ double group_bcast(double x) {
 // whose recurrence anchors would under- or overflow is flagged (flags[line * R + ray]) and redone
 // by the DIRECT instantiation, launched right after, with one exp per sample and lambda pixel.
 // ------------------------------------------------------------------------------------
-template <int RS4, int CH, int G, bool DIRECT, int SL, int NP = 1, bool RECT = false>
-__global__ void __launch_bounds__(256) k_nn_los(const float4* __restrict__ pts, GridInfo g,
-                                                const float4* __restrict__ lslot, const float4* __restrict__ lcand,
-                                                LookupInfo lk, const float4* __restrict__ mpts,
-                                                const float4* __restrict__ nlo, const float4* __restrict__ nhi,
-                                                BvhInfo bv, const ViewGeom* __restrict__ vgp, int nx, int ny, int nz,
-                                                const float4* __restrict__ rec, RecInfo ri, const LineSet ls, int nl,
-                                                int32_t* __restrict__ flags, int64_t out_vs, int64_t flag_vs) {
+// One wave of the pass: rays wave * RPW .. of view `view`. The main pass (DIRECT = false) lists
+// the waves that flagged a ray in dlist (dlist[0]: count, dlist[2 ..]: view * waves per view +
+// wave), for the direct pass.
+template <int RS4, int CH, int G, bool DIRECT, int SL, int NP, bool RECT>
+__device__ __forceinline__ void nn_los_wave(const int view, const int64_t wave, const float4* __restrict__ pts,
+                                            const GridInfo& g, const float4* __restrict__ lslot,
+                                            const float4* __restrict__ lcand, const LookupInfo& lk,
+                                            const float4* __restrict__ mpts, const float4* __restrict__ nlo,
+                                            const float4* __restrict__ nhi, const BvhInfo& bv,
+                                            const ViewGeom* __restrict__ vgp, int nx, int ny, int nz,
+                                            const float4* __restrict__ rec, const RecInfo& ri, const LineSet& ls,
+                                            int nl, int32_t* __restrict__ flags, int64_t out_vs, int64_t flag_vs,
+                                            int32_t* __restrict__ dlist) {
   constexpr int NL = 4 * RS4 - 4;  // lines a record holds
   constexpr int RPW = 64 / G;      // rays per wave
   static_assert(SL == 0 || (G == 4 && SL <= NL && CH % 4 == 0), "a shared profile runs 4 chains per ray");
@@ -559,11 +564,10 @@ __global__ void __launch_bounds__(256) k_nn_los(const float4* __restrict__ pts, 
   constexpr bool kXch = SL > 0 && !DIRECT;
   constexpr int kSlot = NP * 80 + 32;  // per pass {(g, R) x 4, q16, pad}, then peaks (float4), bits
   __shared__ __align__(16) unsigned char xch[kXch ? 256 * kSlot : 16];
-  const int64_t ray = ((int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) * RPW + lane / G;
+  const int64_t ray = wave * RPW + lane / G;
   const bool live = ray < R;
-  // view blockIdx.y of the launch: its geometry vgp[view], outputs ls.out[l] + view * out_vs,
-  // ray flags flags + view * flag_vs
-  const int view = (int)blockIdx.y;
+  // the view's geometry vgp[view], outputs ls.out[l] + view * out_vs, ray flags flags + view *
+  // flag_vs
   const int64_t ooff = (int64_t)view * out_vs;
   int32_t* const flag = flags + (int64_t)view * flag_vs + (size_t)(SL > 0 ? 0 : q) * R + (live ? ray : 0);
   // flagged (ray, line, lambda pass)s of this lane: bit l * NP + p (shared profile; flags[l * R +
@@ -694,10 +698,11 @@ __global__ void __launch_bounds__(256) k_nn_los(const float4* __restrict__ pts, 
       f[4 * c] = a.x, f[4 * c + 1] = a.y, f[4 * c + 2] = a.z, f[4 * c + 3] = a.w;
     }
   };
-  // Rectilinear datasets (bin entries in cache, a small query state) keep two point records in
-  // flight; with a lookup grid the larger query state (the cell record) leaves room for one, and
-  // so does the direct pass (rare rays, one exp per pixel), whose code is kept compact.
-  constexpr bool kDeep = RECT && !DIRECT;
+  // Rectilinear datasets (bin entries in cache, a small query state) keep two point records of
+  // 32 bytes in flight; with a lookup grid the larger query state (the cell record) leaves room
+  // for one, and so do larger records and the direct pass (rare rays, one exp per pixel), whose
+  // code is kept compact.
+  constexpr bool kDeep = RECT && !DIRECT && RS4 == 2;
   // the integration of group kb from its point records fC (winner rC)
   const auto integrate = [&](const int kb, const float (&fC)[4 * RS4], const int rC) {
     // group kb: this lane's sample, the values write_sample would store
@@ -917,6 +922,13 @@ __global__ void __launch_bounds__(256) k_nn_los(const float4* __restrict__ pts, 
       cellN = cellA;
     }
   }
+  if constexpr (!DIRECT) {
+    // a wave with a flagged ray goes on the direct pass's list
+    if (__ballot(bad != 0) != 0 && lane == 0) {
+      const int64_t wpv = (R + RPW - 1) / RPW;
+      dlist[2 + atomicAdd(&dlist[0], 1)] = (int32_t)(view * wpv + wave);
+    }
+  }
   if (!integ) return;
   const double sc = vg.scale;
   if constexpr (SL > 0) {
@@ -949,6 +961,51 @@ __global__ void __launch_bounds__(256) k_nn_los(const float4* __restrict__ pts, 
 }
 
 
+// The fused pass. Main pass (DIRECT = false): grid (waves / 4, views), one wave per 64 / G rays.
+// Direct pass: any grid; its waves take the listed waves of the main pass in turn, and the last
+// block to finish clears the list (dlist[0], dlist[1]) for the next main pass. The list starts
+// cleared: the view finalize kernels clear it before the first main pass of a buffer.
+template <int RS4, int CH, int G, bool DIRECT, int SL, int NP = 1, bool RECT = false>
+__global__ void __launch_bounds__(256) k_nn_los(const float4* __restrict__ pts, GridInfo g,
+                                                const float4* __restrict__ lslot, const float4* __restrict__ lcand,
+                                                LookupInfo lk, const float4* __restrict__ mpts,
+                                                const float4* __restrict__ nlo, const float4* __restrict__ nhi,
+                                                BvhInfo bv, const ViewGeom* __restrict__ vgp, int nx, int ny, int nz,
+                                                const float4* __restrict__ rec, RecInfo ri, const LineSet ls, int nl,
+                                                int32_t* __restrict__ flags, int64_t out_vs, int64_t flag_vs,
+                                                int32_t* __restrict__ dlist) {
+  // the wave's index is uniform: kept in scalar registers
+  const int w4 = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+  if constexpr (!DIRECT) {
+    nn_los_wave<RS4, CH, G, false, SL, NP, RECT>((int)blockIdx.y, (int64_t)blockIdx.x * 4 + w4, pts,
+                                                 g, lslot, lcand, lk, mpts, nlo, nhi, bv, vgp, nx, ny, nz, rec, ri,
+                                                 ls, nl, flags, out_vs, flag_vs, dlist);
+  } else {
+    constexpr int RPW = 64 / G;
+    const int64_t wpv = ((int64_t)nx * ny + RPW - 1) / RPW;
+    const int n = dlist[0];
+    for (int it = (int)blockIdx.x * 4 + w4; it < n; it += (int)gridDim.x * 4) {
+      const int item = __builtin_amdgcn_readfirstlane(dlist[2 + it]);
+      nn_los_wave<RS4, CH, G, true, SL, NP, RECT>((int)(item / wpv), item % wpv, pts, g, lslot, lcand, lk, mpts, nlo,
+                                                  nhi, bv, vgp, nx, ny, nz, rec, ri, ls, nl, flags, out_vs, flag_vs,
+                                                  dlist);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      __threadfence();
+      if (atomicAdd(&dlist[1], 1) == (int)gridDim.x - 1) {
+        atomicExch(&dlist[0], 0);
+        atomicExch(&dlist[1], 0);
+      }
+    }
+  }
+}
+
+// Ints of a direct-pass list for nviews views of R rays (at least 8 rays per wave).
+__host__ __device__ constexpr int64_t nn_los_list_ints(int64_t R, int64_t nviews) {
+  return 2 + nviews * ((R + 7) / 8);
+}
+
 // ------------------------------------------------------------------------------------
 // Host launch of k_nn_los. The host code builds the arguments and picks the variant; the
 // instantiations live in fomo_nnlos_part.hip.
@@ -973,6 +1030,7 @@ struct NnLosArgs {
   int nviews;      // views of one launch (blockIdx.y): geom[v], outputs + v * out_vs, flags + v * flag_vs
   int64_t out_vs;  // floats between two views' outputs of one line
   int64_t flag_vs; // ints between two views' ray flags
+  int32_t* dlist;  // direct-pass list, nn_los_list_ints(R, nviews) ints, cleared before the main pass
 };
 
 // k_nn_los<RS4, CH, G, DIRECT, SL, NP> variants the host launches, as (RS4, G, SL, NP).

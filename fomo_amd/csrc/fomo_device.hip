@@ -432,8 +432,11 @@ __global__ void k_view_rotate_thomson(const float4* __restrict__ pts, int64_t n,
 // NN: maxdistance fomo-nearestneighbour.cpp:138, deltaz :142-143, scale :235-267.
 // Thomson: maxdistance fomo-thomson.cpp:249-260, scale :350-359.
 __device__ void view_finalize(const double* __restrict__ partial, int np, int mode, int nx, int ny, int nz, int dn,
-                              double cb, double sb, double cl, double sl, double bf, ViewGeom* __restrict__ out) {
+                              double cb, double sb, double cl, double sl, double bf, ViewGeom* __restrict__ out,
+                              int32_t* __restrict__ clear2) {
   __shared__ double ext[6];
+  // the fused pass's direct-pass list (count, finished blocks) starts cleared
+  if (clear2 && threadIdx.x < 2) clear2[threadIdx.x] = 0;
   double v[6] = {INFINITY, -INFINITY, INFINITY, -INFINITY, INFINITY, -INFINITY};
   for (int i = threadIdx.x; i < np; i += blockDim.x)
 #pragma unroll
@@ -475,18 +478,20 @@ __device__ void view_finalize(const double* __restrict__ partial, int np, int mo
 }
 
 __global__ void k_view_finalize(const double* __restrict__ partial, int np, int mode, int nx, int ny, int nz, int dn,
-                                double cb, double sb, double cl, double sl, double bf, ViewGeom* __restrict__ out) {
-  view_finalize(partial, np, mode, nx, ny, nz, dn, cb, sb, cl, sl, bf, out);
+                                double cb, double sb, double cl, double sl, double bf, ViewGeom* __restrict__ out,
+                                int32_t* __restrict__ clear2) {
+  view_finalize(partial, np, mode, nx, ny, nz, dn, cb, sb, cl, sl, bf, out, clear2);
 }
 
 // The views of a batch in one launch: block v finalizes view v from its partials
 // (partial + v * pstride) and its host trig (trig[5 v ..]: cos b, sin b, cos l, sin l, blend).
 __global__ void k_view_finalize_views(const double* __restrict__ partial, int64_t pstride, int np, int mode, int nx,
                                       int ny, int nz, int dn, const double* __restrict__ trig,
-                                      ViewGeom* __restrict__ out) {
+                                      ViewGeom* __restrict__ out, int32_t* __restrict__ clear2) {
   const int v = (int)blockIdx.x;
   const double* t = trig + 5 * v;
-  view_finalize(partial + v * pstride, np, mode, nx, ny, nz, dn, t[0], t[1], t[2], t[3], t[4], out + v);
+  view_finalize(partial + v * pstride, np, mode, nx, ny, nz, dn, t[0], t[1], t[2], t[3], t[4], out + v,
+                v == 0 ? clear2 : nullptr);
 }
 
 // ------------------------------------------------------------------------------------
@@ -2997,15 +3002,17 @@ void view_trig(double l, double b, double t[5]) {
   t[4] = 1.0 / (1.0 + exp(-10 * (std::fabs(b) - 0.1745)));
 }
 
+// clear2: a fused pass's direct-pass list to clear (or null).
 void enqueue_view_finalize(hipStream_t s, int mode, double l, double b, int nx, int ny, int nz, int dn,
-                           const double* partial, unsigned nb, ViewGeom* geom) {
+                           const double* partial, unsigned nb, ViewGeom* geom, int32_t* clear2 = nullptr) {
   double t[5];
   view_trig(l, b, t);
-  k_view_finalize<<<1, kBlock, 0, s>>>(partial, nb, mode, nx, ny, nz, dn, t[0], t[1], t[2], t[3], t[4], geom);
+  k_view_finalize<<<1, kBlock, 0, s>>>(partial, nb, mode, nx, ny, nz, dn, t[0], t[1], t[2], t[3], t[4], geom,
+                                       clear2);
 }
 
 void enqueue_view_geom(hipStream_t s, const Dataset& ds, int mode, double l, double b, int nx, int ny, int nz, int dn,
-                       float* aux, double* partial, ViewGeom* geom) {
+                       float* aux, double* partial, ViewGeom* geom, int32_t* clear2 = nullptr) {
   unsigned nb = std::min<unsigned>(nblk(ds.n), kMaxPartials);
   const double cb = cos(b), sb = sin(b), cl = cos(l), sl = sin(l);
   if (mode == MODE_NN) {
@@ -3025,7 +3032,7 @@ void enqueue_view_geom(hipStream_t s, const Dataset& ds, int mode, double l, dou
     ProfScope ps("k_view_rotate_thomson", s);
     k_view_rotate_thomson<<<nb, kBlock, 0, s>>>(ds.pts.as<float4>(), ds.n, cbc, sbc, cl, sl, blend, aux, partial);
   }
-  enqueue_view_finalize(s, mode, l, b, nx, ny, nz, dn, partial, nb, geom);
+  enqueue_view_finalize(s, mode, l, b, nx, ny, nz, dn, partial, nb, geom, clear2);
 }
 
 // Extents of many NN views (a batch), kViewsPerPass views per pass over the points; partial of
@@ -3233,10 +3240,10 @@ void enqueue_los(hipStream_t s, int R, int nz, const LineSet& ls, const float* v
 // compiled in fomo_nnlos_part.hip; this picks the one for lambda chunk ch.
 void launch_nn_los(int variant, int ch, unsigned blocks, hipStream_t s, const Dataset& ds, const ViewGeom* geom,
                    int nx, int ny, int nz, const float4* rec, const RecInfo& ri, const LineSet& ls, int nl,
-                   int32_t* flags, int nviews, int64_t out_vs, int64_t flag_vs) {
+                   int32_t* flags, int nviews, int64_t out_vs, int64_t flag_vs, int32_t* dlist) {
   const NnLosArgs a{ds.pts.as<float4>(), ds.grid, ds.lslot.as<float4>(), ds.lcand.as<float4>(), ds.lk,
                     ds.mpts.as<float4>(), ds.nlo.as<float4>(), ds.nhi.as<float4>(), ds.bvh, geom, nx, ny, nz, rec,
-                    ri, ls, nl, flags, nviews, out_vs, flag_vs};
+                    ri, ls, nl, flags, nviews, out_vs, flag_vs, dlist};
   switch (ch) {
 #define FOMO_NNL_CH(C)                              \
   case C:                                           \
@@ -3278,11 +3285,12 @@ bool fused_single_line(const float4* rec, int rs4, int64_t R, int nl) {
 
 // flags: R * ls.n ints of scratch. False when the view is left to k_nn + the integration kernels.
 // The fused pass of nviews views of one resolution (geom[v]; outputs ls.out[l] + v * out_vs; ray
-// flags flags + v * flag_vs, R * kMaxLines each) in one launch, when it applies to them (false:
-// nothing enqueued). launch = false only answers whether it applies.
+// flags flags + v * flag_vs, R * lines each) in one launch, when it applies to them (false:
+// nothing enqueued). dlist: nn_los_list_ints(R, nviews) ints for the direct pass's list, cleared by
+// the views' finalize before. launch = false only answers whether it applies.
 bool enqueue_nn_los(hipStream_t s, const Dataset& ds, const ViewGeom* geom, int nx, int ny, int nz,
                     const float4* rec, int rs4, const RecInfo& ri, const LineSet& ls, int nl, int32_t* flags,
-                    bool launch = true, int nviews = 1, int64_t out_vs = 0, int64_t flag_vs = 0) {
+                    int32_t* dlist, bool launch = true, int nviews = 1, int64_t out_vs = 0, int64_t flag_vs = 0) {
   if (!rec || nl < 2 || ls.n < 1 || ls.n > kMaxLines) return false;
   const int64_t R = (int64_t)nx * ny;
   // k_los_spectral's lambda passes and their anchors
@@ -3294,7 +3302,7 @@ bool enqueue_nn_los(hipStream_t s, const Dataset& ds, const ViewGeom* geom, int 
     ProfScope ps("k_nn_los", s);
     const unsigned blocks = (unsigned)((R + 63) / 64);
     launch_nn_los(npass == 1 ? kNnLosSingle : kNnLosSingle2, ch, blocks, s, ds, geom, nx, ny, nz, rec, ri, ls, nl,
-                  flags, nviews, out_vs, flag_vs);
+                  flags, nviews, out_vs, flag_vs, dlist);
     return true;
   }
   if (npass != 1) return false;
@@ -3307,7 +3315,7 @@ bool enqueue_nn_los(hipStream_t s, const Dataset& ds, const ViewGeom* geom, int 
   const unsigned blocks = (unsigned)((R + rays_per_block - 1) / rays_per_block);
   ProfScope ps("k_nn_los", s);
   const int variant = shared ? kNnLosShared : (rs4 == 2 && G == 2) ? kNnLosLines2 : rs4 == 2 ? kNnLosLines4 : kNnLosLines8;
-  launch_nn_los(variant, ch, blocks, s, ds, geom, nx, ny, nz, rec, ri, ls, nl, flags, nviews, out_vs, flag_vs);
+  launch_nn_los(variant, ch, blocks, s, ds, geom, nx, ny, nz, rec, ri, ls, nl, flags, nviews, out_vs, flag_vs, dlist);
   return true;
 }
 
@@ -3385,7 +3393,8 @@ void render_sizes(int mode, int64_t n, int64_t R, int64_t S, int nl, int64_t sz[
   sz[2] = (int64_t)sizeof(int32_t) * S;
   sz[3] = (int64_t)sizeof(float) * (mode != MODE_THOMSON ? S : n);
   sz[4] = (int64_t)sizeof(float) * R * std::max(nl, 1);
-  sz[5] = mode != MODE_THOMSON ? (int64_t)sizeof(int32_t) * R : 0;
+  // ray flags, then (NN) the fused pass's direct-pass list
+  sz[5] = mode != MODE_THOMSON ? (int64_t)sizeof(int32_t) * (R + nn_los_list_ints(R, 1)) : 0;
   sz[6] = mode != MODE_THOMSON ? (int64_t)sizeof(float2) * S : 0;
   sz[7] = nn_scratch_bytes(R, S);
 }
@@ -3446,8 +3455,10 @@ int enqueue_projection(hipStream_t s, const Dataset& ds, const Goft& gc, const R
 int enqueue_render(hipStream_t s, const Dataset& ds, const Goft& gc, int mode, double l, double b, int nx, int ny,
                    int nz, int nl, double width, int dn, const RenderBufs& rb) {
   const int64_t R = (int64_t)nx * ny;
+  // NN renders: the fused pass's direct-pass list follows the R ray flags in rb.flag
+  int32_t* const dlist = mode == MODE_NN ? rb.flag + R : nullptr;
   enqueue_view_geom(s, ds, mode == MODE_THOMSON ? MODE_THOMSON : MODE_NN, l, b, nx, ny, nz, dn, rb.aux, rb.partial,
-                    rb.geom);
+                    rb.geom, dlist);
   if (mode == MODE_TET || mode == MODE_LINEAR) {
     // the tetra path leaves rb.idx (S ints) unused: it holds the view's redo list
     if (int rc = enqueue_tet(s, ds, rb.geom, nx, ny, nz, mode == MODE_LINEAR, gc.rec.as<float4>(), gc.rs4, gc.ri, 1,
@@ -3459,7 +3470,7 @@ int enqueue_render(hipStream_t s, const Dataset& ds, const Goft& gc, int mode, d
     if (int rc = enqueue_projection(s, ds, gc, rb, nx, ny, nl, width)) return rc;
   } else if (mode == MODE_NN) {
     LineSet ls = make_lineset({rb.sem}, {rb.out}, {gc.lambda0}, width);
-    if (!enqueue_nn_los(s, ds, rb.geom, nx, ny, nz, gc.rec.as<float4>(), gc.rs4, gc.ri, ls, nl, rb.flag)) {
+    if (!enqueue_nn_los(s, ds, rb.geom, nx, ny, nz, gc.rec.as<float4>(), gc.rs4, gc.ri, ls, nl, rb.flag, dlist)) {
       enqueue_nn(s, ds, rb.geom, nx, ny, nz, rb.idx, rb.rayflag, gc.rec.as<float4>(), gc.rs4, 1, rb.aux, rb.sem,
                  gc.ri);
       enqueue_los(s, (int)R, nz, ls, rb.aux, rb.geom, nl, rb.flag);
@@ -4683,8 +4694,10 @@ static int enqueue_batch(Batch& bt, hipStream_t s, hipStream_t s2) {
   if (bt.fused) {
     // every view's geometry in one launch, then every view's fused pass in one launch (blockIdx.y =
     // view): no launch gaps or per-view tails between the views
+    int32_t* const dlist = bt.flag.as<int32_t>() + (size_t)nv * R * kMaxLines;
     k_view_finalize_views<<<nv, kBlock, 0, s>>>(bt.partial.as<double>(), 6 * kMaxPartials, (int)nbp, MODE_NN, bt.nx,
-                                                bt.ny, bt.nz, bt.dn, bt.trig.as<double>(), bt.geom.as<ViewGeom>());
+                                                bt.ny, bt.nz, bt.dn, bt.trig.as<double>(), bt.geom.as<ViewGeom>(),
+                                                dlist);
     HIPOK(hipGetLastError());
     constexpr int kMaxViewsPerLaunch = 65535;  // grid y extent
     for (int v0 = 0; v0 < nv; v0 += kMaxViewsPerLaunch) {
@@ -4698,7 +4711,7 @@ static int enqueue_batch(Batch& bt, hipStream_t s, hipStream_t s2) {
       }
       const LineSet ls = make_lineset(ems, outs, l0s, bt.width);
       if (!enqueue_nn_los(s, *ds, bt.geom.as<ViewGeom>() + v0, bt.nx, bt.ny, bt.nz, bt.rec.as<float4>(), bt.rs4, bt.ri,
-                          ls, bt.nl, bt.flag.as<int32_t>() + (size_t)v0 * R * kMaxLines, true, nvl,
+                          ls, bt.nl, bt.flag.as<int32_t>() + (size_t)v0 * R * kMaxLines, dlist, true, nvl,
                           (int64_t)L * (int64_t)outv, R * kMaxLines))
         return fail(FOMO_ERR_STATE, "batch: the fused pass no longer applies");
     }
@@ -4792,7 +4805,7 @@ int fomo_batch_create(int ngoft, const int* gofts, int nviews, const double* l, 
     for (int gid : bt->gofts) l0s.push_back(find_obj(g_ctx.gofts, gid)->lambda0);
     const LineSet ls0 = make_lineset(ems0, outs0, l0s, width);
     bt->fused = enqueue_nn_los(g_ctx.stream, *ds, nullptr, nx, ny, nz, bt->rec.as<float4>(), bt->rs4, bt->ri, ls0, nl,
-                               nullptr, false);
+                               nullptr, nullptr, false);
   }
   // the fused pass keeps ray flags per view and no per-sample buffers; the per-view passes
   // double-buffer both
@@ -4803,7 +4816,8 @@ int fomo_batch_create(int ngoft, const int* gofts, int nviews, const double* l, 
       bt->trig.alloc(sizeof(double) * trig.size()) ||
       (!fz && (bt->sem.alloc(sizeof(float2) * S * ngoft * 2) || bt->aux.alloc(sizeof(float) * S * 2) ||
                bt->idx.alloc(sizeof(int32_t) * S * 2) || bt->rayflag.alloc(nn_scratch_bytes(R, S) * 2))) ||
-      bt->flag.alloc(sizeof(int32_t) * R * kMaxLines * (fz ? nviews : 2)) ||
+      bt->flag.alloc(sizeof(int32_t) * (fz ? R * kMaxLines * nviews + nn_los_list_ints(R, nviews)
+                                           : R * kMaxLines * 2)) ||
       bt->out.alloc(sizeof(float) * (size_t)R * nlo * ngoft * nviews))
     return fail(FOMO_ERR_NOMEM, "device allocation failed for batch");
   HIPOK(hipMemcpyAsync(bt->trig.p, trig.data(), sizeof(double) * trig.size(), hipMemcpyHostToDevice, g_ctx.stream));

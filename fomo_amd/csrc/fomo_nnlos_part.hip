@@ -3,6 +3,8 @@
 // and kind (-DFOMO_NNLOS_GROUP=0..7, two CH values each, one of them at least 36 for the two-pass
 // variant; -DFOMO_NNLOS_RECT=0 for candidate lookup grids, 1 for rectilinear datasets) so that the
 // 352 instantiations compile in parallel.
+#include <algorithm>
+
 #include "fomo_common.h"
 
 #ifndef FOMO_NNLOS_GROUP
@@ -17,16 +19,20 @@ namespace fomo_dev {
 namespace {
 
 constexpr bool kRect = FOMO_NNLOS_RECT != 0;
+constexpr int64_t kDirectBlocks = 512;  // blocks of the direct pass
 
 template <int RS4, int CH, int G, int SL, int NP>
 void launch_pair(unsigned blocks, hipStream_t s, const NnLosArgs& a) {
   const dim3 grid(blocks, (unsigned)a.nviews);
   k_nn_los<RS4, CH, G, false, SL, NP, kRect><<<grid, 256, 0, s>>>(a.pts, a.g, a.lslot, a.lcand, a.lk, a.mpts, a.nlo,
                                                                   a.nhi, a.bvh, a.geom, a.nx, a.ny, a.nz, a.rec, a.ri,
-                                                                  a.ls, a.nl, a.flags, a.out_vs, a.flag_vs);
-  k_nn_los<RS4, CH, G, true, SL, NP, kRect><<<grid, 256, 0, s>>>(a.pts, a.g, a.lslot, a.lcand, a.lk, a.mpts, a.nlo,
-                                                                 a.nhi, a.bvh, a.geom, a.nx, a.ny, a.nz, a.rec, a.ri,
-                                                                 a.ls, a.nl, a.flags, a.out_vs, a.flag_vs);
+                                                                  a.ls, a.nl, a.flags, a.out_vs, a.flag_vs, a.dlist);
+  // the direct pass walks the main pass's list of waves with flagged rays
+  const unsigned dblocks = (unsigned)std::min<int64_t>((int64_t)blocks * a.nviews, kDirectBlocks);
+  k_nn_los<RS4, CH, G, true, SL, NP, kRect><<<dblocks, 256, 0, s>>>(a.pts, a.g, a.lslot, a.lcand, a.lk, a.mpts,
+                                                                    a.nlo, a.nhi, a.bvh, a.geom, a.nx, a.ny, a.nz,
+                                                                    a.rec, a.ri, a.ls, a.nl, a.flags, a.out_vs,
+                                                                    a.flag_vs, a.dlist);
 }
 
 }  // namespace
