@@ -50,6 +50,7 @@ _SIGS = {
     "fomo_dataset_lookup_groups": (ctypes.c_int, [ctypes.c_int, _c_i64_p]),
     "fomo_dataset_set_rectilinear": (ctypes.c_int, [ctypes.c_int, ctypes.c_int]),
     "fomo_dataset_rect_info": (ctypes.c_int, [ctypes.c_int, _c_int_p]),
+    "fomo_dataset_grid_info": (ctypes.c_int, [ctypes.c_int, _c_int_p, _c_int_p]),
     "fomo_table_read": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int, _c_dbl_p, _c_dbl_p,
                                        ctypes.c_char_p, ctypes.c_int, _c_int_p, _c_int_p, _c_flt_p, _c_flt_p,
                                        _c_flt_p]),
@@ -295,6 +296,13 @@ class Dataset:
         a = (ctypes.c_int * 3)()
         _check(lib().fomo_dataset_rect_info(self.id, a))
         return tuple(a) if a[0] else None
+
+    def grid_info(self):
+        """Index kind ("scattered", "rectilinear" or "masked") and the grid's values per axis."""
+        k = ctypes.c_int()
+        a = (ctypes.c_int * 3)()
+        _check(lib().fomo_dataset_grid_info(self.id, ctypes.byref(k), a))
+        return ("scattered", "rectilinear", "masked")[k.value], (tuple(a) if a[0] else None)
 
     def lookup_groups(self):
         """Far and outer groups of the lookup grid and how many hold a shared candidate list."""

@@ -60,6 +60,11 @@ struct LookupInfo {
   // grid of the axis values rv[a], and a point's sorted position is its grid position
   // i + n0 (j + n1 k) (rect_geom.h)
   int rect;
+  // masked grids (mrect != 0): the points are a subset of the product grid of rv; gpos gives the
+  // sorted position of the point on each grid node (-1: none). The candidate grid is built too,
+  // for the targets whose nearest grid node holds no point.
+  int mrect;
+  const int32_t* gpos;
   int rn[3];                 // values per axis (bins: kRectBinsPerValue times as many)
   float ro[3], rinv[3];      // bin origin and inverse bin width per axis
   double rdmax;              // largest nearest distance without a possible tie two values away
@@ -278,18 +283,30 @@ __device__ __forceinline__ int rect_resolve(const float4* __restrict__ pts, cons
       [&](int64_t gp) { return __float_as_int(pts[gp].w); });
 }
 
+// Masked grids: the nearest point from the grid, or -2 (nearest point outside the search box) or
+// -3 (no point on the nearest grid node: the caller uses the candidate grid).
+__device__ __forceinline__ int mrect_resolve(const float4* __restrict__ pts, const LookupInfo& lk,
+                                             const float4 (&e)[kSlots], const fomo::SearchBox& box) {
+  return fomo::mrect_pick(
+      e[0], e[1], e[2], lk.rn[0], lk.rn[1], lk.rn[2], lk.rdmax, box,
+      [&](int a, int k) { return lk.rv[(a > 0 ? lk.rn[0] : 0) + (a > 1 ? lk.rn[1] : 0) + k]; },
+      [&](int64_t g) { return lk.gpos[g]; }, [&](int sp) { return __float_as_int(pts[sp].w); });
+}
+
 // Fast path: the unconstrained nearest point from the target's lookup cell. Returns its sorted
 // position if it lies strictly inside the search box, else -2 (the caller runs the exact query).
 __device__ __forceinline__ int nn_lookup(const float4* __restrict__ pts, const float4* __restrict__ lslot,
                                          const float4* __restrict__ lcand, const LookupInfo& lk, float t0, float t1,
                                          float t2, float lo0, float lo1, float lo2, float hi0, float hi1, float hi2) {
-  if (lk.rect) {
+  if (lk.rect || lk.mrect) {
     float4 e[kSlots];
     rect_issue(lk, t0, t1, t2, e);
     fomo::SearchBox box;
     box.t0 = t0, box.t1 = t1, box.t2 = t2;
     box.lo0 = lo0, box.lo1 = lo1, box.lo2 = lo2, box.hi0 = hi0, box.hi1 = hi1, box.hi2 = hi2;
-    return rect_resolve(pts, lk, e, box);
+    if (lk.rect) return rect_resolve(pts, lk, e, box);
+    const int r = mrect_resolve(pts, lk, e, box);
+    if (r != -3 || !lk.enabled) return r == -3 ? -2 : r;
   }
   const int64_t L = lookup_cell(lk, t0, t1, t2);
   if (L < 0) return lk.ohdr ? outer_lookup(pts, lcand, lk, t0, t1, t2, lo0, lo1, lo2, hi0, hi1, hi2) : -2;

@@ -155,6 +155,9 @@ __global__ void k_reduce_partials(const double* __restrict__ partial, int np, do
   block_minmax6(v, out6);
 }
 
+// A product grid may have holes (a masked grid) when it has at most this many nodes per point.
+constexpr int64_t kMaskedGridMaxRatio = 4;
+
 // Grid position of every point on a candidate product grid (axis values v0, v1, v2 ascending):
 // perm[gid] = original index. Sets *bad when a coordinate is not an axis value or two points
 // share a grid position (perm must be all -1 on entry).
@@ -854,7 +857,7 @@ __device__ __forceinline__ void nn_sample(const float4* __restrict__ pts, const 
   const bool may_hold = (float)(p0 + md0) > g.bmin[0] && (float)(p0 - md0) < g.bmax[0] &&
                         (float)(p1 + md0) > g.bmin[1] && (float)(p1 - md0) < g.bmax[1] &&
                         (float)(p2 + md0) > g.bmin[2] && (float)(p2 - md0) < g.bmax[2];
-  if ((lk.enabled || lk.rect) && may_hold) {
+  if ((lk.enabled || lk.rect || lk.mrect) && may_hold) {
     const double md = vg.md;
     r = nn_lookup(pts, lslot, lcand, lk, (float)p0, (float)p1, (float)p2, (float)(p0 - md), (float)(p1 - md),
                   (float)(p2 - md), (float)(p0 + md), (float)(p1 + md), (float)(p2 + md));
@@ -2081,6 +2084,11 @@ struct Dataset {
   // is built (the NN query reads the axes: rect_geom.h)
   bool allow_rect = true;  // detect product grids when indexing (fomo_dataset_set_rectilinear)
   bool rect = false;
+  // masked grids: the points are a subset (at least a quarter) of a product grid, each on its own
+  // node (a Cartesian mesh cut to a region, like the corona's shell); indexed as scattered points,
+  // plus the grid axes and gpos, which k_nn tries first
+  bool mrect = false;
+  DevBuf gpos;         // int32 per grid node: sorted position of the point on it, -1 for none
   int rect_n[3] = {0, 0, 0};
   float rect_o[3] = {0, 0, 0}, rect_inv[3] = {0, 0, 0};
   double rect_gap2[3] = {0, 0, 0};
@@ -2601,9 +2609,11 @@ int build_lookup(Dataset& ds) {
   ds.ncand = ds.nlcells = ds.nfallback = 0;
   ds.far_groups = ds.far_lists = ds.full_groups = ds.outer_groups = ds.outer_lists = 0;
   if (ds.lookup_subdiv <= 0) return FOMO_OK;
-  if (ds.rect) {
+  if (ds.rect || ds.mrect) {
     LookupInfo& lk = ds.lk;
-    lk.rect = 1;
+    lk.rect = ds.rect ? 1 : 0;
+    lk.mrect = ds.mrect ? 1 : 0;
+    lk.gpos = ds.mrect ? ds.gpos.as<int32_t>() : nullptr;
     for (int a = 0; a < 3; ++a) {
       lk.rn[a] = ds.rect_n[a];
       lk.ro[a] = ds.rect_o[a];
@@ -2612,7 +2622,7 @@ int build_lookup(Dataset& ds) {
     lk.rdmax = fomo::rect_dmax(ds.rect_gap2);
     lk.rv = ds.rv.as<float>();
     lk.rbin = ds.rbin.as<float4>();
-    return FOMO_OK;
+    if (ds.rect) return FOMO_OK;
   }
   hipStream_t s = g_ctx.stream;
   const GridInfo& g = ds.grid;
@@ -2634,7 +2644,7 @@ int build_lookup(Dataset& ds) {
     if (ncl_est <= 24.0 * ds.n + 65536 && ncl_est <= (double)(1LL << 30)) break;
   }
   if (subdiv < 1) return FOMO_OK;
-  LookupInfo lk{};
+  LookupInfo lk = ds.lk;  // a masked grid's axes (set above) stay; the rest is zero
   edges(subdiv, lk.h);
   double ext = 0, mag = 0;
   int64_t ncl = 1;
@@ -2765,9 +2775,11 @@ int build_lookup(Dataset& ds) {
 int build_rect(Dataset& ds, bool* ok) {
   *ok = false;
   ds.rect = false;
+  ds.mrect = false;
   ds.rv.release();
   ds.rbin.release();
   ds.corners.release();
+  ds.gpos.release();
   const int64_t n = ds.n;
   if (!ds.allow_rect || n < 2) return FOMO_OK;
   hipStream_t s = g_ctx.stream;
@@ -2792,14 +2804,15 @@ int build_rect(Dataset& ds, bool* ok) {
     HIPOK(hipMemcpyAsync(&na, dcount.p, sizeof(na), hipMemcpyDeviceToHost, s));
     if (int rc = sync_stream()) return rc;
     prod *= (int64_t)na;
-    if (na == 0 || n % (int64_t)na != 0 || prod > n) return FOMO_OK;
+    // a full grid (prod == n) or a masked one, at least a quarter full (prod <= 4 n)
+    if (na == 0 || prod > kMaskedGridMaxRatio * n || prod >= (int64_t)INT32_MAX) return FOMO_OK;
     vals[a].resize(na);
     HIPOK(hipMemcpyAsync(vals[a].data(), uniq, sizeof(float) * na, hipMemcpyDeviceToHost, s));
     if (int rc = sync_stream()) return rc;
     for (float f : vals[a])
       if (!std::isfinite(f)) return FOMO_OK;
   }
-  if (prod != n) return FOMO_OK;
+  const bool masked = prod != n;
   // axis tables
   int64_t nv = 0, nbt = 0;
   for (int a = 0; a < 3; ++a) {
@@ -2829,15 +2842,23 @@ int build_rect(Dataset& ds, bool* ok) {
   HIPOK(hipMemcpyAsync(ds.rv.p, hv.data(), sizeof(float) * nv, hipMemcpyHostToDevice, s));
   HIPOK(hipMemcpyAsync(ds.rbin.p, hb.data(), sizeof(float4) * nbt, hipMemcpyHostToDevice, s));
   HIPOK(hipMemcpyAsync(ds.corners.p, hc.data(), sizeof(float4) * 8, hipMemcpyHostToDevice, s));
-  // grid positions: every point on an axis value, no two on one position
-  HIPOK(ds.perm.ensure(sizeof(int32_t) * n));
-  HIPOK(hipMemsetAsync(ds.perm.p, 0xff, sizeof(int32_t) * n, s));
+  // grid positions: every point on an axis value, no two on one position. A full grid's map is
+  // the sorted order itself (perm); a masked grid's (original indices for now) goes to gpos
+  int32_t* gmap = nullptr;
+  if (masked) {
+    if (ds.gpos.alloc(sizeof(int32_t) * prod)) return FOMO_OK;  // no room: index as scattered points only
+    gmap = ds.gpos.as<int32_t>();
+  } else {
+    HIPOK(ds.perm.ensure(sizeof(int32_t) * n));
+    gmap = ds.perm.as<int32_t>();
+  }
+  HIPOK(hipMemsetAsync(gmap, 0xff, sizeof(int32_t) * prod, s));
   int* bad = reinterpret_cast<int*>(dcount.p);
   HIPOK(hipMemsetAsync(bad, 0, sizeof(int), s));
   const float* v = ds.rv.as<float>();
   k_rect_map<<<nblk(n), kBlock, 0, s>>>(ds.raw[0].as<float>(), ds.raw[1].as<float>(), ds.raw[2].as<float>(), n, v,
                                         v + ds.rect_n[0], v + ds.rect_n[0] + ds.rect_n[1], ds.rect_n[0],
-                                        ds.rect_n[1], ds.rect_n[2], ds.perm.as<int32_t>(), bad);
+                                        ds.rect_n[1], ds.rect_n[2], gmap, bad);
   HIPOK(hipGetLastError());
   int hbad = 1;
   HIPOK(hipMemcpyAsync(&hbad, bad, sizeof(int), hipMemcpyDeviceToHost, s));
@@ -2846,11 +2867,25 @@ int build_rect(Dataset& ds, bool* ok) {
     ds.rv.release();
     ds.rbin.release();
     ds.corners.release();
+    ds.gpos.release();
+    return FOMO_OK;
+  }
+  if (masked) {
+    ds.mrect = true;  // gpos becomes sorted positions once the cell list has sorted the points
+    ds.corners.release();
     return FOMO_OK;
   }
   ds.rect = true;
   *ok = true;
   return FOMO_OK;
+}
+
+// gpos[g] = original index -> sorted position (inv[original] = sorted position), -1 kept.
+__global__ void k_gpos_sorted(int32_t* __restrict__ gpos, int64_t ng, const int32_t* __restrict__ inv) {
+  for (int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; g < ng; g += (int64_t)gridDim.x * blockDim.x) {
+    const int32_t o = gpos[g];
+    if (o >= 0) gpos[g] = inv[o];
+  }
 }
 
 // Cell list of a scattered dataset: points sorted by the key of their cell of ds.grid (x fastest),
@@ -2956,6 +2991,16 @@ int build_index(Dataset& ds, double cell_scale) {
     ds.cstart.release();
   } else if (int rc2 = build_cell_list(ds)) {
     return rc2;
+  }
+  if (ds.mrect) {
+    // the masked grid's map: original indices -> sorted positions of the cell order
+    DevBuf inv;
+    if (inv.alloc(sizeof(int32_t) * n)) return fail(FOMO_ERR_NOMEM, "device allocation failed (masked grid)");
+    k_invert_perm<<<nblk(n), kBlock, 0, s>>>(ds.perm.as<int32_t>(), n, inv.as<int32_t>());
+    const int64_t ng = (int64_t)ds.rect_n[0] * ds.rect_n[1] * ds.rect_n[2];
+    k_gpos_sorted<<<std::min<int64_t>(nblk(ng), 65536), kBlock, 0, s>>>(ds.gpos.as<int32_t>(), ng, inv.as<int32_t>());
+    HIPOK(hipGetLastError());
+    if (int rc2 = sync_stream()) return rc2;
   }
   HIPOK(ds.pts.ensure(sizeof(float4) * n));
   k_permute_points<<<nblk(n), kBlock, 0, s>>>(ds.raw[0].as<float>(), ds.raw[1].as<float>(), ds.raw[2].as<float>(),
@@ -4196,6 +4241,17 @@ int fomo_dataset_rect_info(int id, int* axes3) {
   if (!ds) return fail(FOMO_ERR_NOTFOUND, "unknown dataset id");
   if (!axes3) return fail(FOMO_ERR_ARG, "null output");
   for (int a = 0; a < 3; ++a) axes3[a] = ds->rect ? ds->rect_n[a] : 0;
+  return FOMO_OK;
+}
+
+int fomo_dataset_grid_info(int id, int* kind, int* axes3) {
+  std::lock_guard<std::recursive_mutex> lk(g_mu);
+  if (int rc = require_init()) return rc;
+  Dataset* ds = find_obj(g_ctx.datasets, id);
+  if (!ds) return fail(FOMO_ERR_NOTFOUND, "unknown dataset id");
+  if (!kind || !axes3) return fail(FOMO_ERR_ARG, "null output");
+  *kind = ds->rect ? 1 : (ds->mrect ? 2 : 0);
+  for (int a = 0; a < 3; ++a) axes3[a] = (ds->rect || ds->mrect) ? ds->rect_n[a] : 0;
   return FOMO_OK;
 }
 

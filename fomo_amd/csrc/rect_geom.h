@@ -173,4 +173,58 @@ FOMO_HD inline int rect_pick(const F4& e0, const F4& e1, const F4& e2, int n0, i
   return -2;
 }
 
+// Masked grids: the points are a subset of the product grid, each on its own node; gp(g) gives the
+// point on grid node g (its sorted position) or -1, orig(sp) the original index of a point. The
+// nearest present point is the nearest node when that node holds a point: every node, present or
+// not, is at least D away, so a present node at distance D is nearest, ties going to the lowest
+// original index among the present nodes at D. Returns its sorted position when it lies strictly
+// inside the search box, -2 when it lies outside (the caller runs the exact query), and -3 when no
+// node at distance D holds a point (or D > dmax, or the target is not finite): then the answer is
+// farther away and the caller takes another path.
+template <class F4, class Ld, class Gp, class Orig>
+FOMO_HD inline int mrect_pick(const F4& e0, const F4& e1, const F4& e2, int n0, int n1, int n2, double dmax,
+                              const SearchBox& box, const Ld& ld, const Gp& gp, const Orig& orig) {
+  if (!(std::isfinite(box.t0) && std::isfinite(box.t1) && std::isfinite(box.t2))) return -3;
+  int kb0, kb1, kb2, ko0, ko1, ko2;
+  float vb0, vb1, vb2, vo0, vo1, vo2;
+  double eb0, eb1, eb2, eo0, eo1, eo2;
+  rect_axis(e0, n0, box.t0, [&](int k) { return ld(0, k); }, kb0, ko0, vb0, vo0, eb0, eo0);
+  rect_axis(e1, n1, box.t1, [&](int k) { return ld(1, k); }, kb1, ko1, vb1, vo1, eb1, eo1);
+  rect_axis(e2, n2, box.t2, [&](int k) { return ld(2, k); }, kb2, ko2, vb2, vo2, eb2, eo2);
+  const double D = (eb0 + eb1) + eb2;
+  if (!(D <= dmax)) return -3;
+  const bool two0 = (eo0 + eb1) + eb2 <= D;
+  const bool two1 = (eb0 + eo1) + eb2 <= D;
+  const bool two2 = (eb0 + eb1) + eo2 <= D;
+  int best = gp(kb0 + (int64_t)n0 * (kb1 + (int64_t)n1 * kb2));
+  float q0 = vb0, q1 = vb1, q2 = vb2;
+  if (two0 || two1 || two2) {
+    int bo = -1;  // original index of best (-1: not read yet)
+    for (int m = 1; m < 8; ++m) {
+      const bool u = (m & 1) != 0, v = (m & 2) != 0, w = (m & 4) != 0;
+      if ((u && !two0) || (v && !two1) || (w && !two2)) continue;
+      const double d = ((u ? eo0 : eb0) + (v ? eo1 : eb1)) + (w ? eo2 : eb2);
+      if (!(d <= D)) continue;
+      const int sp = gp((u ? ko0 : kb0) + (int64_t)n0 * ((v ? ko1 : kb1) + (int64_t)n1 * (w ? ko2 : kb2)));
+      if (sp < 0) continue;
+      bool take = best < 0;
+      if (!take) {
+        if (bo < 0) bo = orig(best);
+        const int o = orig(sp);
+        take = o < bo;
+        if (take) bo = o;
+      }
+      if (take) {
+        best = sp;
+        q0 = u ? vo0 : vb0;
+        q1 = v ? vo1 : vb1;
+        q2 = w ? vo2 : vb2;
+      }
+    }
+  }
+  if (best < 0) return -3;
+  if (box.lo0 < q0 && q0 < box.hi0 && box.lo1 < q1 && q1 < box.hi1 && box.lo2 < q2 && q2 < box.hi2) return best;
+  return -2;
+}
+
 }  // namespace fomo

@@ -103,6 +103,13 @@ int fomo_dataset_lookup_info(int id, int* subdiv, int64_t* cells, int64_t* candi
 int fomo_dataset_set_rectilinear(int id, int on);
 /* Values per axis of a rectilinear dataset's grid (axes3), all 0 for a scattered one. */
 int fomo_dataset_rect_info(int id, int* axes3);
+/* Index kind: 0 scattered points (cell list and candidate lookup grid), 1 rectilinear (a full
+ * product grid, indexed by its axes), 2 masked grid (the points fill at least a quarter of a
+ * product grid's nodes, one point per node, as a Cartesian mesh cut to a region: the candidate
+ * lookup grid plus the grid's axes and a node -> point map, which the NN queries of the
+ * unfused passes try first; a target whose nearest node holds no point takes the lookup grid).
+ * axes3: values per axis of the grid (0 for scattered points). */
+int fomo_dataset_grid_info(int id, int* kind, int* axes3);
 
 /* Far and outer groups of the lookup grid (targets more than 2 cells from any point, and targets
  * beyond the grid, whose cells read a group's shared candidate list). counts5: far groups, far

@@ -4,6 +4,9 @@
 // axis values, on exact midpoints (distance ties between grid points, decided by the original
 // index), and far outside the grid. Search boxes are random.
 //
+// Masked grids (random subsets of a product grid's nodes, and spherical shells) check mrect_pick the
+// same way.
+//
 // The query must return the brute-force answer: the nearest point by (d0^2 + d1^2) + d2^2 in double
 // from float coordinates, then the lowest original index, when it lies strictly inside the box,
 // and -2 otherwise. It may also return -2 (the caller then runs the exact query) when an axis's
@@ -176,6 +179,114 @@ int main() {
   }
   std::printf("queries %ld, nearest inside the box %ld, distance ties %ld, far targets %ld, fallbacks %ld, "
               "failures %ld\n", queries, inside, ties, far_q, fallbacks, fails);
+  // masked grids: a random subset of the nodes (a quarter to all of them, or a spherical shell)
+  // holds points; mrect_pick must give the nearest present point's sorted position (inside the
+  // box), -2 (outside it), or -3 only when no node at the grid's nearest distance holds a point
+  long mq = 0, mhit = 0, mpass = 0, mfails = 0;
+  for (int set = 0; set < 2000; ++set) {
+    std::vector<float> ax[3];
+    int n[3];
+    for (int a = 0; a < 3; ++a) {
+      ax[a] = make_axis(rng, 2 + (int)(rng() % 11), (int)(rng() % 5));
+      n[a] = (int)ax[a].size();
+    }
+    const int64_t np = (int64_t)n[0] * n[1] * n[2];
+    std::vector<int> gp(np, -1), orig_of_sp;
+    const int shell = set % 4 == 0;
+    const double fill = 0.25 + 0.75 * std::uniform_real_distribution<double>(0, 1)(rng);
+    double c[3];
+    for (int a = 0; a < 3; ++a) c[a] = 0.5 * ((double)ax[a].front() + ax[a].back());
+    for (int64_t g = 0; g < np; ++g) {
+      const int u = (int)(g % n[0]), v = (int)((g / n[0]) % n[1]), w = (int)(g / ((int64_t)n[0] * n[1]));
+      bool keep;
+      if (shell) {
+        const double r2 = std::pow(ax[0][u] - c[0], 2) + std::pow(ax[1][v] - c[1], 2) + std::pow(ax[2][w] - c[2], 2);
+        keep = r2 > 0.5 && r2 < 9.0;
+      } else {
+        keep = std::uniform_real_distribution<double>(0, 1)(rng) < fill;
+      }
+      if (keep) {
+        gp[g] = (int)orig_of_sp.size();
+        orig_of_sp.push_back(0);
+      }
+    }
+    if (orig_of_sp.empty()) continue;
+    std::vector<int> oidx(orig_of_sp.size());
+    std::iota(oidx.begin(), oidx.end(), 0);
+    std::shuffle(oidx.begin(), oidx.end(), rng);
+    orig_of_sp = oidx;
+    std::vector<F4> bins[3];
+    float o[3], inv[3];
+    double gap2[3];
+    int nb[3];
+    for (int a = 0; a < 3; ++a) {
+      nb[a] = fomo::kRectBinsPerValue * n[a];
+      bins[a].resize(nb[a]);
+      fomo::rect_axis_table(ax[a].data(), n[a], bins[a].data(), &o[a], &inv[a], &gap2[a]);
+    }
+    const auto ld = [&](int a, int k) { return ax[a][k]; };
+    const auto gpf = [&](int64_t g) {
+      if (g < 0 || g >= np) {
+        std::printf("FAIL: grid node out of range\n");
+        std::exit(1);
+      }
+      return gp[g];
+    };
+    const auto og = [&](int sp) { return orig_of_sp[sp]; };
+    for (int qn = 0; qn < 100; ++qn) {
+      float t[3];
+      for (int a = 0; a < 3; ++a) {
+        const int k = (int)(rng() % n[a]);
+        const double span = (double)ax[a].back() - ax[a].front() + 1;
+        std::uniform_real_distribution<double> u(ax[a].front() - 0.3 * span, ax[a].back() + 0.3 * span);
+        switch (qn % 3) {
+          case 0: t[a] = (float)u(rng); break;
+          case 1: t[a] = k + 1 < n[a] ? (float)(0.5 * ((double)ax[a][k] + ax[a][k + 1])) : ax[a][k]; break;
+          default: t[a] = ax[a][k]; break;
+        }
+      }
+      const double md = std::uniform_real_distribution<double>(0.05, 4.0)(rng);
+      fomo::SearchBox box;
+      box.t0 = t[0], box.t1 = t[1], box.t2 = t[2];
+      box.lo0 = (float)(t[0] - md), box.lo1 = (float)(t[1] - md), box.lo2 = (float)(t[2] - md);
+      box.hi0 = (float)(t[0] + md), box.hi1 = (float)(t[1] + md), box.hi2 = (float)(t[2] + md);
+      double dall = INFINITY, best = INFINITY;
+      int bsp = -1;
+      int64_t bg = -1;
+      for (int64_t g = 0; g < np; ++g) {
+        const int u = (int)(g % n[0]), v = (int)((g / n[0]) % n[1]), w = (int)(g / ((int64_t)n[0] * n[1]));
+        const double d0 = (double)ax[0][u] - (double)t[0], d1 = (double)ax[1][v] - (double)t[1],
+                     d2 = (double)ax[2][w] - (double)t[2];
+        const double d = (d0 * d0 + d1 * d1) + d2 * d2;
+        dall = std::fmin(dall, d);
+        if (gp[g] < 0) continue;
+        if (d < best || (d == best && orig_of_sp[gp[g]] < orig_of_sp[bsp])) best = d, bsp = gp[g], bg = g;
+      }
+      const int u = (int)(bg % n[0]), v = (int)((bg / n[0]) % n[1]), w = (int)(bg / ((int64_t)n[0] * n[1]));
+      const bool in = box.lo0 < ax[0][u] && ax[0][u] < box.hi0 && box.lo1 < ax[1][v] && ax[1][v] < box.hi1 &&
+                      box.lo2 < ax[2][w] && ax[2][w] < box.hi2;
+      F4 e[3];
+      for (int a = 0; a < 3; ++a) e[a] = bins[a][fomo::rect_bin(o[a], inv[a], nb[a], t[a])];
+      const int r = fomo::mrect_pick(e[0], e[1], e[2], n[0], n[1], n[2], fomo::rect_dmax(gap2), box, ld, gpf, og);
+      ++mq;
+      if (r == -3) {
+        ++mpass;
+        if (best == dall && dall <= fomo::rect_dmax(gap2) && ++mfails <= 10)
+          std::printf("FAIL masked set %d query %d: -3 with a point on a nearest node\n", set, qn);
+        continue;
+      }
+      ++mhit;
+      const int expect = in ? bsp : -2;
+      if (r != expect && ++mfails <= 10)
+        std::printf("FAIL masked set %d query %d: got %d expected %d\n", set, qn, r, expect);
+    }
+  }
+  std::printf("masked: queries %ld, answered %ld, passed on %ld, failures %ld\n", mq, mhit, mpass, mfails);
+  fails += mfails;
+  if (mhit * 3 < mq) {
+    std::printf("FAIL: masked grids answered too few queries\n");
+    return 1;
+  }
   if (fails) return 1;
   if (ties < 1000) {
     std::printf("FAIL: too few tie cases exercised\n");

@@ -179,3 +179,67 @@ def test_rect_batch_and_renders_match_scattered(engine, fe9_table):
     for e in ems:
         e.close()
     T.close(), ds.close()
+
+
+def _masked(F, keep, xs, ys, zs, seed):
+    d = _product_grid(xs, ys, zs, seed)
+    X, Y, Z = d["x"].astype(np.float64), d["y"].astype(np.float64), d["z"].astype(np.float64)
+    m = keep(X, Y, Z)
+    return {k: v[m] for k, v in d.items()}
+
+
+@pytest.mark.parametrize("kind", ["shell", "random"])
+def test_masked_grid_matches_scattered_exact_and_oracle(engine, kind):
+    """A product grid with holes (a spherical shell cut from a Cartesian mesh, as the corona; or a
+    random half of the nodes) is indexed as a masked grid: the unfused NN pass tries the grid's
+    nearest node first and takes the candidate grid when it holds no point. NN choices equal the
+    same data indexed as scattered points only, the exact query alone and the oracle, in NN and
+    Thomson views."""
+    F = engine
+    v = np.linspace(-3.0, 3.0, 31)
+    rng = np.random.default_rng(17)
+    if kind == "shell":
+        keep = lambda X, Y, Z: ((X * X + Y * Y + Z * Z) > 1.0) & ((X * X + Y * Y + Z * Z) <= 9.0)
+    else:
+        mask = rng.random(31 ** 3) < 0.5
+        keep = lambda X, Y, Z: mask[np.argsort(np.argsort(X + 1e3 * Y + 1e6 * Z))]
+    d = _masked(F, keep, v, v, v, seed=4)
+    ds = _dataset(F, d)
+    assert ds.grid_info() == ("masked", (31, 31, 31))
+    assert ds.rect_axes() is None
+    nx, ny, nz = 40, 36, 42
+    views = VIEWS[:4]
+    got = [F.nn_indices(ds, l, b, nx, ny, nz)[0] for (l, b) in views]
+    got_t = [F.nn_indices(ds, l, b, nx, ny, nz, thomson=True)[0] for (l, b) in views[:2]]
+    ds.set_rectilinear(False)
+    assert ds.grid_info()[0] == "scattered"
+    scat = [F.nn_indices(ds, l, b, nx, ny, nz)[0] for (l, b) in views]
+    ds.set_lookup(0)
+    exact = [F.nn_indices(ds, l, b, nx, ny, nz)[0] for (l, b) in views]
+    exact_t = [F.nn_indices(ds, l, b, nx, ny, nz, thomson=True)[0] for (l, b) in views[:2]]
+    for k, (l, b) in enumerate(views):
+        assert np.array_equal(got[k], scat[k]), f"view {(l, b)}: {(got[k] != scat[k]).sum()} differ from scattered"
+        assert np.array_equal(got[k], exact[k]), f"view {(l, b)}: {(got[k] != exact[k]).sum()} differ from exact"
+        ridx = _oracle_idx(d, l, b, nx, ny, nz)
+        assert np.array_equal(got[k], ridx), f"view {(l, b)}: {(got[k] != ridx).sum()} differ from the oracle"
+    for k in range(2):
+        assert np.array_equal(got_t[k], exact_t[k])
+    ds.close()
+
+
+def test_corona_is_masked_grid(engine):
+    """The corona generator's points are a Cartesian mesh cut to 1 < r <= 3: a masked grid. Its
+    Thomson renders equal those with the dataset indexed as scattered points, bit for bit."""
+    F = engine
+    c = F.gen_corona(48, seed=5)
+    ds = F.Dataset(c["x"], c["y"], c["z"], [c["rho"]])
+    kind, axes = ds.grid_info()
+    # the mesh's outermost planes hold no point of the shell when no node lies on an axis
+    assert kind == "masked" and all(44 <= a <= 48 for a in axes), (kind, axes)
+    em = F.Emission.thomson(ds)
+    views = [(0.0, np.pi / 2), (0.9, 0.3), (np.pi / 4, np.pi / 2)]
+    imgs = [F.render_thomson(em, l, b, 64, 60, 48)[0] for (l, b) in views]
+    ds.set_rectilinear(False)
+    for (l, b), img in zip(views, imgs):
+        assert np.array_equal(F.render_thomson(em, l, b, 64, 60, 48)[0], img)
+    em.close(), ds.close()
