@@ -11,7 +11,7 @@ import numpy as np
 
 __all__ = [
     "lib", "FomoError", "init", "shutdown", "device_info", "Dataset", "Table", "Emission", "Batch",
-    "render_nn", "render_nn_noalloc", "render_thomson", "nn_indices", "Server", "METHOD_NN", "METHOD_THOMSON", "METHOD_TET", "METHOD_LINEAR", "METHOD_PROJECTION", "delaunay", "render_tet",
+    "render_nn", "render_nn_noalloc", "render_thomson", "nn_indices", "Server", "METHOD_NN", "METHOD_THOMSON", "METHOD_TET", "METHOD_LINEAR", "METHOD_PROJECTION", "delaunay", "set_delaunay_threads", "render_tet",
     "autotune", "tet_vertices", "render_projection", "projection_pixels", "gen_loop_cube", "gen_goft_table", "gen_corona",
     "read_table", "profile_enable", "profile_reset", "profile_stats", "LIB_PATH",
 ]
@@ -84,6 +84,8 @@ _SIGS = {
     "fomo_server_cancel": (ctypes.c_int, [ctypes.c_int, ctypes.c_int64]),
     "fomo_server_times": (ctypes.c_int, [ctypes.c_int, ctypes.c_int64, _c_dbl_p]),
     "fomo_server_stats": (ctypes.c_int, [ctypes.c_int, _c_dbl_p]),
+    "fomo_set_delaunay_threads": (ctypes.c_int, [ctypes.c_int]),
+    "fomo_delaunay_threads": (ctypes.c_int, []),
     "fomo_delaunay": (ctypes.c_int, [ctypes.c_int64, _c_flt_p, _c_flt_p, _c_flt_p, ctypes.c_int64, _c_i64_p,
                                      _c_i32_p]),
     "fomo_render_tet": (ctypes.c_int, [ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_int,
@@ -628,8 +630,15 @@ class Server:
             self._bufs.clear()
 
 
+def set_delaunay_threads(threads=0):
+    """Worker threads of the host Delaunay builder (0: FOMO_DELAUNAY_THREADS, else OMP_NUM_THREADS,
+    else the hardware threads). The tets and their order do not depend on it."""
+    _check(lib().fomo_set_delaunay_threads(int(threads)))
+
+
 def delaunay(x, y, z):
-    """Delaunay tets of the points (host builder): int32 array (ntets, 4) of point indices."""
+    """Delaunay tets of the points (host builder): int32 array (ntets, 4) of point indices, in a
+    canonical order (by vertex, in the Morton order of the points)."""
     (xa, xp), (ya, yp), (za, zp) = _f32(x), _f32(y), _f32(z)
     n = xa.size
     cnt = ctypes.c_int64()

@@ -2391,10 +2391,10 @@ int build_tets(Dataset& ds) {
   HIPOK(hipMemcpyAsync(perm.data(), ds.perm.p, sizeof(int32_t) * n, hipMemcpyDeviceToHost, s));
   if (int rc = sync_stream()) return rc;
   fomo::Delaunay dt;
+  dt.threads = fomo_delaunay_threads();
   if (!dt.build(hx.data(), hy.data(), hz.data(), n))
     return fail(FOMO_ERR_ARG, "Delaunay triangulation failed: " + dt.error);
-  int64_t nfin = 0;
-  std::vector<fomo::Tet> T = dt.compact(&nfin);
+  const fomo::TetVec& T = dt.T;
   std::vector<int32_t> sorted_of(n);
   for (int64_t p = 0; p < n; ++p) sorted_of[perm[p]] = (int32_t)p;
   // start grid: cells ~ 4 points each along an axis, at most 64 per axis

@@ -1,6 +1,7 @@
 // fomo_host.cpp — host-side parts of the engine's C ABI: the CHIANTI G(T,n) table reader and
 // the deterministic synthetic-input generators used by the tests and the benchmark.
 
+#include <atomic>
 #include <cmath>
 #include <complex>
 #include <cstdint>
@@ -260,15 +261,27 @@ int fomo_gen_corona(int m, uint64_t seed, double sigma, int64_t* count, float* x
   return FOMO_OK;
 }
 
+// Worker threads of the host Delaunay builder (0: from the environment or the hardware).
+static std::atomic<int> g_delaunay_threads{0};
+
+int fomo_set_delaunay_threads(int threads) {
+  if (threads < 0) return FOMO_ERR_ARG;
+  g_delaunay_threads = threads;
+  return FOMO_OK;
+}
+
+int fomo_delaunay_threads(void) { return g_delaunay_threads.load(); }
+
 // Delaunay triangulation of a point set on the host (the structure the tetra render walks).
 int fomo_delaunay(int64_t n, const float* x, const float* y, const float* z, int64_t cap, int64_t* ntets,
                   int32_t* tets) {
   if (n < 4 || !x || !y || !z || !ntets) return FOMO_ERR_ARG;
   fomo::Delaunay dt;
+  dt.threads = fomo_delaunay_threads();
   if (!dt.build(x, y, z, n)) return FOMO_ERR_ARG;
   int64_t m = 0;
   for (const auto& t : dt.T) {
-    if (t.v[0] == fomo::Delaunay::kDead || !dt.finite(t)) continue;
+    if (!dt.finite(t)) continue;
     if (tets && m < cap)
       for (int k = 0; k < 4; ++k) tets[4 * m + k] = t.v[k];
     ++m;

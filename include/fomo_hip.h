@@ -352,6 +352,13 @@ int fomo_gen_corona(int m, uint64_t seed, double sigma, int64_t* count, float* x
 int fomo_delaunay(int64_t n, const float* x, const float* y, const float* z, int64_t cap,
                   int64_t* ntets, int32_t* tets);
 
+/* Worker threads of the host Delaunay builder (fomo_delaunay and the tetra path's triangulation;
+ * the reference builds its CGAL triangulation serially, fomo-CGAL.cpp:52-57): 0 (the default)
+ * takes FOMO_DELAUNAY_THREADS, else OMP_NUM_THREADS, else the hardware threads, at most 64. The
+ * tets and their order are the same for every thread count (a canonical order by vertex). */
+int fomo_set_delaunay_threads(int threads);
+int fomo_delaunay_threads(void);
+
 #ifdef __cplusplus
 }
 #endif

@@ -174,3 +174,69 @@ def test_duplicates_keep_the_last_index():
     remap = np.arange(len(p))
     remap[:50] = np.arange(450, 500)
     assert set(map(tuple, np.sort(remap[ref], 1))) == set(map(tuple, np.sort(tets, 1)))
+
+
+def _shell(n, seed):
+    r = np.random.default_rng(seed)
+    d = r.normal(size=(n, 3))
+    return (d / np.linalg.norm(d, axis=1)[:, None] * (1 + 0.1 * r.random((n, 1)))).astype(np.float32)
+
+
+def _clusters(n, seed):
+    r = np.random.default_rng(seed)
+    c = r.random((12, 3))
+    return (c[r.integers(0, 12, n)] + 0.02 * r.normal(size=(n, 3))).astype(np.float32)
+
+
+@pytest.mark.parametrize("kind", ["uniform", "jittered", "exact_grid", "shell", "clusters", "duplicates"])
+def test_parallel_build_equals_serial(kind):
+    """The parallel build (levels of a randomized insertion order, each inserted by worker threads
+    that own Morton chunks of the points and defer what crosses a chunk border) against the serial
+    one: the same tets in the same order (the canonical order), for several thread counts. Sizes
+    reach the parallel levels (over 8192 distinct points), so deferral, the shifted passes and the
+    serial remainder all run; the exact grid is all cospherical (the symbolic perturbation decides
+    every cube)."""
+    r = np.random.default_rng(21)
+    if kind == "uniform":
+        p = r.random((30000, 3)).astype(np.float32)
+    elif kind == "jittered":
+        g = np.stack(np.meshgrid(*[np.arange(32.0)] * 3, indexing="ij"), -1).reshape(-1, 3)
+        p = (g + r.uniform(-0.3, 0.3, g.shape)).astype(np.float32)
+    elif kind == "exact_grid":
+        p = np.stack(np.meshgrid(np.arange(26.0), np.arange(24.0), np.arange(22.0), indexing="ij"),
+                     -1).reshape(-1, 3).astype(np.float32)
+    elif kind == "shell":
+        p = _shell(25000, 22)
+    elif kind == "clusters":
+        p = _clusters(25000, 23)
+    else:
+        q = r.random((20000, 3)).astype(np.float32)
+        p = np.concatenate([q, q[::3], q[:1000]])[r.permutation(20000 + 6667 + 1000)]
+    try:
+        F.set_delaunay_threads(1)
+        ref = F.delaunay(p[:, 0], p[:, 1], p[:, 2])
+        for nt in (3, 8):
+            F.set_delaunay_threads(nt)
+            got = F.delaunay(p[:, 0], p[:, 1], p[:, 2])
+            assert got.shape == ref.shape and np.array_equal(got, ref), (kind, nt)
+    finally:
+        F.set_delaunay_threads(0)
+    if kind in ("uniform", "shell"):
+        qh = Delaunay(p.astype(np.float64)).simplices
+        assert set(map(tuple, np.sort(ref, 1))) == set(map(tuple, np.sort(qh, 1)))
+
+
+def test_parallel_build_is_valid_on_a_jittered_cube():
+    """A larger parallel build checked on its own: positively oriented tets tiling the hull with
+    empty circumspheres, equal as a set to Qhull's."""
+    g = np.stack(np.meshgrid(*[np.arange(40.0)] * 3, indexing="ij"), -1).reshape(-1, 3)
+    p = (g + np.random.default_rng(24).uniform(-0.3, 0.3, g.shape)).astype(np.float32)
+    try:
+        F.set_delaunay_threads(6)
+        tets = F.delaunay(p[:, 0], p[:, 1], p[:, 2])
+    finally:
+        F.set_delaunay_threads(0)
+    ref = Delaunay(p.astype(np.float64)).simplices
+    assert set(map(tuple, np.sort(tets, 1))) == set(map(tuple, np.sort(ref, 1)))
+    vmin, vfrac, inside = _valid(p, tets)
+    assert vmin > 0 and abs(vfrac - 1) < 1e-12 and inside == 0
