@@ -692,7 +692,8 @@ def tet(F, args):
     t0 = time.perf_counter()
     F.render_nn(em, l, b, nx, ny, nz, nl, width)
     dtn = time.perf_counter() - t0
-    # roofline of the walk: per sample the containing tet's record (80 B), the 4 vertex records it
+    # roofline of the walk (the profile scope holds the walk and the exact re-walk of the samples it
+    # cannot place): per sample the containing tet's record (80 B), the 4 vertex records it
     # interpolates (4 x 32 B) and the per-sample outputs (12 B)
     cnt, tot = st["k_tet_walk_linear"]
     roof = None

@@ -1438,7 +1438,7 @@ __global__ void k_far_headers(int32_t* __restrict__ cnt, int64_t ncl, LookupInfo
 // sample on a shared face stays in the tet the walk reached first.
 // ------------------------------------------------------------------------------------
 constexpr int kTetMaxSteps = 1 << 16;
-constexpr int kTetRedoBlocks = 64;  // k_tet_redo grid (grid-stride over the redo list)
+constexpr int kTetRedoBlocks = 512;  // k_tet_redo grid (grid-stride over the redo list)
 constexpr int64_t kTetThreads = 1 << 18;  // segments per launch worth splitting rays for
 constexpr int kTetMaxSegment = 16;          // longest run of LOS samples one thread walks
 
@@ -1582,11 +1582,17 @@ __global__ void __launch_bounds__(256) k_tet_walk(const float4* __restrict__ tre
   const int64_t R = (int64_t)nx * ny;
   const int64_t tid = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   const int nseg = (nz + seglen - 1) / seglen;
-  if (tid >= R * nseg) return;
-  // neighbouring threads take neighbouring rays (coalesced sample writes)
-  const int64_t ray = tid % R;
-  const int seg = (int)(tid / R);
-  const int j = (int)(ray % nx), i = (int)(ray / nx);
+  // a wave takes an 8 x 8 pixel tile of one segment: its walks stay in a compact region of the
+  // mesh (shared cache lines), where a row of 64 pixels would spread them over 32 point spacings
+  const int tnx = (nx + 7) >> 3, tny = (ny + 7) >> 3;
+  const int64_t ntile = (int64_t)tnx * tny, w = tid >> 6;
+  const int lane = (int)(tid & 63);
+  const int64_t tl = w % ntile;
+  const int seg = (int)(w / ntile);
+  const int j = (int)(tl % tnx) * 8 + (lane & 7);
+  const int i = (int)(tl / tnx) * 8 + (lane >> 3);
+  if (seg >= nseg || j >= nx || i >= ny) return;
+  const int64_t ray = (int64_t)i * nx + j;
   const ViewGeom vg = *vgp;
   const double minx = vg.ext[0], maxx = vg.ext[1], miny = vg.ext[2], maxy = vg.ext[3], minz = vg.ext[4],
                maxz = vg.ext[5];
@@ -1595,6 +1601,9 @@ __global__ void __launch_bounds__(256) k_tet_walk(const float4* __restrict__ tre
   const double cb = vg.cb, sb = vg.sb, cl = vg.cl, sl = vg.sl;
   const size_t S = (size_t)R * nz;
   int t = -1;  // tet of the previous sample (-1: none)
+  int tq = -1;  // the tet whose record q / nb hold (the previous sample's walk ended there)
+  float4 q[4];
+  int4 nb;
   const int k0 = seg * seglen, k1 = min(nz, k0 + seglen);
   for (int k = k0; k < k1; ++k) {
     // :189 (a single z pixel has no spacing: the sample sits at minz)
@@ -1605,17 +1614,18 @@ __global__ void __launch_bounds__(256) k_tet_walk(const float4* __restrict__ tre
     const size_t sidx = (size_t)k * R + ray;
     bool outside = px < tg.lo[0] || px > tg.hi[0] || py < tg.lo[1] || py > tg.hi[1] || pz < tg.lo[2] ||
                    pz > tg.hi[2];
-    float4 q[4];
-    int4 nb;
     bool flag = false;  // an undecided test at the end of the walk, or the step cap
     if (!outside) {
       const bool warm = t >= 0;
       if (!warm) t = tet_start(tg, tstart, px, py, pz);
       int steps = 0;
       for (;;) {
-        const float4* tr = trec + (size_t)t * 4;
-        q[0] = tr[0], q[1] = tr[1], q[2] = tr[2], q[3] = tr[3];
-        nb = tn[t];
+        if (t != tq) {
+          const float4* tr = trec + (size_t)t * 4;
+          q[0] = tr[0], q[1] = tr[1], q[2] = tr[2], q[3] = tr[3];
+          nb = tn[t];
+          tq = t;
+        }
         int inf = -1;
 #pragma unroll
         for (int m = 0; m < 4; ++m)
@@ -1689,6 +1699,9 @@ __global__ void __launch_bounds__(256) k_tet_walk(const float4* __restrict__ tre
 // reach); redo[1] holds this view's unresolved count afterwards.
 __device__ __forceinline__ int orient_x(const float4& a, const float4& b, const float4& c, double px, double py,
                                         double pz) {
+  // the filtered test's sign is exact where it decides; the expansion only where it cannot
+  const int f = orient_d(a, b, c, px, py, pz);
+  if (f != 0) return f;
   const double A[3] = {a.x, a.y, a.z}, B[3] = {b.x, b.y, b.z}, Cc[3] = {c.x, c.y, c.z}, P[3] = {px, py, pz};
   return fomo::orient_exact_fp(A, B, Cc, P);
 }
@@ -2444,12 +2457,15 @@ int build_tets(Dataset& ds) {
     }
   }
   std::vector<int4> hv(T.size()), hn(T.size());
-  for (size_t q = 0; q < T.size(); ++q) {
-    int32_t vv[4];
-    for (int m = 0; m < 4; ++m) vv[m] = T[q].v[m] < 0 ? -1 : sorted_of[T[q].v[m]];
-    hv[q] = make_int4(vv[0], vv[1], vv[2], vv[3]);
-    hn[q] = make_int4(T[q].n[0], T[q].n[1], T[q].n[2], T[q].n[3]);
-  }
+  fomo::run_threads(std::min(16, dt.nthreads_used), [&](int th) {
+    const size_t nt = std::min(16, dt.nthreads_used);
+    for (size_t q = T.size() * th / nt; q < T.size() * (th + 1) / nt; ++q) {
+      int32_t vv[4];
+      for (int m = 0; m < 4; ++m) vv[m] = T[q].v[m] < 0 ? -1 : sorted_of[T[q].v[m]];
+      hv[q] = make_int4(vv[0], vv[1], vv[2], vv[3]);
+      hn[q] = make_int4(T[q].n[0], T[q].n[1], T[q].n[2], T[q].n[3]);
+    }
+  });
   DevBuf tv;
   if (tv.alloc(sizeof(int4) * hv.size()) || ds.trec.alloc(4 * sizeof(float4) * hv.size()) ||
       ds.tn.alloc(sizeof(int4) * hn.size()) || ds.tstart.alloc(sizeof(int32_t) * ng) ||
@@ -3182,7 +3198,8 @@ int enqueue_tet(hipStream_t s, const Dataset& ds, const ViewGeom* geom, int nx, 
   // split rays into segments only as far as needed to keep kTetThreads walks in flight
   const int64_t want = (R * nz + kTetThreads - 1) / kTetThreads;
   const int seglen = (int)std::max<int64_t>(1, std::min<int64_t>({want, (int64_t)kTetMaxSegment, (int64_t)nz}));
-  const int64_t nthreads = R * ((nz + seglen - 1) / seglen);
+  const int64_t nsegs = (nz + seglen - 1) / seglen;
+  const int64_t nthreads = (int64_t)((nx + 7) / 8) * ((ny + 7) / 8) * 64 * nsegs;  // 8 x 8 pixel tiles
   const unsigned nb = (unsigned)((nthreads + 255) / 256);
   ProfScope ps(linear ? "k_tet_walk_linear" : "k_tet_walk", s);
 #define FOMO_TET_ARGS                                                                                        \
