@@ -160,25 +160,36 @@ class Delaunay {
     par_sort(ord, nt, [](const std::pair<uint64_t, int32_t>& a, const std::pair<uint64_t, int32_t>& b) {
       return a < b;
     });
-    // duplicates: equal coordinates; the vertex is the last input index of the group
+    // duplicates: equal coordinates give equal codes, so they are neighbours within a run of equal
+    // codes of the sorted order; the vertex of a group is its last input index
     std::vector<int32_t> seq;
     seq.reserve(n);
     {
-      std::vector<int32_t> byc(n);
-      for (int64_t i = 0; i < n; ++i) byc[i] = (int32_t)i;
-      const auto& Pc = P;
-      par_sort(byc, nt, [&Pc](int32_t a, int32_t b) {
-        if (Pc[a].x != Pc[b].x) return Pc[a].x < Pc[b].x;
-        if (Pc[a].y != Pc[b].y) return Pc[a].y < Pc[b].y;
-        if (Pc[a].z != Pc[b].z) return Pc[a].z < Pc[b].z;
-        return a < b;
-      });
+      const auto same = [&](int32_t a, int32_t b) { return P[a].x == P[b].x && P[a].y == P[b].y && P[a].z == P[b].z; };
+      std::vector<int32_t> run;
       for (int64_t s = 0; s < n;) {
         int64_t e = s + 1;
-        while (e < n && P[byc[e]].x == P[byc[s]].x && P[byc[e]].y == P[byc[s]].y && P[byc[e]].z == P[byc[s]].z) ++e;
-        const int32_t last = byc[e - 1];
-        for (int64_t k = s; k < e; ++k) rep[byc[k]] = last;
-        nduplicates += e - s - 1;
+        while (e < n && ord[e].first == ord[s].first) ++e;
+        if (e - s == 1) {
+          rep[ord[s].second] = ord[s].second;
+        } else {
+          // a run of equal codes (a cell of 2^-21 of the box): group its points by coordinates
+          run.clear();
+          for (int64_t k = s; k < e; ++k) run.push_back(ord[k].second);
+          std::sort(run.begin(), run.end(), [&](int32_t a, int32_t b) {
+            if (P[a].x != P[b].x) return P[a].x < P[b].x;
+            if (P[a].y != P[b].y) return P[a].y < P[b].y;
+            if (P[a].z != P[b].z) return P[a].z < P[b].z;
+            return a < b;
+          });
+          for (size_t g = 0; g < run.size();) {
+            size_t h = g + 1;
+            while (h < run.size() && same(run[h], run[g])) ++h;
+            for (size_t k = g; k < h; ++k) rep[run[k]] = run[h - 1];
+            nduplicates += (int64_t)(h - g - 1);
+            g = h;
+          }
+        }
         s = e;
       }
       for (auto& o : ord)

@@ -2407,6 +2407,7 @@ int build_tets(Dataset& ds) {
   dt.threads = fomo_delaunay_threads();
   if (!dt.build(hx.data(), hy.data(), hz.data(), n))
     return fail(FOMO_ERR_ARG, "Delaunay triangulation failed: " + dt.error);
+  const auto tdt = std::chrono::steady_clock::now();
   const fomo::TetVec& T = dt.T;
   std::vector<int32_t> sorted_of(n);
   for (int64_t p = 0; p < n; ++p) sorted_of[perm[p]] = (int32_t)p;
@@ -2456,7 +2457,8 @@ int build_tets(Dataset& ds) {
       start[c] = t;
     }
   }
-  std::vector<int4> hv(T.size()), hn(T.size());
+  const auto t1 = std::chrono::steady_clock::now();
+  std::vector<int4, fomo::NoInitAlloc<int4>> hv(T.size()), hn(T.size());
   fomo::run_threads(std::min(16, dt.nthreads_used), [&](int th) {
     const size_t nt = std::min(16, dt.nthreads_used);
     for (size_t q = T.size() * th / nt; q < T.size() * (th + 1) / nt; ++q) {
@@ -2485,7 +2487,12 @@ int build_tets(Dataset& ds) {
   if (int rc = sync_stream()) return rc;
   ds.tk = tk;
   ds.ntets = (int64_t)T.size();
-  ds.tet_build_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  const auto t2 = std::chrono::steady_clock::now();
+  ds.tet_build_s = std::chrono::duration<double>(t2 - t0).count();
+  if (const char* pe = std::getenv("FOMO_PROGRESS"); pe && pe[0] == '1')
+    std::fprintf(stderr, "tets: Delaunay %.2f s, start grid %.2f s, conversion and upload %.2f s\n",
+                 std::chrono::duration<double>(tdt - t0).count(), std::chrono::duration<double>(t1 - tdt).count(),
+                 std::chrono::duration<double>(t2 - t1).count());
   return FOMO_OK;
 }
 
