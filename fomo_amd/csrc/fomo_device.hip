@@ -2426,36 +2426,44 @@ int build_tets(Dataset& ds) {
   const int64_t ng = (int64_t)tk.dims[0] * tk.dims[1] * tk.dims[2];
   std::vector<int32_t> start(ng, 0);
   {
-    // walk with exact predicates on the original-index tets
-    int32_t t = 0;
+    // walks with exact predicates on the original-index tets, one z layer of cells per task (the
+    // cells in order, each from the previous one's tet; each layer from the first finite tet of
+    // the canonical order, so the grid does not depend on the thread count)
+    int32_t t0 = 0;
     for (size_t q = 0; q < T.size(); ++q)
       if (dt.finite(T[q])) {
-        t = (int32_t)q;
+        t0 = (int32_t)q;
         break;
       }
-    for (int64_t c = 0; c < ng; ++c) {
-      const int cx = (int)(c % tk.dims[0]), cy = (int)((c / tk.dims[0]) % tk.dims[1]),
-                cz = (int)(c / ((int64_t)tk.dims[0] * tk.dims[1]));
-      const fomo::pred::P3 q{tk.o[0] + (cx + 0.5) * h, tk.o[1] + (cy + 0.5) * h, tk.o[2] + (cz + 0.5) * h};
-      for (int64_t steps = 0; steps < 10000000; ++steps) {
-        const fomo::Tet& u = T[t];
-        int inf = -1;
-        for (int m = 0; m < 4; ++m)
-          if (u.v[m] < 0) inf = m;
-        if (inf >= 0) {  // outside the hull: take the finite tet across the hull face
-          t = u.n[inf];
-          break;
+    const int64_t layer = (int64_t)tk.dims[0] * tk.dims[1];
+    std::atomic<int> next_z{0};
+    fomo::run_threads(std::min(16, std::max(1, dt.nthreads_used)), [&](int) {
+      for (int cz; (cz = next_z.fetch_add(1)) < tk.dims[2];) {
+        int32_t t = t0;
+        for (int64_t c = cz * layer; c < (cz + 1) * layer; ++c) {
+          const int cx = (int)(c % tk.dims[0]), cy = (int)((c / tk.dims[0]) % tk.dims[1]);
+          const fomo::pred::P3 q{tk.o[0] + (cx + 0.5) * h, tk.o[1] + (cy + 0.5) * h, tk.o[2] + (cz + 0.5) * h};
+          for (int64_t steps = 0; steps < 10000000; ++steps) {
+            const fomo::Tet& u = T[t];
+            int inf = -1;
+            for (int m = 0; m < 4; ++m)
+              if (u.v[m] < 0) inf = m;
+            if (inf >= 0) {  // outside the hull: take the finite tet across the hull face
+              t = u.n[inf];
+              break;
+            }
+            int next = -1;
+            for (int f = 0; f < 4 && next < 0; ++f)
+              if (fomo::pred::orient(dt.P[u.v[fomo::kFace[f][0]]], dt.P[u.v[fomo::kFace[f][1]]],
+                                     dt.P[u.v[fomo::kFace[f][2]]], q) < 0)
+                next = u.n[f];
+            if (next < 0) break;
+            t = next;
+          }
+          start[c] = t;
         }
-        int next = -1;
-        for (int f = 0; f < 4 && next < 0; ++f)
-          if (fomo::pred::orient(dt.P[u.v[fomo::kFace[f][0]]], dt.P[u.v[fomo::kFace[f][1]]],
-                                 dt.P[u.v[fomo::kFace[f][2]]], q) < 0)
-            next = u.n[f];
-        if (next < 0) break;
-        t = next;
       }
-      start[c] = t;
-    }
+    });
   }
   const auto t1 = std::chrono::steady_clock::now();
   std::vector<int4, fomo::NoInitAlloc<int4>> hv(T.size()), hn(T.size());
