@@ -579,7 +579,9 @@ void FoMo::FoMoObject::render(const std::vector<double> lvec, const std::vector<
     for (int i = 0; i < dim; ++i) units.push_back(i < (int)dunits.size() ? dunits[i] : "");
     units.push_back(tab.unit);
     units.push_back("\\AA{}");
-    for (int i = 2; i < datacube.readnvars(); ++i) units.push_back(dim + i < (int)dunits.size() ? dunits[dim + i] : "");
+    // the velocity columns take dataunits.at(i) as the reference indexes them (fomo-CHIANTI.cpp:314):
+    // in 3D the z grid unit and the first two variables' units; empty where .at() would throw
+    for (int i = 2; i < datacube.readnvars(); ++i) units.push_back(i < (int)dunits.size() ? dunits[i] : "");
     meta.setchiantifile(cfile);
     meta.setabundfile(afile);
     meta.setlambda0(tab.lambda0);

@@ -1,10 +1,12 @@
 // Drives the FoMo C++ API the way fomo-c/example/example.cpp does: push_back_datapoint,
 // setchiantifile, setrendermethod, setresolution, render, readrendering, writegoftcube.
-// usage: api_render <table> <outprefix> <method> <m> <nx> <ny> <nz> <nl> <l,...> <b,...> [options]
+// usage: api_render <table> <outprefix> <method> <m> <nx> <ny> <nz> <nl> <l,...> <b,...> [options [setdata]]
 // l and b are comma-separated lists (render(lvec, bvec) renders every pair); options is the
-// writeoptions bitset string (default "0001": binary only). After rendering it prints the
-// rendering's shape and the GoftCube of the render as "goft <ng> <nvars> <sum var0> <sum var1>",
-// once from a copy of the object made before the GoftCube was read, once from the object.
+// writeoptions bitset string (default "0001": binary only); "setdata" after it fills the object with
+// setdata (with units) instead of push_back_datapoint. After rendering it prints the
+// rendering's shape and the GoftCube of the render as "goft <ng> <nvars> <sum var0> <sum var1>"
+// and "units|<unit>|<unit>...", once from a copy of the object made before the GoftCube was read,
+// once from the object.
 #include <cstdlib>
 #include <iomanip>
 #include <iostream>
@@ -30,6 +32,9 @@ static void print_goft(FoMo::GoftCube g) {
   }
   std::cout << std::setprecision(17) << "goft " << g.readngrid() << " " << g.readnvars() << " " << s0 << " " << s1
             << std::endl;
+  std::cout << "units";
+  for (const std::string& u : g.readunit()) std::cout << "|" << u;
+  std::cout << std::endl;
 }
 
 int main(int argc, char** argv) {
@@ -49,10 +54,19 @@ int main(int argc, char** argv) {
     return 3;
   FoMo::FoMoObject Object;
   std::vector<std::string> unitvec{"Mm", "Mm", "Mm", "cm^{-3}", "K", "m s^{-1}", "m s^{-1}", "m s^{-1}"};
-  for (size_t i = 0; i < n; ++i) {
-    std::vector<double> c{x[i], y[i], z[i]};
-    std::vector<double> v{rho[i], T[i], vx[i], vy[i], vz[i]};
-    Object.push_back_datapoint(c, v, &unitvec);
+  if (argc > 12 && std::string(argv[12]) == "setdata") {
+    // the whole cube at once, with units (push_back_datapoint ignores them, fomo-object.cpp:139-141)
+    FoMo::tgrid g{FoMo::tcoord(x.begin(), x.end()), FoMo::tcoord(y.begin(), y.end()), FoMo::tcoord(z.begin(), z.end())};
+    FoMo::tvars v{FoMo::tphysvar(rho.begin(), rho.end()), FoMo::tphysvar(T.begin(), T.end()),
+                  FoMo::tphysvar(vx.begin(), vx.end()), FoMo::tphysvar(vy.begin(), vy.end()),
+                  FoMo::tphysvar(vz.begin(), vz.end())};
+    Object.setdata(g, v, &unitvec);
+  } else {
+    for (size_t i = 0; i < n; ++i) {
+      std::vector<double> c{x[i], y[i], z[i]};
+      std::vector<double> v{rho[i], T[i], vx[i], vy[i], vz[i]};
+      Object.push_back_datapoint(c, v, &unitvec);
+    }
   }
   Object.setchiantifile(table);
   Object.setabundfile("/empty");

@@ -87,13 +87,13 @@ def test_facade_multiview_batch_text_gzip_and_goftcube(engine, api_bin, fe9_tabl
     view's files from a background thread: every view's image equals the single-view C ABI render
     of that view bit for bit; the text (the reference's default format), binary and gzip files
     agree; the GoftCube read after the render (also from a copy of the object made before it was
-    read) holds the emission computed on the device."""
+    read) holds the emission computed on the device, with the reference's units."""
     path, tab = fe9_table
     m, nx, ny, nz, nl = 20, 32, 24, 30, 30
     ls, bs = [0.3, 1.1], [1.0, 0.4]
     prefix = str(tmp_path / "multi.")
     r = subprocess.run([api_bin, path, prefix, "NearestNeighbour", str(m), str(nx), str(ny), str(nz), str(nl),
-                        ",".join(map(str, ls)), ",".join(map(str, bs)), "0111"], capture_output=True, text=True,
+                        ",".join(map(str, ls)), ",".join(map(str, bs)), "0111", "setdata"], capture_output=True, text=True,
                        timeout=300)
     assert r.returncode == 0, r.stderr
     d = F.gen_loop_cube(m, seed=1)
@@ -121,6 +121,12 @@ def test_facade_multiview_batch_text_gzip_and_goftcube(engine, api_bin, fe9_tabl
         assert int(gl[0][1]) == m ** 3 and int(gl[0][2]) == 5
         assert abs(float(gl[0][3]) / pk.astype(np.float64).sum() - 1) < 1e-12
         assert abs(float(gl[0][4]) / fw.astype(np.float64).sum() - 1) < 1e-12
+        # GoftCube units as the reference builds them (fomo-CHIANTI.cpp:301-315): the grid units,
+        # the table's emissivity unit, the FWHM unit, then dataunits.at(i) for the velocity
+        # columns i = 2..4, which in 3D are the z grid unit and the units of n and T
+        ul = [ln.split("|")[1:] for ln in r.stdout.splitlines() if ln.startswith("units")]
+        assert len(ul) == 2 and ul[0] == ul[1], r.stdout
+        assert ul[0] == ["Mm", "Mm", "Mm", tab["unit"], "\\AA{}", "Mm", "cm^{-3}", "K"], ul[0]
     finally:
         em.close()
         T.close()

@@ -39,6 +39,45 @@ def test_c0_emission_matches_fp64(engine, c0_data, fe9_table):
     em.close(), T.close(), ds.close()
 
 
+def test_emission_at_half_bin_density_edges(engine, fe9_table):
+    """The density row is the nearest one (fomo-CHIANTI.cpp:97), so a density whose index lies at a
+    half-bin edge sits where G jumps by the difference of two rows. The reference's float pipeline
+    takes the index from float-rounded logarithms (FoMo::log10 returns float), this engine from
+    FP64 ones, as the FP64 oracle does. Densities within +-60 float ulps of four half-bin edges:
+    the engine equals the FP64 oracle at every one; the float pipeline takes the other row at some
+    of them (the recorded difference between the two)."""
+    F = engine
+    path, tab = fe9_table
+    n0, dn = float(tab["logn"][0]), float(tab["logn"][1]) - float(tab["logn"][0])
+    rho = []
+    for k in (5, 12, 20, 33):
+        c = np.float32(10.0 ** (n0 + (k + 0.5) * dn))
+        up, dw = c, c
+        rho.append(c)
+        for _ in range(60):
+            up, dw = np.nextafter(up, np.float32(np.inf)), np.nextafter(dw, np.float32(0))
+            rho += [up, dw]
+    rho = np.array(rho, np.float32)
+    T = np.full(rho.size, np.float32(10.0 ** 5.9))
+    n = rho.size
+    xyz = np.random.default_rng(8).random((3, n)).astype(np.float32)
+    zeros = np.zeros(n, np.float32)
+    ds = F.Dataset(xyz[0], xyz[1], xyz[2], [rho, T, zeros, zeros, zeros])
+    tb = F.Table.from_file(path)
+    em = F.Emission.chianti(ds, tb, tab["lambda0"], tab["atweight"])
+    try:
+        peak, _ = em.read()
+        pk64, _ = O.emission(1, rho, T, tab)
+        pk32, _ = O.emission(0, rho, T, tab)
+        assert (pk64 > 0).all()
+        np.testing.assert_allclose(peak, pk64, rtol=4e-7, atol=0)
+        other_row = np.abs(pk32 / pk64 - 1) > 1e-3
+        print(f"half-bin edges: {int(other_row.sum())} of {n} densities take the other row in the float pipeline")
+        assert other_row.any()
+    finally:
+        em.close(), tb.close(), ds.close()
+
+
 def test_c0_full_render(engine, c0_data, fe9_table):
     """Config 0 end to end: 64^3 loop, view (0, pi/2), 64x64 image, nz 64, nl 30, 0.2 A."""
     F = engine
