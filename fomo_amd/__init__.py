@@ -58,6 +58,8 @@ _SIGS = {
     "fomo_table_destroy": (ctypes.c_int, [ctypes.c_int]),
     "fomo_emission_chianti": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double,
                                              ctypes.c_int, ctypes.c_double, _c_int_p]),
+    "fomo_emission_chianti_interp": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double,
+                                                    ctypes.c_int, ctypes.c_double, ctypes.c_int, _c_int_p]),
     "fomo_emission_thomson": (ctypes.c_int, [ctypes.c_int, _c_int_p]),
     "fomo_emission_given": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_double, _c_int_p]),
     "fomo_goft_destroy": (ctypes.c_int, [ctypes.c_int]),
@@ -380,10 +382,12 @@ class Emission:
         self.n = n
 
     @classmethod
-    def chianti(cls, ds, table, lambda0, atweight, imaging=False, abundratio=1.0):
+    def chianti(cls, ds, table, lambda0, atweight, imaging=False, abundratio=1.0, bilinear=False):
+        """bilinear=True (an extension) interpolates G(T, n) linearly in log n too, where the
+        reference takes the nearest density row."""
         out = ctypes.c_int()
-        _check(lib().fomo_emission_chianti(ds.id, table.id, lambda0, atweight, 1 if imaging else 0, abundratio,
-                                           ctypes.byref(out)))
+        _check(lib().fomo_emission_chianti_interp(ds.id, table.id, lambda0, atweight, 1 if imaging else 0,
+                                                  abundratio, 1 if bilinear else 0, ctypes.byref(out)))
         return cls(out.value, ds.n)
 
     @classmethod

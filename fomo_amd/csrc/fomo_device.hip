@@ -266,14 +266,16 @@ __global__ void k_pack_vel(const float* __restrict__ vx, const float* __restrict
 // ------------------------------------------------------------------------------------
 // K1: emissivity (emissionfromdatacube, fomo-CHIANTI.cpp:228-322; goft :64-121;
 // linefwhm :220-225), FP64 arithmetic from float inputs, output float2(peak, fwhm)
-// per sorted point. The table is staged in LDS when it fits.
+// per sorted point. The table is staged in LDS when it fits. G(T, n) as the reference reads it
+// (linear in log T, nearest row in log n), or with bilinear = 1 (an extension) linear in both,
+// zero outside the table's cells as the reference's temperature rule has it (:100).
 // ------------------------------------------------------------------------------------
 constexpr int kLdsTable = 12288;  // floats of G staged in LDS (48 KB)
 
 __global__ void k_emission_chianti(const float* __restrict__ rho, const float* __restrict__ T, int64_t n,
                                    const float* __restrict__ tlog, const float* __restrict__ nlog,
                                    const float* __restrict__ gtab, int nt, int nrho, double K, double normc,
-                                   int imaging, float2* __restrict__ em) {
+                                   int imaging, int bilinear, float2* __restrict__ em) {
   __shared__ float sg[kLdsTable];
   const bool staged = (int64_t)nt * nrho <= kLdsTable;
   if (staged) {
@@ -290,7 +292,17 @@ __global__ void k_emission_chianti(const float* __restrict__ rho, const float* _
     const double fl = floor((logT - t0) / dt);     // fomo-CHIANTI.cpp:95
     const double rr = round((logrho - n0) / dn);   // :97
     double res = 0.0;
-    if (!(fl < 0 || rr < 0 || fl + 1 > nt - 1 || rr > nrho - 1)) {  // :100
+    if (bilinear) {
+      const double fn = floor((logrho - n0) / dn);
+      if (!(fl < 0 || fn < 0 || fl + 1 > nt - 1 || fn + 1 > nrho - 1)) {
+        const int ft = (int)fl, ct = ft + 1, fr = (int)fn, cr = fr + 1;
+        const double x1 = tlog[ct], x2 = tlog[ft];
+        const double ga = (logT - x1) * ((double)G[fr * nt + ft] - G[fr * nt + ct]) / (x2 - x1) + G[fr * nt + ct];
+        const double gb = (logT - x1) * ((double)G[cr * nt + ft] - G[cr * nt + ct]) / (x2 - x1) + G[cr * nt + ct];
+        const double r1 = nlog[cr], r2 = nlog[fr];
+        res = (logrho - r1) * (ga - gb) / (r2 - r1) + gb;
+      }
+    } else if (!(fl < 0 || rr < 0 || fl + 1 > nt - 1 || rr > nrho - 1)) {  // :100
       const int ft = (int)fl, ct = ft + 1, ri = (int)rr;
       const int gi = ri * nt + ct;
       const double x1 = tlog[ct], y1 = G[gi], x2 = tlog[ft], y2 = G[gi - 1];
@@ -4340,6 +4352,11 @@ int fomo_table_destroy(int id) {
 
 int fomo_emission_chianti(int dataset, int table, double lambda0, double atweight, int imaging, double abundratio,
                           int* out_goft) {
+  return fomo_emission_chianti_interp(dataset, table, lambda0, atweight, imaging, abundratio, 0, out_goft);
+}
+
+int fomo_emission_chianti_interp(int dataset, int table, double lambda0, double atweight, int imaging,
+                                 double abundratio, int interp, int* out_goft) {
   std::lock_guard<std::recursive_mutex> lk(g_mu);
   if (int rc = require_init()) return rc;
   Dataset* ds = find_obj(g_ctx.datasets, dataset);
@@ -4349,6 +4366,8 @@ int fomo_emission_chianti(int dataset, int table, double lambda0, double atweigh
   if (ds->nvars < 2) return fail(FOMO_ERR_ARG, "emission needs variables n and T");
   if (!out_goft) return fail(FOMO_ERR_ARG, "null output id");
   if (!imaging && !(atweight > 0)) return fail(FOMO_ERR_ARG, "atomic weight must be > 0");
+  if (interp != 0 && interp != 1) return fail(FOMO_ERR_ARG, "interp must be 0 (reference) or 1 (bilinear)");
+  if (interp == 1 && tb->nrho < 2) return fail(FOMO_ERR_ARG, "bilinear G(T, n) needs at least 2 density rows");
   std::unique_ptr<Goft> gc(new Goft());
   gc->dataset = dataset;
   gc->kind = 0;
@@ -4363,7 +4382,8 @@ int fomo_emission_chianti(int dataset, int table, double lambda0, double atweigh
     const unsigned nb = std::min<unsigned>(nblk(ds->n), 4096);
     k_emission_chianti<<<nb, kBlock, 0, g_ctx.stream>>>(ds->svars[0]->as<float>(), ds->svars[1]->as<float>(), ds->n,
                                                          tb->logt.as<float>(), tb->logn.as<float>(), tb->g.as<float>(),
-                                                         tb->nt, tb->nrho, K, normc, imaging, gc->em.as<float2>());
+                                                         tb->nt, tb->nrho, K, normc, imaging, interp,
+                                                         gc->em.as<float2>());
   }
   HIPOK(hipGetLastError());
   gc->K = imaging ? 0.0 : K;

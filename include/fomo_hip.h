@@ -138,6 +138,12 @@ int fomo_table_destroy(int id);
  * ------------------------------------------------------------------------------------- */
 int fomo_emission_chianti(int dataset, int table, double lambda0, double atweight, int imaging,
                           double abundratio, int* out_goft);
+/* fomo_emission_chianti with the G(T, n) interpolation chosen: interp 0 is the reference's (linear
+ * in log T, nearest density row, fomo-CHIANTI.cpp:64-121, what fomo_emission_chianti does); interp
+ * 1, an extension with no counterpart in the reference, is bilinear in (log T, log n), zero outside
+ * the table's cells. */
+int fomo_emission_chianti_interp(int dataset, int table, double lambda0, double atweight, int imaging,
+                                 double abundratio, int interp, int* out_goft);
 int fomo_emission_thomson(int dataset, int* out_goft);
 /* Emission cube of a dataset that holds a GoftCube already computed on the host: variables 0 and
  * 1 are (peak, fwhm) of a line at lambda0 (kind 0; variables 2..4 the velocity), or (n tot,

@@ -571,3 +571,60 @@ def test_far_group_lists_match_exact_query(engine):
         ridx = O.render_thomson(1, d["x"], d["y"], d["z"], rt, rp, *views[k], 56, 52, 60, want_idx=True)[1]
         assert np.array_equal(fast[k], ridx)
     ds.close()
+
+
+def test_bilinear_goft_matches_scipy(engine, fe9_table):
+    """The bilinear G(T, n) extension against scipy's RegularGridInterpolator (linear in log T and
+    log n on the table's grid, FP64): the peak ratio bilinear / reference equals G_bil / G_ref per
+    point (the normalisation, density and width factors cancel), with G_ref the reference's
+    linear-in-T, nearest-row value computed here in numpy. Points span the table, with some outside
+    it in T and n (zero in both modes), and the imaging branch is checked too."""
+    from scipy.interpolate import RegularGridInterpolator
+    F = engine
+    path, tab = fe9_table
+    lt = tab["logt"].astype(np.float64)
+    ln = tab["logn"].astype(np.float64)
+    G = tab["g"].astype(np.float64).reshape(tab["nrho"], tab["nt"])
+    r = np.random.default_rng(12)
+    n = 20000
+    # temperatures near the table's G peak (no float underflow of the peaks), a tenth outside the
+    # table's temperature range on either side
+    lT = r.uniform(5.4, 6.4, n)
+    side = r.random(n)
+    lT = np.where(side < 0.05, r.uniform(lt[0] - 0.3, lt[0], n), np.where(side > 0.95, r.uniform(lt[-1], lt[-1] + 0.3, n), lT))
+    lN = r.uniform(ln[0] - 0.5, ln[-1] + 0.5, n)
+    rho = (10.0 ** lN).astype(np.float32)
+    T = (10.0 ** lT).astype(np.float32)
+    lr, lT2 = np.log10(rho.astype(np.float64)), np.log10(T.astype(np.float64))
+    dt, dn = lt[1] - lt[0], ln[1] - ln[0]
+    ft = np.floor((lT2 - lt[0]) / dt)
+    fn = np.floor((lr - ln[0]) / dn)
+    rn = np.round((lr - ln[0]) / dn)
+    in_t = (ft >= 0) & (ft + 1 <= tab["nt"] - 1)
+    bil_ok = in_t & (fn >= 0) & (fn + 1 <= tab["nrho"] - 1)
+    ref_ok = in_t & (rn >= 0) & (rn <= tab["nrho"] - 1)
+    gb = np.zeros(n)
+    gb[bil_ok] = RegularGridInterpolator((ln, lt), G)(np.stack([lr[bil_ok], lT2[bil_ok]], 1))
+    gr = np.zeros(n)
+    i = np.flatnonzero(ref_ok)
+    ri, t0 = rn[i].astype(int), ft[i].astype(int)
+    gr[i] = (lT2[i] - lt[t0 + 1]) * (G[ri, t0] - G[ri, t0 + 1]) / (lt[t0] - lt[t0 + 1]) + G[ri, t0 + 1]
+    xyz = r.random((3, n)).astype(np.float32)
+    zeros = np.zeros(n, np.float32)
+    ds = F.Dataset(xyz[0], xyz[1], xyz[2], [rho, T, zeros, zeros, zeros])
+    tb = F.Table.from_file(path)
+    try:
+        for imaging in (False, True):
+            e0 = F.Emission.chianti(ds, tb, tab["lambda0"], tab["atweight"], imaging=imaging)
+            e1 = F.Emission.chianti(ds, tb, tab["lambda0"], tab["atweight"], imaging=imaging, bilinear=True)
+            p0, w0 = e0.read()
+            p1, w1 = e1.read()
+            e0.close(), e1.close()
+            assert np.array_equal(w0, w1)
+            assert np.array_equal(p1 != 0, gb != 0) and np.array_equal(p0 != 0, gr != 0)
+            both = (gb != 0) & (gr != 0)
+            assert both.sum() > n // 3
+            np.testing.assert_allclose(p1[both].astype(np.float64) / p0[both], gb[both] / gr[both], rtol=1e-6)
+            assert not np.array_equal(p0, p1)
+    finally:
+        tb.close(), ds.close()
