@@ -1659,26 +1659,21 @@ __global__ void __launch_bounds__(256) k_tet_walk(const float4* __restrict__ tre
         } else {
           // faces opposite v0..v3: (2,1,3), (0,2,3), (1,0,3), (0,1,2); the walk stops in a tet no
           // face of which has the sample strictly beyond it, which is certain only when no test
-          // there was undecided (0)
-          int next = -1, o;
-          bool und = false;
-          if ((o = orient_d(q[2], q[1], q[3], px, py, pz)) < 0) {
-            next = nb.x;
-          } else {
-            und = o == 0;
-            if ((o = orient_d(q[0], q[2], q[3], px, py, pz)) < 0) {
-              next = nb.y;
-            } else {
-              und |= o == 0;
-              if ((o = orient_d(q[1], q[0], q[3], px, py, pz)) < 0) {
-                next = nb.z;
-              } else {
-                und |= o == 0;
-                if ((o = orient_d(q[0], q[1], q[2], px, py, pz)) < 0) next = nb.w;
-                else und |= o == 0;
-              }
-            }
-          }
+          // there was undecided (0). It leaves through the face whose plane the sample is furthest
+          // beyond in the determinant's measure (any such face keeps the walk acyclic on a
+          // Delaunay triangulation; this choice takes fewer steps than the first such face).
+          double dd[4];
+          const int o0 = orient_d(q[2], q[1], q[3], px, py, pz, &dd[0]);
+          const int o1 = orient_d(q[0], q[2], q[3], px, py, pz, &dd[1]);
+          const int o2 = orient_d(q[1], q[0], q[3], px, py, pz, &dd[2]);
+          const int o3 = orient_d(q[0], q[1], q[2], px, py, pz, &dd[3]);
+          const bool und = o0 == 0 || o1 == 0 || o2 == 0 || o3 == 0;
+          int next = -1;
+          double best = 0.0;
+          if (o0 < 0 && dd[0] < best) best = dd[0], next = nb.x;
+          if (o1 < 0 && dd[1] < best) best = dd[1], next = nb.y;
+          if (o2 < 0 && dd[2] < best) best = dd[2], next = nb.z;
+          if (o3 < 0 && dd[3] < best) best = dd[3], next = nb.w;
           if (next < 0) {
             flag = und;
             break;
