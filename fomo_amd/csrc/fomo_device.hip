@@ -418,8 +418,13 @@ __global__ void __launch_bounds__(256) k_view_extents_multi(const float4* __rest
   }
 }
 
+// With em (the Thomson emission: n tot, n pol per point), writes each point's pB term
+// max(0, pol * sin^2 chi) in FP64 to term instead of sin^2 chi to sin2chi: the product of the two
+// floats is exact in FP64, so k_los_thomson adds the same values it would form per sample, with
+// one gather per sample instead of two.
 __global__ void k_view_rotate_thomson(const float4* __restrict__ pts, int64_t n, double cbc, double sbc, double cl,
-                                      double sl, double blend, float* __restrict__ sin2chi,
+                                      double sl, double blend, const float2* __restrict__ em,
+                                      float* __restrict__ sin2chi, double* __restrict__ term,
                                       double* __restrict__ partial) {
   double v[6] = {INFINITY, -INFINITY, INFINITY, -INFINITY, INFINITY, -INFINITY};
   const double u0 = sbc * cl, u1 = -sbc * sl, u2 = cbc;  // :194 (clamped sin_b, cos_b)
@@ -435,7 +440,13 @@ __global__ void k_view_rotate_thomson(const float4* __restrict__ pts, int64_t n,
     const double za = fz * (1.0 - blend) + pz * blend;
     const double ru = ((0.0 + u0 * xa) + u1 * ya) + u2 * za;
     const double rr = ((0.0 + xa * xa) + ya * ya) + za * za;
-    sin2chi[p] = (float)(1. - ru * ru / rr);  // :198
+    const float s2 = (float)(1. - ru * ru / rr);  // :198
+    if (em) {
+      const double t = (double)em[p].y * (double)s2;  // :327-334
+      term[p] = t < 0. ? 0. : t;
+    } else {
+      sin2chi[p] = s2;
+    }
     v[0] = fmin(v[0], xa); v[1] = fmax(v[1], xa);
     v[2] = fmin(v[2], ya); v[3] = fmax(v[3], ya);
     v[4] = fmin(v[4], za); v[5] = fmax(v[5], za);
@@ -2053,19 +2064,16 @@ __global__ void __launch_bounds__(256) k_los_imaging(int R, int nz, const LineSe
 }
 
 // Thomson pB: intens += max(0, pol * sin^2 chi) (fomo-thomson.cpp:325-338, scale :350-359)
+// term: each point's max(0, pol * sin^2 chi) for the view (k_view_rotate_thomson)
 __global__ void __launch_bounds__(256) k_los_thomson(const int32_t* __restrict__ idx, int R, int nz,
-                                                     const float2* __restrict__ em, const float* __restrict__ s2c,
+                                                     const double* __restrict__ term,
                                                      const ViewGeom* __restrict__ vgp, float* __restrict__ out) {
   const int ray = blockIdx.x * blockDim.x + threadIdx.x;
   if (ray >= R) return;
   double acc = 0.0;
   for (int k = 0; k < nz; ++k) {
     const int p = idx[(size_t)k * R + ray];
-    if (p >= 0) {
-      double t = (double)em[p].y * (double)s2c[p];
-      if (t < 0.) t = 0;
-      acc += t;
-    }
+    if (p >= 0) acc += term[p];
   }
   out[ray] = (float)(vgp->scale * acc);
 }
@@ -3094,8 +3102,10 @@ void enqueue_view_finalize(hipStream_t s, int mode, double l, double b, int nx, 
                                        clear2);
 }
 
+// Thomson: with em, aux receives the points' pB terms (n doubles), else sin^2 chi (n floats).
 void enqueue_view_geom(hipStream_t s, const Dataset& ds, int mode, double l, double b, int nx, int ny, int nz, int dn,
-                       float* aux, double* partial, ViewGeom* geom, int32_t* clear2 = nullptr) {
+                       float* aux, double* partial, ViewGeom* geom, int32_t* clear2 = nullptr,
+                       const float2* em = nullptr) {
   unsigned nb = std::min<unsigned>(nblk(ds.n), kMaxPartials);
   const double cb = cos(b), sb = sin(b), cl = cos(l), sl = sin(l);
   if (mode == MODE_NN) {
@@ -3113,7 +3123,8 @@ void enqueue_view_geom(hipStream_t s, const Dataset& ds, int mode, double l, dou
     if (std::abs(sbc) < eps) sbc = eps;
     const double blend = 1.0 / (1.0 + exp(-20.0 * (std::fabs(b) - 1.309)));  // :173-176
     ProfScope ps("k_view_rotate_thomson", s);
-    k_view_rotate_thomson<<<nb, kBlock, 0, s>>>(ds.pts.as<float4>(), ds.n, cbc, sbc, cl, sl, blend, aux, partial);
+    k_view_rotate_thomson<<<nb, kBlock, 0, s>>>(ds.pts.as<float4>(), ds.n, cbc, sbc, cl, sl, blend, em, aux,
+                                                reinterpret_cast<double*>(aux), partial);
   }
   enqueue_view_finalize(s, mode, l, b, nx, ny, nz, dn, partial, nb, geom, clear2);
 }
@@ -3475,7 +3486,7 @@ void render_sizes(int mode, int64_t n, int64_t R, int64_t S, int nl, int64_t sz[
     return;
   }
   sz[2] = (int64_t)sizeof(int32_t) * S;
-  sz[3] = (int64_t)sizeof(float) * (mode != MODE_THOMSON ? S : n);
+  sz[3] = mode != MODE_THOMSON ? (int64_t)sizeof(float) * S : (int64_t)sizeof(double) * n;  // (pB terms)
   sz[4] = (int64_t)sizeof(float) * R * std::max(nl, 1);
   // ray flags, then (NN) the fused pass's direct-pass list
   sz[5] = mode != MODE_THOMSON ? (int64_t)sizeof(int32_t) * (R + nn_los_list_ints(R, 1)) : 0;
@@ -3542,7 +3553,7 @@ int enqueue_render(hipStream_t s, const Dataset& ds, const Goft& gc, int mode, d
   // NN renders: the fused pass's direct-pass list follows the R ray flags in rb.flag
   int32_t* const dlist = mode == MODE_NN ? rb.flag + R : nullptr;
   enqueue_view_geom(s, ds, mode == MODE_THOMSON ? MODE_THOMSON : MODE_NN, l, b, nx, ny, nz, dn, rb.aux, rb.partial,
-                    rb.geom, dlist);
+                    rb.geom, dlist, mode == MODE_THOMSON ? gc.em.as<float2>() : nullptr);
   if (mode == MODE_TET || mode == MODE_LINEAR) {
     // the tetra path leaves rb.idx (S ints) unused: it holds the view's redo list
     if (int rc = enqueue_tet(s, ds, rb.geom, nx, ny, nz, mode == MODE_LINEAR, gc.rec.as<float4>(), gc.rs4, gc.ri, 1,
@@ -3562,7 +3573,8 @@ int enqueue_render(hipStream_t s, const Dataset& ds, const Goft& gc, int mode, d
   } else {
     enqueue_nn(s, ds, rb.geom, nx, ny, nz, rb.idx, rb.rayflag);
     ProfScope ps("k_los_thomson", s);
-    k_los_thomson<<<nblk(R), 256, 0, s>>>(rb.idx, (int)R, nz, gc.em.as<float2>(), rb.aux, rb.geom, rb.out);
+    k_los_thomson<<<nblk(R), 256, 0, s>>>(rb.idx, (int)R, nz, reinterpret_cast<const double*>(rb.aux), rb.geom,
+                                          rb.out);
   }
   HIPOK(hipGetLastError());
   return FOMO_OK;
