@@ -1494,10 +1494,18 @@ __device__ __forceinline__ int tet_start(const TetGrid& tg, const int32_t* __res
   return tstart[((int64_t)c[2] * tg.dims[1] + c[1]) * tg.dims[0] + c[0]];
 }
 
-// Tet records: the vertex coordinates sit beside the vertex ids, so one step of the walk is one
-// round of loads (record + neighbours) instead of two dependent ones.
-__global__ void k_tet_records(const int4* __restrict__ tv, const float4* __restrict__ pts, int64_t n,
-                              float4* __restrict__ trec) {
+// Tet records, kTetRec float4 each: the 4 vertices {x, y, z, sorted position as int bits (-1 =
+// infinite)} and the 4 neighbours (int bits). The vertex coordinates sit beside the vertex ids, so
+// one step of the walk is one round of loads instead of two dependent ones, and the neighbours in
+// the same 80 bytes, so a step touches one 128-byte line or two (separate arrays: always two).
+constexpr int kTetRec = 5;
+
+__device__ __forceinline__ int4 tet_nbrs(const float4& f) {
+  return make_int4(__float_as_int(f.x), __float_as_int(f.y), __float_as_int(f.z), __float_as_int(f.w));
+}
+
+__global__ void k_tet_records(const int4* __restrict__ tv, const int4* __restrict__ tn,
+                              const float4* __restrict__ pts, int64_t n, float4* __restrict__ trec) {
   const int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (q >= n) return;
   const int4 v = tv[q];
@@ -1506,8 +1514,11 @@ __global__ void k_tet_records(const int4* __restrict__ tv, const float4* __restr
     const int id = ivec(v, m);
     float4 r = id >= 0 ? pts[id] : make_float4(0.f, 0.f, 0.f, 0.f);
     r.w = __int_as_float(id);
-    trec[4 * q + m] = r;
+    trec[kTetRec * q + m] = r;
   }
+  const int4 nb = tn[q];
+  trec[kTetRec * q + 4] = make_float4(__int_as_float(nb.x), __int_as_float(nb.y), __int_as_float(nb.z),
+                                      __int_as_float(nb.w));
 }
 
 // The outputs of one sample from its tet q (vertex records) or outside (no tet): the nearest
@@ -1595,7 +1606,7 @@ __device__ __forceinline__ void tet_emit(const float4 (&q)[4], bool outside, dou
 // the step cap, goes to the redo list (redo[0]: entries, redo[1]: samples beyond redo_cap, then
 // the sample indices), which k_tet_redo walks again with exact tests.
 template <int RS4, bool LINEAR>
-__global__ void __launch_bounds__(256) k_tet_walk(const float4* __restrict__ trec, const int4* __restrict__ tn,
+__global__ void __launch_bounds__(256) k_tet_walk(const float4* __restrict__ trec,
                                                   const int32_t* __restrict__ tstart, TetGrid tg,
                                                   const ViewGeom* __restrict__ vgp, int nx, int ny, int nz,
                                                   int seglen, const float4* __restrict__ rec, RecInfo ri, int nlines,
@@ -1644,9 +1655,9 @@ __global__ void __launch_bounds__(256) k_tet_walk(const float4* __restrict__ tre
       int steps = 0;
       for (;;) {
         if (t != tq) {
-          const float4* tr = trec + (size_t)t * 4;
+          const float4* tr = trec + (size_t)t * kTetRec;
           q[0] = tr[0], q[1] = tr[1], q[2] = tr[2], q[3] = tr[3];
-          nb = tn[t];
+          nb = tet_nbrs(tr[4]);
           tq = t;
         }
         int inf = -1;
@@ -1725,7 +1736,7 @@ __device__ __forceinline__ int orient_x(const float4& a, const float4& b, const 
 }
 
 template <int RS4, bool LINEAR>
-__global__ void __launch_bounds__(64) k_tet_redo(const float4* __restrict__ trec, const int4* __restrict__ tn,
+__global__ void __launch_bounds__(64) k_tet_redo(const float4* __restrict__ trec,
                                                  const int32_t* __restrict__ tstart, TetGrid tg,
                                                  const ViewGeom* __restrict__ vgp, int nx, int ny, int nz,
                                                  const float4* __restrict__ rec, RecInfo ri, int nlines,
@@ -1760,9 +1771,9 @@ __global__ void __launch_bounds__(64) k_tet_redo(const float4* __restrict__ trec
         ++unresolved;
         break;
       }
-      const float4* tr = trec + (size_t)t * 4;
+      const float4* tr = trec + (size_t)t * kTetRec;
       q[0] = tr[0], q[1] = tr[1], q[2] = tr[2], q[3] = tr[3];
-      const int4 nb = tn[t];
+      const int4 nb = tet_nbrs(tr[4]);
       int inf = -1;
 #pragma unroll
       for (int m = 0; m < 4; ++m)
@@ -2131,8 +2142,8 @@ struct Dataset {
   int tile_log2 = 3;  // ray tile of k_nn / k_nn_far blocks: 2^tile_log2 x 256 / 2^tile_log2 (autotuned)
   // tetra path: Delaunay triangulation of the points, built on the host on first use
   int64_t ntets = 0;      // 0: not built
-  DevBuf trec;            // 4 float4 per tet: vertex {x, y, z, sorted position as int bits (-1 = infinite)}
-  DevBuf tn;              // int4 per tet: neighbour opposite each vertex
+  DevBuf trec;            // kTetRec float4 per tet: 4 vertices {x, y, z, sorted position as int bits (-1 =
+                          // infinite)}, then the neighbour opposite each vertex (int bits)
   DevBuf tstart;          // int32 per start-grid cell: a finite tet near the cell centre
   DevBuf tetstats;        // uint64 {samples walked again with exact tests, samples left unresolved}
   TetGrid tk{};           // start grid geometry
@@ -2491,21 +2502,20 @@ int build_tets(Dataset& ds) {
       hn[q] = make_int4(T[q].n[0], T[q].n[1], T[q].n[2], T[q].n[3]);
     }
   });
-  DevBuf tv;
-  if (tv.alloc(sizeof(int4) * hv.size()) || ds.trec.alloc(4 * sizeof(float4) * hv.size()) ||
-      ds.tn.alloc(sizeof(int4) * hn.size()) || ds.tstart.alloc(sizeof(int32_t) * ng) ||
+  DevBuf tv, tn;
+  if (tv.alloc(sizeof(int4) * hv.size()) || tn.alloc(sizeof(int4) * hn.size()) ||
+      ds.trec.alloc(kTetRec * sizeof(float4) * hv.size()) || ds.tstart.alloc(sizeof(int32_t) * ng) ||
       ds.tetstats.ensure(2 * sizeof(unsigned long long))) {
     ds.trec.release();
-    ds.tn.release();
     ds.tstart.release();
     return fail(FOMO_ERR_NOMEM, "device allocation failed (tets)");
   }
   HIPOK(hipMemsetAsync(ds.tetstats.p, 0, 2 * sizeof(unsigned long long), s));
   HIPOK(hipMemcpyAsync(tv.p, hv.data(), sizeof(int4) * hv.size(), hipMemcpyHostToDevice, s));
-  HIPOK(hipMemcpyAsync(ds.tn.p, hn.data(), sizeof(int4) * hn.size(), hipMemcpyHostToDevice, s));
+  HIPOK(hipMemcpyAsync(tn.p, hn.data(), sizeof(int4) * hn.size(), hipMemcpyHostToDevice, s));
   HIPOK(hipMemcpyAsync(ds.tstart.p, start.data(), sizeof(int32_t) * ng, hipMemcpyHostToDevice, s));
-  k_tet_records<<<nblk((int64_t)hv.size()), kBlock, 0, s>>>(tv.as<int4>(), ds.pts.as<float4>(), (int64_t)hv.size(),
-                                                             ds.trec.as<float4>());
+  k_tet_records<<<nblk((int64_t)hv.size()), kBlock, 0, s>>>(tv.as<int4>(), tn.as<int4>(), ds.pts.as<float4>(),
+                                                             (int64_t)hv.size(), ds.trec.as<float4>());
   HIPOK(hipGetLastError());
   if (int rc = sync_stream()) return rc;
   ds.tk = tk;
@@ -2979,7 +2989,6 @@ int build_index(Dataset& ds, double cell_scale) {
   // the triangulation refers to sorted positions, which the new cell order changes
   ds.ntets = 0;
   ds.trec.release();
-  ds.tn.release();
   ds.tstart.release();
   if (cell_scale <= 0) cell_scale = 1.0;
   ds.cell_scale = cell_scale;
@@ -3236,10 +3245,10 @@ int enqueue_tet(hipStream_t s, const Dataset& ds, const ViewGeom* geom, int nx, 
   const unsigned nb = (unsigned)((nthreads + 255) / 256);
   ProfScope ps(linear ? "k_tet_walk_linear" : "k_tet_walk", s);
 #define FOMO_TET_ARGS                                                                                        \
-  ds.trec.as<float4>(), ds.tn.as<int4>(), ds.tstart.as<int32_t>(), ds.tk, geom, nx, ny, nz, seglen, rec, ri, \
+  ds.trec.as<float4>(), ds.tstart.as<int32_t>(), ds.tk, geom, nx, ny, nz, seglen, rec, ri, \
       nlines, svlos, sem, vert, redo, redo_cap
 #define FOMO_REDO_ARGS                                                                                    \
-  ds.trec.as<float4>(), ds.tn.as<int4>(), ds.tstart.as<int32_t>(), ds.tk, geom, nx, ny, nz, rec, ri, nlines, \
+  ds.trec.as<float4>(), ds.tstart.as<int32_t>(), ds.tk, geom, nx, ny, nz, rec, ri, nlines, \
       svlos, sem, vert, redo, redo_cap, ds.tetstats.as<unsigned long long>()
 #define FOMO_TET_LAUNCH(RS, LIN)                                       \
   do {                                                                 \
