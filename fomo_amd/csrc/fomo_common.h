@@ -39,7 +39,22 @@ struct GridInfo {
   float bmin[3], bmax[3];
   int dims[3];
   int pad;
+  double sc[3];     // bounding sphere of the points: centre (the bbox centre) and squared radius,
+  double sr2;       // rounded up (every point p has |p - sc|^2 <= sr2)
 };
+
+// A search box [lo, hi] (float bounds) that holds no data point: it misses the data's bounding
+// box, or lies wholly outside its bounding sphere (the exact query would return -1 for it).
+__device__ __forceinline__ bool box_misses_data(const GridInfo& g, float lo0, float lo1, float lo2, float hi0,
+                                                float hi1, float hi2) {
+  if (!(hi0 > g.bmin[0] && lo0 < g.bmax[0] && hi1 > g.bmin[1] && lo1 < g.bmax[1] && hi2 > g.bmin[2] &&
+        lo2 < g.bmax[2]))
+    return true;
+  const double c0 = fmin(fmax(g.sc[0], (double)lo0), (double)hi0) - g.sc[0];
+  const double c1 = fmin(fmax(g.sc[1], (double)lo1), (double)hi1) - g.sc[1];
+  const double c2 = fmin(fmax(g.sc[2], (double)lo2), (double)hi2) - g.sc[2];
+  return (c0 * c0 + c1 * c1) + c2 * c2 > g.sr2;
+}
 
 // Candidate lookup grid: a finer grid over the data bbox whose cells list the only points that
 // can be the (unconstrained) nearest point of any target inside the cell.
@@ -682,9 +697,8 @@ __device__ __forceinline__ void nn_los_wave(const int view, const int64_t wave, 
                         box.hi2);
     }
     if (r == -2) {
-      if (!(box.hi0 > g.bmin[0] && box.lo0 < g.bmax[0] && box.hi1 > g.bmin[1] && box.lo1 < g.bmax[1] &&
-            box.hi2 > g.bmin[2] && box.lo2 < g.bmax[2])) {
-        r = -1;  // no data point can be strictly inside the box
+      if (box_misses_data(g, box.lo0, box.lo1, box.lo2, box.hi0, box.hi1, box.hi2)) {
+        r = -1;  // no data point can be inside the box
       } else {
         double best = INFINITY;
         int bpos = -1, borig = 0x7fffffff;

@@ -147,6 +147,17 @@ __global__ void k_bbox(const float* __restrict__ x, const float* __restrict__ y,
   block_minmax6(v, partial + 6 * blockIdx.x);
 }
 
+// Largest squared distance of the points from c (in slot 1 of the block's 6 partials).
+__global__ void k_sphere_r2(const float* __restrict__ x, const float* __restrict__ y, const float* __restrict__ z,
+                            int64_t n, double c0, double c1, double c2, double* __restrict__ partial) {
+  double v[6] = {INFINITY, 0.0, INFINITY, -INFINITY, INFINITY, -INFINITY};
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const double a = (double)x[i] - c0, b = (double)y[i] - c1, c = (double)z[i] - c2;
+    v[1] = fmax(v[1], (a * a + b * b) + c * c);
+  }
+  block_minmax6(v, partial + 6 * blockIdx.x);
+}
+
 __global__ void k_reduce_partials(const double* __restrict__ partial, int np, double* __restrict__ out6) {
   double v[6] = {INFINITY, -INFINITY, INFINITY, -INFINITY, INFINITY, -INFINITY};
   for (int i = threadIdx.x; i < np; i += blockDim.x)
@@ -889,9 +900,8 @@ __device__ __forceinline__ void nn_sample(const float4* __restrict__ pts, const 
     const double md = vg.md;
     const float lo0 = (float)(p0 - md), lo1 = (float)(p1 - md), lo2 = (float)(p2 - md);
     const float hi0 = (float)(p0 + md), hi1 = (float)(p1 + md), hi2 = (float)(p2 + md);
-    if (!(hi0 > g.bmin[0] && lo0 < g.bmax[0] && hi1 > g.bmin[1] && lo1 < g.bmax[1] && hi2 > g.bmin[2] &&
-          lo2 < g.bmax[2]))
-      r = -1;  // no data point can be strictly inside the box
+    if (box_misses_data(g, lo0, lo1, lo2, hi0, hi1, hi2))
+      r = -1;  // no data point can be inside the box
     else
       rayflag[(size_t)i * nx + j] = 1;  // left as -2 for k_nn_far (exact octree query)
   }
@@ -3007,6 +3017,15 @@ int build_index(Dataset& ds, double cell_scale) {
   int rc = sync_stream();
   if (rc) return rc;
   for (int a = 0; a < 6; ++a) ds.bbox[a] = (float)bb[a];
+  // bounding sphere about the bbox centre: search boxes wholly outside it hold no point
+  const double sc[3] = {0.5 * (bb[0] + bb[1]), 0.5 * (bb[2] + bb[3]), 0.5 * (bb[4] + bb[5])};
+  k_sphere_r2<<<nb, kBlock, 0, s>>>(ds.raw[0].as<float>(), ds.raw[1].as<float>(), ds.raw[2].as<float>(), n, sc[0],
+                                    sc[1], sc[2], part);
+  k_reduce_partials<<<1, kBlock, 0, s>>>(part, nb, part + 6 * kMaxPartials);
+  HIPOK(hipGetLastError());
+  double sp[6];
+  HIPOK(hipMemcpyAsync(sp, part + 6 * kMaxPartials, sizeof(sp), hipMemcpyDeviceToHost, s));
+  if ((rc = sync_stream())) return rc;
   // grid: cell edge h = scale * (V/N)^(1/3) over the non-degenerate axes
   double ext[3] = {bb[1] - bb[0], bb[3] - bb[2], bb[5] - bb[4]};
   double vol = 1.0;
@@ -3037,6 +3056,8 @@ int build_index(Dataset& ds, double cell_scale) {
   g.h = h;
   g.invh = 1.0 / h;
   g.slack = 1e-6 * h;
+  for (int a = 0; a < 3; ++a) g.sc[a] = sc[a];
+  g.sr2 = sp[1] * (1.0 + 1e-9) + 1e-300;  // rounded up: the FP64 distances are within 1e-15 of exact
   ds.ncells = (int64_t)g.dims[0] * g.dims[1] * g.dims[2];
   bool rect = false;
   if (int rc2 = build_rect(ds, &rect)) return rc2;
