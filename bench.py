@@ -44,11 +44,11 @@ FLOP_PER_W = 8.0
 # HBM bytes per view (read + write) of the default workload's fused pass, from two rocprofv3 --pmc
 # runs (FETCH_SIZE x 2.000, WRITE_SIZE x 1.000, the factors a 1 GiB copy kernel gives) divided by
 # the views of a launch, per index kind of the cube: profiles/batched_render_256cube_counters.md
-MEASURED_TRAFFIC = {"scattered": {"k_nn_los": 1.445e9}, "rectilinear": {"k_nn_los": 5.371e8}}
+MEASURED_TRAFFIC = {"scattered": {"k_nn_los": 1.177e9}, "rectilinear": {"k_nn_los": 5.667e8}}
 # FP64 work the fused pass executes per view (FLOP: 2 per FMA, 1 per MUL/ADD/transcendental,
 # from SQ_INSTS_VALU_{FMA,MUL,ADD,TRANS}_F64 x 64 lanes) on the rectilinear default cube, beside the
 # reference formula's count: profiles/batched_render_256cube_counters.md
-EXECUTED_FP64 = {"rectilinear": {"k_nn_los": 3.690e9}}
+EXECUTED_FP64 = {"rectilinear": {"k_nn_los": 1.509e9}}
 
 
 def log(*a):

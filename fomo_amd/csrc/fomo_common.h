@@ -504,6 +504,12 @@ __device__ __forceinline__ void profile_chains(const double (&ex)[3], double (&g
 }
 
 
+// One term of a spectral sum: peak * g added to the float sum of a (ray, line, pixel), rounded
+// once (the reference sums its profiles in float: intens, fomo-nearestneighbour.cpp:128, 213).
+// Every kernel that integrates a profile adds through here, so the fused and the separate passes
+// give the same bits.
+__device__ __forceinline__ float spectral_add(float acc, float peak, double g) { return fmaf(peak, (float)g, acc); }
+
 // Orders the LDS accesses of one wave around it: what its lanes wrote before is visible to the
 // reads of any lane after it (no code motion across, LDS operations complete in order).
 __device__ __forceinline__ void wave_sync() {
@@ -561,7 +567,7 @@ __device__ __forceinline__ double group_bcast(double x) {
 //            and lane q runs the profile's chain q (pixels q, q+4, ...) of every sample, adding
 //            peak_l * g to line l's sums. The profile is evaluated once for all lines.
 // Both add, for each (ray, line, pixel), the same values in the same order as k_nn +
-// k_los_spectral (profile_chains, then fma(peak, g, sum)), so a batch of L lines and L
+// k_los_spectral (profile_chains, then spectral_add), so a batch of L lines and L
 // single-line renders give identical images. As in k_los_spectral, a (ray, line) with a sample
 // whose recurrence anchors would under- or overflow is flagged (flags[line * R + ray]) and redone
 // by the DIRECT instantiation, launched right after, with one exp per sample and lambda pixel.
@@ -648,9 +654,9 @@ __device__ __forceinline__ void nn_los_wave(const int view, const int64_t wave, 
   const double lamc = double(0) / (nl - 1) * wla - wla / 2.;  // lambdaval of the first pixel (:200)
   // lambdaval of the first pixel of pass p (k_los_spectral's l0 = p * CH)
   const auto lamc_of = [&](int p) { return double(p * CH) / (nl - 1) * wla - wla / 2.; };
-  double acc[NACC];
+  float acc[NACC];
 #pragma unroll
-  for (int c = 0; c < NACC; ++c) acc[c] = 0.0;
+  for (int c = 0; c < NACC; ++c) acc[c] = 0.f;
   int prev = -1;  // this lane's last resolved point: the warm start of the exact query
   // Pipeline over groups of G samples. While group kb is integrated, with a lookup grid: the point
   // record of group kb + G (winner picked from its lookup cell record) and the lookup cell record
@@ -765,7 +771,7 @@ __device__ __forceinline__ void nn_los_wave(const int view, const int64_t wave, 
 #pragma unroll
             for (int c = 0; c < CH; ++c) {
               const double d = d0 + (double)c * dl;
-              acc[c] += peak * exp(-kap * d * d);
+              acc[c] = spectral_add(acc[c], (float)peak, exp(-kap * d * d));
             }
           }
           return;
@@ -783,7 +789,7 @@ __device__ __forceinline__ void nn_los_wave(const int view, const int64_t wave, 
         for (int c0 = 0; c0 < CH; c0 += 4) {
 #pragma unroll
           for (int jj = 0; jj < 4; ++jj) {
-            acc[c0 + jj] = fma(peak, gc[jj], acc[c0 + jj]);
+            acc[c0 + jj] = spectral_add(acc[c0 + jj], (float)peak, gc[jj]);
             gc[jj] *= RR[jj];
             RR[jj] *= q16;
           }
@@ -864,7 +870,7 @@ __device__ __forceinline__ void nn_los_wave(const int view, const int64_t wave, 
               const double e = exp(-kap * d * d);
 #pragma unroll
               for (int l = 0; l < SL; ++l)
-                if (use & (1u << l)) acc[(p * SL + l) * CJ + m] += (double)pk[l] * e;
+                if (use & (1u << l)) acc[(p * SL + l) * CJ + m] = spectral_add(acc[(p * SL + l) * CJ + m], pk[l], e);
             }
           }
           return;
@@ -886,7 +892,7 @@ __device__ __forceinline__ void nn_los_wave(const int view, const int64_t wave, 
 #pragma unroll
           for (int m = 0; m < CJ; ++m) {
 #pragma unroll
-            for (int l = 0; l < SL; ++l) acc[(p * SL + l) * CJ + m] = fma((double)pk[l], gq, acc[(p * SL + l) * CJ + m]);
+            for (int l = 0; l < SL; ++l) acc[(p * SL + l) * CJ + m] = spectral_add(acc[(p * SL + l) * CJ + m], pk[l], gq);
             gq *= Rq;
             Rq *= q16;
           }
@@ -977,7 +983,7 @@ __device__ __forceinline__ void nn_los_wave(const int view, const int64_t wave, 
         float* o = ls.out[l] + ooff + (size_t)ray * nl + p * CH;
 #pragma unroll
         for (int m = 0; m < CJ; ++m)
-          if (p * CH + 4 * m + q < nl) o[4 * m + q] = (float)(sc * acc[(p * SL + l) * CJ + m]);
+          if (p * CH + 4 * m + q < nl) o[4 * m + q] = (float)(sc * (double)acc[(p * SL + l) * CJ + m]);
       }
   } else {
     if (!DIRECT) {
@@ -987,7 +993,7 @@ __device__ __forceinline__ void nn_los_wave(const int view, const int64_t wave, 
     float* o = out + (size_t)ray * nl;
 #pragma unroll
     for (int c = 0; c < CH; ++c)
-      if (c < nl) o[c] = (float)(sc * acc[c]);
+      if (c < nl) o[c] = (float)(sc * (double)acc[c]);
   }
 }
 
@@ -996,8 +1002,11 @@ __device__ __forceinline__ void nn_los_wave(const int view, const int64_t wave, 
 // Direct pass: any grid; its waves take the listed waves of the main pass in turn, and the last
 // block to finish clears the list (dlist[0], dlist[1]) for the next main pass. The list starts
 // cleared: the view finalize kernels clear it before the first main pass of a buffer.
+// The main pass of the four-line shared profile on a rectilinear dataset asks for 3 waves per SIMD
+// (at most 168 VGPRs; a few values spill): its FP64 chains and float sums otherwise take about
+// 185 registers and 2 waves (C2 11.64 -> 11.22 ms per step; the scattered index gained nothing).
 template <int RS4, int CH, int G, bool DIRECT, int SL, int NP = 1, bool RECT = false>
-__global__ void __launch_bounds__(256) k_nn_los(const float4* __restrict__ pts, GridInfo g,
+__global__ void __launch_bounds__(256, (RECT && !DIRECT && SL == 4) ? 3 : 1) k_nn_los(const float4* __restrict__ pts, GridInfo g,
                                                 const float4* __restrict__ lslot, const float4* __restrict__ lcand,
                                                 LookupInfo lk, const float4* __restrict__ mpts,
                                                 const float4* __restrict__ nlo, const float4* __restrict__ nhi,

@@ -2011,9 +2011,9 @@ __global__ void __launch_bounds__(256) k_los_spectral(int R, int nz, const LineS
   }
   const double dl = wla / (nl - 1);
   const double lamc = double(l0) / (nl - 1) * wla - wla / 2.;  // lambdaval of the first pixel of the pass (:200)
-  double acc[CH];
+  float acc[CH];
 #pragma unroll
-  for (int c = 0; c < CH; ++c) acc[c] = 0.0;
+  for (int c = 0; c < CH; ++c) acc[c] = 0.f;
   bool bad = false;
   const int rr_ = live ? ray : 0;
   // per-sample (peak, fwhm) and LOS velocity written by k_nn; one-sample lookahead
@@ -2036,7 +2036,7 @@ __global__ void __launch_bounds__(256) k_los_spectral(int R, int nz, const LineS
 #pragma unroll
       for (int c = 0; c < CH; ++c) {
         const double d = d0 + (double)c * dl;
-        acc[c] += peak * exp(-kap * d * d);
+        acc[c] = spectral_add(acc[c], (float)peak, exp(-kap * d * d));
       }
     } else {
       double xa[3];
@@ -2048,7 +2048,7 @@ __global__ void __launch_bounds__(256) k_los_spectral(int R, int nz, const LineS
         for (int c0 = 0; c0 < CH; c0 += 4) {
 #pragma unroll
           for (int j = 0; j < 4; ++j) {
-            acc[c0 + j] = fma(peak, g[j], acc[c0 + j]);
+            acc[c0 + j] = spectral_add(acc[c0 + j], (float)peak, g[j]);
             g[j] *= RR[j];
             RR[j] *= q16;
           }
@@ -2069,7 +2069,7 @@ __global__ void __launch_bounds__(256) k_los_spectral(int R, int nz, const LineS
   float* o = ls.out[line] + (size_t)ray * nl + l0;
 #pragma unroll
   for (int c = 0; c < CH; ++c)
-    if (l0 + c < nl) o[c] = (float)(sc * acc[c]);
+    if (l0 + c < nl) o[c] = (float)(sc * (double)acc[c]);
 }
 
 // nl == 1 (Imaging): intens += peak (fomo-nearestneighbour.cpp:217-225)
