@@ -874,12 +874,11 @@ template <int RS4>
 __device__ __forceinline__ void nn_sample(const float4* __restrict__ pts, const GridInfo& g,
                                           const float4* __restrict__ lslot, const float4* __restrict__ lcand,
                                           const LookupInfo& lk, const ViewGeom& vg, int nx, int ny, int nz, int i,
-                                          int j, int k, int32_t* __restrict__ idx, const float4* __restrict__ rec,
+                                          int j, int k, double xx, double yy, int32_t* __restrict__ idx,
+                                          const float4* __restrict__ rec,
                                           const RecInfo& ri, int nlines, float* __restrict__ svlos,
                                           float2* __restrict__ sem, uint8_t* __restrict__ rayflag) {
-  const double minx = vg.ext[0], maxx = vg.ext[1], miny = vg.ext[2], maxy = vg.ext[3], minz = vg.ext[4];
-  const double xx = double(j) / (nx - 1) * (maxx - minx) + minx;  // :152
-  const double yy = double(i) / (ny - 1) * (maxy - miny) + miny;  // :153
+  const double minz = vg.ext[4];
   const double zz = double(k) * vg.deltaz + minz;                 // :162
   const double cb = vg.cb, sb = vg.sb, cl = vg.cl, sl = vg.sl;
   const double p0 = xx * cb * cl + yy * sl + zz * sb * cl;        // :165
@@ -932,11 +931,26 @@ __global__ void __launch_bounds__(256) k_nn(const float4* __restrict__ pts, Grid
                                             int nlines, float* __restrict__ svlos, float2* __restrict__ sem,
                                             uint8_t* __restrict__ rayflag, int tlog, unsigned* __restrict__ dcount) {
   if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) *dcount = 0;  // k_nn_far's deep queue
+  // the tile's lattice x of each column and y of each row, once per block
+  // (fomo-nearestneighbour.cpp:152-153)
+  __shared__ double tx[64], ty[32];
+  const ViewGeom& vg = *vgp;
+  const int tw = 1 << tlog, th = 256 >> tlog;
+  const int tilesx = (nx + tw - 1) >> tlog;
+  const int j0 = (int)(blockIdx.x % tilesx) * tw, i0 = (int)(blockIdx.x / tilesx) * th;
+  if ((int)threadIdx.x < tw) {
+    const int j = j0 + (int)threadIdx.x;
+    tx[threadIdx.x] = double(j) / (nx - 1) * (vg.ext[1] - vg.ext[0]) + vg.ext[0];
+  } else if ((int)threadIdx.x < tw + th) {
+    const int i = i0 + (int)threadIdx.x - tw;
+    ty[threadIdx.x - tw] = double(i) / (ny - 1) * (vg.ext[3] - vg.ext[2]) + vg.ext[2];
+  }
+  __syncthreads();
   int i, j;
   ray_of_thread(nx, tlog, &i, &j);
   if (j >= nx || i >= ny) return;
-  nn_sample<RS4>(pts, g, lslot, lcand, lk, *vgp, nx, ny, nz, i, j, (int)blockIdx.y, idx, rec, ri, nlines, svlos,
-                 sem, rayflag);
+  nn_sample<RS4>(pts, g, lslot, lcand, lk, vg, nx, ny, nz, i, j, (int)blockIdx.y, tx[j - j0], ty[i - i0], idx, rec,
+                 ri, nlines, svlos, sem, rayflag);
 }
 
 // Search box of LOS sample (i, j, k) of a view (fomo-nearestneighbour.cpp:152-183).
