@@ -350,6 +350,12 @@ __device__ __forceinline__ int nn_lookup(const float4* __restrict__ pts, const f
 struct BvhInfo {
   int64_t nodes;  // 0: no tree
   int64_t n;      // points
+  // 256-byte records of the internal nodes, or null (wide_walk_wave): per node, its children in
+  // slots 0..7, slot s being wide[16 r + 2 s] = (box min, ref) and wide[16 r + 2 s + 1] =
+  // (box max, count) with ref and count as int bits; count 0: internal child whose record is ref,
+  // count 1..8: leaf of mpts[ref .. ref + count); an empty slot has an empty box (min +inf,
+  // max -inf). Record 0 is the root.
+  const float4* wide;
 };
 
 
@@ -376,6 +382,48 @@ __device__ __forceinline__ bool box_may_improve(const float4& l, const float4& h
   return sb.node_overlaps(l, h) && md * 0.99999f <= bestf;
 }
 __device__ __forceinline__ float bvh_bound(double best) { return (float)best * 1.000001f; }
+
+// One point of mpts (x, y, z, sorted position) against a lane's running best: kept when strictly
+// inside the box and nearer, or as near with a lower original index.
+__device__ __forceinline__ void wave_scan_point(const float4* __restrict__ pts, const float4& q, const SearchBox& sb,
+                                                double& best, int& bpos, int& borig) {
+  if (!sb.holds(q)) return;
+  const double e0 = (double)q.x - (double)sb.t0;
+  const double e1 = (double)q.y - (double)sb.t1;
+  const double e2 = (double)q.z - (double)sb.t2;
+  const double d = (e0 * e0 + e1 * e1) + e2 * e2;
+  const int p = __float_as_int(q.w);
+  if (d < best) {
+    best = d;
+    bpos = p;
+    borig = -1;
+  } else if (d == best && bpos >= 0) {
+    // borig of the running best is read from pts only when a tie needs it (-1: unread)
+    if (borig < 0) borig = __float_as_int(pts[bpos].w);
+    const int oi = __float_as_int(pts[p].w);
+    if (oi < borig) {
+      bpos = p;
+      borig = oi;
+    }
+  }
+}
+
+// The points of one leaf, mpts[first .. first + cnt) with cnt <= 8, against the running best of
+// every lane with in set: one load, lane u & 7 holding point u, then every such lane scans them in
+// leaf order, as leaf_scan does. Every lane of the wave must call it.
+__device__ __forceinline__ void wave_leaf_scan(const float4* __restrict__ pts, const float4* __restrict__ mpts,
+                                               int first, int cnt, const SearchBox& sb, bool in, double& best,
+                                               int& bpos, int& borig) {
+  const auto pick = [](float v, int o) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), o));
+  };
+  const int lane = (int)(threadIdx.x & 63);
+  const float4 q8 = mpts[first + min(lane & 7, cnt - 1)];
+  for (int u = 0; u < cnt; ++u) {
+    const float4 q = make_float4(pick(q8.x, u), pick(q8.y, u), pick(q8.z, u), pick(q8.w, u));
+    if (in) wave_scan_point(pts, q, sb, best, bpos, borig);
+  }
+}
 
 // Wave-cooperative walk of the preorder node range [e0, e1) of an n-node tree, from the running
 // best (best, bpos, borig; borig < 0: not read yet). Every lane with act set ends with the nearest
@@ -420,38 +468,112 @@ __device__ bool bvh_walk_wave(const float4* __restrict__ pts, const float4* __re
     const bool any = __ballot(in) != 0;
     if (w < 0) {
       if (any) {
-        // the leaf's points (at most 8) in one load, lane u & 7 holding point u; then every
-        // entering lane scans them in leaf order, as leaf_scan does
-        const int first = __float_as_int(h.w), cnt = -w;
-        const float4 q8 = mpts[first + min(lane & 7, cnt - 1)];
-        for (int u = 0; u < cnt; ++u) {
-          const float4 q = make_float4(pick(q8.x, u), pick(q8.y, u), pick(q8.z, u), pick(q8.w, u));
-          if (!in || !sb.holds(q)) continue;
-          const double e0 = (double)q.x - (double)sb.t0;
-          const double e1 = (double)q.y - (double)sb.t1;
-          const double e2 = (double)q.z - (double)sb.t2;
-          const double d = (e0 * e0 + e1 * e1) + e2 * e2;
-          const int p = __float_as_int(q.w);
-          if (d < best) {
-            best = d;
-            bpos = p;
-            borig = -1;
-          } else if (d == best && bpos >= 0) {
-            // borig of the running best is read from pts only when a tie needs it (-1: unread)
-            if (borig < 0) borig = __float_as_int(pts[bpos].w);
-            const int oi = __float_as_int(pts[p].w);
-            if (oi < borig) {
-              bpos = p;
-              borig = oi;
-            }
-          }
-        }
+        wave_leaf_scan(pts, mpts, __float_as_int(h.w), -w, sb, in, best, bpos, borig);
         bestf = bvh_bound(best);
       }
       ++e;
     } else {
       e = any ? e + 1 : w;
     }
+  }
+  return false;
+}
+
+// Entries of a wave's wide_walk_wave stack (two float4 each). Expanding a node pops one entry
+// and pushes at most 8, so a tree whose internal nodes lie at depth <= kWideDepth needs at most
+// 7 kWideDepth + 8 entries; the builder makes wide records only for such trees.
+constexpr int kWideStack = 128;
+constexpr int kWideDepth = 17;
+
+// Wave-cooperative exact query over the wide records of the octree (BvhInfo::wide), from the
+// running best (best, bpos, borig; borig < 0: not read yet): every lane with act set ends with the
+// nearest point strictly inside its box (distance, then lowest original index), oct_query's
+// answer. Depth first with a stack in LDS (stk: kWideStack entries of this wave). Expanding a node
+// takes all its children's boxes in one 256-byte load (lane l holding slot l & 7), and every lane
+// tests the 8 children. The entered leaves are scanned at once: their points, contiguous per leaf,
+// come in one more load (lane l holding point l & 7 of slot l >> 3). The entered internal children
+// are pushed, the farthest from the first active lane's target first, so the nearest is expanded
+// next, and an entry is tested again against the bounds of the moment when it is popped. A lane's
+// answer is found because every node on the path to it passes that lane's own test at any time
+// (its box holds the answer, which is no farther than the lane's running best). Against the
+// preorder walk, an expansion costs two dependent loads where that walk reloads its window of
+// records at every skip past it and loads each leaf on its own. budget > 0: stops after that many
+// expansions and returns true (wave-uniform) with the running best so far, as bvh_walk_wave does.
+// Every lane of the wave must call it.
+__device__ __forceinline__ bool wide_walk_wave(const float4* __restrict__ pts, const float4* __restrict__ mpts,
+                                               const float4* __restrict__ wide, const SearchBox& sb, bool act,
+                                               double& best, int& bpos, int& borig, int budget,
+                                               float4* __restrict__ stk) {
+  const int lane = (int)(threadIdx.x & 63);
+  const int slot = lane & 7;
+  const unsigned long long am = __ballot(act);
+  if (am == 0) return false;
+  // the first active lane's target: the children are expanded nearest first for it
+  const int leader = (int)__builtin_ctzll(am);
+  const auto bcast = [](float v, int o) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), o));
+  };
+  const float lt0 = bcast(sb.t0, leader), lt1 = bcast(sb.t1, leader), lt2 = bcast(sb.t2, leader);
+  float bestf = bvh_bound(best);
+  int sp = 0, steps = 0, r = 0;  // r: record to expand (-1: pop the next entry)
+  for (;;) {
+    if (r >= 0) {
+      if (budget > 0 && ++steps > budget) return true;
+      const float4 a = wide[(size_t)r * 16 + 2 * slot], b = wide[(size_t)r * 16 + 2 * slot + 1];
+      unsigned mask = 0, mine = 0;
+#pragma unroll
+      for (int c = 0; c < 8; ++c) {
+        const float4 l = make_float4(bcast(a.x, c), bcast(a.y, c), bcast(a.z, c), 0.f);
+        const float4 h = make_float4(bcast(b.x, c), bcast(b.y, c), bcast(b.z, c), 0.f);
+        const bool in = act && box_may_improve(l, h, sb, bestf);
+        mine |= (unsigned)in << c;
+        if (__ballot(in) != 0) mask |= 1u << c;
+      }
+      const unsigned leaves = (unsigned)__ballot(lane < 8 && __float_as_int(b.w) > 0) & 0xffu;
+      const unsigned lmask = mask & leaves, imask = mask & ~leaves;
+      if (lmask != 0) {
+        const int c = lane >> 3, u = lane & 7;
+        const int lref = __shfl(__float_as_int(a.w), c, 64), lcnt = __shfl(__float_as_int(b.w), c, 64);
+        float4 q = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (((lmask >> c) & 1u) && u < lcnt) q = mpts[lref + u];
+#pragma unroll
+        for (int c2 = 0; c2 < 8; ++c2) {
+          if (!((lmask >> c2) & 1u)) continue;
+          const int n2 = __builtin_amdgcn_readlane(__float_as_int(b.w), c2);
+          const bool in = (mine >> c2) & 1u;
+          for (int u2 = 0; u2 < n2; ++u2) {
+            const int o = 8 * c2 + u2;
+            const float4 p = make_float4(bcast(q.x, o), bcast(q.y, o), bcast(q.z, o), bcast(q.w, o));
+            if (in) wave_scan_point(pts, p, sb, best, bpos, borig);
+          }
+        }
+        bestf = bvh_bound(best);
+      }
+      const int cnt = __builtin_popcount(imask);
+      if (sp + cnt > kWideStack) return true;  // not reached for trees of depth <= kWideDepth
+      // lane l < 8 holds slot l: its distance from the leader's target orders the pushes
+      const float d0 = fmaxf(fmaxf(a.x - lt0, lt0 - b.x), 0.f);
+      const float d1 = fmaxf(fmaxf(a.y - lt1, lt1 - b.y), 0.f);
+      const float d2 = fmaxf(fmaxf(a.z - lt2, lt2 - b.z), 0.f);
+      const float key = (d0 * d0 + d1 * d1) + d2 * d2;
+      int rank = 0;  // entered internal slots pushed before this one: farther, or as far with a higher slot
+#pragma unroll
+      for (int c = 0; c < 8; ++c) {
+        const float kc = bcast(key, c);
+        rank += ((imask >> c) & 1u) && (kc > key || (kc == key && c > slot));
+      }
+      if (lane < 8 && ((imask >> lane) & 1u)) {
+        stk[2 * (sp + rank)] = a;
+        stk[2 * (sp + rank) + 1] = b;
+      }
+      sp += cnt;
+      r = -1;
+    }
+    if (sp == 0) break;
+    --sp;
+    const float4 l = stk[2 * sp], h = stk[2 * sp + 1];
+    if (__ballot(act && box_may_improve(l, h, sb, bestf)) == 0) continue;
+    r = __builtin_amdgcn_readfirstlane(__float_as_int(l.w));  // an internal child's record
   }
   return false;
 }

@@ -831,6 +831,27 @@ __global__ void k_oct_boxes(const int32_t* __restrict__ ids, int64_t cnt, const 
   nhi[i] = make_float4(h.x, h.y, h.z, b0.w);
 }
 
+// Wide records of the internal nodes (BvhInfo::wide), one thread per slot: kid = the child's node
+// index (-1: empty slot), kref = the child's record when it is internal (-1: leaf).
+__global__ void k_wide_records(const int32_t* __restrict__ kid, const int32_t* __restrict__ kref, int64_t nslots,
+                               const float4* __restrict__ nlo, const float4* __restrict__ nhi,
+                               float4* __restrict__ wide) {
+  const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (t >= nslots) return;
+  const int c = kid[t];
+  float4 a = make_float4(INFINITY, INFINITY, INFINITY, __int_as_float(0));
+  float4 b = make_float4(-INFINITY, -INFINITY, -INFINITY, __int_as_float(0));
+  if (c >= 0) {
+    const float4 l = nlo[c], h = nhi[c];
+    const int w = __float_as_int(l.w);
+    const bool leaf = w < 0;
+    a = make_float4(l.x, l.y, l.z, __int_as_float(leaf ? __float_as_int(h.w) : kref[t]));
+    b = make_float4(h.x, h.y, h.z, __int_as_float(leaf ? -w : 0));
+  }
+  wide[2 * t] = a;
+  wide[2 * t + 1] = b;
+}
+
 // Per-sample outputs of the NN pass for the point at sorted position r (-1: none):
 // svlos[s] (LOS velocity, fomo-nearestneighbour.cpp:87-89) and sem[l*S + s] = (peak_l, fwhm_l),
 // zero when no point is inside the search box (:190-193).
@@ -1086,6 +1107,7 @@ __global__ void __launch_bounds__(256) k_nn_far(const float4* __restrict__ pts, 
   const size_t sidx = (size_t)k * R + ray;
   // lanes without an unresolved sample still take part in the wave's walk (act = false)
   const bool act = inside && rayflag[ray] && idx[sidx] == -2;
+  __shared__ float4 stk[4][2 * kWideStack];  // wide_walk_wave's stack of each wave
   if (__ballot(act) == 0) return;
   const ViewGeom vg = *vgp;
   const SearchBox box = sample_box(vg, nx, ny, inside ? i : 0, inside ? j : 0, k);
@@ -1104,8 +1126,16 @@ __global__ void __launch_bounds__(256) k_nn_far(const float4* __restrict__ pts, 
   }
   if (bv.nodes > 0) {
     const int n = (int)bv.nodes;
-    if (act && bpos < 0) fomo::oct_descend(pts, mpts, nlo, nhi, box, best, bpos, borig);
-    if (bvh_walk_wave(pts, mpts, nlo, nhi, n, 0, n, box, act, best, bpos, borig, dq.budget)) {
+    bool over;
+    if (bv.wide) {
+      // expansions and leaf scans: one per 8 steps of the preorder walk's budget
+      const int wb = dq.budget > 0 ? max(1, dq.budget / 8) : 0;
+      over = wide_walk_wave(pts, mpts, bv.wide, box, act, best, bpos, borig, wb, stk[threadIdx.x >> 6]);
+    } else {
+      if (act && bpos < 0) fomo::oct_descend(pts, mpts, nlo, nhi, box, best, bpos, borig);
+      over = bvh_walk_wave(pts, mpts, nlo, nhi, n, 0, n, box, act, best, bpos, borig, dq.budget);
+    }
+    if (over) {
       const int lane = (int)(threadIdx.x & 63);
       unsigned job = 0;
       if (lane == 0) {
@@ -2142,6 +2172,7 @@ struct Dataset {
   DevBuf mpts;         // float4 (x, y, z, sorted position) in Morton order
   DevBuf nlo, nhi;     // node boxes, all levels
   DevBuf segbox;       // kDeepSegs bounding boxes of the node ranges of k_nn_deep (lo[], then hi[])
+  DevBuf wide;         // 256-byte records of the internal nodes (BvhInfo::wide), or none
   // rectilinear datasets: the points form a product grid of distinct axis values, each
   // combination once; the sorted order is then the grid order and no cell list or candidate grid
   // is built (the NN query reads the axes: rect_geom.h)
@@ -2430,9 +2461,51 @@ int build_bvh(Dataset& ds) {
   if (ds.segbox.ensure(sizeof(float4) * 2 * kDeepSegs)) return fail(FOMO_ERR_NOMEM, "device allocation failed (octree)");
   k_seg_boxes<<<kDeepSegs, 256, 0, s>>>(ds.nlo.as<float4>(), ds.nhi.as<float4>(), (int)nodes, ds.segbox.as<float4>());
   HIPOK(hipGetLastError());
+  // wide records (BvhInfo::wide) when the root is internal and every internal node lies at depth
+  // <= kWideDepth (the bound of wide_walk_wave's stack); else the exact queries walk the preorder
+  // records
+  ds.wide.release();
+  const int64_t depth = (int64_t)ob.bydepth.size() - 1;  // of the deepest node, a leaf
+  std::vector<int32_t> kid, kref;
+  int64_t nint = 0;
+  if (nodes > 1 && depth - 1 <= kWideDepth) {
+    std::vector<int32_t> rank(nodes, -1);
+    for (int64_t i = 0; i < nodes; ++i)
+      if (ob.w[i] >= 0) rank[i] = (int32_t)nint++;
+    kid.assign((size_t)nint * 8, -1);
+    kref.assign((size_t)nint * 8, -1);
+    bool fits = true;  // the builder gives a node at most 8 children
+    for (int64_t i = 0; i < nodes && fits; ++i) {
+      if (ob.w[i] < 0) continue;
+      int slot = 0;
+      for (int64_t c = i + 1; c < ob.w[i]; c = ob.w[c] < 0 ? c + 1 : ob.w[c]) {
+        if (slot == 8) {
+          fits = false;
+          break;
+        }
+        kid[(size_t)rank[i] * 8 + slot] = (int32_t)c;
+        kref[(size_t)rank[i] * 8 + slot] = rank[c];
+        ++slot;
+      }
+    }
+    if (!fits) nint = 0;
+  }
+  if (nodes > 1 && depth - 1 <= kWideDepth && nint > 0) {
+    DevBuf dkid, dkref;
+    if (ds.wide.alloc(sizeof(float4) * 16 * nint) || dkid.alloc(sizeof(int32_t) * kid.size()) ||
+        dkref.alloc(sizeof(int32_t) * kref.size()))
+      return fail(FOMO_ERR_NOMEM, "device allocation failed (octree)");
+    HIPOK(hipMemcpyAsync(dkid.p, kid.data(), sizeof(int32_t) * kid.size(), hipMemcpyHostToDevice, s));
+    HIPOK(hipMemcpyAsync(dkref.p, kref.data(), sizeof(int32_t) * kref.size(), hipMemcpyHostToDevice, s));
+    k_wide_records<<<nblk(nint * 8), kBlock, 0, s>>>(dkid.as<int32_t>(), dkref.as<int32_t>(), nint * 8,
+                                                      ds.nlo.as<float4>(), ds.nhi.as<float4>(), ds.wide.as<float4>());
+    HIPOK(hipGetLastError());
+    if (int rc = sync_stream()) return rc;
+  }
   if (int rc = sync_stream()) return rc;
   ds.bvh.nodes = nodes;
   ds.bvh.n = n;
+  ds.bvh.wide = ds.wide.p ? ds.wide.as<float4>() : nullptr;
   return FOMO_OK;
 }
 
