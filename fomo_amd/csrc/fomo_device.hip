@@ -831,23 +831,18 @@ __global__ void k_oct_boxes(const int32_t* __restrict__ ids, int64_t cnt, const 
   nhi[i] = make_float4(h.x, h.y, h.z, b0.w);
 }
 
-// Wide records of the internal nodes (BvhInfo::wide), one thread per slot: kid = the child's node
-// index (-1: empty slot), kref = the child's record when it is internal (-1: leaf).
+// Wide records of the internal nodes (BvhInfo::wide, fomo::wide_slot), one thread per slot: kid =
+// the child's node index (-1: empty slot), kref = the child's record when it is internal (-1: leaf)
+// (fomo::wide_children).
 __global__ void k_wide_records(const int32_t* __restrict__ kid, const int32_t* __restrict__ kref, int64_t nslots,
                                const float4* __restrict__ nlo, const float4* __restrict__ nhi,
                                float4* __restrict__ wide) {
   const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (t >= nslots) return;
   const int c = kid[t];
-  float4 a = make_float4(INFINITY, INFINITY, INFINITY, __int_as_float(0));
-  float4 b = make_float4(-INFINITY, -INFINITY, -INFINITY, __int_as_float(0));
-  if (c >= 0) {
-    const float4 l = nlo[c], h = nhi[c];
-    const int w = __float_as_int(l.w);
-    const bool leaf = w < 0;
-    a = make_float4(l.x, l.y, l.z, __int_as_float(leaf ? __float_as_int(h.w) : kref[t]));
-    b = make_float4(h.x, h.y, h.z, __int_as_float(leaf ? -w : 0));
-  }
+  const float4 zero = make_float4(0.f, 0.f, 0.f, 0.f);
+  float4 a, b;
+  fomo::wide_slot(c, kref[t], c >= 0 ? nlo[c] : zero, c >= 0 ? nhi[c] : zero, a, b);
   wide[2 * t] = a;
   wide[2 * t + 1] = b;
 }
@@ -2467,30 +2462,8 @@ int build_bvh(Dataset& ds) {
   ds.wide.release();
   const int64_t depth = (int64_t)ob.bydepth.size() - 1;  // of the deepest node, a leaf
   std::vector<int32_t> kid, kref;
-  int64_t nint = 0;
-  if (nodes > 1 && depth - 1 <= kWideDepth) {
-    std::vector<int32_t> rank(nodes, -1);
-    for (int64_t i = 0; i < nodes; ++i)
-      if (ob.w[i] >= 0) rank[i] = (int32_t)nint++;
-    kid.assign((size_t)nint * 8, -1);
-    kref.assign((size_t)nint * 8, -1);
-    bool fits = true;  // the builder gives a node at most 8 children
-    for (int64_t i = 0; i < nodes && fits; ++i) {
-      if (ob.w[i] < 0) continue;
-      int slot = 0;
-      for (int64_t c = i + 1; c < ob.w[i]; c = ob.w[c] < 0 ? c + 1 : ob.w[c]) {
-        if (slot == 8) {
-          fits = false;
-          break;
-        }
-        kid[(size_t)rank[i] * 8 + slot] = (int32_t)c;
-        kref[(size_t)rank[i] * 8 + slot] = rank[c];
-        ++slot;
-      }
-    }
-    if (!fits) nint = 0;
-  }
-  if (nodes > 1 && depth - 1 <= kWideDepth && nint > 0) {
+  const int64_t nint = nodes > 1 && depth - 1 <= kWideDepth ? fomo::wide_children(ob, kid, kref) : 0;
+  if (nint > 0) {
     DevBuf dkid, dkref;
     if (ds.wide.alloc(sizeof(float4) * 16 * nint) || dkid.alloc(sizeof(int32_t) * kid.size()) ||
         dkref.alloc(sizeof(int32_t) * kref.size()))

@@ -201,6 +201,94 @@ struct OctBuilder {
   }
 };
 
+// Wide records: per internal node, in preorder rank order (the root is rank 0), 8 slots of two
+// 4-float records, slot s of rank r at [16 r + 2 s] = (box min, ref) and [16 r + 2 s + 1] =
+// (box max, count), ref and count as int bits: count 0 for an internal child (ref: its rank),
+// 1..8 for a leaf (ref: its first point in mpts), and an empty slot with an empty box (min +inf,
+// max -inf, never overlapping a search box). The device walks them depth first, a node's 8
+// children in one load.
+//
+// The children of every internal node: kid[8 r + s] = node index of child s of rank r (-1: empty
+// slot), kref[8 r + s] = the child's rank when internal (-1: leaf). Returns the number of internal
+// nodes; 0 when the tree has none or a node has more than 8 children (the builder gives at most 8).
+inline int64_t wide_children(const OctBuilder& ob, std::vector<int32_t>& kid, std::vector<int32_t>& kref) {
+  const int64_t nodes = (int64_t)ob.w.size();
+  std::vector<int32_t> rank(nodes, -1);
+  int64_t nint = 0;
+  for (int64_t i = 0; i < nodes; ++i)
+    if (ob.w[i] >= 0) rank[i] = (int32_t)nint++;
+  kid.assign((size_t)nint * 8, -1);
+  kref.assign((size_t)nint * 8, -1);
+  for (int64_t i = 0; i < nodes; ++i) {
+    if (ob.w[i] < 0) continue;
+    int slot = 0;
+    for (int64_t c = i + 1; c < ob.w[i]; c = ob.w[c] < 0 ? c + 1 : ob.w[c]) {
+      if (slot == 8) {
+        kid.clear();
+        kref.clear();
+        return 0;
+      }
+      kid[(size_t)rank[i] * 8 + slot] = (int32_t)c;
+      kref[(size_t)rank[i] * 8 + slot] = rank[c];
+      ++slot;
+    }
+  }
+  return nint;
+}
+
+FOMO_HD inline float i2f_hd(int32_t v) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return __int_as_float(v);
+#else
+  return i2f(v);
+#endif
+}
+
+// The slot (a, b) of child c (node index, -1: empty slot) with internal rank ref (-1: leaf) from
+// the preorder node records l = lo[c], h = hi[c].
+template <class V>
+FOMO_HD inline void wide_slot(int c, int ref, const V& l, const V& h, V& a, V& b) {
+  if (c < 0) {
+    a.x = a.y = a.z = INFINITY;
+    b.x = b.y = b.z = -INFINITY;
+    a.w = b.w = i2f_hd(0);
+    return;
+  }
+  const int w = f2i(l.w);
+  a.x = l.x, a.y = l.y, a.z = l.z;
+  b.x = h.x, b.y = h.y, b.z = h.z;
+  a.w = i2f_hd(w < 0 ? f2i(h.w) : ref);
+  b.w = i2f_hd(w < 0 ? -w : 0);
+}
+
+// One query over the wide records, depth first (the order the device's wave walk takes is
+// another; the answer is the same): the host model of the device walk, checked against brute
+// force by the octree test.
+template <class A>
+FOMO_HD inline int wide_query(const A& pts, const A& mpts, const A& wide, const SearchBox& sb, double best, int bpos,
+                              int borig) {
+  constexpr int kStack = 128;
+  int stk[kStack];
+  int sp = 0;
+  stk[sp++] = 0;
+  while (sp > 0) {
+    const int r = stk[--sp];
+    for (int s = 0; s < 8; ++s) {
+      const auto a = wide[(int64_t)r * 16 + 2 * s];
+      const auto b = wide[(int64_t)r * 16 + 2 * s + 1];
+      if (!oct_enter(a, b, sb, best)) continue;
+      const int ref = f2i(a.w), cnt = f2i(b.w);
+      if (cnt > 0) {
+        leaf_scan(pts, mpts, ref, cnt, sb, best, bpos, borig);
+      } else {
+        if (sp == kStack) return -3;  // deeper than the device's stack allows
+        stk[sp++] = ref;
+      }
+    }
+  }
+  return bpos;
+}
+
 // 30-bit Morton code of a point in the box [bmin, bmax] (10 bits per axis, x highest).
 FOMO_HD inline uint32_t morton30(float x, float y, float z, const float bmin[3], const float bmax[3]) {
   const float c[3] = {x, y, z};

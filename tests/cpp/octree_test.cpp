@@ -1,9 +1,11 @@
 // Host test of the octree used for the exact NN fallback (fomo_amd/csrc/octree.h): the very query
 // code the device runs, here over bounds-checked arrays. Checks the tree invariants (preorder skip
 // pointers, children partition their parent's range, leaves of 1..8 points, boxes enclose their
-// points) and compares cold and warm-started queries with brute force (nearest point strictly
-// inside the open box, squared double distance of float coordinates, ties to the lowest original
-// index) on uniform, lattice (many exact ties), duplicate and shell point sets.
+// points), the wide records made from it (each node's children in order: boxes, leaf ranges,
+// ranks), and compares cold and warm-started queries, over the preorder nodes and over the wide
+// records, with brute force (nearest point strictly inside the open box, squared double distance
+// of float coordinates, ties to the lowest original index) on uniform, lattice (many exact ties),
+// duplicate and shell point sets.
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
@@ -130,7 +132,46 @@ static void run_set(const char* name, std::vector<F4> pts, std::mt19937_64& rng,
     }
   }
   check(leafpts == n, "leaves cover every point once");
+  // wide records (the device's k_nn_far walk): made by the same functions the device build uses
+  std::vector<int32_t> kid, kref;
+  const int64_t nint = fomo::wide_children(ob, kid, kref);
+  check(nodes == 1 ? nint == 0 : nint > 0, "wide records for every tree with an internal root");
+  std::vector<F4> wide((size_t)nint * 16);
+  for (int64_t t = 0; t < nint * 8; ++t) {
+    const int c = kid[t];
+    const F4 z{0, 0, 0, 0};
+    fomo::wide_slot(c, kref[t], c >= 0 ? lo[c] : z, c >= 0 ? hi[c] : z, wide[2 * t], wide[2 * t + 1]);
+  }
+  {
+    // every record lists its node's children in order; leaf slots hold their point ranges,
+    // internal slots the child's rank, whose own record is that child's
+    int64_t r = 0, wleaf = 0;
+    for (int64_t i = 0; i < nodes; ++i) {
+      if (ob.w[i] < 0) continue;
+      int s = 0;
+      for (int c = (int)i + 1; c < ob.w[i]; c = ob.w[c] < 0 ? c + 1 : ob.w[c], ++s) {
+        const F4 a = wide[r * 16 + 2 * s], b = wide[r * 16 + 2 * s + 1];
+        check(a.x == lo[c].x && a.y == lo[c].y && a.z == lo[c].z && b.x == hi[c].x && b.y == hi[c].y &&
+                  b.z == hi[c].z, "wide slot box is the child's");
+        if (ob.w[c] < 0) {
+          check(fomo::f2i(a.w) == ob.first[c] && fomo::f2i(b.w) == -ob.w[c], "wide leaf slot: first, count");
+          wleaf += -ob.w[c];
+        } else {
+          check(fomo::f2i(b.w) == 0 && fomo::f2i(a.w) == kref[r * 8 + s] && kid[r * 8 + s] == c,
+                "wide internal slot: the child's rank");
+        }
+      }
+      for (; s < 8; ++s) {
+        const F4 a = wide[r * 16 + 2 * s], b = wide[r * 16 + 2 * s + 1];
+        check(kid[r * 8 + s] == -1 && a.x == INFINITY && b.x == -INFINITY && fomo::f2i(b.w) == 0,
+              "wide empty slot");
+      }
+      ++r;
+    }
+    check(r == nint && (nint == 0 || wleaf == n), "wide leaf slots cover every point once");
+  }
   const Arr A{pts.data(), n}, M{mpts.data(), n}, L{lo.data(), nodes}, H{hi.data(), nodes};
+  const Arr W{wide.data(), (int64_t)wide.size()};
   const float ext = std::max({bmax[0] - bmin[0], bmax[1] - bmin[1], bmax[2] - bmin[2], 1e-3f});
   std::uniform_real_distribution<float> U(0.f, 1.f);
   int bad = 0;
@@ -163,8 +204,21 @@ static void run_set(const char* name, std::vector<F4> pts, std::mt19937_64& rng,
       const double d0 = (double)q.x - sb.t0, d1 = (double)q.y - sb.t1, d2 = (double)q.z - sb.t2;
       warm = fomo::oct_query(A, M, L, H, nodes, sb, (d0 * d0 + d1 * d1) + d2 * d2, seed, fomo::f2i(q.w));
     }
-    if (cold != want || warm != want) {
-      if (bad++ < 5) std::printf("%s: query %d cold %d warm %d brute %d\n", name, t, cold, warm, want);
+    // the wide records, cold and from the same warm start
+    int wcold = cold, wwarm = warm;
+    if (nint > 0) {
+      wcold = fomo::wide_query(A, M, W, sb, INFINITY, -1, 0x7fffffff);
+      if (seed >= 0) {
+        const F4 q = pts[seed];
+        const double d0 = (double)q.x - sb.t0, d1 = (double)q.y - sb.t1, d2 = (double)q.z - sb.t2;
+        wwarm = fomo::wide_query(A, M, W, sb, (d0 * d0 + d1 * d1) + d2 * d2, seed, fomo::f2i(q.w));
+      } else {
+        wwarm = wcold;
+      }
+    }
+    if (cold != want || warm != want || wcold != want || wwarm != want) {
+      if (bad++ < 5)
+        std::printf("%s: query %d cold %d warm %d wide %d %d brute %d\n", name, t, cold, warm, wcold, wwarm, want);
     }
   }
   check(bad == 0, "queries equal brute force");
