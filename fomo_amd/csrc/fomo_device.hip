@@ -929,7 +929,7 @@ __device__ __forceinline__ void nn_sample(const float4* __restrict__ pts, const 
 }
 
 
-// Ray tile of a k_nn / k_nn_far block: 256 rays of one LOS plane, 2^tlog wide and 256 / 2^tlog
+// Ray tile of a k_nn block: 256 rays of one LOS plane, 2^tlog wide and 256 / 2^tlog
 // high (tlog 3..6: 8x32 .. 64x4), so each wave of 64 lanes covers 2^tlog x 64 / 2^tlog rays.
 __device__ __forceinline__ void ray_of_thread(int nx, int tlog, int* i, int* j) {
   const int tw = 1 << tlog, th = 256 >> tlog;
@@ -1082,22 +1082,26 @@ __global__ void __launch_bounds__(256) k_seg_boxes(const float4* __restrict__ nl
   }
 }
 
-// Second NN pass over the samples k_nn left unresolved (-2): exact octree query. Same grid as
-// k_nn; rays without such samples exit at once (rayflag, cleared afterwards by k_clear_u8).
-// The starting bound is the previous sample's point on the ray when k_nn resolved it and it lies
-// strictly inside this sample's box, else a greedy descent.
+// Second NN pass over the samples k_nn left unresolved (-2): exact octree query. Rays without
+// such samples exit at once (rayflag, cleared afterwards by k_clear_u8). The starting bound is the
+// previous sample's point on the ray when it is known (k_nn's, or this pass's when that sample's
+// wave has finished) and lies strictly inside this sample's box; else the walk's own first descent.
 template <int RS4>
 __global__ void __launch_bounds__(256) k_nn_far(const float4* __restrict__ pts, const float4* __restrict__ mpts,
                                                 const float4* __restrict__ nlo, const float4* __restrict__ nhi,
                                                 BvhInfo bv, const ViewGeom* __restrict__ vgp, int nx, int ny, int nz,
                                                 int32_t* __restrict__ idx, const float4* __restrict__ rec, RecInfo ri,
                                                 int nlines, float* __restrict__ svlos, float2* __restrict__ sem,
-                                                const uint8_t* __restrict__ rayflag, int tlog, DeepQueue dq) {
-  int i, j;
-  ray_of_thread(nx, tlog, &i, &j);
-  const int k = blockIdx.y;
+                                                const uint8_t* __restrict__ rayflag, DeepQueue dq) {
+  // a block takes an 8 x 8 ray tile through 4 consecutive LOS planes, a wave a 4 x 4 quarter of it
+  // through the 4 planes: the most compact group of 64 samples, whose walks share the most nodes
+  const int tilesx = (nx + 7) >> 3, lane = (int)(threadIdx.x & 63), wv = (int)(threadIdx.x >> 6);
+  const int j = (int)(blockIdx.x % tilesx) * 8 + (wv & 1) * 4 + (lane & 3);
+  const int i = (int)(blockIdx.x / tilesx) * 8 + (wv >> 1) * 4 + ((lane >> 2) & 3);
+  const int kk = (int)blockIdx.y * 4 + (lane >> 4);
   const size_t R = (size_t)nx * ny;
-  const bool inside = j < nx && i < ny;
+  const bool inside = j < nx && i < ny && kk < nz;
+  const int k = inside ? kk : 0;
   const size_t ray = inside ? (size_t)i * nx + j : 0;
   const size_t sidx = (size_t)k * R + ray;
   // lanes without an unresolved sample still take part in the wave's walk (act = false)
@@ -1109,7 +1113,7 @@ __global__ void __launch_bounds__(256) k_nn_far(const float4* __restrict__ pts, 
   double best = INFINITY;
   int bpos = -1, borig = 0x7fffffff;
   if (act && k > 0) {
-    const int prev = idx[sidx - R];  // k_nn's result, or this pass's (any value >= 0 is a point)
+    const int prev = idx[sidx - R];  // k_nn's result, or this pass's when done (any value >= 0 is a point)
     if (prev >= 0) {
       const float4 q = pts[prev];
       if (box.holds(q)) {
@@ -2189,7 +2193,7 @@ struct Dataset {
   // whose cells all take the list (group headers)
   int64_t far_groups = 0, far_lists = 0, full_groups = 0, outer_groups = 0, outer_lists = 0;
   int far_budget = kFarBudget;  // node steps of a k_nn_far walk before the deep pass (0: never)
-  int tile_log2 = 3;  // ray tile of k_nn / k_nn_far blocks: 2^tile_log2 x 256 / 2^tile_log2 (autotuned)
+  int tile_log2 = 3;  // ray tile of k_nn blocks: 2^tile_log2 x 256 / 2^tile_log2 (autotuned)
   // tetra path: Delaunay triangulation of the points, built on the host on first use
   int64_t ntets = 0;      // 0: not built
   DevBuf trec;            // kTetRec float4 per tet: 4 vertices {x, y, z, sorted position as int bits (-1 =
@@ -3277,13 +3281,14 @@ void enqueue_nn(hipStream_t s, const Dataset& ds, const ViewGeom* geom, int nx, 
   }
   {
     ProfScope ps("k_nn_far", s);
+    const dim3 fgrid((unsigned)(((nx + 7) / 8) * ((ny + 7) / 8)), (unsigned)((nz + 3) / 4));
 #define FOMO_NNF_ARGS                                                                                         \
   ds.pts.as<float4>(), ds.mpts.as<float4>(), ds.nlo.as<float4>(), ds.nhi.as<float4>(), ds.bvh, geom, nx, ny, \
-      nz, idx, rec, ri, nlines, svlos, sem, rayflag, tlog, dq
-    if (!rec) k_nn_far<0><<<grid, 256, 0, s>>>(FOMO_NNF_ARGS);
-    else if (rs4 == 2) k_nn_far<2><<<grid, 256, 0, s>>>(FOMO_NNF_ARGS);
-    else if (rs4 == 4) k_nn_far<4><<<grid, 256, 0, s>>>(FOMO_NNF_ARGS);
-    else k_nn_far<8><<<grid, 256, 0, s>>>(FOMO_NNF_ARGS);
+      nz, idx, rec, ri, nlines, svlos, sem, rayflag, dq
+    if (!rec) k_nn_far<0><<<fgrid, 256, 0, s>>>(FOMO_NNF_ARGS);
+    else if (rs4 == 2) k_nn_far<2><<<fgrid, 256, 0, s>>>(FOMO_NNF_ARGS);
+    else if (rs4 == 4) k_nn_far<4><<<fgrid, 256, 0, s>>>(FOMO_NNF_ARGS);
+    else k_nn_far<8><<<fgrid, 256, 0, s>>>(FOMO_NNF_ARGS);
 #undef FOMO_NNF_ARGS
   }
   {

@@ -73,8 +73,9 @@ int fomo_dataset_reindex(int id, double cell_scale);
  * 0 disables it, leaving the exact ring search over the cell list. Results are identical either
  * way; only speed and memory change. */
 int fomo_dataset_set_lookup(int id, int subdiv);
-/* Ray tile of the NN passes' blocks: width 8 (default), 16, 32 or 64 rays, 256 rays per block.
- * Speed only; results do not change. */
+/* Ray tile of the lookup pass's blocks (k_nn): width 8 (default), 16, 32 or 64 rays, 256 rays per
+ * block (the exact-query pass always takes 4 x 4 rays through 4 LOS planes per wave). Speed only;
+ * results do not change. */
 int fomo_dataset_set_ray_tile(int id, int width);
 /* Node steps an exact octree query of the NN passes takes (per wave of 64 samples) before its
  * samples move to a pass that walks 1024 node ranges of the tree with 32 waves in parallel and
@@ -214,8 +215,7 @@ int fomo_projection_pixels(int dataset, double l, double b, int nx, int ny, int 
  * dataset, one kind; the lines of a batch), times the nearest-neighbour passes (k_nn + k_nn_far)
  * of one representative view (l, b, nx x ny x nz; Thomson geometry for a Thomson cube), gathering
  * the same per-point records a batch of these cubes gathers, over cell-list scales {0.7, 1, 1.4} x
- * lookup subdivisions {1, 2, 3} x ray tiles of those passes' blocks {8x32, 16x16, 32x8, 64x4}
- * (a wave of 64 neighbouring rays walks the octree together, so the tile also shapes the walks),
+ * lookup subdivisions {1, 2, 3} x ray tiles of the lookup pass's blocks {8x32, 16x16, 32x8, 64x4},
  * `reps` timed repetitions each. The dataset keeps the fastest configuration when it beats the
  * one on entry by more than 3 %, else the one on entry. result (optional, 5 doubles): chosen cell
  * scale, subdivision, its ms per view, the ms per view on entry, the chosen tile width in rays.
